@@ -1,0 +1,246 @@
+"""Seeded synthetic workloads for the SYNTHE/ATLAS12 hot path.
+
+Builds, from a seed alone, the inputs the line-opacity build and the
+transfer solve consume:
+
+* model-atmosphere decks in the Kurucz ``READ DECK6`` card format read by
+  READIN (src/atlas12_modules.f90:6644-6944), e.g. the 72-layer solar-like
+  structure of the plain-SYNTHE workload;
+* line lists written as gfall ASCII records (the fixed format read by
+  read_gfall, src/mod_mklinelist.f90:860-866) and, for the GPU, the
+  ``lte_line_t`` records that reader produces from them
+  (src/mod_mklinelist.f90:888-1064, restated in :func:`gfall_to_lte`).
+
+Nothing here needs the reference tree: the GPU box regenerates the same
+inputs from the same seeds.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+# lte_line_t (src/mod_mklinelist.f90:30-41), 32 bytes
+LTE_DTYPE = np.dtype([("nbuff", "<i4"), ("cgf", "<f4"), ("nelion", "<i4"),
+                      ("elo", "<f4"), ("gamrf", "<f4"), ("gamsf", "<f4"),
+                      ("gamwf", "<f4"), ("code", "<f4")])
+
+# Species of the plain-SYNTHE workload: Fe I/II, Ca I/II, Ti I, Na I, Mg I
+SYNTHE_SPECIES = (26.00, 26.01, 20.00, 20.01, 22.00, 11.00, 12.00)
+
+EV_TO_CM = 8065.544  # 1 eV in cm^-1
+
+
+@dataclass
+class Grid:
+    """SYNTHE wavelength grid, src/synthe.f90:316-324."""
+    wlbeg: float
+    wlend: float
+    resolu: float
+    ratio: float
+    ratiolg: float
+    length: int
+    ixwlbeg: int
+    wbegin: float
+
+    @staticmethod
+    def make(wlbeg: float, wlend: float, resolu: float) -> "Grid":
+        ratio = 1.0 + 1.0 / resolu
+        ratiolg = math.log(ratio)
+        length = int(math.log(wlend / wlbeg) / ratiolg)
+        ixwlbeg = int(math.log(wlbeg) / ratiolg)
+        wbegin = math.exp(ixwlbeg * ratiolg)
+        if wbegin < wlbeg:
+            ixwlbeg += 1
+            wbegin = math.exp(ixwlbeg * ratiolg)
+        return Grid(wlbeg, wlend, resolu, ratio, ratiolg, length, ixwlbeg, wbegin)
+
+
+# ---------------------------------------------------------------------------
+# Model atmospheres
+# ---------------------------------------------------------------------------
+
+def _saha_h_electrons(t: np.ndarray, p: np.ndarray, donor: float = 1.0e-4) -> np.ndarray:
+    """Electron density from pure-hydrogen Saha ionisation plus a metal
+    donor floor (fraction of nuclei), for a starting deck only; SYNTHE
+    recomputes n_e with its own equation of state."""
+    k = 1.380649e-16
+    h = 6.62607015e-27
+    me = 9.1093837139e-28
+    chi = 13.598 * 1.602176634e-12
+    ntot = p / (k * t)
+    s = (2.0 * math.pi * me * k * t / h**2) ** 1.5 * np.exp(-chi / (k * t))
+    # x^2 n / (1 - x) = s  with n the hydrogen density (electrons neglected)
+    x = (-s + np.sqrt(s * s + 4.0 * ntot * s)) / (2.0 * ntot)
+    x = np.maximum(x, donor)
+    return x * ntot / (1.0 + x)
+
+
+@dataclass
+class Atmosphere:
+    teff: float
+    logg: float
+    rhox: np.ndarray
+    t: np.ndarray
+    p: np.ndarray
+    xne: np.ndarray
+    vturb: np.ndarray  # cm/s
+
+    @property
+    def nrhox(self) -> int:
+        return len(self.rhox)
+
+    def deck(self) -> str:
+        """Kurucz card deck (READ DECK6 RHOX,T,P,XNE,ABROSS,ACCRAD,VTURB)."""
+        out = [f"TEFF   {self.teff:.0f}.  GRAVITY {self.logg:.5f} LTE",
+               "TITLE SYNTHETIC",
+               f"READ DECK6 {self.nrhox} RHOX,T,P,XNE,ABROSS,ACCRAD,VTURB"]
+        for j in range(self.nrhox):
+            out.append(f" {self.rhox[j]:.8E} {self.t[j]:8.1f} {self.p[j]:.3E} {self.xne[j]:.3E}"
+                       f" {1.0:.3E} {0.0:.3E} {self.vturb[j]:.3E}")
+        out.append("PRADK 0.0000E+00")
+        out.append("BEGIN                    ITERATION   1 COMPLETED")
+        return "\n".join(out) + "\n"
+
+
+def scurve_atmosphere(nrhox: int = 72, t_top: float = 4400.0, t_bot: float = 9500.0,
+                      logrhox0: float = -3.0, logrhox1: float = 2.0, logg: float = 4.44,
+                      teff: float = 5777.0, vturb_kms: float = 1.5) -> Atmosphere:
+    """RHOX log-spaced, T rising on the smooth s-curve T = t_top + dT s^2 (3-2s)
+    (the structure of the plain-SYNTHE and large-SYNTHE workloads)."""
+    s = np.arange(nrhox) / (nrhox - 1)
+    rhox = 10.0 ** (logrhox0 + (logrhox1 - logrhox0) * s)
+    t = t_top + (t_bot - t_top) * s * s * (3.0 - 2.0 * s)
+    # the deck carries 1 decimal of T; round so the deck is the model
+    t = np.round(t, 1)
+    p = 10.0 ** logg * rhox
+    xne = _saha_h_electrons(t, p)
+    vt = np.full(nrhox, vturb_kms * 1.0e5)
+    return Atmosphere(teff, logg, rhox, t, p, xne, vt)
+
+
+# ---------------------------------------------------------------------------
+# Line lists
+# ---------------------------------------------------------------------------
+
+GFALL_FIELDS = [("wl", "f8"), ("loggf", "f4"), ("code", "f4"), ("e", "f8"),
+                ("ep", "f8"), ("gr", "f4"), ("gs", "f4"), ("gw", "f4")]
+
+
+def synthetic_gfall(n: int, wl0: float, wl1: float, seed: int,
+                    species=SYNTHE_SPECIES, loggf=("uniform", -5.0, 0.0),
+                    elo=("uniform", 0.0, 8.0), lgr=(7.0, 9.0), lgs=(-6.0, -4.0),
+                    lgw=(-8.0, -7.0)) -> np.ndarray:
+    """Seeded atomic lines, centres uniform in ln(lambda) over [wl0, wl1] nm,
+    quantised to the gfall print precision (1e-4 nm, 1e-3 dex, 1e-3 cm^-1,
+    1e-2 dex) and sorted by wavelength (read_gfall stops at the first record
+    past the window, src/mod_mklinelist.f90:896)."""
+    rng = np.random.default_rng(seed)
+    wl = np.exp(rng.uniform(math.log(wl0), math.log(wl1), n))
+    if loggf[0] == "uniform":
+        gf = rng.uniform(loggf[1], loggf[2], n)
+    else:
+        gf = rng.normal(loggf[1], loggf[2], n)
+    if elo[0] == "uniform":
+        e_ev = rng.uniform(elo[1], elo[2], n)
+    else:
+        e_ev = rng.exponential(elo[1], n)
+    sp = np.asarray(species)[rng.integers(0, len(species), n)]
+    gr = rng.uniform(*lgr, n)
+    gs = rng.uniform(*lgs, n)
+    gw = rng.uniform(*lgw, n)
+    rec = np.zeros(n, dtype=GFALL_FIELDS)
+    rec["wl"] = np.round(wl, 4)
+    rec["loggf"] = np.clip(np.round(gf, 3), -99.0, 99.0)
+    rec["code"] = sp
+    e_cm = np.round(e_ev * EV_TO_CM, 3)
+    rec["e"] = e_cm
+    rec["ep"] = np.round(e_cm + 1.0e7 / rec["wl"], 3)
+    # a zero log-gamma selects the reader's default widths; keep them non-zero
+    for name, v in (("gr", gr), ("gs", gs), ("gw", gw)):
+        q = np.round(v, 2)
+        q[q == 0.0] = 0.01
+        rec[name] = q
+    rec.sort(order="wl", kind="stable")
+    return rec
+
+
+def write_gfall(rec: np.ndarray, path: str) -> None:
+    """gfall ASCII, FORMAT(F11.4,F7.3,F6.2,F12.3,F5.1,1X,A8,A2,F12.3,F5.1,1X,A8,A2,
+    F6.2,F6.2,F6.2,A4,I2,I2,I3,F6.3,I3,F6.3,A8,A2,A8,A2,2I5,I6)."""
+    lab = "SYNTH   " + "  "
+    with open(path, "w") as fh:
+        for r in rec:
+            fh.write(f"{r['wl']:11.4f}{r['loggf']:7.3f}{r['code']:6.2f}{r['e']:12.3f}{0.5:5.1f} {lab}"
+                     f"{r['ep']:12.3f}{1.5:5.1f} {lab}{r['gr']:6.2f}{r['gs']:6.2f}{r['gw']:6.2f}"
+                     f"SYN {0:2d}{0:2d}{0:3d}{0.0:6.3f}{0:3d}{0.0:6.3f}{'':8s}{'':2s}{'':8s}{'':2s}"
+                     f"{0:5d}{0:5d}{0:6d}\n")
+
+
+_LIBM = None
+
+
+def powf10(x: np.ndarray) -> np.ndarray:
+    """10.0**x for REAL(4) x as the reference's runtime evaluates it (C
+    ``powf``).  powf works in double internally and rounds once, so it
+    equals pow() rounded to FP32 except when that double lands within a
+    hair of an FP32 rounding midpoint; only those elements call powf."""
+    global _LIBM
+    x = np.asarray(x, dtype=np.float32)
+    d = np.power(10.0, x.astype(np.float64))
+    f = d.astype(np.float32)
+    # distance of d from the FP32 midpoint next to f, in FP32 ulps
+    up = np.nextafter(f, np.float32(np.inf))
+    dn = np.nextafter(f, np.float32(0))
+    mid = np.where(d >= f.astype(np.float64), (f.astype(np.float64) + up) * 0.5,
+                   (f.astype(np.float64) + dn) * 0.5)
+    ulp = np.abs(up.astype(np.float64) - f.astype(np.float64))
+    near = np.abs(d - mid) < 2.0e-2 * ulp
+    if near.any():
+        if _LIBM is None:
+            import ctypes
+            _LIBM = ctypes.CDLL("libm.so.6")
+            _LIBM.powf.restype = ctypes.c_float
+            _LIBM.powf.argtypes = [ctypes.c_float, ctypes.c_float]
+        idx = np.nonzero(near)[0]
+        f[idx] = np.array([_LIBM.powf(10.0, float(v)) for v in x[idx]], dtype=np.float32)
+    return f
+
+
+def gfall_to_lte(rec: np.ndarray, grid: Grid) -> np.ndarray:
+    """``lte_line_t`` records as read_gfall builds them for plain LTE lines
+    (src/mod_mklinelist.f90:888-1064): window selection with the 0.1 nm
+    margin (lim = 7, :846-847, :903-907), nbuff = INT(ln lambda / ln ratio +
+    0.5) - ixwlbeg + 1, gf = 10.0**loggf in FP32, cgf = 0.026538/1.77245 gf /
+    nu, damping 10**log gamma / (12.5664 nu).  Expression types follow the
+    reference (FP32 where it is REAL(4))."""
+    delfactor = grid.wlbeg / 500.0 if grid.wlbeg > 500.0 else 1.0
+    wl = rec["wl"].astype(np.float64)
+    keep = (wl >= grid.wlbeg - 0.1 * delfactor) & (wl <= grid.wlend + 0.1 * delfactor)
+    r = rec[keep]
+    wl = r["wl"].astype(np.float64)
+    out = np.zeros(len(r), dtype=LTE_DTYPE)
+    ixwl = np.floor(np.log(wl) / grid.ratiolg + 0.5).astype(np.int64)
+    out["nbuff"] = (ixwl - grid.ixwlbeg + 1).astype(np.int32)
+    gf = powf10(r["loggf"])
+    frelin = 2.99792458e17 / wl
+    cgf = 0.026538 / 1.77245 * gf.astype(np.float64) / frelin
+    frq4pi = 12.5664 * frelin
+    code = r["code"].astype(np.float32)
+    nelem = np.floor(code).astype(np.int32)
+    icharge = np.rint((code - nelem.astype(np.float32)) * np.float32(100.0)).astype(np.int32)
+    out["nelion"] = nelem * 6 - 6 + (icharge + 1)
+    out["cgf"] = cgf.astype(np.float32)
+    out["elo"] = np.minimum(np.abs(r["e"]), np.abs(r["ep"])).astype(np.float32)
+    for src, dst in (("gr", "gamrf"), ("gs", "gamsf"), ("gw", "gamwf")):
+        g = np.power(10.0, r[src].astype(np.float32).astype(np.float64))
+        out[dst] = (g / frq4pi).astype(np.float32)
+    out["code"] = code
+    return out
+
+
+def synthe_plain_lines(grid: Grid, n: int = 200_000, seed: int = 1) -> np.ndarray:
+    """The plain-SYNTHE line list: n atomic lines over the grid's window."""
+    rec = synthetic_gfall(n, grid.wlbeg, grid.wlend, seed)
+    return gfall_to_lte(rec, grid)

@@ -1,0 +1,28 @@
+/*
+ * a12_oracle_omp.c -- multi-threaded drivers over the oracle routines, for
+ * the CPU baseline only (test infrastructure).  Each thread owns whole
+ * depths (line opacity) or whole wavelength ranges (transfer), so results
+ * are identical to the single-threaded routines.
+ */
+#include "a12_oracle.h"
+
+#include <omp.h>
+#include <stdlib.h>
+
+int or_omp_threads(int n)
+{
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+}
+
+void or_synthe_transfer_omp(const or_synthe_rt *rt, const float *opac,
+                            const double *opac64, double *h_line,
+                            double *h_cont, int i0, int i1)
+{
+    int chunk = 256;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = i0; b < i1; b += chunk) {
+        int e = b + chunk < i1 ? b + chunk : i1;
+        or_synthe_transfer(rt, opac, opac64, h_line + (b - i0), h_cont + (b - i0), b, e);
+    }
+}

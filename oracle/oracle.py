@@ -1,0 +1,239 @@
+"""ctypes wrapper of the CPU oracle (liba12oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+the cpu_baseline leg of bench.py, as the checker.  The product library in
+atlas12_amd/ never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "liba12oracle.so")
+
+P_D = C.POINTER(C.c_double)
+P_F = C.POINTER(C.c_float)
+
+
+class OrLine(C.Structure):
+    _fields_ = [("nbuff", C.c_int32), ("cgf", C.c_float), ("nelion", C.c_int32),
+                ("elo", C.c_float), ("gamrf", C.c_float), ("gamsf", C.c_float),
+                ("gamwf", C.c_float), ("code", C.c_float)]
+
+
+class OrJoshIO(C.Structure):
+    _fields_ = [("nrhox", C.c_int)] + [(k, P_D) for k in (
+        "acont", "aline", "sigmac", "sigmal", "scont", "sline", "rhox", "bnu",
+        "coefj", "coefh", "taunu", "snu", "jnu", "hnu", "jmins")] + [
+        ("knu1", C.c_double), ("maxj", C.c_int), ("vef_sweeps", C.c_int),
+        ("vef_unconverged", C.c_int), ("pivot_floor_hits", C.c_int)]
+
+
+class OrSyntheAtm(C.Structure):
+    _fields_ = [("nrhox", C.c_int), ("length", C.c_int), ("wbegin", C.c_double),
+                ("ratio", C.c_double), ("resolu", C.c_double), ("nedge", C.c_int),
+                ("wledge", P_D), ("nf", C.c_int), ("continall", P_D), ("mw6", C.c_int),
+                ("xnfpel", P_D), ("dopple", P_D)] + [(k, P_D) for k in (
+                    "rho", "t", "xne", "xnfh", "xnfhe1", "xnfh2", "hckt")]
+
+
+class OrSyntheRT(C.Structure):
+    _fields_ = [("nrhox", C.c_int), ("length", C.c_int), ("wbegin", C.c_double),
+                ("ratio", C.c_double), ("nedge", C.c_int), ("wledge", P_D), ("nf", C.c_int),
+                ("contabs", P_D), ("contscat", P_D), ("hkt_syn", P_D), ("hkt_atl", P_D),
+                ("rhox", P_D), ("coefj", P_D), ("coefh", P_D)]
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            build()
+        L = C.CDLL(LIBPATH)
+        L.or_voigt_ref.restype = C.c_float
+        L.or_voigt_ref.argtypes = [C.c_float, C.c_float]
+        L.or_voigt_f64.restype = C.c_double
+        L.or_voigt_f64.argtypes = [C.c_double, C.c_double]
+        L.or_map1.restype = C.c_int
+        L.or_solvit.restype = C.c_int
+        L.or_expi.restype = C.c_double
+        L.or_expi.argtypes = [C.c_int, C.c_double]
+        L.or_josh.restype = C.c_double
+        L.or_omp_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a, t=P_D):
+    return a.ctypes.data_as(t)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ---------------------------------------------------------------- numerics
+def voigt_ref(v, a):
+    L = lib()
+    return np.array([L.or_voigt_ref(float(x), float(y)) for x, y in zip(v, a)], np.float32)
+
+
+def voigt_f64(v, a):
+    L = lib()
+    return np.array([L.or_voigt_f64(float(x), float(y)) for x, y in zip(v, a)])
+
+
+def solvit(a, b):
+    a = np.asfortranarray(a, dtype=np.float64).copy(order="F")
+    b = _f64(b).copy()
+    lib().or_solvit(a.ctypes.data_as(P_D), C.c_int(len(b)), _p(b))
+    return b
+
+
+def map1(xold, fold, xnew):
+    xold, fold, xnew = _f64(xold), _f64(fold), _f64(xnew)
+    out = np.zeros(len(xnew))
+    r = lib().or_map1(_p(xold), _p(fold), C.c_int(len(xold)), _p(xnew), _p(out),
+                      C.c_int(len(xnew)))
+    return out, r
+
+
+def integ(x, f, start):
+    x, f = _f64(x), _f64(f)
+    out = np.zeros(len(x))
+    lib().or_integ(_p(x), _p(f), _p(out), C.c_int(len(x)), C.c_double(start))
+    return out
+
+
+def parcoe(f, x):
+    f, x = _f64(f), _f64(x)
+    a, b, c = np.zeros(len(x)), np.zeros(len(x)), np.zeros(len(x))
+    lib().or_parcoe(_p(f), _p(x), _p(a), _p(b), _p(c), C.c_int(len(x)))
+    return a, b, c
+
+
+def deriv(x, f):
+    x, f = _f64(x), _f64(f)
+    out = np.zeros(len(x))
+    lib().or_deriv(_p(x), _p(f), _p(out), C.c_int(len(x)))
+    return out
+
+
+def expi(n, x):
+    return lib().or_expi(int(n), float(x))
+
+
+def josh(ifscat, ifsurf, acont, aline, sigmac, sigmal, scont, sline, rhox, bnu, coefj, coefh):
+    n = len(rhox)
+    keep = [_f64(a) for a in (acont, aline, sigmac, sigmal, scont, sline, rhox, bnu, coefj, coefh)]
+    outs = [np.zeros(n) for _ in range(5)]
+    io = OrJoshIO()
+    io.nrhox = n
+    for k, a in zip(("acont", "aline", "sigmac", "sigmal", "scont", "sline", "rhox", "bnu",
+                     "coefj", "coefh"), keep):
+        setattr(io, k, _p(a))
+    for k, a in zip(("taunu", "snu", "jnu", "hnu", "jmins"), outs):
+        setattr(io, k, _p(a))
+    h1 = lib().or_josh(C.byref(io), C.c_int(ifscat), C.c_int(ifsurf))
+    return dict(hnu1=h1, knu1=io.knu1, taunu=outs[0], snu=outs[1], jnu=outs[2], hnu=outs[3],
+                jmins=outs[4], maxj=io.maxj, vef_sweeps=io.vef_sweeps,
+                vef_unconverged=io.vef_unconverged, pivot_floor_hits=io.pivot_floor_hits)
+
+
+# ---------------------------------------------------------------- SYNTHE
+class _Keep:
+    """Holds numpy buffers alive for the lifetime of a ctypes struct."""
+
+
+def _atm(st) -> tuple:
+    k = _Keep()
+    k.arrs = {n: _f64(getattr(st, n)) for n in ("wledge", "continall", "xnfpel", "dopple", "rho",
+                                               "t", "xne", "xnfh", "xnfhe1", "xnfh2", "hckt")}
+    a = OrSyntheAtm()
+    a.nrhox, a.length = st.nrhox, st.length
+    a.wbegin, a.ratio, a.resolu = st.wbegin, st.ratio, st.resolu
+    a.nedge = len(st.wledge)
+    a.nf = st.continall.shape[1]
+    a.mw6 = st.xnfpel.shape[1]
+    for n, arr in k.arrs.items():
+        setattr(a, n, _p(arr))
+    return a, k
+
+
+def synthe_continuum(st, j):
+    a, keep = _atm(st)
+    out = np.zeros(st.length, np.float32)
+    out64 = np.zeros(st.length)
+    lib().or_synthe_continuum(C.byref(a), C.c_int(j), _p(out, P_F), _p(out64))
+    return out, out64
+
+
+def synthe_line_opacity(st, lines, prec="ref", want_nstep=False):
+    """Opacity matrix (nrhox, length); prec 'ref' (FP32, reference types) or
+    'f64'.  Returns (matrix, evals[, nstep])."""
+    a, keep = _atm(st)
+    lines = np.ascontiguousarray(lines)
+    nl = len(lines)
+    ev = C.c_int64(0)
+    ns = np.zeros((st.nrhox, nl), np.int32) if want_nstep else None
+    if prec == "ref":
+        out = np.zeros((st.nrhox, st.length), np.float32)
+        lib().or_synthe_line_opacity(C.byref(a), lines.ctypes.data_as(C.c_void_p), C.c_int(nl),
+                                     C.c_int(0), _p(out, P_F), None, C.byref(ev),
+                                     ns.ctypes.data_as(C.c_void_p) if want_nstep else None)
+    else:
+        out = np.zeros((st.nrhox, st.length))
+        lib().or_synthe_line_opacity(C.byref(a), lines.ctypes.data_as(C.c_void_p), C.c_int(nl),
+                                     C.c_int(1), None, _p(out), C.byref(ev),
+                                     ns.ctypes.data_as(C.c_void_p) if want_nstep else None)
+    if want_nstep:
+        return out, ev.value, ns
+    return out, ev.value
+
+
+def _rt(st, coefj, coefh):
+    k = _Keep()
+    k.arrs = {n: _f64(v) for n, v in (("wledge", st.wledge), ("contabs", st.contabs),
+                                      ("contscat", st.contscat), ("hkt_syn", st.hkt_syn),
+                                      ("hkt_atl", st.hkt_atl), ("rhox", st.rhox),
+                                      ("coefj", coefj), ("coefh", coefh))}
+    r = OrSyntheRT()
+    r.nrhox, r.length, r.wbegin, r.ratio = st.nrhox, st.length, st.wbegin, st.ratio
+    r.nedge = len(st.wledge)
+    r.nf = st.contabs.shape[1]
+    for n, arr in k.arrs.items():
+        setattr(r, n, _p(arr))
+    return r, k
+
+
+def synthe_transfer(st, opac, coefj, coefh, i0=0, i1=None, threads=1):
+    """H_line, H_cont for wavelength points [i0, i1) from the opacity matrix
+    (nrhox, length) FP32 (reference) or FP64."""
+    i1 = st.length if i1 is None else i1
+    r, keep = _rt(st, coefj, coefh)
+    hl = np.zeros(i1 - i0)
+    hc = np.zeros(i1 - i0)
+    opac = np.ascontiguousarray(opac)
+    if opac.dtype == np.float32:
+        args = (_p(opac, P_F), None)
+    else:
+        opac = _f64(opac)
+        args = (None, _p(opac))
+    L = lib()
+    if threads != 1:
+        L.or_omp_threads(int(threads))
+        L.or_synthe_transfer_omp(C.byref(r), *args, _p(hl), _p(hc), C.c_int(i0), C.c_int(i1))
+    else:
+        L.or_synthe_transfer(C.byref(r), *args, _p(hl), _p(hc), C.c_int(i0), C.c_int(i1))
+    return hl, hc

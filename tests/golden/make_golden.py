@@ -1,0 +1,356 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures under tests/golden/ from the reference Fortran.
+
+Runs in the development container only (it needs /root/reference and the
+reference build from ``make -C oracle/ref``).  It
+
+1. builds a synthetic data root: symlinks to the reference's data tables
+   plus a generated ``lines.list`` naming a seeded synthetic gfall file;
+2. runs the unmodified reference ``synthe.exe`` on a seeded synthetic deck
+   (golden emergent spectrum, the ``.spec`` file);
+3. runs ``ref_synthe_dump`` (the reference's own set-up stages) to capture
+   the state SYNTHE's hot loops read, trimmed to the continuum-edge
+   intervals the window touches;
+4. runs ``ref_numerics`` on seeded vectors for the reference's Voigt,
+   SOLVIT, MAP1, INTEG, PARCOE, DERIV, EXPI and JOSH routines.
+
+Fixtures are data only (inputs and the reference's outputs).  Usage:
+    python tests/golden/make_golden.py [--only NAME ...]
+"""
+from __future__ import annotations
+
+import argparse
+import hashlib
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+from atlas12_amd import workloads as wl  # noqa: E402
+
+REF = "/root/reference"
+REFBIN = os.path.join(REPO, "oracle", "_ref", "bin")
+MW = 139
+
+
+def data_root(tmp: str, gfall_path: str) -> str:
+    root = os.path.join(tmp, "a12root")
+    d = os.path.join(root, "data")
+    os.makedirs(d, exist_ok=True)
+    for name in os.listdir(os.path.join(REF, "data")):
+        if name == "lines.list":
+            continue
+        os.symlink(os.path.join(REF, "data", name), os.path.join(d, name))
+    shutil.copy(gfall_path, os.path.join(d, "gfall_synth.dat"))
+    with open(os.path.join(d, "lines.list"), "w") as fh:
+        fh.write("gfall\tgfall_synth.dat\n")
+    return root
+
+
+def read_dump(path: str) -> dict:
+    with open(path, "rb") as fh:
+        buf = fh.read()
+    off = 0
+
+    def take(dt, n):
+        nonlocal off
+        a = np.frombuffer(buf, dtype=dt, count=n, offset=off)
+        off += a.nbytes
+        return a.copy()
+    n, length, ixb, nedge, numnu, mw, nlte, nnlte = take("<i4", 8)
+    wbeg, rat, res = take("<f8", 3)
+    d = dict(nrhox=int(n), length=int(length), ixwlbeg=int(ixb), nedge=int(nedge),
+             numnu=int(numnu), mw=int(mw), nlines=int(nlte), nlines_nlte=int(nnlte),
+             wbegin=float(wbeg), ratio=float(rat), resolu=float(res))
+    d["wledge"] = np.abs(take("<f8", nedge))
+    for k in ("continall", "contabs", "contscat"):
+        d[k] = take("<f8", numnu * n).reshape(n, numnu)
+    d["xnfpel"] = take("<f8", 6 * mw * n).reshape(n, 6 * mw)
+    d["dopple"] = take("<f8", 6 * mw * n).reshape(n, 6 * mw)
+    for k in ("t", "xne", "rho", "xnfh", "xnfhe1", "xnfh2", "hckt", "hkt_syn", "hkt_atl", "rhox"):
+        d[k] = take("<f8", n)
+    d["coefj"] = take("<f8", 51 * 51)
+    d["coefh"] = take("<f8", 51 * 51)
+    d["lines"] = np.frombuffer(buf, dtype=wl.LTE_DTYPE, count=nlte, offset=off).copy()
+    return d
+
+
+def trim_edges(d: dict) -> dict:
+    """Keep only the continuum-edge intervals the grid touches (the
+    interpolation reads no others), renumbered from 1."""
+    w = d["wledge"]
+    L, wb, r = d["length"], d["wbegin"], d["ratio"]
+    wave = wb * r ** np.arange(L, dtype=np.float64)
+    ne = d["nedge"]
+    # interval index per point exactly as the depth loop advances it
+    ie = 0
+    used = set()
+    for x in (wave[0], wave[-1]):
+        pass
+    for x in wave:
+        while x >= w[ie + 1] and ie < ne - 2:
+            ie += 1
+        used.add(ie)
+    lo, hi = min(used), max(used)
+    out = dict(d)
+    out["wledge"] = w[lo:hi + 2].copy()
+    out["nedge"] = hi - lo + 2
+    for k in ("continall", "contabs", "contscat"):
+        out[k] = d[k][:, 3 * lo:3 * (hi + 1)].copy()
+    out["numnu"] = 3 * (hi - lo + 1)
+    return out
+
+
+def save_state(path: str, d: dict, species_nelion=None) -> None:
+    """Atmosphere state for the hot path; xnfpel/dopple kept only for the
+    ion slots the line list uses (others are zero when rebuilt)."""
+    if species_nelion is None:
+        species_nelion = np.unique(d["lines"]["nelion"])
+    idx = np.asarray(sorted(set(int(x) - 1 for x in species_nelion)), dtype=np.int32)
+    keep = {k: d[k] for k in ("t", "xne", "rho", "xnfh", "xnfhe1", "xnfh2", "hckt",
+                              "hkt_syn", "hkt_atl", "rhox", "wledge", "continall",
+                              "contabs", "contscat")}
+    np.savez_compressed(path, nrhox=d["nrhox"], length=d["length"], ixwlbeg=d["ixwlbeg"],
+                        wbegin=d["wbegin"], ratio=d["ratio"], resolu=d["resolu"],
+                        mw6=6 * MW, ion_idx=idx, xnfpel_sub=d["xnfpel"][:, idx],
+                        dopple_sub=d["dopple"][:, idx], **keep)
+
+
+def read_spec(path: str) -> np.ndarray:
+    return np.loadtxt(path)
+
+
+def run(cmd, cwd, env=None):
+    r = subprocess.run(cmd, cwd=cwd, env=env, capture_output=True, text=True)
+    if r.returncode != 0:
+        print(r.stdout[-3000:], r.stderr[-3000:])
+        raise SystemExit(f"command failed: {cmd}")
+    return r
+
+
+def synthe_case(name: str, atm: wl.Atmosphere, wlbeg: float, wlend: float, resolu: float,
+                nlines: int, seed: int, line_span=None, keep_spec=True, **gkw) -> dict:
+    tmp = tempfile.mkdtemp(prefix="a12g_")
+    span = line_span or (wlbeg - 0.5, wlend + 0.5)
+    rec = wl.synthetic_gfall(nlines, span[0], span[1], seed, **gkw)
+    gpath = os.path.join(tmp, "gfall.dat")
+    wl.write_gfall(rec, gpath)
+    root = data_root(tmp, gpath)
+    deck = os.path.join(tmp, f"{name}.dat")
+    with open(deck, "w") as fh:
+        fh.write(atm.deck())
+    env = dict(os.environ, ATLAS12=root)
+    dump = os.path.join(tmp, "dump.bin")
+    run([os.path.join(REFBIN, "ref_synthe_dump"), deck, repr(wlbeg), repr(wlend),
+         repr(resolu), dump], tmp, env)
+    d = read_dump(dump)
+    out = {"dump": d, "gfall": rec}
+    if keep_spec:
+        run([os.path.join(REFBIN, "synthe.exe"), deck, f"wlbeg={wlbeg}", f"wlend={wlend}",
+             f"resolu={resolu}"], tmp, env)
+        out["spec"] = read_spec(os.path.join(tmp, f"{name}.spec"))
+    shutil.rmtree(tmp)
+    return out
+
+
+def golden_synthe_small():
+    """5 nm window at R = 500,000 on the plain-SYNTHE atmosphere; line
+    density of the 200k-line workload, plus lines just outside the window
+    (off-grid centres) and a few strong resonance-like lines."""
+    atm = wl.scurve_atmosphere()
+    c = synthe_case("synsmall", atm, 500.0, 505.0, 500000.0, 4000, seed=11,
+                    line_span=(499.5, 505.5))
+    d = trim_edges(c["dump"])
+    save_state(os.path.join(HERE, "synthe_small_state.npz"), d)
+    grid = wl.Grid.make(500.0, 505.0, 500000.0)
+    np.savez_compressed(os.path.join(HERE, "synthe_small_ref.npz"),
+                        lines=d["lines"].view(np.uint8), spec=c["spec"],
+                        gfall=c["gfall"], grid_length=grid.length)
+    np.savez_compressed(os.path.join(HERE, "josh_coef.npz"), coefj=d["coefj"],
+                        coefh=d["coefh"])
+    return {"synthe_small": {"nlines": int(d["nlines"]), "length": d["length"],
+                             "nedge": d["nedge"]}}
+
+
+def golden_state_plain():
+    """Atmosphere state of the plain-SYNTHE workload (400-700 nm, R=5e5) for
+    full-size runs; lines are regenerated from their seed on any machine."""
+    atm = wl.scurve_atmosphere()
+    c = synthe_case("plain", atm, 400.0, 700.0, 500000.0, 2000, seed=1, keep_spec=False)
+    d = trim_edges(c["dump"])
+    nel = sorted({int(round(x)) for x in (26 * 6 - 6 + 1, 26 * 6 - 6 + 2, 20 * 6 - 6 + 1,
+                                          20 * 6 - 6 + 2, 22 * 6 - 6 + 1, 11 * 6 - 6 + 1,
+                                          12 * 6 - 6 + 1)})
+    save_state(os.path.join(HERE, "synthe_plain_state.npz"), d, nel)
+    return {"synthe_plain_state": {"length": d["length"], "nedge": d["nedge"]}}
+
+
+def _ref_numerics(op, payload: bytes, outbytes: int, extra=()):
+    tmp = tempfile.mkdtemp(prefix="a12n_")
+    fin, fout = os.path.join(tmp, "in.bin"), os.path.join(tmp, "out.bin")
+    with open(fin, "wb") as fh:
+        fh.write(payload)
+    run([os.path.join(REFBIN, "ref_numerics"), op, fin, fout, *extra], tmp)
+    with open(fout, "rb") as fh:
+        b = fh.read()
+    shutil.rmtree(tmp)
+    assert len(b) == outbytes, (op, len(b), outbytes)
+    return b
+
+
+def golden_numerics():
+    rng = np.random.default_rng(7)
+    out = {}
+    # Voigt: a grid over the regimes plus random points
+    v = np.concatenate([np.linspace(0, 20, 201), rng.uniform(0, 30, 3000),
+                        rng.uniform(0, 1, 1000), np.zeros(50)]).astype(np.float32)
+    a = np.concatenate([np.full(201, 0.01), 10 ** rng.uniform(-6, 1.5, 3000),
+                        10 ** rng.uniform(-4, 0, 1000), 10 ** np.linspace(-6, 1, 50)]).astype(np.float32)
+    n = len(v)
+    h = np.frombuffer(_ref_numerics("voigt", np.int32(n).tobytes() + v.tobytes() + a.tobytes(),
+                                    4 * n), dtype="<f4")
+    vec = {"voigt_v": v, "voigt_a": a, "voigt_h": h}
+    # SOLVIT on I - diag(alpha) C style systems and random dense systems
+    coef = np.load(os.path.join(HERE, "josh_coef.npz"))
+    cj = coef["coefj"].reshape(51, 51, order="F")
+    mats, rhs = [], []
+    for k in range(24):
+        if k < 16:
+            al = rng.uniform(0, 0.999 if k % 2 else 0.3, 51)
+            A = np.eye(51) - al[:, None] * cj
+        else:
+            A = rng.normal(size=(51, 51)) + 3 * np.eye(51)
+        mats.append(A)
+        rhs.append(rng.uniform(0.1, 2.0, 51))
+    pay = np.int32([51, len(mats)]).tobytes()
+    for A, b in zip(mats, rhs):
+        pay += np.asfortranarray(A).tobytes(order="F") + b.tobytes()
+    x = np.frombuffer(_ref_numerics("solvit", pay, 8 * 51 * len(mats)), "<f8").reshape(-1, 51)
+    vec.update(solvit_a=np.stack(mats), solvit_b=np.stack(rhs), solvit_x=x)
+    # MAP1 / INTEG / PARCOE / DERIV on log-spaced optical-depth-like grids
+    nold, nnew, cnt = 72, 51, 20
+    xo = np.sort(10 ** rng.uniform(-6, 3, (cnt, nold)), axis=1)
+    fo = np.exp(-xo / rng.uniform(0.5, 50, (cnt, 1))) + rng.uniform(0, 1, (cnt, 1)) * np.log1p(xo)
+    xn = np.tile(np.array([0.0, 3.2e-6, 5.6e-6, 1e-5, 1.8e-5, 3.2e-5, 5.6e-5, 1e-4, 1.8e-4, 3.2e-4,
+                           5.6e-4, 1e-3, 1.8e-3, 3.2e-3, 5.6e-3, 0.01, 0.016, 0.025, 0.042, 0.065,
+                           0.096, 0.139, 0.196, 0.273, 0.375, 0.5, 0.63, 0.78, 0.95, 1.15, 1.35, 1.6,
+                           1.85, 2.15, 2.45, 2.75, 3.15, 3.65, 4.25, 5.0, 6.0, 7.0, 8.0, 9.0, 10.0,
+                           11.5, 13.0, 14.5, 16.0, 18.0, 20.0]), (cnt, 1))
+    xn[cnt // 2:] *= rng.uniform(0.5, 5e3, (cnt - cnt // 2, 1))
+    pay = np.int32([nold, nnew, cnt]).tobytes()
+    for k in range(cnt):
+        pay += xo[k].tobytes() + fo[k].tobytes() + xn[k].tobytes()
+    b = _ref_numerics("map1", pay, cnt * (8 * nnew + 4))
+    fn = np.zeros((cnt, nnew))
+    ret = np.zeros(cnt, np.int32)
+    rec = np.dtype([("f", "<f8", nnew), ("r", "<i4")])
+    arr = np.frombuffer(b, rec)
+    fn[:], ret[:] = arr["f"], arr["r"]
+    vec.update(map1_xold=xo, map1_fold=fo, map1_xnew=xn, map1_fnew=fn, map1_ret=ret)
+    st = rng.uniform(0, 1e-3, cnt)
+    pay = np.int32([nold, cnt]).tobytes()
+    for k in range(cnt):
+        pay += xo[k].tobytes() + fo[k].tobytes() + np.float64(st[k]).tobytes()
+    fi = np.frombuffer(_ref_numerics("integ", pay, 8 * nold * cnt), "<f8").reshape(cnt, nold)
+    vec.update(integ_x=xo, integ_f=fo, integ_start=st, integ_out=fi)
+    pay = np.int32([nold, cnt]).tobytes()
+    for k in range(cnt):
+        pay += fo[k].tobytes() + xo[k].tobytes()
+    pc = np.frombuffer(_ref_numerics("parcoe", pay, 24 * nold * cnt), "<f8").reshape(cnt, 3, nold)
+    vec.update(parcoe_out=pc)
+    pay = np.int32([nold, cnt]).tobytes()
+    for k in range(cnt):
+        pay += xo[k].tobytes() + fo[k].tobytes()
+    dv = np.frombuffer(_ref_numerics("deriv", pay, 8 * nold * cnt), "<f8").reshape(cnt, nold)
+    vec.update(deriv_out=dv)
+    # EXPI over its three ranges and the recurrence
+    ex = np.concatenate([10 ** rng.uniform(-6, 0, 200), rng.uniform(1, 4, 200),
+                         rng.uniform(4, 80, 200), [0.5, 1.0, 4.0, 1e-8]])
+    nn = rng.integers(1, 5, len(ex)).astype(np.int32)
+    pay = np.int32(len(ex)).tobytes() + nn.tobytes() + ex.tobytes()
+    e = np.frombuffer(_ref_numerics("expi", pay, 8 * len(ex)), "<f8")
+    vec.update(expi_n=nn, expi_x=ex, expi_out=e)
+    np.savez_compressed(os.path.join(HERE, "numerics_ref.npz"), **vec)
+    out["numerics"] = {k: list(np.shape(v)) for k, v in vec.items()}
+    return out
+
+
+def golden_josh():
+    """JOSH on the small-case atmosphere at several wavelengths, both output
+    modes, with scattering and line opacity of several strengths, plus the
+    deep-only branch (TAUNU(1) > 20)."""
+    st = np.load(os.path.join(HERE, "synthe_small_state.npz"))
+    rng = np.random.default_rng(5)
+    n = int(st["nrhox"])
+    rhox = st["rhox"]
+    hkt = st["hkt_atl"]
+    cases = []
+    for k in range(40):
+        wave = rng.uniform(300, 2000)
+        freq = 2.99792458e17 / wave
+        ehv = np.exp(-freq * hkt)
+        stim = 1 - ehv
+        bnu = 2 * 6.62607015e-27 / 2.99792458e10 ** 2 * 1e45 * (freq / 1e15) ** 3 * ehv / stim
+        acont = 10 ** rng.uniform(-2, 0, 1) * (rhox / rhox[0]) ** 0.3 * np.exp(rng.normal(0, 0.1, n))
+        sigmac = acont * 10 ** rng.uniform(-4, 0.5)
+        aline = acont * 10 ** rng.uniform(-3, 4) * np.exp(-np.linspace(0, 3, n) * rng.uniform(0, 2))
+        if k % 5 == 0:
+            aline[:] = 0
+        sigmal = np.zeros(n) if k % 3 else aline * rng.uniform(0, 0.5)
+        if k == 39:
+            acont = acont * 1e6  # TAUNU(1) > 20
+        ifscat = 1 if k % 7 else 0
+        ifsurf = k % 2
+        cases.append((ifscat, ifsurf, acont, aline, sigmac, sigmal, bnu.copy(), bnu.copy(), rhox, bnu))
+    pay = np.int32([n, len(cases)]).tobytes()
+    for c in cases:
+        pay += np.int32([c[0], c[1]]).tobytes()
+        for arr in c[2:]:
+            pay += np.asarray(arr, np.float64).tobytes()
+    data = _ref_numerics("josh", pay, len(cases) * 8 * (2 + 5 * n),
+                         extra=(os.path.join(REF, "data") + "/",))
+    o = np.frombuffer(data, "<f8").reshape(len(cases), 2 + 5 * n)
+    np.savez_compressed(os.path.join(HERE, "josh_ref.npz"),
+                        ifscat=np.int32([c[0] for c in cases]),
+                        ifsurf=np.int32([c[1] for c in cases]),
+                        inputs=np.stack([np.stack(c[2:10]) for c in cases]),
+                        out=o)
+    return {"josh": {"cases": len(cases)}}
+
+
+STEPS = {"synthe_small": golden_synthe_small, "state_plain": golden_state_plain,
+         "numerics": golden_numerics, "josh": golden_josh}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--only", nargs="*")
+    args = ap.parse_args()
+    man_path = os.path.join(HERE, "MANIFEST.json")
+    man = json.load(open(man_path)) if os.path.exists(man_path) else {}
+    comp = subprocess.run(["amdflang", "--version"], capture_output=True, text=True).stdout.splitlines()[0]
+    man["compiler"] = comp
+    man["fflags"] = "-O3 -ffp-contract=off"
+    for name, fn in STEPS.items():
+        if args.only and name not in args.only:
+            continue
+        print("golden:", name, flush=True)
+        man.setdefault("cases", {}).update(fn())
+    files = {}
+    for f in sorted(os.listdir(HERE)):
+        if f.endswith(".npz"):
+            files[f] = hashlib.sha256(open(os.path.join(HERE, f), "rb").read()).hexdigest()
+    man["sha256"] = files
+    with open(man_path, "w") as fh:
+        json.dump(man, fh, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()
