@@ -1,0 +1,274 @@
+// a12_capi.hip -- the extern "C" entry points declared in include/a12gpu.h.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <new>
+
+#include "a12_internal.h"
+
+namespace {
+bool ctx_ok(const a12_ctx *ctx) { return ctx != nullptr; }
+}  // namespace
+
+extern "C" {
+
+int a12_ctx_create(int32_t device, a12_ctx **out)
+{
+    if (!out) return A12_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return A12_ERR_HIP;
+    if (device < 0 || device >= ndev) return A12_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return A12_ERR_HIP;
+    a12_ctx *c = new (std::nothrow) a12_ctx();
+    if (!c) return A12_ERR_OOM;
+    c->device = device;
+    // default priority stream
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return A12_ERR_HIP;
+    }
+    *out = c;
+    return A12_OK;
+}
+
+int a12_ctx_destroy(a12_ctx *ctx)
+{
+    if (!ctx) return A12_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    SyntheDev &S = ctx->syn;
+    DevBuf *bufs[] = {&S.wledge, &S.continall, &S.contabs, &S.contscat, &S.xnfpel, &S.dopple,
+                      &S.depth, &S.l_nbuff, &S.l_nelion, &S.l_cgf, &S.l_elo, &S.l_gr, &S.l_gs,
+                      &S.l_gw, &S.xnfdop, &S.dscal, &S.cont32, &S.cont64, &S.p_nstep, &S.p_n10,
+                      &S.p_kap0, &S.p_adamp, &S.p_dv, &S.p_xwing, &S.emax, &S.lclass,
+                      &S.cls_cnt, &S.cls_off, &S.perm, &S.nbuffP, &S.cls_range, &S.blk_evals,
+                      &S.evals_out, &S.opacity, &S.hline, &S.hcont, &ctx->coefj, &ctx->coefh,
+                      &ctx->jb_in, &ctx->jb_out};
+    for (DevBuf *b : bufs) b->release();
+    for (hipEvent_t e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return A12_OK;
+}
+
+const char *a12_last_error(const a12_ctx *ctx)
+{
+    if (!ctx) return "null context";
+    return ctx->err.c_str();
+}
+
+void *a12_stream(a12_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int a12_set_operators(a12_ctx *ctx, const double *coefj, const double *coefh)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    if (!coefj || !coefh) {
+        ctx->err = "a12_set_operators: null matrix";
+        return A12_ERR_ARG;
+    }
+    A12_HIP(hipSetDevice(ctx->device));
+    A12_HIP(ctx->coefj.reserve(sizeof(double) * 51 * 51));
+    A12_HIP(ctx->coefh.reserve(sizeof(double) * 51 * 51));
+    A12_HIP(hipMemcpyAsync(ctx->coefj.p, coefj, sizeof(double) * 51 * 51, hipMemcpyHostToDevice,
+                           ctx->stream));
+    A12_HIP(hipMemcpyAsync(ctx->coefh.p, coefh, sizeof(double) * 51 * 51, hipMemcpyHostToDevice,
+                           ctx->stream));
+    A12_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->have_ops = true;
+    return A12_OK;
+}
+
+int a12_synthe_load(a12_ctx *ctx, const a12_synthe_atm *atm, const a12_lte_line *lines,
+                    int32_t nlines)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_synthe_load(ctx, atm, lines, nlines);
+}
+
+int a12_synthe_opacity_run(a12_ctx *ctx, int32_t prec)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_synthe_run(ctx, prec);
+}
+
+int a12_synthe_opacity_fetch(a12_ctx *ctx, void *opacity_out)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    SyntheDev &S = ctx->syn;
+    if (S.opac_prec < 0) {
+        ctx->err = "a12_synthe_opacity_fetch: nothing computed";
+        return A12_ERR_STATE;
+    }
+    if (!opacity_out) {
+        ctx->err = "a12_synthe_opacity_fetch: null output";
+        return A12_ERR_ARG;
+    }
+    const size_t rb = S.opac_prec == A12_PREC_FP64 ? sizeof(double) : sizeof(float);
+    A12_HIP(hipMemcpyAsync(opacity_out, S.opacity.p, rb * (size_t)S.L * S.nrhox,
+                           hipMemcpyDeviceToHost, ctx->stream));
+    long long ev = 0;
+    A12_HIP(hipMemcpyAsync(&ev, S.evals_out.p, sizeof(long long), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    A12_HIP(hipStreamSynchronize(ctx->stream));
+    S.evals = ev;
+    return A12_OK;
+}
+
+const void *a12_synthe_opacity_device(a12_ctx *ctx, int32_t *prec_out)
+{
+    if (!ctx) return nullptr;
+    if (prec_out) *prec_out = ctx->syn.opac_prec;
+    return ctx->syn.opacity.p;
+}
+
+int64_t a12_synthe_evals(a12_ctx *ctx)
+{
+    if (!ctx || ctx->syn.opac_prec < 0) return -1;
+    long long ev = 0;
+    if (hipMemcpyAsync(&ev, ctx->syn.evals_out.p, sizeof(long long), hipMemcpyDeviceToHost,
+                       ctx->stream) != hipSuccess)
+        return -1;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return -1;
+    ctx->syn.evals = ev;
+    return ev;
+}
+
+int a12_synthe_line_opacity(a12_ctx *ctx, const a12_synthe_atm *atm,
+                            const a12_lte_line *lines, int32_t nlines, int32_t prec,
+                            void *opacity_out, int64_t *evals_out)
+{
+    int rc = a12_synthe_load(ctx, atm, lines, nlines);
+    if (rc) return rc;
+    rc = a12_synthe_opacity_run(ctx, prec);
+    if (rc) return rc;
+    rc = a12_synthe_opacity_fetch(ctx, opacity_out);
+    if (rc) return rc;
+    if (evals_out) *evals_out = ctx->syn.evals;
+    return A12_OK;
+}
+
+int a12_synthe_transfer_run(a12_ctx *ctx)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_synthe_transfer(ctx);
+}
+
+int a12_synthe_transfer_fetch(a12_ctx *ctx, double *hnu_line, double *hnu_cont)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    SyntheDev &S = ctx->syn;
+    if (!S.transferred) {
+        ctx->err = "a12_synthe_transfer_fetch: no transfer result";
+        return A12_ERR_STATE;
+    }
+    if (!hnu_line || !hnu_cont) {
+        ctx->err = "a12_synthe_transfer_fetch: null output";
+        return A12_ERR_ARG;
+    }
+    A12_HIP(hipMemcpyAsync(hnu_line, S.hline.p, sizeof(double) * S.L, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    A12_HIP(hipMemcpyAsync(hnu_cont, S.hcont.p, sizeof(double) * S.L, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    A12_HIP(hipStreamSynchronize(ctx->stream));
+    return A12_OK;
+}
+
+int a12_synthe_transfer(a12_ctx *ctx, const a12_synthe_atm *atm, const void *opacity,
+                        int32_t prec, double *hnu_line, double *hnu_cont)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    if (!opacity || (prec != A12_PREC_MIXED && prec != A12_PREC_FP64)) {
+        ctx->err = "a12_synthe_transfer: null opacity or bad prec";
+        return A12_ERR_ARG;
+    }
+    int rc = a12_synthe_load(ctx, atm, nullptr, 0);
+    if (rc) return rc;
+    SyntheDev &S = ctx->syn;
+    const size_t rb = prec == A12_PREC_FP64 ? sizeof(double) : sizeof(float);
+    A12_HIP(S.opacity.reserve(rb * (size_t)S.L * S.nrhox));
+    A12_HIP(hipMemcpyAsync(S.opacity.p, opacity, rb * (size_t)S.L * S.nrhox,
+                           hipMemcpyHostToDevice, ctx->stream));
+    S.opac_prec = prec;
+    rc = a12_synthe_transfer_run(ctx);
+    if (rc) return rc;
+    return a12_synthe_transfer_fetch(ctx, hnu_line, hnu_cont);
+}
+
+int a12_josh_batch(a12_ctx *ctx, int32_t ifscat, int32_t ifsurf, int32_t nrhox, int32_t nbatch,
+                   const double *rhox, const double *acont, const double *aline,
+                   const double *sigmac, const double *sigmal, const double *scont,
+                   const double *sline, const double *bnu, double *hnu1, double *knu1,
+                   double *taunu, double *snu, double *jnu, double *hnu, double *jmins)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    if (nrhox < 4 || nrhox > A12_NRMAX_LAYERS || nbatch < 0 || (ifscat != 0 && ifscat != 1) ||
+        (ifsurf != 0 && ifsurf != 1) || !rhox || !acont || !aline || !sigmac || !sigmal ||
+        !scont || !sline || !bnu) {
+        ctx->err = "a12_josh_batch: bad arguments";
+        return A12_ERR_ARG;
+    }
+    if (nbatch == 0) return A12_OK;
+    A12_HIP(hipSetDevice(ctx->device));
+    const double *in[7] = {acont, aline, sigmac, sigmal, scont, sline, bnu};
+    double *out[5] = {taunu, snu, jnu, hnu, jmins};
+    return a12i_josh_batch(ctx, ifscat, ifsurf, nrhox, nbatch, rhox, in, hnu1, knu1, out);
+}
+
+int a12_set_profiling(a12_ctx *ctx, int32_t on)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    if (on && !ctx->ev[0])
+        for (hipEvent_t &e : ctx->ev) A12_HIP(hipEventCreate(&e));
+    ctx->prof = on != 0;
+    return A12_OK;
+}
+
+int a12_stage_ms(a12_ctx *ctx, float *ms)
+{
+    if (!ctx_ok(ctx) || !ms) return A12_ERR_ARG;
+    for (int i = 0; i < 5; ++i) ms[i] = -1.0f;
+    if (ctx->syn.prof_valid) {
+        A12_HIP(hipEventSynchronize(ctx->ev[4]));
+        A12_HIP(hipEventElapsedTime(&ms[0], ctx->ev[0], ctx->ev[1]));
+        A12_HIP(hipEventElapsedTime(&ms[1], ctx->ev[1], ctx->ev[2]));
+        A12_HIP(hipEventElapsedTime(&ms[2], ctx->ev[2], ctx->ev[3]));
+        A12_HIP(hipEventElapsedTime(&ms[3], ctx->ev[3], ctx->ev[4]));
+    }
+    if (ctx->prof && ctx->syn.transferred) {
+        A12_HIP(hipEventSynchronize(ctx->ev[6]));
+        A12_HIP(hipEventElapsedTime(&ms[4], ctx->ev[5], ctx->ev[6]));
+    }
+    return A12_OK;
+}
+
+int a12_synthe_eval_split(a12_ctx *ctx, int64_t *total, int64_t *core)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    if (ctx->syn.opac_prec < 0) {
+        ctx->err = "a12_synthe_eval_split: nothing computed";
+        return A12_ERR_STATE;
+    }
+    long long ev[2] = {0, 0};
+    A12_HIP(hipMemcpyAsync(ev, ctx->syn.evals_out.p, sizeof(ev), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    A12_HIP(hipStreamSynchronize(ctx->stream));
+    if (total) *total = ev[0];
+    if (core) *core = ev[1];
+    return A12_OK;
+}
+
+int a12_voigt_profile(a12_ctx *ctx, int32_t prec, int32_t n, const float *v, const float *a,
+                      double *h)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_voigt(ctx, prec, n, v, a, h);
+}
+
+}  // extern "C"

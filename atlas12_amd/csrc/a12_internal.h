@@ -1,0 +1,107 @@
+// a12_internal.h -- context and host-side plumbing shared by the HIP
+// translation units of liba12gpu.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/a12gpu.h"
+
+#define A12_NCLASS 16  // extent classes 0..14 plus "cut at every depth"
+#define A12_NRMAX_LAYERS 128  // layers a context accepts (JOSH keeps them in LDS)
+
+// Device buffer with grow-only reallocation.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t n)
+    {
+        if (n <= bytes) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, n ? n : 1);
+        if (e == hipSuccess) bytes = n;
+        return e;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+struct SyntheDev {
+    // dimensions
+    int nrhox = 0, L = 0, nedge = 0, nf = 0, mw6 = 0, nlines = 0;
+    double wbegin = 0, ratio = 0, resolu = 0;
+    // atmosphere (device)
+    DevBuf wledge, continall, contabs, contscat, xnfpel, dopple;
+    DevBuf depth;  // 10 x nrhox doubles: t xne rho xnfh xnfhe1 xnfh2 hckt hkt hkt_atlas rhox
+    // lines sorted by centre (SoA)
+    DevBuf l_nbuff, l_nelion, l_cgf, l_elo, l_gr, l_gs, l_gw;
+    // derived per depth
+    DevBuf xnfdop, dscal;   // xnfdop (nrhox x mw6); dscal: txnxn32/txnxn64/xne32 per depth
+    DevBuf cont32, cont64;  // (nrhox x L)
+    // per (depth, line) setup
+    DevBuf p_nstep, p_n10, p_kap0, p_adamp, p_dv, p_xwing;
+    DevBuf emax, lclass;
+    DevBuf cls_cnt, cls_off, perm, nbuffP, cls_range;  // partition + per-tile ranges
+    DevBuf blk_evals, evals_out;
+    // outputs
+    DevBuf opacity;  // (nrhox x L) float or double
+    int opac_prec = -1;
+    int64_t evals = 0;
+    bool loaded = false;
+    bool prof_valid = false;  // ev[0..4] bracket the last opacity run
+    // transfer
+    DevBuf hline, hcont;
+    bool transferred = false;
+};
+
+struct a12_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    DevBuf coefj, coefh;
+    bool have_ops = false;
+    SyntheDev syn;
+    // scratch for a12_josh_batch
+    DevBuf jb_in, jb_out;
+    // stage timing (a12_set_profiling): events on the context stream
+    bool prof = false;
+    hipEvent_t ev[8] = {};
+};
+
+#define A12_HIP(call)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);          \
+            return e_ == hipErrorOutOfMemory ? A12_ERR_OOM : A12_ERR_HIP;          \
+        }                                                                          \
+    } while (0)
+
+#define A12_CHECK_LAUNCH()                                                         \
+    do {                                                                           \
+        hipError_t e_ = hipGetLastError();                                         \
+        if (e_ != hipSuccess) {                                                    \
+            ctx->err = std::string("kernel launch: ") + hipGetErrorString(e_);     \
+            return A12_ERR_HIP;                                                    \
+        }                                                                          \
+    } while (0)
+
+// Implemented in a12_synthe.hip
+int a12i_synthe_load(a12_ctx *ctx, const a12_synthe_atm *atm, const a12_lte_line *lines,
+                     int32_t nlines);
+int a12i_synthe_run(a12_ctx *ctx, int prec);
+int a12i_voigt(a12_ctx *ctx, int prec, int n, const float *v, const float *a, double *h);
+// Implemented in a12_josh.hip
+int a12i_synthe_transfer(a12_ctx *ctx);
+int a12i_josh_batch(a12_ctx *ctx, int ifscat, int ifsurf, int nrhox, int nbatch,
+                    const double *rhox, const double *const in[7], double *hnu1,
+                    double *knu1, double *const out[5]);

@@ -1,0 +1,779 @@
+// a12_josh.hip -- JOSH transfer on gfx950, one wavefront per frequency.
+//
+// Restates JOSH(IFSCAT, IFSURF), src/atlas12_modules.f90:18398-18776, with
+// its numerics INTEG/PARCOE (:5293-5434), MAP1 (:5570-5692), SOLVIT
+// (:5739-5847) and DERIV (:5228-5266), in FP64.  A workgroup is one wave of
+// 64 lanes and owns one frequency at a time:
+//   * depth vectors live in LDS, lane j (and j + 64) owns depth j;
+//   * MAP1 maps every new abscissa in parallel; the bracket, the boundary
+//     rules and the choice between a fresh backward parabola and the one
+//     inherited from the previous interval's forward parabola are decided
+//     per lane exactly as the serial routine decides them, so values agree
+//     bit for bit;
+//   * SOLVIT's partial-pivot LU of the 51 x 51 system I - diag(alpha) COEFJ
+//     runs with the matrix in LDS, lane = row;
+//   * sums whose order the reference fixes (INTEG's running integral, the
+//     surface flux sum, the deep-branch error) are taken in that order.
+// SYNTHE's wavelength loop (src/synthe.f90:698-718 with setup_opacity_sv
+// :797-833 and process_wavelength_point :848-902) is k_synthe_transfer.
+
+#include <hip/hip_runtime.h>
+
+#include "a12_common.h"
+#include "a12_internal.h"
+
+namespace {
+
+// CH_J, CK_J, XTAU8: the fixed 51-point quadrature of JOSH,
+// src/atlas12_modules.f90:18409-18451.
+__constant__ double cCHJ[51] = {
+    7.15528131e-07, 1.49142693e-06, 1.52106577e-06, 2.98150826e-06, 5.33941056e-06,
+    9.13329677e-06, 1.61715943e-05, 2.97035986e-05, 5.33166603e-05, 9.11154202e-05,
+    1.61084638e-04, 2.95118050e-04, 5.27450291e-04, 8.67939554e-04, 1.61498412e-03,
+    2.50720908e-03, 3.20994272e-03, 5.61912498e-03, 8.60872678e-03, 1.04706492e-02,
+    1.33110350e-02, 1.62635669e-02, 1.90288834e-02, 2.18877215e-02, 2.36015432e-02,
+    2.10819542e-02, 1.80345085e-02, 1.64786074e-02, 1.49382707e-02, 1.19676525e-02,
+    9.90213640e-03, 8.17766134e-03, 6.11252524e-03, 4.84035723e-03, 3.06078210e-03,
+    2.40512565e-03, 2.01712688e-03, 1.33288081e-03, 7.83530239e-04, 4.31428343e-04,
+    1.76504589e-04, 4.75738016e-05, 1.65963702e-05, 5.41117970e-06, 2.08043571e-06,
+    7.11612643e-07, 8.08788982e-08, 1.95130507e-08, 4.33638281e-09, 8.87765583e-10,
+    3.90236420e-11};
+__constant__ double cCKJ[51] = {
+    3.57771910e-07, 7.45730404e-07, 7.60575176e-07, 1.49091113e-06, 2.67016185e-06,
+    4.56793896e-06, 8.08956065e-06, 1.48632944e-05, 2.66928291e-05, 4.56529851e-05,
+    8.08134864e-05, 1.48363324e-04, 2.66052346e-04, 4.39771306e-04, 8.25088180e-04,
+    1.29440730e-03, 1.67680858e-03, 2.98973685e-03, 4.68314718e-03, 5.84855257e-03,
+    7.64854718e-03, 9.63155832e-03, 1.16419578e-02, 1.38551742e-02, 1.54840983e-02,
+    1.42877987e-02, 1.25930300e-02, 1.17983138e-02, 1.09717194e-02, 8.98320694e-03,
+    7.59950886e-03, 6.38808031e-03, 4.86854184e-03, 3.91568616e-03, 2.51398841e-03,
+    2.00142385e-03, 1.70069211e-03, 1.14058319e-03, 6.80292083e-04, 3.80097074e-04,
+    1.57705377e-04, 4.31706540e-05, 1.51795348e-05, 4.98576401e-06, 1.92979223e-06,
+    6.63957223e-07, 7.65236692e-08, 1.84933668e-08, 4.12596224e-09, 8.47334369e-10,
+    3.81791959e-11};
+__constant__ double cXTAU8[51] = {
+    0.0, 0.0000032, 0.0000056, 0.00001, 0.000018, 0.000032, 0.000056, 0.0001, 0.00018,
+    0.00032, 0.00056, 0.001, 0.0018, 0.0032, 0.0056, 0.01, 0.016, 0.025, 0.042, 0.065,
+    0.096, 0.139, 0.196, 0.273, 0.375, 0.5, 0.63, 0.78, 0.95, 1.15, 1.35, 1.6, 1.85,
+    2.15, 2.45, 2.75, 3.15, 3.65, 4.25, 5.0, 6.0, 7.0, 8.0, 9.0, 10.0, 11.5, 13.0, 14.5,
+    16.0, 18.0, 20.0};
+
+constexpr int NX = A12_NXTAU;
+constexpr int NR = A12_NRMAX;
+
+struct JoshLDS {
+    double amat[NX * NX];
+    double xs[64], xsbar[64], xalpha[64], xtmp[64];
+    double taunu[NR], snubar[NR], alpha[NR], abtot[NR];
+    double snu[NR], hnu[NR], jmins[NR], jnu[NR], bnu[NR], rhox[NR];
+    double pa[NR], pb[NR], pc[NR];
+    int ipiv[64];
+    double red[64];
+    int iflag[4];
+};
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+__device__ __forceinline__ void wsync() { __syncthreads(); }
+
+// Wave-wide max.
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+
+// ---- PARCOE + INTEG (IQUAD = 0) -------------------------------------------
+// f, x, outputs a, b, c in LDS; n <= NR.
+__device__ void parcoe_wave(const double *f, const double *x, double *a, double *b, double *c,
+                            int n)
+{
+#pragma clang fp contract(off)
+    const int ln = lane_id();
+    for (int j = ln; j < n; j += 64) {
+        double aj, bj, cj;
+        if (j == 0) {
+            cj = 0.0;
+            bj = (f[1] - f[0]) / (x[1] - x[0]);
+            aj = f[0] - x[0] * bj;
+        } else if (j == n - 1) {
+            cj = 0.0;
+            bj = (f[n - 1] - f[n - 2]) / (x[n - 1] - x[n - 2]);
+            aj = f[n - 1] - x[n - 1] * bj;
+        } else if (n == 3) {  // j == 1: copy of the last-interval fit
+            cj = 0.0;
+            bj = (f[2] - f[1]) / (x[2] - x[1]);
+            aj = f[2] - x[2] * bj;
+        } else if (j == 1 || j == 2) {  // forced linear (step 3)
+            cj = 0.0;
+            bj = (f[j + 1] - f[j]) / (x[j + 1] - x[j]);
+            aj = f[j] - x[j] * bj;
+        } else {
+            const double d = (f[j] - f[j - 1]) / (x[j] - x[j - 1]);
+            cj = f[j + 1] / ((x[j + 1] - x[j]) * (x[j + 1] - x[j - 1])) -
+                 f[j] / ((x[j] - x[j - 1]) * (x[j + 1] - x[j])) +
+                 f[j - 1] / ((x[j] - x[j - 1]) * (x[j + 1] - x[j - 1]));
+            bj = d - (x[j] + x[j - 1]) * cj;
+            aj = f[j - 1] - x[j - 1] * d + x[j] * x[j - 1] * cj;
+        }
+        a[j] = aj;
+        b[j] = bj;
+        c[j] = cj;
+    }
+    wsync();
+    if (n <= 3) return;
+    // smoothing (step 4): J = 2..N-1 reads the unsmoothed J and J+1
+    double na[2], nb[2], nc[2];
+    int k = 0;
+    for (int j = ln; j < n; j += 64, ++k) {
+        na[k] = a[j];
+        nb[k] = b[j];
+        nc[k] = c[j];
+        if (j >= 1 && j <= n - 2 && c[j] != 0.0) {
+            const double wt = fabs(c[j + 1]) / (fabs(c[j + 1]) + fabs(c[j]));
+            na[k] = a[j + 1] + wt * (a[j] - a[j + 1]);
+            nb[k] = b[j + 1] + wt * (b[j] - b[j + 1]);
+            nc[k] = c[j + 1] + wt * (c[j] - c[j + 1]);
+        }
+    }
+    wsync();
+    k = 0;
+    for (int j = ln; j < n; j += 64, ++k) {
+        a[j] = na[k];
+        b[j] = nb[k];
+        c[j] = nc[k];
+    }
+    wsync();
+    if (ln == 0) {  // step 5
+        a[n - 2] = a[n - 1];
+        b[n - 2] = b[n - 1];
+        c[n - 2] = c[n - 1];
+    }
+    wsync();
+}
+
+__device__ void integ_wave(JoshLDS &W, const double *x, const double *f, double *fint, int n,
+                           double start)
+{
+#pragma clang fp contract(off)
+    parcoe_wave(f, x, W.pa, W.pb, W.pc, n);
+    const int ln = lane_id();
+    // interval integrals in parallel (stored in pa), running sum in order
+    double seg[2];
+    int k = 0;
+    for (int i = ln; i < n - 1; i += 64, ++k) {
+        const double xlo = x[i], xhi = x[i + 1];
+        seg[k] = (W.pa[i] + W.pb[i] * 0.5 * (xhi + xlo) +
+                  W.pc[i] / 3.0 * ((xhi + xlo) * xhi + xlo * xlo)) * (xhi - xlo);
+    }
+    wsync();
+    k = 0;
+    for (int i = ln; i < n - 1; i += 64, ++k) W.pa[i] = seg[k];
+    wsync();
+    if (ln == 0) {
+        double s = start;
+        fint[0] = s;
+        for (int i = 0; i < n - 1; ++i) {
+            s = s + W.pa[i];
+            fint[i + 1] = s;
+        }
+    }
+    wsync();
+}
+
+// ---- DERIV ----------------------------------------------------------------
+__device__ void deriv_wave(const double *x, const double *f, double *dfdx, int n)
+{
+#pragma clang fp contract(off)
+    const int ln = lane_id();
+    double out[2];
+    int k = 0;
+    const double s = fabs(x[1] - x[0]) / (x[1] - x[0]);
+    for (int j = ln; j < n; j += 64, ++k) {
+        if (j == 0) {
+            out[k] = (f[1] - f[0]) / (x[1] - x[0]);
+        } else if (j == n - 1) {
+            out[k] = (f[n - 1] - f[n - 2]) / (x[n - 1] - x[n - 2]);
+        } else {
+            double scale = fmax(fmax(fabs(f[j - 1]), fabs(f[j])), fabs(f[j + 1])) / fabs(x[j]);
+            if (scale == 0.0) scale = 1.0;
+            const double dr = (f[j + 1] - f[j]) / (x[j + 1] - x[j]) / scale;
+            const double dl = (f[j] - f[j - 1]) / (x[j] - x[j - 1]) / scale;
+            const double tr = dr / (s * sqrt(1.0 + dr * dr) + 1.0);
+            const double tl = dl / (s * sqrt(1.0 + dl * dl) + 1.0);
+            out[k] = (tr + tl) / (1.0 - tr * tl) * scale;
+        }
+    }
+    wsync();
+    k = 0;
+    for (int j = ln; j < n; j += 64, ++k) dfdx[j] = out[k];
+    wsync();
+}
+
+// ---- MAP1 -----------------------------------------------------------------
+// xold/fold (nold, ascending xold) -> fnew at xnew (nnew, ascending).
+// Returns the reference's MAP1 value (last interval index, 1-based).
+__device__ int map1_wave(const double *xold, const double *fold, int nold, const double *xnew,
+                         double *fnew, int nnew)
+{
+#pragma clang fp contract(off)
+    const int ln = lane_id();
+    const double *XO = xold - 1, *FO = fold - 1;  // 1-based views
+    int carryL = 0, carryPrev = 0;                // LL before the chunk, its prevDistinct
+    for (int base = 0; base < nnew; base += 64) {
+        const int K = base + ln;
+        const bool on = K < nnew;
+        const double x = on ? xnew[K] : 0.0;
+        int L = 2;
+        bool past = false;
+        if (on) {
+            // cnt = #{i in 1..nold : XO[i] <= x}
+            int lo = 0, hi = nold;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (XO[mid] <= x) lo = mid;
+                else hi = mid - 1;
+            }
+            L = lo + 1;
+            if (L < 2) L = 2;
+            if (L > nold) {
+                past = true;
+                L = nold;
+            }
+        }
+        // run starts and the previous run's interval for each lane
+        const int Lup = __shfl_up(L, 1);
+        const bool start = on && (ln == 0 ? (L != carryL) : (L != Lup));
+        const unsigned long long sm = __ballot(start);
+        const unsigned long long upto = sm & (ln == 63 ? ~0ull : ((2ull << ln) - 1ull));
+        const int s = upto ? 63 - (int)__clzll(upto) : -1;
+        const int Ls1 = __shfl(L, s > 0 ? s - 1 : 0);  // every lane takes part
+        const int prevD = s < 0 ? carryPrev : (s > 0 ? Ls1 : carryL);
+        double A, B, C;
+        if (past || L <= 3) {
+            // linear through L-1, L; a reused linear fit evaluates to the
+            // same value (its C is 0)
+            B = (FO[L] - FO[L - 1]) / (XO[L] - XO[L - 1]);
+            A = FO[L] - XO[L] * B;
+            if (on) fnew[K] = A + B * x;
+        } else {
+            const int L1 = L - 1;
+            double ABAC, BBAC, CBAC;
+            if (L == prevD + 1 && L != 3 && L != 4) {
+                // forward parabola of interval L-1, as the serial routine kept it
+                const int Lp = L - 1, Lp1 = Lp - 1;
+                const double D = (FO[Lp] - FO[Lp1]) / (XO[Lp] - XO[Lp1]);
+                CBAC = FO[Lp + 1] / ((XO[Lp + 1] - XO[Lp]) * (XO[Lp + 1] - XO[Lp1])) +
+                       (FO[Lp1] / (XO[Lp + 1] - XO[Lp1]) - FO[Lp] / (XO[Lp + 1] - XO[Lp])) /
+                           (XO[Lp] - XO[Lp1]);
+                BBAC = D - (XO[Lp] + XO[Lp1]) * CBAC;
+                ABAC = FO[Lp1] - XO[Lp1] * D + XO[Lp] * XO[Lp1] * CBAC;
+            } else {
+                const int L2 = L - 2;
+                const double D = (FO[L1] - FO[L2]) / (XO[L1] - XO[L2]);
+                CBAC = FO[L] / ((XO[L] - XO[L1]) * (XO[L] - XO[L2])) +
+                       (FO[L2] / (XO[L] - XO[L2]) - FO[L1] / (XO[L] - XO[L1])) /
+                           (XO[L1] - XO[L2]);
+                BBAC = D - (XO[L1] + XO[L2]) * CBAC;
+                ABAC = FO[L2] - XO[L2] * D + XO[L1] * XO[L2] * CBAC;
+            }
+            if (L >= nold) {
+                A = ABAC; B = BBAC; C = CBAC;
+            } else {
+                const double D = (FO[L] - FO[L1]) / (XO[L] - XO[L1]);
+                const double CFOR = FO[L + 1] / ((XO[L + 1] - XO[L]) * (XO[L + 1] - XO[L1])) +
+                                    (FO[L1] / (XO[L + 1] - XO[L1]) - FO[L] / (XO[L + 1] - XO[L])) /
+                                        (XO[L] - XO[L1]);
+                const double BFOR = D - (XO[L] + XO[L1]) * CFOR;
+                const double AFOR = FO[L1] - XO[L1] * D + XO[L] * XO[L1] * CFOR;
+                double WT = 0.0;
+                if (fabs(CFOR) != 0.0) WT = fabs(CFOR) / (fabs(CFOR) + fabs(CBAC));
+                A = AFOR + WT * (ABAC - AFOR);
+                B = BFOR + WT * (BBAC - BFOR);
+                C = CFOR + WT * (CBAC - CFOR);
+            }
+            if (on) fnew[K] = A + (B + C * x) * x;
+        }
+        // carry to the next chunk: LL after the last point and its prevDistinct
+        const int lastLane = min(63, nnew - 1 - base);
+        const int newCarryL = __shfl(L, lastLane);
+        const int newCarryPrev = __shfl(prevD, lastLane);
+        carryL = newCarryL;
+        carryPrev = newCarryPrev;
+    }
+    wsync();
+    return carryL - 1;
+}
+
+// ---- SOLVIT: LU with partial pivoting, lane = row ------------------------
+// amat column-major n x n in LDS, b (n) in LDS (solution on return).
+__device__ void solvit_wave(JoshLDS &W, int n)
+{
+#pragma clang fp contract(off)
+    const int ln = lane_id();
+    double *A = W.amat;
+    double *B = W.xs;
+    const double TOL = 1.0e-30;
+#define AM(i, j) A[((j) - 1) * n + ((i) - 1)]
+    for (int I = 1; I <= n - 1; ++I) {
+        // (a) pivot: first row K >= I with the largest |A(K, I)|
+        const int K = ln + 1;
+        double v = (K >= I && K <= n) ? fabs(AM(K, I)) : -1.0;
+        int idx = (K >= I && K <= n) ? K : 0x7fffffff;
+        for (int o = 32; o > 0; o >>= 1) {
+            const double ov = __shfl_xor(v, o);
+            const int oi = __shfl_xor(idx, o);
+            if (ov > v || (ov == v && oi < idx)) {
+                v = ov;
+                idx = oi;
+            }
+        }
+        const int M = idx;
+        // (b) swap rows I and M in columns I+1..n
+        if (M != I && K > I && K <= n) {
+            const double t = AM(I, K);
+            AM(I, K) = AM(M, K);
+            AM(M, K) = t;
+        }
+        wsync();
+        // (c) reciprocal pivot on the diagonal
+        if (ln == 0) {
+            double p = AM(M, I);
+            if (fabs(p) < TOL) p = copysign(TOL, p);
+            const double piv = 1.0 / p;
+            AM(M, I) = AM(I, I);
+            AM(I, I) = piv;
+            W.ipiv[I] = M;
+        }
+        wsync();
+        // (d) multipliers, (e) elimination; lane K > I owns row K
+        const double piv = AM(I, I);
+        double mk = 0.0;
+        if (K > I && K <= n) {
+            mk = AM(K, I) * piv;
+            AM(K, I) = mk;
+        }
+        for (int J = I + 1; J <= n; ++J) {
+            const double c = AM(I, J);
+            if (c == 0.0) continue;
+            if (K > I && K <= n) AM(K, J) = AM(K, J) - mk * c;
+        }
+        wsync();
+    }
+    if (ln == 0) {
+        double p = AM(n, n);
+        if (fabs(p) < TOL) p = copysign(TOL, p);
+        AM(n, n) = 1.0 / p;
+    }
+    wsync();
+    // forward substitution
+    const int K = ln + 1;
+    for (int I = 1; I <= n - 1; ++I) {
+        if (ln == 0) {
+            const int M = W.ipiv[I];
+            if (M != I) {
+                const double t = B[M - 1];
+                B[M - 1] = B[I - 1];
+                B[I - 1] = t;
+            }
+        }
+        wsync();
+        const double c = B[I - 1];
+        if (K > I && K <= n) B[K - 1] = B[K - 1] - AM(K, I) * c;
+        wsync();
+    }
+    // back substitution
+    for (int J = n; J >= 2; --J) {
+        if (ln == 0) B[J - 1] = B[J - 1] * AM(J, J);
+        wsync();
+        const double c = B[J - 1];
+        if (K < J) B[K - 1] = B[K - 1] - AM(K, J) * c;
+        wsync();
+    }
+    if (ln == 0) B[0] = B[0] * AM(1, 1);
+    wsync();
+#undef AM
+}
+
+struct JoshOut {
+    double hnu1, knu1;
+    int maxj;
+    int vef_sweeps, vef_unconverged;
+};
+
+// JOSH on one frequency.  Inputs already in LDS: rhox, bnu, and the
+// per-depth abtot/alpha/snubar computed by the caller from ACONT, ALINE,
+// SIGMAC, SIGMAL, SCONT, SLINE (:18475-18478).  coefj/coefh are global.
+// For ifsurf == 0 the depth outputs are left in W.taunu, snu, jnu, hnu, jmins.
+__device__ JoshOut josh_wave(JoshLDS &W, int n, int ifscat, int ifsurf,
+                             const double *__restrict__ coefj, const double *__restrict__ coefh)
+{
+#pragma clang fp contract(off)
+    const int ln = lane_id();
+    JoshOut out{0.0, 0.0, 0, 0, 0};
+    integ_wave(W, W.rhox, W.abtot, W.taunu, n, W.abtot[0] * W.rhox[0]);
+    int maxj = 0;
+    bool dispatch60 = false;
+    if (ifscat == 0) {
+        for (int j = ln; j < n; j += 64) W.snu[j] = W.snubar[j];
+        wsync();
+        maxj = map1_wave(W.taunu, W.snu, n, cXTAU8, W.xs, NX);
+        if (ifsurf == 1) dispatch60 = true;
+        else {
+            for (int j = ln; j < n; j += 64) W.alpha[j] = 0.0;
+            wsync();
+        }
+    }
+    if (!dispatch60) {
+        if (W.taunu[0] > cXTAU8[NX - 1]) maxj = 1;
+        if (maxj != 1) {
+            maxj = map1_wave(W.taunu, W.snubar, n, cXTAU8, W.xsbar, NX);
+            maxj = map1_wave(W.taunu, W.alpha, n, cXTAU8, W.xalpha, NX);
+            double xa = -1.0;
+            if (ln < NX) {
+                double a8 = fmax(W.xalpha[ln], 0.0);
+                double s8 = fmax(W.xsbar[ln], 1.0e-38);
+                if (cXTAU8[ln] < W.taunu[0]) {
+                    s8 = fmax(W.snubar[0], 1.0e-38);
+                    a8 = fmax(W.alpha[0], 0.0);
+                }
+                W.xalpha[ln] = a8;
+                W.xsbar[ln] = s8;
+                xa = a8;
+            }
+            const double amax = wave_max(xa);
+            wsync();
+            if (amax <= 0.0) {
+                if (ln < NX) W.xs[ln] = fmax(W.xsbar[ln], 1.0e-37);
+                wsync();
+            } else {
+                // AMAT(L, M) = -XALPHA(L) COEFJ(L, M) + delta; lane = row L
+                if (ln < NX) {
+                    const double al = W.xalpha[ln];
+                    for (int m = 0; m < NX; ++m) {
+                        double v = -al * coefj[m * NX + ln];
+                        if (m == ln) v = v + 1.0;
+                        W.amat[m * NX + ln] = v;
+                    }
+                    W.xs[ln] = (1.0 - al) * W.xsbar[ln];
+                }
+                wsync();
+                solvit_wave(W, NX);
+                if (ln < NX) W.xs[ln] = fmax(W.xs[ln], 1.0e-37);
+                wsync();
+            }
+            if (ifsurf == 1) dispatch60 = true;
+            else (void)map1_wave(cXTAU8, W.xs, NX, W.taunu, W.snu, maxj);
+        }
+    }
+    if (dispatch60) {
+        if (ln == 0) {
+            double h = 0.0;
+            for (int m = 0; m < NX; ++m) h = h + cCHJ[m] * W.xs[m];
+            W.red[0] = h;
+        }
+        wsync();
+        out.hnu1 = W.red[0];
+        out.maxj = maxj;
+        return out;
+    }
+    // deep branch: variable Eddington factor (:18581-18639)
+    if (maxj != n) {
+        const int maxj1 = maxj == 1 ? 1 : maxj + 1;
+        for (int j = ln + 1; j <= n; j += 64)
+            if (j >= maxj1) W.snu[j - 1] = W.snubar[j - 1];
+        wsync();
+        const int m = maxj - 1 > 1 ? maxj - 1 : 1;
+        const int nm1 = n - m + 1, nmj = n - maxj + 1;
+        int sweep;
+        double error = 0.0;
+        for (sweep = 1; sweep <= 50; ++sweep) {
+            int bad = 0;
+            for (int j = ln + 1; j <= n; j += 64)
+                if (j >= m && W.snu[j - 1] <= 0.0) bad = 1;
+            bool ifneg = __any(bad);
+            if (ifneg) {
+                for (int j = ln + 1; j <= n; j += 64)
+                    if (j >= m) {
+                        W.snubar[j - 1] = W.bnu[j - 1];
+                        W.snu[j - 1] = W.bnu[j - 1];
+                    }
+                wsync();
+            }
+            deriv_wave(W.taunu + m - 1, W.snu + m - 1, W.hnu + m - 1, nm1);
+            bad = 0;
+            for (int j = ln + 1; j <= n; j += 64)
+                if (j >= m && W.hnu[j - 1] <= 0.0) bad = 1;
+            if (__any(bad)) {
+                ifneg = true;
+                for (int j = ln + 1; j <= n; j += 64)
+                    if (j >= m) {
+                        W.snubar[j - 1] = W.bnu[j - 1];
+                        W.snu[j - 1] = W.bnu[j - 1];
+                    }
+                wsync();
+                deriv_wave(W.taunu + m - 1, W.snu + m - 1, W.hnu + m - 1, nm1);
+            }
+            for (int j = ln + 1; j <= n; j += 64)
+                if (j >= m) W.hnu[j - 1] = W.hnu[j - 1] / 3.0;
+            wsync();
+            deriv_wave(W.taunu + maxj - 1, W.hnu + maxj - 1, W.jmins + maxj - 1, nmj);
+            for (int j = ln + 1; j <= n; j += 64)
+                if (j >= maxj1) {
+                    if (ifneg) W.jmins[j - 1] = 0.0;
+                    W.jnu[j - 1] = W.jmins[j - 1] + W.snu[j - 1];
+                    const double snew = (1.0 - W.alpha[j - 1]) * W.snubar[j - 1] +
+                                        W.alpha[j - 1] * W.jnu[j - 1];
+                    W.pa[j - 1] = snew;  // snew, the error is summed in order below
+                }
+            wsync();
+            if (ln == 0) {
+                double e = 0.0;
+                for (int j = maxj1; j <= n; ++j) e = fabs(W.pa[j - 1] - W.snu[j - 1]) / W.pa[j - 1] + e;
+                W.red[0] = e;
+            }
+            wsync();
+            error = W.red[0];
+            for (int j = ln + 1; j <= n; j += 64)
+                if (j >= maxj1) W.snu[j - 1] = W.pa[j - 1];
+            wsync();
+            if (error < 0.00001) break;
+        }
+        out.vef_sweeps = sweep > 50 ? 50 : sweep;
+        out.vef_unconverged = error >= 0.00001;
+    }
+    if (maxj == 1) {
+        out.knu1 = W.jnu[0] / 3.0;
+        out.hnu1 = W.hnu[0];
+        out.maxj = maxj;
+        return out;
+    }
+    // J, H, K from the 51-point operators (:18654-18676); lane = row L
+    if (ln < NX) {
+        double xjs = -W.xs[ln], xh = 0.0;
+        for (int m = 0; m < NX; ++m) xjs = xjs + coefj[m * NX + ln] * W.xs[m];
+        for (int m = 0; m < NX; ++m) xh = xh + coefh[m * NX + ln] * W.xs[m];
+        W.xsbar[ln] = xjs;  // reuse as XJS8
+        W.xtmp[ln] = xh;    // XH8
+    }
+    if (ln == 0) {
+        double xk = 0.0;
+        for (int m = 0; m < NX; ++m) xk = xk + cCKJ[m] * W.xs[m];
+        W.red[0] = xk;
+    }
+    wsync();
+    out.knu1 = W.red[0];
+    (void)map1_wave(cXTAU8, W.xsbar, NX, W.taunu, W.jmins, maxj);
+    (void)map1_wave(cXTAU8, W.xtmp, NX, W.taunu, W.hnu, maxj);
+    for (int j = ln; j < maxj; j += 64) {
+        W.snu[j] = fmax(W.snu[j], 1.0e-38);
+        W.jnu[j] = fmax(W.jmins[j] + W.snu[j], 1.0e-38);
+    }
+    wsync();
+    out.hnu1 = W.hnu[0];
+    out.maxj = maxj;
+    return out;
+}
+
+// Fill abtot, alpha, snubar (:18475-18478) for lane-held inputs.
+__device__ __forceinline__ void josh_prepare(JoshLDS &W, int j, double acont, double aline,
+                                             double sigmac, double sigmal, double scont,
+                                             double sline)
+{
+#pragma clang fp contract(off)
+    const double ab = acont + aline + sigmac + sigmal;
+    W.abtot[j] = ab;
+    W.alpha[j] = (sigmac + sigmal) / ab;
+    W.snubar[j] = (acont * scont + aline * sline) / (acont + aline);
+}
+
+// --------------------------------------------------------------------------
+// SYNTHE wavelength loop: per point two JOSH(1, 1) calls (continuum, then
+// continuum + line).  Inputs: opacity (nrhox x L), float or double.
+// --------------------------------------------------------------------------
+template <bool F64>
+__global__ __launch_bounds__(64) void k_synthe_transfer(
+    int L, int n, double wbegin, double ratio, int nedge, const double *__restrict__ wledge,
+    int nf, const double *__restrict__ contabs, const double *__restrict__ contscat,
+    const double *__restrict__ depth, const void *__restrict__ opacity,
+    const double *__restrict__ coefj, const double *__restrict__ coefh,
+    double *__restrict__ h_line, double *__restrict__ h_cont)
+{
+#pragma clang fp contract(off)
+    __shared__ JoshLDS W;
+    const int ln = lane_id();
+    const double *hkt_syn = depth + 7 * n, *hkt_atl = depth + 8 * n, *rhox = depth + 9 * n;
+    for (int j = ln; j < n; j += 64) W.rhox[j] = rhox[j];
+    wsync();
+    const double bnu_prefac = 2.0 * A12_HPLANCK / (A12_CLIGHT * A12_CLIGHT) * 1.0e45;
+    for (int iw = blockIdx.x; iw < L; iw += gridDim.x) {
+        const double wave = wbegin * powi_f64(ratio, iw);
+        const double freq8 = A12_CLIGHT_NMS / wave;
+        // edge interval: first ie with wave < wledge(ie+1), ascending edges
+        int lo = 1, hi = nedge - 1;
+        while (lo <= hi) {
+            const int mid = (lo + hi) >> 1;
+            if (wledge[mid] <= wave) lo = mid + 1;
+            else hi = mid - 1;
+        }
+        int ie = lo - 1;
+        if (ie > nedge - 2) ie = nedge - 2;
+        const double w0 = wledge[ie], w1 = wledge[ie + 1];
+        const double half = (w0 + w1) * 0.5;
+        const double del = (w1 - w0) * (w1 - w0) * 0.5;
+        const double c1 = (wave - half) * (wave - w1) / del;
+        const double c2 = (w0 - wave) * (wave - w1) * 2.0 / del;
+        const double c3 = (wave - w0) * (wave - half) / del;
+        const double freq = A12_CLIGHT_NMS / wave;
+        const double f15 = freq / 1.0e15;
+        double acont[2], sigmac[2], asyn[2], ehv[2], stim[2], bnu[2];
+        int k = 0;
+        for (int j = ln; j < n; j += 64, ++k) {
+            const double op = F64 ? static_cast<const double *>(opacity)[(size_t)j * L + iw]
+                                  : (double)static_cast<const float *>(opacity)[(size_t)j * L + iw];
+            const double a = op * (1.0 - exp(-freq8 * hkt_syn[j]));
+            asyn[k] = F64 ? a : (double)(float)a;
+            const double *ca = contabs + (size_t)j * nf + 3 * ie;
+            const double *cs = contscat + (size_t)j * nf + 3 * ie;
+            acont[k] = pow(10.0, c1 * ca[0] + c2 * ca[1] + c3 * ca[2]);
+            sigmac[k] = pow(10.0, c1 * cs[0] + c2 * cs[1] + c3 * cs[2]);
+            ehv[k] = exp(-freq * hkt_atl[j]);
+            stim[k] = 1.0 - ehv[k];
+            bnu[k] = bnu_prefac * (f15 * f15 * f15) * ehv[k] / stim[k];
+            W.bnu[j] = bnu[k];
+            josh_prepare(W, j, acont[k], 0.0, sigmac[k], 0.0, bnu[k], bnu[k]);
+        }
+        wsync();
+        const JoshOut c = josh_wave(W, n, 1, 1, coefj, coefh);
+        k = 0;
+        for (int j = ln; j < n; j += 64, ++k) {
+            const double sline = bnu[k] * stim[k] / (1.0 - ehv[k]);
+            W.bnu[j] = bnu[k];
+            josh_prepare(W, j, acont[k], asyn[k] * (1.0 - 0.0), sigmac[k], asyn[k] * 0.0,
+                         bnu[k], sline);
+        }
+        wsync();
+        const JoshOut l = josh_wave(W, n, 1, 1, coefj, coefh);
+        if (ln == 0) {
+            const double resid = l.hnu1 / c.hnu1;
+            h_line[iw] = resid * c.hnu1;
+            h_cont[iw] = c.hnu1;
+        }
+        wsync();
+    }
+}
+
+// Batched JOSH on caller-provided per-depth inputs, (n, nbatch) layout.
+__global__ __launch_bounds__(64) void k_josh_batch(
+    int n, int nbatch, int ifscat, int ifsurf, const double *__restrict__ rhox,
+    const double *__restrict__ in, const double *__restrict__ coefj,
+    const double *__restrict__ coefh, double *__restrict__ hk, double *__restrict__ outd)
+{
+    __shared__ JoshLDS W;
+    const int ln = lane_id();
+    const size_t plane = (size_t)n * nbatch;
+    for (int j = ln; j < n; j += 64) W.rhox[j] = rhox[j];
+    wsync();
+    for (int b = blockIdx.x; b < nbatch; b += gridDim.x) {
+        for (int j = ln; j < n; j += 64) {
+            const size_t o = (size_t)b * n + j;
+            const double ac = in[o], al = in[plane + o], sc = in[2 * plane + o],
+                         sl = in[3 * plane + o], so = in[4 * plane + o],
+                         sli = in[5 * plane + o], bn = in[6 * plane + o];
+            W.bnu[j] = bn;
+            josh_prepare(W, j, ac, al, sc, sl, so, sli);
+            W.snu[j] = 0.0;
+            W.jnu[j] = 0.0;
+            W.hnu[j] = 0.0;
+            W.jmins[j] = 0.0;
+        }
+        wsync();
+        const JoshOut r = josh_wave(W, n, ifscat, ifsurf, coefj, coefh);
+        if (ln == 0) {
+            hk[2 * b] = r.hnu1;
+            hk[2 * b + 1] = r.knu1;
+        }
+        if (outd && ifsurf == 0) {
+            for (int j = ln; j < n; j += 64) {
+                const size_t o = (size_t)b * n + j;
+                outd[o] = W.taunu[j];
+                outd[plane + o] = W.snu[j];
+                outd[2 * plane + o] = W.jnu[j];
+                outd[3 * plane + o] = W.hnu[j];
+                outd[4 * plane + o] = W.jmins[j];
+            }
+        }
+        wsync();
+    }
+}
+
+}  // namespace
+
+int a12i_synthe_transfer(a12_ctx *ctx)
+{
+    SyntheDev &S = ctx->syn;
+    if (S.opac_prec < 0) {
+        ctx->err = "a12_synthe_transfer_run: no opacity on the device (run the opacity first)";
+        return A12_ERR_STATE;
+    }
+    if (!ctx->have_ops) {
+        ctx->err = "a12_synthe_transfer_run: call a12_set_operators first";
+        return A12_ERR_STATE;
+    }
+    if (!S.contabs.p || !S.contscat.p) {
+        ctx->err = "a12_synthe_transfer_run: atmosphere was loaded without contabs/contscat";
+        return A12_ERR_ARG;
+    }
+    A12_HIP(S.hline.reserve(sizeof(double) * S.L));
+    A12_HIP(S.hcont.reserve(sizeof(double) * S.L));
+    const int grid = std::min(S.L, 4096);
+    auto k = S.opac_prec == A12_PREC_FP64 ? k_synthe_transfer<true> : k_synthe_transfer<false>;
+    if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
+    hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, ctx->stream, S.L, S.nrhox, S.wbegin, S.ratio,
+                       S.nedge, S.wledge.as<double>(), S.nf, S.contabs.as<double>(),
+                       S.contscat.as<double>(), S.depth.as<double>(), S.opacity.p,
+                       ctx->coefj.as<double>(), ctx->coefh.as<double>(), S.hline.as<double>(),
+                       S.hcont.as<double>());
+    A12_CHECK_LAUNCH();
+    if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[6], ctx->stream));
+    S.transferred = true;
+    return A12_OK;
+}
+
+int a12i_josh_batch(a12_ctx *ctx, int ifscat, int ifsurf, int nrhox, int nbatch,
+                    const double *rhox, const double *const in[7], double *hnu1, double *knu1,
+                    double *const out[5])
+{
+    if (!ctx->have_ops) {
+        ctx->err = "a12_josh_batch: call a12_set_operators first";
+        return A12_ERR_STATE;
+    }
+    const size_t plane = (size_t)nrhox * nbatch;
+    A12_HIP(ctx->jb_in.reserve(sizeof(double) * (7 * plane + nrhox)));
+    A12_HIP(ctx->jb_out.reserve(sizeof(double) * (5 * plane + 2 * (size_t)nbatch)));
+    double *din = ctx->jb_in.as<double>();
+    for (int k = 0; k < 7; ++k)
+        A12_HIP(hipMemcpyAsync(din + k * plane, in[k], sizeof(double) * plane,
+                               hipMemcpyHostToDevice, ctx->stream));
+    A12_HIP(hipMemcpyAsync(din + 7 * plane, rhox, sizeof(double) * nrhox, hipMemcpyHostToDevice,
+                           ctx->stream));
+    double *dout = ctx->jb_out.as<double>();
+    double *dhk = dout + 5 * plane;
+    const int grid = std::min(nbatch, 4096);
+    hipLaunchKernelGGL(k_josh_batch, dim3(grid), dim3(64), 0, ctx->stream, nrhox, nbatch, ifscat,
+                       ifsurf, din + 7 * plane, din, ctx->coefj.as<double>(),
+                       ctx->coefh.as<double>(), dhk, ifsurf == 0 ? dout : nullptr);
+    A12_CHECK_LAUNCH();
+    std::vector<double> hk(2 * (size_t)nbatch);
+    A12_HIP(hipMemcpyAsync(hk.data(), dhk, sizeof(double) * 2 * nbatch, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    for (int k = 0; k < 5; ++k)
+        if (out[k] && ifsurf == 0)
+            A12_HIP(hipMemcpyAsync(out[k], dout + k * plane, sizeof(double) * plane,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+    A12_HIP(hipStreamSynchronize(ctx->stream));
+    for (int b = 0; b < nbatch; ++b) {
+        if (hnu1) hnu1[b] = hk[2 * b];
+        if (knu1) knu1[b] = hk[2 * b + 1];
+    }
+    return A12_OK;
+}

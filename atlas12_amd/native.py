@@ -1,0 +1,250 @@
+"""ctypes binding of liba12gpu.so (include/a12gpu.h).
+
+This is the host-side mirror of the reference's hot-path subroutines:
+:func:`synthe_line_opacity` stands for SYNTHE's depth loop
+(src/synthe.f90:476-689), :func:`synthe_transfer` for its wavelength loop
+(:698-718), :func:`josh` for JOSH (src/atlas12_modules.f90:18398-18776).
+There is no CPU fallback: without the built library every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .state import SyntheState
+from .workloads import LTE_DTYPE
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(PKG, "liba12gpu.so")
+
+PREC_MIXED = 0
+PREC_FP64 = 1
+
+P_D = C.POINTER(C.c_double)
+
+EXPORTS = ["a12_ctx_create", "a12_ctx_destroy", "a12_last_error", "a12_stream",
+           "a12_set_operators", "a12_synthe_line_opacity", "a12_synthe_load",
+           "a12_synthe_opacity_run", "a12_synthe_opacity_fetch", "a12_synthe_opacity_device",
+           "a12_synthe_evals", "a12_synthe_transfer_run", "a12_synthe_transfer_fetch",
+           "a12_synthe_transfer", "a12_josh_batch", "a12_voigt_profile",
+           "a12_set_profiling", "a12_stage_ms", "a12_synthe_eval_split"]
+
+
+class A12Error(RuntimeError):
+    pass
+
+
+class LteLine(C.Structure):
+    _fields_ = [("nbuff", C.c_int32), ("cgf", C.c_float), ("nelion", C.c_int32),
+                ("elo", C.c_float), ("gamrf", C.c_float), ("gamsf", C.c_float),
+                ("gamwf", C.c_float), ("code", C.c_float)]
+
+
+class SyntheAtm(C.Structure):
+    _fields_ = [("nrhox", C.c_int32), ("length", C.c_int32), ("wbegin", C.c_double),
+                ("ratio", C.c_double), ("resolu", C.c_double), ("nedge", C.c_int32),
+                ("wledge", P_D), ("nf", C.c_int32), ("continall", P_D), ("contabs", P_D),
+                ("contscat", P_D), ("mw6", C.c_int32), ("xnfpel", P_D), ("dopple", P_D),
+                ("t", P_D), ("xne", P_D), ("rho", P_D), ("xnfh", P_D), ("xnfhe1", P_D),
+                ("xnfh2", P_D), ("hckt", P_D), ("hkt", P_D), ("hkt_atlas", P_D), ("rhox", P_D)]
+
+
+_lib = None
+
+
+def load_library(path: str = LIBPATH):
+    """Load liba12gpu.so; raises A12Error if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise A12Error(f"{path} is missing: build it with `python -m atlas12_amd.build`")
+    L = C.CDLL(path)
+    for name in EXPORTS:
+        if not hasattr(L, name):
+            raise A12Error(f"{path} lacks symbol {name}")
+    L.a12_last_error.restype = C.c_char_p
+    L.a12_stream.restype = C.c_void_p
+    L.a12_synthe_opacity_device.restype = C.c_void_p
+    L.a12_synthe_evals.restype = C.c_int64
+    L.a12_ctx_destroy.argtypes = [C.c_void_p]
+    _lib = L
+    return L
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(P_D)
+
+
+class Context:
+    """One device context (a12_ctx): its stream and device buffers."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        self.h = C.c_void_p()
+        rc = self.lib.a12_ctx_create(C.c_int32(device), C.byref(self.h))
+        if rc != 0:
+            raise A12Error(f"a12_ctx_create({device}) failed with status {rc}")
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            self.lib.a12_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            msg = self.lib.a12_last_error(self.h)
+            raise A12Error(f"{what}: status {rc}: {msg.decode() if msg else ''}")
+
+    @property
+    def stream(self) -> int:
+        return int(self.lib.a12_stream(self.h) or 0)
+
+    # ---------------------------------------------------------------- JOSH
+    def set_operators(self, coefj, coefh):
+        cj, ch = _f64(coefj), _f64(coefh)
+        if cj.size != 2601 or ch.size != 2601:
+            raise A12Error("COEFJ/COEFH must be 51 x 51")
+        self._check(self.lib.a12_set_operators(self.h, _ptr(cj), _ptr(ch)), "a12_set_operators")
+
+    def josh_batch(self, ifscat, ifsurf, rhox, acont, aline, sigmac, sigmal, scont, sline, bnu):
+        """JOSH for a batch of frequencies; inputs (nbatch, nrhox)."""
+        ins = [_f64(np.atleast_2d(a)) for a in (acont, aline, sigmac, sigmal, scont, sline, bnu)]
+        nb, n = ins[0].shape
+        rx = _f64(rhox)
+        hnu1, knu1 = np.zeros(nb), np.zeros(nb)
+        outs = [np.zeros((nb, n)) for _ in range(5)]
+        self._check(self.lib.a12_josh_batch(
+            self.h, C.c_int32(ifscat), C.c_int32(ifsurf), C.c_int32(n), C.c_int32(nb), _ptr(rx),
+            *[_ptr(a) for a in ins], _ptr(hnu1), _ptr(knu1), *[_ptr(o) for o in outs]),
+            "a12_josh_batch")
+        return dict(hnu1=hnu1, knu1=knu1, taunu=outs[0], snu=outs[1], jnu=outs[2],
+                    hnu=outs[3], jmins=outs[4])
+
+    def voigt(self, v, a, prec: int = 0):
+        """H(v, a) on the GPU: prec 0 mixed FP32 path, 1 FP64, 2 the
+        reference's FP32-rounded value."""
+        v = np.ascontiguousarray(v, np.float32)
+        a = np.ascontiguousarray(a, np.float32)
+        h = np.zeros(len(v))
+        self._check(self.lib.a12_voigt_profile(self.h, C.c_int32(prec), C.c_int32(len(v)),
+                                               v.ctypes.data_as(C.c_void_p),
+                                               a.ctypes.data_as(C.c_void_p), _ptr(h)),
+                    "a12_voigt_profile")
+        return h
+
+    # -------------------------------------------------------------- SYNTHE
+    def _atm(self, st: SyntheState) -> SyntheAtm:
+        keep = {k: _f64(getattr(st, k)) for k in (
+            "wledge", "continall", "contabs", "contscat", "xnfpel", "dopple", "t", "xne", "rho",
+            "xnfh", "xnfhe1", "xnfh2", "hckt", "hkt_syn", "hkt_atl", "rhox")}
+        a = SyntheAtm()
+        a.nrhox, a.length = st.nrhox, st.length
+        a.wbegin, a.ratio, a.resolu = st.wbegin, st.ratio, st.resolu
+        a.nedge = len(st.wledge)
+        a.nf = st.continall.shape[1]
+        a.mw6 = st.xnfpel.shape[1]
+        for k in ("wledge", "continall", "contabs", "contscat", "xnfpel", "dopple", "t", "xne",
+                  "rho", "xnfh", "xnfhe1", "xnfh2", "hckt"):
+            setattr(a, k, _ptr(keep[k]))
+        a.hkt = _ptr(keep["hkt_syn"])
+        a.hkt_atlas = _ptr(keep["hkt_atl"])
+        a.rhox = _ptr(keep["rhox"])
+        self._keep = [keep]
+        return a
+
+    def synthe_load(self, st: SyntheState, lines: np.ndarray):
+        lines = np.ascontiguousarray(lines, dtype=LTE_DTYPE)
+        a = self._atm(st)
+        self._check(self.lib.a12_synthe_load(self.h, C.byref(a), lines.ctypes.data_as(C.c_void_p),
+                                             C.c_int32(len(lines))), "a12_synthe_load")
+        self._loaded = (st.nrhox, st.length)
+
+    def synthe_opacity_run(self, prec: int = PREC_MIXED):
+        self._check(self.lib.a12_synthe_opacity_run(self.h, C.c_int32(prec)),
+                    "a12_synthe_opacity_run")
+        self._prec = prec
+
+    def synthe_opacity_fetch(self) -> np.ndarray:
+        n, L = self._loaded
+        out = np.zeros((n, L), np.float64 if self._prec == PREC_FP64 else np.float32)
+        self._check(self.lib.a12_synthe_opacity_fetch(self.h, out.ctypes.data_as(C.c_void_p)),
+                    "a12_synthe_opacity_fetch")
+        return out
+
+    def synthe_evals(self) -> int:
+        return int(self.lib.a12_synthe_evals(self.h))
+
+    def eval_split(self):
+        """(all contributions, Voigt-core contributions) of the last run."""
+        t, c = C.c_int64(0), C.c_int64(0)
+        self._check(self.lib.a12_synthe_eval_split(self.h, C.byref(t), C.byref(c)),
+                    "a12_synthe_eval_split")
+        return t.value, c.value
+
+    def set_profiling(self, on: bool = True):
+        self._check(self.lib.a12_set_profiling(self.h, C.c_int32(1 if on else 0)),
+                    "a12_set_profiling")
+
+    def stage_ms(self):
+        """[prep+continuum, line set-up, partition, tile gather, transfer] in
+        ms from HIP events on the context stream (-1 = not measured)."""
+        ms = (C.c_float * 5)()
+        self._check(self.lib.a12_stage_ms(self.h, ms), "a12_stage_ms")
+        return list(ms)
+
+    def synthe_transfer_run(self):
+        self._check(self.lib.a12_synthe_transfer_run(self.h), "a12_synthe_transfer_run")
+
+    def synthe_transfer_fetch(self):
+        L = self._loaded[1]
+        hl, hc = np.zeros(L), np.zeros(L)
+        self._check(self.lib.a12_synthe_transfer_fetch(self.h, _ptr(hl), _ptr(hc)),
+                    "a12_synthe_transfer_fetch")
+        return hl, hc
+
+    def synchronize(self):
+        import ctypes.util  # noqa: F401
+        # a fetch of the eval count synchronises the context stream
+        self.lib.a12_synthe_evals(self.h)
+
+
+def synthe_line_opacity(ctx: Context, st: SyntheState, lines: np.ndarray, prec: int = PREC_MIXED):
+    """SYNTHE's LTE line-opacity build, src/synthe.f90:476-689.  Returns the
+    opacity matrix (nrhox, length) and the number of accumulated
+    contributions."""
+    lines = np.ascontiguousarray(lines, dtype=LTE_DTYPE)
+    a = ctx._atm(st)
+    out = np.zeros((st.nrhox, st.length), np.float64 if prec == PREC_FP64 else np.float32)
+    ev = C.c_int64(0)
+    ctx._check(ctx.lib.a12_synthe_line_opacity(
+        ctx.h, C.byref(a), lines.ctypes.data_as(C.c_void_p), C.c_int32(len(lines)),
+        C.c_int32(prec), out.ctypes.data_as(C.c_void_p), C.byref(ev)), "a12_synthe_line_opacity")
+    ctx._loaded = (st.nrhox, st.length)
+    ctx._prec = prec
+    return out, ev.value
+
+
+def synthe_transfer(ctx: Context, st: SyntheState, opacity: np.ndarray):
+    """SYNTHE's wavelength loop, src/synthe.f90:698-718: (H_line, H_cont)."""
+    prec = PREC_FP64 if opacity.dtype == np.float64 else PREC_MIXED
+    op = np.ascontiguousarray(opacity)
+    a = ctx._atm(st)
+    hl, hc = np.zeros(st.length), np.zeros(st.length)
+    ctx._check(ctx.lib.a12_synthe_transfer(ctx.h, C.byref(a), op.ctypes.data_as(C.c_void_p),
+                                           C.c_int32(prec), _ptr(hl), _ptr(hc)),
+               "a12_synthe_transfer")
+    return hl, hc

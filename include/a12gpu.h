@@ -1,0 +1,163 @@
+/*
+ * a12gpu.h -- C-ABI of the MI355X (gfx950) implementation of the
+ * ATLAS12/SYNTHE data-parallel hot path.
+ *
+ * The reference's hot routines are Fortran subroutines that communicate
+ * through module arrays (no argument lists).  Each entry point below takes
+ * exactly those arrays, in their Fortran (column-major) layout, so a
+ * Fortran driver reaches it through iso_c_binding without changing its
+ * data (see atlas12_amd/fortran/a12gpu_iface.f90).  REAL(4) <-> float,
+ * REAL(8) <-> double, default INTEGER <-> int32_t.
+ *
+ * All entry points return A12_OK (0) or a negative a12_status; the message
+ * is available from a12_last_error().  A context is not thread-safe: use
+ * one per host thread.  Results are deterministic (no atomics; fixed
+ * reduction order).  The library never frees caller memory.
+ */
+#ifndef A12GPU_H
+#define A12GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    A12_OK = 0,
+    A12_ERR_ARG = -1,        /* bad dimension, null pointer, bad enum  */
+    A12_ERR_HIP = -2,        /* HIP runtime error                       */
+    A12_ERR_OOM = -3,        /* device allocation failed                */
+    A12_ERR_STATE = -4,      /* call out of order (e.g. run before load)*/
+    A12_ERR_NONFINITE = -5   /* non-finite input detected               */
+} a12_status;
+
+/* Arithmetic of the line-opacity build. */
+typedef enum {
+    A12_PREC_MIXED = 0,  /* FP32 Voigt and Neumaier-compensated FP32 sums;
+                            FP64 Boltzmann factor and far-wing anchor   */
+    A12_PREC_FP64 = 1    /* every quantity in FP64 (the FP64 GPU path) */
+} a12_prec;
+
+typedef struct a12_ctx a12_ctx;
+
+/* lte_line_t, src/mod_mklinelist.f90:30-41 (32 bytes, BIND(C) mirror). */
+typedef struct {
+    int32_t nbuff;   /* grid index of the line centre (1-based)       */
+    float   cgf;     /* line-strength factor                          */
+    int32_t nelion;  /* species slot (nelem-1)*6 + ion stage          */
+    float   elo;     /* lower-level energy [cm^-1]                    */
+    float   gamrf, gamsf, gamwf; /* damping / (4 pi nu)               */
+    float   code;    /* Kurucz species code                           */
+} a12_lte_line;
+
+/* State read by SYNTHE's two hot loops (src/synthe.f90:476-718), as the
+ * driver holds it after run_xnfpelsyn (src/synthe_module.f90:250-382) and
+ * READIN (src/atlas12_modules.f90:171, :1719).  Arrays are Fortran layout. */
+typedef struct {
+    int32_t nrhox;          /* layers (<= 128)                                */
+    int32_t length;         /* wavelength points                              */
+    double  wbegin;         /* first grid wavelength [nm]                     */
+    double  ratio;          /* 1 + 1/resolu                                   */
+    double  resolu;         /* resolving power                                */
+    int32_t nedge;          /* continuum edges                                */
+    const double *wledge;   /* (nedge) |wledge_m|                              */
+    int32_t nf;             /* leading dimension of the continuum tables       */
+    const double *continall;/* (nf, nrhox) log10(kappa_abs + kappa_scat)       */
+    const double *contabs;  /* (nf, nrhox) log10 kappa_abs                     */
+    const double *contscat; /* (nf, nrhox) log10 kappa_scat                    */
+    int32_t mw6;            /* 6*mw                                           */
+    const double *xnfpel;   /* (mw6, nrhox) xnfpel_m                           */
+    const double *dopple;   /* (mw6, nrhox) dopple_m                           */
+    const double *t, *xne, *rho, *xnfh, *xnfhe1, *xnfh2, *hckt, *hkt;
+                            /* xf_t ... xf_hckt, xf_hkt (nrhox)                */
+    const double *hkt_atlas;/* mod_atlas_data HKT (nrhox)                       */
+    const double *rhox;     /* mod_atlas_data RHOX (nrhox)                      */
+} a12_synthe_atm;
+
+/* ---- context ------------------------------------------------------------ */
+int a12_ctx_create(int32_t device, a12_ctx **out);
+int a12_ctx_destroy(a12_ctx *ctx);
+const char *a12_last_error(const a12_ctx *ctx);
+/* hipStream_t the context launches on (as void*). */
+void *a12_stream(a12_ctx *ctx);
+
+/* JOSH operator matrices COEFJ, COEFH (51,51), as BLOCKJ/BLOCKH load them
+ * (src/atlas12_modules.f90:18792-18860).  Required before any transfer. */
+int a12_set_operators(a12_ctx *ctx, const double *coefj, const double *coefh);
+
+/* ---- SYNTHE line opacity ------------------------------------------------ */
+/* Replaces the LTE part of SYNTHE's depth loop, src/synthe.f90:476-689
+ * (continuum :481-516, per-depth prep :518-551, line loop :557-684).
+ * opacity_out: (length, nrhox), float for A12_PREC_MIXED, double for
+ * A12_PREC_FP64.  evals_out (optional): accumulated profile contributions.
+ * Host pointers in, host pointer out (synchronous). */
+int a12_synthe_line_opacity(a12_ctx *ctx, const a12_synthe_atm *atm,
+                            const a12_lte_line *lines, int32_t nlines,
+                            int32_t prec, void *opacity_out, int64_t *evals_out);
+
+/* Staged form of the same operation, for callers that keep inputs
+ * resident: load copies atmosphere and lines to the device and sorts the
+ * lines by centre; run executes the device pipeline into the context's
+ * device opacity buffer (no host traffic); fetch copies it back. */
+int a12_synthe_load(a12_ctx *ctx, const a12_synthe_atm *atm,
+                    const a12_lte_line *lines, int32_t nlines);
+int a12_synthe_opacity_run(a12_ctx *ctx, int32_t prec);
+int a12_synthe_opacity_fetch(a12_ctx *ctx, void *opacity_out);
+/* Device pointer of the opacity buffer of the last run and its precision. */
+const void *a12_synthe_opacity_device(a12_ctx *ctx, int32_t *prec_out);
+/* Profile contributions accumulated by the last run (exact count). */
+int64_t a12_synthe_evals(a12_ctx *ctx);
+
+/* ---- SYNTHE transfer ---------------------------------------------------- */
+/* Replaces the wavelength loop, src/synthe.f90:698-718 with
+ * setup_opacity_sv :797-833 and process_wavelength_point :848-902 (two
+ * JOSH(1,1) calls per point).  Uses the opacity of the last
+ * a12_synthe_opacity_run on this context (already on the device).
+ * Writes H_line and H_cont per point as the .spec file records them. */
+int a12_synthe_transfer_run(a12_ctx *ctx);
+int a12_synthe_transfer_fetch(a12_ctx *ctx, double *hnu_line, double *hnu_cont);
+/* Host-array form: opacity (length, nrhox) float or double per prec. */
+int a12_synthe_transfer(a12_ctx *ctx, const a12_synthe_atm *atm,
+                        const void *opacity, int32_t prec,
+                        double *hnu_line, double *hnu_cont);
+
+/* ---- JOSH, batched ------------------------------------------------------ */
+/* JOSH(IFSCAT, IFSURF) for nbatch independent frequencies,
+ * src/atlas12_modules.f90:18398-18776.  Inputs are (nrhox, nbatch) arrays
+ * (depth fastest): acont, aline, sigmac, sigmal, scont, sline, bnu; rhox is
+ * shared (nrhox).  Outputs (nrhox, nbatch): taunu, snu, jnu, hnu, jmins
+ * (any may be NULL for ifsurf = 1), knu1 (nbatch), hnu1 (nbatch).
+ * Host pointers (synchronous). */
+int a12_josh_batch(a12_ctx *ctx, int32_t ifscat, int32_t ifsurf,
+                   int32_t nrhox, int32_t nbatch, const double *rhox,
+                   const double *acont, const double *aline,
+                   const double *sigmac, const double *sigmal,
+                   const double *scont, const double *sline,
+                   const double *bnu, double *hnu1, double *knu1,
+                   double *taunu, double *snu, double *jnu, double *hnu,
+                   double *jmins);
+
+/* ---- measurement --------------------------------------------------------- */
+/* Stage timing with HIP events on the context stream (off by default). */
+int a12_set_profiling(a12_ctx *ctx, int32_t on);
+/* ms[0..4]: depth prep + continuum, line set-up, class partition, tile
+ * gather (of the last opacity run), SYNTHE transfer (of the last transfer
+ * run); -1 where not measured.  Synchronises on the events. */
+int a12_stage_ms(a12_ctx *ctx, float *ms);
+/* Contributions of the last opacity run: all, and those on the Voigt core
+ * (centre included); the difference is the n^-2 far-wing terms. */
+int a12_synthe_eval_split(a12_ctx *ctx, int64_t *total, int64_t *core);
+
+/* ---- Voigt function, batched ------------------------------------------- */
+/* voigt_profile(v, a), src/synthe_module.f90:463-537, on n points.
+ * prec 0: the mixed path's FP32 evaluation; 1: FP64 (the reference's
+ * arithmetic, not rounded); 2: the reference's FP32-rounded result.
+ * Host pointers (synchronous); h receives the values in double. */
+int a12_voigt_profile(a12_ctx *ctx, int32_t prec, int32_t n, const float *v,
+                      const float *a, double *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -15,6 +15,19 @@
 
 #define KW_MAX 128
 
+/* REAL(8)**INTEGER as Fortran evaluates it: repeated squaring, the bits of
+ * the exponent taken from the lowest. */
+static double powi_f64(double x, int m)
+{
+    unsigned n = m < 0 ? -m : m;
+    double y = (n & 1u) ? x : 1.0;
+    while (n >>= 1) {
+        x = x * x;
+        if (n & 1u) y = y * x;
+    }
+    return m < 0 ? 1.0 / y : y;
+}
+
 /* mod_constants, src/atlas12_modules.f90:315-355 */
 static const double C_SQRTPI = 1.7724538509055159;
 static const double C_CLIGHT_NMS = 2.99792458e17;
@@ -546,7 +559,7 @@ void or_synthe_continuum(const or_synthe_atm *atm, int j, float *out,
     const double *ab = atm->continall + (size_t)j * atm->nf;
     int ie = 0; /* 0-based iedge */
     for (int i = 0; i < atm->length; ++i) {
-        double wave8 = atm->wbegin * pow(atm->ratio, (double)i);
+        double wave8 = atm->wbegin * powi_f64(atm->ratio, i);
         while (wave8 >= w[ie + 1] && ie < ne - 2) ie++;
         double x = ((wave8 - half[ie]) * (wave8 - w[ie + 1]) * ab[3 * ie]
                     + (w[ie] - wave8) * (wave8 - w[ie + 1]) * 2.0 * ab[3 * ie + 1]
@@ -561,9 +574,9 @@ void or_synthe_continuum(const or_synthe_atm *atm, int j, float *out,
     }
 }
 
-void or_synthe_line_opacity(const or_synthe_atm *atm, const or_line_t *lines,
-                            int nlines, int prec, float *out32, double *out64,
-                            int64_t *evals, int32_t *nstep_out)
+void or_synthe_line_opacity_range(const or_synthe_atm *atm, const or_line_t *lines,
+                                  int nlines, int prec, int j0, int j1, float *out32,
+                                  double *out64, int64_t *evals, int32_t *nstep_out)
 {
     const int L = atm->length, mw6 = atm->mw6;
     float *cont = (float *)malloc(sizeof(float) * L);
@@ -576,7 +589,7 @@ void or_synthe_line_opacity(const or_synthe_atm *atm, const or_line_t *lines,
     const double resolu = atm->resolu;
     int64_t nev = 0;
 
-    for (int j = 0; j < atm->nrhox; ++j) {
+    for (int j = j0; j < j1; ++j) {
         or_synthe_continuum(atm, j, cont, cont64);
         for (int i = 0; i < L; ++i) { buf[i] = 0.0f; buf64[i] = 0.0; }
         /* per-depth prep, src/synthe.f90:518-551 */
@@ -699,6 +712,14 @@ void or_synthe_line_opacity(const or_synthe_atm *atm, const or_line_t *lines,
     free(xnfdop);
 }
 
+void or_synthe_line_opacity(const or_synthe_atm *atm, const or_line_t *lines,
+                            int nlines, int prec, float *out32, double *out64,
+                            int64_t *evals, int32_t *nstep_out)
+{
+    or_synthe_line_opacity_range(atm, lines, nlines, prec, 0, atm->nrhox, out32, out64,
+                                 evals, nstep_out);
+}
+
 /* ===================================================================== */
 /* SYNTHE transfer per wavelength point                                   */
 /* src/synthe.f90:698-718, setup_opacity_sv :797-833,                     */
@@ -723,7 +744,7 @@ void or_synthe_transfer_point(const or_synthe_rt *rt, int iwave,
     double acont[KW_MAX], sigmac[KW_MAX], aline[KW_MAX], sigmal[KW_MAX];
     double scont[KW_MAX], sline[KW_MAX], bnu[KW_MAX], ehvkt[KW_MAX], stim[KW_MAX];
     double asyn[KW_MAX];
-    double wave = rt->wbegin * pow(rt->ratio, (double)iwave);
+    double wave = rt->wbegin * powi_f64(rt->ratio, iwave);
     double freq8 = C_CLIGHT_NMS / wave;
     for (int j = 0; j < n; ++j) {
         if (opac_col64)

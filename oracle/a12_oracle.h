@@ -103,8 +103,22 @@ void or_synthe_line_opacity(const or_synthe_atm *atm, const or_line_t *lines,
                             int nlines, int prec, float *out32, double *out64,
                             int64_t *evals, int32_t *nstep_out);
 
+/* Depths [j0, j1) only (outputs indexed by absolute depth). */
+void or_synthe_line_opacity_range(const or_synthe_atm *atm, const or_line_t *lines,
+                                  int nlines, int prec, int j0, int j1, float *out32,
+                                  double *out64, int64_t *evals, int32_t *nstep_out);
+/* OpenMP over depths (one whole depth per thread): identical results. */
+void or_synthe_line_opacity_omp(const or_synthe_atm *atm, const or_line_t *lines,
+                                int nlines, int prec, int j0, int j1, float *out32,
+                                double *out64, int64_t *evals);
+int or_omp_threads(int n);
+struct or_synthe_rt_s;
+void or_synthe_transfer_omp(const struct or_synthe_rt_s *rt, const float *opac,
+                            const double *opac64, double *h_line, double *h_cont,
+                            int i0, int i1);
+
 /* ---- SYNTHE transfer (src/synthe.f90:698-718, :797-902) --------------- */
-typedef struct {
+typedef struct or_synthe_rt_s {
     int     nrhox, length;
     double  wbegin, ratio;
     int     nedge;

@@ -15,6 +15,21 @@ int or_omp_threads(int n)
     return omp_get_max_threads();
 }
 
+void or_synthe_line_opacity_omp(const or_synthe_atm *atm, const or_line_t *lines,
+                                int nlines, int prec, int j0, int j1, float *out32,
+                                double *out64, int64_t *evals)
+{
+    int64_t tot = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : tot)
+    for (int j = j0; j < j1; ++j) {
+        int64_t ev = 0;
+        or_synthe_line_opacity_range(atm, lines, nlines, prec, j, j + 1, out32, out64, &ev,
+                                     NULL);
+        tot += ev;
+    }
+    if (evals) *evals = tot;
+}
+
 void or_synthe_transfer_omp(const or_synthe_rt *rt, const float *opac,
                             const double *opac64, double *h_line,
                             double *h_cont, int i0, int i1)
@@ -25,4 +40,19 @@ void or_synthe_transfer_omp(const or_synthe_rt *rt, const float *opac,
         int e = b + chunk < i1 ? b + chunk : i1;
         or_synthe_transfer(rt, opac, opac64, h_line + (b - i0), h_cont + (b - i0), b, e);
     }
+}
+
+void or_synthe_line_opacity_omp_list(const or_synthe_atm *atm, const or_line_t *lines,
+                                     int nlines, int prec, const int *js, int nj,
+                                     float *out32, double *out64, int64_t *evals)
+{
+    int64_t tot = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : tot)
+    for (int k = 0; k < nj; ++k) {
+        int64_t ev = 0;
+        or_synthe_line_opacity_range(atm, lines, nlines, prec, js[k], js[k] + 1, out32, out64,
+                                     &ev, NULL);
+        tot += ev;
+    }
+    if (evals) *evals = tot;
 }

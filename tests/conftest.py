@@ -1,0 +1,57 @@
+"""Shared fixtures.  GPU tests carry @pytest.mark.gpu; everything else runs
+on the CPU (oracle against golden vectors, host logic, ABI surface)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(REPO, "tests", "golden")
+if REPO not in sys.path:
+    sys.path.insert(0, REPO)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+
+
+@pytest.fixture(scope="session")
+def golden():
+    return lambda name: np.load(os.path.join(GOLDEN, name))
+
+
+@pytest.fixture(scope="session")
+def small_state():
+    from atlas12_amd.state import SyntheState
+    return SyntheState.load(os.path.join(GOLDEN, "synthe_small_state.npz"))
+
+
+@pytest.fixture(scope="session")
+def small_ref():
+    from atlas12_amd.workloads import LTE_DTYPE
+    z = np.load(os.path.join(GOLDEN, "synthe_small_ref.npz"))
+    return {"lines": z["lines"].view(LTE_DTYPE), "spec": z["spec"], "gfall": z["gfall"]}
+
+
+@pytest.fixture(scope="session")
+def coef():
+    from atlas12_amd.state import load_coef
+    return load_coef(os.path.join(GOLDEN, "josh_coef.npz"))
+
+
+@pytest.fixture(scope="session")
+def orc():
+    from oracle import oracle
+    oracle.build()
+    return oracle
+
+
+@pytest.fixture(scope="session")
+def gpu_ctx():
+    """A device context on cuda:0 -- fails (does not skip) when the HIP
+    library is missing, so a GPU run can never pass on a fallback."""
+    from atlas12_amd import native
+    ctx = native.Context(0)
+    yield ctx
+    ctx.close()

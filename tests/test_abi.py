@@ -1,0 +1,58 @@
+"""The C-ABI library loads and exports every entry point include/a12gpu.h
+declares (no compute calls: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(REPO, "include", "a12gpu.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(a12_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    for s in ("a12_ctx_create", "a12_synthe_line_opacity", "a12_synthe_transfer", "a12_josh_batch"):
+        assert s in syms
+
+
+def test_library_exports_every_declared_symbol():
+    from atlas12_amd import native
+    lib = native.load_library()
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Sizes and field offsets of the ctypes mirrors equal the C compiler's
+    layout of the header's structs."""
+    import subprocess
+    from atlas12_amd import native
+    from atlas12_amd.workloads import LTE_DTYPE
+    fields = [f[0] for f in native.SyntheAtm._fields_]
+    src = tmp_path / "layout.c"
+    body = "".join(f'printf("%zu\\n", offsetof(a12_synthe_atm, {f}));' for f in fields)
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "a12gpu.h"\n'
+                   'int main(void){printf("%zu\\n%zu\\n", sizeof(a12_lte_line), sizeof(a12_synthe_atm));'
+                   + body + "return 0;}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(REPO, "include"), str(src), "-o", str(exe)], check=True)
+    vals = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
+    assert vals[0] == ctypes.sizeof(native.LteLine) == LTE_DTYPE.itemsize == 32
+    assert vals[1] == ctypes.sizeof(native.SyntheAtm)
+    assert vals[2:] == [getattr(native.SyntheAtm, f).offset for f in fields]
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    from atlas12_amd import native
+    import pytest
+    saved = native._lib
+    native._lib = None
+    try:
+        with pytest.raises(native.A12Error):
+            native.load_library(str(tmp_path / "nope.so"))
+    finally:
+        native._lib = saved

@@ -1,0 +1,147 @@
+"""GPU parity of the SYNTHE hot path (line opacity + transfer) through the
+C-ABI, against the oracle and the reference's own outputs.
+
+Tolerances (stated per check):
+* FP64 GPU path vs FP64 oracle: same decisions, identical eval count,
+  opacity within 1e-11 relative (summation order differs), transfer 1e-12.
+* Mixed path vs the reference-typed oracle: identical eval count (the
+  extent decisions use the reference's FP64-inside Voigt); opacity within
+  FP32 Voigt error (see test body); emergent flux within 1e-4 of FP64.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b, floor=1e-300):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.abs(a - b) / np.maximum(np.abs(b), floor)
+
+
+@pytest.fixture(scope="module")
+def oracle_small(orc, small_state, small_ref):
+    op32, ev32 = orc.synthe_line_opacity(small_state, small_ref["lines"], "ref")
+    op64, ev64 = orc.synthe_line_opacity(small_state, small_ref["lines"], "f64")
+    return dict(op32=op32, ev32=ev32, op64=op64, ev64=ev64)
+
+
+def test_josh_batch_matches_reference(gpu_ctx, coef, golden):
+    """JOSH on 40 golden cases (both output modes, IFSCAT 0/1, the deep-only
+    branch) vs the reference routine's own outputs."""
+    g = golden("josh_ref.npz")
+    gpu_ctx.set_operators(*coef)
+    inp, out = g["inputs"], g["out"]
+    n = inp.shape[2]
+    for k in range(len(g["ifscat"])):
+        x = inp[k]
+        r = gpu_ctx.josh_batch(int(g["ifscat"][k]), int(g["ifsurf"][k]), x[6], *[x[i][None] for i in
+                               (0, 1, 2, 3, 4, 5, 7)])
+        assert _rel(r["hnu1"][0], out[k, 0]) <= 1e-12, k
+        if g["ifsurf"][k] == 0:
+            assert _rel(r["knu1"][0], out[k, 1]) <= 1e-12, k
+            for i, name in enumerate(("taunu", "snu", "jnu", "hnu", "jmins")):
+                ref = out[k, 2 + i * n: 2 + (i + 1) * n]
+                got = r[name][0]
+                scale = np.max(np.abs(ref))
+                assert np.max(np.abs(got - ref)) <= 1e-12 * scale, (k, name)
+
+
+def test_line_opacity_fp64_matches_oracle(gpu_ctx, small_state, small_ref, oracle_small):
+    from atlas12_amd import native
+    op, ev = native.synthe_line_opacity(gpu_ctx, small_state, small_ref["lines"], native.PREC_FP64)
+    assert ev == oracle_small["ev64"]
+    ref = oracle_small["op64"]
+    scale = ref.max()
+    assert np.max(np.abs(op - ref) / np.maximum(ref, 1e-300)) <= 1e-11
+    assert np.all(np.isfinite(op)) and scale > 0
+
+
+def test_line_opacity_mixed_matches_oracle(gpu_ctx, small_state, small_ref, oracle_small):
+    from atlas12_amd import native
+    op, ev = native.synthe_line_opacity(gpu_ctx, small_state, small_ref["lines"], native.PREC_MIXED)
+    assert op.dtype == np.float32
+    # extent decisions follow the reference exactly -> identical count
+    assert ev == oracle_small["ev32"]
+    r = _rel(op, oracle_small["op32"], 1e-30)
+    # FP32 Voigt: relative error grows in the Lorentz part of the core where
+    # Re w(z) is small against |w(z)|; points are sums dominated by cores or
+    # far wings, bounded here at 2e-4 worst case and 1e-6 median.
+    assert np.median(r) <= 1e-6, np.median(r)
+    assert r.max() <= 2e-4, r.max()
+
+
+def test_transfer_matches_oracle(gpu_ctx, coef, small_state, orc, oracle_small):
+    from atlas12_amd import native
+    gpu_ctx.set_operators(*coef)
+    hl, hc = native.synthe_transfer(gpu_ctx, small_state, oracle_small["op32"])
+    ohl, ohc = orc.synthe_transfer(small_state, oracle_small["op32"], *coef)
+    assert _rel(hl, ohl).max() <= 1e-12
+    assert _rel(hc, ohc).max() <= 1e-12
+
+
+def test_end_to_end_matches_reference_spec(gpu_ctx, coef, small_state, small_ref):
+    """Mixed GPU opacity + GPU transfer vs the reference synthe.exe .spec
+    (printed with 6 significant digits: 5e-6 relative rounding)."""
+    from atlas12_amd import native
+    gpu_ctx.set_operators(*coef)
+    gpu_ctx.synthe_load(small_state, small_ref["lines"])
+    gpu_ctx.synthe_opacity_run(native.PREC_FP64)
+    gpu_ctx.synthe_transfer_run()
+    hl64, hc64 = gpu_ctx.synthe_transfer_fetch()
+    gpu_ctx.synthe_opacity_run(native.PREC_MIXED)
+    gpu_ctx.synthe_transfer_run()
+    hl, hc = gpu_ctx.synthe_transfer_fetch()
+    spec = small_ref["spec"]
+    assert _rel(hc, spec[:, 2]).max() <= 5.1e-6
+    assert _rel(hl, spec[:, 1]).max() <= 5.1e-6 + 2e-5
+    # mixed vs FP64 emergent flux
+    err = _rel(hl, hl64)
+    assert err.max() <= 1e-4, err.max()
+
+
+def test_empty_line_list(gpu_ctx, small_state):
+    from atlas12_amd import native
+    op, ev = native.synthe_line_opacity(gpu_ctx, small_state, np.zeros(0, dtype=native.LTE_DTYPE))
+    assert ev == 0 and op.shape == (small_state.nrhox, small_state.length)
+    assert not op.any()
+
+
+def test_edge_lines(gpu_ctx, small_state, small_ref, orc):
+    """Lines centred on the first/last grid point, just off either end,
+    far off-grid with wings reaching in, and strong lines whose far wing
+    covers the whole grid."""
+    from atlas12_amd import native
+    lines = small_ref["lines"]
+    L = small_state.length
+    pick = lines[np.argsort(lines["cgf"])[-8:]].copy()  # strongest lines
+    pick["nbuff"] = [1, L, 0, L + 1, -300, L + 300, L // 2, 2]
+    pick["elo"][:] = 0.0
+    op, ev = native.synthe_line_opacity(gpu_ctx, small_state, pick, native.PREC_FP64)
+    ref, rev = orc.synthe_line_opacity(small_state, pick, "f64")
+    assert ev == rev
+    assert np.max(np.abs(op - ref) / np.maximum(ref, 1e-300)) <= 1e-11
+    op32, ev32 = native.synthe_line_opacity(gpu_ctx, small_state, pick, native.PREC_MIXED)
+    ref32, rev32 = orc.synthe_line_opacity(small_state, pick, "ref")
+    assert ev32 == rev32
+    assert _rel(op32, ref32, 1e-30).max() <= 2e-4
+
+
+def test_unsorted_lines_same_result(gpu_ctx, small_state, small_ref):
+    """The library sorts by centre; result independent of input order up to
+    FP64 summation order."""
+    from atlas12_amd import native
+    lines = small_ref["lines"]
+    perm = np.random.default_rng(3).permutation(len(lines))
+    a, ea = native.synthe_line_opacity(gpu_ctx, small_state, lines, native.PREC_FP64)
+    b, eb = native.synthe_line_opacity(gpu_ctx, small_state, lines[perm], native.PREC_FP64)
+    assert ea == eb
+    assert np.max(np.abs(a - b) / np.maximum(a, 1e-300)) <= 1e-13
+
+
+def test_deterministic(gpu_ctx, small_state, small_ref):
+    from atlas12_amd import native
+    a, _ = native.synthe_line_opacity(gpu_ctx, small_state, small_ref["lines"], native.PREC_MIXED)
+    b, _ = native.synthe_line_opacity(gpu_ctx, small_state, small_ref["lines"], native.PREC_MIXED)
+    assert np.array_equal(a, b)
