@@ -244,3 +244,52 @@ def synthe_plain_lines(grid: Grid, n: int = 200_000, seed: int = 1) -> np.ndarra
     """The plain-SYNTHE line list: n atomic lines over the grid's window."""
     rec = synthetic_gfall(n, grid.wlbeg, grid.wlend, seed)
     return gfall_to_lte(rec, grid)
+
+
+# ---------------------------------------------------------------------------
+# ATLAS12 opacity-sampling line store
+# ---------------------------------------------------------------------------
+
+# ATLAS species slot of atom Z, ion stage i (1 = neutral): Z(Z+1)/2 + i - 1
+# (src/atlas12_modules.f90:10148-10150, NOFF :10194-10197)
+def atlas_nelion(z: int, ion: int) -> int:
+    return z * (z + 1) // 2 + ion - 1
+
+
+ATLAS_SPECIES = tuple(atlas_nelion(z, i) for z, i in
+                      ((26, 1), (26, 2), (20, 1), (20, 2), (22, 1), (11, 1), (12, 1)))
+LOG_RATIO_2E6 = math.log(1.0 + 1.0 / 2000000.0)
+
+
+def _tablog_code(x):
+    """Inverse of TABLOG (src/atlas12_modules.f90:2083-2086): log10 value ->
+    int16 code 1000 x + 16384."""
+    return np.clip(np.rint(np.asarray(x) * 1000.0) + 16384, -32768, 32767).astype(np.int16)
+
+
+def atlas_linedata(n: int, wl0: float, wl1: float, seed: int, species=ATLAS_SPECIES,
+                   loggf=("normal", -3.0, 1.5), elo=("exp", 2.0), lgr=(7.0, 9.0),
+                   lgs=(-6.0, -4.0), lgw=(-8.0, -7.0)) -> np.ndarray:
+    """Seeded LINEDATA records (4 x int32 per line: IWL, then int16 pairs
+    (IELION, IELO), (IGFLOG, IGR), (IGS, IGW)) -- the packed layout that
+    SELECTLINES stores and LINOP1 unpacks (src/atlas12_modules.f90:1950-1963).
+    Sorted by IWL."""
+    rng = np.random.default_rng(seed)
+    lam = np.exp(rng.uniform(math.log(wl0), math.log(wl1), n))
+    iwl = np.rint(np.log(lam) / LOG_RATIO_2E6).astype(np.int64)
+    sp = np.asarray(species)[rng.integers(0, len(species), n)]
+    gf = rng.normal(loggf[1], loggf[2], n) if loggf[0] == "normal" else rng.uniform(loggf[1], loggf[2], n)
+    e_ev = rng.exponential(elo[1], n) if elo[0] == "exp" else rng.uniform(elo[1], elo[2], n)
+    e_cm = np.maximum(e_ev * EV_TO_CM, 1.0e-3)
+    rec = np.zeros((n, 4), np.int32)
+    rec[:, 0] = iwl
+    pairs = np.zeros((n, 6), np.int16)
+    pairs[:, 0] = (sp * 10).astype(np.int16)
+    pairs[:, 1] = _tablog_code(np.log10(e_cm))
+    pairs[:, 2] = _tablog_code(gf)
+    pairs[:, 3] = _tablog_code(rng.uniform(*lgr, n))
+    pairs[:, 4] = _tablog_code(rng.uniform(*lgs, n))
+    pairs[:, 5] = _tablog_code(rng.uniform(*lgw, n))
+    rec[:, 1:] = pairs.view(np.int32).reshape(n, 3)
+    order = np.argsort(rec[:, 0], kind="stable")
+    return np.ascontiguousarray(rec[order])

@@ -237,3 +237,98 @@ def synthe_transfer(st, opac, coefj, coefh, i0=0, i1=None, threads=1):
     else:
         L.or_synthe_transfer(C.byref(r), *args, _p(hl), _p(hc), C.c_int(i0), C.c_int(i1))
     return hl, hc
+
+
+# ---------------------------------------------------------------- ATLAS12
+class OrAtlasLinesIn(C.Structure):
+    _fields_ = [("nrhox", C.c_int), ("numnu", C.c_int), ("nlines", C.c_int), ("mion", C.c_int),
+                ("ldion", C.c_int), ("nwave1", C.c_int), ("waveset", P_D),
+                ("linedata", C.c_void_p), ("tabcont", P_D), ("iwavetab", C.c_void_p),
+                ("xnfdop", P_D), ("dopple", P_D)] + [(k, P_D) for k in (
+                    "t", "xne", "hckt", "xnf1", "xnf3", "xnf841", "h0", "h1", "h2")]
+
+
+class OrRtIn(C.Structure):
+    _fields_ = [("nrhox", C.c_int), ("nfreq", C.c_int), ("teff", C.c_double),
+                ("wave", P_D), ("rcowt", P_D), ("t", P_D), ("hkt", P_D), ("rhox", P_D)] + [
+        (k, P_D) for k in ("acont", "sigmac", "scont", "sline", "aline", "sigmal", "xlines",
+                           "coefj", "coefh")]
+
+
+class OrRtAcc(C.Structure):
+    _fields_ = [(k, C.c_double * 128) for k in ("rjmins", "rdabh", "rdiagj", "flxrad", "raden",
+                                                "h_total", "accrad", "abross")] + [
+        ("pradk0", C.c_double)]
+
+
+def tabvoigt(vsteps=200.0, n=2001):
+    h = [np.zeros(n) for _ in range(3)]
+    lib().or_tabvoigt(C.c_double(vsteps), C.c_int(n), *[_p(a) for a in h])
+    return h
+
+
+def voigt_kurucz(v, a, tabs):
+    L = lib()
+    L.or_voigt_kurucz.restype = C.c_double
+    h0, h1, h2 = [_f64(t) for t in tabs]
+    return np.array([L.or_voigt_kurucz(C.c_double(x), C.c_double(y), _p(h0), _p(h1), _p(h2))
+                     for x, y in zip(v, a)])
+
+
+def full_ion(sub, idx, mion, n):
+    """(mion, n) array with the given species rows filled."""
+    out = np.zeros((mion, n))
+    out[idx] = sub
+    return out
+
+
+def linop1(z):
+    """LINOP1 on an ATLAS state dict (as in tests/golden/atlas_small.npz).
+    Returns (xlines (numnu, nrhox), evals)."""
+    n = int(len(z["t"]))
+    numnu = int(len(z["waveset"]))
+    mion = int(z["mion"])
+    k = _Keep()
+    k.a = {"waveset": _f64(z["waveset"]), "tabcont": _f64(z["tabcont"]),
+           "xnfdop": _f64(full_ion(z["xnfdop"], z["ion_idx"], mion, n)),
+           "dopple": _f64(full_ion(z["dopple"], z["ion_idx"], mion, n))}
+    for kk in ("t", "xne", "hckt", "xnf1", "xnf3", "xnf841", "h0", "h1", "h2"):
+        k.a[kk] = _f64(z[kk])
+    ld = np.ascontiguousarray(z["linedata"], np.int32)
+    iw = np.ascontiguousarray(z["iwavetab"], np.int32)
+    s = OrAtlasLinesIn()
+    s.nrhox, s.numnu, s.nlines, s.mion, s.ldion = n, numnu, len(ld), mion, n
+    s.nwave1 = len(iw)
+    s.linedata = ld.ctypes.data_as(C.c_void_p)
+    s.iwavetab = iw.ctypes.data_as(C.c_void_p)
+    for kk, v in k.a.items():
+        setattr(s, kk, _p(v))
+    out = np.zeros((numnu, n))
+    ev = C.c_int64(0)
+    lib().or_linop1(C.byref(s), _p(out), C.byref(ev))
+    return out, ev.value
+
+
+def rt_pass(z, xlines, coefj, coefh):
+    """RUN_RT_PASS's frequency loop on tabulated KAPP outputs.  Returns
+    (accumulator dict, hnu1 per frequency)."""
+    n = int(len(z["t"]))
+    nf = int(len(z["waveset"]))
+    keep = {k: _f64(z[k]) for k in ("waveset", "rcoset", "t", "hkt", "rhox", "acont", "sigmac",
+                                    "scont", "sline", "aline", "sigmal")}
+    keep["xlines"] = _f64(xlines)
+    keep["coefj"], keep["coefh"] = _f64(coefj), _f64(coefh)
+    s = OrRtIn()
+    s.nrhox, s.nfreq, s.teff = n, nf, float(z["teff"])
+    s.wave = _p(keep["waveset"])
+    s.rcowt = _p(keep["rcoset"])
+    for kk in ("t", "hkt", "rhox", "acont", "sigmac", "scont", "sline", "aline", "sigmal",
+               "xlines", "coefj", "coefh"):
+        setattr(s, kk, _p(keep[kk]))
+    acc = OrRtAcc()
+    h1 = np.zeros(nf)
+    lib().or_rt_pass(C.byref(s), C.byref(acc), _p(h1))
+    out = {k: np.array(getattr(acc, k)[:n]) for k in ("rjmins", "rdabh", "rdiagj", "flxrad",
+                                                      "raden", "h_total", "accrad", "abross")}
+    out["pradk0"] = acc.pradk0
+    return out, h1

@@ -1,0 +1,141 @@
+! ref_atlas_dump -- runs the reference ATLAS12 first-iteration set-up
+! (READIN, populations, Doppler widths, KAPCONT) on a model deck with a
+! given line store, then records what the opacity-sampling hot path reads
+! and what the reference computes from it:
+!   * LINOP1's XLINES,
+!   * KAPP's outputs at every frequency of the grid,
+!   * RUN_RT_PASS's frequency-integrated accumulators.
+! Test infrastructure (built by oracle/ref/Makefile).
+!
+!   ATLAS12=<root> ref_atlas_dump <deck> <nustart> <nuhi> <lines.bin> <out.bin>
+!
+! lines.bin: int32 count, then count LINEDATA records (4 x int32).
+PROGRAM ref_atlas_dump
+  USE mod_atlas_data
+  IMPLICIT NONE
+  CHARACTER(LEN=512) :: deck, fin, fout, arg, envval
+  INTEGER :: envlen, envstat, u, nustart_in, nuhi_in, nl, nu, j, nel, i
+  REAL(8) :: freq15_l, xmax_l
+
+  CALL GET_COMMAND_ARGUMENT(1, deck)
+  CALL GET_COMMAND_ARGUMENT(2, arg); READ(arg, *) nustart_in
+  CALL GET_COMMAND_ARGUMENT(3, arg); READ(arg, *) nuhi_in
+  CALL GET_COMMAND_ARGUMENT(4, fin)
+  CALL GET_COMMAND_ARGUMENT(5, fout)
+
+  CALL GET_ENVIRONMENT_VARIABLE('ATLAS12', envval, envlen, envstat)
+  DATADIR = TRIM(envval) // '/data/'
+  CALL set_bc_data_dir(TRIM(DATADIR))
+  CALL TABVOIGT(200.0D0, 2001)
+  INPUTDATA = 5
+  CALL BLOCKJ
+  CALL BLOCKH
+  CALL IONPOTS
+  ITEMP = 0
+  INPUT_MODEL_FILE = deck
+  NUMITS = 1
+  IFPNCH(1) = 0
+  IFPRNT(1) = 1
+  CALL READIN(1)
+
+  ! Species masses: dominant isotope of each ion (src/atlas12c.f90:418-440)
+  XNF = 0.0D0
+  XNFP = 0.0D0
+  DOPPLE = 0.0D0
+  CALL ISOTOPES
+  DO nel = 1, MION
+    xmax_l = 0.0D0
+    DO i = 1, 10
+      IF (ISOTOPE(i, 2, nel) .GT. xmax_l) THEN
+        AMASSISO(1, nel) = ISOTOPE(i, 1, nel)
+        xmax_l = ISOTOPE(i, 2, nel)
+      END IF
+    END DO
+    IF (xmax_l .LE. 0.0D0 .AND. ISOTOPE(1, 1, nel) .GT. 0.0) AMASSISO(1, nel) = ISOTOPE(1, 1, nel)
+  END DO
+
+  ! Frequency grid of the ATLAS12 driver (src/atlas12c.f90:451-491),
+  ! log-spaced at 1e-4 dex from the given start index.
+  NULO = 1
+  NUHI = nuhi_in
+  NUSTEP = 1
+  NUMNU = NUHI
+  DO nu = NULO, NUHI
+    WAVESET(nu) = 10.0D0 ** (1.0D0 + 0.0001D0 * (nu + nustart_in - 1))
+  END DO
+  RCOSET(1) = (CLIGHT_NMS / WAVESET(1) - CLIGHT_NMS / WAVESET(2)) * 1.5D0
+  DO nu = 2, NUMNU - 1
+    RCOSET(nu) = (CLIGHT_NMS / WAVESET(nu - 1) - CLIGHT_NMS / WAVESET(nu + 1)) * 0.5D0
+  END DO
+  RCOSET(NUMNU) = (CLIGHT_NMS / WAVESET(NUMNU - 1) + CLIGHT_NMS / WAVESET(NUMNU)) * 0.25D0
+
+  ! First-iteration state (src/atlas12c.f90:500-563, without line selection)
+  ITER = 1
+  ITEMP = ITEMP + ITER
+  IF (IFPRES .NE. 0) THEN
+    PZERO = PCON + PRADK0 + PTURB0
+    DO j = 1, NRHOX
+      CHARGESQ(j) = XNE(j) * 2.0D0
+      IF (2.0D0 * XNE(j) - P(j) / TK(j) .GT. 0.0D0) &
+        CHARGESQ(j) = CHARGESQ(j) + 2.0D0 * (2.0D0 * XNE(j) - P(j) / TK(j))
+      PTOTAL(j) = GRAV * RHOX(j) + PZERO
+    END DO
+    IFEDNS = 0
+    CALL COMPUTE_ONE_POP(0.D0, 1, XNE)
+    CALL COMPUTE_ALL_POPS
+  END IF
+  IF (IFEDNS .EQ. 1) CALL ENERGY_DENSITY
+  DO j = 1, NRHOX
+    DO nel = 1, MION - 1
+      IF (AMASSISO(1, nel) .LE. 0.0D0) CYCLE
+      DOPPLE(j, nel) = SQRT(2.0D0 * TK(j) / AMASSISO(1, nel) / AMU + VTURB(j)**2) / CLIGHT
+      XNFDOP(j, nel) = XNFP(j, nel) / DOPPLE(j, nel) / RHO(j)
+    END DO
+  END DO
+  CALL KAPCONT
+
+  ! The line store (SELECTLINES' product) from the caller's file
+  OPEN(NEWUNIT=u, FILE=TRIM(fin), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
+  READ(u) nl
+  IF (ALLOCATED(LINEDATA)) DEALLOCATE(LINEDATA)
+  ALLOCATE(LINEDATA(4, MAX(nl, 1)))
+  READ(u) ((LINEDATA(i, j), i = 1, 4), j = 1, nl)
+  CLOSE(u)
+  NLINES_STORED = nl
+
+  OPEN(NEWUNIT=u, FILE=TRIM(fout), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+  WRITE(u) NRHOX, NUMNU, nl, MION, NWAVE
+  WRITE(u) TEFF, FLUX
+  WRITE(u) WAVESET(1:NUMNU), RCOSET(1:NUMNU)
+  WRITE(u) T(1:NRHOX), HKT(1:NRHOX), HCKT(1:NRHOX), XNE(1:NRHOX), RHOX(1:NRHOX), &
+           XNF(1:NRHOX, 1), XNF(1:NRHOX, 3), XNF(1:NRHOX, 841)
+  WRITE(u) XNFDOP(1:NRHOX, 1:MION), DOPPLE(1:NRHOX, 1:MION)
+  WRITE(u) TABCONT(1:NRHOX, 1:NWAVE + 1), IWAVETAB(1:NWAVE + 1)
+  WRITE(u) H0TAB(1:2001), H1TAB(1:2001), H2TAB(1:2001)
+  WRITE(u) COEFJ, COEFH
+
+  ! LINOP1 on its own
+  CALL LINOP1
+  WRITE(u) XLINES(1:NRHOX, 1:NUMNU)
+
+  ! KAPP at every frequency, with the frequency set-up of RUN_RT_PASS
+  DO nu = NULO, NUHI
+    WAVE = WAVESET(nu)
+    FREQ = CLIGHT_NMS / WAVE
+    WAVENO = 1.0D7 / WAVE
+    FREQLG = LOG(FREQ)
+    freq15_l = FREQ / 1.0D15
+    EHVKT = EXP(-FREQ * HKT)
+    STIM = 1.0D0 - EHVKT
+    BNU = BNU_PREFAC * freq15_l**3 * EHVKT / STIM
+    CALL KAPP
+    WRITE(u) ACONT(1:NRHOX), SIGMAC(1:NRHOX), SCONT(1:NRHOX), SLINE(1:NRHOX), &
+             ALINE(1:NRHOX), SIGMAL(1:NRHOX)
+  END DO
+
+  ! The reference's own frequency loop with its accumulators
+  CALL RUN_RT_PASS(.FALSE.)
+  WRITE(u) RJMINS(1:NRHOX), FLXRAD(1:NRHOX), RADEN(1:NRHOX), ACCRAD(1:NRHOX), &
+           ABROSS(1:NRHOX), PRADK0, RDIAGJNU(1:NRHOX), HNU(1:NRHOX), TAUNU(1:NRHOX)
+  CLOSE(u)
+END PROGRAM ref_atlas_dump

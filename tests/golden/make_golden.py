@@ -325,7 +325,93 @@ def golden_josh():
     return {"josh": {"cases": len(cases)}}
 
 
-STEPS = {"synthe_small": golden_synthe_small, "state_plain": golden_state_plain,
+def read_atlas_dump(path: str) -> dict:
+    with open(path, "rb") as fh:
+        buf = fh.read()
+    off = 0
+
+    def take(dt, n):
+        nonlocal off
+        a = np.frombuffer(buf, dtype=dt, count=n, offset=off)
+        off += a.nbytes
+        return a.copy()
+    n, numnu, nl, mion, nwave = (int(x) for x in take("<i4", 5))
+    d = dict(nrhox=n, numnu=numnu, nlines=nl, mion=mion, nwave1=nwave + 1)
+    d["teff"], d["flux"] = (float(x) for x in take("<f8", 2))
+    d["waveset"] = take("<f8", numnu)
+    d["rcoset"] = take("<f8", numnu)
+    for k in ("t", "hkt", "hckt", "xne", "rhox", "xnf1", "xnf3", "xnf841"):
+        d[k] = take("<f8", n)
+    d["xnfdop"] = take("<f8", n * mion).reshape(mion, n)
+    d["dopple"] = take("<f8", n * mion).reshape(mion, n)
+    d["tabcont"] = take("<f8", n * (nwave + 1)).reshape(nwave + 1, n)
+    d["iwavetab"] = take("<i4", nwave + 1)
+    for k in ("h0", "h1", "h2"):
+        d[k] = take("<f8", 2001)
+    d["coefj"] = take("<f8", 2601)
+    d["coefh"] = take("<f8", 2601)
+    d["xlines"] = take("<f8", n * numnu).reshape(numnu, n)
+    kapp = take("<f8", 6 * n * numnu).reshape(numnu, 6, n)
+    for i, k in enumerate(("acont", "sigmac", "scont", "sline", "aline", "sigmal")):
+        d[k] = np.ascontiguousarray(kapp[:, i, :])
+    acc = take("<f8", 5 * n + 1 + 3 * n)
+    for i, k in enumerate(("rjmins", "flxrad", "raden", "accrad", "abross")):
+        d["acc_" + k] = acc[i * n:(i + 1) * n]
+    d["acc_pradk0"] = acc[5 * n]
+    d["last_rdiagjnu"] = acc[5 * n + 1:6 * n + 1]
+    d["last_hnu"] = acc[6 * n + 1:7 * n + 1]
+    d["last_taunu"] = acc[7 * n + 1:8 * n + 1]
+    return d
+
+
+def atlas_case(atm, nustart, nuhi, linedata):
+    tmp = tempfile.mkdtemp(prefix="a12a_")
+    g = os.path.join(tmp, "gfall.dat")
+    open(g, "w").close()
+    root = data_root(tmp, g)
+    deck = os.path.join(tmp, "model.dat")
+    with open(deck, "w") as fh:
+        fh.write(atm.deck())
+    lp = os.path.join(tmp, "lines.bin")
+    with open(lp, "wb") as fh:
+        fh.write(np.int32(len(linedata)).tobytes() + np.ascontiguousarray(linedata).tobytes())
+    out = os.path.join(tmp, "dump.bin")
+    run([os.path.join(REFBIN, "ref_atlas_dump"), deck, str(nustart), str(nuhi), lp, out], tmp,
+        dict(os.environ, ATLAS12=root))
+    d = read_atlas_dump(out)
+    shutil.rmtree(tmp)
+    return d
+
+
+def golden_atlas_small():
+    """One ATLAS12 opacity-sampling pass on the plain-SYNTHE atmosphere:
+    240 frequencies from 500 nm (1e-4 dex steps), 3000 lines with the
+    distributions of the ATLAS12 workload.  Captures LINOP1's XLINES, the
+    KAPP outputs per frequency and RUN_RT_PASS's accumulators."""
+    atm = wl.scurve_atmosphere()
+    nustart = int(round((np.log10(500.0) - 1.0) / 1e-4)) + 1
+    nuhi = 240
+    w0 = 10 ** (1 + 1e-4 * nustart)
+    w1 = 10 ** (1 + 1e-4 * (nuhi + nustart - 1))
+    lines = wl.atlas_linedata(3000, w0 - 0.05, w1 - 0.2, seed=21)
+    d = atlas_case(atm, nustart, nuhi, lines)
+    ielion = np.ascontiguousarray(lines).view(np.int16).reshape(len(lines), 8)[:, 2]
+    used = np.unique(np.abs(ielion.astype(int)) // 10)
+    keep = {k: v for k, v in d.items() if not isinstance(v, (int, float))}
+    keep["xnfdop"] = d["xnfdop"][used - 1]
+    keep["dopple"] = d["dopple"][used - 1]
+    keep["ion_idx"] = (used - 1).astype(np.int32)
+    keep["linedata"] = lines
+    for k in ("coefj", "coefh"):
+        keep.pop(k)
+    np.savez_compressed(os.path.join(HERE, "atlas_small.npz"), nustart=nustart,
+                        teff=d["teff"], flux=d["flux"], mion=d["mion"],
+                        acc_pradk0=d["acc_pradk0"], **keep)
+    return {"atlas_small": {"nlines": len(lines), "numnu": d["numnu"],
+                            "nwave1": d["nwave1"]}}
+
+
+STEPS = {"synthe_small": golden_synthe_small, "atlas_small": golden_atlas_small, "state_plain": golden_state_plain,
          "numerics": golden_numerics, "josh": golden_josh}
 
 

@@ -88,3 +88,33 @@ def test_powf10_matches_libm():
     x = np.round(np.random.default_rng(0).uniform(-8, 2, 40000), 3).astype(np.float32)
     ref = np.array([libm.powf(10.0, float(v)) for v in x], np.float32)
     assert np.array_equal(powf10(x), ref)
+
+
+# ---------------------------------------------------------------- ATLAS12
+def test_tabvoigt_bitwise(orc, golden):
+    z = golden("atlas_small.npz")
+    for a, k in zip(orc.tabvoigt(), ("h0", "h1", "h2")):
+        assert np.array_equal(a, z[k])
+
+
+def test_linop1_bitwise(orc, golden):
+    """LINOP1's XLINES from the reference's first-iteration state and a
+    3000-line store, coarse/fine depth passes and both Voigt regimes."""
+    z = golden("atlas_small.npz")
+    xl, ev = orc.linop1(z)
+    assert ev > 0
+    assert np.array_equal(xl, z["xlines"])
+
+
+def test_rt_pass_accumulators_bitwise(orc, golden, coef):
+    """The frequency loop of RUN_RT_PASS from tabulated KAPP outputs
+    reproduces the reference's own accumulators (TCORR/RADIAP/ROSS modes
+    1-2) and the last frequency's flux bit for bit."""
+    z = golden("atlas_small.npz")
+    acc, h1 = orc.rt_pass(z, z["xlines"], *coef)
+    for k in ("rjmins", "flxrad", "raden", "accrad", "abross"):
+        assert np.array_equal(acc[k], z["acc_" + k]), k
+    assert acc["pradk0"] == z["acc_pradk0"]
+    assert h1[-1] == z["last_hnu"][0]
+    # H_TOTAL is FLXRAD's twin (the same sum in RADIAP)
+    assert np.array_equal(acc["h_total"], acc["flxrad"])
