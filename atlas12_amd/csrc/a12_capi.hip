@@ -44,7 +44,13 @@ int a12_ctx_destroy(a12_ctx *ctx)
                       &S.p_kap0, &S.p_adamp, &S.p_dv, &S.p_xwing, &S.emax, &S.lclass,
                       &S.cls_cnt, &S.cls_off, &S.perm, &S.nbuffP, &S.cls_range, &S.blk_evals,
                       &S.evals_out, &S.opacity, &S.hline, &S.hcont, &ctx->coefj, &ctx->coefh,
-                      &ctx->jb_in, &ctx->jb_out};
+                      &ctx->jb_in, &ctx->jb_out, &ctx->rt_part};
+    AtlasDev &A = ctx->atl;
+    DevBuf *abufs[] = {&A.depth3, &A.depth_lo, &A.tabcont, &A.xnfdop, &A.dopple, &A.waveset,
+                       &A.rcoset, &A.tabs, &A.tabs32, &A.l_nu, &A.l_nucont, &A.l_nel, &A.l_const,
+                       &A.seg, &A.state, &A.par, &A.reach, &A.xlines, &A.blk, &A.evals, &A.kapp,
+                       &A.hnu1, &A.acc};
+    for (DevBuf *b : abufs) b->release();
     for (DevBuf *b : bufs) b->release();
     for (hipEvent_t e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -217,6 +223,115 @@ int a12_josh_batch(a12_ctx *ctx, int32_t ifscat, int32_t ifsurf, int32_t nrhox, 
     const double *in[7] = {acont, aline, sigmac, sigmal, scont, sline, bnu};
     double *out[5] = {taunu, snu, jnu, hnu, jmins};
     return a12i_josh_batch(ctx, ifscat, ifsurf, nrhox, nbatch, rhox, in, hnu1, knu1, out);
+}
+
+int a12_atlas_load(a12_ctx *ctx, const a12_atlas_atm *atm, const int32_t *linedata,
+                   int32_t nlines)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_atlas_load(ctx, atm, linedata, nlines);
+}
+
+int a12_atlas_linop1_run(a12_ctx *ctx, int32_t prec)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_atlas_linop1_run(ctx, prec);
+}
+
+int a12_atlas_linop1_fetch(a12_ctx *ctx, double *xlines, int64_t *evals)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    AtlasDev &A = ctx->atl;
+    if (!A.xl_ready) {
+        ctx->err = "a12_atlas_linop1_fetch: no XLINES computed";
+        return A12_ERR_STATE;
+    }
+    if (xlines)
+        A12_HIP(hipMemcpyAsync(xlines, A.xlines.p, sizeof(double) * (size_t)A.n * A.numnu,
+                               hipMemcpyDeviceToHost, ctx->stream));
+    long long ev = 0;
+    A12_HIP(hipMemcpyAsync(&ev, A.evals.p, sizeof(ev), hipMemcpyDeviceToHost, ctx->stream));
+    A12_HIP(hipStreamSynchronize(ctx->stream));
+    if (evals) *evals = ev;
+    return A12_OK;
+}
+
+int a12_atlas_set_kapp(a12_ctx *ctx, const double *acont, const double *sigmac,
+                       const double *scont, const double *sline, const double *aline,
+                       const double *sigmal)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    AtlasDev &A = ctx->atl;
+    if (!A.loaded) {
+        ctx->err = "a12_atlas_set_kapp: call a12_atlas_load first";
+        return A12_ERR_STATE;
+    }
+    const double *src[6] = {acont, sigmac, scont, sline, aline, sigmal};
+    for (const double *p : src)
+        if (!p) {
+            ctx->err = "a12_atlas_set_kapp: null table";
+            return A12_ERR_ARG;
+        }
+    A12_HIP(hipSetDevice(ctx->device));
+    const size_t plane = (size_t)A.n * A.numnu;
+    A12_HIP(A.kapp.reserve(sizeof(double) * 6 * plane));
+    for (int k = 0; k < 6; ++k)
+        A12_HIP(hipMemcpyAsync(A.kapp.as<double>() + k * plane, src[k], sizeof(double) * plane,
+                               hipMemcpyHostToDevice, ctx->stream));
+    A12_HIP(hipStreamSynchronize(ctx->stream));
+    A.kapp_ready = true;
+    return A12_OK;
+}
+
+int a12_atlas_rt_run(a12_ctx *ctx, const double *xlines)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    AtlasDev &A = ctx->atl;
+    A12_HIP(hipSetDevice(ctx->device));
+    if (!A.kapp_ready) {
+        ctx->err = "a12_atlas_rt_run: call a12_atlas_set_kapp first";
+        return A12_ERR_STATE;
+    }
+    const size_t plane = (size_t)A.n * A.numnu;
+    if (xlines) {
+        A12_HIP(A.xlines.reserve(sizeof(double) * plane));
+        A12_HIP(hipMemcpyAsync(A.xlines.p, xlines, sizeof(double) * plane, hipMemcpyHostToDevice,
+                               ctx->stream));
+        A.xl_ready = true;
+    }
+    if (!A.xl_ready) {
+        ctx->err = "a12_atlas_rt_run: no XLINES (run linop1 or pass xlines)";
+        return A12_ERR_STATE;
+    }
+    A12_HIP(A.hnu1.reserve(sizeof(double) * A.numnu));
+    A12_HIP(A.acc.reserve(sizeof(double) * 9 * A.n));
+    int rc = a12i_atlas_rt_pass(ctx, A.n, A.numnu, A.teff, A.waveset.as<double>(),
+                                A.rcoset.as<double>(), A.depth3.as<double>(), A.kapp.as<double>(),
+                                A.xlines.as<double>(), A.hnu1.as<double>(), A.acc.as<double>());
+    if (rc) return rc;
+    A.rt_done = true;
+    ctx->syn.transferred = false;
+    return A12_OK;
+}
+
+int a12_atlas_rt_fetch(a12_ctx *ctx, double *acc, double *hnu1)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    AtlasDev &A = ctx->atl;
+    if (!A.rt_done) {
+        ctx->err = "a12_atlas_rt_fetch: no RT pass computed";
+        return A12_ERR_STATE;
+    }
+    if (acc)
+        A12_HIP(hipMemcpyAsync(acc, A.acc.p, sizeof(double) * 9 * A.n, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    if (hnu1)
+        A12_HIP(hipMemcpyAsync(hnu1, A.hnu1.p, sizeof(double) * A.numnu, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    A12_HIP(hipStreamSynchronize(ctx->stream));
+    return A12_OK;
 }
 
 int a12_set_profiling(a12_ctx *ctx, int32_t on)

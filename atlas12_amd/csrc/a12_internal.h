@@ -63,6 +63,17 @@ struct SyntheDev {
     bool transferred = false;
 };
 
+struct AtlasDev {
+    int n = 0, numnu = 0, mion = 0, nl = 0;
+    double teff = 0;
+    DevBuf depth3, depth_lo, tabcont, xnfdop, dopple, waveset, rcoset, tabs, tabs32;
+    DevBuf l_nu, l_nucont, l_nel, l_const, seg;
+    DevBuf state, par, reach, xlines, blk, evals;
+    DevBuf kapp, hnu1, acc;
+    bool loaded = false, xl_ready = false, kapp_ready = false, rt_done = false;
+    bool prof_lo = false;
+};
+
 struct a12_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -70,6 +81,8 @@ struct a12_ctx {
     DevBuf coefj, coefh;
     bool have_ops = false;
     SyntheDev syn;
+    AtlasDev atl;
+    DevBuf rt_part;
     // scratch for a12_josh_batch
     DevBuf jb_in, jb_out;
     // stage timing (a12_set_profiling): events on the context stream
@@ -100,7 +113,13 @@ int a12i_synthe_load(a12_ctx *ctx, const a12_synthe_atm *atm, const a12_lte_line
                      int32_t nlines);
 int a12i_synthe_run(a12_ctx *ctx, int prec);
 int a12i_voigt(a12_ctx *ctx, int prec, int n, const float *v, const float *a, double *h);
+// Implemented in a12_atlas.hip
+int a12i_atlas_load(a12_ctx *ctx, const a12_atlas_atm *a, const int32_t *linedata, int32_t nlines);
+int a12i_atlas_linop1_run(a12_ctx *ctx, int prec);
 // Implemented in a12_josh.hip
+int a12i_atlas_rt_pass(a12_ctx *ctx, int n, int nf, double teff, const double *d_wave,
+                       const double *d_rcoset, const double *d_depth3, const double *d_kapp,
+                       const double *d_xlines, double *d_hnu1, double *d_acc);
 int a12i_synthe_transfer(a12_ctx *ctx);
 int a12i_josh_batch(a12_ctx *ctx, int ifscat, int ifsurf, int nrhox, int nbatch,
                     const double *rhox, const double *const in[7], double *hnu1,

@@ -706,6 +706,182 @@ __global__ __launch_bounds__(64) void k_josh_batch(
     }
 }
 
+// EXPI, src/atlas12_modules.f90:21103-21225: E_1 by the Cody & Thacher
+// (1968) rational approximations (their published coefficients), E_n by
+// upward recurrence.
+__device__ double expi_dev(int n, double x)
+{
+#pragma clang fp contract(off)
+    const double ex = exp(-x);
+    double e1;
+    if (x > 4.0) {
+        e1 = (ex + ex * (-0.999999999998447 +
+                         (-26.6271060431811 +
+                          (-241.055827097015 +
+                           (-895.927957772937 + (-1298.85688746484 +
+                                                 (-545.374158883133 + -5.66575206533869 / x) / x) /
+                                                    x) /
+                               x) /
+                              x) /
+                             x) /
+                            (x + 28.6271060422192 +
+                             (292.310039388533 +
+                              (1332.78537748257 +
+                               (2777.61949509163 + (2404.01713225909 + 631.657483280800 / x) / x) / x) /
+                                  x) /
+                                 x)) /
+             x;
+    } else if (x > 1.0) {
+        e1 = ex *
+             (0.430967839469389 +
+              (6.90522522784444 +
+               (23.0192559391333 +
+                (24.3784088791317 + (9.04161556946329 + (0.999979577051595 + 4.65627107975096e-7 * x) * x) * x) * x) *
+                   x) *
+                  x) /
+             (0.103400130404874 +
+              (3.31909213593302 +
+               (20.4494785013794 + (41.2807841891424 + (32.4264210695138 + (10.0411643829054 + x) * x) * x) * x) *
+                   x) *
+                  x);
+    } else if (x > 0.0) {
+        e1 = (-44178.5471728217 +
+              (57721.7247139444 +
+               (9938.31388962037 + (1842.11088668000 + (101.093806161906 + 5.03416184097568 * x) * x) * x) * x) *
+                  x) /
+                 (76537.3323337614 +
+                  (32597.1881290275 + (6106.10794245759 + (635.419418378382 + (37.2298352833327 + x) * x) * x) * x) *
+                      x) -
+             log(x);
+    } else {
+        e1 = 0.0;
+    }
+    double r = e1;
+    for (int i = 1; i <= n - 1; ++i) r = (ex - x * r) / (double)i;
+    return r;
+}
+
+// --------------------------------------------------------------------------
+// ATLAS12 frequency loop (RUN_RT_PASS, src/atlas12_modules.f90:3905-3949)
+// on tabulated KAPP outputs: per frequency JOSH(1, 0), the negative floors,
+// and the per-frequency terms of TCORR(2) (:2465-2522), RADIAP(2)
+// (:5084-5096) and ROSS(2) (:5180-5190).  One wave per frequency; a wave
+// walks a contiguous chunk of frequencies and sums its terms in frequency
+// order; chunks are combined in order by k_rt_reduce.
+// Inputs (depth fastest, one column per frequency): kapp planes acont,
+// sigmac, scont, sline, aline, sigmal, then xlines.
+// part: (nchunk, 9, n) -- rjmins rdabh rdiagj flxrad raden h_total accrad
+// abross, then pradk0 in slot [8][0].
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_rt_pass(
+    int n, int nf, int chunk, double teff, const double *__restrict__ wave,
+    const double *__restrict__ rcoset, const double *__restrict__ depth3,
+    const double *__restrict__ kapp, const double *__restrict__ xlines,
+    const double *__restrict__ coefj, const double *__restrict__ coefh,
+    double *__restrict__ hnu1, double *__restrict__ part)
+{
+#pragma clang fp contract(off)
+    __shared__ JoshLDS W;
+    __shared__ double term2[NR];
+    const int ln = lane_id();
+    const double *t = depth3, *hkt = depth3 + n, *rhox = depth3 + 2 * n;
+    const size_t plane = (size_t)n * nf;
+    const double euler_m1 = 0.922784335098467;
+    const double bnu_prefac = 2.0 * A12_HPLANCK / (A12_CLIGHT * A12_CLIGHT) * 1.0e45;
+    for (int j = ln; j < n; j += 64) W.rhox[j] = rhox[j];
+    wsync();
+    double acc[8][2] = {};
+    double pradk0 = 0.0;
+    const int k0 = blockIdx.x * chunk, k1 = min(nf, k0 + chunk);
+    for (int k = k0; k < k1; ++k) {
+        const double freq = A12_CLIGHT_NMS / wave[k];
+        const double rcowt = rcoset[k];
+        const double f15 = freq / 1.0e15;
+        double stim[2], bnu[2];
+        int q = 0;
+        for (int j = ln; j < n; j += 64, ++q) {
+            const size_t o = (size_t)k * n + j;
+            const double ehv = exp(-freq * hkt[j]);
+            stim[q] = 1.0 - ehv;
+            bnu[q] = bnu_prefac * (f15 * f15 * f15) * ehv / stim[q];
+            const double aline = kapp[4 * plane + o] + xlines[o] * stim[q];
+            W.bnu[j] = bnu[q];
+            josh_prepare(W, j, kapp[o], aline, kapp[plane + o], kapp[5 * plane + o],
+                         kapp[2 * plane + o], kapp[3 * plane + o]);
+            W.snu[j] = 0.0;
+            W.jnu[j] = 0.0;
+            W.hnu[j] = 0.0;
+            W.jmins[j] = 0.0;
+        }
+        wsync();
+        const JoshOut r = josh_wave(W, n, 1, 0, coefj, coefh);
+        // floor unphysical negatives (:3926-3937)
+        int neg = 0;
+        for (int j = ln; j < n; j += 64)
+            if (W.snu[j] < 0.0 || W.jnu[j] < 0.0 || W.hnu[j] < 0.0) neg = 1;
+        if (__any(neg)) {
+            for (int j = ln; j < n; j += 64) {
+                W.jnu[j] = fmax(W.jnu[j], 1.0e-99);
+                W.snu[j] = fmax(W.snu[j], 1.0e-99);
+                W.hnu[j] = fmax(W.hnu[j], 1.0e-99);
+            }
+        }
+        wsync();
+        if (ln == 0) hnu1[k] = W.hnu[0];
+        // TCORR(2): d(abtot)/d(rhox), then the diagonal Lambda terms
+        deriv_wave(W.rhox, W.abtot, W.pb, n);
+        for (int j = ln; j < n; j += 64) {
+            double d = (j < n - 1 ? W.taunu[j + 1] - W.taunu[j] : W.taunu[n - 1] - W.taunu[n - 2]);
+            d = fmax(1.0e-10, d);
+            double t2;
+            if (d <= 0.01) {
+                t2 = (euler_m1 - log(d)) * d / 4.0 + d * d / 12.0 - d * d * d / 96.0 +
+                     d * d * d * d / 720.0;
+            } else {
+                double ex = 0.0;
+                if (d < 10.0) ex = expi_dev(3, d);
+                if (teff <= 4250.0 && d > 0.005 && d < 0.02) ex = 0.0;
+                t2 = 0.5 * (d + ex - 0.5) / d;
+            }
+            term2[j] = t2;
+        }
+        wsync();
+        q = 0;
+        for (int j = ln; j < n; j += 64, ++q) {
+            const double ab = W.abtot[j], al = W.alpha[j], h = W.hnu[j];
+            acc[0][q] = acc[0][q] + ab * W.jmins[j] * rcowt;
+            acc[1][q] = acc[1][q] + W.pb[j] / ab * h * rcowt;
+            const double diagj = (j > 0 ? term2[j - 1] : 0.0) + term2[j];
+            const double dbdt_t = bnu[q] * freq * hkt[j] / t[j] / fmax(stim[q], 1.0e-30);
+            acc[2][q] = acc[2][q] + ab * (diagj - 1.0) / (1.0 - al * diagj) * (1.0 - al) * dbdt_t * rcowt;
+            acc[3][q] = acc[3][q] + h * rcowt;
+            acc[4][q] = acc[4][q] + W.jnu[j] * rcowt;
+            acc[5][q] = acc[5][q] + h * rcowt;
+            acc[6][q] = acc[6][q] + ab * h * rcowt;
+            const double dbdt_r = bnu[q] * freq * hkt[j] / t[j] / stim[q];
+            acc[7][q] = acc[7][q] + dbdt_r / ab * rcowt;
+        }
+        pradk0 = pradk0 + r.knu1 * rcowt;
+        wsync();
+    }
+    double *pp = part + (size_t)blockIdx.x * 9 * n;
+    int q = 0;
+    for (int j = ln; j < n; j += 64, ++q)
+        for (int a = 0; a < 8; ++a) pp[a * n + j] = acc[a][q];
+    if (ln == 0) pp[8 * n] = pradk0;
+}
+
+// Fixed-order combination of the chunk partials: out(9, n).
+__global__ void k_rt_reduce(int n, int nchunk, const double *__restrict__ part,
+                            double *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // a * n + j
+    if (i >= 9 * n) return;
+    double s = 0.0;
+    for (int c = 0; c < nchunk; ++c) s += part[(size_t)c * 9 * n + i];
+    out[i] = s;
+}
+
 }  // namespace
 
 int a12i_synthe_transfer(a12_ctx *ctx)
@@ -775,5 +951,28 @@ int a12i_josh_batch(a12_ctx *ctx, int ifscat, int ifsurf, int nrhox, int nbatch,
         if (hnu1) hnu1[b] = hk[2 * b];
         if (knu1) knu1[b] = hk[2 * b + 1];
     }
+    return A12_OK;
+}
+
+int a12i_atlas_rt_pass(a12_ctx *ctx, int n, int nf, double teff, const double *d_wave,
+                       const double *d_rcoset, const double *d_depth3, const double *d_kapp,
+                       const double *d_xlines, double *d_hnu1, double *d_acc)
+{
+    if (!ctx->have_ops) {
+        ctx->err = "a12_atlas_rt_pass: call a12_set_operators first";
+        return A12_ERR_STATE;
+    }
+    const int chunk = std::max(1, (nf + 4095) / 4096);
+    const int nchunk = (nf + chunk - 1) / chunk;
+    A12_HIP(ctx->rt_part.reserve(sizeof(double) * 9 * (size_t)n * nchunk));
+    if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
+    hipLaunchKernelGGL(k_rt_pass, dim3(nchunk), dim3(64), 0, ctx->stream, n, nf, chunk, teff,
+                       d_wave, d_rcoset, d_depth3, d_kapp, d_xlines, ctx->coefj.as<double>(),
+                       ctx->coefh.as<double>(), d_hnu1, ctx->rt_part.as<double>());
+    A12_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_rt_reduce, dim3((9 * n + 255) / 256), dim3(256), 0, ctx->stream, n,
+                       nchunk, ctx->rt_part.as<double>(), d_acc);
+    A12_CHECK_LAUNCH();
+    if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[6], ctx->stream));
     return A12_OK;
 }

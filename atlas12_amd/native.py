@@ -29,7 +29,9 @@ EXPORTS = ["a12_ctx_create", "a12_ctx_destroy", "a12_last_error", "a12_stream",
            "a12_synthe_opacity_run", "a12_synthe_opacity_fetch", "a12_synthe_opacity_device",
            "a12_synthe_evals", "a12_synthe_transfer_run", "a12_synthe_transfer_fetch",
            "a12_synthe_transfer", "a12_josh_batch", "a12_voigt_profile",
-           "a12_set_profiling", "a12_stage_ms", "a12_synthe_eval_split"]
+           "a12_set_profiling", "a12_stage_ms", "a12_synthe_eval_split",
+           "a12_atlas_load", "a12_atlas_linop1_run", "a12_atlas_linop1_fetch",
+           "a12_atlas_set_kapp", "a12_atlas_rt_run", "a12_atlas_rt_fetch"]
 
 
 class A12Error(RuntimeError):
@@ -50,6 +52,17 @@ class SyntheAtm(C.Structure):
                 ("t", P_D), ("xne", P_D), ("rho", P_D), ("xnfh", P_D), ("xnfhe1", P_D),
                 ("xnfh2", P_D), ("hckt", P_D), ("hkt", P_D), ("hkt_atlas", P_D), ("rhox", P_D)]
 
+
+class AtlasAtm(C.Structure):
+    _fields_ = [("nrhox", C.c_int32), ("numnu", C.c_int32), ("mion", C.c_int32),
+                ("ldion", C.c_int32), ("ldtab", C.c_int32), ("nwave1", C.c_int32),
+                ("waveset", P_D), ("rcoset", P_D), ("tabcont", P_D), ("iwavetab", C.c_void_p),
+                ("xnfdop", P_D), ("dopple", P_D), ("t", P_D), ("xne", P_D), ("hckt", P_D),
+                ("hkt", P_D), ("rhox", P_D), ("xnf1", P_D), ("xnf3", P_D), ("xnf841", P_D),
+                ("h0tab", P_D), ("h1tab", P_D), ("h2tab", P_D), ("teff", C.c_double)]
+
+
+ACC_NAMES = ("rjmins", "rdabh", "rdiagj", "flxrad", "raden", "h_total", "accrad", "abross")
 
 _lib = None
 
@@ -248,3 +261,79 @@ def synthe_transfer(ctx: Context, st: SyntheState, opacity: np.ndarray):
                                            C.c_int32(prec), _ptr(hl), _ptr(hc)),
                "a12_synthe_transfer")
     return hl, hc
+
+
+# ---------------------------------------------------------------- ATLAS12
+def atlas_state_from(z, mion: int = 1006):
+    """ATLAS12 state dict (keys as in tests/golden/atlas_small.npz: species
+    rows of xnfdop/dopple given by ion_idx) -> arrays in the C-ABI layout."""
+    n = len(z["t"])
+    xnfdop = np.zeros((mion, n))
+    dopple = np.zeros((mion, n))
+    xnfdop[z["ion_idx"]] = z["xnfdop"]
+    dopple[z["ion_idx"]] = z["dopple"]
+    d = {k: _f64(z[k]) for k in ("waveset", "rcoset", "tabcont", "t", "xne", "hckt", "hkt",
+                                 "rhox", "xnf1", "xnf3", "xnf841", "h0", "h1", "h2")}
+    d["xnfdop"], d["dopple"] = xnfdop, dopple
+    d["iwavetab"] = np.ascontiguousarray(z["iwavetab"], np.int32)
+    d["teff"] = float(z["teff"])
+    d["mion"] = mion
+    return d
+
+
+def _atlas_struct(d):
+    a = AtlasAtm()
+    n = len(d["t"])
+    a.nrhox, a.numnu, a.mion, a.ldion, a.ldtab = n, len(d["waveset"]), d["mion"], n, n
+    a.nwave1 = len(d["iwavetab"])
+    for k in ("waveset", "rcoset", "tabcont", "xnfdop", "dopple", "t", "xne", "hckt", "hkt",
+              "rhox", "xnf1", "xnf3", "xnf841"):
+        setattr(a, k, _ptr(d[k]))
+    a.iwavetab = d["iwavetab"].ctypes.data_as(C.c_void_p)
+    a.h0tab, a.h1tab, a.h2tab = _ptr(d["h0"]), _ptr(d["h1"]), _ptr(d["h2"])
+    a.teff = d["teff"]
+    return a
+
+
+def atlas_load(ctx: Context, d, linedata):
+    ld = np.ascontiguousarray(linedata, np.int32)
+    a = _atlas_struct(d)
+    ctx._check(ctx.lib.a12_atlas_load(ctx.h, C.byref(a), ld.ctypes.data_as(C.c_void_p),
+                                      C.c_int32(len(ld))), "a12_atlas_load")
+    ctx._atl = (len(d["t"]), len(d["waveset"]))
+
+
+def atlas_linop1(ctx: Context, prec: int = PREC_FP64, fetch: bool = True):
+    """LINOP1 (src/atlas12_modules.f90:18132-18374): XLINES (numnu, nrhox)
+    and the contribution count."""
+    ctx._check(ctx.lib.a12_atlas_linop1_run(ctx.h, C.c_int32(prec)), "a12_atlas_linop1_run")
+    if not fetch:
+        return None, None
+    n, nf = ctx._atl
+    out = np.zeros((nf, n))
+    ev = C.c_int64(0)
+    ctx._check(ctx.lib.a12_atlas_linop1_fetch(ctx.h, _ptr(out), C.byref(ev)),
+               "a12_atlas_linop1_fetch")
+    return out, ev.value
+
+
+def atlas_set_kapp(ctx: Context, acont, sigmac, scont, sline, aline, sigmal):
+    ts = [_f64(t) for t in (acont, sigmac, scont, sline, aline, sigmal)]
+    ctx._check(ctx.lib.a12_atlas_set_kapp(ctx.h, *[_ptr(t) for t in ts]), "a12_atlas_set_kapp")
+
+
+def atlas_rt_pass(ctx: Context, xlines=None, fetch: bool = True):
+    """RUN_RT_PASS's frequency loop (src/atlas12_modules.f90:3905-3949):
+    accumulator dict and HNU(1) per frequency."""
+    xl = None if xlines is None else _f64(xlines)
+    ctx._check(ctx.lib.a12_atlas_rt_run(ctx.h, None if xl is None else _ptr(xl)),
+               "a12_atlas_rt_run")
+    if not fetch:
+        return None, None
+    n, nf = ctx._atl
+    acc = np.zeros(9 * n)
+    h1 = np.zeros(nf)
+    ctx._check(ctx.lib.a12_atlas_rt_fetch(ctx.h, _ptr(acc), _ptr(h1)), "a12_atlas_rt_fetch")
+    out = {k: acc[i * n:(i + 1) * n].copy() for i, k in enumerate(ACC_NAMES)}
+    out["pradk0"] = float(acc[8 * n])
+    return out, h1

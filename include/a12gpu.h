@@ -138,6 +138,54 @@ int a12_josh_batch(a12_ctx *ctx, int32_t ifscat, int32_t ifsurf,
                    double *taunu, double *snu, double *jnu, double *hnu,
                    double *jmins);
 
+/* ---- ATLAS12 opacity sampling ----------------------------------------- */
+/* State LINOP1 and the RUN_RT_PASS frequency loop read (module arrays of
+ * mod_atlas_data, src/atlas12_modules.f90:1858-1919, and the driver's grid,
+ * src/atlas12c.f90:451-491).  (ld, *) arrays keep the Fortran leading
+ * dimension (kw). */
+typedef struct {
+    int32_t nrhox;           /* layers (<= 128)                                */
+    int32_t numnu;           /* frequencies NUMNU                              */
+    int32_t mion;            /* species slots (1006)                           */
+    int32_t ldion;           /* leading dimension of xnfdop/dopple (kw)        */
+    int32_t ldtab;           /* leading dimension of tabcont (kw)              */
+    int32_t nwave1;          /* NWAVE + 1 continuum bins (362)                 */
+    const double *waveset;   /* (numnu) WAVESET [nm]                           */
+    const double *rcoset;    /* (numnu) RCOSET quadrature weights              */
+    const double *tabcont;   /* (ldtab, nwave1) TABCONT                        */
+    const int32_t *iwavetab; /* (nwave1) IWAVETAB with its sentinel            */
+    const double *xnfdop;    /* (ldion, mion) XNFDOP                           */
+    const double *dopple;    /* (ldion, mion) DOPPLE                           */
+    const double *t, *xne, *hckt, *hkt, *rhox;    /* (nrhox)                    */
+    const double *xnf1, *xnf3, *xnf841;           /* XNF(:,1), (:,3), (:,841)   */
+    const double *h0tab, *h1tab, *h2tab;          /* (2001) TABVOIGT tables     */
+    double teff;             /* TEFF (TCORR's cool-star rule)                  */
+} a12_atlas_atm;
+
+/* Loads the state and the line store LINEDATA(4, nlines) (as SELECTLINES
+ * leaves it); unpacks, bins and sorts the lines by grid index. */
+int a12_atlas_load(a12_ctx *ctx, const a12_atlas_atm *atm, const int32_t *linedata,
+                   int32_t nlines);
+/* LINOP1, src/atlas12_modules.f90:18132-18374, into the context's device
+ * XLINES (nrhox, numnu). */
+int a12_atlas_linop1_run(a12_ctx *ctx, int32_t prec);
+/* Copies XLINES (nrhox, numnu), depth fastest, and the contribution count. */
+int a12_atlas_linop1_fetch(a12_ctx *ctx, double *xlines, int64_t *evals);
+/* KAPP's outputs per frequency, each (nrhox, numnu) depth fastest:
+ * ACONT, SIGMAC, SCONT, SLINE, ALINE, SIGMAL (src/atlas12_modules.f90:3918). */
+int a12_atlas_set_kapp(a12_ctx *ctx, const double *acont, const double *sigmac,
+                       const double *scont, const double *sline, const double *aline,
+                       const double *sigmal);
+/* The frequency loop of RUN_RT_PASS (src/atlas12_modules.f90:3905-3949):
+ * JOSH(1, 0) per frequency on KAPP + XLINES*STIM, the negative floors, and
+ * the TCORR/RADIAP/ROSS mode-1 and mode-2 accumulators.  Uses the device
+ * XLINES of the last linop1 run, or xlines (nrhox, numnu) when given. */
+int a12_atlas_rt_run(a12_ctx *ctx, const double *xlines);
+/* acc (9, nrhox): RJMINS, RDABH, RDIAGJ, FLXRAD, RADEN, H_TOTAL, ACCRAD,
+ * ABROSS sums, then PRADK0 in acc[8*nrhox]; hnu1 (numnu): HNU(1) per
+ * frequency after the floors. */
+int a12_atlas_rt_fetch(a12_ctx *ctx, double *acc, double *hnu1);
+
 /* ---- measurement --------------------------------------------------------- */
 /* Stage timing with HIP events on the context stream (off by default). */
 int a12_set_profiling(a12_ctx *ctx, int32_t on);

@@ -411,7 +411,42 @@ def golden_atlas_small():
                             "nwave1": d["nwave1"]}}
 
 
-STEPS = {"synthe_small": golden_synthe_small, "atlas_small": golden_atlas_small, "state_plain": golden_state_plain,
+def golden_state_atlas():
+    """ATLAS12 state of the one-iteration workload on the plain-SYNTHE
+    atmosphere: the driver's 30,000-point grid for Teff < 7250 K (NUSTART =
+    9599, 91.2 nm - 91.2 um), TABCONT, populations and Doppler widths of the
+    atomic and diatomic species the line store uses, and KAPP's outputs at
+    every 100th frequency (a full KAPP table is 104 MB; the benchmark fills
+    the frequencies between by log-log interpolation)."""
+    atm = wl.scurve_atmosphere()
+    nustart, nuhi = 9599, 30000
+    lines = wl.atlas_linedata(1, 500.0, 500.1, seed=1)
+    d = atlas_case(atm, nustart, nuhi, lines)
+    n = d["nrhox"]
+    # diatomic slots (841..1006) the equilibrium populates most at the
+    # photosphere, for the molecular share of the line store
+    xd = d["xnfdop"]
+    mol = np.arange(840, 1006)
+    score = xd[mol, n // 2]
+    mol_used = mol[np.argsort(score)[::-1][:6]] + 1
+    atoms = np.array(wl.ATLAS_SPECIES)
+    used = np.unique(np.concatenate([atoms, mol_used]))
+    sub = np.arange(0, nuhi, 100)
+    keep = dict(nustart=nustart, teff=d["teff"], flux=d["flux"], mion=d["mion"],
+                waveset=d["waveset"], rcoset=d["rcoset"], tabcont=d["tabcont"],
+                iwavetab=d["iwavetab"], h0=d["h0"], h1=d["h1"], h2=d["h2"],
+                ion_idx=(used - 1).astype(np.int32), xnfdop=xd[used - 1], dopple=d["dopple"][used - 1],
+                mol_nelion=mol_used.astype(np.int32), kapp_sub=sub.astype(np.int32))
+    for k in ("t", "hkt", "hckt", "xne", "rhox", "xnf1", "xnf3", "xnf841"):
+        keep[k] = d[k]
+    for k in ("acont", "sigmac", "scont", "sline", "aline", "sigmal"):
+        keep[k + "_sub"] = d[k][sub]
+    np.savez_compressed(os.path.join(HERE, "atlas_plain_state.npz"), **keep)
+    return {"atlas_plain_state": {"numnu": nuhi, "molecules": mol_used.tolist()}}
+
+
+STEPS = {"synthe_small": golden_synthe_small, "atlas_small": golden_atlas_small,
+         "state_atlas": golden_state_atlas, "state_plain": golden_state_plain,
          "numerics": golden_numerics, "josh": golden_josh}
 
 
