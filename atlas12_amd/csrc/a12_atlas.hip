@@ -77,6 +77,43 @@ __device__ __forceinline__ R lo_cv(R center, R v, R adamp, const R *h0, const R 
     return center * ((h2[iv - 1] * adamp + h1[iv - 1]) * adamp + h0[iv - 1]);
 }
 
+// The same wing value with the arithmetic in FP32 but every table-row and
+// regime decision taken from the FP64 offset v64: a row index computed from
+// a rounded v flips to the neighbouring row near integer 200 v + 1.5, a
+// step of ~0.5 % in that contribution.
+__device__ __forceinline__ float lo_cv_mixed(float center, double v64, float adamp, const float *h0,
+                                             const float *h1, const float *h2)
+{
+#pragma clang fp contract(off)
+    const float v = (float)v64;
+    if (adamp > 0.20f) {
+        int iv = (int)(v64 * 200.0 + 1.5);
+        iv = iv < 1 ? 1 : (iv > kTabN ? kTabN : iv);
+        const float H0 = h0[iv - 1], H1 = h1[iv - 1], H2 = h2[iv - 1];
+        const float a = adamp;
+        if (a <= 1.4f && (double)a + v64 <= 3.2) {
+            const float vv = v * v;
+            const float hh1 = H1 + H0 * 1.12838f;
+            const float hh2 = H2 + hh1 * 1.12838f - H0;
+            const float hh3 = (1.0f - H2) * 0.37613f - hh1 * 0.66667f * vv + hh2 * 1.12838f;
+            const float hh4 = (3.0f * hh3 - hh1) * 0.37613f + H0 * 0.66667f * vv * vv;
+            return center * (((((hh4 * a + hh3) * a + hh2) * a + hh1) * a + H0) *
+                             (((-.122727278f * a + .532770573f) * a - .96284325f) * a + .979895032f));
+        }
+        const float aa = a * a, vv = v * v;
+        const float u = (aa + vv) * 1.4142f;
+        float r = a * 0.79788f / u;
+        if (a <= 100.0f) {
+            const float aau = aa / u, vvu = vv / u, uu = u * u;
+            r = ((((aau - 10.0f * vvu) * aau * 3.0f + 15.0f * vvu * vvu) + 3.0f * vv - aa) / uu + 1.0f) * r;
+        }
+        return center * r;
+    }
+    if (v64 > 10.0) return center * 0.5641895835477563f * adamp / (v * v);
+    const int iv = (int)(v64 * 200.0 + 1.5);
+    return center * ((h2[iv - 1] * adamp + h1[iv - 1]) * adamp + h0[iv - 1]);
+}
+
 // Per line: all depths.  o_state[l*n + j]: bit 31 computed, bits 0-7 red
 // reach, 8-15 blue reach (contributions to points nu.. and nu-1..).
 __global__ void k_lo_setup(int nl, int n, int numnu, int mion, const int *__restrict__ l_nu,
@@ -229,8 +266,8 @@ __global__ __launch_bounds__(128) void k_lo_gather(
                 const double v = (red ? (w - wl) : (wl - w)) / par[3 * o + 2];
                 acc64 = acc64 + lo_cv<double>(par[3 * o], v, par[3 * o + 1], h0, h1, h2);
             } else {
-                const float v = (float)((red ? (w - wl) : (wl - w)) / par[3 * o + 2]);
-                acc32.add(lo_cv<float>((float)par[3 * o], v, (float)par[3 * o + 1], f0, f1, f2));
+                const double v = (red ? (w - wl) : (wl - w)) / par[3 * o + 2];
+                acc32.add(lo_cv_mixed((float)par[3 * o], v, (float)par[3 * o + 1], f0, f1, f2));
             }
         }
     }

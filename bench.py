@@ -42,7 +42,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-depths", type=int, default=24)
     ap.add_argument("--profile-only", action="store_true",
-                    help="a few mixed steps only (for rocprofv3 runs)")
+                    help="warmup+steps mixed builds, one FP64 build, one transfer and a "
+                         "device copy of --calib-mib MiB (for rocprofv3 runs)")
+    ap.add_argument("--calib-mib", type=int, default=256,
+                    help="size of the plain device-to-device copy used to scale HBM counters")
     return ap.parse_args()
 
 
@@ -120,7 +123,16 @@ def main():
 
     if args.profile_only:
         run(native.PREC_MIXED, args.warmup + args.steps)
-        log("profile-only run done")
+        run(native.PREC_FP64, 1)
+        ctx.synthe_transfer_run()
+        ctx.synthe_transfer_fetch()
+        # known-size copy: FETCH_SIZE/WRITE_SIZE of this kernel calibrate the counters
+        nel = args.calib_mib * (1 << 20) // 4
+        src = torch.ones(nel, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"profile-only run done (calibration copy {args.calib_mib} MiB)")
         return
 
     log("warmup")
