@@ -293,3 +293,66 @@ def atlas_linedata(n: int, wl0: float, wl1: float, seed: int, species=ATLAS_SPEC
     rec[:, 1:] = pairs.view(np.int32).reshape(n, 3)
     order = np.argsort(rec[:, 0], kind="stable")
     return np.ascontiguousarray(rec[order])
+
+
+# MOL2 32-byte molecular record, the one binary molecular list SELECTLINES
+# reads (src/atlas12_modules.f90:19540-19647): vacuum wavelength [nm], log gf,
+# the two term energies [cm^-1], molecule code, isotope, 100 log10(gamma_rad)
+# (0 = classical), upper-label flag.  Header: 8 x int32, magic 'MOL2', count.
+MOL2_DTYPE = np.dtype([("wl", "<f4"), ("loggf", "<f4"), ("e1", "<f4"), ("e2", "<f4"),
+                       ("code", "<i4"), ("iso", "<i4"), ("gr100", "<i4"), ("flag", "<i4")])
+MOL2_MAGIC = 0x4D4F4C32
+TIO_CODE = 822
+TIO_NELION = 895
+
+
+def tio_mol2_records(n: int, wl0: float, wl1: float, seed: int,
+                     loggf=(-3.0, 1.5), elo_ev=2.0) -> np.ndarray:
+    """Seeded TiO (code 822, isotopes 46-50) MOL2 records, ascending in
+    wavelength; log gf ~ N(-3, 1.5), lower term energy ~ Exp(2 eV), the upper
+    term 1e7/lambda above it, one record in four with classical damping."""
+    rng = np.random.default_rng(seed)
+    wl = np.sort(np.exp(rng.uniform(math.log(wl0), math.log(wl1), n))).astype(np.float32)
+    rec = np.zeros(n, MOL2_DTYPE)
+    rec["wl"] = wl
+    rec["loggf"] = np.round(rng.normal(loggf[0], loggf[1], n), 3)
+    e_lo = np.round(rng.exponential(elo_ev, n) * EV_TO_CM, 3)
+    e_up = np.round(e_lo + 1.0e7 / wl.astype(np.float64), 3)
+    swap = rng.random(n) < 0.5          # the list does not order the two terms
+    rec["e1"] = np.where(swap, e_up, e_lo)
+    rec["e2"] = np.where(swap, e_lo, e_up)
+    rec["code"] = TIO_CODE
+    rec["iso"] = rng.integers(46, 51, n)
+    gr = np.rint(rng.uniform(7.0, 9.0, n) * 100.0).astype(np.int32)
+    rec["gr100"] = np.where(rng.random(n) < 0.25, 0, gr)
+    return rec
+
+
+def write_mol2(rec: np.ndarray, path: str) -> None:
+    hdr = np.zeros(8, "<i4")
+    hdr[0] = MOL2_MAGIC
+    hdr[1] = len(rec)
+    with open(path, "wb") as fh:
+        fh.write(hdr.tobytes())
+        fh.write(np.ascontiguousarray(rec, MOL2_DTYPE).tobytes())
+
+
+def atlas_iteration_lists(n: int, wl0: float, wl1: float, seed: int, molecules=(),
+                          molecular_share: float = 0.2, tio_loggf=(-3.0, 1.5)):
+    """Raw line lists of one ATLAS12 iteration: (predicted 16-byte records,
+    TiO MOL2 records).  (1 - share) of n are atomic lines over ATLAS_SPECIES;
+    of the molecular share, half are diatomics in the predicted format over
+    the given molecule slots and half TiO MOL2 records.  log gf ~ N(-3, 1.5),
+    E_lo ~ Exp(2 eV).  Both lists ascending in wavelength, as SELECTLINES
+    requires."""
+    n_mol = int(round(n * molecular_share))
+    n_tio = n_mol // 2 if molecules else n_mol
+    n_dia = n_mol - n_tio
+    n_atom = n - n_mol
+    pred = atlas_linedata(n_atom, wl0, wl1, seed)
+    if n_dia:
+        dia = atlas_linedata(n_dia, wl0, wl1, seed + 1000, species=tuple(molecules))
+        pred = np.concatenate([pred, dia])
+        pred = np.ascontiguousarray(pred[np.argsort(pred[:, 0], kind="stable")])
+    tio = tio_mol2_records(n_tio, wl0, wl1, seed + 2000, loggf=tio_loggf)
+    return pred, tio

@@ -297,3 +297,106 @@ void or_rt_pass(const or_rt_in *in, or_rt_acc *acc, double *hnu1_out)
         }
     }
 }
+
+
+/* ------------------------------------------------------------------ */
+/* SELECTLINES (src/atlas12_modules.f90:19171-19781), paths 1 and 5.   */
+/* ------------------------------------------------------------------ */
+
+/* XNFDOPMAX(NELION, NU) = max_J XNFDOP(J, NELION) / TABCONT(J, NU), from
+ * 0 (:19247-19255); stored [nel][nu]. */
+static double *select_xnfdopmax(const or_select_in *in)
+{
+    const int nw1 = in->nwave + 1;
+    double *m = (double *)calloc((size_t)in->mion * nw1, sizeof(double));
+    for (int nu = 0; nu < nw1; ++nu)
+        for (int nel = 0; nel < in->mion; ++nel) {
+            double v = 0.0;
+            for (int j = 0; j < in->nrhox; ++j)
+                v = fmax(v, in->xnfdop[(size_t)nel * in->ldion + j] / in->tabcont[(size_t)nu * in->nrhox + j]);
+            m[(size_t)nel * nw1 + nu] = v;
+        }
+    return m;
+}
+
+static int32_t pack_pair(int16_t lo, int16_t hi)
+{
+    return (int32_t)((uint32_t)(uint16_t)lo | ((uint32_t)(uint16_t)hi << 16));
+}
+
+int64_t or_atlas_select(const or_select_in *in, const int32_t *pred, int64_t npred,
+                        const or_mol2_rec *mol2, int64_t nmol2, int32_t *out)
+{
+    static const int TIO_ISOCORR[5] = {-1101, -1138, -131, -1259, -1272};   /* :19204 */
+    const double cen_prefac = 0.026538 / C_SQRTPI;                        /* :19179 */
+    const double ratiolg = log(1.0 + 1.0 / 2000000.0);
+    const int nw1 = in->nwave + 1;
+    const double hcktn = in->hckt[in->nrhox - 1];
+    double *xm = select_xnfdopmax(in);
+    int64_t nk = 0;
+    double freq = 0.0;
+
+    /* (1) predicted lines, records kept as read (:19262-19313) */
+    int nu = 0;
+    for (int64_t l = 0; l < npred; ++l) {
+        const int32_t *r = pred + 4 * l;
+        const int32_t iwl = r[0];
+        const int16_t ielion = (int16_t)(r[1] & 0xffff), ielo = (int16_t)((uint32_t)r[1] >> 16);
+        const int16_t igflog = (int16_t)(r[2] & 0xffff);
+        while (iwl >= in->iwavetab[nu]) {
+            freq = C_CLIGHT_NMS / in->wavetab[nu];
+            ++nu;
+        }
+        const int nelion = abs(ielion / 10);
+        if (nelion < 1 || nelion > in->mion) continue;
+        const double x = xm[(size_t)(nelion - 1) * nw1 + nu];
+        if (x <= 1.0e-37) continue;
+        const double cen = cen_prefac * tablog(igflog) * x / freq;
+        if (cen < 1.0) continue;
+        const double tablog8 = tablog(ielo);
+        if (cen * exp(-tablog8 * hcktn) < 1.0) continue;
+        memcpy(out + 4 * nk, r, 16);
+        ++nk;
+    }
+
+    /* (5) TiO MOL2 records, repacked (:19557-19647); none above 8000 K */
+    if (in->teff <= 8000.0) {
+        nu = 0;
+        for (int64_t l = 0; l < nmol2; ++l) {
+            const or_mol2_rec *m = mol2 + l;
+            if (m->code != 822) continue;
+            const int iso = m->iso - 45;
+            if (iso < 1 || iso > 5) continue;
+            const double wlv = (double)m->wl;
+            if (wlv <= 0.0) continue;
+            const int32_t iwl = (int32_t)(log(wlv) / ratiolg + 0.5);
+            while (iwl >= in->iwavetab[nu]) {
+                freq = C_CLIGHT_NMS / in->wavetab[nu];
+                ++nu;
+            }
+            const int kgflog = (int)lround((double)m->loggf * 1000.0) + 16384;
+            const int16_t igflog = (int16_t)(kgflog + TIO_ISOCORR[iso - 1] > 1 ? kgflog + TIO_ISOCORR[iso - 1] : 1);
+            const int nelion = 895;
+            if (nelion > in->mion) { free(xm); return -1; }
+            const double x = xm[(size_t)(nelion - 1) * nw1 + nu];
+            if (x == 0.0) continue;
+            const double cen = cen_prefac * tablog(igflog) * x / freq;
+            if (cen < 1.0) continue;
+            const double elotio = fmin(fabs((double)m->e1), fabs((double)m->e2));
+            if (cen * exp(-elotio * hcktn) < 1.0) continue;
+            const int16_t ielion = (int16_t)(8949 + iso);
+            const int16_t ielo = (int16_t)(int)(log10(fmax(elotio, 1.0e-10)) * 1000.0 + 16384.5);
+            const double gr = m->gr100 == 0 ? 2.223e13 / (wlv * wlv)
+                                            : pow(10.0, (double)m->gr100 * 0.01);
+            const int16_t igr = (int16_t)(int)(log10(gr) * 1000.0 + 16384.5);
+            int32_t *o = out + 4 * nk;
+            o[0] = iwl;
+            o[1] = pack_pair(ielion, ielo);
+            o[2] = pack_pair(igflog, igr);
+            o[3] = pack_pair(1, 9384);
+            ++nk;
+        }
+    }
+    free(xm);
+    return nk;
+}

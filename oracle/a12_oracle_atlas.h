@@ -48,6 +48,30 @@ typedef struct {
 
 void or_rt_pass(const or_rt_in *in, or_rt_acc *acc, double *hnu1_out);
 
+/* SELECTLINES' filter (src/atlas12_modules.f90:19171-19781) on the
+ * predicted 16-byte records (path 1) followed by the TiO MOL2 records
+ * (path 5).  tabcont: (nrhox, nwave + 1), J fastest; xnfdop: (ldion, mion). */
+typedef struct {
+    int nrhox, mion, ldion, nwave;
+    double teff;
+    const double *tabcont;
+    const int32_t *iwavetab;   /* (nwave + 1), sentinel last */
+    const double *wavetab;     /* (nwave) [nm] */
+    const double *xnfdop;
+    const double *hckt;        /* (nrhox) */
+} or_select_in;
+
+/* MOL2 record (32 bytes). */
+typedef struct {
+    float wl, loggf, e1, e2;
+    int32_t code, iso, gr100, flag;
+} or_mol2_rec;
+
+/* Writes the kept records (4 x int32 each) to out in store order; returns
+ * their number, or -1 for a NELION outside 1..mion in the TiO path. */
+int64_t or_atlas_select(const or_select_in *in, const int32_t *pred, int64_t npred,
+                        const or_mol2_rec *mol2, int64_t nmol2, int32_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -332,3 +332,40 @@ def rt_pass(z, xlines, coefj, coefh):
                                                       "raden", "h_total", "accrad", "abross")}
     out["pradk0"] = acc.pradk0
     return out, h1
+
+
+class OrSelectIn(C.Structure):
+    _fields_ = [("nrhox", C.c_int), ("mion", C.c_int), ("ldion", C.c_int), ("nwave", C.c_int),
+                ("teff", C.c_double), ("tabcont", P_D), ("iwavetab", C.c_void_p),
+                ("wavetab", P_D), ("xnfdop", P_D), ("hckt", P_D)]
+
+
+def atlas_select(z, pred, mol2):
+    """SELECTLINES' filter on predicted records (n, 4) int32 and MOL2
+    records (atlas12_amd.workloads.MOL2_DTYPE) with the state in z (tabcont,
+    iwavetab, wavetab, xnfdop rows ion_idx, hckt, teff, mion).  Returns the
+    kept store (m, 4) int32."""
+    n = int(len(z["hckt"]))
+    mion = int(z["mion"])
+    k = _Keep()
+    k.a = {"tabcont": _f64(z["tabcont"]), "wavetab": _f64(z["wavetab"]),
+           "xnfdop": _f64(full_ion(z["xnfdop"], z["ion_idx"], mion, n)), "hckt": _f64(z["hckt"])}
+    iw = np.ascontiguousarray(z["iwavetab"], np.int32)
+    pr = np.ascontiguousarray(pred, np.int32)
+    mo = np.ascontiguousarray(mol2)
+    assert mo.dtype.itemsize == 32
+    s = OrSelectIn()
+    s.nrhox, s.mion, s.ldion, s.nwave = n, mion, n, len(z["wavetab"])
+    s.teff = float(z["teff"])
+    s.iwavetab = iw.ctypes.data_as(C.c_void_p)
+    for kk, v in k.a.items():
+        setattr(s, kk, _p(v))
+    out = np.zeros((len(pr) + len(mo), 4), np.int32)
+    L = lib()
+    L.or_atlas_select.restype = C.c_int64
+    m = L.or_atlas_select(C.byref(s), pr.ctypes.data_as(C.c_void_p), C.c_int64(len(pr)),
+                          mo.ctypes.data_as(C.c_void_p), C.c_int64(len(mo)),
+                          out.ctypes.data_as(C.c_void_p))
+    if m < 0:
+        raise ValueError("or_atlas_select: NELION outside 1..mion")
+    return out[:m]

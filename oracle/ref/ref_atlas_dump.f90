@@ -7,13 +7,17 @@
 !   * RUN_RT_PASS's frequency-integrated accumulators.
 ! Test infrastructure (built by oracle/ref/Makefile).
 !
-!   ATLAS12=<root> ref_atlas_dump <deck> <nustart> <nuhi> <lines.bin> <out.bin>
+!   ATLAS12=<root> ref_atlas_dump <deck> <nustart> <nuhi> <lines.bin> <out.bin> [<seldir>]
 !
 ! lines.bin: int32 count, then count LINEDATA records (4 x int32).
+! With <seldir>, the line store is instead SELECTLINES' product from the
+! raw lists in <seldir> (gfpred29dec2014.bin, lines.list naming a MOL2
+! TiO list); it is written to <out.bin>.sel as int32 count, the records,
+! then WAVETAB(1:NWAVE), and lines.bin is not read.
 PROGRAM ref_atlas_dump
   USE mod_atlas_data
   IMPLICIT NONE
-  CHARACTER(LEN=512) :: deck, fin, fout, arg, envval
+  CHARACTER(LEN=512) :: deck, fin, fout, arg, envval, seldir
   INTEGER :: envlen, envstat, u, nustart_in, nuhi_in, nl, nu, j, nel, i
   REAL(8) :: freq15_l, xmax_l
 
@@ -22,6 +26,8 @@ PROGRAM ref_atlas_dump
   CALL GET_COMMAND_ARGUMENT(3, arg); READ(arg, *) nuhi_in
   CALL GET_COMMAND_ARGUMENT(4, fin)
   CALL GET_COMMAND_ARGUMENT(5, fout)
+  seldir = ''
+  IF (COMMAND_ARGUMENT_COUNT() .GE. 6) CALL GET_COMMAND_ARGUMENT(6, seldir)
 
   CALL GET_ENVIRONMENT_VARIABLE('ATLAS12', envval, envlen, envstat)
   DATADIR = TRIM(envval) // '/data/'
@@ -94,14 +100,27 @@ PROGRAM ref_atlas_dump
   END DO
   CALL KAPCONT
 
-  ! The line store (SELECTLINES' product) from the caller's file
-  OPEN(NEWUNIT=u, FILE=TRIM(fin), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
-  READ(u) nl
-  IF (ALLOCATED(LINEDATA)) DEALLOCATE(LINEDATA)
-  ALLOCATE(LINEDATA(4, MAX(nl, 1)))
-  READ(u) ((LINEDATA(i, j), i = 1, 4), j = 1, nl)
-  CLOSE(u)
-  NLINES_STORED = nl
+  IF (LEN_TRIM(seldir) .GT. 0) THEN
+    ! The reference's own selection from the raw lists
+    DATADIR = TRIM(seldir) // '/'
+    CALL SELECTLINES
+    nl = NLINES_STORED
+    OPEN(NEWUNIT=u, FILE=TRIM(fout)//'.sel', ACCESS='STREAM', FORM='UNFORMATTED', &
+         STATUS='REPLACE')
+    WRITE(u) nl
+    WRITE(u) ((LINEDATA(i, j), i = 1, 4), j = 1, nl)
+    WRITE(u) WAVETAB(1:NWAVE)
+    CLOSE(u)
+  ELSE
+    ! The line store (SELECTLINES' product) from the caller's file
+    OPEN(NEWUNIT=u, FILE=TRIM(fin), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
+    READ(u) nl
+    IF (ALLOCATED(LINEDATA)) DEALLOCATE(LINEDATA)
+    ALLOCATE(LINEDATA(4, MAX(nl, 1)))
+    READ(u) ((LINEDATA(i, j), i = 1, 4), j = 1, nl)
+    CLOSE(u)
+    NLINES_STORED = nl
+  END IF
 
   OPEN(NEWUNIT=u, FILE=TRIM(fout), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
   WRITE(u) NRHOX, NUMNU, nl, MION, NWAVE

@@ -364,7 +364,10 @@ def read_atlas_dump(path: str) -> dict:
     return d
 
 
-def atlas_case(atm, nustart, nuhi, linedata):
+def atlas_case(atm, nustart, nuhi, linedata, raw=None):
+    """Runs ref_atlas_dump.  raw = (predicted records, TiO MOL2 records):
+    the store is then SELECTLINES' selection from these lists, returned
+    as d["selected"] with d["wavetab"]."""
     tmp = tempfile.mkdtemp(prefix="a12a_")
     g = os.path.join(tmp, "gfall.dat")
     open(g, "w").close()
@@ -376,9 +379,25 @@ def atlas_case(atm, nustart, nuhi, linedata):
     with open(lp, "wb") as fh:
         fh.write(np.int32(len(linedata)).tobytes() + np.ascontiguousarray(linedata).tobytes())
     out = os.path.join(tmp, "dump.bin")
-    run([os.path.join(REFBIN, "ref_atlas_dump"), deck, str(nustart), str(nuhi), lp, out], tmp,
-        dict(os.environ, ATLAS12=root))
+    extra = []
+    if raw is not None:
+        sel = os.path.join(tmp, "sel")
+        os.makedirs(sel)
+        with open(os.path.join(sel, "gfpred29dec2014.bin"), "wb") as fh:
+            fh.write(np.ascontiguousarray(raw[0], "<i4").tobytes())
+        wl.write_mol2(raw[1], os.path.join(sel, "tio.bin"))
+        with open(os.path.join(sel, "lines.list"), "w") as fh:
+            fh.write("mol tio.bin\n")
+        extra = [sel]
+    run([os.path.join(REFBIN, "ref_atlas_dump"), deck, str(nustart), str(nuhi), lp, out, *extra],
+        tmp, dict(os.environ, ATLAS12=root))
     d = read_atlas_dump(out)
+    if raw is not None:
+        with open(out + ".sel", "rb") as fh:
+            buf = fh.read()
+        ns = int(np.frombuffer(buf, "<i4", 1)[0])
+        d["selected"] = np.frombuffer(buf, "<i4", 4 * ns, 4).reshape(ns, 4).copy()
+        d["wavetab"] = np.frombuffer(buf, "<f8", 361, 4 + 16 * ns).copy()
     shutil.rmtree(tmp)
     return d
 
@@ -445,8 +464,35 @@ def golden_state_atlas():
     return {"atlas_plain_state": {"numnu": nuhi, "molecules": mol_used.tolist()}}
 
 
+def golden_select():
+    """SELECTLINES on the plain-SYNTHE atmosphere's first-iteration state:
+    the raw lists of the one-iteration workload's generator (atomic and
+    diatomic predicted records, TiO MOL2 records) over 95 nm - 20 um, and
+    the store the reference keeps.  The raw lists are regenerated from the
+    seed by the tests (their sha256 is recorded to catch generator drift)."""
+    atm = wl.scurve_atmosphere()
+    st = np.load(os.path.join(HERE, "atlas_plain_state.npz"))
+    mols = tuple(int(m) for m in st["mol_nelion"])
+    # TiO gf raised so that the TiO packing path keeps lines at this Teff
+    pred, tio = wl.atlas_iteration_lists(80_000, 95.0, 20_000.0, seed=31, molecules=mols,
+                                         tio_loggf=(4.0, 1.5))
+    d = atlas_case(atm, 9599, 10, wl.atlas_linedata(1, 500.0, 500.1, seed=1), raw=(pred, tio))
+    used = np.unique(np.concatenate([np.array(wl.ATLAS_SPECIES), np.array(mols),
+                                     [wl.TIO_NELION]]))
+    digest = hashlib.sha256(pred.tobytes() + tio.tobytes()).hexdigest()
+    np.savez_compressed(os.path.join(HERE, "atlas_select.npz"), selected=d["selected"],
+                        wavetab=d["wavetab"], tabcont=d["tabcont"], iwavetab=d["iwavetab"],
+                        hckt=d["hckt"], teff=d["teff"], mion=d["mion"],
+                        ion_idx=(used - 1).astype(np.int32), xnfdop=d["xnfdop"][used - 1],
+                        n_raw=np.int64(80_000), seed=np.int64(31), wl_range=np.array([95.0, 20_000.0]),
+                        tio_loggf=np.array([4.0, 1.5]),
+                        mol_nelion=np.array(mols, np.int32), raw_sha256=np.array(digest))
+    return {"atlas_select": {"raw": [len(pred), len(tio)], "selected": len(d["selected"]),
+                             "raw_sha256": digest}}
+
+
 STEPS = {"synthe_small": golden_synthe_small, "atlas_small": golden_atlas_small,
-         "state_atlas": golden_state_atlas, "state_plain": golden_state_plain,
+         "state_atlas": golden_state_atlas, "select": golden_select, "state_plain": golden_state_plain,
          "numerics": golden_numerics, "josh": golden_josh}
 
 

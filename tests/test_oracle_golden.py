@@ -118,3 +118,27 @@ def test_rt_pass_accumulators_bitwise(orc, golden, coef):
     assert h1[-1] == z["last_hnu"][0]
     # H_TOTAL is FLXRAD's twin (the same sum in RADIAP)
     assert np.array_equal(acc["h_total"], acc["flxrad"])
+
+
+def _select_raw(z):
+    import hashlib
+
+    from atlas12_amd import workloads as wl
+    lo, hi = (float(x) for x in z["wl_range"])
+    pred, tio = wl.atlas_iteration_lists(int(z["n_raw"]), lo, hi, seed=int(z["seed"]),
+                                         molecules=tuple(int(m) for m in z["mol_nelion"]),
+                                         tio_loggf=tuple(float(x) for x in z["tio_loggf"]))
+    digest = hashlib.sha256(pred.tobytes() + tio.tobytes()).hexdigest()
+    assert digest == str(z["raw_sha256"]), "line-list generator drifted from the fixture"
+    return pred, tio
+
+
+def test_selectlines_bitwise(orc, golden):
+    """SELECTLINES on 72,000 predicted (atomic + diatomic) and 8,000 TiO
+    MOL2 records keeps exactly the reference's store, record for record,
+    including the TiO repacking (IWL, IELO, IGFLOG + isotope, IGR)."""
+    z = golden("atlas_select.npz")
+    pred, tio = _select_raw(z)
+    sel = orc.atlas_select(z, pred, tio)
+    assert sel.shape == z["selected"].shape
+    assert np.array_equal(sel, z["selected"])
