@@ -369,8 +369,10 @@ int a12i_atlas_load(a12_ctx *ctx, const a12_atlas_atm *a, const int32_t *linedat
         int16_t p[6];
         std::memcpy(p, r + 1, 12);
         const int iwl = r[0];
-        int nucont = 0;
-        while (nucont < a->nwave1 - 1 && iwl >= a->iwavetab[nucont]) nucont++;
+        // first bin with IWL < IWAVETAB, clamped to the last (:18193-18200)
+        const int nucont = std::min(
+            (int)(std::upper_bound(a->iwavetab, a->iwavetab + a->nwave1, iwl) - a->iwavetab),
+            a->nwave1 - 1);
         const int nel = std::abs(p[0] / 10);
         if (nel < 1 || nel > a->mion) continue;
         const double wl = std::exp(iwl * ratiolg);
@@ -439,7 +441,7 @@ int a12i_atlas_linop1_run(a12_ctx *ctx, int prec)
     A12_HIP(A.evals.reserve(sizeof(long long)));
     const double *tabs = A.tabs.as<double>();
     const float *tabs32 = A.tabs32.as<float>();
-    if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[0], s));
+    if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[8], s));
     if (nl > 0) {
         hipLaunchKernelGGL(k_lo_setup, dim3(nb), dim3(256), 0, s, nl, n, numnu, A.mion,
                            A.l_nu.as<int>(), A.l_nucont.as<int>(), A.l_nel.as<int>(),
@@ -454,19 +456,19 @@ int a12i_atlas_linop1_run(a12_ctx *ctx, int prec)
     } else {
         A12_HIP(hipMemsetAsync(A.evals.p, 0, sizeof(long long), s));
     }
-    if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[1], s));
+    if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[9], s));
     const int nseg = numnu + 1;
     hipLaunchKernelGGL(k_lo_reach, dim3((nseg * n + 255) / 256), dim3(256), 0, s, nseg, n,
                        A.seg.as<int>(), A.state.as<int>(), A.reach.as<int>());
     A12_CHECK_LAUNCH();
-    if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[2], s));
+    if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[10], s));
     auto g = prec == A12_PREC_FP64 ? k_lo_gather<true> : k_lo_gather<false>;
     hipLaunchKernelGGL(g, dim3(numnu), dim3(128), 0, s, n, numnu, A.seg.as<int>(), A.reach.as<int>(),
                        A.state.as<int>(), A.par.as<double>(), A.l_const.as<double>(),
                        A.waveset.as<double>(), tabs, tabs + kTabN, tabs + 2 * kTabN, tabs32,
                        tabs32 + kTabN, tabs32 + 2 * kTabN, A.xlines.as<double>());
     A12_CHECK_LAUNCH();
-    if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[3], s));
+    if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[11], s));
     A.xl_ready = true;
     A.prof_lo = ctx->prof;
     return A12_OK;

@@ -233,6 +233,16 @@ int a12_atlas_load(a12_ctx *ctx, const a12_atlas_atm *atm, const int32_t *lineda
     return a12i_atlas_load(ctx, atm, linedata, nlines);
 }
 
+int a12_atlas_select_lines(a12_ctx *ctx, const a12_atlas_atm *atm, const double *wavetab,
+                           const int32_t *pred, int64_t npred, const a12_mol2_rec *mol2,
+                           int64_t nmol2, int32_t *linedata_out, int64_t cap, int64_t *nkept)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_atlas_select(ctx, atm, wavetab, pred, npred, mol2, nmol2, linedata_out, cap,
+                             nkept);
+}
+
 int a12_atlas_linop1_run(a12_ctx *ctx, int32_t prec)
 {
     if (!ctx_ok(ctx)) return A12_ERR_ARG;
@@ -312,7 +322,7 @@ int a12_atlas_rt_run(a12_ctx *ctx, const double *xlines)
                                 A.xlines.as<double>(), A.hnu1.as<double>(), A.acc.as<double>());
     if (rc) return rc;
     A.rt_done = true;
-    ctx->syn.transferred = false;
+    A.prof_rt = ctx->prof;
     return A12_OK;
 }
 
@@ -358,6 +368,23 @@ int a12_stage_ms(a12_ctx *ctx, float *ms)
     if (ctx->prof && ctx->syn.transferred) {
         A12_HIP(hipEventSynchronize(ctx->ev[6]));
         A12_HIP(hipEventElapsedTime(&ms[4], ctx->ev[5], ctx->ev[6]));
+    }
+    return A12_OK;
+}
+
+int a12_atlas_stage_ms(a12_ctx *ctx, float *ms)
+{
+    if (!ctx_ok(ctx) || !ms) return A12_ERR_ARG;
+    for (int i = 0; i < 4; ++i) ms[i] = -1.0f;
+    if (ctx->atl.prof_lo && ctx->atl.xl_ready) {
+        A12_HIP(hipEventSynchronize(ctx->ev[11]));
+        A12_HIP(hipEventElapsedTime(&ms[0], ctx->ev[8], ctx->ev[9]));
+        A12_HIP(hipEventElapsedTime(&ms[1], ctx->ev[9], ctx->ev[10]));
+        A12_HIP(hipEventElapsedTime(&ms[2], ctx->ev[10], ctx->ev[11]));
+    }
+    if (ctx->atl.prof_rt && ctx->atl.rt_done) {
+        A12_HIP(hipEventSynchronize(ctx->ev[13]));
+        A12_HIP(hipEventElapsedTime(&ms[3], ctx->ev[12], ctx->ev[13]));
     }
     return A12_OK;
 }

@@ -71,7 +71,12 @@ struct AtlasDev {
     DevBuf state, par, reach, xlines, blk, evals;
     DevBuf kapp, hnu1, acc;
     bool loaded = false, xl_ready = false, kapp_ready = false, rt_done = false;
-    bool prof_lo = false;
+    bool prof_lo = false, prof_rt = false;
+};
+
+struct SelectDev {
+    DevBuf tabcont, xnfdop, iwt, wavetab, tablog, xm, cand, flag, cnt, off, kept, mol2;
+    bool tablog_ready = false;
 };
 
 struct a12_ctx {
@@ -82,12 +87,13 @@ struct a12_ctx {
     bool have_ops = false;
     SyntheDev syn;
     AtlasDev atl;
+    SelectDev sel;
     DevBuf rt_part;
     // scratch for a12_josh_batch
     DevBuf jb_in, jb_out;
     // stage timing (a12_set_profiling): events on the context stream
     bool prof = false;
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[16] = {};  // 0-6 SYNTHE stages, 8-11 LINOP1 stages, 12-13 RT pass
 };
 
 #define A12_HIP(call)                                                              \
@@ -116,6 +122,10 @@ int a12i_voigt(a12_ctx *ctx, int prec, int n, const float *v, const float *a, do
 // Implemented in a12_atlas.hip
 int a12i_atlas_load(a12_ctx *ctx, const a12_atlas_atm *a, const int32_t *linedata, int32_t nlines);
 int a12i_atlas_linop1_run(a12_ctx *ctx, int prec);
+// Implemented in a12_select.hip
+int a12i_atlas_select(a12_ctx *ctx, const a12_atlas_atm *a, const double *wavetab,
+                      const int32_t *pred, int64_t npred, const a12_mol2_rec *mol2, int64_t nmol2,
+                      int32_t *out, int64_t cap, int64_t *nkept);
 // Implemented in a12_josh.hip
 int a12i_atlas_rt_pass(a12_ctx *ctx, int n, int nf, double teff, const double *d_wave,
                        const double *d_rcoset, const double *d_depth3, const double *d_kapp,

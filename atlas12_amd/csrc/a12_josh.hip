@@ -965,7 +965,7 @@ int a12i_atlas_rt_pass(a12_ctx *ctx, int n, int nf, double teff, const double *d
     const int chunk = std::max(1, (nf + 4095) / 4096);
     const int nchunk = (nf + chunk - 1) / chunk;
     A12_HIP(ctx->rt_part.reserve(sizeof(double) * 9 * (size_t)n * nchunk));
-    if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
+    if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[12], ctx->stream));
     hipLaunchKernelGGL(k_rt_pass, dim3(nchunk), dim3(64), 0, ctx->stream, n, nf, chunk, teff,
                        d_wave, d_rcoset, d_depth3, d_kapp, d_xlines, ctx->coefj.as<double>(),
                        ctx->coefh.as<double>(), d_hnu1, ctx->rt_part.as<double>());
@@ -973,6 +973,6 @@ int a12i_atlas_rt_pass(a12_ctx *ctx, int n, int nf, double teff, const double *d
     hipLaunchKernelGGL(k_rt_reduce, dim3((9 * n + 255) / 256), dim3(256), 0, ctx->stream, n,
                        nchunk, ctx->rt_part.as<double>(), d_acc);
     A12_CHECK_LAUNCH();
-    if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[6], ctx->stream));
+    if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[13], ctx->stream));
     return A12_OK;
 }

@@ -31,7 +31,8 @@ EXPORTS = ["a12_ctx_create", "a12_ctx_destroy", "a12_last_error", "a12_stream",
            "a12_synthe_transfer", "a12_josh_batch", "a12_voigt_profile",
            "a12_set_profiling", "a12_stage_ms", "a12_synthe_eval_split",
            "a12_atlas_load", "a12_atlas_linop1_run", "a12_atlas_linop1_fetch",
-           "a12_atlas_set_kapp", "a12_atlas_rt_run", "a12_atlas_rt_fetch"]
+           "a12_atlas_set_kapp", "a12_atlas_rt_run", "a12_atlas_rt_fetch",
+           "a12_atlas_select_lines", "a12_atlas_stage_ms"]
 
 
 class A12Error(RuntimeError):
@@ -337,3 +338,41 @@ def atlas_rt_pass(ctx: Context, xlines=None, fetch: bool = True):
     out = {k: acc[i * n:(i + 1) * n].copy() for i, k in enumerate(ACC_NAMES)}
     out["pradk0"] = float(acc[8 * n])
     return out, h1
+
+
+def atlas_stage_ms(ctx: Context):
+    """LINOP1 set-up, reach, gather and RT-pass times [ms] of the last runs
+    (with profiling on); -1 where not measured."""
+    ms = (C.c_float * 4)()
+    ctx._check(ctx.lib.a12_atlas_stage_ms(ctx.h, ms), "a12_atlas_stage_ms")
+    return np.array(ms[:], np.float64)
+
+
+def atlas_select_lines(ctx: Context, z, pred, mol2, mion: int = 1006):
+    """SELECTLINES' filter (src/atlas12_modules.f90:19171-19781) on the GPU:
+    predicted records (n, 4) int32 then MOL2 TiO records.  z holds tabcont,
+    iwavetab, wavetab, hckt, teff and the xnfdop rows ion_idx.  Returns the
+    kept LINEDATA (m, 4) int32."""
+    n = len(z["hckt"])
+    xnfdop = np.zeros((mion, n))
+    xnfdop[z["ion_idx"]] = z["xnfdop"]
+    keep = {"tabcont": _f64(z["tabcont"]), "xnfdop": xnfdop, "hckt": _f64(z["hckt"]),
+            "wavetab": _f64(z["wavetab"])}
+    iw = np.ascontiguousarray(z["iwavetab"], np.int32)
+    a = AtlasAtm()
+    a.nrhox, a.numnu, a.mion, a.ldion, a.ldtab, a.nwave1 = n, 0, mion, n, n, len(iw)
+    a.tabcont, a.xnfdop, a.hckt = _ptr(keep["tabcont"]), _ptr(keep["xnfdop"]), _ptr(keep["hckt"])
+    a.iwavetab = iw.ctypes.data_as(C.c_void_p)
+    a.teff = float(z["teff"])
+    pr = np.ascontiguousarray(pred, np.int32)
+    mo = np.ascontiguousarray(mol2)
+    if mo.dtype.itemsize != 32:
+        raise ValueError("MOL2 records must be 32 bytes")
+    out = np.zeros((len(pr) + len(mo), 4), np.int32)
+    nk = C.c_int64(0)
+    ctx._check(ctx.lib.a12_atlas_select_lines(
+        ctx.h, C.byref(a), _ptr(keep["wavetab"]), pr.ctypes.data_as(C.c_void_p),
+        C.c_int64(len(pr)), mo.ctypes.data_as(C.c_void_p), C.c_int64(len(mo)),
+        out.ctypes.data_as(C.c_void_p), C.c_int64(len(out)), C.byref(nk)),
+        "a12_atlas_select_lines")
+    return out[:nk.value]

@@ -186,6 +186,32 @@ int a12_atlas_rt_run(a12_ctx *ctx, const double *xlines);
  * frequency after the floors. */
 int a12_atlas_rt_fetch(a12_ctx *ctx, double *acc, double *hnu1);
 
+/* MOL2 32-byte molecular record (header: int32 magic 'MOL2', int32 count,
+ * 6 more int32), src/atlas12_modules.f90:19540-19556. */
+typedef struct {
+    float   wl;       /* vacuum wavelength [nm]                         */
+    float   loggf;    /* log gf, bare (isotope correction not applied)  */
+    float   e1, e2;   /* the two term energies [cm^-1]                  */
+    int32_t code;     /* molecule code (822 = TiO)                      */
+    int32_t iso;      /* isotope (46..50 for Ti)                        */
+    int32_t gr100;    /* 100 log10(gamma_rad), 0 = classical            */
+    int32_t flag;     /* upper-label flag (unused)                      */
+} a12_mol2_rec;
+
+/* SELECTLINES' filter, src/atlas12_modules.f90:19171-19781: path 1 (the
+ * predicted 16-byte list, :19262-19313, records kept as read) then path 5
+ * (the TiO MOL2 list, :19557-19647, skipped above TEFF 8000 K, records
+ * repacked).  atm supplies TABCONT, IWAVETAB, XNFDOP, HCKT, TEFF (waveset,
+ * rcoset, dopple and the depth arrays other than hckt are not read);
+ * wavetab is WAVETAB (nwave1 - 1) [nm].  Both lists must ascend in
+ * wavelength and lie within the continuum bins (A12_ERR_ARG otherwise).
+ * Writes the kept store LINEDATA(4, *nkept) to linedata_out; if *nkept
+ * exceeds cap nothing is written and A12_ERR_ARG is returned with the
+ * count in *nkept.  Host pointers, synchronous. */
+int a12_atlas_select_lines(a12_ctx *ctx, const a12_atlas_atm *atm, const double *wavetab,
+                           const int32_t *pred, int64_t npred, const a12_mol2_rec *mol2,
+                           int64_t nmol2, int32_t *linedata_out, int64_t cap, int64_t *nkept);
+
 /* ---- measurement --------------------------------------------------------- */
 /* Stage timing with HIP events on the context stream (off by default). */
 int a12_set_profiling(a12_ctx *ctx, int32_t on);
@@ -193,6 +219,9 @@ int a12_set_profiling(a12_ctx *ctx, int32_t on);
  * gather (of the last opacity run), SYNTHE transfer (of the last transfer
  * run); -1 where not measured.  Synchronises on the events. */
 int a12_stage_ms(a12_ctx *ctx, float *ms);
+/* ms[0..3] of the last ATLAS12 runs: LINOP1 line set-up, segment reach,
+ * gather, and the RUN_RT_PASS frequency loop; -1 where not measured. */
+int a12_atlas_stage_ms(a12_ctx *ctx, float *ms);
 /* Contributions of the last opacity run: all, and those on the Voigt core
  * (centre included); the difference is the n^-2 far-wing terms. */
 int a12_synthe_eval_split(a12_ctx *ctx, int64_t *total, int64_t *core);

@@ -6,6 +6,7 @@
  * accumulators.  Citations are cconroy20/atlas12 file:line.
  */
 #include <math.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -100,9 +101,13 @@ static double linop1_cv(double center, double v, double adamp, const or_atlas_li
     return center * ((in->h2[iv - 1] * adamp + in->h1[iv - 1]) * adamp + in->h0[iv - 1]);
 }
 
-/* LINOP1, src/atlas12_modules.f90:18132-18374.  xlines (nrhox, numnu),
- * depth fastest, FP64.  evals: accumulated contributions. */
-void or_linop1(const or_atlas_lines_in *in, double *xlines, int64_t *evals)
+/* LINOP1 (src/atlas12_modules.f90:18132-18374) over lines [l0, l1),
+ * adding into xlines (nrhox, numnu), depth fastest.  The continuum bin and
+ * grid index are walked forward from the previous line and restart when
+ * IWL decreases (:18186-18227), so each line gets the first bin with
+ * IWL < IWAVETAB and the first grid point with WAVESET > WLVAC. */
+void or_linop1_range(const or_atlas_lines_in *in, int64_t l0, int64_t l1, double *xlines,
+                     int64_t *evals)
 {
     const int n = in->nrhox, numnu = in->numnu, nr = in->ldion;
     const double ratiolg = log(1.0 + 1.0 / 2000000.0);
@@ -111,26 +116,27 @@ void or_linop1(const or_atlas_lines_in *in, double *xlines, int64_t *evals)
     const double start = in->waveset[0] - 1.0, stop = in->waveset[numnu - 1] + 1.0;
     double txnxn[KW_MAX];
     int64_t ev = 0;
-    memset(xlines, 0, sizeof(double) * (size_t)n * numnu);
     for (int j = 0; j < n; ++j)
         txnxn[j] = (in->xnf1[j] + 0.42 * in->xnf3[j] + 0.85 * in->xnf841[j]) *
                    pow(in->t[j] / 10000.0, 0.3);
-    for (int line = 0; line < in->nlines; ++line) {
+    int nucont = 0, nu = 0;
+    int32_t iwlold = INT32_MIN;
+    for (int64_t line = l0; line < l1; ++line) {
         const int32_t *rec = in->linedata + 4 * (size_t)line;
         const int32_t iwl = rec[0];
         int16_t p[6];
         memcpy(p, rec + 1, 12);
         const int16_t ielion = p[0], ielo = p[1], igflog = p[2], igr = p[3], igs = p[4],
                       igw = p[5];
+        if (iwl < iwlold) nucont = nu = 0;
+        iwlold = iwl;
         /* continuum bin: first k with IWL < IWAVETAB(k) (sentinel at the end) */
-        int nucont = 0;
         while (nucont < in->nwave1 - 1 && iwl >= in->iwavetab[nucont]) nucont++;
         const int nelion = abs(ielion / 10);
         if (nelion < 1 || nelion > in->mion) continue;
         const double wlvac = exp(iwl * ratiolg);
         if (wlvac < start || wlvac > stop) continue;
         /* first grid point with WAVESET(NU) > WLVAC (NUMNU+1 past the end) */
-        int nu = 0;
         while (nu < numnu && wlvac >= in->waveset[nu]) nu++;
         const double cgf = cen_prefac4 * wlvac * tablog(igflog);
         const double elo = tablog(ielo);
@@ -188,6 +194,16 @@ void or_linop1(const or_atlas_lines_in *in, double *xlines, int64_t *evals)
             }
         }
     }
+    *evals = ev;
+}
+
+/* LINOP1, src/atlas12_modules.f90:18132-18374.  xlines (nrhox, numnu),
+ * depth fastest, FP64.  evals: accumulated contributions. */
+void or_linop1(const or_atlas_lines_in *in, double *xlines, int64_t *evals)
+{
+    int64_t ev = 0;
+    memset(xlines, 0, sizeof(double) * (size_t)in->nrhox * in->numnu);
+    or_linop1_range(in, 0, in->nlines, xlines, &ev);
     if (evals) *evals = ev;
 }
 

@@ -28,6 +28,13 @@ void or_tabvoigt(double vsteps, int n, double *h0, double *h1, double *h2);
 double or_voigt_kurucz(double v, double a, const double *h0, const double *h1,
                        const double *h2);
 void or_linop1(const or_atlas_lines_in *in, double *xlines, int64_t *evals);
+/* LINOP1 over lines [l0, l1), adding into xlines (not cleared). */
+void or_linop1_range(const or_atlas_lines_in *in, int64_t l0, int64_t l1, double *xlines,
+                     int64_t *evals);
+/* LINOP1 with the lines split into contiguous chunks over OpenMP threads,
+ * each into its own XLINES, summed at the end (a timing port: the sum
+ * order differs from the reference's). */
+void or_linop1_omp(const or_atlas_lines_in *in, double *xlines, int64_t *evals);
 
 /* Frequency loop of RUN_RT_PASS with tabulated KAPP outputs. */
 typedef struct {

@@ -5,9 +5,11 @@
  * are identical to the single-threaded routines.
  */
 #include "a12_oracle.h"
+#include "a12_oracle_atlas.h"
 
 #include <omp.h>
 #include <stdlib.h>
+#include <string.h>
 
 int or_omp_threads(int n)
 {
@@ -54,5 +56,26 @@ void or_synthe_line_opacity_omp_list(const or_synthe_atm *atm, const or_line_t *
                                      &ev, NULL);
         tot += ev;
     }
+    if (evals) *evals = tot;
+}
+
+void or_linop1_omp(const or_atlas_lines_in *in, double *xlines, int64_t *evals)
+{
+    const size_t plane = (size_t)in->nrhox * in->numnu;
+    const int nt = omp_get_max_threads();
+    double *part = (double *)calloc(plane * (size_t)nt, sizeof(double));
+    int64_t tot = 0;
+#pragma omp parallel reduction(+ : tot)
+    {
+        const int t = omp_get_thread_num(), T = omp_get_num_threads();
+        const int64_t l0 = in->nlines * (int64_t)t / T, l1 = in->nlines * (int64_t)(t + 1) / T;
+        int64_t ev = 0;
+        or_linop1_range(in, l0, l1, part + plane * t, &ev);
+        tot += ev;
+    }
+    memset(xlines, 0, sizeof(double) * plane);
+    for (int t = 0; t < nt; ++t)
+        for (size_t i = 0; i < plane; ++i) xlines[i] += part[plane * t + i];
+    free(part);
     if (evals) *evals = tot;
 }

@@ -282,9 +282,11 @@ def full_ion(sub, idx, mion, n):
     return out
 
 
-def linop1(z):
+def linop1(z, threads: int = 1):
     """LINOP1 on an ATLAS state dict (as in tests/golden/atlas_small.npz).
-    Returns (xlines (numnu, nrhox), evals)."""
+    Returns (xlines (numnu, nrhox), evals).  threads > 1: the OpenMP
+    timing port (lines in contiguous chunks per thread; different sum
+    order)."""
     n = int(len(z["t"]))
     numnu = int(len(z["waveset"]))
     mion = int(z["mion"])
@@ -305,7 +307,10 @@ def linop1(z):
         setattr(s, kk, _p(v))
     out = np.zeros((numnu, n))
     ev = C.c_int64(0)
-    lib().or_linop1(C.byref(s), _p(out), C.byref(ev))
+    if threads > 1:
+        lib().or_linop1_omp(C.byref(s), _p(out), C.byref(ev))
+    else:
+        lib().or_linop1(C.byref(s), _p(out), C.byref(ev))
     return out, ev.value
 
 

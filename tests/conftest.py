@@ -48,6 +48,23 @@ def orc():
 
 
 @pytest.fixture(scope="session")
+def select_case():
+    """tests/golden/atlas_select.npz with its raw line lists regenerated
+    from the recorded seed: (state, predicted records, MOL2 records)."""
+    import hashlib
+
+    from atlas12_amd import workloads as wl
+    z = np.load(os.path.join(GOLDEN, "atlas_select.npz"))
+    lo, hi = (float(x) for x in z["wl_range"])
+    pred, tio = wl.atlas_iteration_lists(int(z["n_raw"]), lo, hi, seed=int(z["seed"]),
+                                         molecules=tuple(int(m) for m in z["mol_nelion"]),
+                                         tio_loggf=tuple(float(x) for x in z["tio_loggf"]))
+    digest = hashlib.sha256(pred.tobytes() + tio.tobytes()).hexdigest()
+    assert digest == str(z["raw_sha256"]), "line-list generator drifted from the fixture"
+    return z, pred, tio
+
+
+@pytest.fixture(scope="session")
 def gpu_ctx():
     """A device context on cuda:0 -- fails (does not skip) when the HIP
     library is missing, so a GPU run can never pass on a fallback."""

@@ -376,8 +376,9 @@ def atlas_case(atm, nustart, nuhi, linedata, raw=None):
     with open(deck, "w") as fh:
         fh.write(atm.deck())
     lp = os.path.join(tmp, "lines.bin")
-    with open(lp, "wb") as fh:
-        fh.write(np.int32(len(linedata)).tobytes() + np.ascontiguousarray(linedata).tobytes())
+    if linedata is not None:
+        with open(lp, "wb") as fh:
+            fh.write(np.int32(len(linedata)).tobytes() + np.ascontiguousarray(linedata).tobytes())
     out = os.path.join(tmp, "dump.bin")
     extra = []
     if raw is not None:
@@ -439,8 +440,9 @@ def golden_state_atlas():
     the frequencies between by log-log interpolation)."""
     atm = wl.scurve_atmosphere()
     nustart, nuhi = 9599, 30000
-    lines = wl.atlas_linedata(1, 500.0, 500.1, seed=1)
-    d = atlas_case(atm, nustart, nuhi, lines)
+    # a token selection run, for WAVETAB
+    pred, tio = wl.atlas_iteration_lists(100, 400.0, 600.0, seed=1)
+    d = atlas_case(atm, nustart, nuhi, None, raw=(pred, tio))
     n = d["nrhox"]
     # diatomic slots (841..1006) the equilibrium populates most at the
     # photosphere, for the molecular share of the line store
@@ -449,10 +451,11 @@ def golden_state_atlas():
     score = xd[mol, n // 2]
     mol_used = mol[np.argsort(score)[::-1][:6]] + 1
     atoms = np.array(wl.ATLAS_SPECIES)
-    used = np.unique(np.concatenate([atoms, mol_used]))
+    used = np.unique(np.concatenate([atoms, mol_used, [wl.TIO_NELION]]))
     sub = np.arange(0, nuhi, 100)
     keep = dict(nustart=nustart, teff=d["teff"], flux=d["flux"], mion=d["mion"],
                 waveset=d["waveset"], rcoset=d["rcoset"], tabcont=d["tabcont"],
+                wavetab=d["wavetab"],
                 iwavetab=d["iwavetab"], h0=d["h0"], h1=d["h1"], h2=d["h2"],
                 ion_idx=(used - 1).astype(np.int32), xnfdop=xd[used - 1], dopple=d["dopple"][used - 1],
                 mol_nelion=mol_used.astype(np.int32), kapp_sub=sub.astype(np.int32))
@@ -476,7 +479,7 @@ def golden_select():
     # TiO gf raised so that the TiO packing path keeps lines at this Teff
     pred, tio = wl.atlas_iteration_lists(80_000, 95.0, 20_000.0, seed=31, molecules=mols,
                                          tio_loggf=(4.0, 1.5))
-    d = atlas_case(atm, 9599, 10, wl.atlas_linedata(1, 500.0, 500.1, seed=1), raw=(pred, tio))
+    d = atlas_case(atm, 9599, 10, None, raw=(pred, tio))
     used = np.unique(np.concatenate([np.array(wl.ATLAS_SPECIES), np.array(mols),
                                      [wl.TIO_NELION]]))
     digest = hashlib.sha256(pred.tobytes() + tio.tobytes()).hexdigest()

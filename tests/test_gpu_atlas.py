@@ -95,3 +95,30 @@ def test_iteration_mixed_vs_fp64(gpu_ctx, atl, coef):
     # and the FP64 device iteration against the reference's accumulators
     for k in ("rjmins", "flxrad", "raden", "accrad", "abross"):
         assert _rel(a64[k], z["acc_" + k]).max() <= 1e-12, k
+
+
+def test_selectlines_matches_reference(gpu_ctx, select_case):
+    """SELECTLINES' filter on the GPU keeps the reference's store record for
+    record (72,000 predicted + 8,000 TiO MOL2 records)."""
+    from atlas12_amd import native
+    z, pred, tio = select_case
+    sel = native.atlas_select_lines(gpu_ctx, z, pred, tio)
+    assert np.array_equal(sel, z["selected"])
+
+
+def test_selectlines_edges(gpu_ctx, select_case):
+    """Empty lists, a hot star (TiO skipped above 8000 K), and the input
+    contract (ascending lists) enforced with an error, not a wrong store."""
+    from atlas12_amd import native
+    z, pred, tio = select_case
+    assert native.atlas_select_lines(gpu_ctx, z, pred[:0], tio[:0]).shape == (0, 4)
+    only_pred = native.atlas_select_lines(gpu_ctx, z, pred, tio[:0])
+    ielion = only_pred[:, 1:].copy().view(np.int16).reshape(len(only_pred), 6)[:, 0]
+    assert not np.any(np.abs(ielion.astype(int)) // 10 == 895)
+    hot = dict(z)
+    hot["teff"] = np.float64(9000.0)
+    assert np.array_equal(native.atlas_select_lines(gpu_ctx, hot, pred, tio), only_pred)
+    with pytest.raises(native.A12Error):
+        native.atlas_select_lines(gpu_ctx, z, pred[::-1], tio[:0])
+    with pytest.raises(native.A12Error):
+        native.atlas_select_lines(gpu_ctx, z, pred[:0], tio[::-1])

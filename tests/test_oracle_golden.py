@@ -120,25 +120,21 @@ def test_rt_pass_accumulators_bitwise(orc, golden, coef):
     assert np.array_equal(acc["h_total"], acc["flxrad"])
 
 
-def _select_raw(z):
-    import hashlib
 
-    from atlas12_amd import workloads as wl
-    lo, hi = (float(x) for x in z["wl_range"])
-    pred, tio = wl.atlas_iteration_lists(int(z["n_raw"]), lo, hi, seed=int(z["seed"]),
-                                         molecules=tuple(int(m) for m in z["mol_nelion"]),
-                                         tio_loggf=tuple(float(x) for x in z["tio_loggf"]))
-    digest = hashlib.sha256(pred.tobytes() + tio.tobytes()).hexdigest()
-    assert digest == str(z["raw_sha256"]), "line-list generator drifted from the fixture"
-    return pred, tio
-
-
-def test_selectlines_bitwise(orc, golden):
+def test_selectlines_bitwise(orc, select_case):
     """SELECTLINES on 72,000 predicted (atomic + diatomic) and 8,000 TiO
     MOL2 records keeps exactly the reference's store, record for record,
     including the TiO repacking (IWL, IELO, IGFLOG + isotope, IGR)."""
-    z = golden("atlas_select.npz")
-    pred, tio = _select_raw(z)
+    z, pred, tio = select_case
     sel = orc.atlas_select(z, pred, tio)
     assert sel.shape == z["selected"].shape
     assert np.array_equal(sel, z["selected"])
+
+
+def test_selectlines_planted_mistake_is_caught(orc, select_case):
+    """A dropped Boltzmann factor (HCKT(NRHOX) = 0) keeps lines the
+    reference rejects: the fixture distinguishes it."""
+    z, pred, tio = select_case
+    zz = dict(z)
+    zz["hckt"] = np.array(z["hckt"]) * 0.0
+    assert len(orc.atlas_select(zz, pred, tio)) > len(z["selected"])
