@@ -8,15 +8,15 @@
 //   host          unpack LINEDATA, continuum bin, grid index NU, line
 //                 constants (CGF, ELO, damping), stable sort by NU and the
 //                 per-NU segment offsets;
-//   k_lo_setup    one thread per line walks the depths in the reference's
-//                 coarse (every 8th) then fine order and fixes, per depth,
-//                 kappa_centre, a, the Doppler width in nm and the red/blue
-//                 reach (FP64 decisions, as the reference);
-//   k_lo_reach    per (NU, depth): widest red/blue reach of the segment;
-//   k_lo_gather   one block per grid point, one thread per depth: the sum of
-//                 every reaching line's profile, visiting segments in NU
-//                 order (for a wavelength-sorted store that is the
-//                 reference's own line order).  No atomics.
+//   k_lo_setup    one lane per (line, depth) applies the reference's
+//                 coarse (every 8th) / fine depth rule and fixes kappa_centre,
+//                 a, the Doppler width in nm and the red/blue reach (FP64
+//                 decisions, as the reference; the Lorentz tail by bisection);
+//   k_lo_gather   tiles of 8 grid points x all depths per block: the sum of
+//                 every reaching line's profile, streaming the candidate
+//                 lines through LDS in NU-sorted store order (for a
+//                 wavelength-sorted store, the reference's own line
+//                 order).  No atomics.
 // MIXED evaluates the profile in FP32 and sums with Neumaier compensation;
 // FP64 repeats the reference's arithmetic.
 
@@ -77,21 +77,22 @@ __device__ __forceinline__ R lo_cv(R center, R v, R adamp, const R *h0, const R 
     return center * ((h2[iv - 1] * adamp + h1[iv - 1]) * adamp + h0[iv - 1]);
 }
 
-// The same wing value with the arithmetic in FP32 but every table-row and
-// regime decision taken from the FP64 offset v64: a row index computed from
-// a rounded v flips to the neighbouring row near integer 200 v + 1.5, a
-// step of ~0.5 % in that contribution.
-__device__ __forceinline__ float lo_cv_mixed(float center, double v64, float adamp, const float *h0,
+// The same wing value with the arithmetic in FP32 but every regime and
+// table-row decision taken from the FP64 offset v64 and damping a64, as the
+// FP64 path takes them: a row index from a rounded v flips to the
+// neighbouring row near integer 200 v + 1.5 (~0.5 % of that contribution),
+// and a rounded a at the 0.2 boundary switches between the table and
+// VOIGT (~4 %).
+__device__ __forceinline__ float lo_cv_mixed(float center, double v64, double a64, const float *h0,
                                              const float *h1, const float *h2)
 {
 #pragma clang fp contract(off)
-    const float v = (float)v64;
-    if (adamp > 0.20f) {
+    const float v = (float)v64, a = (float)a64;
+    if (a64 > 0.20) {
         int iv = (int)(v64 * 200.0 + 1.5);
         iv = iv < 1 ? 1 : (iv > kTabN ? kTabN : iv);
         const float H0 = h0[iv - 1], H1 = h1[iv - 1], H2 = h2[iv - 1];
-        const float a = adamp;
-        if (a <= 1.4f && (double)a + v64 <= 3.2) {
+        if (a64 <= 1.4 && a64 + v64 <= 3.2) {
             const float vv = v * v;
             const float hh1 = H1 + H0 * 1.12838f;
             const float hh2 = H2 + hh1 * 1.12838f - H0;
@@ -103,90 +104,120 @@ __device__ __forceinline__ float lo_cv_mixed(float center, double v64, float ada
         const float aa = a * a, vv = v * v;
         const float u = (aa + vv) * 1.4142f;
         float r = a * 0.79788f / u;
-        if (a <= 100.0f) {
+        if (a64 <= 100.0) {
             const float aau = aa / u, vvu = vv / u, uu = u * u;
             r = ((((aau - 10.0f * vvu) * aau * 3.0f + 15.0f * vvu * vvu) + 3.0f * vv - aa) / uu + 1.0f) * r;
         }
         return center * r;
     }
-    if (v64 > 10.0) return center * 0.5641895835477563f * adamp / (v * v);
+    if (v64 > 10.0) return center * 0.5641895835477563f * a / (v * v);
     const int iv = (int)(v64 * 200.0 + 1.5);
-    return center * ((h2[iv - 1] * adamp + h1[iv - 1]) * adamp + h0[iv - 1]);
+    return center * ((h2[iv - 1] * a + h1[iv - 1]) * a + h0[iv - 1]);
 }
 
-// Per line: all depths.  o_state[l*n + j]: bit 31 computed, bits 0-7 red
-// reach, 8-15 blue reach (contributions to points nu.. and nu-1..).
-__global__ void k_lo_setup(int nl, int n, int numnu, int mion, const int *__restrict__ l_nu,
-                           const int *__restrict__ l_nucont, const int *__restrict__ l_nel,
-                           const double *__restrict__ l_const, const double *__restrict__ waveset,
-                           const double *__restrict__ tabcont, const double *__restrict__ xnfdop,
-                           const double *__restrict__ dopple, const double *__restrict__ dep,
-                           const double *__restrict__ h0, const double *__restrict__ h1,
-                           const double *__restrict__ h2, int *__restrict__ o_state,
-                           double *__restrict__ o_par, long long *__restrict__ blk_evals)
+// Number of wing points LINOP1 adds on one side before and including the
+// first value below tc: points base+sgn, base+2 sgn, ... (red: offsets
+// w - wl; blue: wl - w), at most kMaxWing or up to the grid end.  Table and
+// VOIGT regimes are walked point by point; once a <= 0.2 and v > 10 the
+// value is center * INVSQRTPI * a / v^2, which never increases along the
+// walk in FP64 (v = (w - wl) / dopwave is monotone in w under rounding and
+// so is each later operation), so the first point below tc is found by
+// bisection with the same predicate the walk would test.
+__device__ int lo_reach(int first, int last, int step, const double *__restrict__ waveset,
+                        double wlvac, double dopwave, double center, double adamp, double tc,
+                        const double *h0, const double *h1, const double *h2)
 {
 #pragma clang fp contract(off)
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    int cnt = 0;
+    int iw = first;
+    const int nmax = step > 0 ? last - first + 1 : first - last + 1;
+    if (nmax <= 0) return 0;
+    for (; cnt < nmax; ++cnt, iw += step) {
+        const double v = step > 0 ? (waveset[iw] - wlvac) / dopwave : (wlvac - waveset[iw]) / dopwave;
+        if (adamp <= 0.20 && v > 10.0) break;  // monotone Lorentz tail from here
+        if (lo_cv<double>(center, v, adamp, h0, h1, h2) < tc) return cnt + 1;
+    }
+    if (cnt == nmax) return cnt;
+    // bisection over walk positions [cnt, nmax): first with value < tc
+    const double k = center * 0.5641895835477563 * adamp;
+    int lo = cnt, hi = nmax;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const int pw = first + mid * step;
+        const double v = step > 0 ? (waveset[pw] - wlvac) / dopwave : (wlvac - waveset[pw]) / dopwave;
+        if (k / (v * v) < tc) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo < nmax ? lo + 1 : nmax;
+}
+
+// One lane per (line, depth).  LINOP1 computes the coarse depths J = 8, 16,
+// ... first, then each fine depth J in K-7..K-1 only where coarse depth K-8
+// or K passed both centre tests (IFJ, :18239-18365); the decision depends on
+// nothing else, so a fine lane re-tests its two coarse neighbours itself.
+// o_state[l*n + j]: bit 31 computed, bits 0-7 red reach, 8-15 blue reach
+// (points nu.. and nu-1..); o_par: centre, a, Doppler width in nm.
+__device__ __forceinline__ bool lo_center_ok(double cgf, double elo, double xd, double tc,
+                                             double hckt, double *center)
+{
+#pragma clang fp contract(off)
+    double c = cgf * xd;
+    if (c < tc) return false;
+    c = c * exp(-elo * hckt);
+    if (c < tc) return false;
+    *center = c;
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_lo_setup(
+    int nl, int n, int numnu, const int *__restrict__ l_nu, const int *__restrict__ l_nucont,
+    const int *__restrict__ l_nel, const double *__restrict__ l_const,
+    const double *__restrict__ waveset, const double *__restrict__ tabcont,
+    const double *__restrict__ xnfdop, const double *__restrict__ dopple,
+    const double *__restrict__ dep, const double *__restrict__ h0, const double *__restrict__ h1,
+    const double *__restrict__ h2, int *__restrict__ o_state, double *__restrict__ o_par,
+    long long *__restrict__ blk_evals)
+{
+#pragma clang fp contract(off)
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long ev = 0;
-    if (l < nl) {
-        const int nu = l_nu[l], nucont = l_nucont[l], nel = l_nel[l] - 1;
-        const double wlvac = l_const[5 * (size_t)l + 0], cgf = l_const[5 * (size_t)l + 1],
-                     elo = l_const[5 * (size_t)l + 2], gr = l_const[5 * (size_t)l + 3],
-                     gs = l_const[5 * (size_t)l + 4];
-        const double gw = l_const[5 * (size_t)nl + l];
-        const double *xne = dep + 2 * n, *hckt = dep + 3 * n, *txnxn = dep + 5 * n;
-        for (int j = 0; j < n; ++j) o_state[(size_t)l * n + j] = 0;
-        unsigned ifj = 0;  // bit k: coarse depth 8k passed (bit 0 = IFJ(1) = 0)
-        for (int pass = 0; pass < 2; ++pass) {
-            for (int K = 8; K <= n; K += 8) {
-                int jlo, jhi;
-                if (pass == 0) {
-                    jlo = jhi = K;
-                } else {
-                    if (!((ifj >> (K / 8 - 1)) & 1u) && !((ifj >> (K / 8)) & 1u)) continue;
-                    jlo = K - 7;
-                    jhi = K - 1;
-                }
-                for (int J = jlo; J <= jhi; ++J) {
-                    const int j = J - 1;
-                    const size_t o = (size_t)l * n + j;
-                    const double tc = tabcont[(size_t)nucont * n + j];
-                    double center = cgf * xnfdop[(size_t)nel * n + j];
-                    int st = 0;
-                    if (!(center < tc)) {
-                        center = center * exp(-elo * hckt[j]);
-                        if (!(center < tc)) {
-                            if (pass == 0) ifj |= 1u << (K / 8);
-                            const double dop = dopple[(size_t)nel * n + j];
-                            const double adamp = (gr + gs * xne[j] + gw * txnxn[j]) / dop;
-                            const double dopwave = dop * wlvac;
-                            int rn = 0, bn = 0;
-                            const int hi = min(nu + kMaxWing, numnu - 1);
-                            for (int iw = nu; iw <= hi; ++iw) {
-                                ++rn;
-                                if (lo_cv<double>(center, (waveset[iw] - wlvac) / dopwave, adamp, h0,
-                                                  h1, h2) < tc)
-                                    break;
-                            }
-                            for (int i = 1; i <= kMaxWing; ++i) {
-                                const int iw = nu - i;
-                                if (iw < 0) break;
-                                ++bn;
-                                if (lo_cv<double>(center, (wlvac - waveset[iw]) / dopwave, adamp, h0,
-                                                  h1, h2) < tc)
-                                    break;
-                            }
-                            st = (int)(0x80000000u | (unsigned)rn | ((unsigned)bn << 8));
-                            o_par[3 * o + 0] = center;
-                            o_par[3 * o + 1] = adamp;
-                            o_par[3 * o + 2] = dopwave;
-                            ev += rn + bn;
-                        }
-                    }
-                    o_state[o] = st;
-                }
+    if (g < (long long)nl * n) {
+        const int l = (int)(g / n), j = (int)(g - (long long)l * n);
+        const int J = j + 1, ncoarse = (n / 8) * 8;
+        int st = 0;
+        if (J <= ncoarse) {
+            const int nu = l_nu[l], nucont = l_nucont[l], nel = l_nel[l] - 1;
+            const double cgf = l_const[5 * (size_t)l + 1], elo = l_const[5 * (size_t)l + 2];
+            const double *xd = xnfdop + (size_t)nel * n, *tcv = tabcont + (size_t)nucont * n;
+            const double *hckt = dep + 3 * n;
+            bool on = true;
+            if (J % 8 != 0) {  // fine depth: a neighbouring coarse depth must have passed
+                const int K = (J / 8 + 1) * 8;
+                double c;
+                on = (K - 8 >= 8 && lo_center_ok(cgf, elo, xd[K - 9], tcv[K - 9], hckt[K - 9], &c)) ||
+                     lo_center_ok(cgf, elo, xd[K - 1], tcv[K - 1], hckt[K - 1], &c);
+            }
+            double center;
+            const double tc = tcv[j];
+            if (on && lo_center_ok(cgf, elo, xd[j], tc, hckt[j], &center)) {
+                const double wlvac = l_const[5 * (size_t)l + 0], gr = l_const[5 * (size_t)l + 3],
+                             gs = l_const[5 * (size_t)l + 4], gw = l_const[5 * (size_t)nl + l];
+                const double dop = dopple[(size_t)nel * n + j];
+                const double adamp = (gr + gs * dep[2 * n + j] + gw * dep[5 * n + j]) / dop;
+                const double dopwave = dop * wlvac;
+                // red: IW = NU .. min(NU+100, NUMNU); blue: NU-1 down, at most 100
+                const int rn = lo_reach(nu, min(nu + kMaxWing, numnu - 1), 1, waveset, wlvac,
+                                        dopwave, center, adamp, tc, h0, h1, h2);
+                const int bn = lo_reach(nu - 1, max(nu - kMaxWing, 0), -1, waveset, wlvac,
+                                        dopwave, center, adamp, tc, h0, h1, h2);
+                st = (int)(0x80000000u | (unsigned)rn | ((unsigned)bn << 8));
+                o_par[3 * g + 0] = center;
+                o_par[3 * g + 1] = adamp;
+                o_par[3 * g + 2] = dopwave;
+                ev = rn + bn;
             }
         }
+        o_state[g] = st;
     }
     __shared__ long long red[256];
     red[threadIdx.x] = ev;
@@ -212,28 +243,22 @@ __global__ void k_lo_sum(const long long *__restrict__ blk, int nb, long long *o
     if (threadIdx.x == 0) *out = red[0];
 }
 
-// Widest red/blue reach per (segment k, depth j); reach[(k*n + j)] packs
-// red (low 8 bits) and blue (next 8).
-__global__ void k_lo_reach(int nseg, int n, const int *__restrict__ seg,
-                           const int *__restrict__ state, int *__restrict__ reach)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nseg * n) return;
-    const int k = i / n, j = i % n;
-    int r = 0, b = 0;
-    for (int l = seg[k]; l < seg[k + 1]; ++l) {
-        const int st = state[(size_t)l * n + j];
-        if (st < 0) {
-            r = max(r, st & 0xff);
-            b = max(b, (st >> 8) & 0xff);
-        }
-    }
-    reach[i] = r | (b << 8);
-}
+// Gather: a block holds tiles of kLoPts consecutive grid points; lane t
+// serves tile t / n and depth t % n.  The lines that can reach the block's
+// points are one contiguous run of the NU-sorted store (segments NU =
+// first - 100 .. last + 100); the block streams it in batches of kLoBatch
+// lines, staging their per-depth state in LDS with coalesced loads, and
+// each lane takes from every staged line the points of its tile that the
+// line's red/blue reach covers at its depth.  Lines are visited in store
+// order, which for every grid point is the order LINOP1 adds them, into the
+// lane's own LDS column: no atomics, deterministic.
+constexpr int kLoPts = 8;
+constexpr int kLoBlock = 512;
+constexpr int kLoBatch = 32;
 
 template <bool F64>
-__global__ __launch_bounds__(128) void k_lo_gather(
-    int n, int numnu, const int *__restrict__ seg, const int *__restrict__ reach,
+__global__ __launch_bounds__(kLoBlock) void k_lo_gather(
+    int n, int numnu, int tiles, const int *__restrict__ seg, const int *__restrict__ l_nu,
     const int *__restrict__ state, const double *__restrict__ par,
     const double *__restrict__ l_const, const double *__restrict__ waveset,
     const double *__restrict__ h0, const double *__restrict__ h1, const double *__restrict__ h2,
@@ -241,37 +266,71 @@ __global__ __launch_bounds__(128) void k_lo_gather(
     double *__restrict__ xlines)
 {
 #pragma clang fp contract(off)
-    const int iw = blockIdx.x;
-    const int j = threadIdx.x;
-    if (j >= n) return;
-    const double w = waveset[iw];
-    double acc64 = 0.0;
-    Neumaier32 acc32;
-    // red contributions: segments k <= iw; blue: k > iw
-    const int klo = max(0, iw - kMaxWing), khi = min(numnu, iw + kMaxWing);
-    for (int k = klo; k <= khi; ++k) {
-        const int rc = reach[(size_t)k * n + j];
-        const bool red = k <= iw;
-        const int need = red ? iw - k + 1 : k - iw;
-        const int have = red ? (rc & 0xff) : ((rc >> 8) & 0xff);
-        if (have < need) continue;
-        for (int l = seg[k]; l < seg[k + 1]; ++l) {
-            const int st = state[(size_t)l * n + j];
+    // LDS: F64 sums, or FP32 sums + compensations (same bytes)
+    __shared__ double acc_d[kLoPts * kLoBlock];
+    __shared__ int s_state[kLoBatch * A12_NRMAX_LAYERS];
+    __shared__ int s_nu[kLoBatch];
+    __shared__ double s_wl[kLoBatch];
+    float *acc_s = reinterpret_cast<float *>(acc_d);
+    float *acc_c = acc_s + kLoPts * kLoBlock;
+    const int per = kLoBlock / n;  // tiles per block
+    const int t = threadIdx.x;
+    const int lt = t / n, j = t - lt * n;
+    const int tile0 = blockIdx.x * per;
+    const int tile = tile0 + lt;
+    const bool live = lt < per && tile < tiles;
+    const int iw0 = tile * kLoPts, iw1 = min(iw0 + kLoPts, numnu) - 1;
+    for (int p = 0; p < kLoPts; ++p) {
+        if (F64) acc_d[p * kLoBlock + t] = 0.0;
+        else acc_s[p * kLoBlock + t] = acc_c[p * kLoBlock + t] = 0.0f;
+    }
+    // the block's candidate lines: segments NU in [first - 100, last + 100]
+    // (red from NU reaches NU..NU+100, blue NU-100..NU-1)
+    const int bfirst = tile0 * kLoPts, blast = min((tile0 + per) * kLoPts, numnu) - 1;
+    const int lbeg = seg[max(0, bfirst - kMaxWing)], lend = seg[min(numnu, blast + kMaxWing) + 1];
+    for (int b0 = lbeg; b0 < lend; b0 += kLoBatch) {
+        const int nb = min(kLoBatch, lend - b0);
+        __syncthreads();
+        for (int i = t; i < nb * n; i += kLoBlock) s_state[i] = state[(size_t)b0 * n + i];
+        if (t < nb) {
+            s_nu[t] = l_nu[b0 + t];
+            s_wl[t] = l_const[5 * (size_t)(b0 + t)];
+        }
+        __syncthreads();
+        if (!live) continue;
+        for (int i = 0; i < nb; ++i) {
+            const int st = s_state[i * n + j];
             if (st >= 0) continue;
-            const int reachl = red ? (st & 0xff) : ((st >> 8) & 0xff);
-            if (reachl < need) continue;
-            const size_t o = (size_t)l * n + j;
-            const double wl = l_const[5 * (size_t)l];
-            if (F64) {
-                const double v = (red ? (w - wl) : (wl - w)) / par[3 * o + 2];
-                acc64 = acc64 + lo_cv<double>(par[3 * o], v, par[3 * o + 1], h0, h1, h2);
-            } else {
-                const double v = (red ? (w - wl) : (wl - w)) / par[3 * o + 2];
-                acc32.add(lo_cv_mixed((float)par[3 * o], v, (float)par[3 * o + 1], f0, f1, f2));
+            const int k = s_nu[i], rn = st & 0xff, bn = (st >> 8) & 0xff;
+            const int ra = max(k, iw0), rb = min(k + rn - 1, iw1);
+            const int ba = max(k - bn, iw0), bb = min(k - 1, iw1);
+            if (ra > rb && ba > bb) continue;
+            const size_t o = (size_t)(b0 + i) * n + j;
+            const double wl = s_wl[i];
+            const double center = par[3 * o], adamp = par[3 * o + 1], dopwave = par[3 * o + 2];
+            // a point lies on one side of the line only: the ranges are disjoint
+            for (int side = 0; side < 2; ++side) {
+                const int a = side ? ba : ra, b = side ? bb : rb;
+                for (int iw = a; iw <= b; ++iw) {
+                    const double v = (side ? (wl - waveset[iw]) : (waveset[iw] - wl)) / dopwave;
+                    const int q = (iw - iw0) * kLoBlock + t;
+                    if (F64) {
+                        acc_d[q] = acc_d[q] + lo_cv<double>(center, v, adamp, h0, h1, h2);
+                    } else {
+                        const float x = lo_cv_mixed((float)center, v, adamp, f0, f1, f2);
+                        const float s0 = acc_s[q], tt = s0 + x;
+                        acc_c[q] += fabsf(s0) >= fabsf(x) ? (s0 - tt) + x : (x - tt) + s0;
+                        acc_s[q] = tt;
+                    }
+                }
             }
         }
     }
-    xlines[(size_t)iw * n + j] = F64 ? acc64 : (double)acc32.value();
+    if (live)
+        for (int iw = iw0; iw <= iw1; ++iw) {
+            const int q = (iw - iw0) * kLoBlock + t;
+            xlines[(size_t)iw * n + j] = F64 ? acc_d[q] : (double)acc_s[q] + (double)acc_c[q];
+        }
 }
 
 template <class T>
@@ -436,14 +495,14 @@ int a12i_atlas_linop1_run(a12_ctx *ctx, int prec)
     A12_HIP(A.par.reserve(sizeof(double) * 3 * nl1 * n));
     A12_HIP(A.reach.reserve(sizeof(int) * (size_t)(numnu + 1) * n));
     A12_HIP(A.xlines.reserve(sizeof(double) * (size_t)numnu * n));
-    const int nb = (int)((nl1 + 255) / 256);
+    const int nb = (int)((nl1 * n + 255) / 256);  // one lane per (line, depth)
     A12_HIP(A.blk.reserve(sizeof(long long) * nb));
     A12_HIP(A.evals.reserve(sizeof(long long)));
     const double *tabs = A.tabs.as<double>();
     const float *tabs32 = A.tabs32.as<float>();
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[8], s));
     if (nl > 0) {
-        hipLaunchKernelGGL(k_lo_setup, dim3(nb), dim3(256), 0, s, nl, n, numnu, A.mion,
+        hipLaunchKernelGGL(k_lo_setup, dim3(nb), dim3(256), 0, s, nl, n, numnu,
                            A.l_nu.as<int>(), A.l_nucont.as<int>(), A.l_nel.as<int>(),
                            A.l_const.as<double>(), A.waveset.as<double>(), A.tabcont.as<double>(),
                            A.xnfdop.as<double>(), A.dopple.as<double>(), A.depth_lo.as<double>(),
@@ -457,13 +516,11 @@ int a12i_atlas_linop1_run(a12_ctx *ctx, int prec)
         A12_HIP(hipMemsetAsync(A.evals.p, 0, sizeof(long long), s));
     }
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[9], s));
-    const int nseg = numnu + 1;
-    hipLaunchKernelGGL(k_lo_reach, dim3((nseg * n + 255) / 256), dim3(256), 0, s, nseg, n,
-                       A.seg.as<int>(), A.state.as<int>(), A.reach.as<int>());
-    A12_CHECK_LAUNCH();
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[10], s));
     auto g = prec == A12_PREC_FP64 ? k_lo_gather<true> : k_lo_gather<false>;
-    hipLaunchKernelGGL(g, dim3(numnu), dim3(128), 0, s, n, numnu, A.seg.as<int>(), A.reach.as<int>(),
+    const int tiles = (numnu + kLoPts - 1) / kLoPts, per = kLoBlock / n;
+    hipLaunchKernelGGL(g, dim3((tiles + per - 1) / per), dim3(kLoBlock), 0, s, n, numnu, tiles,
+                       A.seg.as<int>(), A.l_nu.as<int>(),
                        A.state.as<int>(), A.par.as<double>(), A.l_const.as<double>(),
                        A.waveset.as<double>(), tabs, tabs + kTabN, tabs + 2 * kTabN, tabs32,
                        tabs32 + kTabN, tabs32 + 2 * kTabN, A.xlines.as<double>());
