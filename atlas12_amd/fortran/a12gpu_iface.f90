@@ -1,0 +1,237 @@
+!=======================================================================
+! a12gpu_iface -- Fortran interface to liba12gpu.so (include/a12gpu.h),
+! the MI355X implementation of the ATLAS12/SYNTHE hot path.
+!
+! The library takes the drivers' own arrays in their Fortran layout:
+! REAL(4) <-> float, REAL(8) <-> double, default INTEGER <-> int32_t.
+! Array arguments are passed by reference (assumed-size dummies), so a
+! module array such as XLINES(kw, 30000) or opacity_matrix(L, nrhox) goes
+! straight through.  Structs hold C pointers (C_LOC of TARGET arrays).
+!
+! Every function returns 0 (A12_OK) or a negative status; a12_errmsg()
+! returns the context's last message as a Fortran string.
+!=======================================================================
+MODULE a12gpu_iface
+  USE, INTRINSIC :: iso_c_binding
+  IMPLICIT NONE
+
+  INTEGER(C_INT32_T), PARAMETER :: A12_OK = 0, A12_ERR_ARG = -1, A12_ERR_HIP = -2, &
+                                   A12_ERR_OOM = -3, A12_ERR_STATE = -4, &
+                                   A12_ERR_NONFINITE = -5
+  INTEGER(C_INT32_T), PARAMETER :: A12_PREC_MIXED = 0, A12_PREC_FP64 = 1
+
+  ! lte_line_t, src/mod_mklinelist.f90:30-41 (32 bytes)
+  TYPE, BIND(C) :: a12_lte_line
+    INTEGER(C_INT32_T) :: nbuff
+    REAL(C_FLOAT)      :: cgf
+    INTEGER(C_INT32_T) :: nelion
+    REAL(C_FLOAT)      :: elo, gamrf, gamsf, gamwf, code
+  END TYPE a12_lte_line
+
+  ! MOL2 32-byte record (SELECTLINES' TiO path)
+  TYPE, BIND(C) :: a12_mol2_rec
+    REAL(C_FLOAT)      :: wl, loggf, e1, e2
+    INTEGER(C_INT32_T) :: code, iso, gr100, flag
+  END TYPE a12_mol2_rec
+
+  ! State of SYNTHE's two hot loops (src/synthe.f90:476-718)
+  TYPE, BIND(C) :: a12_synthe_atm
+    INTEGER(C_INT32_T) :: nrhox, length
+    REAL(C_DOUBLE)     :: wbegin, ratio, resolu
+    INTEGER(C_INT32_T) :: nedge
+    TYPE(C_PTR)        :: wledge
+    INTEGER(C_INT32_T) :: nf
+    TYPE(C_PTR)        :: continall, contabs, contscat
+    INTEGER(C_INT32_T) :: mw6
+    TYPE(C_PTR)        :: xnfpel, dopple
+    TYPE(C_PTR)        :: t, xne, rho, xnfh, xnfhe1, xnfh2, hckt, hkt
+    TYPE(C_PTR)        :: hkt_atlas, rhox
+  END TYPE a12_synthe_atm
+
+  ! State LINOP1 and the RUN_RT_PASS frequency loop read
+  TYPE, BIND(C) :: a12_atlas_atm
+    INTEGER(C_INT32_T) :: nrhox, numnu, mion, ldion, ldtab, nwave1
+    TYPE(C_PTR)        :: waveset, rcoset, tabcont, iwavetab, xnfdop, dopple
+    TYPE(C_PTR)        :: t, xne, hckt, hkt, rhox
+    TYPE(C_PTR)        :: xnf1, xnf3, xnf841
+    TYPE(C_PTR)        :: h0tab, h1tab, h2tab
+    REAL(C_DOUBLE)     :: teff
+  END TYPE a12_atlas_atm
+
+  INTERFACE
+    INTEGER(C_INT32_T) FUNCTION a12_ctx_create(device, ctx) BIND(C, NAME='a12_ctx_create')
+      IMPORT :: C_INT32_T, C_PTR
+      INTEGER(C_INT32_T), VALUE :: device
+      TYPE(C_PTR), INTENT(OUT)  :: ctx
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_ctx_destroy(ctx) BIND(C, NAME='a12_ctx_destroy')
+      IMPORT :: C_INT32_T, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
+    TYPE(C_PTR) FUNCTION a12_last_error(ctx) BIND(C, NAME='a12_last_error')
+      IMPORT :: C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_set_operators(ctx, coefj, coefh) &
+        BIND(C, NAME='a12_set_operators')
+      IMPORT :: C_INT32_T, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), INTENT(IN) :: coefj(*), coefh(*)
+    END FUNCTION
+
+    ! ---- SYNTHE ---------------------------------------------------------
+    INTEGER(C_INT32_T) FUNCTION a12_synthe_line_opacity(ctx, atm, lines, nlines, prec, &
+        opacity_out, evals_out) BIND(C, NAME='a12_synthe_line_opacity')
+      IMPORT :: C_INT32_T, C_INT64_T, C_PTR, a12_synthe_atm, a12_lte_line
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(a12_synthe_atm), INTENT(IN) :: atm
+      TYPE(a12_lte_line), INTENT(IN) :: lines(*)
+      INTEGER(C_INT32_T), VALUE :: nlines, prec
+      TYPE(C_PTR), VALUE :: opacity_out
+      INTEGER(C_INT64_T), INTENT(OUT) :: evals_out
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_synthe_load(ctx, atm, lines, nlines) &
+        BIND(C, NAME='a12_synthe_load')
+      IMPORT :: C_INT32_T, C_PTR, a12_synthe_atm, a12_lte_line
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(a12_synthe_atm), INTENT(IN) :: atm
+      TYPE(a12_lte_line), INTENT(IN) :: lines(*)
+      INTEGER(C_INT32_T), VALUE :: nlines
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_synthe_opacity_run(ctx, prec) &
+        BIND(C, NAME='a12_synthe_opacity_run')
+      IMPORT :: C_INT32_T, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT32_T), VALUE :: prec
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_synthe_opacity_fetch(ctx, opacity_out) &
+        BIND(C, NAME='a12_synthe_opacity_fetch')
+      IMPORT :: C_INT32_T, C_PTR
+      TYPE(C_PTR), VALUE :: ctx, opacity_out
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_synthe_transfer_run(ctx) &
+        BIND(C, NAME='a12_synthe_transfer_run')
+      IMPORT :: C_INT32_T, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_synthe_transfer_fetch(ctx, hnu_line, hnu_cont) &
+        BIND(C, NAME='a12_synthe_transfer_fetch')
+      IMPORT :: C_INT32_T, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), INTENT(OUT) :: hnu_line(*), hnu_cont(*)
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_synthe_transfer(ctx, atm, opacity, prec, hnu_line, &
+        hnu_cont) BIND(C, NAME='a12_synthe_transfer')
+      IMPORT :: C_INT32_T, C_PTR, C_DOUBLE, a12_synthe_atm
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(a12_synthe_atm), INTENT(IN) :: atm
+      TYPE(C_PTR), VALUE :: opacity
+      INTEGER(C_INT32_T), VALUE :: prec
+      REAL(C_DOUBLE), INTENT(OUT) :: hnu_line(*), hnu_cont(*)
+    END FUNCTION
+
+    ! ---- JOSH, batched ----------------------------------------------------
+    INTEGER(C_INT32_T) FUNCTION a12_josh_batch(ctx, ifscat, ifsurf, nrhox, nbatch, rhox, &
+        acont, aline, sigmac, sigmal, scont, sline, bnu, hnu1, knu1, taunu, snu, jnu, hnu, &
+        jmins) BIND(C, NAME='a12_josh_batch')
+      IMPORT :: C_INT32_T, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT32_T), VALUE :: ifscat, ifsurf, nrhox, nbatch
+      REAL(C_DOUBLE), INTENT(IN) :: rhox(*), acont(*), aline(*), sigmac(*), sigmal(*), &
+                                    scont(*), sline(*), bnu(*)
+      REAL(C_DOUBLE), INTENT(OUT) :: hnu1(*), knu1(*), taunu(*), snu(*), jnu(*), hnu(*), &
+                                     jmins(*)
+    END FUNCTION
+
+    ! ---- ATLAS12 ----------------------------------------------------------
+    INTEGER(C_INT32_T) FUNCTION a12_atlas_select_lines(ctx, atm, wavetab, pred, npred, mol2, &
+        nmol2, linedata_out, cap, nkept) BIND(C, NAME='a12_atlas_select_lines')
+      IMPORT :: C_INT32_T, C_INT64_T, C_PTR, C_DOUBLE, a12_atlas_atm, a12_mol2_rec
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(a12_atlas_atm), INTENT(IN) :: atm
+      REAL(C_DOUBLE), INTENT(IN) :: wavetab(*)
+      INTEGER(C_INT32_T), INTENT(IN) :: pred(*)
+      INTEGER(C_INT64_T), VALUE :: npred
+      TYPE(a12_mol2_rec), INTENT(IN) :: mol2(*)
+      INTEGER(C_INT64_T), VALUE :: nmol2
+      INTEGER(C_INT32_T), INTENT(OUT) :: linedata_out(*)
+      INTEGER(C_INT64_T), VALUE :: cap
+      INTEGER(C_INT64_T), INTENT(OUT) :: nkept
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_atlas_load(ctx, atm, linedata, nlines) &
+        BIND(C, NAME='a12_atlas_load')
+      IMPORT :: C_INT32_T, C_PTR, a12_atlas_atm
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(a12_atlas_atm), INTENT(IN) :: atm
+      INTEGER(C_INT32_T), INTENT(IN) :: linedata(*)
+      INTEGER(C_INT32_T), VALUE :: nlines
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_atlas_linop1_run(ctx, prec) &
+        BIND(C, NAME='a12_atlas_linop1_run')
+      IMPORT :: C_INT32_T, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT32_T), VALUE :: prec
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_atlas_linop1_fetch(ctx, xlines, evals) &
+        BIND(C, NAME='a12_atlas_linop1_fetch')
+      IMPORT :: C_INT32_T, C_INT64_T, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), INTENT(OUT) :: xlines(*)
+      INTEGER(C_INT64_T), INTENT(OUT) :: evals
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_atlas_set_kapp(ctx, acont, sigmac, scont, sline, aline, &
+        sigmal) BIND(C, NAME='a12_atlas_set_kapp')
+      IMPORT :: C_INT32_T, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), INTENT(IN) :: acont(*), sigmac(*), scont(*), sline(*), aline(*), sigmal(*)
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_atlas_rt_run(ctx, xlines) BIND(C, NAME='a12_atlas_rt_run')
+      IMPORT :: C_INT32_T, C_PTR
+      TYPE(C_PTR), VALUE :: ctx, xlines   ! C_NULL_PTR: use the device XLINES of linop1
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_atlas_rt_fetch(ctx, acc, hnu1) &
+        BIND(C, NAME='a12_atlas_rt_fetch')
+      IMPORT :: C_INT32_T, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), INTENT(OUT) :: acc(*), hnu1(*)
+    END FUNCTION
+  END INTERFACE
+
+CONTAINS
+
+  ! The context's last error message as a Fortran string.
+  FUNCTION a12_errmsg(ctx) RESULT(msg)
+    TYPE(C_PTR), INTENT(IN) :: ctx
+    CHARACTER(LEN=:), ALLOCATABLE :: msg
+    CHARACTER(KIND=C_CHAR), POINTER :: s(:)
+    TYPE(C_PTR) :: p
+    INTEGER :: n
+    INTERFACE
+      INTEGER(C_SIZE_T) FUNCTION c_strlen(str) BIND(C, NAME='strlen')
+        IMPORT :: C_SIZE_T, C_PTR
+        TYPE(C_PTR), VALUE :: str
+      END FUNCTION
+    END INTERFACE
+    p = a12_last_error(ctx)
+    IF (.NOT. C_ASSOCIATED(p)) THEN
+      msg = ''
+      RETURN
+    END IF
+    n = INT(c_strlen(p))
+    CALL C_F_POINTER(p, s, [n])
+    ALLOCATE(CHARACTER(LEN=n) :: msg)
+    msg = TRANSFER(s(1:n), msg)
+  END FUNCTION a12_errmsg
+
+  ! Fail-fast check in the drivers' style: print and stop on an error.
+  SUBROUTINE a12_check(ctx, rc, what)
+    TYPE(C_PTR), INTENT(IN) :: ctx
+    INTEGER(C_INT32_T), INTENT(IN) :: rc
+    CHARACTER(LEN=*), INTENT(IN) :: what
+    IF (rc .NE. A12_OK) THEN
+      WRITE(6, '(A,A,A,I3,A,A)') ' ', what, ' failed (', rc, '): ', a12_errmsg(ctx)
+      CALL EXIT(1)
+    END IF
+  END SUBROUTINE a12_check
+
+END MODULE a12gpu_iface

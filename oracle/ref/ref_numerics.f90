@@ -2,7 +2,7 @@
 ! vectors supplied in a stream binary file and writes their outputs.
 !
 ! Test infrastructure: built by oracle/ref/Makefile against the reference
-! modules compiled from /root/reference/src; its outputs become golden
+! modules compiled from the reference's src/; its outputs become golden
 ! vectors under tests/golden/.  Usage:
 !     ref_numerics <op> <in.bin> <out.bin> [datadir/]
 ! Every input file starts with int32 header words described per op below;

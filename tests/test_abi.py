@@ -56,3 +56,64 @@ def test_missing_library_fails_loudly(tmp_path):
             native.load_library(str(tmp_path / "nope.so"))
     finally:
         native._lib = saved
+
+
+def test_struct_layouts_atlas(tmp_path):
+    """a12_atlas_atm and a12_mol2_rec: the ctypes mirrors and MOL2_DTYPE
+    match the C layout."""
+    import subprocess
+    from atlas12_amd import native
+    from atlas12_amd.workloads import MOL2_DTYPE
+    fields = [f[0] for f in native.AtlasAtm._fields_]
+    src = tmp_path / "layout2.c"
+    body = "".join(f'printf("%zu\\n", offsetof(a12_atlas_atm, {f}));' for f in fields)
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "a12gpu.h"\n'
+                   'int main(void){printf("%zu\\n%zu\\n", sizeof(a12_mol2_rec), sizeof(a12_atlas_atm));'
+                   + body + "return 0;}\n")
+    exe = tmp_path / "layout2"
+    subprocess.run(["gcc", "-I", os.path.join(REPO, "include"), str(src), "-o", str(exe)], check=True)
+    vals = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
+    assert vals[0] == MOL2_DTYPE.itemsize == 32
+    assert vals[1] == ctypes.sizeof(native.AtlasAtm)
+    assert vals[2:] == [getattr(native.AtlasAtm, f).offset for f in fields]
+
+
+def test_fortran_interface_module(tmp_path):
+    """atlas12_amd/fortran/a12gpu_iface.f90 compiles, its BIND(C) types have
+    the C sizes, and calls through it reach the library (a null context is
+    rejected with A12_ERR_ARG before any device work)."""
+    import shutil
+    import subprocess
+
+    import pytest
+    fc = shutil.which("amdflang") or shutil.which("gfortran")
+    if fc is None:
+        pytest.skip("no Fortran compiler")
+    from atlas12_amd import native
+    lib = native.load_library()
+    prog = tmp_path / "t.f90"
+    prog.write_text("""
+PROGRAM t
+  USE a12gpu_iface
+  IMPLICIT NONE
+  TYPE(a12_lte_line) :: l(1)
+  TYPE(a12_synthe_atm) :: sa
+  TYPE(a12_atlas_atm) :: aa
+  TYPE(a12_mol2_rec) :: m
+  INTEGER(C_INT32_T) :: rc
+  PRINT *, C_SIZEOF(l(1)), C_SIZEOF(sa), C_SIZEOF(aa), C_SIZEOF(m)
+  rc = a12_synthe_opacity_run(C_NULL_PTR, A12_PREC_MIXED)
+  PRINT *, rc
+  rc = a12_atlas_linop1_run(C_NULL_PTR, A12_PREC_FP64)
+  PRINT *, rc
+END PROGRAM t
+""")
+    iface = os.path.join(REPO, "atlas12_amd", "fortran", "a12gpu_iface.f90")
+    exe = tmp_path / "t"
+    mod = ["-module-dir", str(tmp_path)] if "flang" in fc else ["-J", str(tmp_path)]
+    subprocess.run([fc, *mod, iface, str(prog), "-o", str(exe), lib._name,
+                    "-Wl,-rpath," + os.path.dirname(lib._name)], check=True, cwd=tmp_path)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    sizes = [int(x) for x in out[:4]]
+    assert sizes == [32, ctypes.sizeof(native.SyntheAtm), ctypes.sizeof(native.AtlasAtm), 32]
+    assert [int(x) for x in out[4:6]] == [-1, -1]
