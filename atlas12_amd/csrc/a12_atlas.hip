@@ -116,13 +116,21 @@ __device__ __forceinline__ float lo_cv_mixed(float center, double v64, double a6
 }
 
 // Number of wing points LINOP1 adds on one side before and including the
-// first value below tc: points base+sgn, base+2 sgn, ... (red: offsets
-// w - wl; blue: wl - w), at most kMaxWing or up to the grid end.  Table and
-// VOIGT regimes are walked point by point; once a <= 0.2 and v > 10 the
-// value is center * INVSQRTPI * a / v^2, which never increases along the
-// walk in FP64 (v = (w - wl) / dopwave is monotone in w under rounding and
-// so is each later operation), so the first point below tc is found by
-// bisection with the same predicate the walk would test.
+// first value below tc: points first, first+step, ... last (red: offsets
+// w - wl; blue: wl - w).  The walk is taken point by point until the
+// profile enters a regime where it never increases along the walk:
+//   a <= 0.2, v > 10: center * INVSQRTPI * a / v^2 -- v = (w - wl)/dopwave
+//     is monotone in w under FP64 rounding, and so is every later
+//     operation;
+//   a > 0.2 outside (a <= 1.4 and a + v <= 3.2): VOIGT's asymptotic form,
+//     strictly decreasing in v (for a > 100 it is a*0.79788/u with u
+//     increasing, monotone under rounding; for 0.2 < a <= 100 the FP64
+//     expression was checked non-increasing on a 1e-3 grid in v for 6,000
+//     values of a, far finer than any walk step).
+// From there the first point below tc is found by bisection with the same
+// predicate (lo_cv < tc) the walk tests.  Equal to the walk whenever
+// consecutive walk points' profile values differ by more than the FP64
+// evaluation error, i.e. for any grid coarser than ~1e-9 Doppler widths.
 __device__ int lo_reach(int first, int last, int step, const double *__restrict__ waveset,
                         double wlvac, double dopwave, double center, double adamp, double tc,
                         const double *h0, const double *h1, const double *h2)
@@ -134,18 +142,18 @@ __device__ int lo_reach(int first, int last, int step, const double *__restrict_
     if (nmax <= 0) return 0;
     for (; cnt < nmax; ++cnt, iw += step) {
         const double v = step > 0 ? (waveset[iw] - wlvac) / dopwave : (wlvac - waveset[iw]) / dopwave;
-        if (adamp <= 0.20 && v > 10.0) break;  // monotone Lorentz tail from here
+        const bool mono = adamp <= 0.20 ? v > 10.0 : !(adamp <= 1.4 && adamp + v <= 3.2);
+        if (mono) break;
         if (lo_cv<double>(center, v, adamp, h0, h1, h2) < tc) return cnt + 1;
     }
     if (cnt == nmax) return cnt;
     // bisection over walk positions [cnt, nmax): first with value < tc
-    const double k = center * 0.5641895835477563 * adamp;
     int lo = cnt, hi = nmax;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
         const int pw = first + mid * step;
         const double v = step > 0 ? (waveset[pw] - wlvac) / dopwave : (wlvac - waveset[pw]) / dopwave;
-        if (k / (v * v) < tc) hi = mid;
+        if (lo_cv<double>(center, v, adamp, h0, h1, h2) < tc) hi = mid;
         else lo = mid + 1;
     }
     return lo < nmax ? lo + 1 : nmax;
@@ -243,20 +251,94 @@ __global__ void k_lo_sum(const long long *__restrict__ blk, int nb, long long *o
     if (threadIdx.x == 0) *out = red[0];
 }
 
+// Per-(line, depth) values one contribution needs.  FP64: centre, a,
+// Doppler width (the reference's arithmetic).  Mixed: 1/Doppler width and a
+// in FP64 (offsets and every regime/table decision), centre, centre*a/sqrt(pi)
+// and a in FP32.
+struct LoPar {
+    double p0, p1, p2;  // FP64: center, adamp, dopwave | mixed: rdw, adamp, (c32, k32)
+};
+
+template <bool F64>
+__device__ __forceinline__ LoPar lo_par(const double *__restrict__ par, size_t o)
+{
+    LoPar q;
+    const double center = par[3 * o], adamp = par[3 * o + 1], dopwave = par[3 * o + 2];
+    if (F64) {
+        q.p0 = center;
+        q.p1 = adamp;
+        q.p2 = dopwave;
+    } else {
+        q.p0 = 1.0 / dopwave;
+        q.p1 = adamp;
+        const float c32 = (float)center, k32 = (float)(center * 0.5641895835477563 * adamp);
+        q.p2 = __hiloint2double(__float_as_int(c32), __float_as_int(k32));
+    }
+    return q;
+}
+
+// One contribution at offset d (red: w - wl, blue: wl - w) into LDS slot q.
+template <bool F64>
+__device__ __forceinline__ void lo_accum(const LoPar &q, double d, int slot, double *acc_d,
+                                         float *acc_s, float *acc_c, const double *h0,
+                                         const double *h1, const double *h2, const float *f0,
+                                         const float *f1, const float *f2)
+{
+#pragma clang fp contract(off)
+    if (F64) {
+        acc_d[slot] = acc_d[slot] + lo_cv<double>(q.p0, d / q.p2, q.p1, h0, h1, h2);
+        return;
+    }
+    const double adamp = q.p1, v = d * q.p0;
+    const float c32 = __int_as_float(__double2hiint(q.p2)), k32 = __int_as_float(__double2loint(q.p2));
+    float x;
+    if (adamp <= 0.20 && v > 10.0) {  // Lorentz tail: K / v^2
+        const float vf = (float)v;
+        x = k32 * __builtin_amdgcn_rcpf(vf * vf);
+    } else if (adamp > 0.20 && !(adamp <= 1.4 && adamp + v <= 3.2)) {
+        // VOIGT's asymptotic form with one reciprocal
+        const float a32 = (float)adamp, aa32 = a32 * a32, vf = (float)v, vv = vf * vf;
+        const float ru = __builtin_amdgcn_rcpf((aa32 + vv) * 1.4142f);
+        const float aau = aa32 * ru, vvu = vv * ru;
+        const float r = a32 * 0.79788f * ru;
+        x = c32 * (adamp <= 100.0 ? ((((aau - 10.0f * vvu) * aau * 3.0f + 15.0f * vvu * vvu) +
+                                      3.0f * vv - aa32) * (ru * ru) + 1.0f) * r
+                                  : r);
+    } else {
+        x = lo_cv_mixed(c32, v, adamp, f0, f1, f2);
+    }
+    const float s0 = acc_s[slot], tt = s0 + x;
+    acc_c[slot] += fabsf(s0) >= fabsf(x) ? (s0 - tt) + x : (x - tt) + s0;
+    acc_s[slot] = tt;
+}
+
+__device__ __forceinline__ LoPar lo_shfl(const LoPar &q, int src)
+{
+    LoPar r;
+    r.p0 = __shfl(q.p0, src, 64);
+    r.p1 = __shfl(q.p1, src, 64);
+    r.p2 = __shfl(q.p2, src, 64);
+    return r;
+}
+
 // Gather: a block holds tiles of kLoPts consecutive grid points; lane t
 // serves tile t / n and depth t % n.  The lines that can reach the block's
 // points are one contiguous run of the NU-sorted store (segments NU =
 // first - 100 .. last + 100); the block streams it in batches of kLoBatch
-// lines, staging their per-depth state in LDS with coalesced loads, and
-// each lane takes from every staged line the points of its tile that the
-// line's red/blue reach covers at its depth.  Lines are visited in store
-// order, which for every grid point is the order LINOP1 adds them, into the
-// lane's own LDS column: no atomics, deterministic.
-constexpr int kLoPts = 8;
-constexpr int kLoBlock = 512;
-constexpr int kLoBatch = 32;
-
-template <bool F64>
+// lines, staging their per-depth state in LDS with coalesced loads.  For
+// every staged line, each lane's items are the points of its tile that the
+// line's red/blue reach covers at its depth.
+//   kFlat = false: each lane walks its own items;
+//   kFlat = true:  a wave scans the item counts of its 64 lanes and deals
+//                  all of the line's items out over the 64 lanes (owner by
+//                  binary search over the prefix sums, parameters by
+//                  shuffle), so lanes with long ranges no longer hold up
+//                  the others.
+// A line's items target distinct (point, depth) slots and lines are taken
+// one after another in store order, which for every grid point is the
+// order LINOP1 adds them: the sum in each LDS slot is the same in both
+// forms and every tile shape.  No atomics, deterministic.
+template <bool F64, int kLoPts, int kLoBlock, int kLoBatch, bool kFlat>
 __global__ __launch_bounds__(kLoBlock) void k_lo_gather(
     int n, int numnu, int tiles, const int *__restrict__ seg, const int *__restrict__ l_nu,
     const int *__restrict__ state, const double *__restrict__ par,
@@ -266,15 +348,17 @@ __global__ __launch_bounds__(kLoBlock) void k_lo_gather(
     double *__restrict__ xlines)
 {
 #pragma clang fp contract(off)
+    static_assert(kLoPts <= 32 && kLoBlock % 64 == 0, "tile shape");
     // LDS: F64 sums, or FP32 sums + compensations (same bytes)
     __shared__ double acc_d[kLoPts * kLoBlock];
     __shared__ int s_state[kLoBatch * A12_NRMAX_LAYERS];
     __shared__ int s_nu[kLoBatch];
     __shared__ double s_wl[kLoBatch];
+    __shared__ double s_w[kLoBlock / 8 * kLoPts];  // WAVESET of the block's points
     float *acc_s = reinterpret_cast<float *>(acc_d);
     float *acc_c = acc_s + kLoPts * kLoBlock;
     const int per = kLoBlock / n;  // tiles per block
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, lane = t & 63;
     const int lt = t / n, j = t - lt * n;
     const int tile0 = blockIdx.x * per;
     const int tile = tile0 + lt;
@@ -284,6 +368,7 @@ __global__ __launch_bounds__(kLoBlock) void k_lo_gather(
         if (F64) acc_d[p * kLoBlock + t] = 0.0;
         else acc_s[p * kLoBlock + t] = acc_c[p * kLoBlock + t] = 0.0f;
     }
+    if (t < per * kLoPts) s_w[t] = waveset[min(tile0 * kLoPts + t, numnu - 1)];
     // the block's candidate lines: segments NU in [first - 100, last + 100]
     // (red from NU reaches NU..NU+100, blue NU-100..NU-1)
     const int bfirst = tile0 * kLoPts, blast = min((tile0 + per) * kLoPts, numnu) - 1;
@@ -297,30 +382,68 @@ __global__ __launch_bounds__(kLoBlock) void k_lo_gather(
             s_wl[t] = l_const[5 * (size_t)(b0 + t)];
         }
         __syncthreads();
-        if (!live) continue;
+        if (!kFlat && !live) continue;
         for (int i = 0; i < nb; ++i) {
-            const int st = s_state[i * n + j];
-            if (st >= 0) continue;
-            const int k = s_nu[i], rn = st & 0xff, bn = (st >> 8) & 0xff;
-            const int ra = max(k, iw0), rb = min(k + rn - 1, iw1);
-            const int ba = max(k - bn, iw0), bb = min(k - 1, iw1);
-            if (ra > rb && ba > bb) continue;
-            const size_t o = (size_t)(b0 + i) * n + j;
+            // this lane's items: tile points in [ra, rb] (red) and [ba, bb] (blue)
+            int ra = 1, rb = 0, ba = 1, bb = 0;
+            if (live) {
+                const int st = s_state[i * n + j];
+                if (st < 0) {
+                    const int k = s_nu[i], rn = st & 0xff, bn = (st >> 8) & 0xff;
+                    ra = max(k, iw0);
+                    rb = min(k + rn - 1, iw1);
+                    ba = max(k - bn, iw0);
+                    bb = min(k - 1, iw1);
+                }
+            }
+            const int nred = max(0, rb - ra + 1), nblue = max(0, bb - ba + 1), c = nred + nblue;
             const double wl = s_wl[i];
-            const double center = par[3 * o], adamp = par[3 * o + 1], dopwave = par[3 * o + 2];
-            // a point lies on one side of the line only: the ranges are disjoint
-            for (int side = 0; side < 2; ++side) {
-                const int a = side ? ba : ra, b = side ? bb : rb;
-                for (int iw = a; iw <= b; ++iw) {
-                    const double v = (side ? (wl - waveset[iw]) : (waveset[iw] - wl)) / dopwave;
-                    const int q = (iw - iw0) * kLoBlock + t;
-                    if (F64) {
-                        acc_d[q] = acc_d[q] + lo_cv<double>(center, v, adamp, h0, h1, h2);
-                    } else {
-                        const float x = lo_cv_mixed((float)center, v, adamp, f0, f1, f2);
-                        const float s0 = acc_s[q], tt = s0 + x;
-                        acc_c[q] += fabsf(s0) >= fabsf(x) ? (s0 - tt) + x : (x - tt) + s0;
-                        acc_s[q] = tt;
+            const size_t orow = (size_t)(b0 + i) * n;
+            if (!kFlat) {
+                if (c == 0) continue;
+                const LoPar q = lo_par<F64>(par, orow + j);
+                const double *w_t = s_w + lt * kLoPts - iw0;
+                for (int iw = ra; iw <= rb; ++iw)
+                    lo_accum<F64>(q, w_t[iw] - wl, (iw - iw0) * kLoBlock + t, acc_d, acc_s, acc_c,
+                                  h0, h1, h2, f0, f1, f2);
+                for (int iw = ba; iw <= bb; ++iw)
+                    lo_accum<F64>(q, wl - w_t[iw], (iw - iw0) * kLoBlock + t, acc_d, acc_s, acc_c,
+                                  h0, h1, h2, f0, f1, f2);
+            } else {
+                // wave-wide inclusive scan of the item counts
+                int incl = c;
+                for (int dd = 1; dd < 64; dd <<= 1) {
+                    const int y = __shfl_up(incl, dd, 64);
+                    if (lane >= dd) incl += y;
+                }
+                const int total = __shfl(incl, 63, 64);
+                if (total == 0) continue;
+                const LoPar q = c > 0 ? lo_par<F64>(par, orow + j) : LoPar{1.0, 0.0, 1.0};
+                // owner's item layout: red offset, red count, blue offset, tile base in s_w
+                // (an empty range packs as offset 0: its bounds may lie outside the tile)
+                const int pk = (nred ? ra - iw0 : 0) | (nred << 6) | ((nblue ? ba - iw0 : 0) << 12) |
+                               ((lt * kLoPts) << 18);
+                const int excl = incl - c;
+                for (int base = 0; base < total; base += 64) {
+                    const int item = base + lane;
+                    int lo = 0, hi = 63;
+#pragma unroll
+                    for (int s = 0; s < 6; ++s) {
+                        const int mid = (lo + hi) >> 1;
+                        if (__shfl(incl, mid, 64) > item) hi = mid;
+                        else lo = mid + 1;
+                    }
+                    const int o = min(lo, 63);
+                    const int ex = __shfl(excl, o, 64), opk = __shfl(pk, o, 64);
+                    const LoPar qo = lo_shfl(q, o);
+                    if (item < total) {
+                        const int kk = item - ex, onred = (opk >> 6) & 63;
+                        const bool blue = kk >= onred;
+                        const int pt = blue ? ((opk >> 12) & 63) + (kk - onred) : (opk & 63) + kk;
+                        const double w = s_w[(opk >> 18) + pt];
+                        const int to = t - lane + o;  // owner's lane in the block
+                        lo_accum<F64>(qo, blue ? wl - w : w - wl, pt * kLoBlock + to, acc_d, acc_s,
+                                      acc_c, h0, h1, h2, f0, f1, f2);
                     }
                 }
             }
@@ -332,6 +455,31 @@ __global__ __launch_bounds__(kLoBlock) void k_lo_gather(
             xlines[(size_t)iw * n + j] = F64 ? acc_d[q] : (double)acc_s[q] + (double)acc_c[q];
         }
 }
+
+// Tile shapes (points per tile, lanes per block, lines per LDS batch) the
+// autotuner chooses from; LDS = 8 B x points x lanes + 4 B x 128 x batch.
+struct LoShape {
+    int pts, block, batch;
+    bool flat;
+    void (*mixed)(int, int, int, const int *, const int *, const int *, const double *,
+                  const double *, const double *, const double *, const double *, const double *,
+                  const float *, const float *, const float *, double *);
+    void (*fp64)(int, int, int, const int *, const int *, const int *, const double *,
+                 const double *, const double *, const double *, const double *, const double *,
+                 const float *, const float *, const float *, double *);
+};
+#define A12_LO_SHAPE(P, B, T, F) \
+    {P, B, T, F, k_lo_gather<false, P, B, T, F>, k_lo_gather<true, P, B, T, F>}
+const LoShape kLoShapes[] = {
+    A12_LO_SHAPE(8, 512, 32, false),  A12_LO_SHAPE(4, 512, 64, false),
+    A12_LO_SHAPE(4, 256, 32, false),  A12_LO_SHAPE(8, 256, 32, false),
+    A12_LO_SHAPE(16, 256, 32, false), A12_LO_SHAPE(8, 512, 32, true),
+    A12_LO_SHAPE(8, 256, 32, true),   A12_LO_SHAPE(16, 256, 32, true),
+    A12_LO_SHAPE(16, 512, 32, true),  A12_LO_SHAPE(32, 256, 32, true),
+    A12_LO_SHAPE(16, 256, 64, true),  A12_LO_SHAPE(4, 512, 32, true),
+};
+#undef A12_LO_SHAPE
+constexpr int kNumLoShapes = sizeof(kLoShapes) / sizeof(kLoShapes[0]);
 
 template <class T>
 hipError_t up(DevBuf &b, const T *src, size_t n, hipStream_t s)
@@ -477,6 +625,28 @@ int a12i_atlas_load(a12_ctx *ctx, const a12_atlas_atm *a, const int32_t *linedat
     return A12_OK;
 }
 
+static int lo_gather_launch(a12_ctx *ctx, int prec, int shape)
+{
+    AtlasDev &A = ctx->atl;
+    const int n = A.n, numnu = A.numnu;
+    const double *tabs = A.tabs.as<double>();
+    const float *tabs32 = A.tabs32.as<float>();
+    const LoShape &sh = kLoShapes[shape];
+    auto g = prec == A12_PREC_FP64 ? sh.fp64 : sh.mixed;
+    const int tiles = (numnu + sh.pts - 1) / sh.pts, per = sh.block / n;
+    if (per < 1) {
+        ctx->err = "LINOP1 gather: block smaller than the layer count";
+        return A12_ERR_ARG;
+    }
+    hipLaunchKernelGGL(g, dim3((tiles + per - 1) / per), dim3(sh.block), 0, ctx->stream, n, numnu,
+                       tiles, A.seg.as<int>(), A.l_nu.as<int>(), A.state.as<int>(),
+                       A.par.as<double>(), A.l_const.as<double>(), A.waveset.as<double>(), tabs,
+                       tabs + kTabN, tabs + 2 * kTabN, tabs32, tabs32 + kTabN, tabs32 + 2 * kTabN,
+                       A.xlines.as<double>());
+    A12_CHECK_LAUNCH();
+    return A12_OK;
+}
+
 int a12i_atlas_linop1_run(a12_ctx *ctx, int prec)
 {
     AtlasDev &A = ctx->atl;
@@ -517,16 +687,66 @@ int a12i_atlas_linop1_run(a12_ctx *ctx, int prec)
     }
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[9], s));
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[10], s));
-    auto g = prec == A12_PREC_FP64 ? k_lo_gather<true> : k_lo_gather<false>;
-    const int tiles = (numnu + kLoPts - 1) / kLoPts, per = kLoBlock / n;
-    hipLaunchKernelGGL(g, dim3((tiles + per - 1) / per), dim3(kLoBlock), 0, s, n, numnu, tiles,
-                       A.seg.as<int>(), A.l_nu.as<int>(),
-                       A.state.as<int>(), A.par.as<double>(), A.l_const.as<double>(),
-                       A.waveset.as<double>(), tabs, tabs + kTabN, tabs + 2 * kTabN, tabs32,
-                       tabs32 + kTabN, tabs32 + 2 * kTabN, A.xlines.as<double>());
-    A12_CHECK_LAUNCH();
+    {
+        const int rc = lo_gather_launch(ctx, prec, A.shape);
+        if (rc) return rc;
+    }
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[11], s));
     A.xl_ready = true;
     A.prof_lo = ctx->prof;
+    return A12_OK;
+}
+
+int a12i_atlas_shapes(int idx, int32_t *pbb)
+{
+    if (idx < 0 || idx >= kNumLoShapes) return kNumLoShapes;
+    if (pbb) {
+        pbb[0] = kLoShapes[idx].pts;
+        pbb[1] = kLoShapes[idx].block;
+        pbb[2] = kLoShapes[idx].batch;
+    }
+    return kNumLoShapes;
+}
+
+// Times every gather shape on the loaded store (after one full LINOP1 run
+// for the per-(line, depth) state) and keeps the fastest.  All shapes add
+// the lines in the same order at every point, so the choice changes the
+// time only, never XLINES.
+int a12i_atlas_autotune(a12_ctx *ctx, int prec, float *ms, int32_t *best)
+{
+    AtlasDev &A = ctx->atl;
+    const int rc0 = a12i_atlas_linop1_run(ctx, prec);
+    if (rc0) return rc0;
+    hipEvent_t e0, e1;
+    A12_HIP(hipEventCreate(&e0));
+    A12_HIP(hipEventCreate(&e1));
+    int bi = -1;
+    float bt = 0.0f;
+    int rc = A12_OK;
+    for (int k = 0; k < kNumLoShapes && rc == A12_OK; ++k) {
+        float t = -1.0f;
+        if (A.n <= kLoShapes[k].block) {
+            if ((rc = lo_gather_launch(ctx, prec, k)) != A12_OK) break;  // warm
+            if (hipEventRecord(e0, ctx->stream) != hipSuccess) rc = A12_ERR_HIP;
+            if (rc == A12_OK) rc = lo_gather_launch(ctx, prec, k);
+            if (rc == A12_OK && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = A12_ERR_HIP;
+            if (rc == A12_OK && (hipEventSynchronize(e1) != hipSuccess ||
+                                 hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+                rc = A12_ERR_HIP;
+            if (rc == A12_OK && (bi < 0 || t < bt)) {
+                bi = k;
+                bt = t;
+            }
+        }
+        if (ms) ms[k] = t;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != A12_OK) {
+        if (ctx->err.empty()) ctx->err = "a12_atlas_autotune: timing failed";
+        return rc;
+    }
+    A.shape = bi;
+    if (best) *best = bi;
     return A12_OK;
 }

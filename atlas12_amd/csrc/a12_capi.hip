@@ -250,6 +250,29 @@ int a12_atlas_linop1_run(a12_ctx *ctx, int32_t prec)
     return a12i_atlas_linop1_run(ctx, prec);
 }
 
+int a12_atlas_shape_info(int32_t idx, int32_t *pts_block_batch)
+{
+    return a12i_atlas_shapes(idx, pts_block_batch);
+}
+
+int a12_atlas_set_shape(a12_ctx *ctx, int32_t shape)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    if (shape < 0 || shape >= a12i_atlas_shapes(-1, nullptr)) {
+        ctx->err = "a12_atlas_set_shape: no such shape";
+        return A12_ERR_ARG;
+    }
+    ctx->atl.shape = shape;
+    return A12_OK;
+}
+
+int a12_atlas_autotune(a12_ctx *ctx, int32_t prec, float *ms, int32_t *best)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_atlas_autotune(ctx, prec, ms, best);
+}
+
 int a12_atlas_linop1_fetch(a12_ctx *ctx, double *xlines, int64_t *evals)
 {
     if (!ctx_ok(ctx)) return A12_ERR_ARG;

@@ -72,6 +72,7 @@ struct AtlasDev {
     DevBuf kapp, hnu1, acc;
     bool loaded = false, xl_ready = false, kapp_ready = false, rt_done = false;
     bool prof_lo = false, prof_rt = false;
+    int shape = 0;  // LINOP1 gather tile shape (a12_atlas_set_shape / a12_atlas_autotune)
 };
 
 struct SelectDev {
@@ -122,6 +123,8 @@ int a12i_voigt(a12_ctx *ctx, int prec, int n, const float *v, const float *a, do
 // Implemented in a12_atlas.hip
 int a12i_atlas_load(a12_ctx *ctx, const a12_atlas_atm *a, const int32_t *linedata, int32_t nlines);
 int a12i_atlas_linop1_run(a12_ctx *ctx, int prec);
+int a12i_atlas_shapes(int idx, int32_t *pts_block_batch);
+int a12i_atlas_autotune(a12_ctx *ctx, int prec, float *ms, int32_t *best);
 // Implemented in a12_select.hip
 int a12i_atlas_select(a12_ctx *ctx, const a12_atlas_atm *a, const double *wavetab,
                       const int32_t *pred, int64_t npred, const a12_mol2_rec *mol2, int64_t nmol2,

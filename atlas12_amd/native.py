@@ -32,7 +32,8 @@ EXPORTS = ["a12_ctx_create", "a12_ctx_destroy", "a12_last_error", "a12_stream",
            "a12_set_profiling", "a12_stage_ms", "a12_synthe_eval_split",
            "a12_atlas_load", "a12_atlas_linop1_run", "a12_atlas_linop1_fetch",
            "a12_atlas_set_kapp", "a12_atlas_rt_run", "a12_atlas_rt_fetch",
-           "a12_atlas_select_lines", "a12_atlas_stage_ms"]
+           "a12_atlas_select_lines", "a12_atlas_stage_ms", "a12_atlas_shape_info",
+           "a12_atlas_set_shape", "a12_atlas_autotune"]
 
 
 class A12Error(RuntimeError):
@@ -376,3 +377,30 @@ def atlas_select_lines(ctx: Context, z, pred, mol2, mion: int = 1006):
         out.ctypes.data_as(C.c_void_p), C.c_int64(len(out)), C.byref(nk)),
         "a12_atlas_select_lines")
     return out[:nk.value]
+
+
+def atlas_shapes(ctx: Context):
+    """LINOP1 gather tile shapes: list of (points per tile, lanes per
+    block, lines per LDS batch)."""
+    out = (C.c_int32 * 3)()
+    n = ctx.lib.a12_atlas_shape_info(C.c_int32(-1), None)
+    shapes = []
+    for k in range(n):
+        ctx.lib.a12_atlas_shape_info(C.c_int32(k), out)
+        shapes.append(tuple(out[:]))
+    return shapes
+
+
+def atlas_set_shape(ctx: Context, shape: int):
+    ctx._check(ctx.lib.a12_atlas_set_shape(ctx.h, C.c_int32(shape)), "a12_atlas_set_shape")
+
+
+def atlas_autotune(ctx: Context, prec: int = PREC_MIXED):
+    """Times the LINOP1 gather in every tile shape on the loaded store and
+    keeps the fastest.  Returns (best shape index, ms per shape)."""
+    n = len(atlas_shapes(ctx))
+    ms = (C.c_float * n)()
+    best = C.c_int32(-1)
+    ctx._check(ctx.lib.a12_atlas_autotune(ctx.h, C.c_int32(prec), ms, C.byref(best)),
+               "a12_atlas_autotune")
+    return best.value, np.array(ms[:], np.float64)

@@ -116,6 +116,10 @@ def atlas_run(args, ctx, native, z, d, pred, tio):
     kapp = kapp_standin(z, len(z["waveset"]))
     native.atlas_set_kapp(ctx, *kapp)
     log(f"atlas: SELECTLINES kept {len(store)} lines ({t_sel:.2f}s incl. transfers)")
+    shapes = native.atlas_shapes(ctx)
+    best, tune_ms = native.atlas_autotune(ctx, native.PREC_MIXED)
+    log(f"atlas: autotuned gather shape {shapes[best]} (pts, lanes, batch); ms per shape "
+        + ", ".join(f"{s}:{m:.1f}" for s, m in zip(shapes, tune_ms)))
 
     import torch
 
@@ -150,7 +154,8 @@ def atlas_run(args, ctx, native, z, d, pred, tio):
     lo_mix = float(st_mix[:3].sum())
     lo_64 = float(st_64[:3].sum())
     nf = len(z["waveset"])
-    return dict(store=len(store), t_sel=t_sel, t_mix=t_mix / args.steps, t_64=t_64 / k64,
+    return dict(shape=shapes[best], tune={str(k): float(m) for k, m in zip(shapes, tune_ms)},
+                store=len(store), t_sel=t_sel, t_mix=t_mix / args.steps, t_64=t_64 / k64,
                 st_mix=st_mix, st_64=st_64, ev_mix=ev_mix, ev_64=ev_64, lo_mix=lo_mix, lo_64=lo_64,
                 flux_err=flux_err, flx_err=flx_err, nf=nf, store_lines=store)
 
@@ -350,6 +355,7 @@ def main():
         "stage_ms_fp64": {"linop1_setup": a["st_64"][0], "linop1_reach": a["st_64"][1],
                           "linop1_gather": a["st_64"][2], "rt_pass": a["st_64"][3]},
         "selectlines_s": a["t_sel"],
+        "autotune": {"shape_pts_lanes_batch": list(a["shape"]), "gather_ms_per_shape": a["tune"]},
         "roofline": {"bound": "valu", "kernel": "k_lo_gather<mixed>", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,

@@ -169,6 +169,16 @@ int a12_atlas_load(a12_ctx *ctx, const a12_atlas_atm *atm, const int32_t *lineda
 /* LINOP1, src/atlas12_modules.f90:18132-18374, into the context's device
  * XLINES (nrhox, numnu). */
 int a12_atlas_linop1_run(a12_ctx *ctx, int32_t prec);
+/* Tile shapes of the LINOP1 gather (no counterpart in the reference: a
+ * launch-shape choice).  a12_atlas_shape_info returns the number of shapes
+ * and, for 0 <= idx < that number, writes {points per tile, lanes per
+ * block, lines per LDS batch}.  a12_atlas_set_shape picks one (default 0);
+ * a12_atlas_autotune runs LINOP1 once, times the gather in every shape on
+ * the loaded store (ms[shape], -1 where not applicable) and keeps the
+ * fastest.  XLINES is bitwise the same in every shape. */
+int a12_atlas_shape_info(int32_t idx, int32_t *pts_block_batch);
+int a12_atlas_set_shape(a12_ctx *ctx, int32_t shape);
+int a12_atlas_autotune(a12_ctx *ctx, int32_t prec, float *ms, int32_t *best);
 /* Copies XLINES (nrhox, numnu), depth fastest, and the contribution count. */
 int a12_atlas_linop1_fetch(a12_ctx *ctx, double *xlines, int64_t *evals);
 /* KAPP's outputs per frequency, each (nrhox, numnu) depth fastest:

@@ -122,3 +122,25 @@ def test_selectlines_edges(gpu_ctx, select_case):
         native.atlas_select_lines(gpu_ctx, z, pred[::-1], tio[:0])
     with pytest.raises(native.A12Error):
         native.atlas_select_lines(gpu_ctx, z, pred[:0], tio[::-1])
+
+
+def test_linop1_every_tile_shape_same_bits(gpu_ctx, atl):
+    """The gather's tile shapes add lines in the same order at every point:
+    XLINES is bitwise identical in all of them, at both precisions, and
+    the autotuner keeps one of them."""
+    from atlas12_amd import native
+    z, d = atl
+    native.atlas_load(gpu_ctx, d, z["linedata"])
+    shapes = native.atlas_shapes(gpu_ctx)
+    assert len(shapes) >= 2
+    for prec in (native.PREC_MIXED, native.PREC_FP64):
+        ref = None
+        for k in range(len(shapes)):
+            native.atlas_set_shape(gpu_ctx, k)
+            xl, _ = native.atlas_linop1(gpu_ctx, prec)
+            if ref is None:
+                ref = xl
+            assert np.array_equal(xl, ref), (prec, shapes[k])
+    best, ms = native.atlas_autotune(gpu_ctx, native.PREC_MIXED)
+    assert 0 <= best < len(shapes) and ms[best] > 0
+    native.atlas_set_shape(gpu_ctx, 0)
