@@ -704,6 +704,7 @@ int a12i_atlas_shapes(int idx, int32_t *pbb)
         pbb[0] = kLoShapes[idx].pts;
         pbb[1] = kLoShapes[idx].block;
         pbb[2] = kLoShapes[idx].batch;
+        pbb[3] = kLoShapes[idx].flat;
     }
     return kNumLoShapes;
 }

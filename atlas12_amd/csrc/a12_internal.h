@@ -123,7 +123,7 @@ int a12i_voigt(a12_ctx *ctx, int prec, int n, const float *v, const float *a, do
 // Implemented in a12_atlas.hip
 int a12i_atlas_load(a12_ctx *ctx, const a12_atlas_atm *a, const int32_t *linedata, int32_t nlines);
 int a12i_atlas_linop1_run(a12_ctx *ctx, int prec);
-int a12i_atlas_shapes(int idx, int32_t *pts_block_batch);
+int a12i_atlas_shapes(int idx, int32_t *pts_block_batch_flat);
 int a12i_atlas_autotune(a12_ctx *ctx, int prec, float *ms, int32_t *best);
 // Implemented in a12_select.hip
 int a12i_atlas_select(a12_ctx *ctx, const a12_atlas_atm *a, const double *wavetab,

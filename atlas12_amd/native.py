@@ -381,8 +381,8 @@ def atlas_select_lines(ctx: Context, z, pred, mol2, mion: int = 1006):
 
 def atlas_shapes(ctx: Context):
     """LINOP1 gather tile shapes: list of (points per tile, lanes per
-    block, lines per LDS batch)."""
-    out = (C.c_int32 * 3)()
+    block, lines per LDS batch, flat)."""
+    out = (C.c_int32 * 4)()
     n = ctx.lib.a12_atlas_shape_info(C.c_int32(-1), None)
     shapes = []
     for k in range(n):
