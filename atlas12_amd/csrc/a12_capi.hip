@@ -367,6 +367,15 @@ int a12_atlas_rt_fetch(a12_ctx *ctx, double *acc, double *hnu1)
     return A12_OK;
 }
 
+int a12_scale_model(a12_ctx *ctx, int32_t nmodel, int32_t nrhox, int32_t kw, double tau1lg,
+                    double steplg, const double *sc, const double *fields, double *out,
+                    double *so, double *taustd)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_scale_model(ctx, nmodel, nrhox, kw, tau1lg, steplg, sc, fields, out, so, taustd);
+}
+
 int a12_set_profiling(a12_ctx *ctx, int32_t on)
 {
     if (!ctx_ok(ctx)) return A12_ERR_ARG;

@@ -154,7 +154,9 @@ __device__ __forceinline__ float voigt_mixed(float v, float a)
     return voigt_f32(x, y);
 }
 
-// Integer power by repeated squaring -- how Fortran evaluates REAL**INTEGER
+// Integer power by repeated squaring -- how the compiled reference evaluates
+// REAL(8)**INTEGER with a run-time exponent (a constant exponent is expanded
+// into left-to-right products; the two agree up to the cube).
 // (wave = wbegin * ratio**(i-1), src/synthe.f90:495).
 __device__ __forceinline__ double powi_f64(double x, int m)
 {

@@ -134,6 +134,9 @@ int a12i_atlas_rt_pass(a12_ctx *ctx, int n, int nf, double teff, const double *d
                        const double *d_rcoset, const double *d_depth3, const double *d_kapp,
                        const double *d_xlines, double *d_hnu1, double *d_acc);
 int a12i_synthe_transfer(a12_ctx *ctx);
+int a12i_scale_model(a12_ctx *ctx, int nmodel, int n, int kw, double tau1lg, double steplg,
+                     const double *sc, const double *fields, double *out, double *so,
+                     double *taustd_out);
 int a12i_josh_batch(a12_ctx *ctx, int ifscat, int ifsurf, int nrhox, int nbatch,
                     const double *rhox, const double *const in[7], double *hnu1,
                     double *knu1, double *const out[5]);

@@ -19,6 +19,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 #include "a12_common.h"
 #include "a12_internal.h"
 
@@ -882,6 +886,52 @@ __global__ void k_rt_reduce(int n, int nchunk, const double *__restrict__ part,
     out[i] = s;
 }
 
+// ---- SCALE_MODEL ----------------------------------------------------------
+// SCALE_MODEL(TEFF_NEW, LOGG_NEW), src/atlas12_modules.f90:6967-7060, one
+// wavefront per model: TAUROS = INTEG(RHOX, ABROSS) from ABROSS(1) RHOX(1),
+// every field MAP1-ed from TAUROS onto TAUSTD, PRADK = PRAD + PRADK0, and,
+// where the model is rescaled, T (T TEFF_NEW)/TEFF, PRAD and PRADK times
+// (TEFF_NEW/TEFF)**4, PTOTAL = P + PRAD (PTURB = 0), RHOX = PTOTAL/GRAV and
+// PTOTAL + PZERO.  mp per model: rescale flag, TEFF_NEW, TEFF, r4, GNEW,
+// PRADK0 (old), PZERO (new).
+__global__ __launch_bounds__(64) void k_scale_model(int nmodel, int n, int kw,
+                                                    const double *__restrict__ taustd,
+                                                    const double *__restrict__ mp,
+                                                    const double *__restrict__ fields,
+                                                    double *__restrict__ out)
+{
+#pragma clang fp contract(off)
+    __shared__ JoshLDS W;
+    const int ln = lane_id();
+    for (int m = blockIdx.x; m < nmodel; m += gridDim.x) {
+        const double *F = fields + (size_t)m * 7 * n;
+        double *O = out + (size_t)m * 9 * kw;
+        const double *q = mp + (size_t)m * 8;
+        integ_wave(W, F, F + 4 * n, W.taunu, n, F[4 * n] * F[0]);
+        for (int f = 0; f < 7; ++f) (void)map1_wave(W.taunu, F + f * n, n, taustd, O + f * kw, kw);
+        wsync();
+        const bool resc = q[0] != 0.0;
+        const double tn = q[1], t0 = q[2], r4 = q[3], gnew = q[4], pradk0 = q[5], pzero = q[6];
+        for (int i = ln; i < kw; i += 64) {
+            double prad = O[5 * kw + i];
+            double pradk = prad + pradk0;
+            double ptot = 0.0;
+            if (resc) {
+                O[1 * kw + i] = O[1 * kw + i] * tn / t0;
+                pradk = pradk * r4;
+                prad = prad * r4;
+                O[5 * kw + i] = prad;
+                ptot = O[2 * kw + i] + prad + 0.0;
+                O[0 * kw + i] = ptot / gnew;
+                ptot = ptot + pzero;
+            }
+            O[7 * kw + i] = pradk;
+            O[8 * kw + i] = ptot;
+        }
+        wsync();
+    }
+}
+
 }  // namespace
 
 int a12i_synthe_transfer(a12_ctx *ctx)
@@ -974,5 +1024,62 @@ int a12i_atlas_rt_pass(a12_ctx *ctx, int n, int nf, double teff, const double *d
                        nchunk, ctx->rt_part.as<double>(), d_acc);
     A12_CHECK_LAUNCH();
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[13], ctx->stream));
+    return A12_OK;
+}
+
+int a12i_scale_model(a12_ctx *ctx, int nmodel, int n, int kw, double tau1lg, double steplg,
+                     const double *sc, const double *fields, double *out, double *so,
+                     double *taustd_out)
+{
+    if (nmodel < 0 || n < 4 || n > A12_NRMAX || kw < 2 || kw > A12_NRMAX || !sc || !fields ||
+        !out || !so) {
+        ctx->err = "a12_scale_model: bad dimensions or null array";
+        return A12_ERR_ARG;
+    }
+    // TAUSTD(I) = 10**(TAU1LG + (I-1)*STEPLG) and the per-model scalars of
+    // step 4, with the host libm pow and the products in the reference's order
+    std::vector<double> tau(kw), mp(8 * (size_t)std::max(nmodel, 1));
+    for (int i = 0; i < kw; ++i) tau[i] = std::pow(10.0, tau1lg + (double)i * steplg);
+    const double fourpi = 4.0 * 3.141592653589793, sigma_sb = 5.670374419e-5;
+    for (int m = 0; m < nmodel; ++m) {
+        const double *c = sc + 8 * (size_t)m;
+        const double teff = c[0], grav = c[1], pradk0 = c[2], pcon = c[3], pturb0 = c[4],
+                     tn = c[5], gnew = std::pow(10.0, c[6]);
+        const bool resc = !(tn == 0.0) &&
+                          !(std::fabs(tn - teff) < 1.0 && std::fabs(gnew - grav) / grav < 1.0e-3);
+        const double x = tn / teff, r4 = ((x * x) * x) * x;  // constant **4: left to right
+        double *q = &mp[8 * (size_t)m];
+        q[0] = resc ? 1.0 : 0.0;
+        q[1] = tn;
+        q[2] = teff;
+        q[3] = r4;
+        q[4] = gnew;
+        q[5] = pradk0;
+        q[6] = pcon + pradk0 * r4 + pturb0;
+        double *o = so + 6 * (size_t)m;
+        o[0] = resc ? tn : teff;
+        o[1] = resc ? gnew : grav;
+        o[2] = resc ? std::log10(gnew) : NAN;
+        o[3] = resc ? sigma_sb / fourpi * (((tn * tn) * tn) * tn) : NAN;
+        o[4] = resc ? pradk0 * r4 : pradk0;
+        o[5] = resc ? q[6] : NAN;
+    }
+    if (taustd_out) std::copy(tau.begin(), tau.end(), taustd_out);
+    if (nmodel == 0) return A12_OK;
+    const size_t fin = (size_t)7 * n * nmodel, fout = (size_t)9 * kw * nmodel;
+    A12_HIP(ctx->jb_in.reserve(sizeof(double) * (fin + kw + mp.size())));
+    A12_HIP(ctx->jb_out.reserve(sizeof(double) * fout));
+    double *din = ctx->jb_in.as<double>();
+    A12_HIP(hipMemcpyAsync(din, fields, sizeof(double) * fin, hipMemcpyHostToDevice, ctx->stream));
+    A12_HIP(hipMemcpyAsync(din + fin, tau.data(), sizeof(double) * kw, hipMemcpyHostToDevice,
+                           ctx->stream));
+    A12_HIP(hipMemcpyAsync(din + fin + kw, mp.data(), sizeof(double) * mp.size(),
+                           hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_scale_model, dim3(std::min(nmodel, 4096)), dim3(64), 0, ctx->stream,
+                       nmodel, n, kw, din + fin, din + fin + kw, din, ctx->jb_out.as<double>());
+    A12_CHECK_LAUNCH();
+    A12_HIP(hipMemcpyAsync(out, ctx->jb_out.p, sizeof(double) * fout, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    A12_HIP(hipStreamSynchronize(ctx->stream));
     return A12_OK;
 }

@@ -33,7 +33,7 @@ EXPORTS = ["a12_ctx_create", "a12_ctx_destroy", "a12_last_error", "a12_stream",
            "a12_atlas_load", "a12_atlas_linop1_run", "a12_atlas_linop1_fetch",
            "a12_atlas_set_kapp", "a12_atlas_rt_run", "a12_atlas_rt_fetch",
            "a12_atlas_select_lines", "a12_atlas_stage_ms", "a12_atlas_shape_info",
-           "a12_atlas_set_shape", "a12_atlas_autotune"]
+           "a12_atlas_set_shape", "a12_atlas_autotune", "a12_scale_model"]
 
 
 class A12Error(RuntimeError):
@@ -404,3 +404,25 @@ def atlas_autotune(ctx: Context, prec: int = PREC_MIXED):
     ctx._check(ctx.lib.a12_atlas_autotune(ctx.h, C.c_int32(prec), ms, C.byref(best)),
                "a12_atlas_autotune")
     return best.value, np.array(ms[:], np.float64)
+
+
+def scale_model(ctx: Context, sc, fields, kw: int = 80, tau1lg: float = -6.875,
+                steplg: float = 0.125):
+    """SCALE_MODEL (src/atlas12_modules.f90:6967-7060) for a batch of models.
+    sc (nmodel, 7+): TEFF, GRAV, PRADK0, PCON, PTURB0, TEFF_NEW, LOGG_NEW;
+    fields (nmodel, 7, nrhox): RHOX, T, P, XNE, ABROSS, PRAD, VTURB.
+    Returns (out (nmodel, 9, kw): the 7 fields, PRADK, PTOTAL; so (nmodel,
+    6): TEFF, GRAV, GLOG, FLUX, PRADK0, PZERO; TAUSTD (kw))."""
+    f = np.ascontiguousarray(fields, np.float64)
+    nm, nf, n = f.shape
+    if nf != 7:
+        raise ValueError("fields must be (nmodel, 7, nrhox)")
+    s8 = np.zeros((nm, 8))
+    s8[:, :7] = np.asarray(sc, np.float64)[:, :7]
+    out = np.zeros((nm, 9, kw))
+    so = np.zeros((nm, 6))
+    tau = np.zeros(kw)
+    ctx._check(ctx.lib.a12_scale_model(ctx.h, C.c_int32(nm), C.c_int32(n), C.c_int32(kw),
+                                       C.c_double(tau1lg), C.c_double(steplg), _ptr(s8), _ptr(f),
+                                       _ptr(out), _ptr(so), _ptr(tau)), "a12_scale_model")
+    return out, so, tau

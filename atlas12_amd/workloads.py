@@ -86,6 +86,9 @@ class Atmosphere:
     p: np.ndarray
     xne: np.ndarray
     vturb: np.ndarray  # cm/s
+    abross: np.ndarray | None = None   # Rosseland opacity (default 1)
+    accrad: np.ndarray | None = None   # radiative acceleration (default 0)
+    pradk: float = 0.0                 # PRADK card
 
     @property
     def nrhox(self) -> int:
@@ -96,10 +99,12 @@ class Atmosphere:
         out = [f"TEFF   {self.teff:.0f}.  GRAVITY {self.logg:.5f} LTE",
                "TITLE SYNTHETIC",
                f"READ DECK6 {self.nrhox} RHOX,T,P,XNE,ABROSS,ACCRAD,VTURB"]
+        ab = np.ones(self.nrhox) if self.abross is None else self.abross
+        ac = np.zeros(self.nrhox) if self.accrad is None else self.accrad
         for j in range(self.nrhox):
             out.append(f" {self.rhox[j]:.8E} {self.t[j]:8.1f} {self.p[j]:.3E} {self.xne[j]:.3E}"
-                       f" {1.0:.3E} {0.0:.3E} {self.vturb[j]:.3E}")
-        out.append("PRADK 0.0000E+00")
+                       f" {ab[j]:.3E} {ac[j]:.3E} {self.vturb[j]:.3E}")
+        out.append(f"PRADK {self.pradk:.4E}")
         out.append("BEGIN                    ITERATION   1 COMPLETED")
         return "\n".join(out) + "\n"
 

@@ -223,6 +223,25 @@ int a12_atlas_select_lines(a12_ctx *ctx, const a12_atlas_atm *atm, const double 
                            const int32_t *pred, int64_t npred, const a12_mol2_rec *mol2,
                            int64_t nmol2, int32_t *linedata_out, int64_t cap, int64_t *nkept);
 
+/* ---- model start ---------------------------------------------------------- */
+/* SCALE_MODEL(TEFF_NEW, LOGG_NEW), src/atlas12_modules.f90:6967-7060, for
+ * nmodel models at once (one wavefront each): regrid onto TAUSTD(i) =
+ * 10**(tau1lg + (i-1) steplg), i = 1..kw, by INTEG of ABROSS and MAP1, then
+ * rescale to the new Teff and log g (TEFF_NEW = 0: regrid only; no rescale
+ * either when Teff changes by < 1 K and g by < 1e-3 relative).
+ *   sc     (8, nmodel): TEFF, GRAV, PRADK0, PCON, PTURB0, TEFF_NEW, LOGG_NEW, -
+ *   fields (nrhox, 7, nmodel): RHOX, T, P, XNE, ABROSS, PRAD, VTURB
+ *   out    (kw, 9, nmodel): RHOX, T, P, XNE, ABROSS, PRAD, VTURB, PRADK,
+ *          PTOTAL (0 where not rescaled: the reference leaves it untouched)
+ *   so     (6, nmodel): TEFF, GRAV, GLOG, FLUX, PRADK0, PZERO (GLOG, FLUX,
+ *          PZERO NaN where not rescaled)
+ *   taustd (kw, optional): TAUSTD.
+ * The NLTE fields BMIN/BHYD are not carried (NLTE is off by default).
+ * Host pointers, synchronous. */
+int a12_scale_model(a12_ctx *ctx, int32_t nmodel, int32_t nrhox, int32_t kw, double tau1lg,
+                    double steplg, const double *sc, const double *fields, double *out,
+                    double *so, double *taustd);
+
 /* ---- measurement --------------------------------------------------------- */
 /* Stage timing with HIP events on the context stream (off by default). */
 int a12_set_profiling(a12_ctx *ctx, int32_t on);

@@ -15,8 +15,10 @@
 
 #define KW_MAX 128
 
-/* REAL(8)**INTEGER as Fortran evaluates it: repeated squaring, the bits of
- * the exponent taken from the lowest. */
+/* REAL(8)**INTEGER with a run-time exponent as the compiled reference
+ * evaluates it: repeated squaring, the bits of the exponent taken from the
+ * lowest (a constant exponent is expanded into left-to-right products
+ * instead; the two agree up to the cube). */
 static double powi_f64(double x, int m)
 {
     unsigned n = m < 0 ? -m : m;

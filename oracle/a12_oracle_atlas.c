@@ -416,3 +416,59 @@ int64_t or_atlas_select(const or_select_in *in, const int32_t *pred, int64_t npr
     free(xm);
     return nk;
 }
+
+
+/* ------------------------------------------------------------------ */
+/* SCALE_MODEL (src/atlas12_modules.f90:6967-7060)                     */
+/* ------------------------------------------------------------------ */
+/* REAL(8)**4 with the constant exponent written in the source: the
+ * compiler expands it into products taken left to right, ((x*x)*x)*x
+ * (checked against amdflang -O3 on 20,000 arguments; a run-time exponent
+ * goes by repeated squaring instead, powi_f64). */
+static double powi4(double x)
+{
+    return ((x * x) * x) * x;
+}
+
+int or_scale_model(int n, int kw, double tau1lg, double steplg, const double *sc,
+                   double teff_new, double logg_new, const double *const in[7],
+                   double *const out[10], double *so)
+{
+    const double sigma_sb = 5.670374419e-5, fourpi = 4.0 * C_PI;
+    const double teff = sc[0], grav = sc[1], pradk0 = sc[2], pcon = sc[3], pturb0 = sc[4];
+    const double gnew = pow(10.0, logg_new);
+    double tauros[KW_MAX];
+    /* step 1: TAUSTD(I) = 10**(TAU1LG + (I-1)*STEPLG) (:6979-6981) */
+    for (int i = 0; i < kw; ++i) out[0][i] = pow(10.0, tau1lg + (double)i * steplg);
+    /* step 2: TAUROS = INTEG(RHOX, ABROSS) from ABROSS(1)*RHOX(1) (:6984) */
+    or_integ(in[0], in[4], tauros, n, in[4][0] * in[0][0]);
+    /* step 3: every field onto TAUSTD with MAP1 (:6987-6998) */
+    for (int f = 0; f < 7; ++f) (void)or_map1(tauros, in[f], n, out[0], out[1 + f], kw);
+    for (int i = 0; i < kw; ++i) out[8][i] = out[6][i] + pradk0;  /* PRADK = PRAD + PRADK0 */
+    so[0] = teff;
+    so[1] = grav;
+    so[4] = pradk0;
+    /* step 4 (:7017-7033): sentinel, or no meaningful change */
+    if (teff_new == 0.0) return 0;
+    if (fabs(teff_new - teff) < 1.0 && fabs(gnew - grav) / grav < 1.0e-3) return 0;
+    const double r4 = powi4(teff_new / teff);
+    for (int i = 0; i < kw; ++i) {
+        out[2][i] = out[2][i] * teff_new / teff;
+        out[8][i] = out[8][i] * r4;
+        out[6][i] = out[6][i] * r4;
+    }
+    const double pradk0n = pradk0 * r4;
+    const double pzero = pcon + pradk0n + pturb0;
+    for (int i = 0; i < kw; ++i) {
+        double ptotal = out[3][i] + out[6][i] + 0.0;  /* P + PRAD + PTURB (PTURB = 0) */
+        out[1][i] = ptotal / gnew;
+        out[9][i] = ptotal + pzero;
+    }
+    so[0] = teff_new;
+    so[1] = gnew;
+    so[2] = log10(gnew);
+    so[3] = sigma_sb / fourpi * powi4(teff_new);
+    so[4] = pradk0n;
+    so[5] = pzero;
+    return 1;
+}

@@ -79,6 +79,16 @@ typedef struct {
 int64_t or_atlas_select(const or_select_in *in, const int32_t *pred, int64_t npred,
                         const or_mol2_rec *mol2, int64_t nmol2, int32_t *out);
 
+/* SCALE_MODEL(TEFF_NEW, LOGG_NEW), src/atlas12_modules.f90:6967-7060.
+ * in: RHOX, T, P, XNE, ABROSS, PRAD, VTURB, each (n); sc: TEFF, GRAV,
+ * PRADK0, PCON, PTURB0.  out (kw each): TAUSTD, RHOX, T, P, XNE, ABROSS,
+ * PRAD, VTURB, PRADK, PTOTAL (PTOTAL only when rescaled); so: TEFF, GRAV,
+ * GLOG, FLUX, PRADK0, PZERO (FLUX, GLOG, PZERO as given when not rescaled).
+ * Returns 1 when rescaled, 0 when only regridded. */
+int or_scale_model(int n, int kw, double tau1lg, double steplg, const double *sc,
+                   double teff_new, double logg_new, const double *const in[7],
+                   double *const out[10], double *so);
+
 #ifdef __cplusplus
 }
 #endif

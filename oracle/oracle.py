@@ -374,3 +374,21 @@ def atlas_select(z, pred, mol2):
     if m < 0:
         raise ValueError("or_atlas_select: NELION outside 1..mion")
     return out[:m]
+
+
+def scale_model(n, kw, tau1lg, steplg, sc, teff_new, logg_new, fields):
+    """SCALE_MODEL on one model.  fields: (7, n) RHOX, T, P, XNE, ABROSS,
+    PRAD, VTURB; sc: TEFF, GRAV, PRADK0, PCON, PTURB0.  Returns (out (10, kw):
+    TAUSTD, the 7 fields, PRADK, PTOTAL; so (6): TEFF, GRAV, GLOG, FLUX,
+    PRADK0, PZERO; rescaled flag)."""
+    fin = [_f64(f) for f in fields]
+    out = np.zeros((10, kw))
+    so = np.full(6, np.nan)
+    scv = _f64(sc)
+    ins = (P_D * 7)(*[_p(f) for f in fin])
+    outs = (P_D * 10)(*[_p(out[i]) for i in range(10)])
+    L = lib()
+    L.or_scale_model.restype = C.c_int
+    r = L.or_scale_model(C.c_int(n), C.c_int(kw), C.c_double(tau1lg), C.c_double(steplg),
+                         _p(scv), C.c_double(teff_new), C.c_double(logg_new), ins, outs, _p(so))
+    return out, so, bool(r)

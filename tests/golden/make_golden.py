@@ -494,8 +494,56 @@ def golden_select():
                              "raw_sha256": digest}}
 
 
+def scale_atmosphere():
+    """The plain-SYNTHE structure with a rising Rosseland opacity, a
+    radiative acceleration and a PRADK card, so SCALE_MODEL's INTEG, MAP1
+    and radiation-pressure scaling all act on non-trivial data."""
+    atm = wl.scurve_atmosphere()
+    s = np.arange(atm.nrhox) / (atm.nrhox - 1)
+    atm.abross = np.round(10.0 ** (-2.0 + 4.0 * s ** 1.3), 3 - 0)
+    atm.accrad = 10.0 ** atm.logg * 1.0e-3 * (1.0 + s)
+    atm.pradk = 0.35
+    return atm
+
+
+def golden_scale():
+    """SCALE_MODEL (src/atlas12_modules.f90:6967-7060) after READIN on one
+    deck, for a regrid-only call, an unchanged (Teff, log g) and three
+    rescalings."""
+    atm = scale_atmosphere()
+    cases = [(0.0, 0.0), (5777.0, 4.44), (4500.0, 2.5), (7200.0, 4.8), (3500.0, 1.0)]
+    tmp = tempfile.mkdtemp(prefix="a12s_")
+    g = os.path.join(tmp, "gfall.dat")
+    open(g, "w").close()
+    root = data_root(tmp, g)
+    deck = os.path.join(tmp, "model.dat")
+    with open(deck, "w") as fh:
+        fh.write(atm.deck())
+    rec = {k: [] for k in ("params", "inp", "nout", "taustd", "tauros", "out", "scal")}
+    for tn, gn in cases:
+        out = os.path.join(tmp, "o.bin")
+        run([os.path.join(REFBIN, "ref_scale_dump"), deck, repr(tn), repr(gn), out], tmp,
+            dict(os.environ, ATLAS12=root))
+        b = open(out, "rb").read()
+        n, kw = (int(x) for x in np.frombuffer(b, "<i4", 2))
+        o = 8
+        par = np.frombuffer(b, "<f8", 9, o); o += 72
+        inp = np.frombuffer(b, "<f8", 7 * n, o).reshape(7, n); o += 56 * n
+        nout = int(np.frombuffer(b, "<i4", 1, o)[0]); o += 4
+        tt = np.frombuffer(b, "<f8", 2 * kw, o).reshape(2, kw); o += 16 * kw
+        res = np.frombuffer(b, "<f8", 9 * kw, o).reshape(9, kw); o += 72 * kw
+        sc = np.frombuffer(b, "<f8", 6, o)
+        rec["params"].append(par); rec["inp"].append(inp); rec["nout"].append(nout)
+        rec["taustd"].append(tt[0]); rec["tauros"].append(tt[1]); rec["out"].append(res)
+        rec["scal"].append(sc)
+    shutil.rmtree(tmp)
+    np.savez_compressed(os.path.join(HERE, "scale_model.npz"),
+                        **{k: np.array(v) for k, v in rec.items()})
+    return {"scale_model": {"cases": cases, "nrhox": n, "kw": kw}}
+
+
 STEPS = {"synthe_small": golden_synthe_small, "atlas_small": golden_atlas_small,
-         "state_atlas": golden_state_atlas, "select": golden_select, "state_plain": golden_state_plain,
+         "state_atlas": golden_state_atlas, "select": golden_select, "scale": golden_scale, "state_plain": golden_state_plain,
          "numerics": golden_numerics, "josh": golden_josh}
 
 

@@ -138,3 +138,25 @@ def test_selectlines_planted_mistake_is_caught(orc, select_case):
     zz = dict(z)
     zz["hckt"] = np.array(z["hckt"]) * 0.0
     assert len(orc.atlas_select(zz, pred, tio)) > len(z["selected"])
+
+
+def test_scale_model_bitwise(orc, golden):
+    """SCALE_MODEL after READIN: regrid only, an unchanged (Teff, log g) and
+    three rescalings reproduce the reference's TAUSTD, remapped structure,
+    PRADK/PTOTAL and scalars bit for bit."""
+    z = golden("scale_model.npz")
+    for k in range(len(z["params"])):
+        par, inp = z["params"][k], z["inp"][k]
+        n = inp.shape[1]
+        out, so, resc = orc.scale_model(n, 80, par[5], par[6], par[:5], par[7], par[8], inp)
+        assert np.array_equal(out[0], z["taustd"][k]), k
+        assert np.array_equal(out[1:9], z["out"][k][:8]), k
+        if resc:
+            assert np.array_equal(out[9], z["out"][k][8]), k
+            assert np.array_equal(so, z["scal"][k]), k
+        else:
+            assert np.array_equal(so[[0, 1, 4]], z["scal"][k][[0, 1, 4]]), k
+    # the regrid-only and unchanged cases do not rescale; the others do
+    assert [orc.scale_model(z["inp"][k].shape[1], 80, *z["params"][k][5:7], z["params"][k][:5],
+                            *z["params"][k][7:9], z["inp"][k])[2] for k in range(5)] == \
+        [False, False, True, True, True]
