@@ -50,8 +50,13 @@ int a12_ctx_destroy(a12_ctx *ctx)
                        &A.rcoset, &A.tabs, &A.tabs32, &A.l_nu, &A.l_nucont, &A.l_nel, &A.l_const,
                        &A.seg, &A.state, &A.par, &A.reach, &A.xlines, &A.blk, &A.evals, &A.kapp,
                        &A.hnu1, &A.acc};
+    SelectDev &L = ctx->sel;
+    DevBuf *sbufs[] = {&L.tabcont, &L.xnfdop, &L.iwt, &L.wavetab, &L.tablog, &L.xm,
+                       &L.cand, &L.flag, &L.cnt, &L.off, &L.kept, &L.mol2};
+    for (DevBuf *b : sbufs) b->release();
     for (DevBuf *b : abufs) b->release();
     for (DevBuf *b : bufs) b->release();
+    a12i_mlp_free(ctx);
     for (hipEvent_t e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -374,6 +379,63 @@ int a12_scale_model(a12_ctx *ctx, int32_t nmodel, int32_t nrhox, int32_t kw, dou
     if (!ctx_ok(ctx)) return A12_ERR_ARG;
     A12_HIP(hipSetDevice(ctx->device));
     return a12i_scale_model(ctx, nmodel, nrhox, kw, tau1lg, steplg, sc, fields, out, so, taustd);
+}
+
+int a12_mlp_init(a12_ctx *ctx, int32_t batch, uint64_t seed)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_mlp_init(ctx, batch, seed);
+}
+
+int a12_mlp_get_params(a12_ctx *ctx, float *params)
+{
+    if (!ctx_ok(ctx) || !params) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_mlp_params(ctx, params, nullptr);
+}
+
+int a12_mlp_set_params(a12_ctx *ctx, const float *params)
+{
+    if (!ctx_ok(ctx) || !params) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_mlp_params(ctx, nullptr, params);
+}
+
+int a12_mlp_set_data(a12_ctx *ctx, const float *x, const float *t, int32_t n)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_mlp_set_data(ctx, x, t, n);
+}
+
+int a12_mlp_grads(a12_ctx *ctx, float *loss, float *grads)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_mlp_grads(ctx, loss, grads);
+}
+
+int a12_mlp_train(a12_ctx *ctx, int32_t steps, float lr, float *loss)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_mlp_train(ctx, steps, lr, loss);
+}
+
+int a12_mlp_infer(a12_ctx *ctx, const float *x, int32_t n, float *y)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_mlp_infer(ctx, x, n, y);
+}
+
+int a12_mlp_gemm(a12_ctx *ctx, int32_t m, int32_t n, int32_t k, const uint16_t *a,
+                 const uint16_t *b, float *c)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_mlp_gemm_check(ctx, m, n, k, a, b, c);
 }
 
 int a12_set_profiling(a12_ctx *ctx, int32_t on)

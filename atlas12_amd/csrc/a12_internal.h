@@ -80,6 +80,8 @@ struct SelectDev {
     bool tablog_ready = false;
 };
 
+struct MlpDev;  // a12_mlp.hip
+
 struct a12_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -89,6 +91,7 @@ struct a12_ctx {
     SyntheDev syn;
     AtlasDev atl;
     SelectDev sel;
+    MlpDev *mlp = nullptr;
     DevBuf rt_part;
     // scratch for a12_josh_batch
     DevBuf jb_in, jb_out;
@@ -129,6 +132,16 @@ int a12i_atlas_autotune(a12_ctx *ctx, int prec, float *ms, int32_t *best);
 int a12i_atlas_select(a12_ctx *ctx, const a12_atlas_atm *a, const double *wavetab,
                       const int32_t *pred, int64_t npred, const a12_mol2_rec *mol2, int64_t nmol2,
                       int32_t *out, int64_t cap, int64_t *nkept);
+// Implemented in a12_mlp.hip
+int a12i_mlp_init(a12_ctx *ctx, int batch, uint64_t seed);
+int a12i_mlp_params(a12_ctx *ctx, float *get, const float *set);
+int a12i_mlp_set_data(a12_ctx *ctx, const float *x, const float *t, int n);
+int a12i_mlp_grads(a12_ctx *ctx, float *loss, float *grads);
+int a12i_mlp_train(a12_ctx *ctx, int steps, float lr, float *loss);
+int a12i_mlp_infer(a12_ctx *ctx, const float *x, int n, float *y);
+int a12i_mlp_gemm_check(a12_ctx *ctx, int m, int n, int k, const uint16_t *a, const uint16_t *b,
+                        float *c);
+void a12i_mlp_free(a12_ctx *ctx);
 // Implemented in a12_josh.hip
 int a12i_atlas_rt_pass(a12_ctx *ctx, int n, int nf, double teff, const double *d_wave,
                        const double *d_rcoset, const double *d_depth3, const double *d_kapp,

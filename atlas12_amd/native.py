@@ -33,7 +33,9 @@ EXPORTS = ["a12_ctx_create", "a12_ctx_destroy", "a12_last_error", "a12_stream",
            "a12_atlas_load", "a12_atlas_linop1_run", "a12_atlas_linop1_fetch",
            "a12_atlas_set_kapp", "a12_atlas_rt_run", "a12_atlas_rt_fetch",
            "a12_atlas_select_lines", "a12_atlas_stage_ms", "a12_atlas_shape_info",
-           "a12_atlas_set_shape", "a12_atlas_autotune", "a12_scale_model"]
+           "a12_atlas_set_shape", "a12_atlas_autotune", "a12_scale_model", "a12_mlp_init",
+           "a12_mlp_get_params", "a12_mlp_set_params", "a12_mlp_set_data", "a12_mlp_grads",
+           "a12_mlp_train", "a12_mlp_infer", "a12_mlp_gemm"]
 
 
 class A12Error(RuntimeError):
@@ -426,3 +428,75 @@ def scale_model(ctx: Context, sc, fields, kw: int = 80, tau1lg: float = -6.875,
                                        C.c_double(tau1lg), C.c_double(steplg), _ptr(s8), _ptr(f),
                                        _ptr(out), _ptr(so), _ptr(tau)), "a12_scale_model")
     return out, so, tau
+
+
+# ------------------------------------------------- starting-structure network
+MLP_PARAMS = 235_664
+
+
+def mlp_init(ctx: Context, batch: int, seed: int = 0):
+    ctx._check(ctx.lib.a12_mlp_init(ctx.h, C.c_int32(batch), C.c_uint64(seed)), "a12_mlp_init")
+
+
+def mlp_get_params(ctx: Context) -> np.ndarray:
+    p = np.zeros(MLP_PARAMS, np.float32)
+    ctx._check(ctx.lib.a12_mlp_get_params(ctx.h, p.ctypes.data_as(C.c_void_p)),
+               "a12_mlp_get_params")
+    return p
+
+
+def mlp_set_params(ctx: Context, params):
+    p = np.ascontiguousarray(params, np.float32)
+    if p.size != MLP_PARAMS:
+        raise ValueError(f"expected {MLP_PARAMS} parameters")
+    ctx._check(ctx.lib.a12_mlp_set_params(ctx.h, p.ctypes.data_as(C.c_void_p)),
+               "a12_mlp_set_params")
+
+
+def mlp_set_data(ctx: Context, x, t):
+    x = np.ascontiguousarray(x, np.float32)
+    t = np.ascontiguousarray(t, np.float32)
+    if x.shape[1:] != (4,) or t.shape != (len(x), 144):
+        raise ValueError("x (n, 4), t (n, 144)")
+    ctx._keep_mlp = (x, t)
+    ctx._check(ctx.lib.a12_mlp_set_data(ctx.h, x.ctypes.data_as(C.c_void_p),
+                                        t.ctypes.data_as(C.c_void_p), C.c_int32(len(x))),
+               "a12_mlp_set_data")
+
+
+def mlp_grads(ctx: Context):
+    g = np.zeros(MLP_PARAMS, np.float32)
+    loss = C.c_float(0.0)
+    ctx._check(ctx.lib.a12_mlp_grads(ctx.h, C.byref(loss), g.ctypes.data_as(C.c_void_p)),
+               "a12_mlp_grads")
+    return loss.value, g
+
+
+def mlp_train(ctx: Context, steps: int, lr: float) -> float:
+    loss = C.c_float(0.0)
+    ctx._check(ctx.lib.a12_mlp_train(ctx.h, C.c_int32(steps), C.c_float(lr), C.byref(loss)),
+               "a12_mlp_train")
+    return loss.value
+
+
+def mlp_infer(ctx: Context, x) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.zeros((len(x), 144), np.float32)
+    ctx._check(ctx.lib.a12_mlp_infer(ctx.h, x.ctypes.data_as(C.c_void_p), C.c_int32(len(x)),
+                                     y.ctypes.data_as(C.c_void_p)), "a12_mlp_infer")
+    return y
+
+
+def mlp_gemm(ctx: Context, a_bits, b_bits) -> np.ndarray:
+    """C = A B on the MFMA tile kernel; A (m x k), B (k x n) as BF16 bit
+    patterns (uint16), C FP32."""
+    a = np.ascontiguousarray(a_bits, np.uint16)
+    b = np.ascontiguousarray(b_bits, np.uint16)
+    m, k = a.shape
+    k2, n = b.shape
+    assert k == k2
+    c = np.zeros((m, n), np.float32)
+    ctx._check(ctx.lib.a12_mlp_gemm(ctx.h, C.c_int32(m), C.c_int32(n), C.c_int32(k),
+                                    a.ctypes.data_as(C.c_void_p), b.ctypes.data_as(C.c_void_p),
+                                    c.ctypes.data_as(C.c_void_p)), "a12_mlp_gemm")
+    return c

@@ -242,6 +242,32 @@ int a12_scale_model(a12_ctx *ctx, int32_t nmodel, int32_t nrhox, int32_t kw, dou
                     double steplg, const double *sc, const double *fields, double *out,
                     double *so, double *taustd);
 
+/* ---- starting-structure network (no counterpart in the reference) ------- */
+/* A 4 -> 256 -> 256 -> 256 -> 256 -> 144 MLP (SiLU hidden layers, 235,664
+ * parameters) that maps (Teff, log g, [M/H], vturb) -- standardised by the
+ * caller -- to T and log Pgas on 72 depths, the start SCALE_MODEL otherwise
+ * provides.  BF16 operands on MFMA, FP32 accumulation and master weights,
+ * Adam (0.9, 0.999, 1e-8), mean-squared-error loss.
+ *   init:   capacity `batch` samples, Glorot-uniform weights from seed;
+ *   params: 235,664 floats, W1 (4 x 256, row-major: in x out), b1, W2, b2,
+ *           W3, b3, W4, b4, W5 (256 x 144), b5 (set also resets Adam);
+ *   set_data: the training set, x (n x 4), t (n x 144), n <= batch;
+ *   grads:  loss and gradients (params order) of the set data, no update;
+ *   train:  `steps` full-batch Adam steps at rate lr (one hipGraph replay
+ *           per step); loss of the last step's forward pass;
+ *   infer:  y (n x 144) for x (n x 4), any n;
+ *   gemm:   C (m x n, FP32) = A (m x k) B (k x n), BF16 row-major operands
+ *           (the MFMA tile kernel on its own, for checks). */
+int a12_mlp_init(a12_ctx *ctx, int32_t batch, uint64_t seed);
+int a12_mlp_get_params(a12_ctx *ctx, float *params);
+int a12_mlp_set_params(a12_ctx *ctx, const float *params);
+int a12_mlp_set_data(a12_ctx *ctx, const float *x, const float *t, int32_t n);
+int a12_mlp_grads(a12_ctx *ctx, float *loss, float *grads);
+int a12_mlp_train(a12_ctx *ctx, int32_t steps, float lr, float *loss);
+int a12_mlp_infer(a12_ctx *ctx, const float *x, int32_t n, float *y);
+int a12_mlp_gemm(a12_ctx *ctx, int32_t m, int32_t n, int32_t k, const uint16_t *a,
+                 const uint16_t *b, float *c);
+
 /* ---- measurement --------------------------------------------------------- */
 /* Stage timing with HIP events on the context stream (off by default). */
 int a12_set_profiling(a12_ctx *ctx, int32_t on);
