@@ -431,11 +431,11 @@ int a12_mlp_infer(a12_ctx *ctx, const float *x, int32_t n, float *y)
 }
 
 int a12_mlp_gemm(a12_ctx *ctx, int32_t m, int32_t n, int32_t k, const uint16_t *a,
-                 const uint16_t *b, float *c)
+                 const uint16_t *b, float *c, int32_t split)
 {
     if (!ctx_ok(ctx)) return A12_ERR_ARG;
     A12_HIP(hipSetDevice(ctx->device));
-    return a12i_mlp_gemm_check(ctx, m, n, k, a, b, c);
+    return a12i_mlp_gemm_check(ctx, m, n, k, a, b, c, split);
 }
 
 int a12_set_profiling(a12_ctx *ctx, int32_t on)

@@ -140,7 +140,7 @@ int a12i_mlp_grads(a12_ctx *ctx, float *loss, float *grads);
 int a12i_mlp_train(a12_ctx *ctx, int steps, float lr, float *loss);
 int a12i_mlp_infer(a12_ctx *ctx, const float *x, int n, float *y);
 int a12i_mlp_gemm_check(a12_ctx *ctx, int m, int n, int k, const uint16_t *a, const uint16_t *b,
-                        float *c);
+                        float *c, int split);
 void a12i_mlp_free(a12_ctx *ctx);
 // Implemented in a12_josh.hip
 int a12i_atlas_rt_pass(a12_ctx *ctx, int n, int nf, double teff, const double *d_wave,

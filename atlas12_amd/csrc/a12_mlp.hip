@@ -67,31 +67,35 @@ enum { EPI_STORE = 0, EPI_FWD = 1, EPI_BWD = 2 };
 // in LDS (zero outside the matrices); lane l of an MFMA holds A row l % 16,
 // k = 4 (l / 16) .. +3; B column l % 16, same k; result column l % 16,
 // rows 4 (l / 16) .. +3.
+// Split K (gridDim.z > 1, EPI_STORE only): block z sums k in
+// [z kchunk, (z+1) kchunk) into its own slice C + z M ldc.
 template <int EPI>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16 *__restrict__ A,
                                               long sam, long sak, const bf16 *__restrict__ B,
                                               long sbk, long sbn, float *__restrict__ C, int ldc,
                                               bf16 *__restrict__ Cb, const float *__restrict__ bias,
                                               float *__restrict__ Z, const float *__restrict__ Zp,
-                                              int act)
+                                              int act, int kchunk)
 {
     __shared__ bf16 sA[64 * 17];  // [m][k], padded row
     __shared__ bf16 sB[16 * 65];  // [k][n], padded row
     const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    const int kbeg = blockIdx.z * kchunk, kend = min(K, kbeg + kchunk);
+    C += (long)blockIdx.z * M * ldc;
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
     const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
     const int li = lane & 15, kb = 4 * (lane >> 4);
     f32x4 acc[2][2];
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int k0 = 0; k0 < K; k0 += 16) {
+    for (int k0 = kbeg; k0 < kend; k0 += 16) {
         for (int e = t; e < 1024; e += 256) {
             const int r = e >> 4, c = e & 15, gm = m0 + r, gk = k0 + c;
-            sA[r * 17 + c] = (gm < M && gk < K) ? A[(long)gm * sam + (long)gk * sak] : (bf16)0;
+            sA[r * 17 + c] = (gm < M && gk < kend) ? A[(long)gm * sam + (long)gk * sak] : (bf16)0;
         }
         for (int e = t; e < 1024; e += 256) {
             const int r = e >> 6, c = e & 63, gk = k0 + r, gn = n0 + c;
-            sB[r * 65 + c] = (gk < K && gn < N) ? B[(long)gk * sbk + (long)gn * sbn] : (bf16)0;
+            sB[r * 65 + c] = (gk < kend && gn < N) ? B[(long)gk * sbk + (long)gn * sbn] : (bf16)0;
         }
         __syncthreads();
         for (int i = 0; i < 2; ++i) {
@@ -121,6 +125,17 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16 *_
                 C[o] = v;
                 if (Cb) Cb[o] = f2bf(v);
             }
+}
+
+// out[i] = sum over the split slices in slice order
+__global__ void k_sum_split(long n, int split, const float *__restrict__ part,
+                            float *__restrict__ out)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (int z = 0; z < split; ++z) s += part[(long)z * n + i];
+    out[i] = s;
 }
 
 // inputs (n x 4, FP32) -> padded BF16 (rows x 16), zero beyond
@@ -228,14 +243,14 @@ struct MlpDev {
     int n = 0;     // samples of the loaded data
     DevBuf w, wb, g, m, v;                    // parameters (padded layout)
     DevBuf xb, tgt, y, z[kLayers], h, hb[kLayers - 1];
-    DevBuf dz, dzb, dz2, dz2b, part, loss, step, lr, xin, xbi;
+    DevBuf dz, dzb, dz2, dz2b, part, loss, step, lr, xin, xbi, split;
     hipGraphExec_t graph = nullptr;
     bool ready = false;
     ~MlpDev()
     {
         if (graph) (void)hipGraphExecDestroy(graph);
         DevBuf *b[] = {&w, &wb, &g, &m, &v, &xb, &tgt, &y, &h, &dz, &dzb, &dz2, &dz2b,
-                       &part, &loss, &step, &lr, &xin, &xbi};
+                       &part, &loss, &step, &lr, &xin, &xbi, &split};
         for (DevBuf *p : b) p->release();
         for (DevBuf &p : z) p.release();
         for (DevBuf &p : hb) p.release();
@@ -261,13 +276,27 @@ static void gemm(hipStream_t s, int epi, int M, int N, int K, const bf16 *A, lon
     const dim3 grid((N + 63) / 64, (M + 63) / 64);
     if (epi == EPI_FWD)
         hipLaunchKernelGGL(k_gemm<EPI_FWD>, grid, dim3(256), 0, s, M, N, K, A, sam, sak, B, sbk, sbn,
-                           C, ldc, Cb, bias, Z, Zp, act);
+                           C, ldc, Cb, bias, Z, Zp, act, K);
     else if (epi == EPI_BWD)
         hipLaunchKernelGGL(k_gemm<EPI_BWD>, grid, dim3(256), 0, s, M, N, K, A, sam, sak, B, sbk, sbn,
-                           C, ldc, Cb, bias, Z, Zp, act);
+                           C, ldc, Cb, bias, Z, Zp, act, K);
     else
         hipLaunchKernelGGL(k_gemm<EPI_STORE>, grid, dim3(256), 0, s, M, N, K, A, sam, sak, B, sbk,
-                           sbn, C, ldc, Cb, bias, Z, Zp, act);
+                           sbn, C, ldc, Cb, bias, Z, Zp, act, K);
+}
+
+// C (M x N, dense) = A B with K split `split` ways into slices of `part`,
+// summed in slice order (deterministic).
+static void gemm_splitk(hipStream_t s, int M, int N, int K, const bf16 *A, long sam, long sak,
+                        const bf16 *B, long sbk, long sbn, float *C, float *part, int split)
+{
+    const int kchunk = ((K + split - 1) / split + 15) / 16 * 16;
+    const dim3 grid((N + 63) / 64, (M + 63) / 64, split);
+    hipLaunchKernelGGL(k_gemm<EPI_STORE>, grid, dim3(256), 0, s, M, N, K, A, sam, sak, B, sbk, sbn,
+                       part, N, nullptr, nullptr, nullptr, nullptr, 0, kchunk);
+    const long n = (long)M * N;
+    hipLaunchKernelGGL(k_sum_split, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, split,
+                       part, C);
 }
 
 // forward over `rows` rows of the BF16 inputs xb into y (and z, hb)
@@ -288,6 +317,9 @@ static void mlp_forward(a12_ctx *ctx, MlpDev &D, int rows, const bf16 *xb)
     }
 }
 
+// row-splits of the weight-gradient GEMMs: about 128 rows per slice, <= 32
+static int split_of(int rows) { return std::max(1, std::min(32, rows / 128)); }
+
 // loss, backward and (if adam) the update, over D.n valid of D.rows rows
 static void mlp_backward(a12_ctx *ctx, MlpDev &D, bool adam)
 {
@@ -305,9 +337,9 @@ static void mlp_backward(a12_ctx *ctx, MlpDev &D, bool adam)
     for (int l = kLayers - 1; l >= 0; --l) {
         const int K = lay_k(l), N = lay_n(l);
         const bf16 *hin = l == 0 ? D.xb.as<bf16>() : D.hb[l - 1].as<bf16>();
-        // dW_l = Hin^T dZ_l  (K x N, summed over the rows)
-        gemm(s, EPI_STORE, K, N, rows, hin, 1, K, dzb, N, 1, g + w_off(l), N, nullptr, nullptr,
-             nullptr, nullptr, 0);
+        // dW_l = Hin^T dZ_l  (K x N, summed over the rows, K split over the rows)
+        gemm_splitk(s, K, N, rows, hin, 1, K, dzb, N, 1, g + w_off(l), D.split.as<float>(),
+                    split_of(rows));
         hipLaunchKernelGGL(k_colsum, dim3((N + 255) / 256), dim3(256), 0, s, rows, N, dz, N,
                            g + b_off(l));
         if (l > 0) {
@@ -348,6 +380,7 @@ int a12i_mlp_init(a12_ctx *ctx, int batch, uint64_t seed)
     A12_HIP(D.wb.reserve(2 * P));
     A12_HIP(D.xb.reserve(2 * R * kInPad));
     A12_HIP(D.xbi.reserve(2 * R * kInPad));
+    A12_HIP(D.split.reserve(4 * (size_t)split_of(D.rows) * kHid * kHid));
     A12_HIP(D.xin.reserve(4 * R * kIn));
     A12_HIP(D.tgt.reserve(4 * R * kOut));
     A12_HIP(D.y.reserve(4 * R * kOut));
@@ -556,9 +589,10 @@ int a12i_mlp_infer(a12_ctx *ctx, const float *x, int n, float *y)
 }
 
 int a12i_mlp_gemm_check(a12_ctx *ctx, int M, int N, int K, const uint16_t *a, const uint16_t *b,
-                        float *c)
+                        float *c, int split)
 {
-    if (M < 1 || N < 1 || K < 1 || M > 4096 || N > 4096 || K > 4096 || !a || !b || !c) {
+    if (M < 1 || N < 1 || K < 1 || M > 4096 || N > 4096 || K > 4096 || !a || !b || !c ||
+        split < 1 || split > 64) {
         ctx->err = "a12_mlp_gemm: bad arguments";
         return A12_ERR_ARG;
     }
@@ -569,13 +603,21 @@ int a12i_mlp_gemm_check(a12_ctx *ctx, int M, int N, int K, const uint16_t *a, co
     A12_HIP(dc.reserve(4 * (size_t)M * N));
     A12_HIP(hipMemcpyAsync(da.p, a, 2 * (size_t)M * K, hipMemcpyHostToDevice, s));
     A12_HIP(hipMemcpyAsync(db.p, b, 2 * (size_t)K * N, hipMemcpyHostToDevice, s));
-    gemm(s, EPI_STORE, M, N, K, da.as<bf16>(), K, 1, db.as<bf16>(), N, 1, dc.as<float>(), N,
-         nullptr, nullptr, nullptr, nullptr, 0);
+    DevBuf dp;
+    if (split == 1) {
+        gemm(s, EPI_STORE, M, N, K, da.as<bf16>(), K, 1, db.as<bf16>(), N, 1, dc.as<float>(), N,
+             nullptr, nullptr, nullptr, nullptr, 0);
+    } else {
+        A12_HIP(dp.reserve(4 * (size_t)split * M * N));
+        gemm_splitk(s, M, N, K, da.as<bf16>(), K, 1, db.as<bf16>(), N, 1, dc.as<float>(),
+                    dp.as<float>(), split);
+    }
     A12_CHECK_LAUNCH();
     A12_HIP(hipMemcpyAsync(c, dc.p, 4 * (size_t)M * N, hipMemcpyDeviceToHost, s));
     A12_HIP(hipStreamSynchronize(s));
     da.release();
     db.release();
     dc.release();
+    dp.release();
     return A12_OK;
 }

@@ -487,9 +487,9 @@ def mlp_infer(ctx: Context, x) -> np.ndarray:
     return y
 
 
-def mlp_gemm(ctx: Context, a_bits, b_bits) -> np.ndarray:
-    """C = A B on the MFMA tile kernel; A (m x k), B (k x n) as BF16 bit
-    patterns (uint16), C FP32."""
+def mlp_gemm(ctx: Context, a_bits, b_bits, split: int = 1) -> np.ndarray:
+    """C = A B on the MFMA tile kernel (k split `split` ways when > 1);
+    A (m x k), B (k x n) as BF16 bit patterns (uint16), C FP32."""
     a = np.ascontiguousarray(a_bits, np.uint16)
     b = np.ascontiguousarray(b_bits, np.uint16)
     m, k = a.shape
@@ -498,5 +498,6 @@ def mlp_gemm(ctx: Context, a_bits, b_bits) -> np.ndarray:
     c = np.zeros((m, n), np.float32)
     ctx._check(ctx.lib.a12_mlp_gemm(ctx.h, C.c_int32(m), C.c_int32(n), C.c_int32(k),
                                     a.ctypes.data_as(C.c_void_p), b.ctypes.data_as(C.c_void_p),
-                                    c.ctypes.data_as(C.c_void_p)), "a12_mlp_gemm")
+                                    c.ctypes.data_as(C.c_void_p), C.c_int32(split)),
+               "a12_mlp_gemm")
     return c

@@ -233,6 +233,46 @@ def synthe_run(args, ctx, native, wl):
     return res, st, lines
 
 
+# ------------------------------------------------------- starting network
+PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
+MLP_FLOPS_PER_SAMPLE = 2 * 235_664  # forward; a training step is 3x
+
+
+def mlp_run(args, ctx, native):
+    """Full-batch training steps on 2,048 samples and batched inference,
+    with random inputs/targets of the network's shapes (the time does not
+    depend on the values)."""
+    import torch
+    rng = np.random.default_rng(0)
+    nb = 2048
+    x = rng.standard_normal((nb, 4)).astype(np.float32)
+    t = rng.standard_normal((nb, 144)).astype(np.float32)
+    native.mlp_init(ctx, nb, 1)
+    native.mlp_set_data(ctx, x, t)
+    native.mlp_train(ctx, 20, 1e-3)  # warm-up, graph capture
+    steps = 200
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    native.mlp_train(ctx, steps, 1e-3)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    xi = rng.standard_normal((65536, 4)).astype(np.float32)
+    native.mlp_infer(ctx, xi[:nb])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    native.mlp_infer(ctx, xi)
+    torch.cuda.synchronize()
+    di = time.perf_counter() - t0
+    tflops = 3 * MLP_FLOPS_PER_SAMPLE * nb / dt / 1e12
+    return {"train_step_ms": 1e3 * dt, "train_batch": nb,
+            "train_samples_per_s": nb / dt,
+            "infer_samples_per_s": len(xi) / di,
+            "infer_note": "includes host-device copies of inputs and outputs",
+            "roofline": {"bound": "mfma", "kernel": "k_gemm (BF16 MFMA) training step",
+                         "achieved": tflops, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                         "frac": tflops / PEAK_BF16_TFLOPS}}
+
+
 # ------------------------------------------------------------ CPU legs
 def cpu_atlas(z, store):
     """The oracle's LINOP1 over the whole selected store on the host cores
@@ -302,6 +342,7 @@ def main():
     syn = None
     if not args.no_synthe:
         syn = synthe_run(args, ctx, native, wl)
+    net = None if args.profile_only else mlp_run(args, ctx, native)
     if args.profile_only:
         nel = args.calib_mib * (1 << 20) // 4
         src = torch.ones(nel, dtype=torch.float32, device="cuda")
@@ -369,6 +410,8 @@ def main():
                              "frac": rt_tf / PEAK_FP64_TFLOPS,
                              "traffic": traffic.get("k_rt_pass"), "avg_launch_ms": rt_ms},
     }
+    if net is not None:
+        result["starting_network"] = net
     if syn is not None:
         result["synthe_plain"] = syn[0]
         result["synthe_plain"]["roofline"]["traffic"] = traffic.get("k_tile<false>")

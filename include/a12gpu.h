@@ -256,8 +256,10 @@ int a12_scale_model(a12_ctx *ctx, int32_t nmodel, int32_t nrhox, int32_t kw, dou
  *   train:  `steps` full-batch Adam steps at rate lr (one hipGraph replay
  *           per step); loss of the last step's forward pass;
  *   infer:  y (n x 144) for x (n x 4), any n;
- *   gemm:   C (m x n, FP32) = A (m x k) B (k x n), BF16 row-major operands
- *           (the MFMA tile kernel on its own, for checks). */
+ *   gemm:   C (m x n, FP32) = A (m x k) B (k x n), BF16 row-major operands,
+ *           the MFMA tile kernel on its own (split = 1) or with k split
+ *           `split` ways and the slices summed in order (as the weight
+ *           gradients are), for checks. */
 int a12_mlp_init(a12_ctx *ctx, int32_t batch, uint64_t seed);
 int a12_mlp_get_params(a12_ctx *ctx, float *params);
 int a12_mlp_set_params(a12_ctx *ctx, const float *params);
@@ -266,7 +268,7 @@ int a12_mlp_grads(a12_ctx *ctx, float *loss, float *grads);
 int a12_mlp_train(a12_ctx *ctx, int32_t steps, float lr, float *loss);
 int a12_mlp_infer(a12_ctx *ctx, const float *x, int32_t n, float *y);
 int a12_mlp_gemm(a12_ctx *ctx, int32_t m, int32_t n, int32_t k, const uint16_t *a,
-                 const uint16_t *b, float *c);
+                 const uint16_t *b, float *c, int32_t split);
 
 /* ---- measurement --------------------------------------------------------- */
 /* Stage timing with HIP events on the context stream (off by default). */

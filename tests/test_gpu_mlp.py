@@ -19,14 +19,15 @@ def _bits(a):
     return mlp.bf16(a).view(np.uint32).__rshift__(16).astype(np.uint16)
 
 
-@pytest.mark.parametrize("m,n,k", [(80, 144, 48), (64, 64, 16), (130, 70, 37)])
-def test_mfma_gemm_matches_numpy(gpu_ctx, m, n, k):
+@pytest.mark.parametrize("m,n,k,split", [(80, 144, 48, 1), (64, 64, 16, 1), (130, 70, 37, 1),
+                                         (256, 144, 2048, 16), (16, 256, 1000, 7)])
+def test_mfma_gemm_matches_numpy(gpu_ctx, m, n, k, split):
     from atlas12_amd import native
     from oracle import mlp
     rng = np.random.default_rng(m + n + k)
     a = mlp.bf16(rng.standard_normal((m, k)).astype(np.float32))
     b = mlp.bf16(rng.standard_normal((k, n)).astype(np.float32))
-    c = native.mlp_gemm(gpu_ctx, _bits(a), _bits(b))
+    c = native.mlp_gemm(gpu_ctx, _bits(a), _bits(b), split)
     ref = a.astype(np.float64) @ b.astype(np.float64)
     assert np.max(np.abs(c - ref)) <= 1e-5 * np.max(np.abs(ref)) + 1e-6
 
