@@ -438,6 +438,14 @@ int a12_mlp_gemm(a12_ctx *ctx, int32_t m, int32_t n, int32_t k, const uint16_t *
     return a12i_mlp_gemm_check(ctx, m, n, k, a, b, c, split);
 }
 
+int a12_atlas_tcorr3(a12_ctx *ctx, const a12_tcorr_in *in, double *rdabh, double *oldt1,
+                     double *out)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_atlas_tcorr3(ctx, in, rdabh, oldt1, out);
+}
+
 int a12_set_profiling(a12_ctx *ctx, int32_t on)
 {
     if (!ctx_ok(ctx)) return A12_ERR_ARG;

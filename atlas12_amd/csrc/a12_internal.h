@@ -147,6 +147,8 @@ int a12i_atlas_rt_pass(a12_ctx *ctx, int n, int nf, double teff, const double *d
                        const double *d_rcoset, const double *d_depth3, const double *d_kapp,
                        const double *d_xlines, double *d_hnu1, double *d_acc);
 int a12i_synthe_transfer(a12_ctx *ctx);
+int a12i_atlas_tcorr3(a12_ctx *ctx, const a12_tcorr_in *in, double *rdabh, double *oldt1,
+                      double *out);
 int a12i_scale_model(a12_ctx *ctx, int nmodel, int n, int kw, double tau1lg, double steplg,
                      const double *sc, const double *fields, double *out, double *so,
                      double *taustd_out);

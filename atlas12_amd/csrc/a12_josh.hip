@@ -932,6 +932,133 @@ __global__ __launch_bounds__(64) void k_scale_model(int nmodel, int n, int kw,
     }
 }
 
+// ---- TCORR mode 3, sections A-F --------------------------------------------
+// src/atlas12_modules.f90:2528-2752, one wavefront: DERIV/INTEG/MAP1 lane-
+// parallel as in JOSH, the depth recurrences (the DTLAMB halving of the five
+// layers above a convective one, the damping) on lane 0.  in: 16 arrays of
+// n (TAUROS RHOX T DLTDLP ABROSS FLXCNV FLXCNV0 GRDADB PTOTAL RHO DLRDLT
+// HEATCP HSCALE FLXRAD RJMINS RDIAGJ); rdabh, oldt1 updated in place; out
+// (7, n): T1 DTFLUX DTLAMB DTSURF FLXERR FLXDRV HRATIO.
+struct TcorrLDS {
+    double dtdrhx[NR], ddlt[NR], dabros[NR], cnvflx[NR], ddel[NR], codrhx[NR], g[NR];
+    double gflux[NR], dtau[NR], dum[NR], tinteg[NR], xs[2], ys[2];
+};
+
+__global__ __launch_bounds__(64) void k_tcorr3(int n, int ifconv, int iter, double flux,
+                                               double teff, double mixlth,
+                                               const double *__restrict__ in,
+                                               double *__restrict__ rdabh,
+                                               double *__restrict__ oldt1, double *__restrict__ out)
+{
+#pragma clang fp contract(off)
+    __shared__ JoshLDS W;
+    __shared__ TcorrLDS L;
+    const int ln = lane_id();
+    const double fourpi = 4.0 * 3.141592653589793, sigma_sb = 5.670374419e-5;
+    const double abfl = 1.0e-30, rdfl = 1.0e-30, delfl = 1.0e-10, hoff = 1.0e-3;
+    const double *tauros = in, *rhox = in + n, *t = in + 2 * n, *dltdlp = in + 3 * n,
+                 *abross = in + 4 * n, *flxcnv = in + 5 * n, *flxcnv0 = in + 6 * n,
+                 *grdadb = in + 7 * n, *ptotal = in + 8 * n, *rho = in + 9 * n,
+                 *dlrdlt = in + 10 * n, *heatcp = in + 11 * n, *hscale = in + 12 * n,
+                 *flxrad = in + 13 * n, *rjmins = in + 14 * n, *rdiagj = in + 15 * n;
+    double *t1 = out, *dtflux = out + n, *dtlamb = out + 2 * n, *dtsurf = out + 3 * n;
+    double *flxerr = out + 4 * n, *flxdrv = out + 5 * n, *hratio = out + 6 * n;
+    deriv_wave(rhox, t, L.dtdrhx, n);
+    deriv_wave(rhox, dltdlp, L.ddlt, n);
+    deriv_wave(rhox, abross, L.dabros, n);
+    for (int j = ln; j < n; j += 64) {  // (A), (B)
+        L.cnvflx[j] = (ifconv == 1 && j + 1 >= 3) ? flxcnv[j] : 0.0;
+        const double cnv = L.cnvflx[j];
+        L.ddel[j] = 1.0;
+        const double abs_ = fmax(abross[j], abfl);
+        rdabh[j] = rdabh[j] - flxrad[j] * L.dabros[j] / abs_;
+        double del = 1.0, d = 0.0;
+        if (cnv > 0.0 && flxcnv0[j] > 0.0) {
+            del = fmax(dltdlp[j] - grdadb[j], delfl);
+            const double vco = 0.5 * mixlth * sqrt(fmax(-0.5 * ptotal[j] / rho[j] * dlrdlt[j], 0.0));
+            const double fluxco = 0.5 * rho[j] * heatcp[j] * t[j] * mixlth / fourpi;
+            if (mixlth > 0.0 && vco > 0.0) {
+                const double t4 = ((t[j] * t[j]) * t[j]) * t[j];
+                d = 8.0 * sigma_sb * t4 / (abs_ * hscale[j] * rho[j]) / (fluxco * fourpi) / vco;
+            }
+            const double taub = abs_ * rho[j] * mixlth * hscale[j];
+            d = d * (taub * taub) / (2.0 + taub * taub);
+            d = d * d / 2.0;
+            L.ddel[j] = (1.0 + d / (d + del)) / del;
+        }
+        double cnvfl = 0.0;
+        if (fmax(flxrad[j], 1.0e-30) > 0.0)
+            if (cnv / fmax(flxrad[j], 1.0e-30) > 1.0e-3 && flxcnv0[j] / fmax(flxrad[j], 1.0e-30) > 1.0e-3)
+                cnvfl = cnv;
+        L.codrhx[j] = (j < 2) ? 0.0
+                              : (rdabh[j] + cnvfl * (L.dtdrhx[j] / t[j] * (1.0 - 9.0 * d / (d + del)) +
+                                                     1.5 * L.ddlt[j] / del * (1.0 + d / (d + del)))) /
+                                    (flxrad[j] + cnv * 1.5 * dltdlp[j] / del * (1.0 + d / (d + del)));
+    }
+    wsync();
+    integ_wave(W, rhox, L.codrhx, L.g, n, 0.0);  // (C)
+    for (int j = ln; j < n; j += 64) {
+        L.g[j] = exp(L.g[j]);
+        L.gflux[j] = L.g[j] * (flxrad[j] + L.cnvflx[j] - flux) /
+                     (flxrad[j] + L.cnvflx[j] * 1.5 * dltdlp[j] * L.ddel[j]);
+    }
+    wsync();
+    integ_wave(W, tauros, L.gflux, L.dtau, n, 0.0);
+    for (int j = ln; j < n; j += 64) {
+        double dt = L.dtau[j] / L.g[j];
+        dt = fmax(-tauros[j] / 3.0, fmin(tauros[j] / 3.0, dt));
+        dtflux[j] = -dt * L.dtdrhx[j] / fmax(abross[j], abfl);
+        flxerr[j] = (flxrad[j] + L.cnvflx[j] - flux) / flux * 100.0;  // (D)
+        hratio[j] = L.cnvflx[j] / (L.cnvflx[j] + fmax(flxrad[j], 1.0e-30));
+    }
+    wsync();
+    deriv_wave(tauros, flxerr, flxdrv, n);
+    const double maxdt = 0.04 * teff;
+    if (ln == 0) {
+        for (int j = 0; j < n; ++j) {
+            if (hratio[j] < hoff) flxdrv[j] = rjmins[j] / fmax(abross[j], abfl) / flux * 100.0;
+            dtlamb[j] = -flxdrv[j] * flux / 100.0 / fmax(fabs(rdiagj[j]), rdfl) *
+                        copysign(1.0, rdiagj[j]) * fmax(abross[j], abfl);
+            if (hratio[j] >= hoff) {
+                dtlamb[j] = 0.0;
+                for (int k = 1; k <= 5; ++k)
+                    if (j - k >= 0) dtlamb[j - k] = dtlamb[j - k] / 2.0;
+            }
+            dtlamb[j] = fmax(-maxdt, fmin(maxdt, dtlamb[j]));
+        }
+        for (int j = 0; j < n; ++j) L.dum[j] = dtflux[j] + dtlamb[j];  // (E)
+        L.xs[0] = 0.1;
+        L.xs[1] = 2.0;
+    }
+    wsync();
+    integ_wave(W, tauros, L.dum, L.tinteg, n, 0.0);
+    (void)map1_wave(tauros, L.tinteg, n, L.xs, L.ys, 1);
+    (void)map1_wave(tauros, L.tinteg, n, L.xs + 1, L.ys + 1, 1);
+    if (ln == 0) {
+        double dtsur = (flux - flxrad[0]) / flux * 0.25 * t[0];
+        dtsur = fmax(-maxdt, fmin(maxdt, dtsur));
+        double tav = (L.ys[1] - L.ys[0]) / 2.0;
+        if (dtsur * tav <= 0.0) tav = 0.0;
+        if (fabs(tav) > fabs(dtsur)) tav = dtsur;
+        dtsur = dtsur - tav;
+        for (int j = 0; j < n; ++j) {  // (F)
+            dtsurf[j] = dtsur;
+            double x = dtflux[j] + dtlamb[j] + dtsur;
+            const int J = j + 1;
+            if (ifconv == 1 && hratio[j] > 0.0) {
+            } else if (ifconv == 1 && J >= n / 3) {
+            } else if (iter == 1) {
+            } else if (oldt1[j] * x > 0.0 && fabs(oldt1[j]) > fabs(x)) {
+                x = x * 1.25;
+            } else if (oldt1[j] * x < 0.0) {
+                x = x * 0.5;
+            }
+            t1[j] = x;
+            oldt1[j] = x;
+        }
+    }
+}
+
 }  // namespace
 
 int a12i_synthe_transfer(a12_ctx *ctx)
@@ -1081,5 +1208,62 @@ int a12i_scale_model(a12_ctx *ctx, int nmodel, int n, int kw, double tau1lg, dou
     A12_HIP(hipMemcpyAsync(out, ctx->jb_out.p, sizeof(double) * fout, hipMemcpyDeviceToHost,
                            ctx->stream));
     A12_HIP(hipStreamSynchronize(ctx->stream));
+    return A12_OK;
+}
+
+int a12i_atlas_tcorr3(a12_ctx *ctx, const a12_tcorr_in *in, double *rdabh, double *oldt1,
+                      double *out)
+{
+    if (!in || !out || !oldt1 || in->nrhox < 4 || in->nrhox > A12_NRMAX) {
+        ctx->err = "a12_atlas_tcorr3: bad dimensions or null array";
+        return A12_ERR_ARG;
+    }
+    const int n = in->nrhox;
+    const double *cols[13] = {in->tauros, in->rhox, in->t, in->dltdlp, in->abross, in->flxcnv,
+                              in->flxcnv0, in->grdadb, in->ptotal, in->rho, in->dlrdlt,
+                              in->heatcp, in->hscale};
+    for (const double *c : cols)
+        if (!c) {
+            ctx->err = "a12_atlas_tcorr3: null state array";
+            return A12_ERR_ARG;
+        }
+    const bool dev = in->flxrad == nullptr;
+    AtlasDev &A = ctx->atl;
+    if (dev && (!A.rt_done || A.n != n)) {
+        ctx->err = "a12_atlas_tcorr3: no RT pass sums on the device for this depth count";
+        return A12_ERR_STATE;
+    }
+    if (!dev && (!in->rjmins || !in->rdiagj || !rdabh)) {
+        ctx->err = "a12_atlas_tcorr3: flxrad, rjmins, rdiagj and rdabh go together";
+        return A12_ERR_ARG;
+    }
+    hipStream_t s = ctx->stream;
+    A12_HIP(ctx->jb_in.reserve(sizeof(double) * (16 * (size_t)n + 2 * (size_t)n)));
+    A12_HIP(ctx->jb_out.reserve(sizeof(double) * 7 * (size_t)n));
+    double *d = ctx->jb_in.as<double>();
+    for (int k = 0; k < 13; ++k)
+        A12_HIP(hipMemcpyAsync(d + k * n, cols[k], sizeof(double) * n, hipMemcpyHostToDevice, s));
+    double *drd = d + 16 * n, *dold = d + 17 * n;
+    if (dev) {
+        // acc layout: RJMINS RDABH RDIAGJ FLXRAD ... (a12_atlas_rt_fetch)
+        const double *acc = A.acc.as<double>();
+        A12_HIP(hipMemcpyAsync(d + 13 * n, acc + 3 * n, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        A12_HIP(hipMemcpyAsync(d + 14 * n, acc, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        A12_HIP(hipMemcpyAsync(d + 15 * n, acc + 2 * n, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        A12_HIP(hipMemcpyAsync(drd, acc + n, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    } else {
+        A12_HIP(hipMemcpyAsync(d + 13 * n, in->flxrad, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        A12_HIP(hipMemcpyAsync(d + 14 * n, in->rjmins, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        A12_HIP(hipMemcpyAsync(d + 15 * n, in->rdiagj, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        A12_HIP(hipMemcpyAsync(drd, rdabh, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    }
+    A12_HIP(hipMemcpyAsync(dold, oldt1, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_tcorr3, dim3(1), dim3(64), 0, s, n, in->ifconv, in->iter, in->flux,
+                       in->teff, in->mixlth, d, drd, dold, ctx->jb_out.as<double>());
+    A12_CHECK_LAUNCH();
+    A12_HIP(hipMemcpyAsync(out, ctx->jb_out.p, sizeof(double) * 7 * n, hipMemcpyDeviceToHost, s));
+    A12_HIP(hipMemcpyAsync(oldt1, dold, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    if (rdabh) A12_HIP(hipMemcpyAsync(rdabh, drd, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    A12_HIP(hipStreamSynchronize(s));
     return A12_OK;
 }

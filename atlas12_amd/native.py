@@ -35,7 +35,7 @@ EXPORTS = ["a12_ctx_create", "a12_ctx_destroy", "a12_last_error", "a12_stream",
            "a12_atlas_select_lines", "a12_atlas_stage_ms", "a12_atlas_shape_info",
            "a12_atlas_set_shape", "a12_atlas_autotune", "a12_scale_model", "a12_mlp_init",
            "a12_mlp_get_params", "a12_mlp_set_params", "a12_mlp_set_data", "a12_mlp_grads",
-           "a12_mlp_train", "a12_mlp_infer", "a12_mlp_gemm"]
+           "a12_mlp_train", "a12_mlp_infer", "a12_mlp_gemm", "a12_atlas_tcorr3"]
 
 
 class A12Error(RuntimeError):
@@ -341,6 +341,46 @@ def atlas_rt_pass(ctx: Context, xlines=None, fetch: bool = True):
     out = {k: acc[i * n:(i + 1) * n].copy() for i, k in enumerate(ACC_NAMES)}
     out["pradk0"] = float(acc[8 * n])
     return out, h1
+
+
+TCORR_STATE = ("tauros", "rhox", "t", "dltdlp", "abross", "flxcnv", "flxcnv0", "grdadb",
+               "ptotal", "rho", "dlrdlt", "heatcp", "hscale")
+TCORR_SUMS = ("flxrad", "rjmins", "rdiagj")
+TCORR_OUT = ("t1", "dtflux", "dtlamb", "dtsurf", "flxerr", "flxdrv", "hratio")
+
+
+class TcorrIn(C.Structure):
+    _fields_ = [("nrhox", C.c_int32), ("ifconv", C.c_int32), ("iter", C.c_int32),
+                ("flux", C.c_double), ("teff", C.c_double), ("mixlth", C.c_double)] + \
+               [(k, P_D) for k in TCORR_STATE + TCORR_SUMS]
+
+
+def atlas_tcorr3(ctx: Context, z, rdabh=None, oldt1=None, iter_=None, device_sums=False):
+    """TCORR(3) sections A-F (src/atlas12_modules.f90:2528-2752) on the
+    device.  z: mapping with n, ifconv, iter, flux, teff, mixlth and the
+    TCORR_STATE arrays; with device_sums False also TCORR_SUMS and rdabh
+    (RDABH before TCORR), with device_sums True those four come from the
+    last atlas_rt_pass on this context.  Returns (out dict of TCORR_OUT,
+    updated rdabh, updated oldt1)."""
+    n = int(z["n"])
+    keep = {k: _f64(z[k]) for k in TCORR_STATE + (() if device_sums else TCORR_SUMS)}
+    for k, a in keep.items():
+        if a.shape != (n,):
+            raise ValueError(f"{k} must have shape ({n},)")
+    s = TcorrIn()
+    s.nrhox, s.ifconv = n, int(z["ifconv"])
+    s.iter = int(z["iter"]) if iter_ is None else int(iter_)
+    s.flux, s.teff, s.mixlth = float(z["flux"]), float(z["teff"]), float(z["mixlth"])
+    for k, a in keep.items():
+        setattr(s, k, _ptr(a))
+    rd = np.zeros(n) if rdabh is None else _f64(np.array(rdabh, np.float64).copy())
+    if rd.shape != (n,):
+        raise ValueError(f"rdabh must have shape ({n},)")
+    ot = np.zeros(n) if oldt1 is None else _f64(np.array(oldt1, np.float64).copy())
+    out = np.zeros((7, n))
+    ctx._check(ctx.lib.a12_atlas_tcorr3(ctx.h, C.byref(s), _ptr(rd), _ptr(ot), _ptr(out)),
+               "a12_atlas_tcorr3")
+    return {k: out[i] for i, k in enumerate(TCORR_OUT)}, rd, ot
 
 
 def atlas_stage_ms(ctx: Context):

@@ -197,6 +197,27 @@ int a12_atlas_rt_run(a12_ctx *ctx, const double *xlines);
  * frequency after the floors. */
 int a12_atlas_rt_fetch(a12_ctx *ctx, double *acc, double *hnu1);
 
+/* TCORR(3) sections A-F, src/atlas12_modules.f90:2528-2752: the
+ * flux-constancy (DTFLUX), Lambda-iteration (DTLAMB) and surface (DTSURF)
+ * temperature corrections and the iteration damping, after ROSS(3),
+ * RADIAP(3), COMPUTE_HEIGHT and CONVEC (src/atlas12c.f90:582-587).  The
+ * driver keeps CZ_CONSTRUCT and the RHOX correction that follow (:2816-).
+ * Arrays (nrhox) are the module arrays of those names.  flxrad, rjmins,
+ * rdiagj NULL: the sums of the last a12_atlas_rt_run, read on the device
+ * (rdabh then optional, receiving the updated sum).  rdabh (TCORR's RDABH)
+ * and oldt1 (OLDT1) are TCORR's SAVE state, updated in place.
+ * out (7, nrhox): T1 (after damping), DTFLUX, DTLAMB, DTSURF, FLXERR (%),
+ * FLXDRV, HRATIO.  Host pointers, synchronous. */
+typedef struct {
+    int32_t nrhox, ifconv, iter;
+    double flux, teff, mixlth;
+    const double *tauros, *rhox, *t, *dltdlp, *abross, *flxcnv, *flxcnv0, *grdadb, *ptotal,
+        *rho, *dlrdlt, *heatcp, *hscale;
+    const double *flxrad, *rjmins, *rdiagj;
+} a12_tcorr_in;
+int a12_atlas_tcorr3(a12_ctx *ctx, const a12_tcorr_in *in, double *rdabh, double *oldt1,
+                     double *out);
+
 /* MOL2 32-byte molecular record (header: int32 magic 'MOL2', int32 count,
  * 6 more int32), src/atlas12_modules.f90:19540-19556. */
 typedef struct {

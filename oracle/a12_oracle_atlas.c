@@ -472,3 +472,112 @@ int or_scale_model(int n, int kw, double tau1lg, double steplg, const double *sc
     so[5] = pzero;
     return 1;
 }
+
+
+/* ------------------------------------------------------------------ */
+/* TCORR mode 3, sections A-F (src/atlas12_modules.f90:2528-2752)      */
+/* ------------------------------------------------------------------ */
+void or_tcorr3(const or_tcorr_in *in, double *rdabh, double *oldt1, double *out)
+{
+    const int n = in->n;
+    const double fourpi = 4.0 * C_PI, sigma_sb = 5.670374419e-5, flux = in->flux;
+    const double abfl = 1.0e-30, rdfl = 1.0e-30, delfl = 1.0e-10, hoff = 1.0e-3;
+    double dtdrhx[KW_MAX], ddlt[KW_MAX], dabros[KW_MAX], cnvflx[KW_MAX], ddel[KW_MAX];
+    double codrhx[KW_MAX], g[KW_MAX], gflux[KW_MAX], dtau[KW_MAX];
+    double *t1 = out, *dtflux = out + n, *dtlamb = out + 2 * n, *dtsurf = out + 3 * n;
+    double *flxerr = out + 4 * n, *flxdrv = out + 5 * n, *hratio = out + 6 * n;
+    or_deriv(in->rhox, in->t, dtdrhx, n);        /* :2531-2533 */
+    or_deriv(in->rhox, in->dltdlp, ddlt, n);
+    or_deriv(in->rhox, in->abross, dabros, n);
+    for (int j = 0; j < n; ++j)                  /* (A) :2539-2542 */
+        cnvflx[j] = (in->ifconv == 1 && j + 1 >= 3) ? in->flxcnv[j] : 0.0;
+    for (int j = 0; j < n; ++j) {                /* (B) :2551-2593 */
+        ddel[j] = 1.0;
+        const double abs_ = fmax(in->abross[j], abfl);
+        rdabh[j] = rdabh[j] - in->flxrad[j] * dabros[j] / abs_;
+        double del = 1.0, d = 0.0;
+        if (cnvflx[j] > 0.0 && in->flxcnv0[j] > 0.0) {
+            del = in->dltdlp[j] - in->grdadb[j];
+            del = fmax(del, delfl);
+            const double vco = 0.5 * in->mixlth *
+                               sqrt(fmax(-0.5 * in->ptotal[j] / in->rho[j] * in->dlrdlt[j], 0.0));
+            const double fluxco = 0.5 * in->rho[j] * in->heatcp[j] * in->t[j] * in->mixlth / fourpi;
+            if (in->mixlth > 0.0 && vco > 0.0) {
+                const double t4 = ((in->t[j] * in->t[j]) * in->t[j]) * in->t[j];
+                d = 8.0 * sigma_sb * t4 / (abs_ * in->hscale[j] * in->rho[j]) / (fluxco * fourpi) / vco;
+            }
+            const double taub = abs_ * in->rho[j] * in->mixlth * in->hscale[j];
+            d = d * (taub * taub) / (2.0 + taub * taub);
+            d = d * d / 2.0;
+            ddel[j] = (1.0 + d / (d + del)) / del;
+        }
+        double cnvfl = 0.0;
+        if (fmax(in->flxrad[j], 1.0e-30) > 0.0)
+            if (cnvflx[j] / fmax(in->flxrad[j], 1.0e-30) > 1.0e-3 &&
+                in->flxcnv0[j] / fmax(in->flxrad[j], 1.0e-30) > 1.0e-3)
+                cnvfl = cnvflx[j];
+        codrhx[j] = (rdabh[j] + cnvfl * (dtdrhx[j] / in->t[j] * (1.0 - 9.0 * d / (d + del)) +
+                                         1.5 * ddlt[j] / del * (1.0 + d / (d + del)))) /
+                    (in->flxrad[j] + cnvflx[j] * 1.5 * in->dltdlp[j] / del * (1.0 + d / (d + del)));
+    }
+    codrhx[0] = 0.0;
+    if (n > 1) codrhx[1] = 0.0;
+    or_integ(in->rhox, codrhx, g, n, 0.0);       /* (C) :2613-2630 */
+    for (int j = 0; j < n; ++j) {
+        g[j] = exp(g[j]);
+        gflux[j] = g[j] * (in->flxrad[j] + cnvflx[j] - flux) /
+                   (in->flxrad[j] + cnvflx[j] * 1.5 * in->dltdlp[j] * ddel[j]);
+    }
+    or_integ(in->tauros, gflux, dtau, n, 0.0);
+    for (int j = 0; j < n; ++j) {
+        dtau[j] = dtau[j] / g[j];
+        dtau[j] = fmax(-in->tauros[j] / 3.0, fmin(in->tauros[j] / 3.0, dtau[j]));
+        dtflux[j] = -dtau[j] * dtdrhx[j] / fmax(in->abross[j], abfl);
+    }
+    const double maxdt = 0.04 * in->teff;
+    for (int j = 0; j < n; ++j)                  /* (D) :2644-2685 */
+        flxerr[j] = (in->flxrad[j] + cnvflx[j] - flux) / flux * 100.0;
+    or_deriv(in->tauros, flxerr, flxdrv, n);
+    for (int j = 0; j < n; ++j) hratio[j] = cnvflx[j] / (cnvflx[j] + fmax(in->flxrad[j], 1.0e-30));
+    for (int j = 0; j < n; ++j) {
+        if (hratio[j] < hoff)
+            flxdrv[j] = in->rjmins[j] / fmax(in->abross[j], abfl) / flux * 100.0;
+        dtlamb[j] = -flxdrv[j] * flux / 100.0 / fmax(fabs(in->rdiagj[j]), rdfl) *
+                    copysign(1.0, in->rdiagj[j]) * fmax(in->abross[j], abfl);
+        if (hratio[j] >= hoff) {
+            dtlamb[j] = 0.0;
+            for (int k = 1; k <= 5; ++k)
+                if (j - k >= 0) dtlamb[j - k] = dtlamb[j - k] / 2.0;
+        }
+        dtlamb[j] = fmax(-maxdt, fmin(maxdt, dtlamb[j]));
+    }
+    /* (E) :2693-2712 */
+    double dtsur = (flux - in->flxrad[0]) / flux * 0.25 * in->t[0];
+    dtsur = fmax(-maxdt, fmin(maxdt, dtsur));
+    double dum[KW_MAX], tinteg[KW_MAX], x1 = 0.1, x2 = 2.0, tone, ttwo;
+    for (int j = 0; j < n; ++j) dum[j] = dtflux[j] + dtlamb[j];
+    or_integ(in->tauros, dum, tinteg, n, 0.0);
+    (void)or_map1(in->tauros, tinteg, n, &x1, &tone, 1);
+    (void)or_map1(in->tauros, tinteg, n, &x2, &ttwo, 1);
+    double tav = (ttwo - tone) / 2.0;
+    if (dtsur * tav <= 0.0) tav = 0.0;
+    if (fabs(tav) > fabs(dtsur)) tav = dtsur;
+    dtsur = dtsur - tav;
+    /* (F) :2717-2752 */
+    for (int j = 0; j < n; ++j) {
+        dtsurf[j] = dtsur;
+        t1[j] = dtflux[j] + dtlamb[j] + dtsur;
+    }
+    for (int j = 0; j < n; ++j) {
+        const int J = j + 1;
+        if (in->ifconv == 1 && hratio[j] > 0.0) {
+        } else if (in->ifconv == 1 && J >= n / 3) {
+        } else if (in->iter == 1) {
+        } else if (oldt1[j] * t1[j] > 0.0 && fabs(oldt1[j]) > fabs(t1[j])) {
+            t1[j] = t1[j] * 1.25;
+        } else if (oldt1[j] * t1[j] < 0.0) {
+            t1[j] = t1[j] * 0.5;
+        }
+        oldt1[j] = t1[j];
+    }
+}

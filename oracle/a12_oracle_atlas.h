@@ -89,6 +89,19 @@ int or_scale_model(int n, int kw, double tau1lg, double steplg, const double *sc
                    double teff_new, double logg_new, const double *const in[7],
                    double *const out[10], double *so);
 
+/* TCORR(3) sections A-F, src/atlas12_modules.f90:2528-2752: the
+ * flux-constancy (DTFLUX), Lambda (DTLAMB) and surface (DTSURF)
+ * corrections and the iteration damping.  Arrays (n).  rdabh and oldt1 are
+ * TCORR's SAVE state, updated in place.  out (7 x n): T1, DTFLUX, DTLAMB,
+ * DTSURF, FLXERR, FLXDRV, HRATIO. */
+typedef struct {
+    int n, ifconv, iter;
+    double flux, teff, mixlth;
+    const double *tauros, *rhox, *t, *dltdlp, *abross, *flxcnv, *flxcnv0, *grdadb, *ptotal,
+        *rho, *dlrdlt, *heatcp, *hscale, *flxrad, *rjmins, *rdiagj;
+} or_tcorr_in;
+void or_tcorr3(const or_tcorr_in *in, double *rdabh, double *oldt1, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -392,3 +392,32 @@ def scale_model(n, kw, tau1lg, steplg, sc, teff_new, logg_new, fields):
     r = L.or_scale_model(C.c_int(n), C.c_int(kw), C.c_double(tau1lg), C.c_double(steplg),
                          _p(scv), C.c_double(teff_new), C.c_double(logg_new), ins, outs, _p(so))
     return out, so, bool(r)
+
+
+TCORR_FIELDS = ("tauros", "rhox", "t", "dltdlp", "abross", "flxcnv", "flxcnv0", "grdadb",
+                "ptotal", "rho", "dlrdlt", "heatcp", "hscale", "flxrad", "rjmins", "rdiagj")
+
+
+class OrTcorrIn(C.Structure):
+    _fields_ = [("n", C.c_int), ("ifconv", C.c_int), ("iter", C.c_int), ("flux", C.c_double),
+                ("teff", C.c_double), ("mixlth", C.c_double)] + [(k, P_D) for k in TCORR_FIELDS]
+
+
+def tcorr3(z, rdabh, oldt1=None, iter_=None):
+    """TCORR(3) sections A-F on a state dict (keys as tests/golden/
+    tcorr_ref.npz).  Returns (out dict T1, DTFLUX, DTLAMB, DTSURF, FLXERR,
+    FLXDRV, HRATIO; updated rdabh; updated oldt1)."""
+    n = int(z["n"])
+    keep = {k: _f64(z[k]) for k in TCORR_FIELDS}
+    s = OrTcorrIn()
+    s.n, s.ifconv = n, int(z["ifconv"])
+    s.iter = int(z["iter"]) if iter_ is None else int(iter_)
+    s.flux, s.teff, s.mixlth = float(z["flux"]), float(z["teff"]), float(z["mixlth"])
+    for k in TCORR_FIELDS:
+        setattr(s, k, _p(keep[k]))
+    rd = _f64(np.array(rdabh, np.float64).copy())
+    ot = _f64(np.zeros(n) if oldt1 is None else np.array(oldt1, np.float64).copy())
+    out = np.zeros((7, n))
+    lib().or_tcorr3(C.byref(s), _p(rd), _p(ot), _p(out))
+    names = ("t1", "dtflux", "dtlamb", "dtsurf", "flxerr", "flxdrv", "hratio")
+    return {k: out[i] for i, k in enumerate(names)}, rd, ot

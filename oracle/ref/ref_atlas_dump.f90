@@ -157,4 +157,28 @@ PROGRAM ref_atlas_dump
   WRITE(u) RJMINS(1:NRHOX), FLXRAD(1:NRHOX), RADEN(1:NRHOX), ACCRAD(1:NRHOX), &
            ABROSS(1:NRHOX), PRADK0, RDIAGJNU(1:NRHOX), HNU(1:NRHOX), TAUNU(1:NRHOX)
   CLOSE(u)
+
+  ! With A12_TCORR_OUT set: the driver's end of iteration up to the
+  ! temperature correction (src/atlas12c.f90:582-587), the state TCORR(3)
+  ! reads, and TCORR(3) itself with its per-iteration table on unit 66.
+  CALL GET_ENVIRONMENT_VARIABLE('A12_TCORR_OUT', envval, envlen, envstat)
+  IF (envstat .EQ. 0 .AND. envlen .GT. 0) THEN
+    CALL ROSS(3, 0.0D0)
+    CALL RADIAP(3, 0.0D0)
+    CALL COMPUTE_HEIGHT
+    IF (IFPRES .EQ. 1 .AND. IFCONV .EQ. 1) CALL CONVEC(.FALSE.)
+    OPEN(NEWUNIT=u, FILE=TRIM(envval), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+    WRITE(u) NRHOX, IFCONV, ITER
+    WRITE(u) FLUX, TEFF, MIXLTH
+    WRITE(u) TAUROS(1:NRHOX), RHOX(1:NRHOX), T(1:NRHOX), DLTDLP(1:NRHOX), ABROSS(1:NRHOX), &
+             FLXCNV(1:NRHOX), FLXCNV0(1:NRHOX), GRDADB(1:NRHOX), PTOTAL(1:NRHOX), &
+             RHO(1:NRHOX), DLRDLT(1:NRHOX), HEATCP(1:NRHOX), HSCALE(1:NRHOX), &
+             FLXRAD(1:NRHOX), RJMINS(1:NRHOX)
+    CLOSE(u)
+    OPEN(UNIT=66, FILE=TRIM(envval)//'.iter', STATUS='REPLACE')
+    IFCORR = 1
+    IFPRNT(ITER) = 1
+    CALL TCORR(3, 0.0D0)
+    CLOSE(66)
+  END IF
 END PROGRAM ref_atlas_dump

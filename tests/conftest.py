@@ -41,6 +41,14 @@ def coef():
 
 
 @pytest.fixture(scope="session")
+def atl_small():
+    """tests/golden/atlas_small.npz and its C-ABI state."""
+    from atlas12_amd import native
+    z = np.load(os.path.join(GOLDEN, "atlas_small.npz"))
+    return z, native.atlas_state_from(z)
+
+
+@pytest.fixture(scope="session")
 def orc():
     from oracle import oracle
     oracle.build()

@@ -364,7 +364,7 @@ def read_atlas_dump(path: str) -> dict:
     return d
 
 
-def atlas_case(atm, nustart, nuhi, linedata, raw=None):
+def atlas_case(atm, nustart, nuhi, linedata, raw=None, tcorr=False):
     """Runs ref_atlas_dump.  raw = (predicted records, TiO MOL2 records):
     the store is then SELECTLINES' selection from these lists, returned
     as d["selected"] with d["wavetab"]."""
@@ -390,9 +390,16 @@ def atlas_case(atm, nustart, nuhi, linedata, raw=None):
         with open(os.path.join(sel, "lines.list"), "w") as fh:
             fh.write("mol tio.bin\n")
         extra = [sel]
+    env = dict(os.environ, ATLAS12=root)
+    tco = os.path.join(tmp, "tcorr.bin")
+    if tcorr:
+        env["A12_TCORR_OUT"] = tco
     run([os.path.join(REFBIN, "ref_atlas_dump"), deck, str(nustart), str(nuhi), lp, out, *extra],
-        tmp, dict(os.environ, ATLAS12=root))
+        tmp, env)
     d = read_atlas_dump(out)
+    if tcorr:
+        d["tcorr_in"] = read_tcorr_in(tco)
+        d["tcorr_iter"] = read_iter_table(tco + ".iter", d["tcorr_in"]["n"])
     if raw is not None:
         with open(out + ".sel", "rb") as fh:
             buf = fh.read()
@@ -401,6 +408,70 @@ def atlas_case(atm, nustart, nuhi, linedata, raw=None):
         d["wavetab"] = np.frombuffer(buf, "<f8", 361, 4 + 16 * ns).copy()
     shutil.rmtree(tmp)
     return d
+
+
+TCORR_FIELDS = ("tauros", "rhox", "t", "dltdlp", "abross", "flxcnv", "flxcnv0", "grdadb",
+                "ptotal", "rho", "dlrdlt", "heatcp", "hscale", "flxrad", "rjmins")
+ITER_COLS = ("j", "logtau", "t", "dtlamb", "dtsurf", "dtflux", "t1", "flxerr", "flxdrv",
+             "r_newt", "dltdlp", "grdadb", "hratio", "p", "xne", "height", "accrad")
+# FORMAT(I3, F8.3, F10.1, 4F8.1, 1X,ES11.2, 1X,ES11.2, 1X,ES10.2, 2F8.3, 1X,ES11.2,
+#        1X,ES12.3, 1X,ES12.3, 1X,ES10.1, 1X,ES11.2), src/atlas12_modules.f90:2765-2774
+ITER_WIDTHS = (3, 8, 10, 8, 8, 8, 8, 12, 12, 11, 8, 8, 12, 13, 13, 11, 12)
+
+
+def read_tcorr_in(path):
+    b = open(path, "rb").read()
+    n, ifconv, it = (int(x) for x in np.frombuffer(b, "<i4", 3))
+    flux, teff, mixlth = np.frombuffer(b, "<f8", 3, 12)
+    arr = np.frombuffer(b, "<f8", len(TCORR_FIELDS) * n, 36).reshape(len(TCORR_FIELDS), n)
+    d = dict(n=n, ifconv=ifconv, iter=it, flux=flux, teff=teff, mixlth=mixlth)
+    d.update({k: arr[i].copy() for i, k in enumerate(TCORR_FIELDS)})
+    return d
+
+
+def read_iter_table(path, n):
+    """The per-iteration table TCORR(3) prints: n rows after its 3 header lines."""
+    rows = []
+    for line in open(path).read().splitlines():
+        if len(rows) == n:
+            break
+        if len(line) < 60 or not line[:3].strip().isdigit():
+            continue
+        vals, o = [], 0
+        for w in ITER_WIDTHS:
+            vals.append(float(line[o:o + w]))
+            o += w
+        rows.append(vals)
+    a = np.array(rows)
+    return {k: a[:, i] for i, k in enumerate(ITER_COLS)}
+
+
+def golden_tcorr():
+    """TCORR(3) after a full 30,000-frequency pass (100,000 raw lines through
+    SELECTLINES) on the plain-SYNTHE atmosphere: the state it reads, RDABH
+    and RDIAGJ from the oracle's RT pass on the reference's own KAPP tables
+    and XLINES (TCORR keeps those two private), and the per-iteration table
+    it prints (DTLAMB, DTSURF, DTFLUX, T1 to 0.1 K; ERROR, DERIV to 3
+    digits; CONV/TOT)."""
+    from oracle import oracle as orc
+    atm = wl.scurve_atmosphere()
+    st = np.load(os.path.join(HERE, "atlas_plain_state.npz"))
+    mols = tuple(int(m) for m in st["mol_nelion"])
+    pred, tio = wl.atlas_iteration_lists(100_000, 95.0, 90_000.0, seed=41, molecules=mols)
+    d = atlas_case(atm, 9599, 30000, None, raw=(pred, tio), tcorr=True)
+    orc.build()
+    cj, ch = d["coefj"], d["coefh"]
+    acc, _ = orc.rt_pass(d, d["xlines"], cj, ch)
+    for k in ("rjmins", "flxrad", "raden", "accrad", "abross"):
+        assert np.array_equal(acc[k], d["acc_" + k]), k  # the oracle's pass is the reference's
+    ti, tab = d["tcorr_in"], d["tcorr_iter"]
+    keep = {k: ti[k] for k in TCORR_FIELDS}
+    keep.update({"iter_" + k: v for k, v in tab.items()})
+    np.savez_compressed(os.path.join(HERE, "tcorr_ref.npz"), n=ti["n"], ifconv=ti["ifconv"],
+                        iter=ti["iter"], flux=ti["flux"], teff=ti["teff"], mixlth=ti["mixlth"],
+                        rdabh=acc["rdabh"], rdiagj=acc["rdiagj"], **keep)
+    return {"tcorr": {"selected_lines": int(d["nlines"]), "ifconv": ti["ifconv"],
+                      "rows": int(len(tab["j"]))}}
 
 
 def golden_atlas_small():
@@ -543,7 +614,8 @@ def golden_scale():
 
 
 STEPS = {"synthe_small": golden_synthe_small, "atlas_small": golden_atlas_small,
-         "state_atlas": golden_state_atlas, "select": golden_select, "scale": golden_scale, "state_plain": golden_state_plain,
+         "state_atlas": golden_state_atlas, "select": golden_select, "scale": golden_scale,
+         "tcorr": golden_tcorr, "state_plain": golden_state_plain,
          "numerics": golden_numerics, "josh": golden_josh}
 
 

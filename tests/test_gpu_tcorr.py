@@ -1,0 +1,87 @@
+"""GPU TCORR(3) (src/atlas12_modules.f90:2528-2752) through the C-ABI:
+bit for bit against the oracle, which is itself pinned to the reference's
+printed iteration table (tests/golden/tcorr_ref.npz); the damping branches
+over two iterations; and the device-resident path that reads the sums of
+the preceding RT pass without a host round trip.
+
+Tolerance: the kernel evaluates the same expressions in the same order as
+the oracle with FMA contraction off, so outputs are compared exactly,
+except where exp() (g = exp(integral)) enters: 1e-14 relative there."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ("t1", "dtflux", "dtlamb", "dtsurf", "flxerr", "flxdrv", "hratio")
+
+
+@pytest.fixture(scope="module")
+def tz(golden):
+    return golden("tcorr_ref.npz")
+
+
+def _close(a, b, rel=1e-14):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    scale = max(np.max(np.abs(b)), 1e-300)
+    return np.max(np.abs(a - b)) <= rel * scale
+
+
+def test_tcorr3_matches_oracle(gpu_ctx, tz, orc):
+    from atlas12_amd import native
+    out, rd, ot = native.atlas_tcorr3(gpu_ctx, tz, tz["rdabh"])
+    ref, rrd, rot = orc.tcorr3(tz, tz["rdabh"])
+    assert np.array_equal(rd, rrd)
+    for k in NAMES:
+        assert _close(out[k], ref[k]), k
+    assert np.array_equal(ot, out["t1"])
+
+
+def test_tcorr3_matches_reference_table(gpu_ctx, tz):
+    from atlas12_amd import native
+    out, _, _ = native.atlas_tcorr3(gpu_ctx, tz, tz["rdabh"])
+    for k in ("dtlamb", "dtsurf", "dtflux", "t1"):
+        assert [f"{v:.1f}" for v in out[k]] == [f"{v:.1f}" for v in tz["iter_" + k]], k
+    for k in ("flxerr", "flxdrv"):
+        assert [f"{v:.2E}" for v in out[k]] == [f"{v:.2E}" for v in tz["iter_" + k]], k
+    assert [f"{v:.3f}" for v in out["hratio"]] == [f"{v:.3f}" for v in tz["iter_hratio"]]
+
+
+def test_tcorr3_damping_second_iteration(gpu_ctx, tz, orc):
+    """A second call with OLDT1 from the first at ITER = 2 takes the
+    x1.25 / x0.5 branches; device and oracle agree on every layer."""
+    from atlas12_amd import native
+    rng = np.random.default_rng(4)
+    old = tz["iter_t1"] * rng.choice([-1.0, 0.5, 2.0], tz["iter_t1"].shape)
+    out, _, ot = native.atlas_tcorr3(gpu_ctx, tz, tz["rdabh"], oldt1=old, iter_=2)
+    ref, _, rot = orc.tcorr3(tz, tz["rdabh"], oldt1=old, iter_=2)
+    assert _close(out["t1"], ref["t1"]) and _close(ot, rot)
+    assert np.any(out["t1"] != out["dtflux"] + out["dtlamb"] + out["dtsurf"])
+
+
+def test_tcorr3_reads_rt_pass_sums_on_device(gpu_ctx, atl_small, coef, tz, orc):
+    """RT pass on the reference's XLINES, then TCORR(3) fed from the
+    device-resident sums: the same result as the oracle fed the fetched
+    sums."""
+    from atlas12_amd import native
+    z, d = atl_small
+    gpu_ctx.set_operators(*coef)
+    native.atlas_load(gpu_ctx, d, z["linedata"])
+    native.atlas_set_kapp(gpu_ctx, z["acont"], z["sigmac"], z["scont"], z["sline"], z["aline"],
+                          z["sigmal"])
+    acc, _ = native.atlas_rt_pass(gpu_ctx, z["xlines"])
+    out, rd, _ = native.atlas_tcorr3(gpu_ctx, tz, device_sums=True)
+    host = dict(tz)
+    for k in ("flxrad", "rjmins", "rdiagj"):
+        host[k] = acc[k]
+    ref, rrd, _ = orc.tcorr3(host, acc["rdabh"])
+    assert np.array_equal(rd, rrd)
+    for k in NAMES:
+        assert _close(out[k], ref[k]), k
+
+
+def test_tcorr3_rejects_bad_input(gpu_ctx, tz):
+    from atlas12_amd import native
+    bad = dict(tz)
+    bad["n"] = 200
+    with pytest.raises(ValueError):
+        native.atlas_tcorr3(gpu_ctx, bad, tz["rdabh"])

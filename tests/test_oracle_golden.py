@@ -160,3 +160,42 @@ def test_scale_model_bitwise(orc, golden):
     assert [orc.scale_model(z["inp"][k].shape[1], 80, *z["params"][k][5:7], z["params"][k][:5],
                             *z["params"][k][7:9], z["inp"][k])[2] for k in range(5)] == \
         [False, False, True, True, True]
+
+
+def _printed(v, fmt):
+    return np.array([float(fmt % x) for x in v])
+
+
+def test_tcorr3_matches_reference_table(orc, golden):
+    """TCORR(3) after a full 30,000-frequency pass: DTLAMB, DTSURF, DTFLUX
+    and T1 equal the reference's per-iteration table at every printed digit
+    (F8.1), ERROR and DERIV too (ES11.2), CONV/TOT (F8.3).  The convective
+    branch is not reached in this model (CONVEC gives no convective flux)."""
+    z = golden("tcorr_ref.npz")
+    out, _, _ = orc.tcorr3(z, z["rdabh"])
+    for k in ("dtlamb", "dtsurf", "dtflux", "t1"):
+        assert np.array_equal(_printed(out[k], "%.1f") + 0.0, z["iter_" + k] + 0.0), k
+    for k in ("flxerr", "flxdrv"):
+        assert np.array_equal(_printed(out[k], "%.2E"), z["iter_" + k]), k
+    assert np.array_equal(_printed(out["hratio"], "%.3f"), z["iter_hratio"])
+
+
+def test_tcorr3_damping_rules(orc, golden):
+    """Iteration 2 against a planted previous correction: same sign and
+    shrinking -> x1.25, sign flip -> x0.5, same sign growing -> unchanged
+    (src/atlas12_modules.f90:2735-2751); the undamped sum otherwise equals
+    iteration 1's."""
+    z = golden("tcorr_ref.npz")
+    base, _, _ = orc.tcorr3(z, z["rdabh"])
+    t1 = base["t1"]
+    old = np.where(np.arange(len(t1)) % 3 == 0, 2.0 * t1, np.where(np.arange(len(t1)) % 3 == 1,
+                   -t1, 0.5 * t1))
+    zz = dict(z)
+    zz["ifconv"] = 0  # damping at every depth
+    out, _, new_old = orc.tcorr3(zz, z["rdabh"], oldt1=old, iter_=2)
+    k = np.arange(len(t1)) % 3
+    nz = t1 != 0
+    assert np.array_equal(out["t1"][(k == 0) & nz], t1[(k == 0) & nz] * 1.25)
+    assert np.array_equal(out["t1"][(k == 1) & nz], t1[(k == 1) & nz] * 0.5)
+    assert np.array_equal(out["t1"][(k == 2) & nz], t1[(k == 2) & nz])
+    assert np.array_equal(new_old, out["t1"])
