@@ -54,6 +54,8 @@ int a12_ctx_destroy(a12_ctx *ctx)
     DevBuf *sbufs[] = {&L.tabcont, &L.xnfdop, &L.iwt, &L.wavetab, &L.tablog, &L.xm,
                        &L.cand, &L.flag, &L.cnt, &L.off, &L.kept, &L.mol2};
     for (DevBuf *b : sbufs) b->release();
+    ctx->mol.tab.release();
+    ctx->mol.io.release();
     for (DevBuf *b : abufs) b->release();
     for (DevBuf *b : bufs) b->release();
     a12i_mlp_free(ctx);
@@ -444,6 +446,24 @@ int a12_atlas_tcorr3(a12_ctx *ctx, const a12_tcorr_in *in, double *rdabh, double
     if (!ctx_ok(ctx)) return A12_ERR_ARG;
     A12_HIP(hipSetDevice(ctx->device));
     return a12i_atlas_tcorr3(ctx, in, rdabh, oldt1, out);
+}
+
+int a12_molec_set_table(a12_ctx *ctx, const a12_moltab *tab)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_molec_set_table(ctx, tab);
+}
+
+int a12_molec_newton_batched(a12_ctx *ctx, int32_t nsys, int32_t nchain,
+                             const int32_t *chain_off, const double *xntot, const double *pratio,
+                             const double *xab, const double *equilj, double *xn, double *xnmol,
+                             int32_t *iters, int32_t *status)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_molec_newton(ctx, nsys, nchain, chain_off, xntot, pratio, xab, equilj, xn, xnmol,
+                             iters, status);
 }
 
 int a12_set_profiling(a12_ctx *ctx, int32_t on)

@@ -80,6 +80,12 @@ struct SelectDev {
     bool tablog_ready = false;
 };
 
+struct MolDev {
+    int nequa = 0, nummol = 0, nloc = 0;
+    bool charge = false, ready = false;
+    DevBuf tab, io;
+};
+
 struct MlpDev;  // a12_mlp.hip
 
 struct a12_ctx {
@@ -91,6 +97,7 @@ struct a12_ctx {
     SyntheDev syn;
     AtlasDev atl;
     SelectDev sel;
+    MolDev mol;
     MlpDev *mlp = nullptr;
     DevBuf rt_part;
     // scratch for a12_josh_batch
@@ -132,6 +139,12 @@ int a12i_atlas_autotune(a12_ctx *ctx, int prec, float *ms, int32_t *best);
 int a12i_atlas_select(a12_ctx *ctx, const a12_atlas_atm *a, const double *wavetab,
                       const int32_t *pred, int64_t npred, const a12_mol2_rec *mol2, int64_t nmol2,
                       int32_t *out, int64_t cap, int64_t *nkept);
+// Implemented in a12_molec.hip
+int a12i_molec_set_table(a12_ctx *ctx, const a12_moltab *tb);
+int a12i_molec_newton(a12_ctx *ctx, int32_t nsys, int32_t nchain, const int32_t *chain_off,
+                      const double *xntot, const double *pratio, const double *xab,
+                      const double *equilj, double *xn, double *xnmol, int32_t *iters,
+                      int32_t *status);
 // Implemented in a12_mlp.hip
 int a12i_mlp_init(a12_ctx *ctx, int batch, uint64_t seed);
 int a12i_mlp_params(a12_ctx *ctx, float *get, const float *set);

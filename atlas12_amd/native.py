@@ -35,7 +35,8 @@ EXPORTS = ["a12_ctx_create", "a12_ctx_destroy", "a12_last_error", "a12_stream",
            "a12_atlas_select_lines", "a12_atlas_stage_ms", "a12_atlas_shape_info",
            "a12_atlas_set_shape", "a12_atlas_autotune", "a12_scale_model", "a12_mlp_init",
            "a12_mlp_get_params", "a12_mlp_set_params", "a12_mlp_set_data", "a12_mlp_grads",
-           "a12_mlp_train", "a12_mlp_infer", "a12_mlp_gemm", "a12_atlas_tcorr3"]
+           "a12_mlp_train", "a12_mlp_infer", "a12_mlp_gemm", "a12_atlas_tcorr3",
+           "a12_molec_set_table", "a12_molec_newton_batched"]
 
 
 class A12Error(RuntimeError):
@@ -381,6 +382,46 @@ def atlas_tcorr3(ctx: Context, z, rdabh=None, oldt1=None, iter_=None, device_sum
     ctx._check(ctx.lib.a12_atlas_tcorr3(ctx.h, C.byref(s), _ptr(rd), _ptr(ot), _ptr(out)),
                "a12_atlas_tcorr3")
     return {k: out[i] for i, k in enumerate(TCORR_OUT)}, rd, ot
+
+
+class MolTab(C.Structure):
+    _fields_ = [("nequa", C.c_int32), ("nummol", C.c_int32), ("locj", C.POINTER(C.c_int32)),
+                ("kcomps", C.POINTER(C.c_int32)), ("idequa", C.POINTER(C.c_int32))]
+
+
+def molec_set_table(ctx: Context, locj, kcomps, idequa):
+    """READMOL's molecule table for NMOLEC's Newton solve (1-based arrays
+    as the reference holds them)."""
+    a = [np.ascontiguousarray(x, np.int32) for x in (locj, kcomps, idequa)]
+    t = MolTab(len(a[2]), len(a[0]) - 1, *[x.ctypes.data_as(C.POINTER(C.c_int32)) for x in a])
+    ctx._check(ctx.lib.a12_molec_set_table(ctx.h, C.byref(t)), "a12_molec_set_table")
+    ctx._moltab = (len(a[2]), len(a[0]) - 1)
+
+
+def molec_newton(ctx: Context, xntot, xab, equilj, xn0, chain_off=None, pratio=None,
+                 xnmol: bool = True):
+    """NMOLEC's gas-phase Newton solve (src/atlas12_modules.f90:9078-9217)
+    for a batch of systems on the device.  xntot (nsys), xab (nsys, nequa),
+    equilj (nsys, nummol), xn0 (nsys, nequa): the start of each chain head.
+    chain_off (nchain + 1) groups consecutive systems into NMOLEC depth
+    loops that warm-start with pratio[s] = P(J)/P(J-1); None: every system
+    on its own.  Returns (xn, xnmol or None, iterations, status)."""
+    ne, nm = ctx._moltab
+    xt = _f64(xntot)
+    n = len(xt)
+    xa, eq = _f64(xab).reshape(n, ne), _f64(equilj).reshape(n, nm)
+    xn = _f64(np.array(xn0, np.float64).reshape(n, ne).copy())
+    co = np.ascontiguousarray(np.arange(n + 1) if chain_off is None else chain_off, np.int32)
+    pr = None if pratio is None else _f64(pratio)
+    xm = np.zeros((n, nm)) if xnmol else None
+    it = np.zeros(n, np.int32)
+    st = np.zeros(n, np.int32)
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))  # noqa: E731
+    ctx._check(ctx.lib.a12_molec_newton_batched(
+        ctx.h, C.c_int32(n), C.c_int32(len(co) - 1), ip(co), _ptr(xt),
+        None if pr is None else _ptr(pr), _ptr(xa), _ptr(eq), _ptr(xn),
+        None if xm is None else _ptr(xm), ip(it), ip(st)), "a12_molec_newton_batched")
+    return xn, xm, it, st
 
 
 def atlas_stage_ms(ctx: Context):

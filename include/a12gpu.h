@@ -218,6 +218,39 @@ typedef struct {
 int a12_atlas_tcorr3(a12_ctx *ctx, const a12_tcorr_in *in, double *rdabh, double *oldt1,
                      double *out);
 
+/* NMOLEC's gas-phase Newton solve, src/atlas12_modules.f90:9078-9217, for
+ * a batch of systems (depths of one or many models).  The molecule table
+ * is READMOL's, 1-based as in the reference: locj (nummol + 1) the first
+ * component of each entry in kcomps; kcomps (locj[nummol] - 1) the
+ * equation of each component (2..nequa are elements, nequa + 1 is the
+ * electron as a divisor); idequa (nequa) the element of each equation, 100
+ * for the electron (charge) equation.  Sizes as the reference's maxeq and
+ * maxmol (:162-167). */
+#define A12_MOL_MAXEQ  64
+#define A12_MOL_MAXMOL 400
+typedef struct {
+    int32_t nequa, nummol;
+    const int32_t *locj, *kcomps, *idequa;
+} a12_moltab;
+int a12_molec_set_table(a12_ctx *ctx, const a12_moltab *tab);
+/* nsys systems in nchain chains: chain c is systems chain_off[c] ..
+ * chain_off[c+1]-1 and runs as NMOLEC's depth loop (:8879-8884): its first
+ * system starts from xn, each later one from the previous solution times
+ * pratio[s] (= P(J)/P(J-1)).  Chains of length one are independent solves
+ * (pratio may then be NULL).  Per system s: xntot[s] = P/kT; xab (nsys,
+ * nequa) element abundances, max(XABUND, 1e-20), 0 for the electron row
+ * (entry 1 unused); equilj (nsys, nummol) the equilibrium constants as
+ * NMOLEC forms them (:8897-8946; the caller's PFSAHA); xn (nsys, nequa)
+ * start in (chain heads), solution out.  xnmol (nsys, nummol, may be NULL)
+ * = XNMOL (:9624-9636).  iters: Newton iterations; status 0 converged, 1
+ * NaN (the reference prints a warning and keeps the state), 2 no
+ * convergence in 5000 iterations.  Host pointers, synchronous.  One
+ * wavefront per chain, bit-identical to the serial routine. */
+int a12_molec_newton_batched(a12_ctx *ctx, int32_t nsys, int32_t nchain,
+                             const int32_t *chain_off, const double *xntot, const double *pratio,
+                             const double *xab, const double *equilj, double *xn, double *xnmol,
+                             int32_t *iters, int32_t *status);
+
 /* MOL2 32-byte molecular record (header: int32 magic 'MOL2', int32 count,
  * 6 more int32), src/atlas12_modules.f90:19540-19556. */
 typedef struct {

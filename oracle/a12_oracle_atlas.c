@@ -581,3 +581,107 @@ void or_tcorr3(const or_tcorr_in *in, double *rdabh, double *oldt1, double *out)
         oldt1[j] = t1[j];
     }
 }
+
+/* ---- NMOLEC gas-phase Newton, src/atlas12_modules.f90:9078-9217 ------- */
+#define MOL_MAXEQ 64
+int or_molec_newton(const or_moltab *tb, double xntot, const double *xab, const double *equilj,
+                    double *xn, int *status)
+{
+    const int ne = tb->nequa, ne1 = ne + 1;
+    double deq[MOL_MAXEQ * MOL_MAXEQ], eq[MOL_MAXEQ], eqold[MOL_MAXEQ];
+#define DEQ(i, j) deq[((j) - 1) * ne + ((i) - 1)]
+#define XN(k) xn[(k) - 1]
+#define EQ(k) eq[(k) - 1]
+    for (int k = 0; k < ne; ++k) eqold[k] = 0.0;
+    const int charge = tb->idequa[ne - 1] >= 100;
+    int it = 0;
+    *status = 0;
+    for (;;) {
+        ++it;
+        for (int k = 0; k < ne * ne; ++k) deq[k] = 0.0;
+        EQ(1) = -xntot;                        /* total particles :9084 */
+        for (int k = 2; k <= ne; ++k) {
+            EQ(1) = EQ(1) + XN(k);
+            DEQ(1, k) = 1.0;
+            EQ(k) = XN(k) - xab[k - 1] * XN(1);  /* element k :9092 */
+            DEQ(k, k) = 1.0;
+            DEQ(k, 1) = -xab[k - 1];
+        }
+        if (charge) {                          /* charge :9099-9102 */
+            EQ(ne) = -XN(ne);
+            DEQ(ne, ne) = -1.0;
+        }
+        for (int jm = 0; jm < tb->nummol; ++jm) {  /* molecules :9105-9162 */
+            const int l1 = tb->locj[jm], l2 = tb->locj[jm + 1] - 1;
+            if (l2 - l1 + 1 == 1) continue;
+            double term = equilj[jm];
+            for (int lk = l1; lk <= l2; ++lk) {
+                const int k = tb->kcomps[lk - 1];
+                if (k == ne1) term = term / XN(ne);
+                else term = term * XN(k);
+            }
+            EQ(1) = EQ(1) + term;
+            for (int lk = l1; lk <= l2; ++lk) {
+                int k = tb->kcomps[lk - 1];
+                double d;
+                if (k >= ne1) {
+                    k = ne;
+                    d = -term / XN(k);
+                } else {
+                    d = term / XN(k);
+                }
+                EQ(k) = EQ(k) + term;
+                DEQ(1, k) = DEQ(1, k) + d;
+                for (int lm = l1; lm <= l2; ++lm) {
+                    int m = tb->kcomps[lm - 1];
+                    if (m == ne1) m = ne;
+                    DEQ(m, k) = DEQ(m, k) + d;
+                }
+            }
+            /* negative ions: the last component is the electron equation
+             * (IDEQUA beyond NEQUA reads as 0) :9145-9160 */
+            const int kl = tb->kcomps[l2 - 1];
+            if (kl <= ne && tb->idequa[kl - 1] == 100) {
+                for (int lk = l1; lk <= l2; ++lk) {
+                    const int k = tb->kcomps[lk - 1];
+                    const double d = term / XN(k);
+                    if (k == ne) EQ(k) = EQ(k) - term - term;
+                    for (int lm = l1; lm <= l2; ++lm) {
+                        const int m = tb->kcomps[lm - 1];
+                        if (m == ne) DEQ(m, k) = DEQ(m, k) - d - d;
+                    }
+                }
+            }
+        }
+        or_solvit(deq, ne, eq);                /* :9165 */
+        int iferr = 0;                         /* damped update :9168-9189 */
+        double scale = 100.0;
+        for (int k = 1; k <= ne; ++k) {
+            const double ratio = fabs(EQ(k) / XN(k));
+            if (ratio > 1.0e-4) iferr = 1;
+            if (eqold[k - 1] * EQ(k) < 0.0) EQ(k) = EQ(k) * 0.69;
+            const double xneq = XN(k) - EQ(k);
+            const double xn100 = XN(k) / 100.0;
+            if (fabs(xneq) >= xn100) {
+                XN(k) = fabs(xneq);
+            } else {
+                XN(k) = XN(k) / scale;
+                if (eqold[k - 1] * EQ(k) < 0.0) scale = sqrt(scale);
+            }
+            eqold[k - 1] = EQ(k);
+        }
+        if (XN(1) != XN(1) || EQ(1) != EQ(1)) { /* :9193-9202 */
+            *status = 1;
+            break;
+        }
+        if (iferr == 0) break;
+        if (it >= 5000) {
+            *status = 2;
+            break;
+        }
+    }
+#undef DEQ
+#undef XN
+#undef EQ
+    return it;
+}

@@ -102,6 +102,21 @@ typedef struct {
 } or_tcorr_in;
 void or_tcorr3(const or_tcorr_in *in, double *rdabh, double *oldt1, double *out);
 
+/* NMOLEC's molecule table (READMOL): 1-based as in the reference.  locj
+ * (nummol + 1) start of each entry's components in kcomps; kcomps the
+ * equation index of each component (nequa + 1 = electron, divides);
+ * idequa (nequa) element of each equation (100 = electrons). */
+typedef struct {
+    int nequa, nummol;
+    const int *locj, *kcomps, *idequa;
+} or_moltab;
+/* The gas-phase Newton solve of one depth, src/atlas12_modules.f90:
+ * 9078-9217.  xab (nequa) element abundances (entry 1 unused), equilj
+ * (nummol), xn (nequa) start in, solution out.  Returns the iteration
+ * count; *status 0 converged, 1 NaN, 2 iteration cap (5000). */
+int or_molec_newton(const or_moltab *tb, double xntot, const double *xab, const double *equilj,
+                    double *xn, int *status);
+
 #ifdef __cplusplus
 }
 #endif

@@ -421,3 +421,86 @@ def tcorr3(z, rdabh, oldt1=None, iter_=None):
     lib().or_tcorr3(C.byref(s), _p(rd), _p(ot), _p(out))
     names = ("t1", "dtflux", "dtlamb", "dtsurf", "flxerr", "flxdrv", "hratio")
     return {k: out[i] for i, k in enumerate(names)}, rd, ot
+
+
+class OrMoltab(C.Structure):
+    _fields_ = [("nequa", C.c_int), ("nummol", C.c_int), ("locj", C.POINTER(C.c_int)),
+                ("kcomps", C.POINTER(C.c_int)), ("idequa", C.POINTER(C.c_int))]
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def molec_newton(z, xntot, xab, equilj, xn0):
+    """NMOLEC's gas-phase Newton solve of one depth
+    (src/atlas12_modules.f90:9078-9217).  z: the molecule table (locj,
+    kcomps, idequa as READMOL, 1-based).  Returns (xn, iterations,
+    status 0 converged / 1 NaN / 2 cap)."""
+    keep = [_i32(z["locj"]), _i32(z["kcomps"]), _i32(z["idequa"])]
+    tb = OrMoltab(len(keep[2]), len(keep[0]) - 1,
+                  *[k.ctypes.data_as(C.POINTER(C.c_int)) for k in keep])
+    xn = _f64(np.array(xn0, np.float64).copy())
+    xa, eq = _f64(xab), _f64(equilj)
+    st = C.c_int(0)
+    L = lib()
+    L.or_molec_newton.restype = C.c_int
+    it = L.or_molec_newton(C.byref(tb), C.c_double(xntot), _p(xa), _p(eq), _p(xn), C.byref(st))
+    return xn, int(it), int(st.value)
+
+
+def nmolec_inputs(z, case):
+    """Per-depth XNTOT = P/kT and XAB (:8864-8876) of a tests/golden/
+    molec_ref.npz case, and the surface start (:8851-8860)."""
+    p, tk, t = z[case + "_p"], z[case + "_tk"], z[case + "_t"]
+    ide = np.asarray(z["idequa"])
+    xab = np.zeros((len(p), len(ide)))
+    for k in range(1, len(ide)):
+        if ide[k] < 100:
+            xab[:, k] = np.maximum(z[case + "_xabund"][ide[k] - 1], 1.0e-20)
+    xntot = p / tk
+    xn1 = xntot[0] / 2.0 if t[0] >= 4000.0 else xntot[0]
+    x = xn1 / 10.0
+    start = x * xab[0]
+    start[0] = xn1
+    if ide[-1] == 100:
+        start[-1] = x
+    return xntot, xab, start
+
+
+def nmolec_chain(z, case):
+    """NMOLEC's depth loop (:8862-9217 without condensation, which is off
+    above COND_TMAX = 2600 K): each depth starts from the previous one's
+    solution times P(J)/P(J-1).  Returns (xn (nrhox, nequa), iterations,
+    statuses)."""
+    xntot, xab, xn = nmolec_inputs(z, case)
+    p = z[case + "_p"]
+    eqj = z[case + "_equilj"]
+    out, its, sts = [], [], []
+    for j in range(len(p)):
+        if j > 0:
+            xn = xn * (p[j] / p[j - 1])
+        xn, it, st = molec_newton(z, xntot[j], xab[j], eqj[j], xn)
+        out.append(xn)
+        its.append(it)
+        sts.append(st)
+    return np.array(out), np.array(its), np.array(sts)
+
+
+def nmolec_reference_xn(z, case):
+    """The reference's converged solution vector per depth, read back from
+    its outputs: XN(1) = XNATOM, XN(NEQUA) = XNE, and each atom equation's
+    XN(K) = XNMOL of its one-component entry (EQUILJ = 1, :8937-8939)."""
+    ide = np.asarray(z["idequa"])
+    locj, kc = np.asarray(z["locj"]), np.asarray(z["kcomps"])
+    n = len(z[case + "_p"])
+    xn = np.full((n, len(ide)), np.nan)
+    xn[:, 0] = z[case + "_xnatom"]
+    if ide[-1] == 100:
+        xn[:, -1] = z[case + "_xne"]
+    for jm in range(len(locj) - 1):
+        if locj[jm + 1] - locj[jm] == 1:
+            k = kc[locj[jm] - 1]
+            if k <= len(ide) and ide[k - 1] < 100:
+                xn[:, k - 1] = z[case + "_xnmol"][jm]
+    return xn

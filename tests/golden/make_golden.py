@@ -613,10 +613,68 @@ def golden_scale():
     return {"scale_model": {"cases": cases, "nrhox": n, "kw": kw}}
 
 
+def read_molec_dump(path):
+    b = open(path, "rb").read()
+    n, nequa, nummol, nloc = (int(x) for x in np.frombuffer(b, "<i4", 4))
+    o = 16
+    ints = np.frombuffer(b, "<i4", nummol + 1 + nloc + nequa, o).astype(np.int32)
+    o += 4 * len(ints)
+    d = {"locj": ints[:nummol + 1], "kcomps": ints[nummol + 1:nummol + 1 + nloc],
+         "idequa": ints[nummol + 1 + nloc:]}
+
+    def f8(cnt, shape=None):
+        nonlocal o
+        a = np.frombuffer(b, "<f8", cnt, o).copy()
+        o += 8 * cnt
+        return a if shape is None else a.reshape(shape)
+    d["xnmolcode"] = f8(nummol)
+    d["p"], d["tk"], d["t"] = f8(n), f8(n), f8(n)
+    d["xabund"] = f8(99 * n, (99, n))            # (element, depth)
+    d["equilj"] = f8(nummol * n, (n, nummol))    # (depth, molecule)
+    d["xnatom"], d["xne"] = f8(n), f8(n)
+    d["xnmol"] = f8(nummol * n, (nummol, n))     # (molecule, depth)
+    assert o == len(b)
+    return d
+
+
+def golden_molec():
+    """NMOLEC's gas-phase Newton solve (src/atlas12_modules.f90:9078-9217)
+    on two decks: the plain-SYNTHE atmosphere (4400-9500 K) and a cool dwarf
+    (2800-6000 K, log g 5) where molecules hold most of C, N, O and H.
+    Inputs as NMOLEC forms them and the reference's converged outputs."""
+    cases = {"solar": wl.scurve_atmosphere(),
+             "cool": wl.scurve_atmosphere(t_top=2800.0, t_bot=6000.0, logg=5.0, teff=3500.0)}
+    tmp = tempfile.mkdtemp(prefix="a12m_")
+    g = os.path.join(tmp, "gfall.dat")
+    open(g, "w").close()
+    root = data_root(tmp, g)
+    rec, info = {}, {}
+    for name, atm in cases.items():
+        deck = os.path.join(tmp, f"{name}.dat")
+        with open(deck, "w") as fh:
+            fh.write(atm.deck())
+        out = os.path.join(tmp, f"{name}.bin")
+        run([os.path.join(REFBIN, "ref_molec_dump"), deck, out], tmp, dict(os.environ, ATLAS12=root))
+        d = read_molec_dump(out)
+        for k, v in d.items():
+            if k in ("locj", "kcomps", "idequa", "xnmolcode"):
+                if k in rec:
+                    assert np.array_equal(rec[k], v), k  # one table for both decks
+                rec[k] = v
+            else:
+                rec[f"{name}_{k}"] = v
+        info[name] = {"nrhox": int(len(d["p"])), "t_range": [float(d["t"][0]), float(d["t"][-1])]}
+    shutil.rmtree(tmp)
+    np.savez_compressed(os.path.join(HERE, "molec_ref.npz"), **rec)
+    info["nequa"] = int(len(rec["idequa"]))
+    info["nummol"] = int(len(rec["xnmolcode"]))
+    return {"molec": info}
+
+
 STEPS = {"synthe_small": golden_synthe_small, "atlas_small": golden_atlas_small,
          "state_atlas": golden_state_atlas, "select": golden_select, "scale": golden_scale,
          "tcorr": golden_tcorr, "state_plain": golden_state_plain,
-         "numerics": golden_numerics, "josh": golden_josh}
+         "numerics": golden_numerics, "josh": golden_josh, "molec": golden_molec}
 
 
 def main():
