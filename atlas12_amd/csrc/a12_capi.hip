@@ -40,10 +40,7 @@ int a12_ctx_destroy(a12_ctx *ctx)
     SyntheDev &S = ctx->syn;
     DevBuf *bufs[] = {&S.wledge, &S.continall, &S.contabs, &S.contscat, &S.xnfpel, &S.dopple,
                       &S.depth, &S.l_nbuff, &S.l_nelion, &S.l_cgf, &S.l_elo, &S.l_gr, &S.l_gs,
-                      &S.l_gw, &S.xnfdop, &S.dscal, &S.cont32, &S.cont64, &S.p_nstep, &S.p_n10,
-                      &S.p_kap0, &S.p_adamp, &S.p_dv, &S.p_xwing, &S.emax, &S.lclass,
-                      &S.cls_cnt, &S.cls_off, &S.perm, &S.nbuffP, &S.cls_range, &S.blk_evals,
-                      &S.evals_out, &S.opacity, &S.hline, &S.hcont, &ctx->coefj, &ctx->coefh,
+                      &S.l_gw, &S.work, &S.opacity, &S.hline, &S.hcont, &ctx->coefj, &ctx->coefh,
                       &ctx->jb_in, &ctx->jb_out, &ctx->rt_part};
     AtlasDev &A = ctx->atl;
     DevBuf *abufs[] = {&A.depth3, &A.depth_lo, &A.tabcont, &A.xnfdop, &A.dopple, &A.waveset,
@@ -59,6 +56,7 @@ int a12_ctx_destroy(a12_ctx *ctx)
     for (DevBuf *b : abufs) b->release();
     for (DevBuf *b : bufs) b->release();
     a12i_mlp_free(ctx);
+    a12i_synthe_batch_free(ctx);
     for (hipEvent_t e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -124,7 +122,7 @@ int a12_synthe_opacity_fetch(a12_ctx *ctx, void *opacity_out)
     A12_HIP(hipMemcpyAsync(opacity_out, S.opacity.p, rb * (size_t)S.L * S.nrhox,
                            hipMemcpyDeviceToHost, ctx->stream));
     long long ev = 0;
-    A12_HIP(hipMemcpyAsync(&ev, S.evals_out.p, sizeof(long long), hipMemcpyDeviceToHost,
+    A12_HIP(hipMemcpyAsync(&ev, S.evals_dev, sizeof(long long), hipMemcpyDeviceToHost,
                            ctx->stream));
     A12_HIP(hipStreamSynchronize(ctx->stream));
     S.evals = ev;
@@ -142,7 +140,7 @@ int64_t a12_synthe_evals(a12_ctx *ctx)
 {
     if (!ctx || ctx->syn.opac_prec < 0) return -1;
     long long ev = 0;
-    if (hipMemcpyAsync(&ev, ctx->syn.evals_out.p, sizeof(long long), hipMemcpyDeviceToHost,
+    if (hipMemcpyAsync(&ev, ctx->syn.evals_dev, sizeof(long long), hipMemcpyDeviceToHost,
                        ctx->stream) != hipSuccess)
         return -1;
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) return -1;
@@ -466,6 +464,53 @@ int a12_molec_newton_batched(a12_ctx *ctx, int32_t nsys, int32_t nchain,
                              iters, status);
 }
 
+int a12_synthe_shape_info(int32_t idx, int32_t pts_chunk[2])
+{
+    return a12i_synthe_shapes(idx, pts_chunk);
+}
+
+int a12_synthe_set_shape(a12_ctx *ctx, int32_t shape)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    return a12i_synthe_set_shape(ctx, shape);
+}
+
+int a12_synthe_batch_load(a12_ctx *ctx, int32_t nmodel, const a12_synthe_atm *atms,
+                          const int32_t *nb_shift, const int64_t *line_lo,
+                          const int64_t *line_hi, const a12_lte_line *lines, int64_t nlines)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_synthe_batch_load(ctx, nmodel, atms, nb_shift, line_lo, line_hi, lines, nlines);
+}
+
+int a12_synthe_batch_set_budget(a12_ctx *ctx, int64_t bytes)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    return a12i_synthe_batch_set_budget(ctx, bytes);
+}
+
+int a12_synthe_batch_run(a12_ctx *ctx, int32_t prec, int32_t transfer, int32_t use_graph)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_synthe_batch_run(ctx, prec, transfer, use_graph);
+}
+
+int a12_synthe_batch_fetch(a12_ctx *ctx, int32_t model, void *opacity_out, double *hnu_line,
+                           double *hnu_cont, int64_t evals[2])
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_synthe_batch_fetch(ctx, model, opacity_out, hnu_line, hnu_cont, evals);
+}
+
+int a12_synthe_batch_info(a12_ctx *ctx, int64_t info[5])
+{
+    if (!ctx_ok(ctx) || !info) return A12_ERR_ARG;
+    return a12i_synthe_batch_info(ctx, info);
+}
+
 int a12_set_profiling(a12_ctx *ctx, int32_t on)
 {
     if (!ctx_ok(ctx)) return A12_ERR_ARG;
@@ -519,7 +564,7 @@ int a12_synthe_eval_split(a12_ctx *ctx, int64_t *total, int64_t *core)
         return A12_ERR_STATE;
     }
     long long ev[2] = {0, 0};
-    A12_HIP(hipMemcpyAsync(ev, ctx->syn.evals_out.p, sizeof(ev), hipMemcpyDeviceToHost,
+    A12_HIP(hipMemcpyAsync(ev, ctx->syn.evals_dev, sizeof(ev), hipMemcpyDeviceToHost,
                            ctx->stream));
     A12_HIP(hipStreamSynchronize(ctx->stream));
     if (total) *total = ev[0];

@@ -44,14 +44,11 @@ struct SyntheDev {
     DevBuf depth;  // 10 x nrhox doubles: t xne rho xnfh xnfhe1 xnfh2 hckt hkt hkt_atlas rhox
     // lines sorted by centre (SoA)
     DevBuf l_nbuff, l_nelion, l_cgf, l_elo, l_gr, l_gs, l_gw;
-    // derived per depth
-    DevBuf xnfdop, dscal;   // xnfdop (nrhox x mw6); dscal: txnxn32/txnxn64/xne32 per depth
-    DevBuf cont32, cont64;  // (nrhox x L)
-    // per (depth, line) setup
-    DevBuf p_nstep, p_n10, p_kap0, p_adamp, p_dv, p_xwing;
-    DevBuf emax, lclass;
-    DevBuf cls_cnt, cls_off, perm, nbuffP, cls_range;  // partition + per-tile ranges
-    DevBuf blk_evals, evals_out;
+    // work arrays of the pipeline (per-depth prep, continuum, per (depth, line)
+    // set-up, class partition, tile ranges, evaluation counts), one allocation
+    DevBuf work;
+    long long *evals_dev = nullptr;  // [all, Voigt core] of the last run, in work
+    int shape = 0;                   // gather tile shape (a12_synthe_set_shape)
     // outputs
     DevBuf opacity;  // (nrhox x L) float or double
     int opac_prec = -1;
@@ -86,7 +83,8 @@ struct MolDev {
     DevBuf tab, io;
 };
 
-struct MlpDev;  // a12_mlp.hip
+struct MlpDev;    // a12_mlp.hip
+struct SynBatch;  // a12_synthe.hip
 
 struct a12_ctx {
     int device = 0;
@@ -99,6 +97,7 @@ struct a12_ctx {
     SelectDev sel;
     MolDev mol;
     MlpDev *mlp = nullptr;
+    SynBatch *sbatch = nullptr;
     DevBuf rt_part;
     // scratch for a12_josh_batch
     DevBuf jb_in, jb_out;
@@ -130,6 +129,17 @@ int a12i_synthe_load(a12_ctx *ctx, const a12_synthe_atm *atm, const a12_lte_line
                      int32_t nlines);
 int a12i_synthe_run(a12_ctx *ctx, int prec);
 int a12i_voigt(a12_ctx *ctx, int prec, int n, const float *v, const float *a, double *h);
+int a12i_synthe_shapes(int idx, int32_t *pts_chunk);
+int a12i_synthe_set_shape(a12_ctx *ctx, int shape);
+int a12i_synthe_batch_load(a12_ctx *ctx, int32_t nmodel, const a12_synthe_atm *atms,
+                           const int32_t *nb_shift, const int64_t *line_lo,
+                           const int64_t *line_hi, const a12_lte_line *lines, int64_t nlines);
+int a12i_synthe_batch_set_budget(a12_ctx *ctx, int64_t bytes);
+int a12i_synthe_batch_run(a12_ctx *ctx, int prec, int transfer, int use_graph);
+int a12i_synthe_batch_fetch(a12_ctx *ctx, int32_t model, void *opacity_out, double *hline,
+                            double *hcont, int64_t *evals);
+int a12i_synthe_batch_info(a12_ctx *ctx, int64_t *info);
+void a12i_synthe_batch_free(a12_ctx *ctx);
 // Implemented in a12_atlas.hip
 int a12i_atlas_load(a12_ctx *ctx, const a12_atlas_atm *a, const int32_t *linedata, int32_t nlines);
 int a12i_atlas_linop1_run(a12_ctx *ctx, int prec);
@@ -160,6 +170,11 @@ int a12i_atlas_rt_pass(a12_ctx *ctx, int n, int nf, double teff, const double *d
                        const double *d_rcoset, const double *d_depth3, const double *d_kapp,
                        const double *d_xlines, double *d_hnu1, double *d_acc);
 int a12i_synthe_transfer(a12_ctx *ctx);
+int a12i_synthe_transfer_launch(a12_ctx *ctx, hipStream_t st, bool f64, int L, int n,
+                                double wbegin, double ratio, int nedge, const double *wledge,
+                                int nf, const double *contabs, const double *contscat,
+                                const double *depth, const void *opacity, double *hline,
+                                double *hcont);
 int a12i_atlas_tcorr3(a12_ctx *ctx, const a12_tcorr_in *in, double *rdabh, double *oldt1,
                       double *out);
 int a12i_scale_model(a12_ctx *ctx, int nmodel, int n, int kw, double tau1lg, double steplg,

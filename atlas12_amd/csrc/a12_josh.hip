@@ -973,6 +973,21 @@ __global__ __launch_bounds__(64) void k_tcorr3(int n, int ifconv, int iter, doub
 
 }  // namespace
 
+int a12i_synthe_transfer_launch(a12_ctx *ctx, hipStream_t st, bool f64, int L, int n,
+                                double wbegin, double ratio, int nedge, const double *wledge,
+                                int nf, const double *contabs, const double *contscat,
+                                const double *depth, const void *opacity, double *hline,
+                                double *hcont)
+{
+    const int grid = std::min(L, 4096);
+    auto k = f64 ? k_synthe_transfer<true> : k_synthe_transfer<false>;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, st, L, n, wbegin, ratio, nedge, wledge, nf,
+                       contabs, contscat, depth, opacity, ctx->coefj.as<double>(),
+                       ctx->coefh.as<double>(), hline, hcont);
+    A12_CHECK_LAUNCH();
+    return A12_OK;
+}
+
 int a12i_synthe_transfer(a12_ctx *ctx)
 {
     SyntheDev &S = ctx->syn;
@@ -990,15 +1005,12 @@ int a12i_synthe_transfer(a12_ctx *ctx)
     }
     A12_HIP(S.hline.reserve(sizeof(double) * S.L));
     A12_HIP(S.hcont.reserve(sizeof(double) * S.L));
-    const int grid = std::min(S.L, 4096);
-    auto k = S.opac_prec == A12_PREC_FP64 ? k_synthe_transfer<true> : k_synthe_transfer<false>;
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
-    hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, ctx->stream, S.L, S.nrhox, S.wbegin, S.ratio,
-                       S.nedge, S.wledge.as<double>(), S.nf, S.contabs.as<double>(),
-                       S.contscat.as<double>(), S.depth.as<double>(), S.opacity.p,
-                       ctx->coefj.as<double>(), ctx->coefh.as<double>(), S.hline.as<double>(),
-                       S.hcont.as<double>());
-    A12_CHECK_LAUNCH();
+    const int rc = a12i_synthe_transfer_launch(
+        ctx, ctx->stream, S.opac_prec == A12_PREC_FP64, S.L, S.nrhox, S.wbegin, S.ratio, S.nedge,
+        S.wledge.as<double>(), S.nf, S.contabs.as<double>(), S.contscat.as<double>(),
+        S.depth.as<double>(), S.opacity.p, S.hline.as<double>(), S.hcont.as<double>());
+    if (rc != A12_OK) return rc;
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[6], ctx->stream));
     S.transferred = true;
     return A12_OK;

@@ -146,7 +146,7 @@ __device__ inline long evals_of(int nb, int ns, int L)
 // n10dop (which also decides whether the n^-2 far wing follows).
 // --------------------------------------------------------------------------
 template <bool F64>
-__global__ void k_line_setup(int nlines, int L, int mw6, double resolu,
+__global__ void k_line_setup(int nlines, int L, int mw6, double resolu, int nb_shift,
                              const int *__restrict__ nbuff, const int *__restrict__ nelion,
                              const float *__restrict__ cgf, const float *__restrict__ elo,
                              const float *__restrict__ gr, const float *__restrict__ gs,
@@ -165,7 +165,7 @@ __global__ void k_line_setup(int nlines, int L, int mw6, double resolu,
     long ev = 0, evc = 0;
     if (l < nlines) {
         const size_t o = (size_t)j * nlines + l;
-        const int nb = nbuff[l];
+        const int nb = nbuff[l] - nb_shift;
         const int nel = nelion[l] - 1;
         const int ci = nb < 1 ? 0 : (nb > L ? L - 1 : nb - 1);
         const double dop = dopple[(size_t)j * mw6 + nel];
@@ -310,6 +310,13 @@ __global__ void k_sum_evals(const long long *__restrict__ blk, int n, long long 
     }
 }
 
+// Zero fill (graph-capturable, unlike a memset node).
+__global__ void k_zero_ll(long long *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 // Widest extent of each line over all depths, and its extent class.
 __global__ void k_line_emax(int nlines, int nrhox, const int *__restrict__ nstep,
                             int *__restrict__ emax, int *__restrict__ lclass)
@@ -363,7 +370,7 @@ __global__ void k_cls_scan(int n, const int *__restrict__ cnt, int *__restrict__
     if (threadIdx.x == blockDim.x - 1) off[n] = run;
 }
 
-__global__ void k_cls_scatter(int nlines, const int *__restrict__ lclass,
+__global__ void k_cls_scatter(int nlines, int nb_shift, const int *__restrict__ lclass,
                               const int *__restrict__ off, const int *__restrict__ nbuff,
                               int *__restrict__ perm, int *__restrict__ nbuffP)
 {
@@ -382,13 +389,13 @@ __global__ void k_cls_scatter(int nlines, const int *__restrict__ lclass,
         int pos = off[(size_t)c * gridDim.x + blockIdx.x] + rank_in_wave;
         for (int w = 0; w < wv; ++w) pos += wcnt[w][c];
         perm[pos] = l;
-        nbuffP[pos] = nbuff[l];
+        nbuffP[pos] = nbuff[l] - nb_shift;
     }
 }
 
 // Per (tile, class): [lo, hi) of class positions whose centre lies within the
 // class radius of the tile (1-based grid points t0+1 .. t1).
-__global__ void k_tile_ranges(int ntiles, int L, const int *__restrict__ off, int nblk,
+__global__ void k_tile_ranges(int ntiles, int tw, int L, const int *__restrict__ off, int nblk,
                               const int *__restrict__ nbuffP, int *__restrict__ rng)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -396,7 +403,7 @@ __global__ void k_tile_ranges(int ntiles, int L, const int *__restrict__ off, in
     const int tile = t / (A12_NCLASS - 1), c = t % (A12_NCLASS - 1);
     const int cs = off[(size_t)c * nblk], ce = off[(size_t)(c + 1) * nblk];
     const long R = class_radius(c);
-    const long g0 = (long)tile * kTile + 1, g1 = min((long)(tile + 1) * kTile, (long)L);
+    const long g0 = (long)tile * tw + 1, g1 = min((long)(tile + 1) * tw, (long)L);
     const long vlo = g0 - R, vhi = g1 + R;
     int a = cs, b = ce;  // first position with nbuffP >= vlo
     while (a < b) {
@@ -416,105 +423,186 @@ __global__ void k_tile_ranges(int ntiles, int L, const int *__restrict__ off, in
 }
 
 // --------------------------------------------------------------------------
-// Tile gather: one workgroup per (tile of 256 points, depth), one point per
-// thread.  Candidate lines of each class are staged in LDS in chunks of 256;
-// lines whose window at this depth misses the tile are dropped while staging
-// (wave ballot compaction, order preserved).  Each thread then adds the
-// profile of every staged line that covers its point:
+// Tile gather: one workgroup of 256 threads per (tile, depth); a tile is
+// 256 * PTS grid points, thread t owning points t, t + 256, ...  Candidate
+// lines of each class are staged in LDS CHUNK at a time; lines whose window
+// at this depth misses the tile are dropped while staging (wave ballot
+// compaction, order preserved).  Each thread then adds the profile of every
+// staged line that covers its points:
 //   n = |i - nbuff| <= nstep:  n <= n10dop -> kappa0 H(n dv, a)
 //                              else        -> x_wing / n^2
 // MIXED evaluates H in FP32 and accumulates with Neumaier compensation;
-// FP64 does both in double.
+// FP64 does both in double.  Every point sees the lines in the same order
+// (class, then centre position) whatever PTS and CHUNK are, so the result
+// does not depend on the tile shape.
 // --------------------------------------------------------------------------
-template <bool F64>
-__global__ __launch_bounds__(256) void k_tile(int L, int nlines, const int *__restrict__ rng,
-                                              const int *__restrict__ perm,
-                                              const int *__restrict__ nbuffP,
-                                              const int *__restrict__ p_nstep,
-                                              const int *__restrict__ p_n10,
-                                              const void *__restrict__ p_kap0,
-                                              const void *__restrict__ p_adamp,
-                                              const void *__restrict__ p_dv,
-                                              const void *__restrict__ p_xwing,
-                                              void *__restrict__ out)
+struct TileArgs {
+    int L, nlines;
+    const int *rng, *perm, *nbuffP, *nstep, *n10;
+    const void *kap0, *adamp, *dv, *xwing;
+    void *out;
+};
+
+template <bool F64, int PTS, int CHUNK>
+__device__ __forceinline__ void tile_body(const TileArgs &a, int tile, int j)
 {
     using R = typename std::conditional<F64, double, float>::type;
-    __shared__ int s_nb[kChunk], s_ns[kChunk], s_n10[kChunk];
-    __shared__ R s_k0[kChunk], s_a[kChunk], s_dv[kChunk], s_xw[kChunk];
-    __shared__ int s_wcnt[kChunk / A12_WAVE];
+    constexpr int kTw = kTile * PTS;
+    __shared__ int s_nb[CHUNK], s_ns[CHUNK], s_n10[CHUNK];
+    __shared__ R s_k0[CHUNK], s_a[CHUNK], s_dv[CHUNK], s_xw[CHUNK];
+    __shared__ int s_wcnt[kTile / A12_WAVE];
     __shared__ int s_nact;
 
-    const int tile = blockIdx.x, j = blockIdx.y;
     const int tid = threadIdx.x, lane = tid & (A12_WAVE - 1), wv = tid / A12_WAVE;
-    const int g0 = tile * kTile + 1;                  // first 1-based point
-    const int g1 = min((tile + 1) * kTile, L);        // last
-    const int gi = g0 + tid;                          // this thread's point
-    const size_t dj = (size_t)j * nlines;
-    const R *k0a = static_cast<const R *>(p_kap0) + dj;
-    const R *aa = static_cast<const R *>(p_adamp) + dj;
-    const R *dva = static_cast<const R *>(p_dv) + dj;
-    const R *xwa = static_cast<const R *>(p_xwing) + dj;
+    const int L = a.L;
+    const int g0 = tile * kTw + 1;        // first 1-based point
+    const int g1 = min(g0 + kTw - 1, L);  // last
+    const size_t dj = (size_t)j * a.nlines;
+    const R *k0a = static_cast<const R *>(a.kap0) + dj;
+    const R *aa = static_cast<const R *>(a.adamp) + dj;
+    const R *dva = static_cast<const R *>(a.dv) + dj;
+    const R *xwa = static_cast<const R *>(a.xwing) + dj;
 
-    Neumaier32 acc32;
-    double acc64 = 0.0;
+    Neumaier32 acc32[PTS];
+    double acc64[PTS];
+    for (int p = 0; p < PTS; ++p) acc64[p] = 0.0;
 
-    for (int c = 0; c < A12_NCLASS - 1; ++c) {
-        const int lo = rng[2 * (tile * (A12_NCLASS - 1) + c)];
-        const int hi = rng[2 * (tile * (A12_NCLASS - 1) + c) + 1];
-        for (int base = lo; base < hi; base += kChunk) {
-            const int q = base + tid;
-            bool act = false;
-            int l = 0, nb = 0, ns = -1;
-            if (q < hi) {
-                l = perm[q];
-                nb = nbuffP[q];
-                ns = p_nstep[dj + l];
-                act = ns >= 0 && nb - ns <= g1 && nb + ns >= g0;
+    const int ncls = a.nlines > 0 ? A12_NCLASS - 1 : 0;  // no lines: zeros
+    for (int c = 0; c < ncls; ++c) {
+        const int lo = a.rng[2 * (tile * (A12_NCLASS - 1) + c)];
+        const int hi = a.rng[2 * (tile * (A12_NCLASS - 1) + c) + 1];
+        for (int base = lo; base < hi; base += CHUNK) {
+            int nact = 0;
+            for (int sub = 0; sub < CHUNK; sub += kTile) {  // stage CHUNK candidates
+                const int q = base + sub + tid;
+                bool act = false;
+                int l = 0, nb = 0, ns = -1;
+                if (sub + tid < CHUNK && q < hi) {
+                    l = a.perm[q];
+                    nb = a.nbuffP[q];
+                    ns = a.nstep[dj + l];
+                    act = ns >= 0 && nb - ns <= g1 && nb + ns >= g0;
+                }
+                const unsigned long long m = __ballot(act);
+                if (lane == 0) s_wcnt[wv] = __popcll(m);
+                __syncthreads();
+                if (act) {
+                    int pos = nact + __popcll(m & ((1ull << lane) - 1ull));
+                    for (int w = 0; w < wv; ++w) pos += s_wcnt[w];
+                    s_nb[pos] = nb;
+                    s_ns[pos] = ns;
+                    s_n10[pos] = a.n10[dj + l];
+                    s_k0[pos] = k0a[l];
+                    s_a[pos] = aa[l];
+                    s_dv[pos] = dva[l];
+                    s_xw[pos] = xwa[l];
+                }
+                if (tid == 0) {
+                    int sm = nact;
+                    for (int w = 0; w < kTile / A12_WAVE; ++w) sm += s_wcnt[w];
+                    s_nact = sm;
+                }
+                __syncthreads();
+                nact = s_nact;
+                __syncthreads();  // s_wcnt / s_nact are rewritten by the next sub-chunk
             }
-            const unsigned long long m = __ballot(act);
-            if (lane == 0) s_wcnt[wv] = __popcll(m);
-            __syncthreads();
-            if (act) {
-                int pos = __popcll(m & ((1ull << lane) - 1ull));
-                for (int w = 0; w < wv; ++w) pos += s_wcnt[w];
-                s_nb[pos] = nb;
-                s_ns[pos] = ns;
-                s_n10[pos] = p_n10[dj + l];
-                s_k0[pos] = k0a[l];
-                s_a[pos] = aa[l];
-                s_dv[pos] = dva[l];
-                s_xw[pos] = xwa[l];
-            }
-            if (tid == 0) {
-                int s = 0;
-                for (int w = 0; w < kChunk / A12_WAVE; ++w) s += s_wcnt[w];
-                s_nact = s;
-            }
-            __syncthreads();
-            const int nact = s_nact;
-            if (gi <= g1) {
-                for (int k = 0; k < nact; ++k) {
-                    const int n = abs(gi - s_nb[k]);
-                    if (n > s_ns[k]) continue;
+            for (int k = 0; k < nact; ++k) {
+                const int nb = s_nb[k], ns = s_ns[k], n10 = s_n10[k];
+#pragma unroll
+                for (int p = 0; p < PTS; ++p) {
+                    const int gi = g0 + tid + p * kTile;
+                    const int n = abs(gi - nb);
+                    if (gi > g1 || n > ns) continue;
                     if (!F64) {
                         float v;
-                        if (n <= s_n10[k]) v = (float)s_k0[k] * voigt_mixed((float)n * (float)s_dv[k], (float)s_a[k]);
+                        if (n <= n10) v = (float)s_k0[k] * voigt_mixed((float)n * (float)s_dv[k], (float)s_a[k]);
                         else v = (float)s_xw[k] / ((float)n * (float)n);
-                        acc32.add(v);
+                        acc32[p].add(v);
                     } else {
                         double v;
-                        if (n <= s_n10[k]) v = (double)s_k0[k] * voigt_f64((double)n * (double)s_dv[k], (double)s_a[k]);
+                        if (n <= n10) v = (double)s_k0[k] * voigt_f64((double)n * (double)s_dv[k], (double)s_a[k]);
                         else v = (double)s_xw[k] / ((double)n * (double)n);
-                        acc64 += v;
+                        acc64[p] += v;
                     }
                 }
             }
             __syncthreads();
         }
     }
-    if (gi <= g1) {
-        if (!F64) static_cast<float *>(out)[(size_t)j * L + gi - 1] = acc32.value();
-        else static_cast<double *>(out)[(size_t)j * L + gi - 1] = acc64;
+#pragma unroll
+    for (int p = 0; p < PTS; ++p) {
+        const int gi = g0 + tid + p * kTile;
+        if (gi > g1) continue;
+        if (!F64) static_cast<float *>(a.out)[(size_t)j * L + gi - 1] = acc32[p].value();
+        else static_cast<double *>(a.out)[(size_t)j * L + gi - 1] = acc64[p];
+    }
+}
+
+template <bool F64, int PTS, int CHUNK>
+__global__ __launch_bounds__(256) void k_tile(TileArgs a)
+{
+    tile_body<F64, PTS, CHUNK>(a, blockIdx.x, blockIdx.y);
+}
+
+// The whole batch in one persistent launch: work item w = (model, tile,
+// depth), models in the order given (largest first), every workgroup
+// striding through the items.  item_off (nmodel + 1): first item of each
+// model; a model's items run tile-fastest.
+struct BatchItemModel {
+    TileArgs a;
+    int ntiles;
+};
+
+template <bool F64, int PTS, int CHUNK>
+__global__ __launch_bounds__(256) void k_tile_batch(const BatchItemModel *__restrict__ models,
+                                                    const long long *__restrict__ item_off,
+                                                    int nmodel)
+{
+    const long long nitems = item_off[nmodel];
+    int m = 0;
+    for (long long w = blockIdx.x; w < nitems; w += gridDim.x) {
+        while (w >= item_off[m + 1]) ++m;  // items ascend, so the model only moves forward
+        const long long r = w - item_off[m];
+        const int nt = models[m].ntiles;
+        tile_body<F64, PTS, CHUNK>(models[m].a, (int)(r % nt), (int)(r / nt));
+    }
+}
+
+// Gather shapes: points per thread and lines per LDS stage.
+struct TileShape {
+    int pts, chunk;
+};
+constexpr TileShape kTileShapes[] = {{1, 256}, {1, 128}, {1, 512}, {2, 256}, {2, 512}, {4, 256}};
+constexpr int kNumTileShapes = sizeof(kTileShapes) / sizeof(kTileShapes[0]);
+
+template <bool F64>
+using TileKernel = void (*)(TileArgs);
+template <bool F64>
+using TileBatchKernel = void (*)(const BatchItemModel *, const long long *, int);
+
+template <bool F64>
+TileKernel<F64> tile_kernel(int shape)
+{
+    switch (shape) {
+    case 1: return k_tile<F64, 1, 128>;
+    case 2: return k_tile<F64, 1, 512>;
+    case 3: return k_tile<F64, 2, 256>;
+    case 4: return k_tile<F64, 2, 512>;
+    case 5: return k_tile<F64, 4, 256>;
+    default: return k_tile<F64, 1, 256>;
+    }
+}
+
+template <bool F64>
+TileBatchKernel<F64> tile_batch_kernel(int shape)
+{
+    switch (shape) {
+    case 1: return k_tile_batch<F64, 1, 128>;
+    case 2: return k_tile_batch<F64, 1, 512>;
+    case 3: return k_tile_batch<F64, 2, 256>;
+    case 4: return k_tile_batch<F64, 2, 512>;
+    case 5: return k_tile_batch<F64, 4, 256>;
+    default: return k_tile_batch<F64, 1, 256>;
     }
 }
 
@@ -537,6 +625,7 @@ hipError_t upload(DevBuf &b, const T *src, size_t n, hipStream_t s)
     if (e != hipSuccess) return e;
     return hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, s);
 }
+
 
 }  // namespace
 
@@ -622,6 +711,171 @@ int a12i_synthe_load(a12_ctx *ctx, const a12_synthe_atm *atm, const a12_lte_line
     return A12_OK;
 }
 
+namespace {
+
+// One model's device arrays for the opacity pipeline: inputs, work arrays
+// and the output.  nb_shift is subtracted from every nbuff of the shared line
+// arrays (0 for a single model; the model's window offset in a batch).
+struct SynModel {
+    int n = 0, L = 0, N = 0, mw6 = 0, nedge = 0, nf = 0, nb_shift = 0;
+    double wbegin = 0, ratio = 0, resolu = 0;
+    const double *wledge = nullptr, *continall = nullptr, *xnfpel = nullptr,
+                 *dopple = nullptr, *depth = nullptr;
+    const int *l_nbuff = nullptr, *l_nelion = nullptr;
+    const float *l_cgf = nullptr, *l_elo = nullptr, *l_gr = nullptr, *l_gs = nullptr,
+                *l_gw = nullptr;
+    double *xnfdop = nullptr, *dscal = nullptr, *cont64 = nullptr;
+    float *cont32 = nullptr;
+    int *p_nstep = nullptr, *p_n10 = nullptr;
+    void *p_kap0 = nullptr, *p_adamp = nullptr, *p_dv = nullptr, *p_xwing = nullptr;
+    int *emax = nullptr, *lclass = nullptr, *cls_cnt = nullptr, *cls_off = nullptr,
+        *perm = nullptr, *nbuffP = nullptr, *cls_range = nullptr;
+    long long *blk_evals = nullptr, *evals_out = nullptr;
+    void *opacity = nullptr;
+};
+
+int tile_width(int shape) { return kTile * kTileShapes[shape].pts; }
+int ntiles_of(int L, int shape) { return (L + tile_width(shape) - 1) / tile_width(shape); }
+int nblk_lines(int N) { return (std::max(N, 1) + 255) / 256; }
+int nblk_part(int N) { return (std::max(N, 1) + kPartBlock - 1) / kPartBlock; }
+
+TileArgs tile_args(const SynModel &M)
+{
+    TileArgs a;
+    a.L = M.L;
+    a.nlines = M.N;
+    a.rng = M.cls_range;
+    a.perm = M.perm;
+    a.nbuffP = M.nbuffP;
+    a.nstep = M.p_nstep;
+    a.n10 = M.p_n10;
+    a.kap0 = M.p_kap0;
+    a.adamp = M.p_adamp;
+    a.dv = M.p_dv;
+    a.xwing = M.p_xwing;
+    a.out = M.opacity;
+    return a;
+}
+
+int launch_ranges(a12_ctx *ctx, const SynModel &M, int shape, hipStream_t st)
+{
+    if (M.N == 0) return A12_OK;
+    const int ntiles = ntiles_of(M.L, shape);
+    const int nr = ntiles * (A12_NCLASS - 1);
+    hipLaunchKernelGGL(k_tile_ranges, dim3((nr + 255) / 256), dim3(256), 0, st, ntiles,
+                       tile_width(shape), M.L, M.cls_off, nblk_part(M.N), M.nbuffP, M.cls_range);
+    A12_CHECK_LAUNCH();
+    return A12_OK;
+}
+
+// Everything before the gather, on stream st: per-depth prep, continuum,
+// line set-up with the evaluation count, extent classes, the class
+// partition and the per-tile candidate ranges for tile shape `shape`.
+// ev (may be null): events recorded after the continuum and the set-up.
+int launch_prep(a12_ctx *ctx, const SynModel &M, bool f64, int shape, hipStream_t st,
+                hipEvent_t ev_cont, hipEvent_t ev_setup)
+{
+    const int n = M.n, L = M.L, N = M.N;
+    hipLaunchKernelGGL(k_depth_prep, dim3(n), dim3(256), 0, st, n, M.mw6, M.xnfpel, M.dopple,
+                       M.depth, M.xnfdop, M.dscal);
+    A12_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_continuum, dim3((L + 255) / 256, n), dim3(256), 0, st, L, n, M.wbegin,
+                       M.ratio, M.nedge, M.wledge, M.nf, M.continall, M.cont32,
+                       f64 ? M.cont64 : nullptr);
+    A12_CHECK_LAUNCH();
+    if (ev_cont) A12_HIP(hipEventRecord(ev_cont, st));
+    if (N == 0) {
+        hipLaunchKernelGGL(k_zero_ll, dim3(1), dim3(64), 0, st, M.evals_out, 2);
+        A12_CHECK_LAUNCH();
+        return A12_OK;
+    }
+    const int nblkl = nblk_lines(N), nblk = nblk_part(N);
+    auto setup = f64 ? k_line_setup<true> : k_line_setup<false>;
+    hipLaunchKernelGGL(setup, dim3(nblkl, n), dim3(256), 0, st, N, L, M.mw6, M.resolu, M.nb_shift,
+                       M.l_nbuff, M.l_nelion, M.l_cgf, M.l_elo, M.l_gr, M.l_gs, M.l_gw, M.xnfdop,
+                       M.dopple, M.dscal, M.depth + 6 * n, M.cont32, f64 ? M.cont64 : nullptr,
+                       M.p_nstep, M.p_n10, M.p_kap0, M.p_adamp, M.p_dv, M.p_xwing, M.blk_evals);
+    A12_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_sum_evals, dim3(1), dim3(256), 0, st, M.blk_evals, nblkl * n,
+                       M.evals_out);
+    A12_CHECK_LAUNCH();
+    if (ev_setup) A12_HIP(hipEventRecord(ev_setup, st));
+    hipLaunchKernelGGL(k_line_emax, dim3(nblkl), dim3(256), 0, st, N, n, M.p_nstep, M.emax,
+                       M.lclass);
+    A12_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_cls_count, dim3(nblk), dim3(kPartBlock), 0, st, N, M.lclass, M.cls_cnt);
+    A12_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_cls_scan, dim3(1), dim3(1024), 0, st, A12_NCLASS * nblk, M.cls_cnt,
+                       M.cls_off);
+    A12_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_cls_scatter, dim3(nblk), dim3(kPartBlock), 0, st, N, M.nb_shift,
+                       M.lclass, M.cls_off, M.l_nbuff, M.perm, M.nbuffP);
+    A12_CHECK_LAUNCH();
+    return launch_ranges(ctx, M, shape, st);
+}
+
+int launch_gather(a12_ctx *ctx, const SynModel &M, bool f64, int shape, hipStream_t st)
+{
+    auto tk = f64 ? tile_kernel<true>(shape) : tile_kernel<false>(shape);
+    hipLaunchKernelGGL(tk, dim3(ntiles_of(M.L, shape), M.n), dim3(256), 0, st, tile_args(M));
+    A12_CHECK_LAUNCH();
+    return A12_OK;
+}
+
+// Work-array bytes of one model (everything but inputs and the opacity).
+size_t work_bytes(int n, int L, int N, int mw6, bool f64, int max_tiles)
+{
+    const size_t rb = f64 ? sizeof(double) : sizeof(float);
+    const size_t nl = (size_t)std::max(N, 1);
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    size_t b = al(sizeof(double) * mw6 * n) + al(sizeof(double) * 4 * n) +
+               al(sizeof(float) * (size_t)L * n) + (f64 ? al(sizeof(double) * (size_t)L * n) : 0);
+    b += 2 * al(sizeof(int) * nl * n) + 4 * al(rb * nl * n);
+    b += 2 * al(sizeof(int) * nl);
+    b += al(2 * sizeof(long long) * (size_t)nblk_lines(N) * n) + al(2 * sizeof(long long));
+    b += al(sizeof(int) * (size_t)A12_NCLASS * nblk_part(N)) +
+         al(sizeof(int) * ((size_t)A12_NCLASS * nblk_part(N) + 1));
+    b += 2 * al(sizeof(int) * nl) + al(sizeof(int) * 2 * (size_t)max_tiles * (A12_NCLASS - 1));
+    return b;
+}
+
+// Carve a model's work arrays out of one allocation (same order as
+// work_bytes).
+void carve_work(SynModel &M, char *base, bool f64, int max_tiles)
+{
+    const size_t rb = f64 ? sizeof(double) : sizeof(float);
+    const int n = M.n, L = M.L, N = M.N;
+    const size_t nl = (size_t)std::max(N, 1);
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    char *p = base;
+    auto take = [&](size_t bytes) {
+        char *q = p;
+        p += al(bytes);
+        return q;
+    };
+    M.xnfdop = reinterpret_cast<double *>(take(sizeof(double) * M.mw6 * n));
+    M.dscal = reinterpret_cast<double *>(take(sizeof(double) * 4 * n));
+    M.cont32 = reinterpret_cast<float *>(take(sizeof(float) * (size_t)L * n));
+    M.cont64 = f64 ? reinterpret_cast<double *>(take(sizeof(double) * (size_t)L * n)) : nullptr;
+    M.p_nstep = reinterpret_cast<int *>(take(sizeof(int) * nl * n));
+    M.p_n10 = reinterpret_cast<int *>(take(sizeof(int) * nl * n));
+    M.p_kap0 = take(rb * nl * n);
+    M.p_adamp = take(rb * nl * n);
+    M.p_dv = take(rb * nl * n);
+    M.p_xwing = take(rb * nl * n);
+    M.emax = reinterpret_cast<int *>(take(sizeof(int) * nl));
+    M.lclass = reinterpret_cast<int *>(take(sizeof(int) * nl));
+    M.blk_evals = reinterpret_cast<long long *>(take(2 * sizeof(long long) * (size_t)nblk_lines(N) * n));
+    M.evals_out = reinterpret_cast<long long *>(take(2 * sizeof(long long)));
+    M.cls_cnt = reinterpret_cast<int *>(take(sizeof(int) * (size_t)A12_NCLASS * nblk_part(N)));
+    M.cls_off = reinterpret_cast<int *>(take(sizeof(int) * ((size_t)A12_NCLASS * nblk_part(N) + 1)));
+    M.perm = reinterpret_cast<int *>(take(sizeof(int) * nl));
+    M.nbuffP = reinterpret_cast<int *>(take(sizeof(int) * nl));
+    M.cls_range = reinterpret_cast<int *>(take(sizeof(int) * 2 * (size_t)max_tiles * (A12_NCLASS - 1)));
+}
+
+}  // namespace
+
 int a12i_synthe_run(a12_ctx *ctx, int prec)
 {
     SyntheDev &S = ctx->syn;
@@ -637,89 +891,66 @@ int a12i_synthe_run(a12_ctx *ctx, int prec)
     hipStream_t st = ctx->stream;
     const int n = S.nrhox, L = S.L, N = S.nlines;
     const size_t rb = f64 ? sizeof(double) : sizeof(float);
-    const size_t nl = (size_t)std::max(N, 1);
-    A12_HIP(S.xnfdop.reserve(sizeof(double) * S.mw6 * n));
-    A12_HIP(S.dscal.reserve(sizeof(double) * 4 * n));
-    A12_HIP(S.cont32.reserve(sizeof(float) * (size_t)L * n));
-    if (f64) A12_HIP(S.cont64.reserve(sizeof(double) * (size_t)L * n));
+    const int max_tiles = (L + kTile - 1) / kTile;  // the narrowest shape has the most tiles
+    A12_HIP(S.work.reserve(work_bytes(n, L, N, S.mw6, f64, max_tiles)));
     A12_HIP(S.opacity.reserve(rb * (size_t)L * n));
-    A12_HIP(S.p_nstep.reserve(sizeof(int) * nl * n));
-    A12_HIP(S.p_n10.reserve(sizeof(int) * nl * n));
-    A12_HIP(S.p_kap0.reserve(rb * nl * n));
-    A12_HIP(S.p_adamp.reserve(rb * nl * n));
-    A12_HIP(S.p_dv.reserve(rb * nl * n));
-    A12_HIP(S.p_xwing.reserve(rb * nl * n));
-    A12_HIP(S.emax.reserve(sizeof(int) * nl));
-    A12_HIP(S.lclass.reserve(sizeof(int) * nl));
-    const int nblkl = (int)((nl + 255) / 256);
-    A12_HIP(S.blk_evals.reserve(2 * sizeof(long long) * (size_t)nblkl * n));
-    A12_HIP(S.evals_out.reserve(2 * sizeof(long long)));
-    const int nblk = (int)((nl + kPartBlock - 1) / kPartBlock);
-    A12_HIP(S.cls_cnt.reserve(sizeof(int) * (size_t)A12_NCLASS * nblk));
-    A12_HIP(S.cls_off.reserve(sizeof(int) * ((size_t)A12_NCLASS * nblk + 1)));
-    A12_HIP(S.perm.reserve(sizeof(int) * nl));
-    A12_HIP(S.nbuffP.reserve(sizeof(int) * nl));
-    const int ntiles = (L + kTile - 1) / kTile;
-    A12_HIP(S.cls_range.reserve(sizeof(int) * 2 * (size_t)ntiles * (A12_NCLASS - 1)));
-
-    const double *dep = S.depth.as<double>();
+    SynModel M;
+    M.n = n;
+    M.L = L;
+    M.N = N;
+    M.mw6 = S.mw6;
+    M.nedge = S.nedge;
+    M.nf = S.nf;
+    M.wbegin = S.wbegin;
+    M.ratio = S.ratio;
+    M.resolu = S.resolu;
+    M.wledge = S.wledge.as<double>();
+    M.continall = S.continall.as<double>();
+    M.xnfpel = S.xnfpel.as<double>();
+    M.dopple = S.dopple.as<double>();
+    M.depth = S.depth.as<double>();
+    M.l_nbuff = S.l_nbuff.as<int>();
+    M.l_nelion = S.l_nelion.as<int>();
+    M.l_cgf = S.l_cgf.as<float>();
+    M.l_elo = S.l_elo.as<float>();
+    M.l_gr = S.l_gr.as<float>();
+    M.l_gs = S.l_gs.as<float>();
+    M.l_gw = S.l_gw.as<float>();
+    M.opacity = S.opacity.p;
+    carve_work(M, S.work.as<char>(), f64, max_tiles);
+    const int shape = S.shape;
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[0], st));
-    hipLaunchKernelGGL(k_depth_prep, dim3(n), dim3(256), 0, st, n, S.mw6, S.xnfpel.as<double>(),
-                       S.dopple.as<double>(), dep, S.xnfdop.as<double>(), S.dscal.as<double>());
-    A12_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_continuum, dim3((L + 255) / 256, n), dim3(256), 0, st, L, n, S.wbegin,
-                       S.ratio, S.nedge, S.wledge.as<double>(), S.nf, S.continall.as<double>(),
-                       S.cont32.as<float>(), f64 ? S.cont64.as<double>() : nullptr);
-    A12_CHECK_LAUNCH();
-    if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[1], st));
-    if (N > 0) {
-        auto setup = f64 ? k_line_setup<true> : k_line_setup<false>;
-        hipLaunchKernelGGL(setup, dim3(nblkl, n), dim3(256), 0, st, N, L, S.mw6, S.resolu,
-                           S.l_nbuff.as<int>(), S.l_nelion.as<int>(), S.l_cgf.as<float>(),
-                           S.l_elo.as<float>(), S.l_gr.as<float>(), S.l_gs.as<float>(),
-                           S.l_gw.as<float>(), S.xnfdop.as<double>(), S.dopple.as<double>(),
-                           S.dscal.as<double>(), dep + 6 * n, S.cont32.as<float>(),
-                           f64 ? S.cont64.as<double>() : nullptr, S.p_nstep.as<int>(),
-                           S.p_n10.as<int>(), S.p_kap0.p, S.p_adamp.p, S.p_dv.p, S.p_xwing.p,
-                           S.blk_evals.as<long long>());
-        A12_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_sum_evals, dim3(1), dim3(256), 0, st, S.blk_evals.as<long long>(),
-                           nblkl * n, S.evals_out.as<long long>());
-        A12_CHECK_LAUNCH();
-        if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[2], st));
-        hipLaunchKernelGGL(k_line_emax, dim3(nblkl), dim3(256), 0, st, N, n, S.p_nstep.as<int>(),
-                           S.emax.as<int>(), S.lclass.as<int>());
-        A12_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_cls_count, dim3(nblk), dim3(kPartBlock), 0, st, N,
-                           S.lclass.as<int>(), S.cls_cnt.as<int>());
-        A12_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_cls_scan, dim3(1), dim3(1024), 0, st, A12_NCLASS * nblk,
-                           S.cls_cnt.as<int>(), S.cls_off.as<int>());
-        A12_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_cls_scatter, dim3(nblk), dim3(kPartBlock), 0, st, N,
-                           S.lclass.as<int>(), S.cls_off.as<int>(), S.l_nbuff.as<int>(),
-                           S.perm.as<int>(), S.nbuffP.as<int>());
-        A12_CHECK_LAUNCH();
-        const int nr = ntiles * (A12_NCLASS - 1);
-        hipLaunchKernelGGL(k_tile_ranges, dim3((nr + 255) / 256), dim3(256), 0, st, ntiles, L,
-                           S.cls_off.as<int>(), nblk, S.nbuffP.as<int>(), S.cls_range.as<int>());
-        A12_CHECK_LAUNCH();
-        if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[3], st));
-        auto tk = f64 ? k_tile<true> : k_tile<false>;
-        hipLaunchKernelGGL(tk, dim3(ntiles, n), dim3(256), 0, st, L, N, S.cls_range.as<int>(),
-                           S.perm.as<int>(), S.nbuffP.as<int>(), S.p_nstep.as<int>(),
-                           S.p_n10.as<int>(), S.p_kap0.p, S.p_adamp.p, S.p_dv.p, S.p_xwing.p,
-                           S.opacity.p);
-        A12_CHECK_LAUNCH();
-        if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[4], st));
-        S.prof_valid = ctx->prof;
-    } else {
-        S.prof_valid = false;
-        A12_HIP(hipMemsetAsync(S.opacity.p, 0, rb * (size_t)L * n, st));
-        A12_HIP(hipMemsetAsync(S.evals_out.p, 0, 2 * sizeof(long long), st));
-    }
+    int rc = launch_prep(ctx, M, f64, shape, st, ctx->prof ? ctx->ev[1] : nullptr,
+                         ctx->prof ? ctx->ev[2] : nullptr);
+    if (rc != A12_OK) return rc;
+    if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[3], st));
+    rc = launch_gather(ctx, M, f64, shape, st);
+    if (rc != A12_OK) return rc;
+    if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[4], st));
+    S.prof_valid = ctx->prof && N > 0;
+    S.evals_dev = M.evals_out;
     S.opac_prec = prec;
     S.transferred = false;
+    return A12_OK;
+}
+
+int a12i_synthe_shapes(int idx, int32_t *pts_chunk)
+{
+    if (idx < 0 || idx >= kNumTileShapes) return -1;
+    if (pts_chunk) {
+        pts_chunk[0] = kTileShapes[idx].pts;
+        pts_chunk[1] = kTileShapes[idx].chunk;
+    }
+    return kNumTileShapes;
+}
+
+int a12i_synthe_set_shape(a12_ctx *ctx, int shape)
+{
+    if (shape < 0 || shape >= kNumTileShapes) {
+        ctx->err = "a12_synthe_set_shape: no such shape";
+        return A12_ERR_ARG;
+    }
+    ctx->syn.shape = shape;
     return A12_OK;
 }
 
@@ -742,5 +973,468 @@ int a12i_voigt(a12_ctx *ctx, int prec, int n, const float *v, const float *a, do
     A12_CHECK_LAUNCH();
     A12_HIP(hipMemcpyAsync(h, dh, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
     A12_HIP(hipStreamSynchronize(ctx->stream));
+    return A12_OK;
+}
+
+// ==========================================================================
+// Batched SYNTHE: many models, each with its own atmosphere, depth count and
+// wavelength window, over one shared line list sorted by centre.  A model's
+// lines are a contiguous range of that list and its nbuff is the list's
+// nbuff minus the model's window offset (all windows on one log-lambda
+// grid).  Per group of models whose work arrays fit the work budget: the
+// set-up kernels per model, then ONE persistent gather launch over every
+// (model, tile, depth) item of the group; then, optionally, the FP64
+// transfer per model.  The whole sequence can be captured once into a
+// hipGraph and replayed.  Results per model are those of the single-model
+// pipeline on the same inputs, bit for bit.
+// ==========================================================================
+struct SynBatch {
+    int nmodel = 0, mw6 = 0;
+    int64_t nlines = 0;
+    std::vector<SynModel> m;           // inputs and output set; work carved per run
+    std::vector<int> order;            // models, largest first (gather item order)
+    std::vector<size_t> in_off, out_off, h_off;
+    std::vector<int> has_tables;       // contabs/contscat given
+    std::vector<const double *> cab, csc;  // device contabs / contscat per model
+    int grid = 0;                      // persistent gather workgroups
+    DevBuf lines, inputs, opacity, hout, work, evals, desc, item_off;
+    std::vector<std::vector<int>> groups;
+    std::vector<size_t> wk_off;        // work offset of each model within its group
+    size_t budget = 0, max_model_work = 0;
+    int shape = 0, prec = -1;
+    bool transfer = false, ran = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_prec = -1, graph_shape = -1;
+    bool graph_transfer = false;
+    void drop_graph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        graph_prec = graph_shape = -1;
+    }
+};
+
+void a12i_synthe_batch_free(a12_ctx *ctx)
+{
+    SynBatch *B = ctx->sbatch;
+    if (!B) return;
+    B->drop_graph();
+    for (DevBuf *b : {&B->lines, &B->inputs, &B->opacity, &B->hout, &B->work, &B->evals, &B->desc,
+                      &B->item_off})
+        b->release();
+    delete B;
+    ctx->sbatch = nullptr;
+}
+
+int a12i_synthe_batch_load(a12_ctx *ctx, int32_t nmodel, const a12_synthe_atm *atms,
+                           const int32_t *nb_shift, const int64_t *line_lo,
+                           const int64_t *line_hi, const a12_lte_line *lines, int64_t nlines)
+{
+    a12i_synthe_batch_free(ctx);
+    if (nmodel < 1 || !atms || !nb_shift || !line_lo || !line_hi || nlines < 0 ||
+        (nlines > 0 && !lines) || nlines > INT32_MAX) {
+        ctx->err = "a12_synthe_batch_load: bad arguments";
+        return A12_ERR_ARG;
+    }
+    for (int64_t i = 1; i < nlines; ++i)
+        if (lines[i].nbuff < lines[i - 1].nbuff) {
+            ctx->err = "a12_synthe_batch_load: the shared line list must be sorted by nbuff";
+            return A12_ERR_ARG;
+        }
+    const int mw6 = atms[0].mw6;
+    for (int k = 0; k < nmodel; ++k) {
+        const a12_synthe_atm *a = &atms[k];
+        const std::string id = "a12_synthe_batch_load: model " + std::to_string(k) + ": ";
+        if (a->nrhox < 2 || a->nrhox > A12_NRMAX_LAYERS || a->length < 1 || a->nedge < 2 ||
+            a->nf < 3 * (a->nedge - 1) || a->mw6 != mw6 || mw6 < 6 || !a->wledge ||
+            !a->continall || !a->xnfpel || !a->dopple || !a->t || !a->xne || !a->rho ||
+            !a->xnfh || !a->xnfhe1 || !a->xnfh2 || !a->hckt || !a->hkt || !a->hkt_atlas ||
+            !a->rhox || !(a->ratio > 1.0) || !(a->resolu > 0.0) || !(a->wbegin > 0.0) ||
+            (!a->contabs) != (!a->contscat)) {
+            ctx->err = id + "bad dimensions or null array (mw6 must be the same for all)";
+            return A12_ERR_ARG;
+        }
+        for (int e = 1; e < a->nedge; ++e)
+            if (!(a->wledge[e] > a->wledge[e - 1])) {
+                ctx->err = id + "continuum edges must be strictly ascending";
+                return A12_ERR_ARG;
+            }
+        if (line_lo[k] < 0 || line_hi[k] < line_lo[k] || line_hi[k] > nlines ||
+            line_hi[k] - line_lo[k] > INT32_MAX) {
+            ctx->err = id + "line range outside the list";
+            return A12_ERR_ARG;
+        }
+    }
+    for (int64_t i = 0; i < nlines; ++i)
+        if (lines[i].nelion < 1 || lines[i].nelion > mw6) {
+            ctx->err = "a12_synthe_batch_load: line " + std::to_string(i) + " has nelion out of range";
+            return A12_ERR_ARG;
+        }
+    SynBatch *B = new SynBatch;
+    ctx->sbatch = B;
+    B->nmodel = nmodel;
+    B->mw6 = mw6;
+    B->nlines = nlines;
+    hipStream_t st = ctx->stream;
+    // shared lines (SoA): nbuff, nelion, cgf, elo, gr, gs, gw
+    const size_t nl = (size_t)std::max<int64_t>(nlines, 1);
+    A12_HIP(B->lines.reserve(7 * 4 * nl));
+    {
+        std::vector<int32_t> buf(7 * nl, 0);
+        int32_t *nb = buf.data(), *ne = nb + nl;
+        float *f = reinterpret_cast<float *>(buf.data() + 2 * nl);
+        for (int64_t i = 0; i < nlines; ++i) {
+            nb[i] = lines[i].nbuff;
+            ne[i] = lines[i].nelion;
+            f[i] = lines[i].cgf;
+            f[nl + i] = lines[i].elo;
+            f[2 * nl + i] = lines[i].gamrf;
+            f[3 * nl + i] = lines[i].gamsf;
+            f[4 * nl + i] = lines[i].gamwf;
+        }
+        A12_HIP(hipMemcpyAsync(B->lines.p, buf.data(), 7 * 4 * nl, hipMemcpyHostToDevice, st));
+        A12_HIP(hipStreamSynchronize(st));
+    }
+    const int32_t *d_nb = B->lines.as<int32_t>(), *d_ne = d_nb + nl;
+    const float *d_f = reinterpret_cast<const float *>(d_nb + 2 * nl);
+    // per-model inputs in one allocation
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    B->in_off.resize(nmodel);
+    B->out_off.resize(nmodel);
+    B->h_off.resize(nmodel);
+    B->has_tables.resize(nmodel);
+    B->cab.resize(nmodel);
+    B->csc.resize(nmodel);
+    size_t in_bytes = 0, out_elems = 0, h_elems = 0;
+    for (int k = 0; k < nmodel; ++k) {
+        const a12_synthe_atm *a = &atms[k];
+        const size_t n = a->nrhox;
+        B->in_off[k] = in_bytes;
+        in_bytes += al(8 * a->nedge) + 3 * al(8 * (size_t)a->nf * n) + 2 * al(8 * (size_t)mw6 * n) +
+                    al(8 * 10 * n);
+        B->out_off[k] = out_elems;
+        out_elems += (size_t)a->length * n;
+        B->h_off[k] = h_elems;
+        h_elems += 2 * (size_t)a->length;
+    }
+    A12_HIP(B->inputs.reserve(in_bytes));
+    A12_HIP(B->hout.reserve(sizeof(double) * h_elems));
+    A12_HIP(B->evals.reserve(2 * sizeof(long long) * nmodel));
+    B->m.resize(nmodel);
+    std::vector<double> depth;
+    for (int k = 0; k < nmodel; ++k) {
+        const a12_synthe_atm *a = &atms[k];
+        const int n = a->nrhox;
+        SynModel &M = B->m[k];
+        M.n = n;
+        M.L = a->length;
+        M.N = (int)(line_hi[k] - line_lo[k]);
+        M.mw6 = mw6;
+        M.nedge = a->nedge;
+        M.nf = a->nf;
+        M.nb_shift = nb_shift[k];
+        M.wbegin = a->wbegin;
+        M.ratio = a->ratio;
+        M.resolu = a->resolu;
+        char *p = B->inputs.as<char>() + B->in_off[k];
+        auto put = [&](const double *src, size_t cnt) -> const double * {
+            char *q = p;
+            p += al(8 * cnt);
+            if (src && hipMemcpyAsync(q, src, 8 * cnt, hipMemcpyHostToDevice, st) != hipSuccess)
+                return nullptr;
+            return reinterpret_cast<const double *>(q);
+        };
+        M.wledge = put(a->wledge, a->nedge);
+        M.continall = put(a->continall, (size_t)a->nf * n);
+        B->cab[k] = put(a->contabs, (size_t)a->nf * n);
+        B->csc[k] = put(a->contscat, (size_t)a->nf * n);
+        B->has_tables[k] = a->contabs != nullptr;
+        M.xnfpel = put(a->xnfpel, (size_t)mw6 * n);
+        M.dopple = put(a->dopple, (size_t)mw6 * n);
+        depth.assign((size_t)10 * n, 0.0);
+        const double *src[10] = {a->t, a->xne, a->rho, a->xnfh, a->xnfhe1,
+                                 a->xnfh2, a->hckt, a->hkt, a->hkt_atlas, a->rhox};
+        for (int q = 0; q < 10; ++q) std::memcpy(&depth[(size_t)q * n], src[q], sizeof(double) * n);
+        M.depth = put(depth.data(), depth.size());
+        if (!M.wledge || !M.continall || !M.xnfpel || !M.dopple || !M.depth) {
+            ctx->err = "a12_synthe_batch_load: upload failed";
+            return A12_ERR_HIP;
+        }
+        A12_HIP(hipStreamSynchronize(st));  // depth is reused by the next model
+        const int64_t lo = line_lo[k];
+        M.l_nbuff = d_nb + lo;
+        M.l_nelion = d_ne + lo;
+        M.l_cgf = d_f + lo;
+        M.l_elo = d_f + nl + lo;
+        M.l_gr = d_f + 2 * nl + lo;
+        M.l_gs = d_f + 3 * nl + lo;
+        M.l_gw = d_f + 4 * nl + lo;
+    }
+    B->order.resize(nmodel);
+    std::iota(B->order.begin(), B->order.end(), 0);
+    std::stable_sort(B->order.begin(), B->order.end(), [&](int x, int y) {
+        return (size_t)B->m[x].L * B->m[x].n > (size_t)B->m[y].L * B->m[y].n;
+    });
+    size_t fr = 0, tot = 0;
+    (void)hipMemGetInfo(&fr, &tot);
+    B->budget = std::min<size_t>(fr / 2, (size_t)64 << 30);
+    (void)out_elems;
+    return A12_OK;
+}
+
+int a12i_synthe_batch_set_budget(a12_ctx *ctx, int64_t bytes)
+{
+    if (!ctx->sbatch) {
+        ctx->err = "a12_synthe_batch_set_budget: no batch loaded";
+        return A12_ERR_STATE;
+    }
+    if (bytes < (int64_t)1 << 20) {
+        ctx->err = "a12_synthe_batch_set_budget: budget below 1 MiB";
+        return A12_ERR_ARG;
+    }
+    ctx->sbatch->budget = (size_t)bytes;
+    ctx->sbatch->drop_graph();
+    return A12_OK;
+}
+
+namespace {
+
+// Work layout for precision f64: groups in model order, each model's
+// arrays carved within its group's allocation; per-group gather
+// descriptors.  Returns the allocation it needs.
+int batch_plan(a12_ctx *ctx, SynBatch *B, bool f64, size_t *work_need)
+{
+    B->groups.clear();
+    B->wk_off.assign(B->nmodel, 0);
+    size_t cur = 0, need = 0;
+    std::vector<int> g;
+    B->max_model_work = 0;
+    for (int k : B->order) {
+        const SynModel &M = B->m[k];
+        const size_t w = (work_bytes(M.n, M.L, M.N, M.mw6, f64, (M.L + kTile - 1) / kTile) + 255) &
+                         ~(size_t)255;
+        B->max_model_work = std::max(B->max_model_work, w);
+        if (w > B->budget) {
+            ctx->err = "a12_synthe_batch: model " + std::to_string(k) +
+                       " alone needs more work memory than the budget";
+            return A12_ERR_OOM;
+        }
+        if (!g.empty() && cur + w > B->budget) {
+            B->groups.push_back(g);
+            need = std::max(need, cur);
+            g.clear();
+            cur = 0;
+        }
+        B->wk_off[k] = cur;
+        g.push_back(k);
+        cur += w;
+    }
+    if (!g.empty()) {
+        B->groups.push_back(g);
+        need = std::max(need, cur);
+    }
+    *work_need = need;
+    return A12_OK;
+}
+
+}  // namespace
+
+// Launch (or capture) the prepared batch on stream st: no allocation, no
+// host copies, no memset nodes.
+static int batch_launch(a12_ctx *ctx, SynBatch *B, bool f64, int shape, bool transfer,
+                        hipStream_t st)
+{
+    const size_t dstride = (sizeof(BatchItemModel) * B->nmodel + 255) & ~(size_t)255;
+    int rc;
+    for (size_t gi = 0; gi < B->groups.size(); ++gi) {
+        const std::vector<int> &g = B->groups[gi];
+        for (int k : g) {
+            rc = launch_prep(ctx, B->m[k], f64, shape, st, nullptr, nullptr);
+            if (rc != A12_OK) return rc;
+            A12_HIP(hipMemcpyAsync(B->evals.as<long long>() + 2 * k, B->m[k].evals_out,
+                                   2 * sizeof(long long), hipMemcpyDeviceToDevice, st));
+        }
+        auto kb = f64 ? tile_batch_kernel<true>(shape) : tile_batch_kernel<false>(shape);
+        const BatchItemModel *dm =
+            reinterpret_cast<const BatchItemModel *>(B->desc.as<char>() + gi * dstride);
+        const long long *dio = B->item_off.as<long long>() + gi * (size_t)(B->nmodel + 1);
+        hipLaunchKernelGGL(kb, dim3(B->grid), dim3(256), 0, st, dm, dio, (int)g.size());
+        A12_CHECK_LAUNCH();
+    }
+    if (transfer)
+        for (int k = 0; k < B->nmodel; ++k) {
+            const SynModel &M = B->m[k];
+            double *h = B->hout.as<double>() + B->h_off[k];
+            rc = a12i_synthe_transfer_launch(ctx, st, f64, M.L, M.n, M.wbegin, M.ratio, M.nedge,
+                                             M.wledge, M.nf, B->cab[k], B->csc[k], M.depth,
+                                             M.opacity, h, h + M.L);
+            if (rc != A12_OK) return rc;
+        }
+    return A12_OK;
+}
+
+// Prepare (carve, descriptors) for precision/shape; allocations only here.
+static int batch_prepare(a12_ctx *ctx, SynBatch *B, int prec, int shape, bool transfer)
+{
+    const bool f64 = prec == A12_PREC_FP64;
+    if (transfer) {
+        if (!ctx->have_ops) {
+            ctx->err = "a12_synthe_batch: the transfer needs a12_set_operators first";
+            return A12_ERR_STATE;
+        }
+        for (int k = 0; k < B->nmodel; ++k)
+            if (!B->has_tables[k]) {
+                ctx->err = "a12_synthe_batch: model " + std::to_string(k) +
+                           " was loaded without contabs/contscat";
+                return A12_ERR_ARG;
+            }
+    }
+    size_t need = 0;
+    int rc = batch_plan(ctx, B, f64, &need);
+    if (rc != A12_OK) return rc;
+    const size_t rb = f64 ? sizeof(double) : sizeof(float);
+    size_t out = 0;
+    for (int k = 0; k < B->nmodel; ++k) out += (size_t)B->m[k].L * B->m[k].n;
+    A12_HIP(B->opacity.reserve(rb * out));
+    A12_HIP(B->work.reserve(need));
+    const size_t dstride = (sizeof(BatchItemModel) * B->nmodel + 255) & ~(size_t)255;
+    A12_HIP(B->desc.reserve(dstride * B->groups.size()));
+    A12_HIP(B->item_off.reserve(sizeof(long long) * (size_t)(B->nmodel + 1) * B->groups.size()));
+    for (int k = 0; k < B->nmodel; ++k) {
+        SynModel &M = B->m[k];
+        M.opacity = B->opacity.as<char>() + rb * B->out_off[k];
+        carve_work(M, B->work.as<char>() + B->wk_off[k], f64, (M.L + kTile - 1) / kTile);
+    }
+    std::vector<char> dbuf(dstride * B->groups.size(), 0);
+    std::vector<long long> ibuf((size_t)(B->nmodel + 1) * B->groups.size(), 0);
+    for (size_t gi = 0; gi < B->groups.size(); ++gi) {
+        BatchItemModel *d = reinterpret_cast<BatchItemModel *>(dbuf.data() + gi * dstride);
+        long long *io = ibuf.data() + gi * (size_t)(B->nmodel + 1);
+        long long run = 0;
+        for (size_t q = 0; q < B->groups[gi].size(); ++q) {
+            const SynModel &M = B->m[B->groups[gi][q]];
+            d[q].a = tile_args(M);
+            d[q].ntiles = ntiles_of(M.L, shape);
+            io[q] = run;
+            run += (long long)d[q].ntiles * M.n;
+        }
+        io[B->groups[gi].size()] = run;
+    }
+    A12_HIP(hipMemcpyAsync(B->desc.p, dbuf.data(), dbuf.size(), hipMemcpyHostToDevice, ctx->stream));
+    A12_HIP(hipMemcpyAsync(B->item_off.p, ibuf.data(), sizeof(long long) * ibuf.size(),
+                           hipMemcpyHostToDevice, ctx->stream));
+    A12_HIP(hipStreamSynchronize(ctx->stream));
+    int ncu = 256, per = 1;
+    A12_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    A12_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per, reinterpret_cast<const void *>(f64 ? tile_batch_kernel<true>(shape)
+                                                 : tile_batch_kernel<false>(shape)),
+        256, 0));
+    B->grid = ncu * std::max(per, 1);
+    B->prec = prec;
+    B->shape = shape;
+    B->transfer = transfer;
+    return A12_OK;
+}
+
+int a12i_synthe_batch_run(a12_ctx *ctx, int prec, int transfer, int use_graph)
+{
+    SynBatch *B = ctx->sbatch;
+    if (!B) {
+        ctx->err = "a12_synthe_batch_run: call a12_synthe_batch_load first";
+        return A12_ERR_STATE;
+    }
+    if (prec != A12_PREC_MIXED && prec != A12_PREC_FP64) {
+        ctx->err = "a12_synthe_batch_run: prec must be A12_PREC_MIXED or A12_PREC_FP64";
+        return A12_ERR_ARG;
+    }
+    const int shape = ctx->syn.shape;
+    hipStream_t st = ctx->stream;
+    const bool same = B->exec && B->graph_prec == prec && B->graph_shape == shape &&
+                      B->graph_transfer == (transfer != 0);
+    if (!(use_graph && same)) {
+        B->drop_graph();
+        int rc = batch_prepare(ctx, B, prec, shape, transfer != 0);
+        if (rc != A12_OK) return rc;
+    }
+    if (!use_graph) {
+        int rc = batch_launch(ctx, B, prec == A12_PREC_FP64, shape, transfer != 0, st);
+        if (rc != A12_OK) return rc;
+    } else {
+        if (!same) {
+            A12_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeGlobal));
+            int rc = batch_launch(ctx, B, prec == A12_PREC_FP64, shape, transfer != 0, st);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(st, &g);
+            if (rc != A12_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            A12_HIP(e);
+            B->graph = g;
+            A12_HIP(hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0));
+            B->graph_prec = prec;
+            B->graph_shape = shape;
+            B->graph_transfer = transfer != 0;
+        }
+        A12_HIP(hipGraphLaunch(B->exec, st));
+    }
+    A12_HIP(hipStreamSynchronize(st));
+    B->ran = true;
+    return A12_OK;
+}
+
+int a12i_synthe_batch_fetch(a12_ctx *ctx, int32_t model, void *opacity_out, double *hline,
+                            double *hcont, int64_t *evals)
+{
+    SynBatch *B = ctx->sbatch;
+    if (!B || !B->ran) {
+        ctx->err = "a12_synthe_batch_fetch: nothing computed";
+        return A12_ERR_STATE;
+    }
+    if (model < 0 || model >= B->nmodel) {
+        ctx->err = "a12_synthe_batch_fetch: no such model";
+        return A12_ERR_ARG;
+    }
+    if ((hline || hcont) && !B->transfer) {
+        ctx->err = "a12_synthe_batch_fetch: the last run had no transfer";
+        return A12_ERR_STATE;
+    }
+    const SynModel &M = B->m[model];
+    hipStream_t st = ctx->stream;
+    const size_t rb = B->prec == A12_PREC_FP64 ? sizeof(double) : sizeof(float);
+    if (opacity_out)
+        A12_HIP(hipMemcpyAsync(opacity_out, M.opacity, rb * (size_t)M.L * M.n,
+                               hipMemcpyDeviceToHost, st));
+    const double *h = B->hout.as<double>() + B->h_off[model];
+    if (hline) A12_HIP(hipMemcpyAsync(hline, h, sizeof(double) * M.L, hipMemcpyDeviceToHost, st));
+    if (hcont)
+        A12_HIP(hipMemcpyAsync(hcont, h + M.L, sizeof(double) * M.L, hipMemcpyDeviceToHost, st));
+    long long ev[2] = {0, 0};
+    A12_HIP(hipMemcpyAsync(ev, B->evals.as<long long>() + 2 * model, sizeof(ev),
+                           hipMemcpyDeviceToHost, st));
+    A12_HIP(hipStreamSynchronize(st));
+    if (evals) {
+        evals[0] = ev[0];
+        evals[1] = ev[1];
+    }
+    return A12_OK;
+}
+
+int a12i_synthe_batch_info(a12_ctx *ctx, int64_t *info)
+{
+    SynBatch *B = ctx->sbatch;
+    if (!B) {
+        ctx->err = "a12_synthe_batch_info: no batch loaded";
+        return A12_ERR_STATE;
+    }
+    info[0] = B->nmodel;
+    info[1] = (int64_t)B->groups.size();
+    info[2] = (int64_t)B->work.bytes;
+    info[3] = (int64_t)B->budget;
+    info[4] = (int64_t)B->max_model_work;
     return A12_OK;
 }

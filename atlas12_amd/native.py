@@ -36,7 +36,10 @@ EXPORTS = ["a12_ctx_create", "a12_ctx_destroy", "a12_last_error", "a12_stream",
            "a12_atlas_set_shape", "a12_atlas_autotune", "a12_scale_model", "a12_mlp_init",
            "a12_mlp_get_params", "a12_mlp_set_params", "a12_mlp_set_data", "a12_mlp_grads",
            "a12_mlp_train", "a12_mlp_infer", "a12_mlp_gemm", "a12_atlas_tcorr3",
-           "a12_molec_set_table", "a12_molec_newton_batched"]
+           "a12_molec_set_table", "a12_molec_newton_batched",
+           "a12_synthe_shape_info", "a12_synthe_set_shape", "a12_synthe_batch_load",
+           "a12_synthe_batch_set_budget", "a12_synthe_batch_run", "a12_synthe_batch_fetch",
+           "a12_synthe_batch_info"]
 
 
 class A12Error(RuntimeError):
@@ -184,6 +187,64 @@ class Context:
         self._keep = [keep]
         return a
 
+    def synthe_set_shape(self, shape: int):
+        """Tile shape of the line-opacity gather (synthe_shapes())."""
+        self._check(self.lib.a12_synthe_set_shape(self.h, C.c_int32(shape)),
+                    "a12_synthe_set_shape")
+
+    # ------------------------------------------------------ batched SYNTHE
+    def synthe_batch_load(self, states, lines: np.ndarray, nb_shift, line_lo, line_hi):
+        """Many models over one shared line list sorted by nbuff: model k
+        uses lines[line_lo[k]:line_hi[k]] with nbuff - nb_shift[k]."""
+        lines = np.ascontiguousarray(lines, dtype=LTE_DTYPE)
+        atms = (SyntheAtm * len(states))()
+        kept = []
+        for k, st in enumerate(states):
+            atms[k] = self._atm(st)
+            kept.append(self._keep)
+        sh = np.ascontiguousarray(nb_shift, np.int32)
+        lo = np.ascontiguousarray(line_lo, np.int64)
+        hi = np.ascontiguousarray(line_hi, np.int64)
+        self._check(self.lib.a12_synthe_batch_load(
+            self.h, C.c_int32(len(states)), atms, sh.ctypes.data_as(C.c_void_p),
+            lo.ctypes.data_as(C.c_void_p), hi.ctypes.data_as(C.c_void_p),
+            lines.ctypes.data_as(C.c_void_p), C.c_int64(len(lines))), "a12_synthe_batch_load")
+        self._batch = [(st.nrhox, st.length) for st in states]
+
+    def synthe_batch_set_budget(self, nbytes: int):
+        self._check(self.lib.a12_synthe_batch_set_budget(self.h, C.c_int64(nbytes)),
+                    "a12_synthe_batch_set_budget")
+
+    def synthe_batch_run(self, prec: int = PREC_MIXED, transfer: bool = False,
+                         graph: bool = False):
+        self._check(self.lib.a12_synthe_batch_run(self.h, C.c_int32(prec), C.c_int32(transfer),
+                                                  C.c_int32(graph)), "a12_synthe_batch_run")
+        self._batch_prec = prec
+        self._batch_transfer = transfer
+
+    def synthe_batch_fetch(self, k: int, opacity: bool = True):
+        """(opacity (nrhox, length) or None, hnu_line, hnu_cont (None
+        without transfer), evals [all, Voigt core]) of model k."""
+        n, L = self._batch[k]
+        op = None
+        if opacity:
+            op = np.zeros((n, L), np.float64 if self._batch_prec == PREC_FP64 else np.float32)
+        hl = np.zeros(L) if self._batch_transfer else None
+        hc = np.zeros(L) if self._batch_transfer else None
+        ev = np.zeros(2, np.int64)
+        self._check(self.lib.a12_synthe_batch_fetch(
+            self.h, C.c_int32(k), None if op is None else op.ctypes.data_as(C.c_void_p),
+            None if hl is None else _ptr(hl), None if hc is None else _ptr(hc),
+            ev.ctypes.data_as(C.c_void_p)), "a12_synthe_batch_fetch")
+        return op, hl, hc, ev
+
+    def synthe_batch_info(self):
+        info = np.zeros(5, np.int64)
+        self._check(self.lib.a12_synthe_batch_info(self.h, info.ctypes.data_as(C.c_void_p)),
+                    "a12_synthe_batch_info")
+        return dict(zip(("nmodel", "groups", "work_bytes", "budget", "largest_model_work"),
+                        (int(v) for v in info)))
+
     def synthe_load(self, st: SyntheState, lines: np.ndarray):
         lines = np.ascontiguousarray(lines, dtype=LTE_DTYPE)
         a = self._atm(st)
@@ -238,6 +299,17 @@ class Context:
         import ctypes.util  # noqa: F401
         # a fetch of the eval count synchronises the context stream
         self.lib.a12_synthe_evals(self.h)
+
+
+def synthe_shapes():
+    """Gather tile shapes: list of (points per thread, lines per LDS stage)."""
+    lib = load_library()
+    out, buf = [], (C.c_int32 * 2)()
+    n = lib.a12_synthe_shape_info(C.c_int32(0), buf)
+    for i in range(n):
+        lib.a12_synthe_shape_info(C.c_int32(i), buf)
+        out.append((buf[0], buf[1]))
+    return out
 
 
 def synthe_line_opacity(ctx: Context, st: SyntheState, lines: np.ndarray, prec: int = PREC_MIXED):

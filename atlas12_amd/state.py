@@ -85,6 +85,32 @@ class SyntheState:
         return s
 
 
+    def window(self, offset: int, length: int) -> "SyntheState":
+        """The same atmosphere on another window of the same log-lambda
+        grid: first point ixwlbeg + offset, ``length`` points, wbegin as
+        the reference forms it (EXP(ixwlbeg * ratiolg),
+        src/mod_mklinelist.f90:535-540)."""
+        import copy
+        s = copy.copy(self)
+        s.ixwlbeg = self.ixwlbeg + int(offset)
+        s.length = int(length)
+        s.wbegin = float(np.exp(s.ixwlbeg * np.log(self.ratio)))
+        return s
+
+    def depths(self, idx) -> "SyntheState":
+        """The state on a subset of its layers (idx ascending)."""
+        import copy
+        idx = np.asarray(idx)
+        s = copy.copy(self)
+        s.nrhox = len(idx)
+        for k in ("continall", "contabs", "contscat", "xnfpel", "dopple"):
+            setattr(s, k, np.ascontiguousarray(getattr(self, k)[idx]))
+        for k in ("t", "xne", "rho", "xnfh", "xnfhe1", "xnfh2", "hckt", "hkt_syn", "hkt_atl",
+                  "rhox"):
+            setattr(s, k, np.ascontiguousarray(getattr(self, k)[idx]))
+        return s
+
+
 def load_coef(path: str):
     z = np.load(path)
     return (np.ascontiguousarray(z["coefj"], dtype=np.float64),

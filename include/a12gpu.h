@@ -104,6 +104,40 @@ int a12_synthe_load(a12_ctx *ctx, const a12_synthe_atm *atm,
                     const a12_lte_line *lines, int32_t nlines);
 int a12_synthe_opacity_run(a12_ctx *ctx, int32_t prec);
 int a12_synthe_opacity_fetch(a12_ctx *ctx, void *opacity_out);
+/* Tile shapes of the line-opacity gather (no counterpart in the reference;
+ * results do not depend on the shape): shape idx -> {points per thread,
+ * lines per LDS stage}; returns the number of shapes, -1 past the end.
+ * set_shape picks the shape of later runs (single and batched). */
+int a12_synthe_shape_info(int32_t idx, int32_t pts_chunk[2]);
+int a12_synthe_set_shape(a12_ctx *ctx, int32_t shape);
+
+/* Batched SYNTHE: nmodel models, each with its own atmosphere (its own
+ * nrhox, continuum tables and window: atms[k].length, wbegin; one
+ * resolving power), over one shared line list sorted by nbuff.  Model k
+ * takes lines line_lo[k] .. line_hi[k]-1 of the list, with nbuff
+ * relative to its window = list nbuff - nb_shift[k] (windows on one
+ * log-lambda grid); each model's result is what a12_synthe_line_opacity
+ * (and a12_synthe_transfer_run) gives for that model and those lines,
+ * replacing src/synthe.f90:476-718 per model.  The batch runs in groups
+ * whose work arrays fit a budget (default half the free memory, at most
+ * 64 GiB): per group the set-up kernels of each model and one persistent
+ * gather launch over all (model, tile, depth) items.  transfer != 0 adds
+ * the FP64 transfer of every model.  use_graph != 0 captures the whole
+ * sequence into a hipGraph on the first call and replays it on later calls
+ * with the same precision, shape and transfer flag. */
+int a12_synthe_batch_load(a12_ctx *ctx, int32_t nmodel, const a12_synthe_atm *atms,
+                          const int32_t *nb_shift, const int64_t *line_lo,
+                          const int64_t *line_hi, const a12_lte_line *lines, int64_t nlines);
+int a12_synthe_batch_set_budget(a12_ctx *ctx, int64_t bytes);
+int a12_synthe_batch_run(a12_ctx *ctx, int32_t prec, int32_t transfer, int32_t use_graph);
+/* Model k's opacity (length, nrhox) as a12_synthe_opacity_fetch, its
+ * transfer output and evals [all, Voigt core]; any pointer may be NULL. */
+int a12_synthe_batch_fetch(a12_ctx *ctx, int32_t model, void *opacity_out, double *hnu_line,
+                           double *hnu_cont, int64_t evals[2]);
+/* info: nmodel, groups of the last run, work bytes, budget, work bytes of
+ * the largest model (the smallest budget that runs). */
+int a12_synthe_batch_info(a12_ctx *ctx, int64_t info[5]);
+
 /* Device pointer of the opacity buffer of the last run and its precision. */
 const void *a12_synthe_opacity_device(a12_ctx *ctx, int32_t *prec_out);
 /* Profile contributions accumulated by the last run (exact count). */
