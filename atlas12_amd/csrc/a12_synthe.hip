@@ -431,8 +431,9 @@ __global__ void k_tile_ranges(int ntiles, int tw, int L, const int *__restrict__
 // staged line that covers its points:
 //   n = |i - nbuff| <= nstep:  n <= n10dop -> kappa0 H(n dv, a)
 //                              else        -> x_wing / n^2
-// MIXED evaluates H in FP32 and accumulates with Neumaier compensation;
-// FP64 does both in double.  Every point sees the lines in the same order
+// MIXED evaluates H in FP32, the n^-2 wing with the hardware reciprocal
+// (1 ulp), and accumulates with Neumaier compensation; FP64 does all of it
+// in double with IEEE division.  Every point sees the lines in the same order
 // (class, then centre position) whatever PTS and CHUNK are, so the result
 // does not depend on the tile shape.
 // --------------------------------------------------------------------------
@@ -516,7 +517,7 @@ __device__ __forceinline__ void tile_body(const TileArgs &a, int tile, int j)
                     if (!F64) {
                         float v;
                         if (n <= n10) v = (float)s_k0[k] * voigt_mixed((float)n * (float)s_dv[k], (float)s_a[k]);
-                        else v = (float)s_xw[k] / ((float)n * (float)n);
+                        else v = (float)s_xw[k] * __builtin_amdgcn_rcpf((float)n * (float)n);
                         acc32[p].add(v);
                     } else {
                         double v;

@@ -361,3 +361,113 @@ def atlas_iteration_lists(n: int, wl0: float, wl1: float, seed: int, molecules=(
         pred = np.ascontiguousarray(pred[np.argsort(pred[:, 0], kind="stable")])
     tio = tio_mol2_records(n_tio, wl0, wl1, seed + 2000, loggf=tio_loggf)
     return pred, tio
+
+
+# ---------------------------------------------------------------------------
+# Batched SYNTHE: many models over one shared line list
+# ---------------------------------------------------------------------------
+
+@dataclass
+class BatchModel:
+    nrhox: int
+    teff: float
+    logg: float
+    mh: float
+    wl0: float
+    wl1: float
+    offset: int   # first grid point, relative to the shared grid
+    length: int
+
+
+def _depth_resample(a: np.ndarray, n: int) -> np.ndarray:
+    """Resample a (nrhox, ...) array onto n layers spanning the same
+    column-mass range (linear in layer index; logarithmic where positive)."""
+    a = np.asarray(a, np.float64)
+    src = np.linspace(0.0, 1.0, a.shape[0])
+    dst = np.linspace(0.0, 1.0, n)
+    flat = a.reshape(a.shape[0], -1)
+    out = np.empty((n, flat.shape[1]))
+    for c in range(flat.shape[1]):
+        col = flat[:, c]
+        if np.all(col > 0):
+            out[:, c] = np.exp(np.interp(dst, src, np.log(col)))
+        elif np.all(col == 0):
+            out[:, c] = 0.0
+        else:
+            out[:, c] = np.interp(dst, src, col)
+    return out.reshape((n,) + a.shape[1:])
+
+
+def batch_model_state(base, m: BatchModel, grid: Grid, nedge: int = 8):
+    """A synthetic model from a reference-derived state: layers resampled
+    to m.nrhox, T and Doppler widths scaled with Teff, densities with log g
+    and metals with [M/H], and a smooth continuum (log kappa linear in log
+    lambda, 90 % absorption) tabulated on nedge edges over the model's
+    window of the shared grid.  Not a converged atmosphere: a workload of
+    the right shape and opacity level."""
+    from .state import SyntheState
+    n = m.nrhox
+    ts = m.teff / 5777.0
+    gs = 10.0 ** (m.logg - 4.44)
+    r = lambda k: _depth_resample(getattr(base, k), n)  # noqa: E731
+    xnfpel = r("xnfpel")
+    xnfpel[:, 12:] *= 10.0 ** m.mh          # elements beyond H and He
+    xnfpel *= gs
+    wbegin = math.exp((grid.ixwlbeg + m.offset) * grid.ratiolg)
+    wend = wbegin * grid.ratio ** (m.length - 1)
+    wledge = np.exp(np.linspace(math.log(wbegin * 0.999), math.log(wend * 1.001), nedge))
+    mid = base.continall[:, base.continall.shape[1] // 2]
+    c0 = np.interp(np.linspace(0.0, 1.0, n), np.linspace(0.0, 1.0, base.nrhox), mid)
+    c0 = c0 + 0.5 * (m.logg - 4.44) + 0.3 * m.mh
+    cont = np.empty((n, 3 * (nedge - 1)))
+    for e in range(nedge - 1):
+        w0, w1 = wledge[e], wledge[e + 1]
+        for q, w in enumerate((w0, 0.5 * (w0 + w1), w1)):
+            cont[:, 3 * e + q] = c0 + 1.5 * math.log10(w / 500.0)
+    return SyntheState(
+        nrhox=n, length=m.length, ixwlbeg=grid.ixwlbeg + m.offset, wbegin=wbegin,
+        ratio=grid.ratio, resolu=grid.resolu, wledge=wledge, continall=cont,
+        contabs=cont + math.log10(0.9), contscat=cont + math.log10(0.1), xnfpel=xnfpel,
+        dopple=r("dopple") * math.sqrt(ts), t=r("t") * ts, xne=r("xne") * math.sqrt(gs) * ts ** 4,
+        rho=r("rho") * gs / ts, xnfh=r("xnfh") * gs, xnfhe1=r("xnfhe1") * gs,
+        xnfh2=r("xnfh2") * gs, hckt=r("hckt") / ts, hkt_syn=r("hkt_syn") / ts,
+        hkt_atl=r("hkt_atl") / ts, rhox=r("rhox"))
+
+
+def synthe_batch_workload(base, nmodel: int = 256, nlines: int = 10_000_000, seed: int = 3,
+                          wl_lo: float = 300.0, wl_hi: float = 1000.0,
+                          resolu: float = 300_000.0):
+    """The batched-SYNTHE workload: nmodel models with layers uniform in
+    48-96, Teff 3500-8000 K, log g 1-5, [M/H] -2..+0.5, each with its own
+    window 10-300 nm wide in wl_lo..wl_hi at resolu, over one list of
+    nlines atomic lines (log gf ~ N(-3, 1.5), lower level exponential with a
+    2 eV scale) sorted by centre on the shared grid.  Returns (grid, models,
+    states, lines, nb_shift, line_lo, line_hi): model k takes
+    lines[line_lo[k]:line_hi[k]] with nbuff - nb_shift[k]; the range is
+    read_gfall's (centres within the window plus its 0.1 nm margin scaled
+    by wlbeg / 500, src/mod_mklinelist.f90:846-847, :903-907)."""
+    grid = Grid.make(wl_lo, wl_hi, resolu)
+    rec = synthetic_gfall(nlines, wl_lo, wl_hi, seed, loggf=("normal", -3.0, 1.5),
+                          elo=("exp", 2.0))
+    lines = gfall_to_lte(rec, grid)
+    wl = rec["wl"][(rec["wl"] >= grid.wlbeg - 0.1 * grid.wlbeg / 500.0)
+                   & (rec["wl"] <= grid.wlend + 0.1 * grid.wlbeg / 500.0)].astype(np.float64)
+    order = np.argsort(lines["nbuff"], kind="stable")
+    lines, wl = lines[order], wl[order]
+    rng = np.random.default_rng(seed + 1)
+    models, states, shift, lo, hi = [], [], [], [], []
+    for _ in range(nmodel):
+        width = rng.uniform(10.0, 300.0)
+        w0 = rng.uniform(wl_lo, wl_hi - width)
+        ix0 = int(math.ceil(math.log(w0) / grid.ratiolg))
+        length = int(math.log((w0 + width) / w0) / grid.ratiolg)
+        m = BatchModel(int(rng.integers(48, 97)), rng.uniform(3500.0, 8000.0),
+                       rng.uniform(1.0, 5.0), rng.uniform(-2.0, 0.5), w0, w0 + width,
+                       ix0 - grid.ixwlbeg, length)
+        models.append(m)
+        states.append(batch_model_state(base, m, grid))
+        margin = 0.1 * (m.wl0 / 500.0 if m.wl0 > 500.0 else 1.0)
+        shift.append(m.offset)
+        lo.append(int(np.searchsorted(wl, m.wl0 - margin)))
+        hi.append(int(np.searchsorted(wl, m.wl1 + margin, side="right")))
+    return grid, models, states, lines, np.array(shift), np.array(lo), np.array(hi)

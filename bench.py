@@ -57,6 +57,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-synthe", action="store_true", help="skip the plain-SYNTHE block")
     ap.add_argument("--cpu-depths", type=int, default=24)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-SYNTHE block")
+    ap.add_argument("--batch-models", type=int, default=32,
+                    help="models of the 256-model batched-SYNTHE workload that are run "
+                         "(the first ones drawn; 256 = all)")
+    ap.add_argument("--batch-lines", type=int, default=10_000_000)
     ap.add_argument("--profile-only", action="store_true",
                     help="warmup+steps iterations of both workloads at both precisions and a "
                          "device copy of --calib-mib MiB (for rocprofv3 runs); no JSON")
@@ -181,6 +186,14 @@ def synthe_run(args, ctx, native, wl):
         return time.perf_counter() - t, stages / max(k, 1)
 
     run(native.PREC_MIXED, args.warmup)
+    tune = {}
+    for shape, (pts, chunk) in enumerate(native.synthe_shapes()):
+        ctx.synthe_set_shape(shape)
+        run(native.PREC_MIXED, 1)
+        tune[shape] = float(np.median([run(native.PREC_MIXED, 1)[1][3] for _ in range(3)]))
+    best = min(tune, key=tune.get)
+    ctx.synthe_set_shape(best)
+    shapes = native.synthe_shapes()
     if args.profile_only:
         run(native.PREC_MIXED, args.steps)
         run(native.PREC_FP64, 1)
@@ -215,12 +228,15 @@ def synthe_run(args, ctx, native, wl):
                      "evals_per_step": int(evals64)},
         "mixed_over_fp64": value / value64,
         "flux_max_rel_err_mixed_vs_fp64": flux_err,
+        "autotune": {"shape_pts_chunk": list(shapes[best]),
+                     "gather_ms_per_shape": {str(shapes[k]): v for k, v in tune.items()}},
         "transfer": {"ms": t_tr, "points_per_s": st.length / (t_tr * 1e-3) if t_tr > 0 else None},
         "stage_ms_mixed": {"prep_continuum": st_mix[0], "line_setup": st_mix[1],
                            "partition": st_mix[2], "tile_gather": st_mix[3]},
         "stage_ms_fp64": {"prep_continuum": st_64[0], "line_setup": st_64[1],
                           "partition": st_64[2], "tile_gather": st_64[3]},
-        "roofline": {"bound": "valu", "kernel": "k_tile<mixed>",
+        "roofline": {"bound": "valu",
+                     "kernel": f"k_tile<false, {shapes[best][0]}, {shapes[best][1]}>",
                      "achieved": flops / (tile_ms * 1e-3) / 1e12, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": flops / (tile_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,
                      "flops_per_launch": flops, "avg_launch_ms": tile_ms},
@@ -231,6 +247,68 @@ def synthe_run(args, ctx, native, wl):
                                     "achieved": tf, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                                     "frac": tf / PEAK_FP64_TFLOPS}
     return res, st, lines
+
+
+# ------------------------------------------------------- batched SYNTHE
+def batch_run(args, ctx, native, wl):
+    """The 256-model batched-SYNTHE workload (its first --batch-models
+    models): shape from the autotuner on a 4-model tuning batch, then one
+    hipGraph-captured mixed run (opacity + transfer of every model) and one
+    FP64 run of the same batch, timed on the host around the synchronous
+    calls."""
+    from atlas12_amd.state import SyntheState
+    base = SyntheState.load(os.path.join(GOLDEN, "synthe_plain_state.npz"))
+    log("batch: generating the workload")
+    g, models, states, lines, sh, lo, hi = wl.synthe_batch_workload(base, 256, args.batch_lines)
+    nm = max(1, min(args.batch_models, 256))
+    sel = list(range(nm))
+    shapes = native.synthe_shapes()
+    tune_sel = sorted(range(256), key=lambda k: models[k].length)[124:128]  # median-size models
+    ctx.synthe_batch_load([states[k] for k in tune_sel], lines, sh[tune_sel], lo[tune_sel],
+                          hi[tune_sel])
+    tune = {}
+    for shape in range(len(shapes)):
+        ctx.synthe_set_shape(shape)
+        ctx.synthe_batch_run(native.PREC_MIXED)
+        t = time.perf_counter()
+        ctx.synthe_batch_run(native.PREC_MIXED)
+        tune[shape] = time.perf_counter() - t
+    best = min(tune, key=tune.get)
+    ctx.synthe_set_shape(best)
+    ctx.synthe_batch_load([states[k] for k in sel], lines, sh[sel], lo[sel], hi[sel])
+    out = {}
+    for prec in (native.PREC_MIXED, native.PREC_FP64):
+        log(f"batch: {nm} models, prec {prec}")
+        t = time.perf_counter()
+        ctx.synthe_batch_run(prec, transfer=True, graph=prec == native.PREC_MIXED)
+        dt = time.perf_counter() - t
+        ev = np.zeros(2, np.int64)
+        hl = []
+        for k in range(nm):
+            _, h, _, e = ctx.synthe_batch_fetch(k, opacity=False)
+            ev += e
+            hl.append(h)
+        out[prec] = (dt, ev, hl, ctx.synthe_batch_info())
+    (t_m, ev_m, hl_m, info), (t_d, ev_d, hl_d, _) = out[native.PREC_MIXED], out[native.PREC_FP64]
+    err = max(float(np.max(np.abs(a - b) / np.abs(b))) for a, b in zip(hl_m, hl_d))
+    pts = sum(models[k].length for k in sel)
+    return {
+        "workload": "synthe_batch: 256 models (layers 48-96, Teff 3500-8000 K, log g 1-5, "
+                    "[M/H] -2..+0.5, windows 10-300 nm in 300-1000 nm at R=300,000) over one "
+                    f"{args.batch_lines:,}-line list; run: the first {nm} models drawn",
+        "models_run": nm, "points": int(pts),
+        "layers_mean": float(np.mean([models[k].nrhox for k in sel])),
+        "lines_per_model_mean": float(np.mean(hi[sel] - lo[sel])),
+        "value": float(ev_m[0]) / t_m, "unit": "evals/s (opacity + transfer wall time)",
+        "s_per_batch": t_m, "evals": int(ev_m[0]), "core_evals": int(ev_m[1]),
+        "fp64_gpu": {"value": float(ev_d[0]) / t_d, "s_per_batch": t_d, "evals": int(ev_d[0])},
+        "mixed_over_fp64": (float(ev_m[0]) / t_m) / (float(ev_d[0]) / t_d),
+        "flux_max_rel_err_mixed_vs_fp64": err,
+        "groups": info["groups"], "work_bytes": info["work_bytes"],
+        "graph": "mixed run captured in one hipGraph (first launch timed)",
+        "autotune": {"shape_pts_chunk": list(shapes[best]),
+                     "tuning_batch_s_per_shape": {str(shapes[k]): v for k, v in tune.items()}},
+    }
 
 
 # ------------------------------------------------------- starting network
@@ -343,6 +421,7 @@ def main():
     if not args.no_synthe:
         syn = synthe_run(args, ctx, native, wl)
     net = None if args.profile_only else mlp_run(args, ctx, native)
+    bat = None if (args.profile_only or args.no_batch) else batch_run(args, ctx, native, wl)
     if args.profile_only:
         nel = args.calib_mib * (1 << 20) // 4
         src = torch.ones(nel, dtype=torch.float32, device="cuda")
@@ -412,6 +491,8 @@ def main():
     }
     if net is not None:
         result["starting_network"] = net
+    if bat is not None:
+        result["synthe_batch"] = bat
     if syn is not None:
         result["synthe_plain"] = syn[0]
         result["synthe_plain"]["roofline"]["traffic"] = traffic.get("k_tile<false>")
