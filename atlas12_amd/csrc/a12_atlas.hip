@@ -140,6 +140,11 @@ __device__ int lo_reach(int first, int last, int step, const double *__restrict_
     int iw = first;
     const int nmax = step > 0 ? last - first + 1 : first - last + 1;
     if (nmax <= 0) return 0;
+    auto below = [&](int pos) {  // the walk's test at walk position pos
+        const int pw = first + pos * step;
+        const double v = step > 0 ? (waveset[pw] - wlvac) / dopwave : (wlvac - waveset[pw]) / dopwave;
+        return lo_cv<double>(center, v, adamp, h0, h1, h2) < tc;
+    };
     for (; cnt < nmax; ++cnt, iw += step) {
         const double v = step > 0 ? (waveset[iw] - wlvac) / dopwave : (wlvac - waveset[iw]) / dopwave;
         const bool mono = adamp <= 0.20 ? v > 10.0 : !(adamp <= 1.4 && adamp + v <= 3.2);
@@ -147,13 +152,13 @@ __device__ int lo_reach(int first, int last, int step, const double *__restrict_
         if (lo_cv<double>(center, v, adamp, h0, h1, h2) < tc) return cnt + 1;
     }
     if (cnt == nmax) return cnt;
-    // bisection over walk positions [cnt, nmax): first with value < tc
+    // first walk position in [cnt, nmax) with value < tc; every position
+    // below lo is known >= tc, every position from hi on < tc (hi = nmax:
+    // none known)
     int lo = cnt, hi = nmax;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        const int pw = first + mid * step;
-        const double v = step > 0 ? (waveset[pw] - wlvac) / dopwave : (wlvac - waveset[pw]) / dopwave;
-        if (lo_cv<double>(center, v, adamp, h0, h1, h2) < tc) hi = mid;
+        if (below(mid)) hi = mid;
         else lo = mid + 1;
     }
     return lo < nmax ? lo + 1 : nmax;
@@ -162,7 +167,9 @@ __device__ int lo_reach(int first, int last, int step, const double *__restrict_
 // One lane per (line, depth).  LINOP1 computes the coarse depths J = 8, 16,
 // ... first, then each fine depth J in K-7..K-1 only where coarse depth K-8
 // or K passed both centre tests (IFJ, :18239-18365); the decision depends on
-// nothing else, so a fine lane re-tests its two coarse neighbours itself.
+// nothing else, so every lane takes its own centre test and a fine lane
+// reads its two coarse neighbours' results from LDS (re-testing them itself
+// when they fall in another workgroup).
 // o_state[l*n + j]: bit 31 computed, bits 0-7 red reach, 8-15 blue reach
 // (points nu.. and nu-1..); o_par: centre, a, Doppler width in nm.
 __device__ __forceinline__ bool lo_center_ok(double cgf, double elo, double xd, double tc,
@@ -187,46 +194,67 @@ __global__ __launch_bounds__(256) void k_lo_setup(
     long long *__restrict__ blk_evals)
 {
 #pragma clang fp contract(off)
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    long long ev = 0;
-    if (g < (long long)nl * n) {
-        const int l = (int)(g / n), j = (int)(g - (long long)l * n);
-        const int J = j + 1, ncoarse = (n / 8) * 8;
-        int st = 0;
-        if (J <= ncoarse) {
-            const int nu = l_nu[l], nucont = l_nucont[l], nel = l_nel[l] - 1;
-            const double cgf = l_const[5 * (size_t)l + 1], elo = l_const[5 * (size_t)l + 2];
-            const double *xd = xnfdop + (size_t)nel * n, *tcv = tabcont + (size_t)nucont * n;
-            const double *hckt = dep + 3 * n;
-            bool on = true;
-            if (J % 8 != 0) {  // fine depth: a neighbouring coarse depth must have passed
-                const int K = (J / 8 + 1) * 8;
-                double c;
-                on = (K - 8 >= 8 && lo_center_ok(cgf, elo, xd[K - 9], tcv[K - 9], hckt[K - 9], &c)) ||
-                     lo_center_ok(cgf, elo, xd[K - 1], tcv[K - 1], hckt[K - 1], &c);
-            }
-            double center;
-            const double tc = tcv[j];
-            if (on && lo_center_ok(cgf, elo, xd[j], tc, hckt[j], &center)) {
-                const double wlvac = l_const[5 * (size_t)l + 0], gr = l_const[5 * (size_t)l + 3],
-                             gs = l_const[5 * (size_t)l + 4], gw = l_const[5 * (size_t)nl + l];
-                const double dop = dopple[(size_t)nel * n + j];
-                const double adamp = (gr + gs * dep[2 * n + j] + gw * dep[5 * n + j]) / dop;
-                const double dopwave = dop * wlvac;
-                // red: IW = NU .. min(NU+100, NUMNU); blue: NU-1 down, at most 100
-                const int rn = lo_reach(nu, min(nu + kMaxWing, numnu - 1), 1, waveset, wlvac,
-                                        dopwave, center, adamp, tc, h0, h1, h2);
-                const int bn = lo_reach(nu - 1, max(nu - kMaxWing, 0), -1, waveset, wlvac,
-                                        dopwave, center, adamp, tc, h0, h1, h2);
-                st = (int)(0x80000000u | (unsigned)rn | ((unsigned)bn << 8));
-                o_par[3 * g + 0] = center;
-                o_par[3 * g + 1] = adamp;
-                o_par[3 * g + 2] = dopwave;
-                ev = rn + bn;
-            }
-        }
-        o_state[g] = st;
+    __shared__ int s_ok[256];
+    const long long b0 = (long long)blockIdx.x * blockDim.x;
+    const long long g = b0 + threadIdx.x;
+    // (line, depth) of this lane without a 64-bit integer division: the
+    // double quotient is exact to +-1 for any g below 2^52
+    long long lq = (long long)((double)g * (1.0 / (double)n));
+    if (lq * n > g) --lq;
+    if ((lq + 1) * n <= g) ++lq;
+    const int l = (int)lq, j = (int)(g - lq * n);
+    const int J = j + 1, ncoarse = (n / 8) * 8;
+    const bool valid = g < (long long)nl * n && J <= ncoarse;
+    int nu = 0, nucont = 0, nel = 0;
+    double cgf = 0.0, elo = 0.0, center = 0.0;
+    const double *xd = xnfdop, *tcv = tabcont, *hckt = dep + 3 * n;
+    bool own = false;
+    if (valid) {
+        nu = l_nu[l];
+        nucont = l_nucont[l];
+        nel = l_nel[l] - 1;
+        cgf = l_const[5 * (size_t)l + 1];
+        elo = l_const[5 * (size_t)l + 2];
+        xd = xnfdop + (size_t)nel * n;
+        tcv = tabcont + (size_t)nucont * n;
+        own = lo_center_ok(cgf, elo, xd[j], tcv[j], hckt[j], &center);
     }
+    s_ok[threadIdx.x] = own;
+    __syncthreads();
+    long long ev = 0;
+    int st = 0;
+    if (valid && own) {
+        bool on = true;
+        if (J % 8 != 0) {  // fine depth: a neighbouring coarse depth must have passed
+            const int K = (J / 8 + 1) * 8;
+            auto coarse_ok = [&](int Kd) {  // the coarse lane's own test, from LDS when in block
+                const long long gk = (long long)l * n + (Kd - 1);
+                if (gk >= b0 && gk < b0 + (long long)blockDim.x) return s_ok[gk - b0] != 0;
+                double c;
+                return lo_center_ok(cgf, elo, xd[Kd - 1], tcv[Kd - 1], hckt[Kd - 1], &c);
+            };
+            on = (K - 8 >= 8 && coarse_ok(K - 8)) || coarse_ok(K);
+        }
+        if (on) {
+            const double tc = tcv[j];
+            const double wlvac = l_const[5 * (size_t)l + 0], gr = l_const[5 * (size_t)l + 3],
+                         gs = l_const[5 * (size_t)l + 4], gw = l_const[5 * (size_t)nl + l];
+            const double dop = dopple[(size_t)nel * n + j];
+            const double adamp = (gr + gs * dep[2 * n + j] + gw * dep[5 * n + j]) / dop;
+            const double dopwave = dop * wlvac;
+            // red: IW = NU .. min(NU+100, NUMNU); blue: NU-1 down, at most 100
+            const int rn = lo_reach(nu, min(nu + kMaxWing, numnu - 1), 1, waveset, wlvac, dopwave,
+                                    center, adamp, tc, h0, h1, h2);
+            const int bn = lo_reach(nu - 1, max(nu - kMaxWing, 0), -1, waveset, wlvac, dopwave,
+                                    center, adamp, tc, h0, h1, h2);
+            st = (int)(0x80000000u | (unsigned)rn | ((unsigned)bn << 8));
+            o_par[3 * g + 0] = center;
+            o_par[3 * g + 1] = adamp;
+            o_par[3 * g + 2] = dopwave;
+            ev = rn + bn;
+        }
+    }
+    if (g < (long long)nl * n) o_state[g] = st;
     __shared__ long long red[256];
     red[threadIdx.x] = ev;
     __syncthreads();
@@ -237,18 +265,58 @@ __global__ __launch_bounds__(256) void k_lo_setup(
     if (threadIdx.x == 0) blk_evals[blockIdx.x] = red[0];
 }
 
-__global__ void k_lo_sum(const long long *__restrict__ blk, int nb, long long *out)
+// Sum of nb per-block counts in a fixed order: pass 1, block b of gridDim.x
+// sums its contiguous share into part[b]; pass 2 (one block, part = null)
+// sums the parts into *out.
+__global__ void k_lo_sum(const long long *__restrict__ blk, int nb, long long *__restrict__ part,
+                         long long *__restrict__ out)
 {
     __shared__ long long red[256];
+    const int per = (nb + gridDim.x - 1) / gridDim.x;
+    const int b0 = blockIdx.x * per, b1 = min(nb, b0 + per);
     long long s = 0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) s += blk[i];
+    for (int i = b0 + threadIdx.x; i < b1; i += blockDim.x) s += blk[i];
     red[threadIdx.x] = s;
     __syncthreads();
     for (int k = blockDim.x >> 1; k > 0; k >>= 1) {
         if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *out = red[0];
+    if (threadIdx.x == 0) {
+        if (part) part[blockIdx.x] = red[0];
+        else *out = red[0];
+    }
+}
+
+constexpr int kSumParts = 1024;
+
+// Widest red and blue reach of each line over its depths (0 when no depth
+// passed the centre tests): l_reach[l] = red | blue << 8.  Lets the gather
+// skip, before staging anything, the candidate lines that cannot reach a
+// block's points.
+__global__ __launch_bounds__(256) void k_lo_linereach(int nl, int n,
+                                                      const int *__restrict__ state,
+                                                      int *__restrict__ l_reach)
+{
+    // one wavefront per line: coalesced loads of its n states, shuffle max
+    const int lane = threadIdx.x & 63;
+    const int nw = gridDim.x * (blockDim.x >> 6);
+    for (int l = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); l < nl; l += nw) {
+        int r = 0, b = 0;
+        const int *st = state + (size_t)l * n;
+        for (int j = lane; j < n; j += 64) {
+            const int v = st[j];
+            if (v < 0) {
+                r = max(r, v & 0xff);
+                b = max(b, (v >> 8) & 0xff);
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            r = max(r, __shfl_xor(r, o, 64));
+            b = max(b, __shfl_xor(b, o, 64));
+        }
+        if (lane == 0) l_reach[l] = r | (b << 8);
+    }
 }
 
 // Per-(line, depth) values one contribution needs.  FP64: centre, a,
@@ -341,7 +409,7 @@ __device__ __forceinline__ LoPar lo_shfl(const LoPar &q, int src)
 template <bool F64, int kLoPts, int kLoBlock, int kLoBatch, bool kFlat>
 __global__ __launch_bounds__(kLoBlock) void k_lo_gather(
     int n, int numnu, int tiles, const int *__restrict__ seg, const int *__restrict__ l_nu,
-    const int *__restrict__ state, const double *__restrict__ par,
+    const int *__restrict__ l_reach, const int *__restrict__ state, const double *__restrict__ par,
     const double *__restrict__ l_const, const double *__restrict__ waveset,
     const double *__restrict__ h0, const double *__restrict__ h1, const double *__restrict__ h2,
     const float *__restrict__ f0, const float *__restrict__ f1, const float *__restrict__ f2,
@@ -352,7 +420,8 @@ __global__ __launch_bounds__(kLoBlock) void k_lo_gather(
     // LDS: F64 sums, or FP32 sums + compensations (same bytes)
     __shared__ double acc_d[kLoPts * kLoBlock];
     __shared__ int s_state[kLoBatch * A12_NRMAX_LAYERS];
-    __shared__ int s_nu[kLoBatch];
+    __shared__ int s_nu[kLoBatch], s_row[kLoBatch];
+    __shared__ int s_idx[kLoBlock], s_wcnt[kLoBlock / 64], s_nact;
     __shared__ double s_wl[kLoBatch];
     __shared__ double s_w[kLoBlock / 8 * kLoPts];  // WAVESET of the block's points
     float *acc_s = reinterpret_cast<float *>(acc_d);
@@ -373,77 +442,112 @@ __global__ __launch_bounds__(kLoBlock) void k_lo_gather(
     // (red from NU reaches NU..NU+100, blue NU-100..NU-1)
     const int bfirst = tile0 * kLoPts, blast = min((tile0 + per) * kLoPts, numnu) - 1;
     const int lbeg = seg[max(0, bfirst - kMaxWing)], lend = seg[min(numnu, blast + kMaxWing) + 1];
-    for (int b0 = lbeg; b0 < lend; b0 += kLoBatch) {
-        const int nb = min(kLoBatch, lend - b0);
-        __syncthreads();
-        for (int i = t; i < nb * n; i += kLoBlock) s_state[i] = state[(size_t)b0 * n + i];
-        if (t < nb) {
-            s_nu[t] = l_nu[b0 + t];
-            s_wl[t] = l_const[5 * (size_t)(b0 + t)];
+    // candidates are scanned kLoBlock at a time; those whose widest reach
+    // meets [bfirst, blast] are compacted in store order into s_idx, then
+    // staged and processed kLoBatch at a time
+    for (int c0 = lbeg; c0 < lend; c0 += kLoBlock) {
+        {
+            const int q = c0 + t;
+            bool act = false;
+            if (q < lend) {
+                const int k = l_nu[q], rr = l_reach[q], rmax = rr & 0xff, bmax = (rr >> 8) & 0xff;
+                act = (rmax > 0 && k <= blast && k + rmax - 1 >= bfirst) ||
+                      (bmax > 0 && k - 1 >= bfirst && k - bmax <= blast);
+            }
+            const unsigned long long m = __ballot(act);
+            __syncthreads();  // the previous chunk is done with s_idx
+            if (lane == 0) s_wcnt[t >> 6] = __popcll(m);
+            __syncthreads();
+            if (act) {
+                int pos = __popcll(m & ((1ull << lane) - 1ull));
+                for (int w = 0; w < (t >> 6); ++w) pos += s_wcnt[w];
+                s_idx[pos] = q;
+            }
+            if (t == 0) {
+                int sm = 0;
+                for (int w = 0; w < kLoBlock / 64; ++w) sm += s_wcnt[w];
+                s_nact = sm;
+            }
         }
         __syncthreads();
-        if (!kFlat && !live) continue;
-        for (int i = 0; i < nb; ++i) {
-            // this lane's items: tile points in [ra, rb] (red) and [ba, bb] (blue)
-            int ra = 1, rb = 0, ba = 1, bb = 0;
-            if (live) {
-                const int st = s_state[i * n + j];
-                if (st < 0) {
-                    const int k = s_nu[i], rn = st & 0xff, bn = (st >> 8) & 0xff;
-                    ra = max(k, iw0);
-                    rb = min(k + rn - 1, iw1);
-                    ba = max(k - bn, iw0);
-                    bb = min(k - 1, iw1);
-                }
+        const int nact = s_nact;
+        for (int b0 = 0; b0 < nact; b0 += kLoBatch) {
+            const int nb = min(kLoBatch, nact - b0);
+            __syncthreads();
+            for (int i = t; i < nb * n; i += kLoBlock) {
+                const int li = i / n;
+                s_state[i] = state[(size_t)s_idx[b0 + li] * n + (i - li * n)];
             }
-            const int nred = max(0, rb - ra + 1), nblue = max(0, bb - ba + 1), c = nred + nblue;
-            const double wl = s_wl[i];
-            const size_t orow = (size_t)(b0 + i) * n;
-            if (!kFlat) {
-                if (c == 0) continue;
-                const LoPar q = lo_par<F64>(par, orow + j);
-                const double *w_t = s_w + lt * kLoPts - iw0;
-                for (int iw = ra; iw <= rb; ++iw)
-                    lo_accum<F64>(q, w_t[iw] - wl, (iw - iw0) * kLoBlock + t, acc_d, acc_s, acc_c,
-                                  h0, h1, h2, f0, f1, f2);
-                for (int iw = ba; iw <= bb; ++iw)
-                    lo_accum<F64>(q, wl - w_t[iw], (iw - iw0) * kLoBlock + t, acc_d, acc_s, acc_c,
-                                  h0, h1, h2, f0, f1, f2);
-            } else {
-                // wave-wide inclusive scan of the item counts
-                int incl = c;
-                for (int dd = 1; dd < 64; dd <<= 1) {
-                    const int y = __shfl_up(incl, dd, 64);
-                    if (lane >= dd) incl += y;
-                }
-                const int total = __shfl(incl, 63, 64);
-                if (total == 0) continue;
-                const LoPar q = c > 0 ? lo_par<F64>(par, orow + j) : LoPar{1.0, 0.0, 1.0};
-                // owner's item layout: red offset, red count, blue offset, tile base in s_w
-                // (an empty range packs as offset 0: its bounds may lie outside the tile)
-                const int pk = (nred ? ra - iw0 : 0) | (nred << 6) | ((nblue ? ba - iw0 : 0) << 12) |
-                               ((lt * kLoPts) << 18);
-                const int excl = incl - c;
-                for (int base = 0; base < total; base += 64) {
-                    const int item = base + lane;
-                    int lo = 0, hi = 63;
-#pragma unroll
-                    for (int s = 0; s < 6; ++s) {
-                        const int mid = (lo + hi) >> 1;
-                        if (__shfl(incl, mid, 64) > item) hi = mid;
-                        else lo = mid + 1;
+            if (t < nb) {
+                const int lq = s_idx[b0 + t];
+                s_nu[t] = l_nu[lq];
+                s_wl[t] = l_const[5 * (size_t)lq];
+                s_row[t] = lq;
+            }
+            __syncthreads();
+            if (!kFlat && !live) continue;
+            for (int i = 0; i < nb; ++i) {
+                // this lane's items: tile points in [ra, rb] (red) and [ba, bb] (blue)
+                int ra = 1, rb = 0, ba = 1, bb = 0;
+                if (live) {
+                    const int st = s_state[i * n + j];
+                    if (st < 0) {
+                        const int k = s_nu[i], rn = st & 0xff, bn = (st >> 8) & 0xff;
+                        ra = max(k, iw0);
+                        rb = min(k + rn - 1, iw1);
+                        ba = max(k - bn, iw0);
+                        bb = min(k - 1, iw1);
                     }
-                    const int o = min(lo, 63);
-                    const int ex = __shfl(excl, o, 64), opk = __shfl(pk, o, 64);
-                    const LoPar qo = lo_shfl(q, o);
-                    if (item < total) {
-                        const int kk = item - ex, onred = (opk >> 6) & 63;
-                        const bool blue = kk >= onred;
-                        const int pt = blue ? ((opk >> 12) & 63) + (kk - onred) : (opk & 63) + kk;
-                        const double w = s_w[(opk >> 18) + pt];
-                        const int to = t - lane + o;  // owner's lane in the block
-                        lo_accum<F64>(qo, blue ? wl - w : w - wl, pt * kLoBlock + to, acc_d, acc_s,
-                                      acc_c, h0, h1, h2, f0, f1, f2);
+                }
+                const int nred = max(0, rb - ra + 1), nblue = max(0, bb - ba + 1), c = nred + nblue;
+                const double wl = s_wl[i];
+                const size_t orow = (size_t)s_row[i] * n;
+                if (!kFlat) {
+                    if (c == 0) continue;
+                    const LoPar q = lo_par<F64>(par, orow + j);
+                    const double *w_t = s_w + lt * kLoPts - iw0;
+                    for (int iw = ra; iw <= rb; ++iw)
+                        lo_accum<F64>(q, w_t[iw] - wl, (iw - iw0) * kLoBlock + t, acc_d, acc_s, acc_c,
+                                      h0, h1, h2, f0, f1, f2);
+                    for (int iw = ba; iw <= bb; ++iw)
+                        lo_accum<F64>(q, wl - w_t[iw], (iw - iw0) * kLoBlock + t, acc_d, acc_s, acc_c,
+                                      h0, h1, h2, f0, f1, f2);
+                } else {
+                    // wave-wide inclusive scan of the item counts
+                    int incl = c;
+                    for (int dd = 1; dd < 64; dd <<= 1) {
+                        const int y = __shfl_up(incl, dd, 64);
+                        if (lane >= dd) incl += y;
+                    }
+                    const int total = __shfl(incl, 63, 64);
+                    if (total == 0) continue;
+                    const LoPar q = c > 0 ? lo_par<F64>(par, orow + j) : LoPar{1.0, 0.0, 1.0};
+                    // owner's item layout: red offset, red count, blue offset, tile base in s_w
+                    // (an empty range packs as offset 0: its bounds may lie outside the tile)
+                    const int pk = (nred ? ra - iw0 : 0) | (nred << 6) | ((nblue ? ba - iw0 : 0) << 12) |
+                                   ((lt * kLoPts) << 18);
+                    const int excl = incl - c;
+                    for (int base = 0; base < total; base += 64) {
+                        const int item = base + lane;
+                        int lo = 0, hi = 63;
+    #pragma unroll
+                        for (int s = 0; s < 6; ++s) {
+                            const int mid = (lo + hi) >> 1;
+                            if (__shfl(incl, mid, 64) > item) hi = mid;
+                            else lo = mid + 1;
+                        }
+                        const int o = min(lo, 63);
+                        const int ex = __shfl(excl, o, 64), opk = __shfl(pk, o, 64);
+                        const LoPar qo = lo_shfl(q, o);
+                        if (item < total) {
+                            const int kk = item - ex, onred = (opk >> 6) & 63;
+                            const bool blue = kk >= onred;
+                            const int pt = blue ? ((opk >> 12) & 63) + (kk - onred) : (opk & 63) + kk;
+                            const double w = s_w[(opk >> 18) + pt];
+                            const int to = t - lane + o;  // owner's lane in the block
+                            lo_accum<F64>(qo, blue ? wl - w : w - wl, pt * kLoBlock + to, acc_d, acc_s,
+                                          acc_c, h0, h1, h2, f0, f1, f2);
+                        }
                     }
                 }
             }
@@ -461,12 +565,12 @@ __global__ __launch_bounds__(kLoBlock) void k_lo_gather(
 struct LoShape {
     int pts, block, batch;
     bool flat;
-    void (*mixed)(int, int, int, const int *, const int *, const int *, const double *,
+    void (*mixed)(int, int, int, const int *, const int *, const int *, const int *,
                   const double *, const double *, const double *, const double *, const double *,
-                  const float *, const float *, const float *, double *);
-    void (*fp64)(int, int, int, const int *, const int *, const int *, const double *,
+                  const double *, const float *, const float *, const float *, double *);
+    void (*fp64)(int, int, int, const int *, const int *, const int *, const int *,
                  const double *, const double *, const double *, const double *, const double *,
-                 const float *, const float *, const float *, double *);
+                 const double *, const float *, const float *, const float *, double *);
 };
 #define A12_LO_SHAPE(P, B, T, F) \
     {P, B, T, F, k_lo_gather<false, P, B, T, F>, k_lo_gather<true, P, B, T, F>}
@@ -639,7 +743,7 @@ static int lo_gather_launch(a12_ctx *ctx, int prec, int shape)
         return A12_ERR_ARG;
     }
     hipLaunchKernelGGL(g, dim3((tiles + per - 1) / per), dim3(sh.block), 0, ctx->stream, n, numnu,
-                       tiles, A.seg.as<int>(), A.l_nu.as<int>(), A.state.as<int>(),
+                       tiles, A.seg.as<int>(), A.l_nu.as<int>(), A.reach.as<int>(), A.state.as<int>(),
                        A.par.as<double>(), A.l_const.as<double>(), A.waveset.as<double>(), tabs,
                        tabs + kTabN, tabs + 2 * kTabN, tabs32, tabs32 + kTabN, tabs32 + 2 * kTabN,
                        A.xlines.as<double>());
@@ -663,10 +767,10 @@ int a12i_atlas_linop1_run(a12_ctx *ctx, int prec)
     const size_t nl1 = std::max(nl, 1);
     A12_HIP(A.state.reserve(sizeof(int) * nl1 * n));
     A12_HIP(A.par.reserve(sizeof(double) * 3 * nl1 * n));
-    A12_HIP(A.reach.reserve(sizeof(int) * (size_t)(numnu + 1) * n));
+    A12_HIP(A.reach.reserve(sizeof(int) * nl1));
     A12_HIP(A.xlines.reserve(sizeof(double) * (size_t)numnu * n));
     const int nb = (int)((nl1 * n + 255) / 256);  // one lane per (line, depth)
-    A12_HIP(A.blk.reserve(sizeof(long long) * nb));
+    A12_HIP(A.blk.reserve(sizeof(long long) * (nb + kSumParts)));
     A12_HIP(A.evals.reserve(sizeof(long long)));
     const double *tabs = A.tabs.as<double>();
     const float *tabs32 = A.tabs32.as<float>();
@@ -679,13 +783,23 @@ int a12i_atlas_linop1_run(a12_ctx *ctx, int prec)
                            tabs, tabs + kTabN, tabs + 2 * kTabN, A.state.as<int>(),
                            A.par.as<double>(), A.blk.as<long long>());
         A12_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_lo_sum, dim3(1), dim3(256), 0, s, A.blk.as<long long>(), nb,
+        const int np = std::min(nb, kSumParts);
+        long long *part = A.blk.as<long long>() + nb;
+        hipLaunchKernelGGL(k_lo_sum, dim3(np), dim3(256), 0, s, A.blk.as<long long>(), nb, part,
+                           nullptr);
+        A12_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_lo_sum, dim3(1), dim3(256), 0, s, part, np, nullptr,
                            A.evals.as<long long>());
         A12_CHECK_LAUNCH();
     } else {
         A12_HIP(hipMemsetAsync(A.evals.p, 0, sizeof(long long), s));
     }
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[9], s));
+    if (nl > 0) {
+        hipLaunchKernelGGL(k_lo_linereach, dim3(std::min((nl + 3) / 4, 65536)), dim3(256), 0, s,
+                           nl, n, A.state.as<int>(), A.reach.as<int>());
+        A12_CHECK_LAUNCH();
+    }
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[10], s));
     {
         const int rc = lo_gather_launch(ctx, prec, A.shape);
