@@ -577,10 +577,10 @@ struct LoShape {
 const LoShape kLoShapes[] = {
     A12_LO_SHAPE(8, 512, 32, false),  A12_LO_SHAPE(4, 512, 64, false),
     A12_LO_SHAPE(4, 256, 32, false),  A12_LO_SHAPE(8, 256, 32, false),
-    A12_LO_SHAPE(16, 256, 32, false), A12_LO_SHAPE(8, 512, 32, true),
-    A12_LO_SHAPE(8, 256, 32, true),   A12_LO_SHAPE(16, 256, 32, true),
-    A12_LO_SHAPE(16, 512, 32, true),  A12_LO_SHAPE(32, 256, 32, true),
-    A12_LO_SHAPE(16, 256, 64, true),  A12_LO_SHAPE(4, 512, 32, true),
+    A12_LO_SHAPE(4, 256, 8, false),   A12_LO_SHAPE(8, 512, 32, true),
+    A12_LO_SHAPE(2, 256, 32, false),  A12_LO_SHAPE(4, 256, 16, false),
+    A12_LO_SHAPE(2, 256, 16, false),  A12_LO_SHAPE(8, 256, 16, false),
+    A12_LO_SHAPE(4, 384, 32, false),  A12_LO_SHAPE(4, 448, 16, false),
 };
 #undef A12_LO_SHAPE
 constexpr int kNumLoShapes = sizeof(kLoShapes) / sizeof(kLoShapes[0]);
