@@ -9,7 +9,7 @@
 //                16 staged through LDS (any strides, so transposes are free);
 //                epilogues: store; + bias, keep Z, SiLU, BF16 copy (forward);
 //                x SiLU'(Z) and BF16 copy (backward)
-//   k_colsum     bias gradients, one thread per column in row order
+//   k_colsum     bias gradients: column sums by row slices, slices added in order
 //   k_loss       dY = 2 (Y - T) / (n 144) and per-block squared error
 //   k_adam       Adam on the FP32 master weights, BF16 copies refreshed
 // A training step (forward, loss, backward, Adam) is captured once in a
@@ -175,22 +175,41 @@ __global__ __launch_bounds__(256) void k_loss(int n, int rows, const float *__re
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-__global__ void k_loss_sum(int nb, int n, const float *__restrict__ part, float *__restrict__ loss)
+// loss = (sum of the nb block partials, in a fixed order) / (n 144): each
+// thread sums a strided share, then a tree in LDS
+__global__ __launch_bounds__(256) void k_loss_sum(int nb, int n, const float *__restrict__ part,
+                                                  float *__restrict__ loss)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ float red[256];
     float s = 0.0f;
-    for (int b = 0; b < nb; ++b) s += part[b];
-    *loss = s / ((float)n * (float)kOut);
+    for (int b = threadIdx.x; b < nb; b += 256) s += part[b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *loss = red[0] / ((float)n * (float)kOut);
 }
 
-// out[c] = sum over rows r < M of g[r ldg + c], in row order
-__global__ void k_colsum(int M, int N, const float *__restrict__ g, int ldg, float *__restrict__ out)
+// Column sums of g (M x N, row stride ldg) over row slice blockIdx.y of
+// `chunk` rows: block = 64 columns x 4 row groups, each group summing every
+// 4th row of the slice, the groups combined in order; part[y N + c].
+// k_sum_split then adds the slices in slice order: a fixed order throughout.
+__global__ __launch_bounds__(256) void k_colsum(int M, int N, const float *__restrict__ g,
+                                                int ldg, int chunk, float *__restrict__ part)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= N) return;
+    __shared__ float red[4][64];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int c = blockIdx.x * 64 + tx;
+    const int r0 = blockIdx.y * chunk, r1 = min(M, r0 + chunk);
     float s = 0.0f;
-    for (int r = 0; r < M; ++r) s += g[(long)r * ldg + c];
-    out[c] = s;
+    if (c < N)
+        for (int r = r0 + ty; r < r1; r += 4) s += g[(long)r * ldg + c];
+    red[ty][tx] = s;
+    __syncthreads();
+    if (ty == 0 && c < N)
+        part[(long)blockIdx.y * N + c] = ((red[0][tx] + red[1][tx]) + red[2][tx]) + red[3][tx];
 }
 
 __global__ void k_step(int *__restrict__ t)
@@ -328,7 +347,7 @@ static void mlp_backward(a12_ctx *ctx, MlpDev &D, bool adam)
     const int nb = (rows * kOut + 255) / 256;
     hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, s, D.n, rows, D.y.as<float>(),
                        D.tgt.as<float>(), D.dz.as<float>(), D.dzb.as<bf16>(), D.part.as<float>());
-    hipLaunchKernelGGL(k_loss_sum, dim3(1), dim3(64), 0, s, nb, D.n, D.part.as<float>(),
+    hipLaunchKernelGGL(k_loss_sum, dim3(1), dim3(256), 0, s, nb, D.n, D.part.as<float>(),
                        D.loss.as<float>());
     const bf16 *wb = D.wb.as<bf16>();
     float *g = D.g.as<float>();
@@ -340,8 +359,12 @@ static void mlp_backward(a12_ctx *ctx, MlpDev &D, bool adam)
         // dW_l = Hin^T dZ_l  (K x N, summed over the rows, K split over the rows)
         gemm_splitk(s, K, N, rows, hin, 1, K, dzb, N, 1, g + w_off(l), D.split.as<float>(),
                     split_of(rows));
-        hipLaunchKernelGGL(k_colsum, dim3((N + 255) / 256), dim3(256), 0, s, rows, N, dz, N,
-                           g + b_off(l));
+        // bias gradient: column sums of dZ_l by row slices (reusing the split buffer)
+        const int sp = split_of(rows), chunk = (rows + sp - 1) / sp;
+        hipLaunchKernelGGL(k_colsum, dim3((N + 63) / 64, sp), dim3(256), 0, s, rows, N, dz, N,
+                           chunk, D.split.as<float>());
+        hipLaunchKernelGGL(k_sum_split, dim3((N + 255) / 256), dim3(256), 0, s, (long)N, sp,
+                           D.split.as<float>(), g + b_off(l));
         if (l > 0) {
             // dZ_{l-1} = (dZ_l W_l^T) * SiLU'(Z_{l-1})
             gemm(s, EPI_BWD, rows, K, N, dzb, N, 1, wb + w_off(l), 1, N, dz2, K, dz2b, nullptr,

@@ -58,6 +58,22 @@ MODULE a12gpu_iface
     REAL(C_DOUBLE)     :: teff
   END TYPE a12_atlas_atm
 
+  ! What TCORR(3) sections A-F read (src/atlas12_modules.f90:2528-2752);
+  ! flxrad, rjmins, rdiagj C_NULL_PTR: the device sums of the last RT pass
+  TYPE, BIND(C) :: a12_tcorr_in
+    INTEGER(C_INT32_T) :: nrhox, ifconv, iter
+    REAL(C_DOUBLE)     :: flux, teff, mixlth
+    TYPE(C_PTR)        :: tauros, rhox, t, dltdlp, abross, flxcnv, flxcnv0, grdadb, ptotal
+    TYPE(C_PTR)        :: rho, dlrdlt, heatcp, hscale
+    TYPE(C_PTR)        :: flxrad, rjmins, rdiagj
+  END TYPE a12_tcorr_in
+
+  ! READMOL's molecule table for NMOLEC's Newton solve (1-based arrays)
+  TYPE, BIND(C) :: a12_moltab
+    INTEGER(C_INT32_T) :: nequa, nummol
+    TYPE(C_PTR)        :: locj, kcomps, idequa
+  END TYPE a12_moltab
+
   INTERFACE
     INTEGER(C_INT32_T) FUNCTION a12_ctx_create(device, ctx) BIND(C, NAME='a12_ctx_create')
       IMPORT :: C_INT32_T, C_PTR
@@ -194,6 +210,74 @@ MODULE a12gpu_iface
       IMPORT :: C_INT32_T, C_PTR, C_DOUBLE
       TYPE(C_PTR), VALUE :: ctx
       REAL(C_DOUBLE), INTENT(OUT) :: acc(*), hnu1(*)
+    END FUNCTION
+
+    ! ---- TCORR(3), SCALE_MODEL, NMOLEC -----------------------------------
+    INTEGER(C_INT32_T) FUNCTION a12_atlas_tcorr3(ctx, tin, rdabh, oldt1, out) &
+        BIND(C, NAME='a12_atlas_tcorr3')
+      IMPORT :: C_INT32_T, C_PTR, C_DOUBLE, a12_tcorr_in
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(a12_tcorr_in), INTENT(IN) :: tin
+      REAL(C_DOUBLE), INTENT(INOUT) :: rdabh(*), oldt1(*)
+      REAL(C_DOUBLE), INTENT(OUT) :: out(*)
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_scale_model(ctx, nmodel, nrhox, kw, tau1lg, steplg, sc, &
+        fields, out, so, taustd) BIND(C, NAME='a12_scale_model')
+      IMPORT :: C_INT32_T, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT32_T), VALUE :: nmodel, nrhox, kw
+      REAL(C_DOUBLE), VALUE :: tau1lg, steplg
+      REAL(C_DOUBLE), INTENT(IN) :: sc(*), fields(*)
+      REAL(C_DOUBLE), INTENT(OUT) :: out(*), so(*), taustd(*)
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_molec_set_table(ctx, tab) BIND(C, NAME='a12_molec_set_table')
+      IMPORT :: C_INT32_T, C_PTR, a12_moltab
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(a12_moltab), INTENT(IN) :: tab
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_molec_newton_batched(ctx, nsys, nchain, chain_off, xntot, &
+        pratio, xab, equilj, xn, xnmol, iters, status) BIND(C, NAME='a12_molec_newton_batched')
+      IMPORT :: C_INT32_T, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT32_T), VALUE :: nsys, nchain
+      INTEGER(C_INT32_T), INTENT(IN) :: chain_off(*)
+      REAL(C_DOUBLE), INTENT(IN) :: xntot(*), pratio(*), xab(*), equilj(*)
+      REAL(C_DOUBLE), INTENT(INOUT) :: xn(*)
+      REAL(C_DOUBLE), INTENT(OUT) :: xnmol(*)
+      INTEGER(C_INT32_T), INTENT(OUT) :: iters(*), status(*)
+    END FUNCTION
+
+    ! ---- batched SYNTHE and gather shapes ---------------------------------
+    INTEGER(C_INT32_T) FUNCTION a12_synthe_set_shape(ctx, shape) &
+        BIND(C, NAME='a12_synthe_set_shape')
+      IMPORT :: C_INT32_T, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT32_T), VALUE :: shape
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_synthe_batch_load(ctx, nmodel, atms, nb_shift, line_lo, &
+        line_hi, lines, nlines) BIND(C, NAME='a12_synthe_batch_load')
+      IMPORT :: C_INT32_T, C_INT64_T, C_PTR, a12_synthe_atm, a12_lte_line
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT32_T), VALUE :: nmodel
+      TYPE(a12_synthe_atm), INTENT(IN) :: atms(*)
+      INTEGER(C_INT32_T), INTENT(IN) :: nb_shift(*)
+      INTEGER(C_INT64_T), INTENT(IN) :: line_lo(*), line_hi(*)
+      TYPE(a12_lte_line), INTENT(IN) :: lines(*)
+      INTEGER(C_INT64_T), VALUE :: nlines
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_synthe_batch_run(ctx, prec, transfer, use_graph) &
+        BIND(C, NAME='a12_synthe_batch_run')
+      IMPORT :: C_INT32_T, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT32_T), VALUE :: prec, transfer, use_graph
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_synthe_batch_fetch(ctx, model, opacity_out, hnu_line, &
+        hnu_cont, evals) BIND(C, NAME='a12_synthe_batch_fetch')
+      IMPORT :: C_INT32_T, C_INT64_T, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx, opacity_out
+      INTEGER(C_INT32_T), VALUE :: model
+      REAL(C_DOUBLE), INTENT(OUT) :: hnu_line(*), hnu_cont(*)
+      INTEGER(C_INT64_T), INTENT(OUT) :: evals(2)
     END FUNCTION
   END INTERFACE
 

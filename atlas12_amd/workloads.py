@@ -471,3 +471,92 @@ def synthe_batch_workload(base, nmodel: int = 256, nlines: int = 10_000_000, see
         lo.append(int(np.searchsorted(wl, m.wl0 - margin)))
         hi.append(int(np.searchsorted(wl, m.wl1 + margin, side="right")))
     return grid, models, states, lines, np.array(shift), np.array(lo), np.array(hi)
+
+
+# ---------------------------------------------------------------------------
+# Large SYNTHE run: 50M lines, molecular band progressions, cool dwarf
+# ---------------------------------------------------------------------------
+
+# SYNTHE species slots of the diatomics in the xnfpelsyn molecule table
+# (nelion = 6 (39 + position), src/mod_mklinelist.f90:1706, :2016-2025,
+# :2467-2489): H2, CH, NH, C2, CN, CO; and their masses [amu].
+LARGE_MOLECULES = ((240, 2.016), (246, 13.019), (252, 15.015), (264, 24.022), (270, 26.018),
+                   (276, 28.010))
+
+
+def cool_dwarf_state(base, grid: Grid, nrhox: int = 72, t_top: float = 2800.0):
+    """The plain-SYNTHE state turned into a cool dwarf (T rising from t_top
+    to t_top * 9500/4400, log g 5) on the given window, with the diatomics
+    of LARGE_MOLECULES populated: number densities a fixed fraction of
+    hydrogen falling with temperature, Doppler widths from their masses and
+    the atmosphere's turbulence.  A workload of the right shape, not a
+    converged model."""
+    teff = 5777.0 * t_top / 4400.0
+    m = BatchModel(nrhox, teff, 5.0, 0.0, grid.wlbeg, grid.wlend, 0, grid.length)
+    st = batch_model_state(base, m, grid, nedge=16)
+    k_b, amu, c = 1.380649e-16, 1.66053907e-24, 2.99792458e10
+    vt2 = (1.5e5) ** 2
+    for q, (slot, mass) in enumerate(LARGE_MOLECULES):
+        frac = 10.0 ** (-3.0 - 0.5 * q) * (4000.0 / st.t) ** 6
+        st.xnfpel[:, slot - 1] = frac * st.xnfh
+        st.dopple[:, slot - 1] = np.sqrt(2.0 * k_b * st.t / (mass * amu) + vt2) / c
+    return st
+
+
+def large_synthe_lines(grid: Grid, n: int = 50_000_000, seed: int = 7,
+                       atomic_share: float = 0.3, band_len: int = 200) -> np.ndarray:
+    """n lte_line_t records over the grid, sorted by nbuff: atomic_share
+    atomic lines (SYNTHE_SPECIES, log gf uniform in [-5, 0], lower level
+    uniform in 0-8 eV) and the rest in molecular band progressions of
+    band_len lines each -- a band origin uniform in ln(lambda), lines every
+    2e-5 in lambda (10 grid points at R = 500,000), log gf ~ N(-5, 1), lower
+    level B J (J + 1) with B = 1.5 cm^-1 over a vibrational term uniform in
+    0-1 eV -- on the diatomics of LARGE_MOLECULES.  Built as read_gfall
+    forms records (nbuff, cgf = 0.026538/1.77245 gf / nu, damping over
+    12.5664 nu, src/mod_mklinelist.f90:898-1005), vectorised."""
+    rng = np.random.default_rng(seed)
+    na = int(n * atomic_share)
+    nb = (n - na) // band_len
+    nm = nb * band_len
+    lw0, lw1 = math.log(grid.wlbeg), math.log(grid.wlend)
+    wl = np.empty(na + nm)
+    wl[:na] = np.exp(rng.uniform(lw0, lw1, na))
+    origin = np.exp(rng.uniform(lw0, lw1 - band_len * 2.0e-5, nb))
+    jj = np.arange(band_len, dtype=np.float64)
+    wl[na:] = (origin[:, None] * (1.0 + 2.0e-5 * jj[None, :])).ravel()
+    loggf = np.empty(na + nm, np.float32)
+    loggf[:na] = rng.uniform(-5.0, 0.0, na)
+    loggf[na:] = rng.normal(-5.0, 1.0, nm)
+    e_cm = np.empty(na + nm)
+    e_cm[:na] = rng.uniform(0.0, 8.0, na) * EV_TO_CM
+    evib = rng.uniform(0.0, 1.0, nb) * EV_TO_CM
+    e_cm[na:] = (evib[:, None] + 1.5 * jj[None, :] * (jj[None, :] + 1.0)).ravel()
+    nel = np.empty(na + nm, np.int32)
+    sp = np.asarray(SYNTHE_SPECIES)[rng.integers(0, len(SYNTHE_SPECIES), na)]
+    nelem = np.floor(sp).astype(np.int32)
+    nel[:na] = nelem * 6 - 6 + np.rint((sp - nelem) * 100.0).astype(np.int32) + 1
+    slots = np.array([s for s, _ in LARGE_MOLECULES], np.int32)
+    nel[na:] = np.repeat(slots[rng.integers(0, len(slots), nb)], band_len)
+    code = np.empty(na + nm, np.float32)
+    code[:na] = sp
+    code[na:] = 100.0 + (nel[na:] // 6 - 39)  # a molecule tag; not read on the hot path
+    lg = rng.uniform(7.0, 9.0, na + nm), rng.uniform(-6.0, -4.0, na + nm), \
+        rng.uniform(-8.0, -7.0, na + nm)
+    keep = (wl >= grid.wlbeg) & (wl <= grid.wlend)
+    idx = np.flatnonzero(keep)
+    ixwl = np.floor(np.log(wl[idx]) / grid.ratiolg + 0.5).astype(np.int64)
+    nbuff = (ixwl - grid.ixwlbeg + 1).astype(np.int32)
+    order = np.argsort(nbuff, kind="stable")
+    idx, nbuff = idx[order], nbuff[order]
+    out = np.zeros(len(idx), dtype=LTE_DTYPE)
+    out["nbuff"] = nbuff
+    frelin = 2.99792458e17 / wl[idx]
+    gf = np.power(np.float32(10.0), loggf[idx]).astype(np.float64)
+    out["cgf"] = (0.026538 / 1.77245 * gf / frelin).astype(np.float32)
+    out["nelion"] = nel[idx]
+    out["elo"] = e_cm[idx].astype(np.float32)
+    frq4pi = 12.5664 * frelin
+    for g, dst in zip(lg, ("gamrf", "gamsf", "gamwf")):
+        out[dst] = (np.power(10.0, g[idx]) / frq4pi).astype(np.float32)
+    out["code"] = code[idx]
+    return out

@@ -62,6 +62,8 @@ def parse():
                     help="models of the 256-model batched-SYNTHE workload that are run "
                          "(the first ones drawn; 256 = all)")
     ap.add_argument("--batch-lines", type=int, default=10_000_000)
+    ap.add_argument("--no-large", action="store_true", help="skip the 50M-line SYNTHE block")
+    ap.add_argument("--large-lines", type=int, default=50_000_000)
     ap.add_argument("--profile-only", action="store_true",
                     help="warmup+steps iterations of both workloads at both precisions and a "
                          "device copy of --calib-mib MiB (for rocprofv3 runs); no JSON")
@@ -166,14 +168,12 @@ def atlas_run(args, ctx, native, z, d, pred, tio):
 
 
 # ----------------------------------------------------------- plain SYNTHE
-def synthe_run(args, ctx, native, wl):
-    from atlas12_amd.state import SyntheState
-    st = SyntheState.load(os.path.join(GOLDEN, "synthe_plain_state.npz"))
-    grid = wl.Grid.make(400.0, 700.0, 500000.0)
-    assert grid.length == st.length
-    lines = wl.synthe_plain_lines(grid, 200_000, seed=1)
-    ctx.synthe_load(st, lines)
+def synthe_block(args, ctx, native, st, lines, steps, warmup, shape=None):
+    """One SYNTHE model: tile shape (autotuned unless given), timed mixed
+    and FP64 opacity builds with HIP-event stage times, the transfer of
+    both, the mixed-vs-FP64 emergent-flux error."""
     import torch
+    ctx.synthe_load(st, lines)
 
     def run(prec, k):
         stages = np.zeros(5)
@@ -185,29 +185,29 @@ def synthe_run(args, ctx, native, wl):
         torch.cuda.synchronize()
         return time.perf_counter() - t, stages / max(k, 1)
 
-    run(native.PREC_MIXED, args.warmup)
-    tune = {}
-    for shape, (pts, chunk) in enumerate(native.synthe_shapes()):
-        ctx.synthe_set_shape(shape)
-        run(native.PREC_MIXED, 1)
-        tune[shape] = float(np.median([run(native.PREC_MIXED, 1)[1][3] for _ in range(3)]))
-    best = min(tune, key=tune.get)
-    ctx.synthe_set_shape(best)
     shapes = native.synthe_shapes()
+    tune = {}
+    if shape is None:
+        run(native.PREC_MIXED, warmup)
+        for k in range(len(shapes)):
+            ctx.synthe_set_shape(k)
+            run(native.PREC_MIXED, 1)
+            tune[k] = float(np.median([run(native.PREC_MIXED, 1)[1][3] for _ in range(3)]))
+        shape = min(tune, key=tune.get)
+    ctx.synthe_set_shape(shape)
+    run(native.PREC_MIXED, warmup)
     if args.profile_only:
-        run(native.PREC_MIXED, args.steps)
+        run(native.PREC_MIXED, steps)
         run(native.PREC_FP64, 1)
         ctx.synthe_transfer_run()
         ctx.synthe_transfer_fetch()
         return None
-    log("synthe: timed mixed")
-    t_mix, st_mix = run(native.PREC_MIXED, args.steps)
+    t_mix, st_mix = run(native.PREC_MIXED, steps)
     evals, core = ctx.eval_split()
     ctx.synthe_transfer_run()
     hl_mix, _ = ctx.synthe_transfer_fetch()
     t_tr = float(ctx.stage_ms()[4])
-    log("synthe: timed fp64")
-    k64 = max(1, args.steps // 2)
+    k64 = max(1, steps // 2)
     run(native.PREC_FP64, 1)
     t_64, st_64 = run(native.PREC_FP64, k64)
     evals64, _ = ctx.eval_split()
@@ -217,36 +217,83 @@ def synthe_run(args, ctx, native, wl):
     wing = evals - core
     flops = FLOP_SYN_CORE * core + FLOP_SYN_WING * wing
     tile_ms = float(st_mix[3])
-    value = evals * args.steps / t_mix
+    value = evals * steps / t_mix
     value64 = evals64 * k64 / t_64
     res = {
-        "workload": "synthe_plain: 72 layers, 200,000 atomic lines, 400-700 nm at R=500,000 "
-                    "(279,808 points)",
-        "value": value, "unit": "evals/s", "ms_per_step": 1e3 * t_mix / args.steps,
+        "value": value, "unit": "evals/s", "ms_per_step": 1e3 * t_mix / steps,
         "evals_per_step": int(evals), "core_evals": int(core), "wing_evals": int(wing),
         "fp64_gpu": {"value": value64, "unit": "evals/s", "ms_per_step": 1e3 * t_64 / k64,
                      "evals_per_step": int(evals64)},
         "mixed_over_fp64": value / value64,
         "flux_max_rel_err_mixed_vs_fp64": flux_err,
-        "autotune": {"shape_pts_chunk": list(shapes[best]),
-                     "gather_ms_per_shape": {str(shapes[k]): v for k, v in tune.items()}},
+        "tile_shape_pts_chunk": list(shapes[shape]),
         "transfer": {"ms": t_tr, "points_per_s": st.length / (t_tr * 1e-3) if t_tr > 0 else None},
         "stage_ms_mixed": {"prep_continuum": st_mix[0], "line_setup": st_mix[1],
                            "partition": st_mix[2], "tile_gather": st_mix[3]},
         "stage_ms_fp64": {"prep_continuum": st_64[0], "line_setup": st_64[1],
                           "partition": st_64[2], "tile_gather": st_64[3]},
         "roofline": {"bound": "valu",
-                     "kernel": f"k_tile<false, {shapes[best][0]}, {shapes[best][1]}>",
+                     "kernel": f"k_tile<false, {shapes[shape][0]}, {shapes[shape][1]}>",
                      "achieved": flops / (tile_ms * 1e-3) / 1e12, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": flops / (tile_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,
                      "flops_per_launch": flops, "avg_launch_ms": tile_ms},
     }
+    if tune:
+        res["autotune"] = {"shape_pts_chunk": list(shapes[shape]),
+                           "gather_ms_per_shape": {str(shapes[k]): v for k, v in tune.items()}}
     if t_tr > 0:
         tf = 2 * 103.0e3 * st.length / (t_tr * 1e-3) / 1e12
         res["roofline_transfer"] = {"bound": "valu", "kernel": "k_synthe_transfer",
                                     "achieved": tf, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                                     "frac": tf / PEAK_FP64_TFLOPS}
-    return res, st, lines
+    res["_shape"] = shape
+    return res
+
+
+def synthe_run(args, ctx, native, wl):
+    from atlas12_amd.state import SyntheState
+    st = SyntheState.load(os.path.join(GOLDEN, "synthe_plain_state.npz"))
+    grid = wl.Grid.make(400.0, 700.0, 500000.0)
+    assert grid.length == st.length
+    lines = wl.synthe_plain_lines(grid, 200_000, seed=1)
+    log("synthe: plain workload")
+    res = synthe_block(args, ctx, native, st, lines, args.steps, args.warmup)
+    if res is None:
+        return None
+    out = {"workload": "synthe_plain: 72 layers, 200,000 atomic lines, 400-700 nm at "
+                       "R=500,000 (279,808 points)"}
+    out.update(res)
+    return out, st, lines
+
+
+def large_run(args, ctx, native, wl, shape):
+    """The large SYNTHE run: a cool-dwarf structure (T 2800-6045 K, 72
+    layers), 300-1000 nm at R=500,000, --large-lines lines (30 % atomic, 70 %
+    in molecular band progressions).  Gather shape: the plain-SYNTHE
+    autotuner's.  HBM rates: bytes of the per-(line, depth) parameter planes
+    the set-up writes and the opacity matrix the gather writes, over the
+    stage times (algorithmic lower bounds, not counter readings)."""
+    from atlas12_amd.state import SyntheState
+    base = SyntheState.load(os.path.join(GOLDEN, "synthe_plain_state.npz"))
+    grid = wl.Grid.make(300.0, 1000.0, 500000.0)
+    st = wl.cool_dwarf_state(base, grid)
+    t0 = time.perf_counter()
+    lines = wl.large_synthe_lines(grid, args.large_lines)
+    log(f"large: {len(lines):,} lines generated in {time.perf_counter() - t0:.1f}s")
+    res = synthe_block(args, ctx, native, st, lines, 1, 1, shape)
+    n, L, N = st.nrhox, st.length, len(lines)
+    setup_bytes = N * n * (4 + 4 + 4 * 4)
+    opac_bytes = L * n * 4
+    sm = res["stage_ms_mixed"]
+    out = {"workload": f"synthe_large: cool dwarf (T 2800-6045 K), 72 layers, {N:,} lines "
+                       "(30 % atomic, 70 % molecular band progressions), 300-1000 nm at "
+                       f"R=500,000 ({L:,} points)"}
+    out.update(res)
+    out["hbm"] = {"setup_planes_gb_per_s": setup_bytes / (sm["line_setup"] * 1e-3) / 1e9,
+                  "opacity_write_gb_per_s": opac_bytes / (sm["tile_gather"] * 1e-3) / 1e9,
+                  "setup_plane_bytes": setup_bytes, "opacity_bytes": opac_bytes,
+                  "note": "algorithmic bytes over HIP-event stage times"}
+    return out
 
 
 # ------------------------------------------------------- batched SYNTHE
@@ -420,6 +467,9 @@ def main():
     syn = None
     if not args.no_synthe:
         syn = synthe_run(args, ctx, native, wl)
+    big = None
+    if not (args.profile_only or args.no_large or syn is None):
+        big = large_run(args, ctx, native, wl, syn[0]["_shape"])
     net = None if args.profile_only else mlp_run(args, ctx, native)
     bat = None if (args.profile_only or args.no_batch) else batch_run(args, ctx, native, wl)
     if args.profile_only:
@@ -494,8 +544,13 @@ def main():
     if bat is not None:
         result["synthe_batch"] = bat
     if syn is not None:
+        syn[0].pop("_shape", None)
         result["synthe_plain"] = syn[0]
-        result["synthe_plain"]["roofline"]["traffic"] = traffic.get("k_tile<false>")
+    if big is not None:
+        big.pop("_shape", None)
+        result["synthe_large"] = big
+        result["synthe_plain"]["roofline"]["traffic"] = traffic.get(
+            result["synthe_plain"]["roofline"]["kernel"])
     if not args.no_cpu:
         log("cpu baseline: atlas LINOP1")
         result["cpu_baseline"] = cpu_atlas(z, a["store_lines"])

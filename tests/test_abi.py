@@ -100,8 +100,10 @@ PROGRAM t
   TYPE(a12_synthe_atm) :: sa
   TYPE(a12_atlas_atm) :: aa
   TYPE(a12_mol2_rec) :: m
+  TYPE(a12_tcorr_in) :: tc
+  TYPE(a12_moltab) :: mt
   INTEGER(C_INT32_T) :: rc
-  PRINT *, C_SIZEOF(l(1)), C_SIZEOF(sa), C_SIZEOF(aa), C_SIZEOF(m)
+  PRINT *, C_SIZEOF(l(1)), C_SIZEOF(sa), C_SIZEOF(aa), C_SIZEOF(m), C_SIZEOF(tc), C_SIZEOF(mt)
   rc = a12_synthe_opacity_run(C_NULL_PTR, A12_PREC_MIXED)
   PRINT *, rc
   rc = a12_atlas_linop1_run(C_NULL_PTR, A12_PREC_FP64)
@@ -114,6 +116,7 @@ END PROGRAM t
     subprocess.run([fc, *mod, iface, str(prog), "-o", str(exe), lib._name,
                     "-Wl,-rpath," + os.path.dirname(lib._name)], check=True, cwd=tmp_path)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
-    sizes = [int(x) for x in out[:4]]
-    assert sizes == [32, ctypes.sizeof(native.SyntheAtm), ctypes.sizeof(native.AtlasAtm), 32]
-    assert [int(x) for x in out[4:6]] == [-1, -1]
+    sizes = [int(x) for x in out[:6]]
+    assert sizes == [32, ctypes.sizeof(native.SyntheAtm), ctypes.sizeof(native.AtlasAtm), 32,
+                     ctypes.sizeof(native.TcorrIn), ctypes.sizeof(native.MolTab)]
+    assert [int(x) for x in out[6:8]] == [-1, -1]
