@@ -65,7 +65,8 @@ def parse():
     ap.add_argument("--no-large", action="store_true", help="skip the 50M-line SYNTHE block")
     ap.add_argument("--large-lines", type=int, default=50_000_000)
     ap.add_argument("--profile-only", action="store_true",
-                    help="warmup+steps iterations of both workloads at both precisions and a "
+                    help="warmup+steps iterations of the ATLAS12 and plain-SYNTHE workloads at "
+                         "both precisions, 25 network training steps, a 4-model batch, and a "
                          "device copy of --calib-mib MiB (for rocprofv3 runs); no JSON")
     ap.add_argument("--calib-mib", type=int, default=256,
                     help="size of the plain device-to-device copy used to scale HBM counters")
@@ -375,6 +376,9 @@ def mlp_run(args, ctx, native):
     native.mlp_init(ctx, nb, 1)
     native.mlp_set_data(ctx, x, t)
     native.mlp_train(ctx, 20, 1e-3)  # warm-up, graph capture
+    if args.profile_only:
+        native.mlp_train(ctx, 5, 1e-3)
+        return None
     steps = 200
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -470,8 +474,10 @@ def main():
     big = None
     if not (args.profile_only or args.no_large or syn is None):
         big = large_run(args, ctx, native, wl, syn[0]["_shape"])
-    net = None if args.profile_only else mlp_run(args, ctx, native)
-    bat = None if (args.profile_only or args.no_batch) else batch_run(args, ctx, native, wl)
+    net = mlp_run(args, ctx, native)
+    if args.profile_only and not args.no_batch:
+        args.batch_models, args.batch_lines = 4, 1_000_000
+    bat = None if args.no_batch else batch_run(args, ctx, native, wl)
     if args.profile_only:
         nel = args.calib_mib * (1 << 20) // 4
         src = torch.ones(nel, dtype=torch.float32, device="cuda")
