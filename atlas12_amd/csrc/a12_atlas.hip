@@ -328,10 +328,9 @@ struct LoPar {
 };
 
 template <bool F64>
-__device__ __forceinline__ LoPar lo_par(const double *__restrict__ par, size_t o)
+__device__ __forceinline__ LoPar lo_par_of(double center, double adamp, double dopwave)
 {
     LoPar q;
-    const double center = par[3 * o], adamp = par[3 * o + 1], dopwave = par[3 * o + 2];
     if (F64) {
         q.p0 = center;
         q.p1 = adamp;
@@ -343,6 +342,12 @@ __device__ __forceinline__ LoPar lo_par(const double *__restrict__ par, size_t o
         q.p2 = __hiloint2double(__float_as_int(c32), __float_as_int(k32));
     }
     return q;
+}
+
+template <bool F64>
+__device__ __forceinline__ LoPar lo_par(const double *__restrict__ par, size_t o)
+{
+    return lo_par_of<F64>(par[3 * o], par[3 * o + 1], par[3 * o + 2]);
 }
 
 // One contribution at offset d (red: w - wl, blue: wl - w) into LDS slot q.
@@ -560,11 +565,121 @@ __global__ __launch_bounds__(kLoBlock) void k_lo_gather(
         }
 }
 
-// Tile shapes (points per tile, lanes per block, lines per LDS batch) the
-// autotuner chooses from; LDS = 8 B x points x lanes + 4 B x 128 x batch.
+// Barrier-free form of the gather: after the block's first barrier every
+// wavefront works alone.  A wave takes the candidate lines that can reach
+// its own points (its lanes' tiles), 64 at a time, keeps those whose widest
+// reach meets them (ballot) and walks them in store order (lowest bit
+// first): per line its NU and wavelength are wave-uniform loads, each lane
+// reads its depth's state and parameters and adds its items into its own
+// LDS slots.  The same items in the same per-slot order as k_lo_gather.
+template <bool F64, int kLoPts, int kLoBlock>
+__global__ __launch_bounds__(kLoBlock) void k_lo_gather_w(
+    int n, int numnu, int tiles, const int *__restrict__ seg, const int *__restrict__ l_nu,
+    const int *__restrict__ l_reach, const int *__restrict__ state, const double *__restrict__ par,
+    const double *__restrict__ l_const, const double *__restrict__ waveset,
+    const double *__restrict__ h0, const double *__restrict__ h1, const double *__restrict__ h2,
+    const float *__restrict__ f0, const float *__restrict__ f1, const float *__restrict__ f2,
+    double *__restrict__ xlines)
+{
+#pragma clang fp contract(off)
+    static_assert(kLoPts <= 32 && kLoBlock % 64 == 0, "tile shape");
+    __shared__ double acc_d[kLoPts * kLoBlock];
+    __shared__ double s_w[kLoBlock / 8 * kLoPts];
+    float *acc_s = reinterpret_cast<float *>(acc_d);
+    float *acc_c = acc_s + kLoPts * kLoBlock;
+    const int per = kLoBlock / n;
+    const int t = threadIdx.x, lane = t & 63;
+    const int lt = t / n, j = t - lt * n;
+    const int tile0 = blockIdx.x * per;
+    const int tile = tile0 + lt;
+    const bool live = lt < per && tile < tiles;
+    const int iw0 = tile * kLoPts, iw1 = min(iw0 + kLoPts, numnu) - 1;
+    for (int p = 0; p < kLoPts; ++p) {
+        if (F64) acc_d[p * kLoBlock + t] = 0.0;
+        else acc_s[p * kLoBlock + t] = acc_c[p * kLoBlock + t] = 0.0f;
+    }
+    if (t < per * kLoPts) s_w[t] = waveset[min(tile0 * kLoPts + t, numnu - 1)];
+    __syncthreads();
+    const int t0 = t - lane;  // this wave's first lane
+    const int wlt0 = t0 / n, wlt1 = min((t0 + 63) / n, per - 1);
+    if (wlt0 < per && tile0 + wlt0 < tiles) {  // uniform over the wave
+        const int wfirst = (tile0 + wlt0) * kLoPts;
+        const int wlast = min((tile0 + wlt1 + 1) * kLoPts, numnu) - 1;
+        const int lbeg = seg[max(0, wfirst - kMaxWing)];
+        const int lend = seg[min(numnu, wlast + kMaxWing) + 1];
+        const double *w_t = s_w + lt * kLoPts - iw0;
+        for (int c0 = lbeg; c0 < lend; c0 += 64) {
+            const int q = c0 + lane;
+            bool act = false;
+            if (q < lend) {
+                const int k = l_nu[q], rr = l_reach[q], rmax = rr & 0xff, bmax = (rr >> 8) & 0xff;
+                act = (rmax > 0 && k <= wlast && k + rmax - 1 >= wfirst) ||
+                      (bmax > 0 && k - 1 >= wfirst && k - bmax <= wlast);
+            }
+            unsigned long long m = __ballot(act);
+            if (!m) continue;
+            // one line ahead: the next line's state, parameters, NU and
+            // wavelength are loaded while the current one is added
+            int row = c0 + __builtin_ctzll(m);  // lowest first: store order
+            m &= m - 1;
+            int st = live ? state[(size_t)row * n + j] : 0;
+            double p0 = par[3 * ((size_t)row * n + j)], p1 = par[3 * ((size_t)row * n + j) + 1],
+                   p2 = par[3 * ((size_t)row * n + j) + 2];
+            int k = l_nu[row];
+            double wl = l_const[5 * (size_t)row];
+            for (;;) {
+                const bool more = m != 0;
+                int st_n = 0, k_n = 0, row_n = 0;
+                double q0 = 0.0, q1 = 0.0, q2 = 0.0, wl_n = 0.0;
+                if (more) {
+                    row_n = c0 + __builtin_ctzll(m);
+                    m &= m - 1;
+                    const size_t o = (size_t)row_n * n + j;
+                    st_n = live ? state[o] : 0;
+                    q0 = par[3 * o];
+                    q1 = par[3 * o + 1];
+                    q2 = par[3 * o + 2];
+                    k_n = l_nu[row_n];
+                    wl_n = l_const[5 * (size_t)row_n];
+                }
+                if (st < 0) {
+                    const int rn = st & 0xff, bn = (st >> 8) & 0xff;
+                    const int ra = max(k, iw0), rb = min(k + rn - 1, iw1);
+                    const int ba = max(k - bn, iw0), bb = min(k - 1, iw1);
+                    if (ra <= rb || ba <= bb) {
+                        const LoPar qp = lo_par_of<F64>(p0, p1, p2);
+                        for (int iw = ra; iw <= rb; ++iw)
+                            lo_accum<F64>(qp, w_t[iw] - wl, (iw - iw0) * kLoBlock + t, acc_d,
+                                          acc_s, acc_c, h0, h1, h2, f0, f1, f2);
+                        for (int iw = ba; iw <= bb; ++iw)
+                            lo_accum<F64>(qp, wl - w_t[iw], (iw - iw0) * kLoBlock + t, acc_d,
+                                          acc_s, acc_c, h0, h1, h2, f0, f1, f2);
+                    }
+                }
+                if (!more) break;
+                row = row_n;
+                st = st_n;
+                p0 = q0;
+                p1 = q1;
+                p2 = q2;
+                k = k_n;
+                wl = wl_n;
+            }
+        }
+    }
+    if (live)
+        for (int iw = iw0; iw <= iw1; ++iw) {
+            const int q = (iw - iw0) * kLoBlock + t;
+            xlines[(size_t)iw * n + j] = F64 ? acc_d[q] : (double)acc_s[q] + (double)acc_c[q];
+        }
+}
+
+// Tile shapes (points per tile, lanes per block, lines per LDS batch, form:
+// 0 staged, 1 flat, 2 barrier-free per wave) the autotuner chooses from;
+// LDS = 8 B x points x lanes (+ 4 B x 128 x batch when staged).
 struct LoShape {
     int pts, block, batch;
-    bool flat;
+    int flat;
     void (*mixed)(int, int, int, const int *, const int *, const int *, const int *,
                   const double *, const double *, const double *, const double *, const double *,
                   const double *, const float *, const float *, const float *, double *);
@@ -573,16 +688,19 @@ struct LoShape {
                  const double *, const float *, const float *, const float *, double *);
 };
 #define A12_LO_SHAPE(P, B, T, F) \
-    {P, B, T, F, k_lo_gather<false, P, B, T, F>, k_lo_gather<true, P, B, T, F>}
+    {P, B, T, F ? 1 : 0, k_lo_gather<false, P, B, T, F>, k_lo_gather<true, P, B, T, F>}
+#define A12_LO_WAVE(P, B) {P, B, 0, 2, k_lo_gather_w<false, P, B>, k_lo_gather_w<true, P, B>}
 const LoShape kLoShapes[] = {
     A12_LO_SHAPE(8, 512, 32, false),  A12_LO_SHAPE(4, 512, 64, false),
     A12_LO_SHAPE(4, 256, 32, false),  A12_LO_SHAPE(8, 256, 32, false),
     A12_LO_SHAPE(4, 256, 8, false),   A12_LO_SHAPE(8, 512, 32, true),
     A12_LO_SHAPE(2, 256, 32, false),  A12_LO_SHAPE(4, 256, 16, false),
     A12_LO_SHAPE(2, 256, 16, false),  A12_LO_SHAPE(8, 256, 16, false),
-    A12_LO_SHAPE(4, 384, 32, false),  A12_LO_SHAPE(4, 448, 16, false),
+    A12_LO_WAVE(4, 256),              A12_LO_WAVE(2, 256),
+    A12_LO_WAVE(8, 256),              A12_LO_WAVE(4, 512),
 };
 #undef A12_LO_SHAPE
+#undef A12_LO_WAVE
 constexpr int kNumLoShapes = sizeof(kLoShapes) / sizeof(kLoShapes[0]);
 
 template <class T>

@@ -126,7 +126,8 @@ def atlas_run(args, ctx, native, z, d, pred, tio):
     log(f"atlas: SELECTLINES kept {len(store)} lines ({t_sel:.2f}s incl. transfers)")
     shapes = native.atlas_shapes(ctx)
     best, tune_ms = native.atlas_autotune(ctx, native.PREC_MIXED)
-    log(f"atlas: autotuned gather shape {shapes[best]} (pts, lanes, batch, flat); ms per shape "
+    log(f"atlas: autotuned gather shape {shapes[best]} (pts, lanes, batch, form 0 staged / 1 flat "
+        "/ 2 per wave); ms per shape "
         + ", ".join(f"{s}:{m:.1f}" for s, m in zip(shapes, tune_ms)))
 
     import torch
@@ -491,7 +492,8 @@ def main():
 
     traffic = traffic_table()
     pts, lanes, batch, flat = a["shape"]
-    gather_name = f"k_lo_gather<false, {pts}, {lanes}, {batch}, {'true' if flat else 'false'}>"
+    gather_name = (f"k_lo_gather_w<false, {pts}, {lanes}>" if flat == 2 else
+                   f"k_lo_gather<false, {pts}, {lanes}, {batch}, {'true' if flat else 'false'}>")
     ev = a["ev_mix"]
     value = ev / (a["lo_mix"] * 1e-3)
     value64 = a["ev_64"] / (a["lo_64"] * 1e-3)
@@ -533,7 +535,7 @@ def main():
         "stage_ms_fp64": {"linop1_setup": a["st_64"][0], "linop1_reach": a["st_64"][1],
                           "linop1_gather": a["st_64"][2], "rt_pass": a["st_64"][3]},
         "selectlines_s": a["t_sel"],
-        "autotune": {"shape_pts_lanes_batch_flat": list(a["shape"]),
+        "autotune": {"shape_pts_lanes_batch_form": list(a["shape"]),
                      "gather_ms_per_shape": a["tune"]},
         "roofline": {"bound": "valu", "kernel": gather_name, "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
