@@ -89,6 +89,7 @@ __constant__ static const float kWeiA32[32] = {
 
 __device__ __forceinline__ float voigt_f32(float v, float a)
 {
+#pragma clang fp contract(off)
     const float x = fabsf(v), y = a, s = x + y;
     if (s > 15.0f) {
         const float dr = x * x - y * y - 0.5f;
@@ -131,6 +132,9 @@ __device__ __forceinline__ float voigt_f32(float v, float a)
 // Worst relative error against Re w(z) is below 2e-5 (see tests).
 __device__ __forceinline__ float voigt_mixed(float v, float a)
 {
+    // fused steps are written as fmaf; nothing else is contracted, so the
+    // value is the same in every kernel this is inlined into
+#pragma clang fp contract(off)
     const float x = fabsf(v), y = a;
     if (x + y > 15.0f) {
         const float x2 = x * x, y2 = y * y;
@@ -142,11 +146,11 @@ __device__ __forceinline__ float voigt_mixed(float v, float a)
         float tr = x, ti = y;
 #pragma unroll
         for (int k = 12; k >= 1; --k) {
-            const float q = (0.5f * (float)k) / (tr * tr + ti * ti);
-            tr = x - q * tr;
-            ti = y + q * ti;
+            const float q = (0.5f * (float)k) / fmaf(tr, tr, ti * ti);
+            tr = fmaf(-q, tr, x);
+            ti = fmaf(q, ti, y);
         }
-        float h = ti / ((float)A12_SQRTPI * (tr * tr + ti * ti));
+        float h = ti / ((float)A12_SQRTPI * fmaf(tr, tr, ti * ti));
         if (y < 1.0f) h += __expf(y * y - x * x) * __cosf(2.0f * x * y);
         return h;
     }

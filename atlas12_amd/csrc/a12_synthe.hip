@@ -447,6 +447,7 @@ struct TileArgs {
 template <bool F64, int PTS, int CHUNK>
 __device__ __forceinline__ void tile_body(const TileArgs &a, int tile, int j)
 {
+#pragma clang fp contract(off)
     using R = typename std::conditional<F64, double, float>::type;
     constexpr int kTw = kTile * PTS;
     __shared__ int s_nb[CHUNK], s_ns[CHUNK], s_n10[CHUNK];
@@ -569,11 +570,136 @@ __global__ __launch_bounds__(256) void k_tile_batch(const BatchItemModel *__rest
     }
 }
 
+// Barrier-free form: wavefront w of the block owns the contiguous points
+// g0 + w 64 PTS .. + 64 PTS - 1 (lane + 64 p for p < PTS) and works alone:
+// it walks the tile's candidate ranges 64 at a time, keeps (ballot) the
+// lines whose window at this depth meets its points and adds them in
+// candidate order, lowest bit first -- the order of every other shape.  A
+// kept line's NU and extent come from its lane by shuffle, its other
+// parameters by wave-uniform loads fetched one line ahead.
+template <bool F64, int PTS>
+__device__ __forceinline__ void tile_body_w(const TileArgs &a, int tile, int j)
+{
+#pragma clang fp contract(off)
+    using R = typename std::conditional<F64, double, float>::type;
+    constexpr int kTw = kTile * PTS;
+    const int tid = threadIdx.x, lane = tid & (A12_WAVE - 1), wv = tid / A12_WAVE;
+    const int L = a.L;
+    const int g0 = tile * kTw + 1;
+    const int g1 = min(g0 + kTw - 1, L);
+    const int w0 = g0 + wv * A12_WAVE * PTS;            // this wave's first point
+    const int w1 = min(w0 + A12_WAVE * PTS - 1, g1);    // and last
+    const size_t dj = (size_t)j * a.nlines;
+    const R *k0a = static_cast<const R *>(a.kap0) + dj;
+    const R *aa = static_cast<const R *>(a.adamp) + dj;
+    const R *dva = static_cast<const R *>(a.dv) + dj;
+    const R *xwa = static_cast<const R *>(a.xwing) + dj;
+    Neumaier32 acc32[PTS];
+    double acc64[PTS];
+    for (int p = 0; p < PTS; ++p) acc64[p] = 0.0;
+    const int ncls = (a.nlines > 0 && w0 <= g1) ? A12_NCLASS - 1 : 0;
+    for (int c = 0; c < ncls; ++c) {
+        const int lo = a.rng[2 * (tile * (A12_NCLASS - 1) + c)];
+        const int hi = a.rng[2 * (tile * (A12_NCLASS - 1) + c) + 1];
+        for (int c0 = lo; c0 < hi; c0 += A12_WAVE) {
+            const int q = c0 + lane;
+            int l = 0, nb = 0, ns = -1;
+            bool act = false;
+            if (q < hi) {
+                l = a.perm[q];
+                nb = a.nbuffP[q];
+                ns = a.nstep[dj + l];
+                act = ns >= 0 && nb - ns <= w1 && nb + ns >= w0;
+            }
+            unsigned long long m = __ballot(act);
+            if (!m) continue;
+            int b = __builtin_ctzll(m);
+            m &= m - 1;
+            int cl = __shfl(l, b), cnb = __shfl(nb, b), cns = __shfl(ns, b);
+            int cn10 = a.n10[dj + cl];
+            R ck0 = k0a[cl], ca = aa[cl], cdv = dva[cl], cxw = xwa[cl];
+            for (;;) {
+                const bool more = m != 0;
+                int nl_ = 0, nnb = 0, nns = 0, nn10 = 0;
+                R nk0 = 0, na = 0, ndv = 0, nxw = 0;
+                if (more) {
+                    const int bn = __builtin_ctzll(m);
+                    m &= m - 1;
+                    nl_ = __shfl(l, bn);
+                    nnb = __shfl(nb, bn);
+                    nns = __shfl(ns, bn);
+                    nn10 = a.n10[dj + nl_];
+                    nk0 = k0a[nl_];
+                    na = aa[nl_];
+                    ndv = dva[nl_];
+                    nxw = xwa[nl_];
+                }
+#pragma unroll
+                for (int p = 0; p < PTS; ++p) {
+                    const int gi = w0 + p * A12_WAVE + lane;
+                    const int n = abs(gi - cnb);
+                    if (gi > w1 || n > cns) continue;
+                    if (!F64) {
+                        float v;
+                        if (n <= cn10) v = (float)ck0 * voigt_mixed((float)n * (float)cdv, (float)ca);
+                        else v = (float)cxw * __builtin_amdgcn_rcpf((float)n * (float)n);
+                        acc32[p].add(v);
+                    } else {
+                        double v;
+                        if (n <= cn10) v = (double)ck0 * voigt_f64((double)n * (double)cdv, (double)ca);
+                        else v = (double)cxw / ((double)n * (double)n);
+                        acc64[p] += v;
+                    }
+                }
+                if (!more) break;
+                cl = nl_;
+                cnb = nnb;
+                cns = nns;
+                cn10 = nn10;
+                ck0 = nk0;
+                ca = na;
+                cdv = ndv;
+                cxw = nxw;
+            }
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < PTS; ++p) {
+        const int gi = w0 + p * A12_WAVE + lane;
+        if (gi > w1) continue;
+        if (!F64) static_cast<float *>(a.out)[(size_t)j * L + gi - 1] = acc32[p].value();
+        else static_cast<double *>(a.out)[(size_t)j * L + gi - 1] = acc64[p];
+    }
+}
+
+template <bool F64, int PTS>
+__global__ __launch_bounds__(256) void k_tile_w(TileArgs a)
+{
+    tile_body_w<F64, PTS>(a, blockIdx.x, blockIdx.y);
+}
+
+template <bool F64, int PTS>
+__global__ __launch_bounds__(256) void k_tile_batch_w(const BatchItemModel *__restrict__ models,
+                                                      const long long *__restrict__ item_off,
+                                                      int nmodel)
+{
+    const long long nitems = item_off[nmodel];
+    int m = 0;
+    for (long long w = blockIdx.x; w < nitems; w += gridDim.x) {
+        while (w >= item_off[m + 1]) ++m;
+        const long long r = w - item_off[m];
+        const int nt = models[m].ntiles;
+        tile_body_w<F64, PTS>(models[m].a, (int)(r % nt), (int)(r / nt));
+    }
+}
+
 // Gather shapes: points per thread and lines per LDS stage.
 struct TileShape {
     int pts, chunk;
 };
-constexpr TileShape kTileShapes[] = {{1, 256}, {1, 128}, {1, 512}, {2, 256}, {2, 512}, {4, 256}};
+// chunk 0: the barrier-free per-wave form
+constexpr TileShape kTileShapes[] = {{1, 256}, {1, 128}, {1, 512}, {2, 256}, {2, 512},
+                                     {4, 256}, {1, 0},   {2, 0},   {4, 0}};
 constexpr int kNumTileShapes = sizeof(kTileShapes) / sizeof(kTileShapes[0]);
 
 template <bool F64>
@@ -590,6 +716,9 @@ TileKernel<F64> tile_kernel(int shape)
     case 3: return k_tile<F64, 2, 256>;
     case 4: return k_tile<F64, 2, 512>;
     case 5: return k_tile<F64, 4, 256>;
+    case 6: return k_tile_w<F64, 1>;
+    case 7: return k_tile_w<F64, 2>;
+    case 8: return k_tile_w<F64, 4>;
     default: return k_tile<F64, 1, 256>;
     }
 }
@@ -603,6 +732,9 @@ TileBatchKernel<F64> tile_batch_kernel(int shape)
     case 3: return k_tile_batch<F64, 2, 256>;
     case 4: return k_tile_batch<F64, 2, 512>;
     case 5: return k_tile_batch<F64, 4, 256>;
+    case 6: return k_tile_batch_w<F64, 1>;
+    case 7: return k_tile_batch_w<F64, 2>;
+    case 8: return k_tile_batch_w<F64, 4>;
     default: return k_tile_batch<F64, 1, 256>;
     }
 }

@@ -235,7 +235,8 @@ def synthe_block(args, ctx, native, st, lines, steps, warmup, shape=None):
         "stage_ms_fp64": {"prep_continuum": st_64[0], "line_setup": st_64[1],
                           "partition": st_64[2], "tile_gather": st_64[3]},
         "roofline": {"bound": "valu",
-                     "kernel": f"k_tile<false, {shapes[shape][0]}, {shapes[shape][1]}>",
+                     "kernel": (f"k_tile_w<false, {shapes[shape][0]}>" if shapes[shape][1] == 0
+                                else f"k_tile<false, {shapes[shape][0]}, {shapes[shape][1]}>"),
                      "achieved": flops / (tile_ms * 1e-3) / 1e12, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": flops / (tile_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,
                      "flops_per_launch": flops, "avg_launch_ms": tile_ms},

@@ -106,7 +106,8 @@ int a12_synthe_opacity_run(a12_ctx *ctx, int32_t prec);
 int a12_synthe_opacity_fetch(a12_ctx *ctx, void *opacity_out);
 /* Tile shapes of the line-opacity gather (no counterpart in the reference;
  * results do not depend on the shape): shape idx -> {points per thread,
- * lines per LDS stage}; returns the number of shapes, -1 past the end.
+ * lines per LDS stage, 0 for the barrier-free per-wave form}; returns the
+ * number of shapes, -1 past the end.
  * set_shape picks the shape of later runs (single and batched). */
 int a12_synthe_shape_info(int32_t idx, int32_t pts_chunk[2]);
 int a12_synthe_set_shape(a12_ctx *ctx, int32_t shape);
