@@ -129,7 +129,9 @@ __device__ __forceinline__ float voigt_f32(float v, float a)
 //                            near the real axis (a < 1)
 //   2.5 <= |v| < 5, a < 0.1: FP64 Weideman
 //   otherwise              : FP32 Weideman
-// Worst relative error against Re w(z) is below 2e-5 (see tests).
+// The asymptote and the continued fraction divide with the hardware
+// reciprocal (1 ulp).  Worst relative error against Re w(z) is below 2e-5
+// (see tests).
 __device__ __forceinline__ float voigt_mixed(float v, float a)
 {
     // fused steps are written as fmaf; nothing else is contracted, so the
@@ -140,17 +142,18 @@ __device__ __forceinline__ float voigt_mixed(float v, float a)
         const float x2 = x * x, y2 = y * y;
         const float dr = x2 - y2 - 0.5f;
         const float di = 2.0f * x * y;
-        return y * (x2 + y2 + 0.5f) / ((float)A12_SQRTPI * (dr * dr + di * di));
+        return y * (x2 + y2 + 0.5f) *
+               __builtin_amdgcn_rcpf((float)A12_SQRTPI * (dr * dr + di * di));
     }
     if (x >= 5.0f) {
         float tr = x, ti = y;
 #pragma unroll
         for (int k = 12; k >= 1; --k) {
-            const float q = (0.5f * (float)k) / fmaf(tr, tr, ti * ti);
+            const float q = (0.5f * (float)k) * __builtin_amdgcn_rcpf(fmaf(tr, tr, ti * ti));
             tr = fmaf(-q, tr, x);
             ti = fmaf(q, ti, y);
         }
-        float h = ti / ((float)A12_SQRTPI * fmaf(tr, tr, ti * ti));
+        float h = ti * __builtin_amdgcn_rcpf((float)A12_SQRTPI * fmaf(tr, tr, ti * ti));
         if (y < 1.0f) h += __expf(y * y - x * x) * __cosf(2.0f * x * y);
         return h;
     }
