@@ -361,6 +361,52 @@ def batch_run(args, ctx, native, wl):
     }
 
 
+# ------------------------------------------------- molecular equilibrium
+def molec_run(args, ctx, native):
+    """NMOLEC's gas-phase Newton solve for 256 models x 72 depths: the cool-
+    dwarf deck's systems (the reference's own equilibrium constants,
+    tests/golden/molec_ref.npz) with every start perturbed by a factor in
+    [0.5, 2], one independent system per wavefront, one launch; the oracle
+    on a 2,048-system sample of the same batch on the host."""
+    from oracle import oracle as orc
+    import torch
+    z = np.load(os.path.join(GOLDEN, "molec_ref.npz"))
+    native.molec_set_table(ctx, z["locj"], z["kcomps"], z["idequa"])
+    xntot, xab, start = orc.nmolec_inputs(z, "cool")
+    p = z["cool_p"]
+    oxn, _, _ = orc.nmolec_chain(z, "cool")
+    base = np.vstack([start, oxn[:-1] * (p[1:] / p[:-1])[:, None]])
+    rng = np.random.default_rng(9)
+    nrep = 256
+    x0 = np.concatenate([base * rng.uniform(0.5, 2.0, base.shape) for _ in range(nrep)])
+    xt = np.tile(xntot, nrep)
+    xa = np.tile(xab, (nrep, 1))
+    eq = np.tile(z["cool_equilj"], (nrep, 1))
+    native.molec_newton(ctx, xt[:1024], xa[:1024], eq[:1024], x0[:1024])  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xn, _, its, st = native.molec_newton(ctx, xt, xa, eq, x0, xnmol=False)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ns = len(xt)
+    res = {"workload": f"molecular equilibrium: {nrep} x {len(p)} depth systems of the cool-dwarf "
+                       f"deck ({len(z['idequa'])} equations, {len(z['xnmolcode'])} species), "
+                       "starts perturbed x0.5-2, one wavefront per system",
+           "systems": ns, "s_per_batch": dt, "systems_per_s": ns / dt,
+           "newton_iterations_mean": float(np.mean(its)), "converged": int(np.sum(st == 0)),
+           "note": "host-to-device copies of inputs and outputs included"}
+    if not args.no_cpu:
+        sel = np.arange(0, ns, ns // 2048)[:2048]
+        t0 = time.perf_counter()
+        for s_ in sel:
+            orc.molec_newton(z, xt[s_], xa[s_], eq[s_], x0[s_])
+        dc = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": len(sel) / dc, "unit": "systems/s", "cores": 1,
+                               "kind": "port", "sample": f"{len(sel)} systems of the batch, "
+                                                         "the oracle's C Newton solve"}
+    return res
+
+
 # ------------------------------------------------------- starting network
 PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
 MLP_FLOPS_PER_SAMPLE = 2 * 235_664  # forward; a training step is 3x
@@ -477,6 +523,7 @@ def main():
     if not (args.profile_only or args.no_large or syn is None):
         big = large_run(args, ctx, native, wl, syn[0]["_shape"])
     net = mlp_run(args, ctx, native)
+    mol = None if args.profile_only else molec_run(args, ctx, native)
     if args.profile_only and not args.no_batch:
         args.batch_models, args.batch_lines = 4, 1_000_000
     bat = None if args.no_batch else batch_run(args, ctx, native, wl)
@@ -552,6 +599,8 @@ def main():
         result["starting_network"] = net
     if bat is not None:
         result["synthe_batch"] = bat
+    if mol is not None:
+        result["molecular_equilibrium"] = mol
     if syn is not None:
         syn[0].pop("_shape", None)
         result["synthe_plain"] = syn[0]
