@@ -115,6 +115,24 @@ __device__ __forceinline__ float lo_cv_mixed(float center, double v64, double a6
     return center * ((h2[iv - 1] * a + h1[iv - 1]) * a + h0[iv - 1]);
 }
 
+// lo_cv in the two regimes where the profile never increases along a walk
+// (a <= 0.2 with v > 10: the Lorentz wing; a > 0.2 outside a <= 1.4 and
+// a + v <= 3.2: VOIGT's asymptotic form), with the same operations in the
+// same order, so the value has the same bits; no table reads.
+__device__ __forceinline__ double lo_tail(double center, double v, double adamp)
+{
+#pragma clang fp contract(off)
+    if (adamp <= 0.20) return center * 0.5641895835477563 * adamp / (v * v);
+    const double aa = adamp * adamp, vv = v * v;
+    const double u = (aa + vv) * 1.4142;
+    double r = adamp * 0.79788 / u;
+    if (adamp <= 100.0) {
+        const double aau = aa / u, vvu = vv / u, uu = u * u;
+        r = ((((aau - 10.0 * vvu) * aau * 3.0 + 15.0 * vvu * vvu) + 3.0 * vv - aa) / uu + 1.0) * r;
+    }
+    return center * r;
+}
+
 // Number of wing points LINOP1 adds on one side before and including the
 // first value below tc: points first, first+step, ... last (red: offsets
 // w - wl; blue: wl - w).  The walk is taken point by point until the
@@ -127,24 +145,27 @@ __device__ __forceinline__ float lo_cv_mixed(float center, double v64, double a6
 //     increasing, monotone under rounding; for 0.2 < a <= 100 the FP64
 //     expression was checked non-increasing on a 1e-3 grid in v for 6,000
 //     values of a, far finer than any walk step).
-// From there the first point below tc is found by bisection with the same
-// predicate (lo_cv < tc) the walk tests.  Equal to the walk whenever
-// consecutive walk points' profile values differ by more than the FP64
-// evaluation error, i.e. for any grid coarser than ~1e-9 Doppler widths.
+// From there the first point below tc is searched for with the same FP64
+// predicate the walk tests, starting where the tail's closed form crosses
+// tc: v* = sqrt(center a / (sqrt(pi) tc)) for the Lorentz wing, the leading
+// term of the asymptote (a^2 + v*^2 = 0.79788 center a / (1.4142 tc))
+// otherwise, placed on the grid with its mean log step (lg2inv = 1 /
+// log2 of the point-to-point ratio).  The estimate p is only a starting
+// point: the predicate at p and at one neighbour fixes the answer when p is
+// right (over 99.8 % of searches on a log-uniform grid, the rest one point
+// off), and a bisection over what they leave open finishes otherwise.
+// Equal to the walk whenever consecutive walk points' profile values differ
+// by more than the FP64 evaluation error, i.e. for any grid coarser than
+// ~1e-9 Doppler widths.
 __device__ int lo_reach(int first, int last, int step, const double *__restrict__ waveset,
                         double wlvac, double dopwave, double center, double adamp, double tc,
-                        const double *h0, const double *h1, const double *h2)
+                        const double *h0, const double *h1, const double *h2, float lg2inv)
 {
 #pragma clang fp contract(off)
     int cnt = 0;
     int iw = first;
     const int nmax = step > 0 ? last - first + 1 : first - last + 1;
     if (nmax <= 0) return 0;
-    auto below = [&](int pos) {  // the walk's test at walk position pos
-        const int pw = first + pos * step;
-        const double v = step > 0 ? (waveset[pw] - wlvac) / dopwave : (wlvac - waveset[pw]) / dopwave;
-        return lo_cv<double>(center, v, adamp, h0, h1, h2) < tc;
-    };
     for (; cnt < nmax; ++cnt, iw += step) {
         const double v = step > 0 ? (waveset[iw] - wlvac) / dopwave : (wlvac - waveset[iw]) / dopwave;
         const bool mono = adamp <= 0.20 ? v > 10.0 : !(adamp <= 1.4 && adamp + v <= 3.2);
@@ -152,10 +173,45 @@ __device__ int lo_reach(int first, int last, int step, const double *__restrict_
         if (lo_cv<double>(center, v, adamp, h0, h1, h2) < tc) return cnt + 1;
     }
     if (cnt == nmax) return cnt;
-    // first walk position in [cnt, nmax) with value < tc; every position
-    // below lo is known >= tc, every position from hi on < tc (hi = nmax:
-    // none known)
+    auto below = [&](int pos) {  // the walk's test at walk position pos (tail regimes)
+        const int pw = first + pos * step;
+        const double v = step > 0 ? (waveset[pw] - wlvac) / dopwave : (wlvac - waveset[pw]) / dopwave;
+        return lo_tail(center, v, adamp) < tc;
+    };
+    // estimated first walk position past the crossing, in [cnt, nmax]
+    int p;
+    const double q = adamp <= 0.20
+                         ? center * 0.5641895835477563 * adamp * __builtin_amdgcn_rcp(tc)
+                         : center * adamp * 0.5641895835477563 * __builtin_amdgcn_rcp(tc) - adamp * adamp;
+    if (!(q > 0.0)) {
+        p = cnt;
+    } else {
+        const float off = __builtin_sqrtf((float)q) * (float)dopwave;
+        const float wt = step > 0 ? (float)wlvac + off : (float)wlvac - off;
+        const float pe = (float)step * __log2f(wt * __builtin_amdgcn_rcpf((float)waveset[first])) * lg2inv;
+        p = (wt <= 0.0f || !(pe < (float)nmax)) ? nmax : max((int)ceilf(pe), cnt);
+    }
+    // every position below lo is known >= tc, every position from hi on < tc
+    // (hi = nmax: none known)
     int lo = cnt, hi = nmax;
+    if (p < nmax) {
+        if (below(p)) {
+            hi = p;
+            if (p - 1 >= lo) {
+                if (below(p - 1)) hi = p - 1;
+                else lo = p;
+            }
+        } else {
+            lo = p + 1;
+            if (p + 1 < hi) {
+                if (below(p + 1)) hi = p + 1;
+                else lo = p + 2;
+            }
+        }
+    } else if (nmax - 1 >= lo) {
+        if (below(nmax - 1)) hi = nmax - 1;
+        else lo = nmax;
+    }
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
         if (below(mid)) hi = mid;
@@ -190,8 +246,8 @@ __global__ __launch_bounds__(256) void k_lo_setup(
     const double *__restrict__ waveset, const double *__restrict__ tabcont,
     const double *__restrict__ xnfdop, const double *__restrict__ dopple,
     const double *__restrict__ dep, const double *__restrict__ h0, const double *__restrict__ h1,
-    const double *__restrict__ h2, int *__restrict__ o_state, double *__restrict__ o_par,
-    long long *__restrict__ blk_evals)
+    const double *__restrict__ h2, float lg2inv, int *__restrict__ o_state,
+    double *__restrict__ o_par, long long *__restrict__ blk_evals)
 {
 #pragma clang fp contract(off)
     __shared__ int s_ok[256];
@@ -244,9 +300,9 @@ __global__ __launch_bounds__(256) void k_lo_setup(
             const double dopwave = dop * wlvac;
             // red: IW = NU .. min(NU+100, NUMNU); blue: NU-1 down, at most 100
             const int rn = lo_reach(nu, min(nu + kMaxWing, numnu - 1), 1, waveset, wlvac, dopwave,
-                                    center, adamp, tc, h0, h1, h2);
+                                    center, adamp, tc, h0, h1, h2, lg2inv);
             const int bn = lo_reach(nu - 1, max(nu - kMaxWing, 0), -1, waveset, wlvac, dopwave,
-                                    center, adamp, tc, h0, h1, h2);
+                                    center, adamp, tc, h0, h1, h2, lg2inv);
             st = (int)(0x80000000u | (unsigned)rn | ((unsigned)bn << 8));
             o_par[3 * g + 0] = center;
             o_par[3 * g + 1] = adamp;
@@ -742,6 +798,11 @@ int a12i_atlas_load(a12_ctx *ctx, const a12_atlas_atm *a, const int32_t *linedat
     A.numnu = numnu;
     A.mion = a->mion;
     A.teff = a->teff;
+    // mean points per octave of the grid: places the reach search's first
+    // probe (a starting point only; any ascending grid gives exact reaches)
+    A.lg2inv = numnu > 1 ? (float)((numnu - 1) / std::log2(a->waveset[numnu - 1] / a->waveset[0]))
+                         : 0.0f;
+    if (!std::isfinite(A.lg2inv)) A.lg2inv = 0.0f;
     // depth vectors: t hkt rhox | xne hckt txnxn (TXNXN as LINOP1 forms it, :18164)
     std::vector<double> dep((size_t)6 * n);
     for (int j = 0; j < n; ++j) {
@@ -898,7 +959,7 @@ int a12i_atlas_linop1_run(a12_ctx *ctx, int prec)
                            A.l_nu.as<int>(), A.l_nucont.as<int>(), A.l_nel.as<int>(),
                            A.l_const.as<double>(), A.waveset.as<double>(), A.tabcont.as<double>(),
                            A.xnfdop.as<double>(), A.dopple.as<double>(), A.depth_lo.as<double>(),
-                           tabs, tabs + kTabN, tabs + 2 * kTabN, A.state.as<int>(),
+                           tabs, tabs + kTabN, tabs + 2 * kTabN, A.lg2inv, A.state.as<int>(),
                            A.par.as<double>(), A.blk.as<long long>());
         A12_CHECK_LAUNCH();
         const int np = std::min(nb, kSumParts);

@@ -63,6 +63,7 @@ struct SyntheDev {
 struct AtlasDev {
     int n = 0, numnu = 0, mion = 0, nl = 0;
     double teff = 0;
+    float lg2inv = 0;  // 1 / log2 of WAVESET's mean point-to-point ratio
     DevBuf depth3, depth_lo, tabcont, xnfdop, dopple, waveset, rcoset, tabs, tabs32;
     DevBuf l_nu, l_nucont, l_nel, l_const, seg;
     DevBuf state, par, reach, xlines, blk, evals;

@@ -44,6 +44,27 @@ def test_linop1_mixed_matches_reference(gpu_ctx, atl, orc):
     assert _rel(xl, z["xlines"]).max() <= 5e-6
 
 
+def test_linop1_uneven_grid_matches_oracle(gpu_ctx, atl, orc):
+    """The wing-reach search starts at the grid position where the tail's
+    closed form crosses the rejection level, placed with the grid's mean
+    log step; on a grid whose spacing varies by 3x between neighbouring
+    stretches that start is often wrong, and the exact search behind it
+    must still give the oracle's contribution count and XLINES."""
+    from atlas12_amd import native
+    z, _ = atl
+    zz = {k: z[k] for k in z.files}
+    w = np.asarray(z["waveset"], float)
+    lg = np.diff(np.log(w))
+    lg = lg * np.where((np.arange(len(lg)) // 7) % 2 == 0, 0.5, 1.5)
+    zz["waveset"] = w[0] * np.exp(np.concatenate([[0.0], np.cumsum(lg)]))
+    assert np.all(np.diff(zz["waveset"]) > 0)
+    native.atlas_load(gpu_ctx, native.atlas_state_from(zz), zz["linedata"])
+    xl, ev = native.atlas_linop1(gpu_ctx, native.PREC_FP64)
+    oxl, oev = orc.linop1(zz)
+    assert ev == oev and ev > 0
+    assert _rel(xl, oxl).max() <= 1e-13
+
+
 def test_rt_pass_matches_reference(gpu_ctx, atl, coef):
     """Frequency loop on the reference's XLINES and KAPP tables vs the
     reference's RUN_RT_PASS accumulators."""
