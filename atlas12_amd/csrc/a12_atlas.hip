@@ -173,16 +173,38 @@ __device__ int lo_reach(int first, int last, int step, const double *__restrict_
         if (lo_cv<double>(center, v, adamp, h0, h1, h2) < tc) return cnt + 1;
     }
     if (cnt == nmax) return cnt;
-    auto below = [&](int pos) {  // the walk's test at walk position pos (tail regimes)
+    // The walk's test at walk position pos (tail regimes), screened in FP32:
+    // the profile over tc, evaluated with reciprocals from the FP64 offset
+    // (relative error below ~1e-6), decides when it is more than 1e-4 away
+    // from 1; otherwise the reference's FP64 expression does.  Either way the
+    // answer is the FP64 predicate's.
+    const double rtc = 1.0 / tc;
+    const float qf = (float)(center * 0.5641895835477563 * adamp * rtc);  // Lorentz: qf / v^2
+    const float cf = (float)(center * adamp * 0.79788 * rtc);  // asymptote: cf / u * (1 + ...)
+    const float af = (float)adamp, aaf = af * af, rdf = (float)(1.0 / dopwave);
+    auto below = [&](int pos) {
         const int pw = first + pos * step;
-        const double v = step > 0 ? (waveset[pw] - wlvac) / dopwave : (wlvac - waveset[pw]) / dopwave;
-        return lo_tail(center, v, adamp) < tc;
+        const double d = step > 0 ? waveset[pw] - wlvac : wlvac - waveset[pw];
+        const float v = (float)d * rdf, vv = v * v;
+        float ratio;
+        if (adamp <= 0.20) {
+            ratio = qf * __builtin_amdgcn_rcpf(vv);
+        } else {
+            const float ru = __builtin_amdgcn_rcpf((aaf + vv) * 1.4142f);
+            ratio = cf * ru;
+            if (adamp <= 100.0) {
+                const float aau = aaf * ru, vvu = vv * ru;
+                ratio = ((((aau - 10.0f * vvu) * aau * 3.0f + 15.0f * vvu * vvu) + 3.0f * vv - aaf) *
+                             (ru * ru) + 1.0f) * ratio;
+            }
+        }
+        if (ratio < 0.9999f) return true;
+        if (ratio > 1.0001f) return false;
+        return lo_tail(center, d / dopwave, adamp) < tc;
     };
     // estimated first walk position past the crossing, in [cnt, nmax]
     int p;
-    const double q = adamp <= 0.20
-                         ? center * 0.5641895835477563 * adamp * __builtin_amdgcn_rcp(tc)
-                         : center * adamp * 0.5641895835477563 * __builtin_amdgcn_rcp(tc) - adamp * adamp;
+    const double q = adamp <= 0.20 ? (double)qf : (double)cf * (1.0 / 1.4142) - adamp * adamp;
     if (!(q > 0.0)) {
         p = cnt;
     } else {
