@@ -32,6 +32,11 @@
 namespace {
 
 constexpr int kTile = 256;       // grid points per tile (one per thread)
+// MIXED sums the lines of each run of 2^kGroupShift consecutive class
+// positions in plain FP32 (at most 15 roundings, relative error <= 15 eps),
+// and adds each run's sum into a Neumaier-compensated total.  Runs are fixed
+// by position, so every tile shape forms the same partial sums.
+constexpr int kGroupShift = 4;
 constexpr int kChunk = 256;      // lines staged in LDS per pass
 constexpr int kBaseExt = 64;     // extent of class 0
 constexpr int kMaxProf = 1000000;  // MAXPROF, src/synthe_module.f90:219
@@ -432,8 +437,9 @@ __global__ void k_tile_ranges(int ntiles, int tw, int L, const int *__restrict__
 //   n = |i - nbuff| <= nstep:  n <= n10dop -> kappa0 H(n dv, a)
 //                              else        -> x_wing / n^2
 // MIXED evaluates H in FP32, the n^-2 wing with the hardware reciprocal
-// (1 ulp), and accumulates with Neumaier compensation; FP64 does all of it
-// in double with IEEE division.  Every point sees the lines in the same order
+// (1 ulp), sums each run of 16 class positions in plain FP32 and adds the
+// run sums with Neumaier compensation; FP64 does all of it in double with
+// IEEE division.  Every point sees the lines in the same order
 // (class, then centre position) whatever PTS and CHUNK are, so the result
 // does not depend on the tile shape.
 // --------------------------------------------------------------------------
@@ -450,7 +456,7 @@ __device__ __forceinline__ void tile_body(const TileArgs &a, int tile, int j)
 #pragma clang fp contract(off)
     using R = typename std::conditional<F64, double, float>::type;
     constexpr int kTw = kTile * PTS;
-    __shared__ int s_nb[CHUNK], s_ns[CHUNK], s_n10[CHUNK];
+    __shared__ int s_nb[CHUNK], s_ns[CHUNK], s_n10[CHUNK], s_g[CHUNK];
     __shared__ R s_k0[CHUNK], s_a[CHUNK], s_dv[CHUNK], s_xw[CHUNK];
     __shared__ int s_wcnt[kTile / A12_WAVE];
     __shared__ int s_nact;
@@ -466,8 +472,10 @@ __device__ __forceinline__ void tile_body(const TileArgs &a, int tile, int j)
     const R *xwa = static_cast<const R *>(a.xwing) + dj;
 
     Neumaier32 acc32[PTS];
+    float part[PTS];  // plain FP32 sum of the current line group
     double acc64[PTS];
-    for (int p = 0; p < PTS; ++p) acc64[p] = 0.0;
+    for (int p = 0; p < PTS; ++p) acc64[p] = 0.0, part[p] = 0.0f;
+    int grp = -1;
 
     const int ncls = a.nlines > 0 ? A12_NCLASS - 1 : 0;  // no lines: zeros
     for (int c = 0; c < ncls; ++c) {
@@ -493,6 +501,7 @@ __device__ __forceinline__ void tile_body(const TileArgs &a, int tile, int j)
                     for (int w = 0; w < wv; ++w) pos += s_wcnt[w];
                     s_nb[pos] = nb;
                     s_ns[pos] = ns;
+                    s_g[pos] = q >> kGroupShift;
                     s_n10[pos] = a.n10[dj + l];
                     s_k0[pos] = k0a[l];
                     s_a[pos] = aa[l];
@@ -510,6 +519,13 @@ __device__ __forceinline__ void tile_body(const TileArgs &a, int tile, int j)
             }
             for (int k = 0; k < nact; ++k) {
                 const int nb = s_nb[k], ns = s_ns[k], n10 = s_n10[k];
+                if (!F64) {
+                    const int g = s_g[k];
+                    if (g != grp) {
+                        for (int p = 0; p < PTS; ++p) acc32[p].add(part[p]), part[p] = 0.0f;
+                        grp = g;
+                    }
+                }
 #pragma unroll
                 for (int p = 0; p < PTS; ++p) {
                     const int gi = g0 + tid + p * kTile;
@@ -519,7 +535,7 @@ __device__ __forceinline__ void tile_body(const TileArgs &a, int tile, int j)
                         float v;
                         if (n <= n10) v = (float)s_k0[k] * voigt_mixed((float)n * (float)s_dv[k], (float)s_a[k]);
                         else v = (float)s_xw[k] * __builtin_amdgcn_rcpf((float)n * (float)n);
-                        acc32[p].add(v);
+                        part[p] += v;
                     } else {
                         double v;
                         if (n <= n10) v = (double)s_k0[k] * voigt_f64((double)n * (double)s_dv[k], (double)s_a[k]);
@@ -535,6 +551,7 @@ __device__ __forceinline__ void tile_body(const TileArgs &a, int tile, int j)
     for (int p = 0; p < PTS; ++p) {
         const int gi = g0 + tid + p * kTile;
         if (gi > g1) continue;
+        acc32[p].add(part[p]);
         if (!F64) static_cast<float *>(a.out)[(size_t)j * L + gi - 1] = acc32[p].value();
         else static_cast<double *>(a.out)[(size_t)j * L + gi - 1] = acc64[p];
     }
@@ -595,8 +612,10 @@ __device__ __forceinline__ void tile_body_w(const TileArgs &a, int tile, int j)
     const R *dva = static_cast<const R *>(a.dv) + dj;
     const R *xwa = static_cast<const R *>(a.xwing) + dj;
     Neumaier32 acc32[PTS];
+    float part[PTS];  // plain FP32 sum of the current line group
     double acc64[PTS];
-    for (int p = 0; p < PTS; ++p) acc64[p] = 0.0;
+    for (int p = 0; p < PTS; ++p) acc64[p] = 0.0, part[p] = 0.0f;
+    int grp = -1;
     const int ncls = (a.nlines > 0 && w0 <= g1) ? A12_NCLASS - 1 : 0;
     for (int c = 0; c < ncls; ++c) {
         const int lo = a.rng[2 * (tile * (A12_NCLASS - 1) + c)];
@@ -615,16 +634,18 @@ __device__ __forceinline__ void tile_body_w(const TileArgs &a, int tile, int j)
             if (!m) continue;
             int b = __builtin_ctzll(m);
             m &= m - 1;
+            int cq = c0 + b;
             int cl = __shfl(l, b), cnb = __shfl(nb, b), cns = __shfl(ns, b);
             int cn10 = a.n10[dj + cl];
             R ck0 = k0a[cl], ca = aa[cl], cdv = dva[cl], cxw = xwa[cl];
             for (;;) {
                 const bool more = m != 0;
-                int nl_ = 0, nnb = 0, nns = 0, nn10 = 0;
+                int nl_ = 0, nnb = 0, nns = 0, nn10 = 0, nq = 0;
                 R nk0 = 0, na = 0, ndv = 0, nxw = 0;
                 if (more) {
                     const int bn = __builtin_ctzll(m);
                     m &= m - 1;
+                    nq = c0 + bn;
                     nl_ = __shfl(l, bn);
                     nnb = __shfl(nb, bn);
                     nns = __shfl(ns, bn);
@@ -633,6 +654,10 @@ __device__ __forceinline__ void tile_body_w(const TileArgs &a, int tile, int j)
                     na = aa[nl_];
                     ndv = dva[nl_];
                     nxw = xwa[nl_];
+                }
+                if (!F64 && (cq >> kGroupShift) != grp) {
+                    for (int p = 0; p < PTS; ++p) acc32[p].add(part[p]), part[p] = 0.0f;
+                    grp = cq >> kGroupShift;
                 }
 #pragma unroll
                 for (int p = 0; p < PTS; ++p) {
@@ -643,7 +668,7 @@ __device__ __forceinline__ void tile_body_w(const TileArgs &a, int tile, int j)
                         float v;
                         if (n <= cn10) v = (float)ck0 * voigt_mixed((float)n * (float)cdv, (float)ca);
                         else v = (float)cxw * __builtin_amdgcn_rcpf((float)n * (float)n);
-                        acc32[p].add(v);
+                        part[p] += v;
                     } else {
                         double v;
                         if (n <= cn10) v = (double)ck0 * voigt_f64((double)n * (double)cdv, (double)ca);
@@ -652,6 +677,7 @@ __device__ __forceinline__ void tile_body_w(const TileArgs &a, int tile, int j)
                     }
                 }
                 if (!more) break;
+                cq = nq;
                 cl = nl_;
                 cnb = nnb;
                 cns = nns;
@@ -667,6 +693,7 @@ __device__ __forceinline__ void tile_body_w(const TileArgs &a, int tile, int j)
     for (int p = 0; p < PTS; ++p) {
         const int gi = w0 + p * A12_WAVE + lane;
         if (gi > w1) continue;
+        acc32[p].add(part[p]);
         if (!F64) static_cast<float *>(a.out)[(size_t)j * L + gi - 1] = acc32[p].value();
         else static_cast<double *>(a.out)[(size_t)j * L + gi - 1] = acc64[p];
     }
