@@ -66,12 +66,10 @@ constexpr int NX = A12_NXTAU;
 constexpr int NR = A12_NRMAX;
 
 struct JoshLDS {
-    double amat[NX * NX];
     double xs[64], xsbar[64], xalpha[64], xtmp[64];
     double taunu[NR], snubar[NR], alpha[NR], abtot[NR];
     double snu[NR], hnu[NR], jmins[NR], jnu[NR], bnu[NR], rhox[NR];
     double pa[NR], pb[NR], pc[NR];
-    int ipiv[64];
     double red[64];
     int iflag[4];
 };
@@ -308,7 +306,6 @@ __device__ int map1_wave(const double *xold, const double *fold, int nold, const
 
 // ---- SOLVIT: LU with partial pivoting, lane = row ------------------------
 // amat column-major n x n in LDS, b (n) in LDS (solution on return).
-__device__ void solvit_wave(JoshLDS &W, int n) { solvit_lds(W.amat, W.xs, W.ipiv, n); }
 
 struct JoshOut {
     double hnu1, knu1;
@@ -362,19 +359,20 @@ __device__ JoshOut josh_wave(JoshLDS &W, int n, int ifscat, int ifsurf,
                 if (ln < NX) W.xs[ln] = fmax(W.xsbar[ln], 1.0e-37);
                 wsync();
             } else {
-                // AMAT(L, M) = -XALPHA(L) COEFJ(L, M) + delta; lane = row L
-                if (ln < NX) {
-                    const double al = W.xalpha[ln];
-                    for (int m = 0; m < NX; ++m) {
-                        double v = -al * coefj[m * NX + ln];
-                        if (m == ln) v = v + 1.0;
-                        W.amat[m * NX + ln] = v;
-                    }
-                    W.xs[ln] = (1.0 - al) * W.xsbar[ln];
+                // AMAT(L, M) = -XALPHA(L) COEFJ(L, M) + delta, solved in
+                // registers; lane = row L
+                double arow[NX];
+                const double al = ln < NX ? W.xalpha[ln] : 0.0;
+#pragma unroll
+                for (int m = 0; m < NX; ++m) {
+                    double v = ln < NX ? -al * coefj[m * NX + ln] : 0.0;
+                    if (m == ln) v = v + 1.0;
+                    arow[m] = v;
                 }
-                wsync();
-                solvit_wave(W, NX);
-                if (ln < NX) W.xs[ln] = fmax(W.xs[ln], 1.0e-37);
+                const double rhs = ln < NX ? (1.0 - al) * W.xsbar[ln] : 0.0;
+                int row;
+                const double x = solvit_reg<NX>(arow, rhs, &row);
+                if (row >= 1) W.xs[row - 1] = fmax(x, 1.0e-37);
                 wsync();
             }
             if (ifsurf == 1) dispatch60 = true;
