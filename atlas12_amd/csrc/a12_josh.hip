@@ -696,30 +696,32 @@ __global__ __launch_bounds__(64) void k_rt_pass(
 {
 #pragma clang fp contract(off)
     __shared__ JoshLDS W;
-    __shared__ double term2[NR];
+    __shared__ double term2[NR], s_stim[NR];
     const int ln = lane_id();
     const double *t = depth3, *hkt = depth3 + n, *rhox = depth3 + 2 * n;
     const size_t plane = (size_t)n * nf;
     const double euler_m1 = 0.922784335098467;
     const double bnu_prefac = 2.0 * A12_HPLANCK / (A12_CLIGHT * A12_CLIGHT) * 1.0e45;
     for (int j = ln; j < n; j += 64) W.rhox[j] = rhox[j];
+    // this chunk's accumulators live in its own slice of part (nothing stays
+    // in registers across JOSH); the same additions in the same order
+    double *pp = part + (size_t)blockIdx.x * 9 * n;
+    for (int i = ln; i < 8 * n; i += 64) pp[i] = 0.0;
     wsync();
-    double acc[8][2] = {};
     double pradk0 = 0.0;
     const int k0 = blockIdx.x * chunk, k1 = min(nf, k0 + chunk);
     for (int k = k0; k < k1; ++k) {
         const double freq = A12_CLIGHT_NMS / wave[k];
         const double rcowt = rcoset[k];
         const double f15 = freq / 1.0e15;
-        double stim[2], bnu[2];
-        int q = 0;
-        for (int j = ln; j < n; j += 64, ++q) {
+        for (int j = ln; j < n; j += 64) {
             const size_t o = (size_t)k * n + j;
             const double ehv = exp(-freq * hkt[j]);
-            stim[q] = 1.0 - ehv;
-            bnu[q] = bnu_prefac * (f15 * f15 * f15) * ehv / stim[q];
-            const double aline = kapp[4 * plane + o] + xlines[o] * stim[q];
-            W.bnu[j] = bnu[q];
+            const double stim = 1.0 - ehv;
+            const double bnu = bnu_prefac * (f15 * f15 * f15) * ehv / stim;
+            s_stim[j] = stim;
+            const double aline = kapp[4 * plane + o] + xlines[o] * stim;
+            W.bnu[j] = bnu;
             josh_prepare(W, j, kapp[o], aline, kapp[plane + o], kapp[5 * plane + o],
                          kapp[2 * plane + o], kapp[3 * plane + o]);
             W.snu[j] = 0.0;
@@ -760,28 +762,24 @@ __global__ __launch_bounds__(64) void k_rt_pass(
             term2[j] = t2;
         }
         wsync();
-        q = 0;
-        for (int j = ln; j < n; j += 64, ++q) {
+        for (int j = ln; j < n; j += 64) {
             const double ab = W.abtot[j], al = W.alpha[j], h = W.hnu[j];
-            acc[0][q] = acc[0][q] + ab * W.jmins[j] * rcowt;
-            acc[1][q] = acc[1][q] + W.pb[j] / ab * h * rcowt;
+            const double bnu = W.bnu[j], stim = s_stim[j];
+            pp[0 * n + j] = pp[0 * n + j] + ab * W.jmins[j] * rcowt;
+            pp[1 * n + j] = pp[1 * n + j] + W.pb[j] / ab * h * rcowt;
             const double diagj = (j > 0 ? term2[j - 1] : 0.0) + term2[j];
-            const double dbdt_t = bnu[q] * freq * hkt[j] / t[j] / fmax(stim[q], 1.0e-30);
-            acc[2][q] = acc[2][q] + ab * (diagj - 1.0) / (1.0 - al * diagj) * (1.0 - al) * dbdt_t * rcowt;
-            acc[3][q] = acc[3][q] + h * rcowt;
-            acc[4][q] = acc[4][q] + W.jnu[j] * rcowt;
-            acc[5][q] = acc[5][q] + h * rcowt;
-            acc[6][q] = acc[6][q] + ab * h * rcowt;
-            const double dbdt_r = bnu[q] * freq * hkt[j] / t[j] / stim[q];
-            acc[7][q] = acc[7][q] + dbdt_r / ab * rcowt;
+            const double dbdt_t = bnu * freq * hkt[j] / t[j] / fmax(stim, 1.0e-30);
+            pp[2 * n + j] = pp[2 * n + j] + ab * (diagj - 1.0) / (1.0 - al * diagj) * (1.0 - al) * dbdt_t * rcowt;
+            pp[3 * n + j] = pp[3 * n + j] + h * rcowt;
+            pp[4 * n + j] = pp[4 * n + j] + W.jnu[j] * rcowt;
+            pp[5 * n + j] = pp[5 * n + j] + h * rcowt;
+            pp[6 * n + j] = pp[6 * n + j] + ab * h * rcowt;
+            const double dbdt_r = bnu * freq * hkt[j] / t[j] / stim;
+            pp[7 * n + j] = pp[7 * n + j] + dbdt_r / ab * rcowt;
         }
         pradk0 = pradk0 + r.knu1 * rcowt;
         wsync();
     }
-    double *pp = part + (size_t)blockIdx.x * 9 * n;
-    int q = 0;
-    for (int j = ln; j < n; j += 64, ++q)
-        for (int a = 0; a < 8; ++a) pp[a * n + j] = acc[a][q];
     if (ln == 0) pp[8 * n] = pradk0;
 }
 
