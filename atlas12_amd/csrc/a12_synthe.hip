@@ -726,7 +726,8 @@ struct TileShape {
 };
 // chunk 0: the barrier-free per-wave form
 constexpr TileShape kTileShapes[] = {{1, 256}, {1, 128}, {1, 512}, {2, 256}, {2, 512},
-                                     {4, 256}, {1, 0},   {2, 0},   {4, 0}};
+                                     {4, 256}, {1, 0},   {2, 0},   {4, 0},   {8, 0},
+                                     {4, 512}, {8, 256}};
 constexpr int kNumTileShapes = sizeof(kTileShapes) / sizeof(kTileShapes[0]);
 
 template <bool F64>
@@ -746,6 +747,9 @@ TileKernel<F64> tile_kernel(int shape)
     case 6: return k_tile_w<F64, 1>;
     case 7: return k_tile_w<F64, 2>;
     case 8: return k_tile_w<F64, 4>;
+    case 9: return k_tile_w<F64, 8>;
+    case 10: return k_tile<F64, 4, 512>;
+    case 11: return k_tile<F64, 8, 256>;
     default: return k_tile<F64, 1, 256>;
     }
 }
@@ -762,6 +766,9 @@ TileBatchKernel<F64> tile_batch_kernel(int shape)
     case 6: return k_tile_batch_w<F64, 1>;
     case 7: return k_tile_batch_w<F64, 2>;
     case 8: return k_tile_batch_w<F64, 4>;
+    case 9: return k_tile_batch_w<F64, 8>;
+    case 10: return k_tile_batch<F64, 4, 512>;
+    case 11: return k_tile_batch<F64, 8, 256>;
     default: return k_tile_batch<F64, 1, 256>;
     }
 }
