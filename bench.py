@@ -272,10 +272,11 @@ def synthe_run(args, ctx, native, wl):
 def large_run(args, ctx, native, wl, shape):
     """The large SYNTHE run: a cool-dwarf structure (T 2800-6045 K, 72
     layers), 300-1000 nm at R=500,000, --large-lines lines (30 % atomic, 70 %
-    in molecular band progressions).  Gather shape: the plain-SYNTHE
-    autotuner's.  HBM rates: bytes of the per-(line, depth) parameter planes
-    the set-up writes and the opacity matrix the gather writes, over the
-    stage times (algorithmic lower bounds, not counter readings)."""
+    in molecular band progressions).  Gather shape: autotuned on 8 of the 72
+    layers of this workload (every shape's gather timed once).  HBM rates:
+    bytes of the per-(line, depth) parameter planes the set-up writes and the
+    opacity matrix the gather writes, over the stage times (algorithmic lower
+    bounds, not counter readings)."""
     from atlas12_amd.state import SyntheState
     base = SyntheState.load(os.path.join(GOLDEN, "synthe_plain_state.npz"))
     grid = wl.Grid.make(300.0, 1000.0, 500000.0)
@@ -283,7 +284,20 @@ def large_run(args, ctx, native, wl, shape):
     t0 = time.perf_counter()
     lines = wl.large_synthe_lines(grid, args.large_lines)
     log(f"large: {len(lines):,} lines generated in {time.perf_counter() - t0:.1f}s")
+    shapes = native.synthe_shapes()
+    ctx.synthe_load(st.depths(list(range(4, st.nrhox, 9))), lines)
+    ctx.synthe_set_shape(shape)
+    ctx.synthe_opacity_run(native.PREC_MIXED)
+    tune = {}
+    for k in range(len(shapes)):
+        ctx.synthe_set_shape(k)
+        ctx.synthe_opacity_run(native.PREC_MIXED)
+        tune[k] = float(ctx.stage_ms()[3])
+    shape = min(tune, key=tune.get)
+    log(f"large: tuned shape {shapes[shape]} on 8 layers")
     res = synthe_block(args, ctx, native, st, lines, 1, 1, shape)
+    res["autotune"] = {"shape_pts_chunk": list(shapes[shape]), "layers_timed": 8,
+                       "gather_ms_per_shape_8_layers": {str(shapes[k]): v for k, v in tune.items()}}
     n, L, N = st.nrhox, st.length, len(lines)
     setup_bytes = N * n * (4 + 4 + 4 * 4)
     opac_bytes = L * n * 4
