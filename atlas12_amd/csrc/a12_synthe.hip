@@ -159,9 +159,8 @@ __global__ void k_line_setup(int nlines, int L, int mw6, double resolu, int nb_s
                              const double *__restrict__ dopple, const double *__restrict__ dscal,
                              const double *__restrict__ hckt, const float *__restrict__ cont32,
                              const double *__restrict__ cont64, int *__restrict__ o_nstep,
-                             int *__restrict__ o_n10, void *__restrict__ o_kap0,
-                             void *__restrict__ o_adamp, void *__restrict__ o_dv,
-                             void *__restrict__ o_xwing, long long *__restrict__ blk_evals)
+                             int *__restrict__ o_n10, void *__restrict__ o_par,
+                             long long *__restrict__ blk_evals)
 {
 #pragma clang fp contract(off)
     using R = typename std::conditional<F64, double, float>::type;
@@ -259,10 +258,13 @@ __global__ void k_line_setup(int nlines, int L, int mw6, double resolu, int nb_s
         }
         o_nstep[o] = nstep;
         o_n10[o] = n10;
-        static_cast<R *>(o_kap0)[o] = kap0;
-        static_cast<R *>(o_adamp)[o] = adamp;
-        static_cast<R *>(o_dv)[o] = dv;
-        static_cast<R *>(o_xwing)[o] = xw;
+        using R4 = typename std::conditional<F64, double4, float4>::type;
+        R4 q;
+        q.x = kap0;
+        q.y = adamp;
+        q.z = dv;
+        q.w = xw;
+        static_cast<R4 *>(o_par)[o] = q;
         // the reference skips lines whose whole window is off the grid (:653)
         if (nstep >= 0 && !(nb + nstep < 1 || nb - nstep > L)) {
             ev = evals_of(nb, nstep, L);
@@ -446,7 +448,7 @@ __global__ void k_tile_ranges(int ntiles, int tw, int L, const int *__restrict__
 struct TileArgs {
     int L, nlines;
     const int *rng, *perm, *nbuffP, *nstep, *n10;
-    const void *kap0, *adamp, *dv, *xwing;
+    const void *par;  // per (depth, line): (kappa0, a, dv, x_wing), float4 or double4
     void *out;
 };
 
@@ -466,10 +468,8 @@ __device__ __forceinline__ void tile_body(const TileArgs &a, int tile, int j)
     const int g0 = tile * kTw + 1;        // first 1-based point
     const int g1 = min(g0 + kTw - 1, L);  // last
     const size_t dj = (size_t)j * a.nlines;
-    const R *k0a = static_cast<const R *>(a.kap0) + dj;
-    const R *aa = static_cast<const R *>(a.adamp) + dj;
-    const R *dva = static_cast<const R *>(a.dv) + dj;
-    const R *xwa = static_cast<const R *>(a.xwing) + dj;
+    using R4 = typename std::conditional<F64, double4, float4>::type;
+    const R4 *pa = static_cast<const R4 *>(a.par) + dj;
 
     Neumaier32 acc32[PTS];
     float part[PTS];  // plain FP32 sum of the current line group
@@ -503,10 +503,11 @@ __device__ __forceinline__ void tile_body(const TileArgs &a, int tile, int j)
                     s_ns[pos] = ns;
                     s_g[pos] = q >> kGroupShift;
                     s_n10[pos] = a.n10[dj + l];
-                    s_k0[pos] = k0a[l];
-                    s_a[pos] = aa[l];
-                    s_dv[pos] = dva[l];
-                    s_xw[pos] = xwa[l];
+                    const R4 f = pa[l];
+                    s_k0[pos] = f.x;
+                    s_a[pos] = f.y;
+                    s_dv[pos] = f.z;
+                    s_xw[pos] = f.w;
                 }
                 if (tid == 0) {
                     int sm = nact;
@@ -607,10 +608,8 @@ __device__ __forceinline__ void tile_body_w(const TileArgs &a, int tile, int j)
     const int w0 = g0 + wv * A12_WAVE * PTS;            // this wave's first point
     const int w1 = min(w0 + A12_WAVE * PTS - 1, g1);    // and last
     const size_t dj = (size_t)j * a.nlines;
-    const R *k0a = static_cast<const R *>(a.kap0) + dj;
-    const R *aa = static_cast<const R *>(a.adamp) + dj;
-    const R *dva = static_cast<const R *>(a.dv) + dj;
-    const R *xwa = static_cast<const R *>(a.xwing) + dj;
+    using R4 = typename std::conditional<F64, double4, float4>::type;
+    const R4 *pa = static_cast<const R4 *>(a.par) + dj;
     Neumaier32 acc32[PTS];
     float part[PTS];  // plain FP32 sum of the current line group
     double acc64[PTS];
@@ -637,7 +636,8 @@ __device__ __forceinline__ void tile_body_w(const TileArgs &a, int tile, int j)
             int cq = c0 + b;
             int cl = __shfl(l, b), cnb = __shfl(nb, b), cns = __shfl(ns, b);
             int cn10 = a.n10[dj + cl];
-            R ck0 = k0a[cl], ca = aa[cl], cdv = dva[cl], cxw = xwa[cl];
+            R4 cf = pa[cl];
+            R ck0 = cf.x, ca = cf.y, cdv = cf.z, cxw = cf.w;
             for (;;) {
                 const bool more = m != 0;
                 int nl_ = 0, nnb = 0, nns = 0, nn10 = 0, nq = 0;
@@ -650,10 +650,11 @@ __device__ __forceinline__ void tile_body_w(const TileArgs &a, int tile, int j)
                     nnb = __shfl(nb, bn);
                     nns = __shfl(ns, bn);
                     nn10 = a.n10[dj + nl_];
-                    nk0 = k0a[nl_];
-                    na = aa[nl_];
-                    ndv = dva[nl_];
-                    nxw = xwa[nl_];
+                    const R4 nf = pa[nl_];
+                    nk0 = nf.x;
+                    na = nf.y;
+                    ndv = nf.z;
+                    nxw = nf.w;
                 }
                 if (!F64 && (cq >> kGroupShift) != grp) {
                     for (int p = 0; p < PTS; ++p) acc32[p].add(part[p]), part[p] = 0.0f;
@@ -894,7 +895,7 @@ struct SynModel {
     double *xnfdop = nullptr, *dscal = nullptr, *cont64 = nullptr;
     float *cont32 = nullptr;
     int *p_nstep = nullptr, *p_n10 = nullptr;
-    void *p_kap0 = nullptr, *p_adamp = nullptr, *p_dv = nullptr, *p_xwing = nullptr;
+    void *p_par = nullptr;
     int *emax = nullptr, *lclass = nullptr, *cls_cnt = nullptr, *cls_off = nullptr,
         *perm = nullptr, *nbuffP = nullptr, *cls_range = nullptr;
     long long *blk_evals = nullptr, *evals_out = nullptr;
@@ -916,10 +917,7 @@ TileArgs tile_args(const SynModel &M)
     a.nbuffP = M.nbuffP;
     a.nstep = M.p_nstep;
     a.n10 = M.p_n10;
-    a.kap0 = M.p_kap0;
-    a.adamp = M.p_adamp;
-    a.dv = M.p_dv;
-    a.xwing = M.p_xwing;
+    a.par = M.p_par;
     a.out = M.opacity;
     return a;
 }
@@ -961,7 +959,7 @@ int launch_prep(a12_ctx *ctx, const SynModel &M, bool f64, int shape, hipStream_
     hipLaunchKernelGGL(setup, dim3(nblkl, n), dim3(256), 0, st, N, L, M.mw6, M.resolu, M.nb_shift,
                        M.l_nbuff, M.l_nelion, M.l_cgf, M.l_elo, M.l_gr, M.l_gs, M.l_gw, M.xnfdop,
                        M.dopple, M.dscal, M.depth + 6 * n, M.cont32, f64 ? M.cont64 : nullptr,
-                       M.p_nstep, M.p_n10, M.p_kap0, M.p_adamp, M.p_dv, M.p_xwing, M.blk_evals);
+                       M.p_nstep, M.p_n10, M.p_par, M.blk_evals);
     A12_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_sum_evals, dim3(1), dim3(256), 0, st, M.blk_evals, nblkl * n,
                        M.evals_out);
@@ -1026,10 +1024,7 @@ void carve_work(SynModel &M, char *base, bool f64, int max_tiles)
     M.cont64 = f64 ? reinterpret_cast<double *>(take(sizeof(double) * (size_t)L * n)) : nullptr;
     M.p_nstep = reinterpret_cast<int *>(take(sizeof(int) * nl * n));
     M.p_n10 = reinterpret_cast<int *>(take(sizeof(int) * nl * n));
-    M.p_kap0 = take(rb * nl * n);
-    M.p_adamp = take(rb * nl * n);
-    M.p_dv = take(rb * nl * n);
-    M.p_xwing = take(rb * nl * n);
+    M.p_par = take(4 * rb * nl * n);
     M.emax = reinterpret_cast<int *>(take(sizeof(int) * nl));
     M.lclass = reinterpret_cast<int *>(take(sizeof(int) * nl));
     M.blk_evals = reinterpret_cast<long long *>(take(2 * sizeof(long long) * (size_t)nblk_lines(N) * n));
