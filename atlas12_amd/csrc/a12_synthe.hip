@@ -158,8 +158,8 @@ __global__ void k_line_setup(int nlines, int L, int mw6, double resolu, int nb_s
                              const float *__restrict__ gw, const double *__restrict__ xnfdop,
                              const double *__restrict__ dopple, const double *__restrict__ dscal,
                              const double *__restrict__ hckt, const float *__restrict__ cont32,
-                             const double *__restrict__ cont64, int *__restrict__ o_nstep,
-                             int *__restrict__ o_n10, void *__restrict__ o_par,
+                             const double *__restrict__ cont64, int2 *__restrict__ o_ns10,
+                             void *__restrict__ o_par,
                              long long *__restrict__ blk_evals)
 {
 #pragma clang fp contract(off)
@@ -256,8 +256,7 @@ __global__ void k_line_setup(int nlines, int L, int mw6, double resolu, int nb_s
                 }
             }
         }
-        o_nstep[o] = nstep;
-        o_n10[o] = n10;
+        o_ns10[o] = make_int2(nstep, n10);
         using R4 = typename std::conditional<F64, double4, float4>::type;
         R4 q;
         q.x = kap0;
@@ -325,13 +324,13 @@ __global__ void k_zero_ll(long long *p, int n)
 }
 
 // Widest extent of each line over all depths, and its extent class.
-__global__ void k_line_emax(int nlines, int nrhox, const int *__restrict__ nstep,
+__global__ void k_line_emax(int nlines, int nrhox, const int2 *__restrict__ ns10,
                             int *__restrict__ emax, int *__restrict__ lclass)
 {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nlines) return;
     int e = -1;
-    for (int j = 0; j < nrhox; ++j) e = max(e, nstep[(size_t)j * nlines + l]);
+    for (int j = 0; j < nrhox; ++j) e = max(e, ns10[(size_t)j * nlines + l].x);
     emax[l] = e;
     lclass[l] = ext_class(e);
 }
@@ -447,7 +446,8 @@ __global__ void k_tile_ranges(int ntiles, int tw, int L, const int *__restrict__
 // --------------------------------------------------------------------------
 struct TileArgs {
     int L, nlines;
-    const int *rng, *perm, *nbuffP, *nstep, *n10;
+    const int *rng, *perm, *nbuffP;
+    const int2 *ns10;  // per (depth, line): (wing half extent nstep, core half width n10)
     const void *par;  // per (depth, line): (kappa0, a, dv, x_wing), float4 or double4
     void *out;
 };
@@ -486,11 +486,13 @@ __device__ __forceinline__ void tile_body(const TileArgs &a, int tile, int j)
             for (int sub = 0; sub < CHUNK; sub += kTile) {  // stage CHUNK candidates
                 const int q = base + sub + tid;
                 bool act = false;
-                int l = 0, nb = 0, ns = -1;
+                int l = 0, nb = 0, ns = -1, n10 = 0;
                 if (sub + tid < CHUNK && q < hi) {
                     l = a.perm[q];
                     nb = a.nbuffP[q];
-                    ns = a.nstep[dj + l];
+                    const int2 q10 = a.ns10[dj + l];
+                    ns = q10.x;
+                    n10 = q10.y;
                     act = ns >= 0 && nb - ns <= g1 && nb + ns >= g0;
                 }
                 const unsigned long long m = __ballot(act);
@@ -502,7 +504,7 @@ __device__ __forceinline__ void tile_body(const TileArgs &a, int tile, int j)
                     s_nb[pos] = nb;
                     s_ns[pos] = ns;
                     s_g[pos] = q >> kGroupShift;
-                    s_n10[pos] = a.n10[dj + l];
+                    s_n10[pos] = n10;
                     const R4 f = pa[l];
                     s_k0[pos] = f.x;
                     s_a[pos] = f.y;
@@ -621,12 +623,14 @@ __device__ __forceinline__ void tile_body_w(const TileArgs &a, int tile, int j)
         const int hi = a.rng[2 * (tile * (A12_NCLASS - 1) + c) + 1];
         for (int c0 = lo; c0 < hi; c0 += A12_WAVE) {
             const int q = c0 + lane;
-            int l = 0, nb = 0, ns = -1;
+            int l = 0, nb = 0, ns = -1, n10 = 0;
             bool act = false;
             if (q < hi) {
                 l = a.perm[q];
                 nb = a.nbuffP[q];
-                ns = a.nstep[dj + l];
+                const int2 q10 = a.ns10[dj + l];
+                ns = q10.x;
+                n10 = q10.y;
                 act = ns >= 0 && nb - ns <= w1 && nb + ns >= w0;
             }
             unsigned long long m = __ballot(act);
@@ -635,7 +639,7 @@ __device__ __forceinline__ void tile_body_w(const TileArgs &a, int tile, int j)
             m &= m - 1;
             int cq = c0 + b;
             int cl = __shfl(l, b), cnb = __shfl(nb, b), cns = __shfl(ns, b);
-            int cn10 = a.n10[dj + cl];
+            int cn10 = __shfl(n10, b);
             R4 cf = pa[cl];
             R ck0 = cf.x, ca = cf.y, cdv = cf.z, cxw = cf.w;
             for (;;) {
@@ -649,7 +653,7 @@ __device__ __forceinline__ void tile_body_w(const TileArgs &a, int tile, int j)
                     nl_ = __shfl(l, bn);
                     nnb = __shfl(nb, bn);
                     nns = __shfl(ns, bn);
-                    nn10 = a.n10[dj + nl_];
+                    nn10 = __shfl(n10, bn);
                     const R4 nf = pa[nl_];
                     nk0 = nf.x;
                     na = nf.y;
@@ -894,7 +898,7 @@ struct SynModel {
                 *l_gw = nullptr;
     double *xnfdop = nullptr, *dscal = nullptr, *cont64 = nullptr;
     float *cont32 = nullptr;
-    int *p_nstep = nullptr, *p_n10 = nullptr;
+    int2 *p_ns10 = nullptr;
     void *p_par = nullptr;
     int *emax = nullptr, *lclass = nullptr, *cls_cnt = nullptr, *cls_off = nullptr,
         *perm = nullptr, *nbuffP = nullptr, *cls_range = nullptr;
@@ -915,8 +919,7 @@ TileArgs tile_args(const SynModel &M)
     a.rng = M.cls_range;
     a.perm = M.perm;
     a.nbuffP = M.nbuffP;
-    a.nstep = M.p_nstep;
-    a.n10 = M.p_n10;
+    a.ns10 = M.p_ns10;
     a.par = M.p_par;
     a.out = M.opacity;
     return a;
@@ -959,13 +962,13 @@ int launch_prep(a12_ctx *ctx, const SynModel &M, bool f64, int shape, hipStream_
     hipLaunchKernelGGL(setup, dim3(nblkl, n), dim3(256), 0, st, N, L, M.mw6, M.resolu, M.nb_shift,
                        M.l_nbuff, M.l_nelion, M.l_cgf, M.l_elo, M.l_gr, M.l_gs, M.l_gw, M.xnfdop,
                        M.dopple, M.dscal, M.depth + 6 * n, M.cont32, f64 ? M.cont64 : nullptr,
-                       M.p_nstep, M.p_n10, M.p_par, M.blk_evals);
+                       M.p_ns10, M.p_par, M.blk_evals);
     A12_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_sum_evals, dim3(1), dim3(256), 0, st, M.blk_evals, nblkl * n,
                        M.evals_out);
     A12_CHECK_LAUNCH();
     if (ev_setup) A12_HIP(hipEventRecord(ev_setup, st));
-    hipLaunchKernelGGL(k_line_emax, dim3(nblkl), dim3(256), 0, st, N, n, M.p_nstep, M.emax,
+    hipLaunchKernelGGL(k_line_emax, dim3(nblkl), dim3(256), 0, st, N, n, M.p_ns10, M.emax,
                        M.lclass);
     A12_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_cls_count, dim3(nblk), dim3(kPartBlock), 0, st, N, M.lclass, M.cls_cnt);
@@ -995,7 +998,7 @@ size_t work_bytes(int n, int L, int N, int mw6, bool f64, int max_tiles)
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     size_t b = al(sizeof(double) * mw6 * n) + al(sizeof(double) * 4 * n) +
                al(sizeof(float) * (size_t)L * n) + (f64 ? al(sizeof(double) * (size_t)L * n) : 0);
-    b += 2 * al(sizeof(int) * nl * n) + 4 * al(rb * nl * n);
+    b += al(sizeof(int2) * nl * n) + al(4 * rb * nl * n);  // (nstep, n10), profile records
     b += 2 * al(sizeof(int) * nl);
     b += al(2 * sizeof(long long) * (size_t)nblk_lines(N) * n) + al(2 * sizeof(long long));
     b += al(sizeof(int) * (size_t)A12_NCLASS * nblk_part(N)) +
@@ -1022,8 +1025,7 @@ void carve_work(SynModel &M, char *base, bool f64, int max_tiles)
     M.dscal = reinterpret_cast<double *>(take(sizeof(double) * 4 * n));
     M.cont32 = reinterpret_cast<float *>(take(sizeof(float) * (size_t)L * n));
     M.cont64 = f64 ? reinterpret_cast<double *>(take(sizeof(double) * (size_t)L * n)) : nullptr;
-    M.p_nstep = reinterpret_cast<int *>(take(sizeof(int) * nl * n));
-    M.p_n10 = reinterpret_cast<int *>(take(sizeof(int) * nl * n));
+    M.p_ns10 = reinterpret_cast<int2 *>(take(sizeof(int2) * nl * n));
     M.p_par = take(4 * rb * nl * n);
     M.emax = reinterpret_cast<int *>(take(sizeof(int) * nl));
     M.lclass = reinterpret_cast<int *>(take(sizeof(int) * nl));
