@@ -41,7 +41,7 @@ int a12_ctx_destroy(a12_ctx *ctx)
     DevBuf *bufs[] = {&S.wledge, &S.continall, &S.contabs, &S.contscat, &S.xnfpel, &S.dopple,
                       &S.depth, &S.l_nbuff, &S.l_nelion, &S.l_cgf, &S.l_elo, &S.l_gr, &S.l_gs,
                       &S.l_gw, &S.work, &S.opacity, &S.hline, &S.hcont, &ctx->coefj, &ctx->coefh,
-                      &ctx->jb_in, &ctx->jb_out, &ctx->rt_part};
+                      &ctx->jb_in, &ctx->jb_out, &ctx->sv_a, &ctx->sv_b, &ctx->rt_part};
     AtlasDev &A = ctx->atl;
     DevBuf *abufs[] = {&A.depth3, &A.depth_lo, &A.tabcont, &A.xnfdop, &A.dopple, &A.waveset,
                        &A.rcoset, &A.tabs, &A.tabs32, &A.l_nu, &A.l_nucont, &A.l_nel, &A.l_const,
@@ -228,6 +228,19 @@ int a12_josh_batch(a12_ctx *ctx, int32_t ifscat, int32_t ifsurf, int32_t nrhox, 
     const double *in[7] = {acont, aline, sigmac, sigmal, scont, sline, bnu};
     double *out[5] = {taunu, snu, jnu, hnu, jmins};
     return a12i_josh_batch(ctx, ifscat, ifsurf, nrhox, nbatch, rhox, in, hnu1, knu1, out);
+}
+
+int a12_solvit_batched(a12_ctx *ctx, int32_t n, int32_t nsys, const double *a, const double *b,
+                       double *x)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    if (n < 1 || n > 64 || nsys < 0 || (nsys > 0 && (!a || !b || !x))) {
+        ctx->err = "a12_solvit_batched: bad arguments (1 <= n <= 64, nsys >= 0)";
+        return A12_ERR_ARG;
+    }
+    if (nsys == 0) return A12_OK;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_solvit_batched(ctx, n, nsys, a, b, x);
 }
 
 int a12_atlas_load(a12_ctx *ctx, const a12_atlas_atm *atm, const int32_t *linedata,

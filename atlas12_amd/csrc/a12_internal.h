@@ -100,8 +100,8 @@ struct a12_ctx {
     MlpDev *mlp = nullptr;
     SynBatch *sbatch = nullptr;
     DevBuf rt_part;
-    // scratch for a12_josh_batch
-    DevBuf jb_in, jb_out;
+    // scratch for a12_josh_batch and a12_solvit_batched
+    DevBuf jb_in, jb_out, sv_a, sv_b;
     // stage timing (a12_set_profiling): events on the context stream
     bool prof = false;
     hipEvent_t ev[16] = {};  // 0-6 SYNTHE stages, 8-11 LINOP1 stages, 12-13 RT pass
@@ -178,6 +178,8 @@ int a12i_synthe_transfer_launch(a12_ctx *ctx, hipStream_t st, bool f64, int L, i
                                 double *hcont);
 int a12i_atlas_tcorr3(a12_ctx *ctx, const a12_tcorr_in *in, double *rdabh, double *oldt1,
                       double *out);
+int a12i_solvit_batched(a12_ctx *ctx, int n, int nsys, const double *a, const double *b,
+                        double *x);
 int a12i_scale_model(a12_ctx *ctx, int nmodel, int n, int kw, double tau1lg, double steplg,
                      const double *sc, const double *fields, double *out, double *so,
                      double *taustd_out);

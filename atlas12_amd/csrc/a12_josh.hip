@@ -1012,6 +1012,56 @@ int a12i_synthe_transfer(a12_ctx *ctx)
     return A12_OK;
 }
 
+// ---- SOLVIT, batched ------------------------------------------------------
+// One wavefront per system: n = 51 (JOSH's order) through the
+// register-resident form, any other n through the LDS form.
+template <bool REG>
+__global__ __launch_bounds__(64) void k_solvit_batch(int n, const double *__restrict__ a,
+                                                     const double *__restrict__ b,
+                                                     double *__restrict__ x)
+{
+#pragma clang fp contract(off)
+    const int ln = lane_id();
+    const size_t sys = blockIdx.x;
+    const double *as = a + sys * (size_t)n * n;
+    if (REG) {
+        double arow[NX];
+#pragma unroll
+        for (int m = 0; m < NX; ++m) arow[m] = ln < NX ? as[(size_t)m * NX + ln] : 0.0;
+        int row;
+        const double v = solvit_reg<NX>(arow, ln < NX ? b[sys * NX + ln] : 0.0, &row);
+        if (row >= 1) x[sys * NX + row - 1] = v;
+    } else {
+        __shared__ double A[64 * 64], B[64];
+        __shared__ int ipiv[64];
+        for (int i = ln; i < n * n; i += 64) A[i] = as[i];
+        if (ln < n) B[ln] = b[sys * n + ln];
+        wsync();
+        solvit_lds(A, B, ipiv, n);
+        if (ln < n) x[sys * n + ln] = B[ln];
+    }
+}
+
+int a12i_solvit_batched(a12_ctx *ctx, int n, int nsys, const double *a, const double *b,
+                        double *x)
+{
+    hipStream_t s = ctx->stream;
+    const size_t na = (size_t)nsys * n * n, nb = (size_t)nsys * n;
+    A12_HIP(ctx->sv_a.reserve(sizeof(double) * na));
+    A12_HIP(ctx->sv_b.reserve(sizeof(double) * 2 * nb));
+    double *da = ctx->sv_a.as<double>(), *db = ctx->sv_b.as<double>(), *dx = db + nb;
+    A12_HIP(hipMemcpyAsync(da, a, sizeof(double) * na, hipMemcpyHostToDevice, s));
+    A12_HIP(hipMemcpyAsync(db, b, sizeof(double) * nb, hipMemcpyHostToDevice, s));
+    if (n == NX)
+        hipLaunchKernelGGL(k_solvit_batch<true>, dim3(nsys), dim3(64), 0, s, n, da, db, dx);
+    else
+        hipLaunchKernelGGL(k_solvit_batch<false>, dim3(nsys), dim3(64), 0, s, n, da, db, dx);
+    A12_CHECK_LAUNCH();
+    A12_HIP(hipMemcpyAsync(x, dx, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+    A12_HIP(hipStreamSynchronize(s));
+    return A12_OK;
+}
+
 int a12i_josh_batch(a12_ctx *ctx, int ifscat, int ifsurf, int nrhox, int nbatch,
                     const double *rhox, const double *const in[7], double *hnu1, double *knu1,
                     double *const out[5])

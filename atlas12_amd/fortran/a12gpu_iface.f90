@@ -159,6 +159,15 @@ MODULE a12gpu_iface
                                      jmins(*)
     END FUNCTION
 
+    INTEGER(C_INT32_T) FUNCTION a12_solvit_batched(ctx, n, nsys, a, b, x) &
+        BIND(C, NAME='a12_solvit_batched')
+      IMPORT :: C_INT32_T, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT32_T), VALUE :: n, nsys
+      REAL(C_DOUBLE), INTENT(IN) :: a(*), b(*)
+      REAL(C_DOUBLE), INTENT(OUT) :: x(*)
+    END FUNCTION
+
     ! ---- ATLAS12 ----------------------------------------------------------
     INTEGER(C_INT32_T) FUNCTION a12_atlas_select_lines(ctx, atm, wavetab, pred, npred, mol2, &
         nmol2, linedata_out, cap, nkept) BIND(C, NAME='a12_atlas_select_lines')

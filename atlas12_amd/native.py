@@ -39,7 +39,7 @@ EXPORTS = ["a12_ctx_create", "a12_ctx_destroy", "a12_last_error", "a12_stream",
            "a12_molec_set_table", "a12_molec_newton_batched",
            "a12_synthe_shape_info", "a12_synthe_set_shape", "a12_synthe_batch_load",
            "a12_synthe_batch_set_budget", "a12_synthe_batch_run", "a12_synthe_batch_fetch",
-           "a12_synthe_batch_info"]
+           "a12_synthe_batch_info", "a12_solvit_batched"]
 
 
 class A12Error(RuntimeError):
@@ -154,6 +154,21 @@ class Context:
             "a12_josh_batch")
         return dict(hnu1=hnu1, knu1=knu1, taunu=outs[0], snu=outs[1], jnu=outs[2],
                     hnu=outs[3], jmins=outs[4])
+
+    def solvit_batched(self, a, b):
+        """SOLVIT on the GPU for a batch of systems: a (nsys, n, n) with
+        a[s, i, j] = A(i+1, j+1), b (nsys, n); returns x (nsys, n)."""
+        a = np.asarray(a, np.float64)
+        b = np.asarray(b, np.float64)
+        nsys, n = b.shape
+        if a.shape != (nsys, n, n):
+            raise A12Error("solvit_batched: a must be (nsys, n, n) for b (nsys, n)")
+        acm = np.ascontiguousarray(a.transpose(0, 2, 1))  # column-major per system
+        bb = np.ascontiguousarray(b)
+        x = np.zeros((nsys, n))
+        self._check(self.lib.a12_solvit_batched(self.h, C.c_int32(n), C.c_int32(nsys), _ptr(acm),
+                                                _ptr(bb), _ptr(x)), "a12_solvit_batched")
+        return x
 
     def voigt(self, v, a, prec: int = 0):
         """H(v, a) on the GPU: prec 0 mixed FP32 path, 1 FP64, 2 the

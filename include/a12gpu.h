@@ -173,6 +173,17 @@ int a12_josh_batch(a12_ctx *ctx, int32_t ifscat, int32_t ifsurf,
                    double *taunu, double *snu, double *jnu, double *hnu,
                    double *jmins);
 
+/* ---- SOLVIT, batched ---------------------------------------------------- */
+/* SOLVIT(A, N, B, IPIVOT), src/atlas12_modules.f90:5739-5847, on nsys
+ * independent systems of order n (1 <= n <= 64): a is (n, n, nsys), each
+ * system column-major as the reference holds A; b is (n, nsys); x (n, nsys)
+ * receives the solutions (b is not modified).  It runs the device forms
+ * that JOSH (n = 51, the system in registers) and NMOLEC (the system in LDS)
+ * use; both equal the serial routine bit for bit.  Host pointers
+ * (synchronous). */
+int a12_solvit_batched(a12_ctx *ctx, int32_t n, int32_t nsys, const double *a,
+                       const double *b, double *x);
+
 /* ---- ATLAS12 opacity sampling ----------------------------------------- */
 /* State LINOP1 and the RUN_RT_PASS frequency loop read (module arrays of
  * mod_atlas_data, src/atlas12_modules.f90:1858-1919, and the driver's grid,
