@@ -783,16 +783,29 @@ __global__ __launch_bounds__(64) void k_rt_pass(
     if (ln == 0) pp[8 * n] = pradk0;
 }
 
-// Fixed-order combination of the chunk partials: out(9, n).
-__global__ void k_rt_reduce(int n, int nchunk, const double *__restrict__ part,
-                            double *__restrict__ out)
+// Fixed-order combination of the chunk partials: out(9, n), one workgroup
+// per element i = a * n + j.  Thread t sums chunks t, t + 256, ... in order,
+// then a fixed tree over the 256 partial sums.
+__global__ __launch_bounds__(256) void k_rt_reduce(int n, int nchunk,
+                                                   const double *__restrict__ part,
+                                                   double *__restrict__ out)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // a * n + j
-    if (i >= 9 * n) return;
+    __shared__ double red[256];
+    const int i = blockIdx.x, t = threadIdx.x;
     double s = 0.0;
-    for (int c = 0; c < nchunk; ++c) s += part[(size_t)c * 9 * n + i];
-    out[i] = s;
+    for (int c = t; c < nchunk; c += 256) s += part[(size_t)c * 9 * n + i];
+    red[t] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (t < k) red[t] += red[t + k];
+        __syncthreads();
+    }
+    if (t == 0) out[i] = red[0];
 }
+
+// Frequencies per RT-pass wavefront: small chunks balance the uneven
+// per-frequency cost (the deep branch's sweeps) over the resident waves.
+constexpr int kRtChunk = 2;
 
 // ---- SCALE_MODEL ----------------------------------------------------------
 // SCALE_MODEL(TEFF_NEW, LOGG_NEW), src/atlas12_modules.f90:6967-7060, one
@@ -1109,7 +1122,7 @@ int a12i_atlas_rt_pass(a12_ctx *ctx, int n, int nf, double teff, const double *d
         ctx->err = "a12_atlas_rt_pass: call a12_set_operators first";
         return A12_ERR_STATE;
     }
-    const int chunk = std::max(1, (nf + 4095) / 4096);
+    const int chunk = kRtChunk;
     const int nchunk = (nf + chunk - 1) / chunk;
     A12_HIP(ctx->rt_part.reserve(sizeof(double) * 9 * (size_t)n * nchunk));
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[12], ctx->stream));
@@ -1117,8 +1130,8 @@ int a12i_atlas_rt_pass(a12_ctx *ctx, int n, int nf, double teff, const double *d
                        d_wave, d_rcoset, d_depth3, d_kapp, d_xlines, ctx->coefj.as<double>(),
                        ctx->coefh.as<double>(), d_hnu1, ctx->rt_part.as<double>());
     A12_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_rt_reduce, dim3((9 * n + 255) / 256), dim3(256), 0, ctx->stream, n,
-                       nchunk, ctx->rt_part.as<double>(), d_acc);
+    hipLaunchKernelGGL(k_rt_reduce, dim3(9 * n), dim3(256), 0, ctx->stream, n, nchunk,
+                       ctx->rt_part.as<double>(), d_acc);
     A12_CHECK_LAUNCH();
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[13], ctx->stream));
     return A12_OK;
