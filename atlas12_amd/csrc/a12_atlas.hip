@@ -376,24 +376,36 @@ __global__ __launch_bounds__(256) void k_lo_linereach(int nl, int n,
                                                       const int *__restrict__ state,
                                                       int *__restrict__ l_reach)
 {
-    // one wavefront per line: coalesced loads of its n states, shuffle max
+    // one wavefront per line, four lines in flight: coalesced loads of each
+    // line's n states (n <= 128), shuffle max
+    constexpr int kLines = 4;
     const int lane = threadIdx.x & 63;
     const int nw = gridDim.x * (blockDim.x >> 6);
-    for (int l = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); l < nl; l += nw) {
-        int r = 0, b = 0;
-        const int *st = state + (size_t)l * n;
-        for (int j = lane; j < n; j += 64) {
-            const int v = st[j];
-            if (v < 0) {
-                r = max(r, v & 0xff);
-                b = max(b, (v >> 8) & 0xff);
+    for (int l0 = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * kLines; l0 < nl;
+         l0 += nw * kLines) {
+        int v[kLines][2];
+#pragma unroll
+        for (int u = 0; u < kLines; ++u) {
+            const int l = l0 + u;
+            const int *st = state + (size_t)l * n;
+            v[u][0] = l < nl && lane < n ? st[lane] : 0;
+            v[u][1] = l < nl && lane + 64 < n ? st[lane + 64] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kLines; ++u) {
+            int r = 0, b = 0;
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                if (v[u][h] < 0) {
+                    r = max(r, v[u][h] & 0xff);
+                    b = max(b, (v[u][h] >> 8) & 0xff);
+                }
+            for (int o = 32; o > 0; o >>= 1) {
+                r = max(r, __shfl_xor(r, o, 64));
+                b = max(b, __shfl_xor(b, o, 64));
             }
+            if (lane == 0 && l0 + u < nl) l_reach[l0 + u] = r | (b << 8);
         }
-        for (int o = 32; o > 0; o >>= 1) {
-            r = max(r, __shfl_xor(r, o, 64));
-            b = max(b, __shfl_xor(b, o, 64));
-        }
-        if (lane == 0) l_reach[l] = r | (b << 8);
     }
 }
 
