@@ -988,7 +988,7 @@ int a12i_synthe_transfer_launch(a12_ctx *ctx, hipStream_t st, bool f64, int L, i
                                 const double *depth, const void *opacity, double *hline,
                                 double *hcont)
 {
-    const int grid = std::min(L, 4096);
+    const int grid = L;  // one wavefront per wavelength: the uneven JOSH cost balances best
     auto k = f64 ? k_synthe_transfer<true> : k_synthe_transfer<false>;
     hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, st, L, n, wbegin, ratio, nedge, wledge, nf,
                        contabs, contscat, depth, opacity, ctx->coefj.as<double>(),
