@@ -17,7 +17,7 @@
 //                 lines through LDS in NU-sorted store order (for a
 //                 wavelength-sorted store, the reference's own line
 //                 order).  No atomics.
-// MIXED evaluates the profile in FP32 and sums with Neumaier compensation;
+// MIXED evaluates the profile in FP32 and sums with Kahan compensation;
 // FP64 repeats the reference's arithmetic.
 
 #include <hip/hip_runtime.h>
@@ -470,8 +470,11 @@ __device__ __forceinline__ void lo_accum(const LoPar &q, double d, int slot, dou
     } else {
         x = lo_cv_mixed(c32, v, adamp, f0, f1, f2);
     }
-    const float s0 = acc_s[slot], tt = s0 + x;
-    acc_c[slot] += fabsf(s0) >= fabsf(x) ? (s0 - tt) + x : (x - tt) + s0;
+    // Kahan's compensated sum (acc_c holds the running compensation): every
+    // contribution is positive, so its bound, 2 eps times the sum, holds
+    // with four operations
+    const float s0 = acc_s[slot], y = x - acc_c[slot], tt = s0 + y;
+    acc_c[slot] = (tt - s0) - y;
     acc_s[slot] = tt;
 }
 
@@ -651,7 +654,7 @@ __global__ __launch_bounds__(kLoBlock) void k_lo_gather(
     if (live)
         for (int iw = iw0; iw <= iw1; ++iw) {
             const int q = (iw - iw0) * kLoBlock + t;
-            xlines[(size_t)iw * n + j] = F64 ? acc_d[q] : (double)acc_s[q] + (double)acc_c[q];
+            xlines[(size_t)iw * n + j] = F64 ? acc_d[q] : (double)acc_s[q] - (double)acc_c[q];
         }
 }
 
@@ -760,7 +763,7 @@ __global__ __launch_bounds__(kLoBlock) void k_lo_gather_w(
     if (live)
         for (int iw = iw0; iw <= iw1; ++iw) {
             const int q = (iw - iw0) * kLoBlock + t;
-            xlines[(size_t)iw * n + j] = F64 ? acc_d[q] : (double)acc_s[q] + (double)acc_c[q];
+            xlines[(size_t)iw * n + j] = F64 ? acc_d[q] : (double)acc_s[q] - (double)acc_c[q];
         }
 }
 
