@@ -478,6 +478,64 @@ __device__ __forceinline__ void lo_accum(const LoPar &q, double d, int slot, dou
     acc_s[slot] = tt;
 }
 
+// MIXED: the items of one side of one line for this lane, walked away from
+// the centre (red: iw = i0, i0 + 1, ..., i1; blue: iw = i0, i0 - 1, ..., i1),
+// so that v grows along the walk.  Once v is in a tail regime (a <= 0.2 and
+// v > 10: the Lorentz wing; a > 0.2 outside a <= 1.4 and a + v <= 3.2:
+// VOIGT's asymptote) every later item is too, and the rest of the side runs
+// without regime tests.  Each item is the value lo_accum adds, with the same
+// operations: the sums are those of lo_accum's loop.
+__device__ __forceinline__ void lo_side_mixed(const LoPar &q, const double *w_t, double wl,
+                                              int i0, int i1, int dir, int iw0, int stride, int t,
+                                              float *acc_s, float *acc_c, const float *f0,
+                                              const float *f1, const float *f2)
+{
+#pragma clang fp contract(off)
+    const double adamp = q.p1;
+    const float c32 = __int_as_float(__double2hiint(q.p2)), k32 = __int_as_float(__double2loint(q.p2));
+    const int n_items = dir > 0 ? i1 - i0 + 1 : i0 - i1 + 1;
+    int k = 0;
+    for (; k < n_items; ++k) {
+        const int iw = i0 + dir * k;
+        const double d = dir > 0 ? w_t[iw] - wl : wl - w_t[iw], v = d * q.p0;
+        if (adamp <= 0.20 ? v > 10.0 : !(adamp <= 1.4 && adamp + v <= 3.2)) break;
+        const float x = lo_cv_mixed(c32, v, adamp, f0, f1, f2);
+        const int slot = (iw - iw0) * stride + t;
+        const float s0 = acc_s[slot], y = x - acc_c[slot], tt = s0 + y;
+        acc_c[slot] = (tt - s0) - y;
+        acc_s[slot] = tt;
+    }
+    if (adamp <= 0.20) {
+        for (; k < n_items; ++k) {  // Lorentz tail: K / v^2
+            const int iw = i0 + dir * k;
+            const double d = dir > 0 ? w_t[iw] - wl : wl - w_t[iw], v = d * q.p0;
+            const float vf = (float)v;
+            const float x = k32 * __builtin_amdgcn_rcpf(vf * vf);
+            const int slot = (iw - iw0) * stride + t;
+            const float s0 = acc_s[slot], y = x - acc_c[slot], tt = s0 + y;
+            acc_c[slot] = (tt - s0) - y;
+            acc_s[slot] = tt;
+        }
+    } else {
+        const float a32 = (float)adamp, aa32 = a32 * a32;
+        for (; k < n_items; ++k) {  // VOIGT's asymptotic form with one reciprocal
+            const int iw = i0 + dir * k;
+            const double d = dir > 0 ? w_t[iw] - wl : wl - w_t[iw], v = d * q.p0;
+            const float vf = (float)v, vv = vf * vf;
+            const float ru = __builtin_amdgcn_rcpf((aa32 + vv) * 1.4142f);
+            const float aau = aa32 * ru, vvu = vv * ru;
+            const float r = a32 * 0.79788f * ru;
+            const float x = c32 * (adamp <= 100.0 ? ((((aau - 10.0f * vvu) * aau * 3.0f + 15.0f * vvu * vvu) +
+                                                      3.0f * vv - aa32) * (ru * ru) + 1.0f) * r
+                                                  : r);
+            const int slot = (iw - iw0) * stride + t;
+            const float s0 = acc_s[slot], y = x - acc_c[slot], tt = s0 + y;
+            acc_c[slot] = (tt - s0) - y;
+            acc_s[slot] = tt;
+        }
+    }
+}
+
 __device__ __forceinline__ LoPar lo_shfl(const LoPar &q, int src)
 {
     LoPar r;
@@ -741,12 +799,19 @@ __global__ __launch_bounds__(kLoBlock) void k_lo_gather_w(
                     const int ba = max(k - bn, iw0), bb = min(k - 1, iw1);
                     if (ra <= rb || ba <= bb) {
                         const LoPar qp = lo_par_of<F64>(p0, p1, p2);
-                        for (int iw = ra; iw <= rb; ++iw)
-                            lo_accum<F64>(qp, w_t[iw] - wl, (iw - iw0) * kLoBlock + t, acc_d,
-                                          acc_s, acc_c, h0, h1, h2, f0, f1, f2);
-                        for (int iw = ba; iw <= bb; ++iw)
-                            lo_accum<F64>(qp, wl - w_t[iw], (iw - iw0) * kLoBlock + t, acc_d,
-                                          acc_s, acc_c, h0, h1, h2, f0, f1, f2);
+                        if (F64) {
+                            for (int iw = ra; iw <= rb; ++iw)
+                                lo_accum<F64>(qp, w_t[iw] - wl, (iw - iw0) * kLoBlock + t, acc_d,
+                                              acc_s, acc_c, h0, h1, h2, f0, f1, f2);
+                            for (int iw = ba; iw <= bb; ++iw)
+                                lo_accum<F64>(qp, wl - w_t[iw], (iw - iw0) * kLoBlock + t, acc_d,
+                                              acc_s, acc_c, h0, h1, h2, f0, f1, f2);
+                        } else {
+                            lo_side_mixed(qp, w_t, wl, ra, rb, 1, iw0, kLoBlock, t, acc_s, acc_c,
+                                          f0, f1, f2);
+                            lo_side_mixed(qp, w_t, wl, bb, ba, -1, iw0, kLoBlock, t, acc_s,
+                                          acc_c, f0, f1, f2);
+                        }
                     }
                 }
                 if (!more) break;
