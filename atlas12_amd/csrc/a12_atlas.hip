@@ -159,7 +159,8 @@ __device__ __forceinline__ double lo_tail(double center, double v, double adamp)
 // ~1e-9 Doppler widths.
 __device__ int lo_reach(int first, int last, int step, const double *__restrict__ waveset,
                         double wlvac, double dopwave, double center, double adamp, double tc,
-                        const double *h0, const double *h1, const double *h2, float lg2inv)
+                        double rtc, float rdf, const double *h0, const double *h1,
+                        const double *h2, float lg2inv)
 {
 #pragma clang fp contract(off)
     int cnt = 0;
@@ -178,10 +179,10 @@ __device__ int lo_reach(int first, int last, int step, const double *__restrict_
     // (relative error below ~1e-6), decides when it is more than 1e-4 away
     // from 1; otherwise the reference's FP64 expression does.  Either way the
     // answer is the FP64 predicate's.
-    const double rtc = 1.0 / tc;
+    // rtc = 1 / tc, rdf = (float)(1 / dopwave), shared by both sides
     const float qf = (float)(center * 0.5641895835477563 * adamp * rtc);  // Lorentz: qf / v^2
     const float cf = (float)(center * adamp * 0.79788 * rtc);  // asymptote: cf / u * (1 + ...)
-    const float af = (float)adamp, aaf = af * af, rdf = (float)(1.0 / dopwave);
+    const float af = (float)adamp, aaf = af * af;
     auto below = [&](int pos) {
         const int pw = first + pos * step;
         const double d = step > 0 ? waveset[pw] - wlvac : wlvac - waveset[pw];
@@ -321,10 +322,12 @@ __global__ __launch_bounds__(256) void k_lo_setup(
             const double adamp = (gr + gs * dep[2 * n + j] + gw * dep[5 * n + j]) / dop;
             const double dopwave = dop * wlvac;
             // red: IW = NU .. min(NU+100, NUMNU); blue: NU-1 down, at most 100
+            const double rtc = 1.0 / tc;
+            const float rdf = (float)(1.0 / dopwave);
             const int rn = lo_reach(nu, min(nu + kMaxWing, numnu - 1), 1, waveset, wlvac, dopwave,
-                                    center, adamp, tc, h0, h1, h2, lg2inv);
+                                    center, adamp, tc, rtc, rdf, h0, h1, h2, lg2inv);
             const int bn = lo_reach(nu - 1, max(nu - kMaxWing, 0), -1, waveset, wlvac, dopwave,
-                                    center, adamp, tc, h0, h1, h2, lg2inv);
+                                    center, adamp, tc, rtc, rdf, h0, h1, h2, lg2inv);
             st = (int)(0x80000000u | (unsigned)rn | ((unsigned)bn << 8));
             o_par[3 * g + 0] = center;
             o_par[3 * g + 1] = adamp;
