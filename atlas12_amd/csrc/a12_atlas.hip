@@ -31,6 +31,9 @@
 #include "a12_common.h"
 #include "a12_internal.h"
 
+// Wavefront sum of the device library (DPP).
+extern "C" __device__ long long __ockl_wfred_add_i64(long long);
+
 namespace {
 
 constexpr int kMaxWing = 100;
@@ -336,14 +339,13 @@ __global__ __launch_bounds__(256) void k_lo_setup(
         }
     }
     if (g < (long long)nl * n) o_state[g] = st;
-    __shared__ long long red[256];
-    red[threadIdx.x] = ev;
+    // block total of the contribution counts: a DPP sum per wavefront, then
+    // the four wave sums (integers: exact in any order)
+    __shared__ long long s_wsum[4];
+    const long long wsum = __ockl_wfred_add_i64(ev);
+    if ((threadIdx.x & 63) == 0) s_wsum[threadIdx.x >> 6] = wsum;
     __syncthreads();
-    for (int s = blockDim.x >> 1; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) blk_evals[blockIdx.x] = red[0];
+    if (threadIdx.x == 0) blk_evals[blockIdx.x] = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
 }
 
 // Sum of nb per-block counts in a fixed order: pass 1, block b of gridDim.x
