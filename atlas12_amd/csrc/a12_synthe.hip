@@ -29,6 +29,9 @@
 #include "a12_common.h"
 #include "a12_internal.h"
 
+// Wavefront sum of the device library (DPP).
+extern "C" __device__ long long __ockl_wfred_add_i64(long long);
+
 namespace {
 
 constexpr int kTile = 256;       // grid points per tile (one per thread)
@@ -270,23 +273,20 @@ __global__ void k_line_setup(int nlines, int L, int mw6, double resolu, int nb_s
             evc = evals_of(nb, min(nstep, n10), L);  // centre + Voigt core
         }
     }
-    // block reduction of the counts in a fixed order
-    __shared__ long long red[256], redc[256];
-    red[threadIdx.x] = ev;
-    redc[threadIdx.x] = evc;
-    __syncthreads();
-    for (int s = blockDim.x >> 1; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            red[threadIdx.x] += red[threadIdx.x + s];
-            redc[threadIdx.x] += redc[threadIdx.x + s];
-        }
-        __syncthreads();
+    // block totals of the counts: a DPP sum per wavefront, then the four wave
+    // sums (integers: exact in any order)
+    __shared__ long long s_w[4], s_wc[4];
+    const long long ws = __ockl_wfred_add_i64(ev), wc = __ockl_wfred_add_i64(evc);
+    if ((threadIdx.x & 63) == 0) {
+        s_w[threadIdx.x >> 6] = ws;
+        s_wc[threadIdx.x >> 6] = wc;
     }
+    __syncthreads();
     if (threadIdx.x == 0) {
         const size_t b = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
         const size_t nb_all = (size_t)gridDim.x * gridDim.y;
-        blk_evals[b] = red[0];
-        blk_evals[nb_all + b] = redc[0];
+        blk_evals[b] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        blk_evals[nb_all + b] = s_wc[0] + s_wc[1] + s_wc[2] + s_wc[3];
     }
 }
 
