@@ -11,11 +11,13 @@
 //   k_lo_setup    one lane per (line, depth) applies the reference's
 //                 coarse (every 8th) / fine depth rule and fixes kappa_centre,
 //                 a, the Doppler width in nm and the red/blue reach (FP64
-//                 decisions, as the reference; the Lorentz tail by bisection);
-//   k_lo_gather   tiles of 8 grid points x all depths per block: the sum of
-//                 every reaching line's profile, streaming the candidate
-//                 lines through LDS in NU-sorted store order (for a
-//                 wavelength-sorted store, the reference's own line
+//                 decisions, as the reference; in the tails a search started
+//                 at the closed-form crossing);
+//   k_lo_linereach  each line's widest reach over its depths;
+//   k_lo_gather   tiles of kLoPts grid points x all depths per block (staged
+//   k_lo_gather_w through LDS, or per wavefront without barriers): the sum
+//                 of every reaching line's profile in NU-sorted store order
+//                 (for a wavelength-sorted store, the reference's own line
 //                 order).  No atomics.
 // MIXED evaluates the profile in FP32 and sums with Kahan compensation;
 // FP64 repeats the reference's arithmetic.
