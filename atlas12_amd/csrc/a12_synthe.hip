@@ -664,21 +664,34 @@ __device__ __forceinline__ void tile_body_w(const TileArgs &a, int tile, int j)
                     for (int p = 0; p < PTS; ++p) acc32[p].add(part[p]), part[p] = 0.0f;
                     grp = cq >> kGroupShift;
                 }
+                if (!F64 && (cnb + cn10 < w0 || cnb - cn10 > w1)) {
+                    // no core point of this line in the wave's points: every item
+                    // is an n^-2 wing term; points outside its extent add exactly 0
+                    const float xw = (float)cxw;
 #pragma unroll
-                for (int p = 0; p < PTS; ++p) {
-                    const int gi = w0 + p * A12_WAVE + lane;
-                    const int n = abs(gi - cnb);
-                    if (gi > w1 || n > cns) continue;
-                    if (!F64) {
-                        float v;
-                        if (n <= cn10) v = (float)ck0 * voigt_mixed((float)n * (float)cdv, (float)ca);
-                        else v = (float)cxw * __builtin_amdgcn_rcpf((float)n * (float)n);
-                        part[p] += v;
-                    } else {
-                        double v;
-                        if (n <= cn10) v = (double)ck0 * voigt_f64((double)n * (double)cdv, (double)ca);
-                        else v = (double)cxw / ((double)n * (double)n);
-                        acc64[p] += v;
+                    for (int p = 0; p < PTS; ++p) {
+                        const int gi = w0 + p * A12_WAVE + lane;
+                        const int n = abs(gi - cnb);
+                        const float w = xw * __builtin_amdgcn_rcpf((float)n * (float)n);
+                        part[p] += (gi <= w1 && n <= cns) ? w : 0.0f;
+                    }
+                } else {
+#pragma unroll
+                    for (int p = 0; p < PTS; ++p) {
+                        const int gi = w0 + p * A12_WAVE + lane;
+                        const int n = abs(gi - cnb);
+                        if (gi > w1 || n > cns) continue;
+                        if (!F64) {
+                            float v;
+                            if (n <= cn10) v = (float)ck0 * voigt_mixed((float)n * (float)cdv, (float)ca);
+                            else v = (float)cxw * __builtin_amdgcn_rcpf((float)n * (float)n);
+                            part[p] += v;
+                        } else {
+                            double v;
+                            if (n <= cn10) v = (double)ck0 * voigt_f64((double)n * (double)cdv, (double)ca);
+                            else v = (double)cxw / ((double)n * (double)n);
+                            acc64[p] += v;
+                        }
                     }
                 }
                 if (!more) break;
