@@ -756,7 +756,7 @@ __global__ __launch_bounds__(kLoBlock) void k_lo_gather_w(
         if (F64) acc_d[p * kLoBlock + t] = 0.0;
         else acc_s[p * kLoBlock + t] = acc_c[p * kLoBlock + t] = 0.0f;
     }
-    if (t < per * kLoPts) s_w[t] = waveset[min(tile0 * kLoPts + t, numnu - 1)];
+    for (int i = t; i < per * kLoPts; i += kLoBlock) s_w[i] = waveset[min(tile0 * kLoPts + i, numnu - 1)];
     __syncthreads();
     const int t0 = t - lane;  // this wave's first lane
     const int wlt0 = t0 / n, wlt1 = min((t0 + 63) / n, per - 1);

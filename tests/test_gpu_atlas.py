@@ -65,6 +65,47 @@ def test_linop1_uneven_grid_matches_oracle(gpu_ctx, atl, orc):
     assert _rel(xl, oxl).max() <= 1e-13
 
 
+def test_linop1_empty_rejected_and_edge_lines(gpu_ctx, atl, orc):
+    """Line lists at the edges of LINOP1's input contract, in every gather
+    shape and at both precisions:
+
+    - no lines at all: XLINES zero and no contributions;
+    - only lines that LINOP1 drops (below the wavelength window, or an
+      element code of 0): the same;
+    - the three bluest and reddest lines, some repeated, with the bluest
+      below WAVESET(1): the oracle's XLINES (FP64 1e-13, mixed 5e-6) and
+      contribution count."""
+    from atlas12_amd import native
+    z, _ = atl
+    zz = {k: z[k] for k in z.files}
+    ld = np.asarray(z["linedata"], np.int32)
+    dropped = ld[:50].copy()
+    dropped[:25, 0] = -np.abs(dropped[:25, 0])
+    codes = dropped[25:, 1:].copy().view(np.int16).reshape(25, 6)
+    codes[:, 0] = 0
+    dropped[25:, 1:] = codes.view(np.int32).reshape(25, 3)
+    o = np.argsort(ld[:, 0], kind="stable")
+    edge = ld[np.concatenate([o[:3], o[-3:], o[-3:], o[:1]])]
+    d = native.atlas_state_from(z)
+    nshape = None
+    for lines, want_zero in ((ld[:0], True), (dropped, True), (edge, False)):
+        zz["linedata"] = lines
+        oxl, oev = orc.linop1(zz)
+        assert (oev == 0) == want_zero
+        native.atlas_load(gpu_ctx, d, lines)
+        nshape = nshape or len(native.atlas_shapes(gpu_ctx))
+        for k in range(nshape):
+            native.atlas_set_shape(gpu_ctx, k)
+            for prec, tol in ((native.PREC_FP64, 1e-13), (native.PREC_MIXED, 5e-6)):
+                xl, ev = native.atlas_linop1(gpu_ctx, prec)
+                assert ev == oev, (len(lines), k, prec)
+                if want_zero:
+                    assert not np.any(xl), (len(lines), k, prec)
+                else:
+                    assert _rel(xl, oxl).max() <= tol, (k, prec)
+    native.atlas_set_shape(gpu_ctx, 0)
+
+
 def test_rt_pass_matches_reference(gpu_ctx, atl, coef):
     """Frequency loop on the reference's XLINES and KAPP tables vs the
     reference's RUN_RT_PASS accumulators."""
