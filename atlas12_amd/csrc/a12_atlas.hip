@@ -248,6 +248,33 @@ __device__ int lo_reach(int first, int last, int step, const double *__restrict_
     return lo < nmax ? lo + 1 : nmax;
 }
 
+// Per-(line, depth) values one contribution needs.  FP64: centre, a,
+// Doppler width (the reference's arithmetic).  Mixed: 1/Doppler width and a
+// in FP64 (offsets and every regime/table decision), centre, centre*a/sqrt(pi)
+// and a in FP32.
+// k_lo_setup stores them in this layout for the run's precision, so the
+// gathers read them as they are.
+struct LoPar {
+    double p0, p1, p2;  // FP64: center, adamp, dopwave | mixed: rdw, adamp, (c32, k32)
+};
+
+template <bool F64>
+__device__ __forceinline__ LoPar lo_par_of(double center, double adamp, double dopwave)
+{
+    LoPar q;
+    if (F64) {
+        q.p0 = center;
+        q.p1 = adamp;
+        q.p2 = dopwave;
+    } else {
+        q.p0 = 1.0 / dopwave;
+        q.p1 = adamp;
+        const float c32 = (float)center, k32 = (float)(center * 0.5641895835477563 * adamp);
+        q.p2 = __hiloint2double(__float_as_int(c32), __float_as_int(k32));
+    }
+    return q;
+}
+
 // One lane per (line, depth).  LINOP1 computes the coarse depths J = 8, 16,
 // ... first, then each fine depth J in K-7..K-1 only where coarse depth K-8
 // or K passed both centre tests (IFJ, :18239-18365); the decision depends on
@@ -255,7 +282,8 @@ __device__ int lo_reach(int first, int last, int step, const double *__restrict_
 // reads its two coarse neighbours' results from LDS (re-testing them itself
 // when they fall in another workgroup).
 // o_state[l*n + j]: bit 31 computed, bits 0-7 red reach, 8-15 blue reach
-// (points nu.. and nu-1..); o_par: centre, a, Doppler width in nm.
+// (points nu.. and nu-1..); o_par: the line's LoPar for the run's precision
+// (mixed != 0: 1/Doppler width, a, packed FP32 centre and centre*a/sqrt(pi)).
 __device__ __forceinline__ bool lo_center_ok(double cgf, double elo, double xd, double tc,
                                              double hckt, double *center)
 {
@@ -274,7 +302,7 @@ __global__ __launch_bounds__(256) void k_lo_setup(
     const double *__restrict__ waveset, const double *__restrict__ tabcont,
     const double *__restrict__ xnfdop, const double *__restrict__ dopple,
     const double *__restrict__ dep, const double *__restrict__ h0, const double *__restrict__ h1,
-    const double *__restrict__ h2, float lg2inv, int *__restrict__ o_state,
+    const double *__restrict__ h2, float lg2inv, int mixed, int *__restrict__ o_state,
     double *__restrict__ o_par, long long *__restrict__ blk_evals)
 {
 #pragma clang fp contract(off)
@@ -334,9 +362,11 @@ __global__ __launch_bounds__(256) void k_lo_setup(
             const int bn = lo_reach(nu - 1, max(nu - kMaxWing, 0), -1, waveset, wlvac, dopwave,
                                     center, adamp, tc, rtc, rdf, h0, h1, h2, lg2inv);
             st = (int)(0x80000000u | (unsigned)rn | ((unsigned)bn << 8));
-            o_par[3 * g + 0] = center;
-            o_par[3 * g + 1] = adamp;
-            o_par[3 * g + 2] = dopwave;
+            const LoPar qp = mixed ? lo_par_of<false>(center, adamp, dopwave)
+                                   : lo_par_of<true>(center, adamp, dopwave);
+            o_par[3 * g + 0] = qp.p0;
+            o_par[3 * g + 1] = qp.p1;
+            o_par[3 * g + 2] = qp.p2;
             ev = rn + bn;
         }
     }
@@ -416,35 +446,10 @@ __global__ __launch_bounds__(256) void k_lo_linereach(int nl, int n,
     }
 }
 
-// Per-(line, depth) values one contribution needs.  FP64: centre, a,
-// Doppler width (the reference's arithmetic).  Mixed: 1/Doppler width and a
-// in FP64 (offsets and every regime/table decision), centre, centre*a/sqrt(pi)
-// and a in FP32.
-struct LoPar {
-    double p0, p1, p2;  // FP64: center, adamp, dopwave | mixed: rdw, adamp, (c32, k32)
-};
-
-template <bool F64>
-__device__ __forceinline__ LoPar lo_par_of(double center, double adamp, double dopwave)
-{
-    LoPar q;
-    if (F64) {
-        q.p0 = center;
-        q.p1 = adamp;
-        q.p2 = dopwave;
-    } else {
-        q.p0 = 1.0 / dopwave;
-        q.p1 = adamp;
-        const float c32 = (float)center, k32 = (float)(center * 0.5641895835477563 * adamp);
-        q.p2 = __hiloint2double(__float_as_int(c32), __float_as_int(k32));
-    }
-    return q;
-}
-
 template <bool F64>
 __device__ __forceinline__ LoPar lo_par(const double *__restrict__ par, size_t o)
 {
-    return lo_par_of<F64>(par[3 * o], par[3 * o + 1], par[3 * o + 2]);
+    return LoPar{par[3 * o], par[3 * o + 1], par[3 * o + 2]};
 }
 
 // One contribution at offset d (red: w - wl, blue: wl - w) into LDS slot q.
@@ -805,7 +810,7 @@ __global__ __launch_bounds__(kLoBlock) void k_lo_gather_w(
                     const int ra = max(k, iw0), rb = min(k + rn - 1, iw1);
                     const int ba = max(k - bn, iw0), bb = min(k - 1, iw1);
                     if (ra <= rb || ba <= bb) {
-                        const LoPar qp = lo_par_of<F64>(p0, p1, p2);
+                        const LoPar qp = {p0, p1, p2};
                         if (F64) {
                             for (int iw = ra; iw <= rb; ++iw)
                                 lo_accum<F64>(qp, w_t[iw] - wl, (iw - iw0) * kLoBlock + t, acc_d,
@@ -1068,7 +1073,8 @@ int a12i_atlas_linop1_run(a12_ctx *ctx, int prec)
                            A.l_nu.as<int>(), A.l_nucont.as<int>(), A.l_nel.as<int>(),
                            A.l_const.as<double>(), A.waveset.as<double>(), A.tabcont.as<double>(),
                            A.xnfdop.as<double>(), A.dopple.as<double>(), A.depth_lo.as<double>(),
-                           tabs, tabs + kTabN, tabs + 2 * kTabN, A.lg2inv, A.state.as<int>(),
+                           tabs, tabs + kTabN, tabs + 2 * kTabN, A.lg2inv,
+                           prec == A12_PREC_MIXED ? 1 : 0, A.state.as<int>(),
                            A.par.as<double>(), A.blk.as<long long>());
         A12_CHECK_LAUNCH();
         const int np = std::min(nb, kSumParts);
