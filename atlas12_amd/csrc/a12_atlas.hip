@@ -88,15 +88,16 @@ __device__ __forceinline__ R lo_cv(R center, R v, R adamp, const R *h0, const R 
 // neighbouring row near integer 200 v + 1.5 (~0.5 % of that contribution),
 // and a rounded a at the 0.2 boundary switches between the table and
 // VOIGT (~4 %).
-__device__ __forceinline__ float lo_cv_mixed(float center, double v64, double a64, const float *h0,
-                                             const float *h1, const float *h2)
+__device__ __forceinline__ float lo_cv_mixed(float center, double v64, double a64,
+                                             const float4 *__restrict__ h)
 {
 #pragma clang fp contract(off)
     const float v = (float)v64, a = (float)a64;
     if (a64 > 0.20) {
         int iv = (int)(v64 * 200.0 + 1.5);
         iv = iv < 1 ? 1 : (iv > kTabN ? kTabN : iv);
-        const float H0 = h0[iv - 1], H1 = h1[iv - 1], H2 = h2[iv - 1];
+        const float4 Hr = h[iv - 1];
+        const float H0 = Hr.x, H1 = Hr.y, H2 = Hr.z;
         if (a64 <= 1.4 && a64 + v64 <= 3.2) {
             const float vv = v * v;
             const float hh1 = H1 + H0 * 1.12838f;
@@ -117,7 +118,8 @@ __device__ __forceinline__ float lo_cv_mixed(float center, double v64, double a6
     }
     if (v64 > 10.0) return center * 0.5641895835477563f * a / (v * v);
     const int iv = (int)(v64 * 200.0 + 1.5);
-    return center * ((h2[iv - 1] * a + h1[iv - 1]) * a + h0[iv - 1]);
+    const float4 Hr = h[iv - 1];
+    return center * ((Hr.z * a + Hr.y) * a + Hr.x);
 }
 
 // lo_cv in the two regimes where the profile never increases along a walk
@@ -456,8 +458,7 @@ __device__ __forceinline__ LoPar lo_par(const double *__restrict__ par, size_t o
 template <bool F64>
 __device__ __forceinline__ void lo_accum(const LoPar &q, double d, int slot, double *acc_d,
                                          float *acc_s, float *acc_c, const double *h0,
-                                         const double *h1, const double *h2, const float *f0,
-                                         const float *f1, const float *f2)
+                                         const double *h1, const double *h2, const float4 *f)
 {
 #pragma clang fp contract(off)
     if (F64) {
@@ -480,7 +481,7 @@ __device__ __forceinline__ void lo_accum(const LoPar &q, double d, int slot, dou
                                       3.0f * vv - aa32) * (ru * ru) + 1.0f) * r
                                   : r);
     } else {
-        x = lo_cv_mixed(c32, v, adamp, f0, f1, f2);
+        x = lo_cv_mixed(c32, v, adamp, f);
     }
     // Kahan's compensated sum (acc_c holds the running compensation): every
     // contribution is positive, so its bound, 2 eps times the sum, holds
@@ -499,8 +500,7 @@ __device__ __forceinline__ void lo_accum(const LoPar &q, double d, int slot, dou
 // operations: the sums are those of lo_accum's loop.
 __device__ __forceinline__ void lo_side_mixed(const LoPar &q, const double *w_t, double wl,
                                               int i0, int i1, int dir, int iw0, int stride, int t,
-                                              float *acc_s, float *acc_c, const float *f0,
-                                              const float *f1, const float *f2)
+                                              float *acc_s, float *acc_c, const float4 *f)
 {
 #pragma clang fp contract(off)
     const double adamp = q.p1;
@@ -511,7 +511,7 @@ __device__ __forceinline__ void lo_side_mixed(const LoPar &q, const double *w_t,
         const int iw = i0 + dir * k;
         const double d = dir > 0 ? w_t[iw] - wl : wl - w_t[iw], v = d * q.p0;
         if (adamp <= 0.20 ? v > 10.0 : !(adamp <= 1.4 && adamp + v <= 3.2)) break;
-        const float x = lo_cv_mixed(c32, v, adamp, f0, f1, f2);
+        const float x = lo_cv_mixed(c32, v, adamp, f);
         const int slot = (iw - iw0) * stride + t;
         const float s0 = acc_s[slot], y = x - acc_c[slot], tt = s0 + y;
         acc_c[slot] = (tt - s0) - y;
@@ -580,7 +580,7 @@ __global__ __launch_bounds__(kLoBlock) void k_lo_gather(
     const int *__restrict__ l_reach, const int *__restrict__ state, const double *__restrict__ par,
     const double *__restrict__ l_const, const double *__restrict__ waveset,
     const double *__restrict__ h0, const double *__restrict__ h1, const double *__restrict__ h2,
-    const float *__restrict__ f0, const float *__restrict__ f1, const float *__restrict__ f2,
+    const float4 *__restrict__ f,
     double *__restrict__ xlines)
 {
 #pragma clang fp contract(off)
@@ -676,10 +676,10 @@ __global__ __launch_bounds__(kLoBlock) void k_lo_gather(
                     const double *w_t = s_w + lt * kLoPts - iw0;
                     for (int iw = ra; iw <= rb; ++iw)
                         lo_accum<F64>(q, w_t[iw] - wl, (iw - iw0) * kLoBlock + t, acc_d, acc_s, acc_c,
-                                      h0, h1, h2, f0, f1, f2);
+                                      h0, h1, h2, f);
                     for (int iw = ba; iw <= bb; ++iw)
                         lo_accum<F64>(q, wl - w_t[iw], (iw - iw0) * kLoBlock + t, acc_d, acc_s, acc_c,
-                                      h0, h1, h2, f0, f1, f2);
+                                      h0, h1, h2, f);
                 } else {
                     // wave-wide inclusive scan of the item counts
                     int incl = c;
@@ -714,7 +714,7 @@ __global__ __launch_bounds__(kLoBlock) void k_lo_gather(
                             const double w = s_w[(opk >> 18) + pt];
                             const int to = t - lane + o;  // owner's lane in the block
                             lo_accum<F64>(qo, blue ? wl - w : w - wl, pt * kLoBlock + to, acc_d, acc_s,
-                                          acc_c, h0, h1, h2, f0, f1, f2);
+                                          acc_c, h0, h1, h2, f);
                         }
                     }
                 }
@@ -741,7 +741,7 @@ __global__ __launch_bounds__(kLoBlock) void k_lo_gather_w(
     const int *__restrict__ l_reach, const int *__restrict__ state, const double *__restrict__ par,
     const double *__restrict__ l_const, const double *__restrict__ waveset,
     const double *__restrict__ h0, const double *__restrict__ h1, const double *__restrict__ h2,
-    const float *__restrict__ f0, const float *__restrict__ f1, const float *__restrict__ f2,
+    const float4 *__restrict__ f,
     double *__restrict__ xlines)
 {
 #pragma clang fp contract(off)
@@ -814,15 +814,15 @@ __global__ __launch_bounds__(kLoBlock) void k_lo_gather_w(
                         if (F64) {
                             for (int iw = ra; iw <= rb; ++iw)
                                 lo_accum<F64>(qp, w_t[iw] - wl, (iw - iw0) * kLoBlock + t, acc_d,
-                                              acc_s, acc_c, h0, h1, h2, f0, f1, f2);
+                                              acc_s, acc_c, h0, h1, h2, f);
                             for (int iw = ba; iw <= bb; ++iw)
                                 lo_accum<F64>(qp, wl - w_t[iw], (iw - iw0) * kLoBlock + t, acc_d,
-                                              acc_s, acc_c, h0, h1, h2, f0, f1, f2);
+                                              acc_s, acc_c, h0, h1, h2, f);
                         } else {
                             lo_side_mixed(qp, w_t, wl, ra, rb, 1, iw0, kLoBlock, t, acc_s, acc_c,
-                                          f0, f1, f2);
+                                          f);
                             lo_side_mixed(qp, w_t, wl, bb, ba, -1, iw0, kLoBlock, t, acc_s,
-                                          acc_c, f0, f1, f2);
+                                          acc_c, f);
                         }
                     }
                 }
@@ -852,10 +852,10 @@ struct LoShape {
     int flat;
     void (*mixed)(int, int, int, const int *, const int *, const int *, const int *,
                   const double *, const double *, const double *, const double *, const double *,
-                  const double *, const float *, const float *, const float *, double *);
+                  const double *, const float4 *, double *);
     void (*fp64)(int, int, int, const int *, const int *, const int *, const int *,
                  const double *, const double *, const double *, const double *, const double *,
-                 const double *, const float *, const float *, const float *, double *);
+                 const double *, const float4 *, double *);
 };
 #define A12_LO_SHAPE(P, B, T, F) \
     {P, B, T, F ? 1 : 0, k_lo_gather<false, P, B, T, F>, k_lo_gather<true, P, B, T, F>}
@@ -953,7 +953,10 @@ int a12i_atlas_load(a12_ctx *ctx, const a12_atlas_atm *a, const int32_t *linedat
     std::memcpy(&tabs[0], a->h0tab, sizeof(double) * kTabN);
     std::memcpy(&tabs[kTabN], a->h1tab, sizeof(double) * kTabN);
     std::memcpy(&tabs[2 * kTabN], a->h2tab, sizeof(double) * kTabN);
-    std::vector<float> tabs32(tabs.begin(), tabs.end());
+    // FP32 rows (H0, H1, H2, 0) of the three tables: one 16-byte load per value
+    std::vector<float4> tabs32(kTabN);
+    for (int i = 0; i < kTabN; ++i)
+        tabs32[i] = make_float4((float)tabs[i], (float)tabs[kTabN + i], (float)tabs[2 * kTabN + i], 0.0f);
     A12_HIP(up(A.tabs, tabs.data(), tabs.size(), s));
     A12_HIP(up(A.tabs32, tabs32.data(), tabs32.size(), s));
     // lines: unpack and prepare as LINOP1 does per record (:18176-18236)
@@ -1027,7 +1030,7 @@ static int lo_gather_launch(a12_ctx *ctx, int prec, int shape)
     AtlasDev &A = ctx->atl;
     const int n = A.n, numnu = A.numnu;
     const double *tabs = A.tabs.as<double>();
-    const float *tabs32 = A.tabs32.as<float>();
+    const float4 *tabs32 = A.tabs32.as<float4>();
     const LoShape &sh = kLoShapes[shape];
     auto g = prec == A12_PREC_FP64 ? sh.fp64 : sh.mixed;
     const int tiles = (numnu + sh.pts - 1) / sh.pts, per = sh.block / n;
@@ -1038,7 +1041,7 @@ static int lo_gather_launch(a12_ctx *ctx, int prec, int shape)
     hipLaunchKernelGGL(g, dim3((tiles + per - 1) / per), dim3(sh.block), 0, ctx->stream, n, numnu,
                        tiles, A.seg.as<int>(), A.l_nu.as<int>(), A.reach.as<int>(), A.state.as<int>(),
                        A.par.as<double>(), A.l_const.as<double>(), A.waveset.as<double>(), tabs,
-                       tabs + kTabN, tabs + 2 * kTabN, tabs32, tabs32 + kTabN, tabs32 + 2 * kTabN,
+                       tabs + kTabN, tabs + 2 * kTabN, tabs32,
                        A.xlines.as<double>());
     A12_CHECK_LAUNCH();
     return A12_OK;
@@ -1066,7 +1069,7 @@ int a12i_atlas_linop1_run(a12_ctx *ctx, int prec)
     A12_HIP(A.blk.reserve(sizeof(long long) * (nb + kSumParts)));
     A12_HIP(A.evals.reserve(sizeof(long long)));
     const double *tabs = A.tabs.as<double>();
-    const float *tabs32 = A.tabs32.as<float>();
+    const float4 *tabs32 = A.tabs32.as<float4>();
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[8], s));
     if (nl > 0) {
         hipLaunchKernelGGL(k_lo_setup, dim3(nb), dim3(256), 0, s, nl, n, numnu,
