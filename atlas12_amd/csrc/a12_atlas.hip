@@ -1069,7 +1069,6 @@ int a12i_atlas_linop1_run(a12_ctx *ctx, int prec)
     A12_HIP(A.blk.reserve(sizeof(long long) * (nb + kSumParts)));
     A12_HIP(A.evals.reserve(sizeof(long long)));
     const double *tabs = A.tabs.as<double>();
-    const float4 *tabs32 = A.tabs32.as<float4>();
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[8], s));
     if (nl > 0) {
         hipLaunchKernelGGL(k_lo_setup, dim3(nb), dim3(256), 0, s, nl, n, numnu,
