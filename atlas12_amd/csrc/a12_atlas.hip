@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "a12_common.h"
@@ -886,37 +887,166 @@ hipError_t up(DevBuf &b, const T *src, size_t n, hipStream_t s)
 // --------------------------------------------------------------------------
 // Host side
 // --------------------------------------------------------------------------
-int a12i_atlas_load(a12_ctx *ctx, const a12_atlas_atm *a, const int32_t *linedata, int32_t nlines)
+// The line store, once per store (src/atlas12_modules.f90:18176-18236 per
+// record: TABLOG decoding, continuum bin NUCONT, vacuum wavelength, grid
+// index NU, CGF and the three damping constants).  None of it depends on
+// the atmosphere, so an iteration re-reads nothing here.  The grid fields
+// of a (numnu, waveset, nwave1, iwavetab, mion) are the only ones read.
+int a12i_atlas_load_lines(a12_ctx *ctx, const a12_atlas_atm *a, const int32_t *linedata,
+                          int32_t nlines)
 {
     AtlasDev &A = ctx->atl;
     if (!a || (!linedata && nlines > 0) || nlines < 0) {
-        ctx->err = "a12_atlas_load: null argument";
+        ctx->err = "a12_atlas_load_lines: null argument";
         return A12_ERR_ARG;
     }
-    if (a->nrhox < 8 || a->nrhox > A12_NRMAX_LAYERS || a->numnu < 3 || a->mion < 1 ||
-        a->ldion < a->nrhox || a->ldtab < a->nrhox || a->nwave1 < 2 || !a->waveset ||
-        !a->rcoset || !a->tabcont || !a->iwavetab || !a->xnfdop || !a->dopple || !a->t ||
-        !a->xne || !a->hckt || !a->hkt || !a->rhox || !a->xnf1 || !a->xnf3 || !a->xnf841 ||
-        !a->h0tab || !a->h1tab || !a->h2tab) {
-        ctx->err = "a12_atlas_load: bad dimensions or null array";
+    if (a->numnu < 3 || a->mion < 1 || a->nwave1 < 2 || !a->waveset || !a->iwavetab) {
+        ctx->err = "a12_atlas_load_lines: bad grid dimensions or null array";
         return A12_ERR_ARG;
     }
-    const int n = a->nrhox, numnu = a->numnu;
+    const int numnu = a->numnu;
     for (int i = 1; i < numnu; ++i)
         if (!(a->waveset[i] > a->waveset[i - 1])) {
-            ctx->err = "a12_atlas_load: WAVESET must ascend";
+            ctx->err = "a12_atlas_load_lines: WAVESET must ascend";
             return A12_ERR_ARG;
         }
+    // a new store invalidates everything computed from the previous one
+    A.lines_ok = A.loaded = A.xl_ready = A.kapp_ready = A.rt_done = false;
+    A.prof_lo = A.prof_rt = false;
     hipStream_t s = ctx->stream;
-    A.n = n;
     A.numnu = numnu;
     A.mion = a->mion;
-    A.teff = a->teff;
+    A.nwave1 = a->nwave1;
+    A.waveset_h.assign(a->waveset, a->waveset + numnu);
+    A.iwavetab_h.assign(a->iwavetab, a->iwavetab + a->nwave1);
     // mean points per octave of the grid: places the reach search's first
     // probe (a starting point only; any ascending grid gives exact reaches)
-    A.lg2inv = numnu > 1 ? (float)((numnu - 1) / std::log2(a->waveset[numnu - 1] / a->waveset[0]))
-                         : 0.0f;
+    A.lg2inv = (float)((numnu - 1) / std::log2(a->waveset[numnu - 1] / a->waveset[0]));
     if (!std::isfinite(A.lg2inv)) A.lg2inv = 0.0f;
+    A12_HIP(up(A.waveset, a->waveset, numnu, s));
+    // TABLOG (:2083-2086): 10**((I - 16384) * 0.001) for every int16 code,
+    // the same expression the per-record decode would evaluate
+    std::vector<double> tablog(65536);
+    for (int i = 0; i < 65536; ++i)
+        tablog[i] = std::pow(10.0, (double)((int)(int16_t)(uint16_t)i - 16384) * .001);
+    auto tl = [&](int16_t c) { return tablog[(uint16_t)c]; };
+    const double ratiolg = std::log(1.0 + 1.0 / 2000000.0);
+    const double cen_prefac4 = 0.026538 / 1.7724538509055159 / 2.99792458e17;
+    const double fourpi_c_inv = 1.0 / ((4.0 * 3.141592653589793) * 2.99792458e17);
+    const double start = a->waveset[0] - 1.0, stop = a->waveset[numnu - 1] + 1.0;
+    struct Rec {
+        int nu, nucont, nel;
+        double c[6];
+    };
+    std::vector<Rec> recs((size_t)nlines);
+    std::vector<char> keep((size_t)nlines, 0);
+    // records are independent: decode them on a few host threads
+    const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    auto decode = [&](int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; ++i) {
+            const int32_t *r = linedata + 4 * (size_t)i;
+            int16_t p[6];
+            std::memcpy(p, r + 1, 12);
+            const int iwl = r[0];
+            // first bin with IWL < IWAVETAB, clamped to the last (:18193-18200)
+            const int nucont = std::min(
+                (int)(std::upper_bound(a->iwavetab, a->iwavetab + a->nwave1, iwl) - a->iwavetab),
+                a->nwave1 - 1);
+            const int nel = std::abs(p[0] / 10);
+            if (nel < 1 || nel > a->mion) continue;
+            const double wl = std::exp(iwl * ratiolg);
+            if (wl < start || wl > stop) continue;
+            const int nu = (int)(std::upper_bound(a->waveset, a->waveset + numnu, wl) - a->waveset);
+            Rec &q = recs[i];
+            q.nu = nu;
+            q.nucont = nucont;
+            q.nel = nel;
+            q.c[0] = wl;
+            q.c[1] = cen_prefac4 * wl * tl(p[2]);
+            q.c[2] = tl(p[1]);
+            q.c[3] = tl(p[3]) * wl * fourpi_c_inv;
+            q.c[4] = tl(p[4]) * wl * fourpi_c_inv;
+            q.c[5] = tl(p[5]) * wl * fourpi_c_inv;
+            keep[i] = 1;
+        }
+    };
+    {
+        std::vector<std::thread> th;
+        const int64_t per = ((int64_t)nlines + nth - 1) / nth;
+        for (int k = 0; k < nth; ++k) {
+            const int64_t i0 = k * per, i1 = std::min<int64_t>(nlines, i0 + per);
+            if (i0 < i1) th.emplace_back(decode, i0, i1);
+        }
+        for (auto &t : th) t.join();
+    }
+    // stable counting sort by NU (0..numnu): kept records in store order
+    // within each NU, as the reference meets them along the store
+    std::vector<int> seg(numnu + 2, 0);
+    for (int64_t i = 0; i < nlines; ++i)
+        if (keep[i]) ++seg[recs[i].nu + 1];
+    for (int k = 1; k <= numnu + 1; ++k) seg[k] += seg[k - 1];
+    const int nl = seg[numnu + 1];
+    std::vector<int> l_nu(nl), l_nucont(nl), l_nel(nl);
+    std::vector<double> l_const((size_t)6 * std::max(nl, 1));
+    {
+        std::vector<int> pos(seg.begin(), seg.end() - 1);
+        for (int64_t i = 0; i < nlines; ++i) {
+            if (!keep[i]) continue;
+            const Rec &q = recs[i];
+            const int o = pos[q.nu]++;
+            l_nu[o] = q.nu;
+            l_nucont[o] = q.nucont;
+            l_nel[o] = q.nel;
+            for (int k = 0; k < 5; ++k) l_const[5 * (size_t)o + k] = q.c[k];
+            l_const[5 * (size_t)nl + o] = q.c[5];
+        }
+    }
+    // seg[k] = first sorted line with nu >= k, k = 0..numnu+1
+    A.nl = nl;
+    A12_HIP(up(A.l_nu, l_nu.data(), nl, s));
+    A12_HIP(up(A.l_nucont, l_nucont.data(), nl, s));
+    A12_HIP(up(A.l_nel, l_nel.data(), nl, s));
+    A12_HIP(up(A.l_const, l_const.data(), (size_t)6 * nl, s));
+    A12_HIP(up(A.seg, seg.data(), seg.size(), s));
+    A12_HIP(hipStreamSynchronize(s));
+    A.lines_ok = true;
+    return A12_OK;
+}
+
+// The per-iteration state LINOP1 and the RT pass read (T, XNE, HCKT, HKT,
+// RHOX, TXNXN from XNF, XNFDOP, DOPPLE, TABCONT, RCOSET, TABVOIGT, TEFF),
+// on the grid of the loaded store.  Invalidates XLINES, the KAPP tables and
+// the RT-pass sums of the previous state.
+int a12i_atlas_set_state(a12_ctx *ctx, const a12_atlas_atm *a)
+{
+    AtlasDev &A = ctx->atl;
+    if (!A.lines_ok) {
+        ctx->err = "a12_atlas_set_state: call a12_atlas_load_lines first";
+        return A12_ERR_STATE;
+    }
+    if (!a) {
+        ctx->err = "a12_atlas_set_state: null argument";
+        return A12_ERR_ARG;
+    }
+    if (a->nrhox < 8 || a->nrhox > A12_NRMAX_LAYERS || a->ldion < a->nrhox ||
+        a->ldtab < a->nrhox || !a->rcoset || !a->tabcont || !a->xnfdop || !a->dopple || !a->t ||
+        !a->xne || !a->hckt || !a->hkt || !a->rhox || !a->xnf1 || !a->xnf3 || !a->xnf841 ||
+        !a->h0tab || !a->h1tab || !a->h2tab) {
+        ctx->err = "a12_atlas_set_state: bad dimensions or null array";
+        return A12_ERR_ARG;
+    }
+    if (a->numnu != A.numnu || a->nwave1 != A.nwave1 || a->mion != A.mion ||
+        (a->waveset && std::memcmp(a->waveset, A.waveset_h.data(), sizeof(double) * A.numnu)) ||
+        (a->iwavetab && std::memcmp(a->iwavetab, A.iwavetab_h.data(), sizeof(int32_t) * A.nwave1))) {
+        ctx->err = "a12_atlas_set_state: grid differs from the loaded line store's";
+        return A12_ERR_STATE;
+    }
+    A.loaded = A.xl_ready = A.kapp_ready = A.rt_done = false;
+    A.prof_lo = A.prof_rt = false;
+    hipStream_t s = ctx->stream;
+    const int n = a->nrhox;
+    A.n = n;
+    A.teff = a->teff;
     // depth vectors: t hkt rhox | xne hckt txnxn (TXNXN as LINOP1 forms it, :18164)
     std::vector<double> dep((size_t)6 * n);
     for (int j = 0; j < n; ++j) {
@@ -932,23 +1062,26 @@ int a12i_atlas_load(a12_ctx *ctx, const a12_atlas_atm *a, const int32_t *linedat
     // the copy for it out as [t hkt xne hckt . txnxn]
     std::vector<double> dset(dep);
     for (int j = 0; j < n; ++j) dset[2 * n + j] = a->xne[j];
-    A12_HIP(up(A.depth3, dep.data(), (size_t)3 * n, s));  // t hkt rhox for the RT pass
-    A12_HIP(up(A.depth_lo, dset.data(), dset.size(), s));
-    // compact copies of the (ld, *) arrays
-    std::vector<double> tmp((size_t)n * std::max(a->mion, a->nwave1));
-    for (int i = 0; i < a->nwave1; ++i)
-        std::memcpy(&tmp[(size_t)i * n], a->tabcont + (size_t)i * a->ldtab, sizeof(double) * n);
-    A12_HIP(up(A.tabcont, tmp.data(), (size_t)a->nwave1 * n, s));
-    A12_HIP(hipStreamSynchronize(s));
-    for (int i = 0; i < a->mion; ++i)
-        std::memcpy(&tmp[(size_t)i * n], a->xnfdop + (size_t)i * a->ldion, sizeof(double) * n);
-    A12_HIP(up(A.xnfdop, tmp.data(), (size_t)a->mion * n, s));
-    A12_HIP(hipStreamSynchronize(s));
-    for (int i = 0; i < a->mion; ++i)
-        std::memcpy(&tmp[(size_t)i * n], a->dopple + (size_t)i * a->ldion, sizeof(double) * n);
-    A12_HIP(up(A.dopple, tmp.data(), (size_t)a->mion * n, s));
-    A12_HIP(up(A.waveset, a->waveset, numnu, s));
-    A12_HIP(up(A.rcoset, a->rcoset, numnu, s));
+    // one pinned-free host staging buffer: depth3 | depth_lo | tabcont |
+    // xnfdop | dopple, copied with one call per device array
+    const size_t ntab = (size_t)A.nwave1 * n, nion = (size_t)A.mion * n;
+    A.stage_h.resize(3 * (size_t)n + dset.size() + ntab + 2 * nion);
+    double *h = A.stage_h.data();
+    std::memcpy(h, dep.data(), sizeof(double) * 3 * n);
+    std::memcpy(h + 3 * n, dset.data(), sizeof(double) * dset.size());
+    double *ht = h + 3 * n + dset.size(), *hx = ht + ntab, *hd = hx + nion;
+    for (int i = 0; i < A.nwave1; ++i)
+        std::memcpy(ht + (size_t)i * n, a->tabcont + (size_t)i * a->ldtab, sizeof(double) * n);
+    for (int i = 0; i < A.mion; ++i) {
+        std::memcpy(hx + (size_t)i * n, a->xnfdop + (size_t)i * a->ldion, sizeof(double) * n);
+        std::memcpy(hd + (size_t)i * n, a->dopple + (size_t)i * a->ldion, sizeof(double) * n);
+    }
+    A12_HIP(up(A.depth3, h, (size_t)3 * n, s));  // t hkt rhox for the RT pass
+    A12_HIP(up(A.depth_lo, h + 3 * n, dset.size(), s));
+    A12_HIP(up(A.tabcont, ht, ntab, s));
+    A12_HIP(up(A.xnfdop, hx, nion, s));
+    A12_HIP(up(A.dopple, hd, nion, s));
+    A12_HIP(up(A.rcoset, a->rcoset, A.numnu, s));
     std::vector<double> tabs((size_t)3 * kTabN);
     std::memcpy(&tabs[0], a->h0tab, sizeof(double) * kTabN);
     std::memcpy(&tabs[kTabN], a->h1tab, sizeof(double) * kTabN);
@@ -959,70 +1092,15 @@ int a12i_atlas_load(a12_ctx *ctx, const a12_atlas_atm *a, const int32_t *linedat
         tabs32[i] = make_float4((float)tabs[i], (float)tabs[kTabN + i], (float)tabs[2 * kTabN + i], 0.0f);
     A12_HIP(up(A.tabs, tabs.data(), tabs.size(), s));
     A12_HIP(up(A.tabs32, tabs32.data(), tabs32.size(), s));
-    // lines: unpack and prepare as LINOP1 does per record (:18176-18236)
-    const double ratiolg = std::log(1.0 + 1.0 / 2000000.0);
-    const double cen_prefac4 = 0.026538 / 1.7724538509055159 / 2.99792458e17;
-    const double fourpi_c_inv = 1.0 / ((4.0 * 3.141592653589793) * 2.99792458e17);
-    const double start = a->waveset[0] - 1.0, stop = a->waveset[numnu - 1] + 1.0;
-    struct Rec {
-        int nu, nucont, nel;
-        double c[6];
-    };
-    std::vector<Rec> recs;
-    recs.reserve(nlines);
-    auto tablog = [](int16_t i) { return std::pow(10.0, (double)(i - 16384) * .001); };
-    for (int i = 0; i < nlines; ++i) {
-        const int32_t *r = linedata + 4 * (size_t)i;
-        int16_t p[6];
-        std::memcpy(p, r + 1, 12);
-        const int iwl = r[0];
-        // first bin with IWL < IWAVETAB, clamped to the last (:18193-18200)
-        const int nucont = std::min(
-            (int)(std::upper_bound(a->iwavetab, a->iwavetab + a->nwave1, iwl) - a->iwavetab),
-            a->nwave1 - 1);
-        const int nel = std::abs(p[0] / 10);
-        if (nel < 1 || nel > a->mion) continue;
-        const double wl = std::exp(iwl * ratiolg);
-        if (wl < start || wl > stop) continue;
-        const int nu = (int)(std::upper_bound(a->waveset, a->waveset + numnu, wl) - a->waveset);
-        Rec q;
-        q.nu = nu;
-        q.nucont = nucont;
-        q.nel = nel;
-        q.c[0] = wl;
-        q.c[1] = cen_prefac4 * wl * tablog(p[2]);
-        q.c[2] = tablog(p[1]);
-        q.c[3] = tablog(p[3]) * wl * fourpi_c_inv;
-        q.c[4] = tablog(p[4]) * wl * fourpi_c_inv;
-        q.c[5] = tablog(p[5]) * wl * fourpi_c_inv;
-        recs.push_back(q);
-    }
-    std::stable_sort(recs.begin(), recs.end(), [](const Rec &x, const Rec &y) { return x.nu < y.nu; });
-    const int nl = (int)recs.size();
-    std::vector<int> l_nu(nl), l_nucont(nl), l_nel(nl), seg(numnu + 2, 0);
-    std::vector<double> l_const((size_t)6 * std::max(nl, 1));
-    for (int i = 0; i < nl; ++i) {
-        l_nu[i] = recs[i].nu;
-        l_nucont[i] = recs[i].nucont;
-        l_nel[i] = recs[i].nel;
-        for (int k = 0; k < 5; ++k) l_const[5 * (size_t)i + k] = recs[i].c[k];
-        l_const[5 * (size_t)nl + i] = recs[i].c[5];
-    }
-    // seg[k] = first sorted line with nu >= k, k = 0..numnu+1
-    for (int k = 0, i = 0; k <= numnu + 1; ++k) {
-        while (i < nl && l_nu[i] < k) ++i;
-        seg[k] = i;
-    }
-    A.nl = nl;
-    A12_HIP(up(A.l_nu, l_nu.data(), nl, s));
-    A12_HIP(up(A.l_nucont, l_nucont.data(), nl, s));
-    A12_HIP(up(A.l_nel, l_nel.data(), nl, s));
-    A12_HIP(up(A.l_const, l_const.data(), (size_t)6 * nl, s));
-    A12_HIP(up(A.seg, seg.data(), seg.size(), s));
-    A12_HIP(hipStreamSynchronize(s));
+    A12_HIP(hipStreamSynchronize(s));  // the host staging is reused by the next state
     A.loaded = true;
-    A.xl_ready = false;
     return A12_OK;
+}
+
+int a12i_atlas_load(a12_ctx *ctx, const a12_atlas_atm *a, const int32_t *linedata, int32_t nlines)
+{
+    const int rc = a12i_atlas_load_lines(ctx, a, linedata, nlines);
+    return rc ? rc : a12i_atlas_set_state(ctx, a);
 }
 
 static int lo_gather_launch(a12_ctx *ctx, int prec, int shape)
@@ -1051,7 +1129,7 @@ int a12i_atlas_linop1_run(a12_ctx *ctx, int prec)
 {
     AtlasDev &A = ctx->atl;
     if (!A.loaded) {
-        ctx->err = "a12_atlas_linop1_run: call a12_atlas_load first";
+        ctx->err = "a12_atlas_linop1_run: call a12_atlas_set_state (or a12_atlas_load) first";
         return A12_ERR_STATE;
     }
     if (prec != A12_PREC_MIXED && prec != A12_PREC_FP64) {

@@ -251,6 +251,21 @@ int a12_atlas_load(a12_ctx *ctx, const a12_atlas_atm *atm, const int32_t *lineda
     return a12i_atlas_load(ctx, atm, linedata, nlines);
 }
 
+int a12_atlas_load_lines(a12_ctx *ctx, const a12_atlas_atm *atm, const int32_t *linedata,
+                         int32_t nlines)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_atlas_load_lines(ctx, atm, linedata, nlines);
+}
+
+int a12_atlas_set_state(a12_ctx *ctx, const a12_atlas_atm *atm)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_atlas_set_state(ctx, atm);
+}
+
 int a12_atlas_select_lines(a12_ctx *ctx, const a12_atlas_atm *atm, const double *wavetab,
                            const int32_t *pred, int64_t npred, const a12_mol2_rec *mol2,
                            int64_t nmol2, int32_t *linedata_out, int64_t cap, int64_t *nkept)
@@ -316,7 +331,7 @@ int a12_atlas_set_kapp(a12_ctx *ctx, const double *acont, const double *sigmac,
     if (!ctx_ok(ctx)) return A12_ERR_ARG;
     AtlasDev &A = ctx->atl;
     if (!A.loaded) {
-        ctx->err = "a12_atlas_set_kapp: call a12_atlas_load first";
+        ctx->err = "a12_atlas_set_kapp: call a12_atlas_set_state (or a12_atlas_load) first";
         return A12_ERR_STATE;
     }
     const double *src[6] = {acont, sigmac, scont, sline, aline, sigmal};
@@ -333,6 +348,7 @@ int a12_atlas_set_kapp(a12_ctx *ctx, const double *acont, const double *sigmac,
                                hipMemcpyHostToDevice, ctx->stream));
     A12_HIP(hipStreamSynchronize(ctx->stream));
     A.kapp_ready = true;
+    A.kapp_plane = plane;
     return A12_OK;
 }
 
@@ -346,6 +362,10 @@ int a12_atlas_rt_run(a12_ctx *ctx, const double *xlines)
         return A12_ERR_STATE;
     }
     const size_t plane = (size_t)A.n * A.numnu;
+    if (A.kapp_plane != plane) {
+        ctx->err = "a12_atlas_rt_run: KAPP tables were set for another grid";
+        return A12_ERR_STATE;
+    }
     if (xlines) {
         A12_HIP(A.xlines.reserve(sizeof(double) * plane));
         A12_HIP(hipMemcpyAsync(A.xlines.p, xlines, sizeof(double) * plane, hipMemcpyHostToDevice,

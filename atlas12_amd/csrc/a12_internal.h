@@ -61,14 +61,20 @@ struct SyntheDev {
 };
 
 struct AtlasDev {
-    int n = 0, numnu = 0, mion = 0, nl = 0;
+    int n = 0, numnu = 0, mion = 0, nl = 0, nwave1 = 0;
     double teff = 0;
     float lg2inv = 0;  // 1 / log2 of WAVESET's mean point-to-point ratio
     DevBuf depth3, depth_lo, tabcont, xnfdop, dopple, waveset, rcoset, tabs, tabs32;
     DevBuf l_nu, l_nucont, l_nel, l_const, seg;
     DevBuf state, par, reach, xlines, blk, evals;
     DevBuf kapp, hnu1, acc;
-    bool loaded = false, xl_ready = false, kapp_ready = false, rt_done = false;
+    // the loaded store's grid (set_state checks a state against it) and the
+    // host staging of set_state
+    std::vector<double> waveset_h, stage_h;
+    std::vector<int32_t> iwavetab_h;
+    size_t kapp_plane = 0;  // nrhox * numnu of the KAPP tables on the device
+    // lines_ok: a store is loaded; loaded: a state is set on it
+    bool lines_ok = false, loaded = false, xl_ready = false, kapp_ready = false, rt_done = false;
     bool prof_lo = false, prof_rt = false;
     int shape = 0;  // LINOP1 gather tile shape (a12_atlas_set_shape / a12_atlas_autotune)
 };
@@ -143,6 +149,9 @@ int a12i_synthe_batch_info(a12_ctx *ctx, int64_t *info);
 void a12i_synthe_batch_free(a12_ctx *ctx);
 // Implemented in a12_atlas.hip
 int a12i_atlas_load(a12_ctx *ctx, const a12_atlas_atm *a, const int32_t *linedata, int32_t nlines);
+int a12i_atlas_load_lines(a12_ctx *ctx, const a12_atlas_atm *a, const int32_t *linedata,
+                          int32_t nlines);
+int a12i_atlas_set_state(a12_ctx *ctx, const a12_atlas_atm *a);
 int a12i_atlas_linop1_run(a12_ctx *ctx, int prec);
 int a12i_atlas_shapes(int idx, int32_t *pts_block_batch_flat);
 int a12i_atlas_autotune(a12_ctx *ctx, int prec, float *ms, int32_t *best);

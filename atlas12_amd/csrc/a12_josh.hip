@@ -1114,6 +1114,21 @@ int a12i_josh_batch(a12_ctx *ctx, int ifscat, int ifsurf, int nrhox, int nbatch,
     return A12_OK;
 }
 
+static int rt_pass_launch(a12_ctx *ctx, int n, int nf, int nchunk, double teff,
+                          const double *d_wave, const double *d_rcoset, const double *d_depth3,
+                          const double *d_kapp, const double *d_xlines, double *d_hnu1,
+                          double *d_part, double *d_acc)
+{
+    hipLaunchKernelGGL(k_rt_pass, dim3(nchunk), dim3(64), 0, ctx->stream, n, nf, kRtChunk, teff,
+                       d_wave, d_rcoset, d_depth3, d_kapp, d_xlines, ctx->coefj.as<double>(),
+                       ctx->coefh.as<double>(), d_hnu1, d_part);
+    A12_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_rt_reduce, dim3(9 * n), dim3(256), 0, ctx->stream, n, nchunk, d_part,
+                       d_acc);
+    A12_CHECK_LAUNCH();
+    return A12_OK;
+}
+
 int a12i_atlas_rt_pass(a12_ctx *ctx, int n, int nf, double teff, const double *d_wave,
                        const double *d_rcoset, const double *d_depth3, const double *d_kapp,
                        const double *d_xlines, double *d_hnu1, double *d_acc)
@@ -1122,17 +1137,12 @@ int a12i_atlas_rt_pass(a12_ctx *ctx, int n, int nf, double teff, const double *d
         ctx->err = "a12_atlas_rt_pass: call a12_set_operators first";
         return A12_ERR_STATE;
     }
-    const int chunk = kRtChunk;
-    const int nchunk = (nf + chunk - 1) / chunk;
+    const int nchunk = (nf + kRtChunk - 1) / kRtChunk;
     A12_HIP(ctx->rt_part.reserve(sizeof(double) * 9 * (size_t)n * nchunk));
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[12], ctx->stream));
-    hipLaunchKernelGGL(k_rt_pass, dim3(nchunk), dim3(64), 0, ctx->stream, n, nf, chunk, teff,
-                       d_wave, d_rcoset, d_depth3, d_kapp, d_xlines, ctx->coefj.as<double>(),
-                       ctx->coefh.as<double>(), d_hnu1, ctx->rt_part.as<double>());
-    A12_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_rt_reduce, dim3(9 * n), dim3(256), 0, ctx->stream, n, nchunk,
-                       ctx->rt_part.as<double>(), d_acc);
-    A12_CHECK_LAUNCH();
+    const int rc = rt_pass_launch(ctx, n, nf, nchunk, teff, d_wave, d_rcoset, d_depth3, d_kapp,
+                                  d_xlines, d_hnu1, ctx->rt_part.as<double>(), d_acc);
+    if (rc) return rc;
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[13], ctx->stream));
     return A12_OK;
 }

@@ -39,7 +39,8 @@ EXPORTS = ["a12_ctx_create", "a12_ctx_destroy", "a12_last_error", "a12_stream",
            "a12_molec_set_table", "a12_molec_newton_batched",
            "a12_synthe_shape_info", "a12_synthe_set_shape", "a12_synthe_batch_load",
            "a12_synthe_batch_set_budget", "a12_synthe_batch_run", "a12_synthe_batch_fetch",
-           "a12_synthe_batch_info", "a12_solvit_batched"]
+           "a12_synthe_batch_info", "a12_solvit_batched", "a12_atlas_load_lines",
+           "a12_atlas_set_state"]
 
 
 class A12Error(RuntimeError):
@@ -311,9 +312,11 @@ class Context:
         return hl, hc
 
     def synchronize(self):
-        import ctypes.util  # noqa: F401
-        # a fetch of the eval count synchronises the context stream
-        self.lib.a12_synthe_evals(self.h)
+        """Waits for the context's stream (HIP runtime's hipStreamSynchronize)."""
+        hip = C.CDLL("libamdhip64.so.7")  # the runtime the library links
+        rc = hip.hipStreamSynchronize(C.c_void_p(self.stream))
+        if rc != 0:
+            raise A12Error(f"hipStreamSynchronize failed with HIP error {rc}")
 
 
 def synthe_shapes():
@@ -387,12 +390,44 @@ def _atlas_struct(d):
     return a
 
 
-def atlas_load(ctx: Context, d, linedata):
+def _linedata(linedata):
     ld = np.ascontiguousarray(linedata, np.int32)
+    if ld.ndim != 2 or ld.shape[1] != 4:
+        raise ValueError("linedata must have shape (nlines, 4)")
+    return ld
+
+
+def atlas_load(ctx: Context, d, linedata):
+    """Line store and state in one call (a12_atlas_load)."""
+    ld = _linedata(linedata)
     a = _atlas_struct(d)
     ctx._check(ctx.lib.a12_atlas_load(ctx.h, C.byref(a), ld.ctypes.data_as(C.c_void_p),
                                       C.c_int32(len(ld))), "a12_atlas_load")
     ctx._atl = (len(d["t"]), len(d["waveset"]))
+
+
+def atlas_load_lines(ctx: Context, d, linedata):
+    """The line store alone, once per store (a12_atlas_load_lines); d
+    supplies the grid (waveset, iwavetab, mion)."""
+    ld = _linedata(linedata)
+    a = _atlas_struct(d)
+    ctx._check(ctx.lib.a12_atlas_load_lines(ctx.h, C.byref(a), ld.ctypes.data_as(C.c_void_p),
+                                            C.c_int32(len(ld))), "a12_atlas_load_lines")
+    ctx._atl = None
+
+
+def atlas_set_state(ctx: Context, d):
+    """The per-iteration state on the loaded store's grid (a12_atlas_set_state)."""
+    a = _atlas_struct(d)
+    ctx._check(ctx.lib.a12_atlas_set_state(ctx.h, C.byref(a)), "a12_atlas_set_state")
+    ctx._atl = (len(d["t"]), len(d["waveset"]))
+
+
+def _atl_dims(ctx: Context):
+    dims = getattr(ctx, "_atl", None)
+    if dims is None:
+        raise A12Error("no ATLAS12 state set on this context (atlas_set_state / atlas_load)")
+    return dims
 
 
 def atlas_linop1(ctx: Context, prec: int = PREC_FP64, fetch: bool = True):
@@ -401,7 +436,7 @@ def atlas_linop1(ctx: Context, prec: int = PREC_FP64, fetch: bool = True):
     ctx._check(ctx.lib.a12_atlas_linop1_run(ctx.h, C.c_int32(prec)), "a12_atlas_linop1_run")
     if not fetch:
         return None, None
-    n, nf = ctx._atl
+    n, nf = _atl_dims(ctx)
     out = np.zeros((nf, n))
     ev = C.c_int64(0)
     ctx._check(ctx.lib.a12_atlas_linop1_fetch(ctx.h, _ptr(out), C.byref(ev)),
@@ -410,19 +445,27 @@ def atlas_linop1(ctx: Context, prec: int = PREC_FP64, fetch: bool = True):
 
 
 def atlas_set_kapp(ctx: Context, acont, sigmac, scont, sline, aline, sigmal):
+    """KAPP's six tables, each (numnu, nrhox) as the device holds them
+    (nrhox fastest); any other size is rejected before the call."""
+    n, nf = _atl_dims(ctx)
     ts = [_f64(t) for t in (acont, sigmac, scont, sline, aline, sigmal)]
+    for name, t in zip(("acont", "sigmac", "scont", "sline", "aline", "sigmal"), ts):
+        if t.size != n * nf:
+            raise ValueError(f"{name} has {t.size} elements, the state needs {nf} x {n}")
     ctx._check(ctx.lib.a12_atlas_set_kapp(ctx.h, *[_ptr(t) for t in ts]), "a12_atlas_set_kapp")
 
 
 def atlas_rt_pass(ctx: Context, xlines=None, fetch: bool = True):
     """RUN_RT_PASS's frequency loop (src/atlas12_modules.f90:3905-3949):
     accumulator dict and HNU(1) per frequency."""
+    n, nf = _atl_dims(ctx)
     xl = None if xlines is None else _f64(xlines)
+    if xl is not None and xl.size != n * nf:
+        raise ValueError(f"xlines has {xl.size} elements, the state needs {nf} x {n}")
     ctx._check(ctx.lib.a12_atlas_rt_run(ctx.h, None if xl is None else _ptr(xl)),
                "a12_atlas_rt_run")
     if not fetch:
         return None, None
-    n, nf = ctx._atl
     acc = np.zeros(9 * n)
     h1 = np.zeros(nf)
     ctx._check(ctx.lib.a12_atlas_rt_fetch(ctx.h, _ptr(acc), _ptr(h1)), "a12_atlas_rt_fetch")
@@ -669,3 +712,4 @@ def mlp_gemm(ctx: Context, a_bits, b_bits, split: int = 1) -> np.ndarray:
                                     c.ctypes.data_as(C.c_void_p), C.c_int32(split)),
                "a12_mlp_gemm")
     return c
+

@@ -208,8 +208,23 @@ typedef struct {
     double teff;             /* TEFF (TCORR's cool-star rule)                  */
 } a12_atlas_atm;
 
-/* Loads the state and the line store LINEDATA(4, nlines) (as SELECTLINES
- * leaves it); unpacks, bins and sorts the lines by grid index. */
+/* The line store LINEDATA(4, nlines), as SELECTLINES leaves it, on the
+ * grid of atm (only numnu, waveset, nwave1, iwavetab and mion are read):
+ * the per-record decoding of LINOP1 (src/atlas12_modules.f90:18176-18236:
+ * TABLOG fields, continuum bin, vacuum wavelength, grid index NU, CGF, the
+ * damping constants), then a stable sort by NU.  Once per store; nothing in
+ * it depends on the atmosphere.  Invalidates any state set before. */
+int a12_atlas_load_lines(a12_ctx *ctx, const a12_atlas_atm *atm, const int32_t *linedata,
+                         int32_t nlines);
+/* The per-iteration state LINOP1 and the RUN_RT_PASS loop read (module
+ * arrays T, XNE, HCKT, HKT, RHOX, XNF(:,1/3/841) -> TXNXN as LINOP1 forms it
+ * at :18163-18164, XNFDOP, DOPPLE, TABCONT, RCOSET, TABVOIGT, TEFF) on the
+ * grid of the loaded store (A12_ERR_STATE if numnu, waveset, nwave1,
+ * iwavetab or mion differ).  Once per iteration, before linop1/set_kapp;
+ * it invalidates XLINES, the KAPP tables and the RT-pass sums of the
+ * previous state. */
+int a12_atlas_set_state(a12_ctx *ctx, const a12_atlas_atm *atm);
+/* Both of the above in one call. */
 int a12_atlas_load(a12_ctx *ctx, const a12_atlas_atm *atm, const int32_t *linedata,
                    int32_t nlines);
 /* LINOP1, src/atlas12_modules.f90:18132-18374, into the context's device

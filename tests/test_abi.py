@@ -120,3 +120,27 @@ END PROGRAM t
     assert sizes == [32, ctypes.sizeof(native.SyntheAtm), ctypes.sizeof(native.AtlasAtm), 32,
                      ctypes.sizeof(native.TcorrIn), ctypes.sizeof(native.MolTab)]
     assert [int(x) for x in out[6:8]] == [-1, -1]
+
+
+def test_atlas_wrappers_reject_wrong_shapes():
+    """KAPP tables and XLINES of another state's size are refused on the
+    host before the library is called (the C side copies nrhox * numnu
+    doubles from each pointer and cannot see a short array)."""
+    import numpy as np
+    import pytest
+    from atlas12_amd import native
+
+    class Ctx:  # a context whose state is 72 layers x 240 frequencies; no GPU needed
+        _atl = (72, 240)
+        lib = None
+        h = None
+
+    good = np.zeros((240, 72))
+    bad = np.zeros((200, 72))
+    with pytest.raises(ValueError, match="sigmac"):
+        native.atlas_set_kapp(Ctx(), good, bad, good, good, good, good)
+    with pytest.raises(ValueError, match="xlines"):
+        native.atlas_rt_pass(Ctx(), bad)
+    with pytest.raises(native.A12Error, match="no ATLAS12 state"):
+        native.atlas_set_kapp(type("C0", (), {"lib": None, "h": None})(), good, good, good,
+                              good, good, good)

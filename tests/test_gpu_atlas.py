@@ -206,3 +206,59 @@ def test_linop1_every_tile_shape_same_bits(gpu_ctx, atl):
     best, ms = native.atlas_autotune(gpu_ctx, native.PREC_MIXED)
     assert 0 <= best < len(shapes) and ms[best] > 0
     native.atlas_set_shape(gpu_ctx, 0)
+
+
+def test_new_state_invalidates_previous_results(gpu_ctx, atl, coef):
+    """A state set on a larger grid after a complete RT pass: the KAPP
+    tables, the RT-pass sums and XLINES of the previous state are gone, so
+    rt_run, rt_fetch and TCORR(3) on device sums refuse with A12_ERR_STATE
+    (-4) before launching anything, instead of reading the old, smaller
+    buffers."""
+    from atlas12_amd import native
+    z, d = atl
+    gpu_ctx.set_operators(*coef)
+    native.atlas_load(gpu_ctx, d, z["linedata"])
+    native.atlas_set_kapp(gpu_ctx, z["acont"], z["sigmac"], z["scont"], z["sline"], z["aline"],
+                          z["sigmal"])
+    native.atlas_rt_pass(gpu_ctx, z["xlines"])
+    zz = {k: z[k] for k in z.files}
+    w = np.asarray(z["waveset"], float)
+    zz["waveset"] = np.sort(np.concatenate([w, np.sqrt(w[:-1] * w[1:])]))
+    zz["rcoset"] = np.full(len(zz["waveset"]), float(np.mean(z["rcoset"])) / 2)
+    native.atlas_load(gpu_ctx, native.atlas_state_from(zz), z["linedata"])
+    with pytest.raises(native.A12Error, match="status -4"):
+        native.atlas_rt_pass(gpu_ctx)
+    assert gpu_ctx.lib.a12_atlas_rt_fetch(gpu_ctx.h, None, None) == -4
+    n = len(z["t"])
+    tz = {"n": n, "ifconv": 0, "iter": 1, "flux": float(z["flux"]), "teff": float(z["teff"]),
+          "mixlth": 1.25}
+    for k in native.TCORR_STATE:
+        tz[k] = np.ones(n)
+    with pytest.raises(native.A12Error, match="status -4"):
+        native.atlas_tcorr3(gpu_ctx, tz, device_sums=True)
+    # a state on another grid than the store's is refused as well
+    native.atlas_load_lines(gpu_ctx, native.atlas_state_from(z), z["linedata"])
+    with pytest.raises(native.A12Error, match="status -4"):
+        native.atlas_set_state(gpu_ctx, native.atlas_state_from(zz))
+
+
+def test_split_load_equals_one_call_load(gpu_ctx, atl, orc):
+    """load_lines once, then set_state per iteration: XLINES and the
+    contribution count equal a12_atlas_load's, and a second state on the
+    same store (T scaled by 1.02) equals a fresh load of that state."""
+    from atlas12_amd import native
+    z, d = atl
+    native.atlas_load(gpu_ctx, d, z["linedata"])
+    xl0, ev0 = native.atlas_linop1(gpu_ctx, native.PREC_FP64)
+    native.atlas_load_lines(gpu_ctx, d, z["linedata"])
+    native.atlas_set_state(gpu_ctx, d)
+    xl1, ev1 = native.atlas_linop1(gpu_ctx, native.PREC_FP64)
+    assert ev1 == ev0 and np.array_equal(xl1, xl0)
+    d2 = dict(d)
+    d2["t"] = d["t"] * 1.02
+    native.atlas_set_state(gpu_ctx, d2)
+    xl2, ev2 = native.atlas_linop1(gpu_ctx, native.PREC_FP64)
+    native.atlas_load(gpu_ctx, d2, z["linedata"])
+    xl3, ev3 = native.atlas_linop1(gpu_ctx, native.PREC_FP64)
+    assert ev2 == ev3 and np.array_equal(xl2, xl3)
+    assert not np.array_equal(xl2, xl0)
