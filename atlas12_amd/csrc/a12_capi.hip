@@ -23,6 +23,12 @@ int a12_ctx_create(int32_t device, a12_ctx **out)
     a12_ctx *c = new (std::nothrow) a12_ctx();
     if (!c) return A12_ERR_OOM;
     c->device = device;
+    // the no-allocation RT pass's fixed scratch array: its address on this
+    // device, fetched once per device with the device current
+    if (a12i_rt_scratch_init(device) != A12_OK) {
+        delete c;
+        return A12_ERR_HIP;
+    }
     // default priority stream
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
@@ -403,6 +409,17 @@ int a12_atlas_rt_fetch(a12_ctx *ctx, double *acc, double *hnu1)
                                ctx->stream));
     A12_HIP(hipStreamSynchronize(ctx->stream));
     return A12_OK;
+}
+
+int a12_atlas_rt_pass_noalloc(a12_ctx *ctx, int32_t nrhox, int32_t numnu, double teff,
+                              const double *d_waveset, const double *d_rcoset,
+                              const double *d_depth3, const double *d_kapp,
+                              const double *d_xlines, double *d_hnu1, double *d_acc)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_atlas_rt_pass_noalloc(ctx, nrhox, numnu, teff, d_waveset, d_rcoset, d_depth3,
+                                      d_kapp, d_xlines, d_hnu1, d_acc);
 }
 
 int a12_scale_model(a12_ctx *ctx, int32_t nmodel, int32_t nrhox, int32_t kw, double tau1lg,

@@ -179,6 +179,14 @@ void a12i_mlp_free(a12_ctx *ctx);
 int a12i_atlas_rt_pass(a12_ctx *ctx, int n, int nf, double teff, const double *d_wave,
                        const double *d_rcoset, const double *d_depth3, const double *d_kapp,
                        const double *d_xlines, double *d_hnu1, double *d_acc);
+// The RT pass with its chunk partials in the library's fixed device array
+// (no allocation); a12i_rt_scratch_init fetches that array's address on
+// the current device (once per device, from a12_ctx_create).
+int a12i_rt_scratch_init(int device);
+int a12i_atlas_rt_pass_noalloc(a12_ctx *ctx, int n, int nf, double teff, const double *d_wave,
+                               const double *d_rcoset, const double *d_depth3,
+                               const double *d_kapp, const double *d_xlines, double *d_hnu1,
+                               double *d_acc);
 int a12i_synthe_transfer(a12_ctx *ctx);
 int a12i_synthe_transfer_launch(a12_ctx *ctx, hipStream_t st, bool f64, int L, int n,
                                 double wbegin, double ratio, int nedge, const double *wledge,

@@ -21,6 +21,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "a12_common.h"
@@ -706,7 +708,7 @@ __global__ __launch_bounds__(64) void k_rt_pass(
     // this chunk's accumulators live in its own slice of part (nothing stays
     // in registers across JOSH); the same additions in the same order
     double *pp = part + (size_t)blockIdx.x * 9 * n;
-    for (int i = ln; i < 8 * n; i += 64) pp[i] = 0.0;
+    for (int i = ln; i < 9 * n; i += 64) pp[i] = 0.0;  // row 8: PRADK0, then zeros
     wsync();
     double pradk0 = 0.0;
     const int k0 = blockIdx.x * chunk, k1 = min(nf, k0 + chunk);
@@ -1114,6 +1116,8 @@ int a12i_josh_batch(a12_ctx *ctx, int ifscat, int ifsurf, int nrhox, int nbatch,
     return A12_OK;
 }
 
+// Both RT-pass entry points launch the same two kernels with the same
+// chunking: their sums are the same bits.
 static int rt_pass_launch(a12_ctx *ctx, int n, int nf, int nchunk, double teff,
                           const double *d_wave, const double *d_rcoset, const double *d_depth3,
                           const double *d_kapp, const double *d_xlines, double *d_hnu1,
@@ -1145,6 +1149,69 @@ int a12i_atlas_rt_pass(a12_ctx *ctx, int n, int nf, double teff, const double *d
     if (rc) return rc;
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[13], ctx->stream));
     return A12_OK;
+}
+
+// ---- RT pass without allocation ------------------------------------------
+// The chunk partials (nchunk, 9, nrhox) of a12_atlas_rt_pass_noalloc live in
+// this fixed array: 9 x 1,179,648 doubles (84.9 MB), enough for nrhox *
+// ceil(numnu / 2) <= 1,179,648 (72 layers x 32,768 frequencies, 128 x
+// 18,432).  Every GPU the library runs on holds its own copy until the
+// process ends.  Host code never passes `scratch` itself to a kernel (on the
+// host that name is a stand-in, not a device address): it passes the
+// address hipGetSymbolAddress returned on that device.
+constexpr size_t kRtScratchDoubles = (size_t)9 * A12_RT_NOALLOC_MAX_CELLS;
+static __device__ __align__(256) double scratch[kRtScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int a12i_rt_scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return A12_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return A12_OK;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return A12_ERR_HIP;
+    g_scratch_ptr[device] = static_cast<double *>(p);
+    return A12_OK;
+}
+
+int a12i_atlas_rt_pass_noalloc(a12_ctx *ctx, int n, int nf, double teff, const double *d_wave,
+                               const double *d_rcoset, const double *d_depth3,
+                               const double *d_kapp, const double *d_xlines, double *d_hnu1,
+                               double *d_acc)
+{
+    if (!ctx->have_ops) {
+        ctx->err = "a12_atlas_rt_pass_noalloc: call a12_set_operators first";
+        return A12_ERR_STATE;
+    }
+    if (n < 8 || n > A12_NRMAX_LAYERS || nf < 3 || !d_wave || !d_rcoset || !d_depth3 ||
+        !d_kapp || !d_xlines || !d_hnu1 || !d_acc) {
+        ctx->err = "a12_atlas_rt_pass_noalloc: bad dimensions or null pointer";
+        return A12_ERR_ARG;
+    }
+    const int nchunk = (nf + kRtChunk - 1) / kRtChunk;
+    if ((size_t)n * nchunk > (size_t)A12_RT_NOALLOC_MAX_CELLS) {
+        ctx->err = "a12_atlas_rt_pass_noalloc: nrhox * ceil(numnu / 2) exceeds " +
+                   std::to_string(A12_RT_NOALLOC_MAX_CELLS) + " (the fixed scratch array)";
+        return A12_ERR_ARG;
+    }
+    double *part = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        part = ctx->device < kMaxDevices ? g_scratch_ptr[ctx->device] : nullptr;
+    }
+    if (!part) {
+        ctx->err = "a12_atlas_rt_pass_noalloc: scratch address unknown on this device";
+        return A12_ERR_STATE;
+    }
+    // k_rt_pass zeroes each chunk's slice before summing into it, so the
+    // previous call's contents never leak into this one
+    return rt_pass_launch(ctx, n, nf, nchunk, teff, d_wave, d_rcoset, d_depth3, d_kapp, d_xlines,
+                          d_hnu1, part, d_acc);
 }
 
 int a12i_scale_model(a12_ctx *ctx, int nmodel, int n, int kw, double tau1lg, double steplg,

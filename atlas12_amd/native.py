@@ -40,7 +40,7 @@ EXPORTS = ["a12_ctx_create", "a12_ctx_destroy", "a12_last_error", "a12_stream",
            "a12_synthe_shape_info", "a12_synthe_set_shape", "a12_synthe_batch_load",
            "a12_synthe_batch_set_budget", "a12_synthe_batch_run", "a12_synthe_batch_fetch",
            "a12_synthe_batch_info", "a12_solvit_batched", "a12_atlas_load_lines",
-           "a12_atlas_set_state"]
+           "a12_atlas_set_state", "a12_atlas_rt_pass_noalloc"]
 
 
 class A12Error(RuntimeError):
@@ -713,3 +713,15 @@ def mlp_gemm(ctx: Context, a_bits, b_bits, split: int = 1) -> np.ndarray:
                "a12_mlp_gemm")
     return c
 
+
+def atlas_rt_pass_noalloc(ctx: Context, n: int, nf: int, teff: float, d_waveset: int,
+                          d_rcoset: int, d_depth3: int, d_kapp: int, d_xlines: int,
+                          d_hnu1: int, d_acc: int):
+    """a12_atlas_rt_pass_noalloc on device addresses (ints): the RT pass
+    with no allocation, launched on the context's stream (not synchronised).
+    Sizes are checked by the library against its fixed scratch array."""
+    vp = C.c_void_p
+    ctx._check(ctx.lib.a12_atlas_rt_pass_noalloc(
+        ctx.h, C.c_int32(n), C.c_int32(nf), C.c_double(teff), vp(d_waveset), vp(d_rcoset),
+        vp(d_depth3), vp(d_kapp), vp(d_xlines), vp(d_hnu1), vp(d_acc)),
+        "a12_atlas_rt_pass_noalloc")

@@ -254,9 +254,32 @@ int a12_atlas_set_kapp(a12_ctx *ctx, const double *acont, const double *sigmac,
  * XLINES of the last linop1 run, or xlines (nrhox, numnu) when given. */
 int a12_atlas_rt_run(a12_ctx *ctx, const double *xlines);
 /* acc (9, nrhox): RJMINS, RDABH, RDIAGJ, FLXRAD, RADEN, H_TOTAL, ACCRAD,
- * ABROSS sums, then PRADK0 in acc[8*nrhox]; hnu1 (numnu): HNU(1) per
- * frequency after the floors. */
+ * ABROSS sums, then PRADK0 in acc[8*nrhox] (the rest of that row is 0);
+ * hnu1 (numnu): HNU(1) per frequency after the floors. */
 int a12_atlas_rt_fetch(a12_ctx *ctx, double *acc, double *hnu1);
+
+/* No counterpart in the reference: the RT pass of a12_atlas_rt_run for
+ * callers that cannot allocate device memory (code under HIP graph
+ * capture, tight iteration loops, harnesses that forbid hipMalloc).  Every
+ * pointer is device memory the caller owns:
+ *   d_waveset, d_rcoset (numnu); d_depth3 (nrhox, 3): T, HKT, RHOX;
+ *   d_kapp (nrhox, numnu, 6): ACONT, SIGMAC, SCONT, SLINE, ALINE, SIGMAL;
+ *   d_xlines (nrhox, numnu); out d_hnu1 (numnu), d_acc (9, nrhox) as
+ *   a12_atlas_rt_fetch lays it out.
+ * Launches on the context's stream and returns without synchronising.
+ * The per-chunk partial sums go to one fixed device array of the library
+ * (9 * A12_RT_NOALLOC_MAX_CELLS doubles, 84.9 MB per GPU, held until the
+ * process ends): nrhox * ceil(numnu / 2) must not exceed
+ * A12_RT_NOALLOC_MAX_CELLS (A12_ERR_ARG otherwise).  All calls on one GPU
+ * share that array, so calls that overlap in time on one GPU (two streams
+ * or two host threads) race; calls in one stream do not.  Results are the
+ * bits of a12_atlas_rt_run on the same inputs.  Requires
+ * a12_set_operators. */
+#define A12_RT_NOALLOC_MAX_CELLS 1179648
+int a12_atlas_rt_pass_noalloc(a12_ctx *ctx, int32_t nrhox, int32_t numnu, double teff,
+                              const double *d_waveset, const double *d_rcoset,
+                              const double *d_depth3, const double *d_kapp,
+                              const double *d_xlines, double *d_hnu1, double *d_acc);
 
 /* TCORR(3) sections A-F, src/atlas12_modules.f90:2528-2752: the
  * flux-constancy (DTFLUX), Lambda-iteration (DTLAMB) and surface (DTSURF)
