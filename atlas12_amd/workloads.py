@@ -560,3 +560,20 @@ def large_synthe_lines(grid: Grid, n: int = 50_000_000, seed: int = 7,
         out[dst] = (np.power(10.0, g[idx]) / frq4pi).astype(np.float32)
     out["code"] = code[idx]
     return out
+
+
+def kapp_standin(z, numnu):
+    """KAPP outputs at every frequency from the reference's values at every
+    100th: log-linear in frequency index where a table is positive, linear
+    where it holds zeros."""
+    sub = np.asarray(z["kapp_sub"], np.float64)
+    x = np.arange(numnu, dtype=np.float64)
+    out = []
+    for k in ("acont", "sigmac", "scont", "sline", "aline", "sigmal"):
+        s = np.asarray(z[k + "_sub"])  # (nsub, nrhox)
+        if np.all(s > 0):
+            f = np.exp(np.stack([np.interp(x, sub, np.log(s[:, j])) for j in range(s.shape[1])], 1))
+        else:
+            f = np.stack([np.interp(x, sub, s[:, j]) for j in range(s.shape[1])], 1)
+        out.append(np.ascontiguousarray(f))
+    return out

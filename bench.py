@@ -58,7 +58,7 @@ def parse():
     ap.add_argument("--no-synthe", action="store_true", help="skip the plain-SYNTHE block")
     ap.add_argument("--cpu-depths", type=int, default=24)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-SYNTHE block")
-    ap.add_argument("--batch-models", type=int, default=32,
+    ap.add_argument("--batch-models", type=int, default=256,
                     help="models of the 256-model batched-SYNTHE workload that are run "
                          "(the first ones drawn; 256 = all)")
     ap.add_argument("--batch-lines", type=int, default=10_000_000)
@@ -87,23 +87,6 @@ def traffic_table():
 
 
 # --------------------------------------------------------------- ATLAS12
-def kapp_standin(z, numnu):
-    """KAPP outputs at every frequency from the reference's values at every
-    100th: log-linear in frequency index where a table is positive, linear
-    where it holds zeros."""
-    sub = np.asarray(z["kapp_sub"], np.float64)
-    x = np.arange(numnu, dtype=np.float64)
-    out = []
-    for k in ("acont", "sigmac", "scont", "sline", "aline", "sigmal"):
-        s = np.asarray(z[k + "_sub"])  # (nsub, nrhox)
-        if np.all(s > 0):
-            f = np.exp(np.stack([np.interp(x, sub, np.log(s[:, j])) for j in range(s.shape[1])], 1))
-        else:
-            f = np.stack([np.interp(x, sub, s[:, j]) for j in range(s.shape[1])], 1)
-        out.append(np.ascontiguousarray(f))
-    return out
-
-
 def atlas_workload(args, native, wl):
     z = np.load(os.path.join(GOLDEN, "atlas_plain_state.npz"))
     d = native.atlas_state_from(z)
@@ -120,34 +103,64 @@ def atlas_run(args, ctx, native, z, d, pred, tio):
     t0 = time.perf_counter()
     store = native.atlas_select_lines(ctx, z, pred, tio)
     t_sel = time.perf_counter() - t0
-    native.atlas_load(ctx, d, store)
-    kapp = kapp_standin(z, len(z["waveset"]))
-    native.atlas_set_kapp(ctx, *kapp)
     log(f"atlas: SELECTLINES kept {len(store)} lines ({t_sel:.2f}s incl. transfers)")
+    # the line store: decoded, binned and sorted once per store
+    t0 = time.perf_counter()
+    native.atlas_load_lines(ctx, d, store)
+    t_lines = time.perf_counter() - t0
+    log(f"atlas: line store loaded in {t_lines:.2f}s (once per store)")
+    native.atlas_set_state(ctx, d)
+    kapp = wl.kapp_standin(z, len(z["waveset"]))
+    native.atlas_set_kapp(ctx, *kapp)
     shapes = native.atlas_shapes(ctx)
     best, tune_ms = native.atlas_autotune(ctx, native.PREC_MIXED)
     log(f"atlas: autotuned gather shape {shapes[best]} (pts, lanes, batch, form 0 staged / 1 flat "
         "/ 2 per wave); ms per shape "
         + ", ".join(f"{s}:{m:.1f}" for s, m in zip(shapes, tune_ms)))
+    # TCORR(3)'s structure columns for this deck's first iteration (the
+    # reference's own ROSS(3)/CONVEC outputs); its sums come from the RT pass
+    # on the device
+    tz = dict(np.load(os.path.join(GOLDEN, "tcorr_ref.npz")))
 
     import torch
 
     def steps(prec, k):
+        """k iterations of the driver's per-iteration call sequence: the
+        state, LINOP1, KAPP's tables, the RT pass, TCORR(3) on the device
+        sums (synchronous: it returns T1 to the host)."""
         st = np.zeros(4)
         torch.cuda.synchronize()
         t = time.perf_counter()
         for _ in range(k):
+            native.atlas_set_state(ctx, d)
             native.atlas_linop1(ctx, prec, fetch=False)
+            native.atlas_set_kapp(ctx, *kapp)
             native.atlas_rt_pass(ctx, fetch=False)
+            native.atlas_tcorr3(ctx, tz, device_sums=True)
             st += native.atlas_stage_ms(ctx)  # waits on this step's events
         torch.cuda.synchronize()
         return time.perf_counter() - t, st / max(k, 1)
+
+    def host_ms(fn, reps=5):
+        """Wall time of one synchronous call, stream idle before it."""
+        out = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            fn()
+            out.append(time.perf_counter() - t)
+        return 1e3 * float(np.median(out))
 
     steps(native.PREC_MIXED, args.warmup)
     if args.profile_only:
         steps(native.PREC_MIXED, args.steps)
         steps(native.PREC_FP64, 1)
         return None
+    parts = {"set_state": host_ms(lambda: native.atlas_set_state(ctx, d))}
+    native.atlas_linop1(ctx, native.PREC_MIXED, fetch=False)
+    parts["set_kapp_upload"] = host_ms(lambda: native.atlas_set_kapp(ctx, *kapp))
+    native.atlas_rt_pass(ctx, fetch=False)
+    parts["tcorr3"] = host_ms(lambda: native.atlas_tcorr3(ctx, tz, device_sums=True))
     log("atlas: timed mixed")
     t_mix, st_mix = steps(native.PREC_MIXED, args.steps)
     _, ev_mix = native.atlas_linop1(ctx, native.PREC_MIXED)
@@ -164,7 +177,7 @@ def atlas_run(args, ctx, native, z, d, pred, tio):
     lo_64 = float(st_64[:3].sum())
     nf = len(z["waveset"])
     return dict(shape=shapes[best], tune={str(k): float(m) for k, m in zip(shapes, tune_ms)},
-                store=len(store), t_sel=t_sel, t_mix=t_mix / args.steps, t_64=t_64 / k64,
+                store=len(store), t_sel=t_sel, t_lines=t_lines, parts=parts, t_mix=t_mix / args.steps, t_64=t_64 / k64,
                 st_mix=st_mix, st_64=st_64, ev_mix=ev_mix, ev_64=ev_64, lo_mix=lo_mix, lo_64=lo_64,
                 flux_err=flux_err, flx_err=flx_err, nf=nf, store_lines=store)
 
@@ -465,18 +478,47 @@ def mlp_run(args, ctx, native):
 
 
 # ------------------------------------------------------------ CPU legs
+def host_cpu():
+    """The host the CPU legs run on: logical CPUs of the machine, those this
+    process may run on, the OpenMP thread setting, and the CPU model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"nproc": os.cpu_count(), "affinity_cpus": aff,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "model": model}
+
+
+def cpu_threads():
+    """Threads for the CPU legs: every CPU this process may run on, capped by
+    OMP_NUM_THREADS when the environment sets it (the host's share for one
+    GPU)."""
+    n = host_cpu()["affinity_cpus"] or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
+
+
 def cpu_atlas(z, store):
     """The oracle's LINOP1 over the whole selected store on the host cores
     (lines split into contiguous chunks, one per OpenMP thread)."""
     from oracle import oracle as orc
     orc.build()
-    threads = orc.lib().or_omp_threads(min(os.cpu_count() or 1, 16))
+    threads = orc.lib().or_omp_threads(cpu_threads())
     zz = {k: z[k] for k in z.files}
     zz["linedata"] = store
     t0 = time.perf_counter()
     _, ev = orc.linop1(zz, threads=threads)
     dt = time.perf_counter() - t0
     return {"value": ev / dt, "unit": "evals/s", "cores": int(threads), "kind": "port",
+            "host": host_cpu(),
             "sample": f"LINOP1 over the whole selected store ({len(store)} lines, 72 depths, "
                       f"30,000 points): {ev} evals in {dt:.2f} s"}
 
@@ -490,7 +532,7 @@ def cpu_synthe(st, lines, ndepth):
     from oracle import oracle as orc
     orc.build()
     L = orc.lib()
-    threads = L.or_omp_threads(min(os.cpu_count() or 1, 16))
+    threads = L.or_omp_threads(cpu_threads())
     a, keep = orc._atm(st)
     idx = np.unique(np.linspace(0, st.nrhox - 1, ndepth).round().astype(np.int32))
     out = np.zeros((st.nrhox, st.length), np.float32)
@@ -503,7 +545,7 @@ def cpu_synthe(st, lines, ndepth):
                                       orc._p(out, orc.P_F), None, ctypes.byref(e))
     dt = time.perf_counter() - t0
     return {"value": e.value / dt, "unit": "evals/s", "cores": int(min(threads, len(idx))),
-            "kind": "port",
+            "kind": "port", "host": host_cpu(),
             "sample": f"{len(idx)} of {st.nrhox} depths (one per thread), all {len(lines)} lines "
                       f"and {st.length} points: {e.value} evals in {dt:.1f} s"}
 
@@ -586,6 +628,14 @@ def main():
                    "raw_lines": int(args.lines), "selected_lines": int(a["store"]),
                    "evals_per_step": int(ev)},
         "s_per_iter": a["t_mix"],
+        "s_per_iter_covers": "per iteration: a12_atlas_set_state (host arrays -> device), "
+                             "LINOP1, a12_atlas_set_kapp (upload of KAPP's six tables, computed "
+                             "on the host), the RUN_RT_PASS loop, TCORR(3) on the device sums; "
+                             "not KAPP itself",
+        "host_ms_per_call": a["parts"],
+        "line_store_load_s": a["t_lines"],
+        "line_store_load_note": "once per store (decode, bin and sort of the selected lines); "
+                                "before the store/state split it was paid on every state",
         "s_per_iter_fp64": a["t_64"],
         "fp64_gpu": {"value": value64, "unit": "evals/s", "s_per_iter": a["t_64"],
                      "evals_per_step": int(a["ev_64"])},

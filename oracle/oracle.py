@@ -202,6 +202,47 @@ def synthe_line_opacity(st, lines, prec="ref", want_nstep=False):
     return out, ev.value
 
 
+def synthe_depth_split(st, lines, j, prec="f64", threads=16):
+    """Opacity of one depth j over all points with the lines split across
+    threads (or_synthe_depth_split): (row (length,), evals).  prec 'f64':
+    FP64; 'ref': the reference's FP32 arithmetic.  Every profile value and
+    the count are the serial routine's; only the order of the sums differs."""
+    a, keep = _atm(st)
+    lines = np.ascontiguousarray(lines)
+    f64 = prec != "ref"
+    row = np.zeros(st.length, np.float64 if f64 else np.float32)
+    ev = C.c_int64(0)
+    L = lib()
+    L.or_omp_threads(int(threads))
+    L.or_synthe_depth_split(C.byref(a), lines.ctypes.data_as(C.c_void_p), C.c_int(len(lines)),
+                            C.c_int(1 if f64 else 0), C.c_int(int(j)),
+                            None if f64 else _p(row, P_F), _p(row) if f64 else None,
+                            C.byref(ev))
+    return row, ev.value
+
+
+def synthe_depths(st, lines, js, prec="f64", threads=16):
+    """Opacity rows of the depths js (one depth per thread, each the serial
+    routine): (rows (len(js), length), evals over those depths)."""
+    a, keep = _atm(st)
+    lines = np.ascontiguousarray(lines)
+    idx = np.ascontiguousarray(js, np.int32)
+    ev = C.c_int64(0)
+    L = lib()
+    L.or_omp_threads(int(threads))
+    if prec == "ref":
+        out = np.zeros((st.nrhox, st.length), np.float32)
+        args = (_p(out, P_F), None)
+    else:
+        out = np.zeros((st.nrhox, st.length))
+        args = (None, _p(out))
+    L.or_synthe_line_opacity_omp_list(C.byref(a), lines.ctypes.data_as(C.c_void_p),
+                                      C.c_int(len(lines)), C.c_int(0 if prec == "ref" else 1),
+                                      idx.ctypes.data_as(C.c_void_p), C.c_int(len(idx)), *args,
+                                      C.byref(ev))
+    return out[idx], ev.value
+
+
 def _rt(st, coefj, coefh):
     k = _Keep()
     k.arrs = {n: _f64(v) for n, v in (("wledge", st.wledge), ("contabs", st.contabs),
