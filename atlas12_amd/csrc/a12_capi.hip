@@ -57,6 +57,10 @@ int a12_ctx_destroy(a12_ctx *ctx)
     DevBuf *sbufs[] = {&L.tabcont, &L.xnfdop, &L.iwt, &L.wavetab, &L.tablog, &L.xm,
                        &L.cand, &L.flag, &L.cnt, &L.off, &L.kept, &L.mol2};
     for (DevBuf *b : sbufs) b->release();
+    ModelDev &M = ctx->mdl;
+    DevBuf *mbufs[] = {&M.ft, &M.taustd, &M.par, &M.t, &M.ot, &M.rd, &M.dep, &M.part,
+                       &M.acc, &M.hnu, &M.tin, &M.tout, &M.conv, &M.hist, &M.ctl};
+    for (DevBuf *b : mbufs) b->release();
     ctx->mol.tab.release();
     ctx->mol.io.release();
     for (DevBuf *b : abufs) b->release();
@@ -420,6 +424,15 @@ int a12_atlas_rt_pass_noalloc(a12_ctx *ctx, int32_t nrhox, int32_t numnu, double
     A12_HIP(hipSetDevice(ctx->device));
     return a12i_atlas_rt_pass_noalloc(ctx, nrhox, numnu, teff, d_waveset, d_rcoset, d_depth3,
                                       d_kapp, d_xlines, d_hnu1, d_acc);
+}
+
+int a12_model_converge(a12_ctx *ctx, int32_t nmodel, const a12_model_par *par,
+                       const double *t_start, int32_t itmax, double *t_out, double *s_out,
+                       int32_t *iters, double *hist)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_model_converge(ctx, nmodel, par, t_start, itmax, t_out, s_out, iters, hist);
 }
 
 int a12_scale_model(a12_ctx *ctx, int32_t nmodel, int32_t nrhox, int32_t kw, double tau1lg,

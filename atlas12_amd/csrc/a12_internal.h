@@ -90,6 +90,12 @@ struct MolDev {
     DevBuf tab, io;
 };
 
+// Model iteration (a12_model_converge): frequency table, per-model state
+// and work arrays, sized for the largest batch run so far.
+struct ModelDev {
+    DevBuf ft, taustd, par, t, ot, rd, dep, part, acc, hnu, tin, tout, conv, hist, ctl;
+};
+
 struct MlpDev;    // a12_mlp.hip
 struct SynBatch;  // a12_synthe.hip
 
@@ -103,6 +109,7 @@ struct a12_ctx {
     AtlasDev atl;
     SelectDev sel;
     MolDev mol;
+    ModelDev mdl;
     MlpDev *mlp = nullptr;
     SynBatch *sbatch = nullptr;
     DevBuf rt_part;
@@ -197,6 +204,9 @@ int a12i_atlas_tcorr3(a12_ctx *ctx, const a12_tcorr_in *in, double *rdabh, doubl
                       double *out);
 int a12i_solvit_batched(a12_ctx *ctx, int n, int nsys, const double *a, const double *b,
                         double *x);
+int a12i_model_converge(a12_ctx *ctx, int nmodel, const a12_model_par *par,
+                        const double *t_start, int itmax, double *t_out, double *s_out,
+                        int32_t *iters, double *hist);
 int a12i_scale_model(a12_ctx *ctx, int nmodel, int n, int kw, double tau1lg, double steplg,
                      const double *sc, const double *fields, double *out, double *so,
                      double *taustd_out);

@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -689,6 +690,73 @@ __device__ double expi_dev(int n, double x)
 // part: (nchunk, 9, n) -- rjmins rdabh rdiagj flxrad raden h_total accrad
 // abross, then pradk0 in slot [8][0].
 // --------------------------------------------------------------------------
+// One frequency of the RUN_RT_PASS loop after KAPP: W.bnu, W.abtot, W.alpha,
+// W.snubar and s_stim are set for every depth by the caller (the floors of
+// W.snu/jnu/hnu/jmins zeroed); JOSH(1, 0), the negative floors, HNU(1) to
+// *hnu1 (lane 0), and the TCORR(2)/RADIAP(2)/ROSS(2) terms added into the
+// accumulators pp (9, n) and pradk0 in the reference's order.
+__device__ void rt_freq_accumulate(JoshLDS &W, double *term2, const double *s_stim, int n,
+                                   double freq, double rcowt, double teff,
+                                   const double *__restrict__ t, const double *__restrict__ hkt,
+                                   const double *__restrict__ coefj,
+                                   const double *__restrict__ coefh, double *__restrict__ hnu1,
+                                   double *__restrict__ pp, double &pradk0)
+{
+#pragma clang fp contract(off)
+    const int ln = lane_id();
+    const double euler_m1 = 0.922784335098467;
+    wsync();
+    const JoshOut r = josh_wave(W, n, 1, 0, coefj, coefh);
+    // floor unphysical negatives (:3926-3937)
+    int neg = 0;
+    for (int j = ln; j < n; j += 64)
+        if (W.snu[j] < 0.0 || W.jnu[j] < 0.0 || W.hnu[j] < 0.0) neg = 1;
+    if (__any(neg)) {
+        for (int j = ln; j < n; j += 64) {
+            W.jnu[j] = fmax(W.jnu[j], 1.0e-99);
+            W.snu[j] = fmax(W.snu[j], 1.0e-99);
+            W.hnu[j] = fmax(W.hnu[j], 1.0e-99);
+        }
+    }
+    wsync();
+    if (ln == 0) *hnu1 = W.hnu[0];
+    // TCORR(2): d(abtot)/d(rhox), then the diagonal Lambda terms
+    deriv_wave(W.rhox, W.abtot, W.pb, n);
+    for (int j = ln; j < n; j += 64) {
+        double d = (j < n - 1 ? W.taunu[j + 1] - W.taunu[j] : W.taunu[n - 1] - W.taunu[n - 2]);
+        d = fmax(1.0e-10, d);
+        double t2;
+        if (d <= 0.01) {
+            t2 = (euler_m1 - log(d)) * d / 4.0 + d * d / 12.0 - d * d * d / 96.0 +
+                 d * d * d * d / 720.0;
+        } else {
+            double ex = 0.0;
+            if (d < 10.0) ex = expi_dev(3, d);
+            if (teff <= 4250.0 && d > 0.005 && d < 0.02) ex = 0.0;
+            t2 = 0.5 * (d + ex - 0.5) / d;
+        }
+        term2[j] = t2;
+    }
+    wsync();
+    for (int j = ln; j < n; j += 64) {
+        const double ab = W.abtot[j], al = W.alpha[j], h = W.hnu[j];
+        const double bnu = W.bnu[j], stim = s_stim[j];
+        pp[0 * n + j] = pp[0 * n + j] + ab * W.jmins[j] * rcowt;
+        pp[1 * n + j] = pp[1 * n + j] + W.pb[j] / ab * h * rcowt;
+        const double diagj = (j > 0 ? term2[j - 1] : 0.0) + term2[j];
+        const double dbdt_t = bnu * freq * hkt[j] / t[j] / fmax(stim, 1.0e-30);
+        pp[2 * n + j] = pp[2 * n + j] + ab * (diagj - 1.0) / (1.0 - al * diagj) * (1.0 - al) * dbdt_t * rcowt;
+        pp[3 * n + j] = pp[3 * n + j] + h * rcowt;
+        pp[4 * n + j] = pp[4 * n + j] + W.jnu[j] * rcowt;
+        pp[5 * n + j] = pp[5 * n + j] + h * rcowt;
+        pp[6 * n + j] = pp[6 * n + j] + ab * h * rcowt;
+        const double dbdt_r = bnu * freq * hkt[j] / t[j] / stim;
+        pp[7 * n + j] = pp[7 * n + j] + dbdt_r / ab * rcowt;
+    }
+    pradk0 = pradk0 + r.knu1 * rcowt;
+    wsync();
+}
+
 __global__ __launch_bounds__(64) void k_rt_pass(
     int n, int nf, int chunk, double teff, const double *__restrict__ wave,
     const double *__restrict__ rcoset, const double *__restrict__ depth3,
@@ -702,7 +770,6 @@ __global__ __launch_bounds__(64) void k_rt_pass(
     const int ln = lane_id();
     const double *t = depth3, *hkt = depth3 + n, *rhox = depth3 + 2 * n;
     const size_t plane = (size_t)n * nf;
-    const double euler_m1 = 0.922784335098467;
     const double bnu_prefac = 2.0 * A12_HPLANCK / (A12_CLIGHT * A12_CLIGHT) * 1.0e45;
     for (int j = ln; j < n; j += 64) W.rhox[j] = rhox[j];
     // this chunk's accumulators live in its own slice of part (nothing stays
@@ -714,7 +781,6 @@ __global__ __launch_bounds__(64) void k_rt_pass(
     const int k0 = blockIdx.x * chunk, k1 = min(nf, k0 + chunk);
     for (int k = k0; k < k1; ++k) {
         const double freq = A12_CLIGHT_NMS / wave[k];
-        const double rcowt = rcoset[k];
         const double f15 = freq / 1.0e15;
         for (int j = ln; j < n; j += 64) {
             const size_t o = (size_t)k * n + j;
@@ -731,56 +797,8 @@ __global__ __launch_bounds__(64) void k_rt_pass(
             W.hnu[j] = 0.0;
             W.jmins[j] = 0.0;
         }
-        wsync();
-        const JoshOut r = josh_wave(W, n, 1, 0, coefj, coefh);
-        // floor unphysical negatives (:3926-3937)
-        int neg = 0;
-        for (int j = ln; j < n; j += 64)
-            if (W.snu[j] < 0.0 || W.jnu[j] < 0.0 || W.hnu[j] < 0.0) neg = 1;
-        if (__any(neg)) {
-            for (int j = ln; j < n; j += 64) {
-                W.jnu[j] = fmax(W.jnu[j], 1.0e-99);
-                W.snu[j] = fmax(W.snu[j], 1.0e-99);
-                W.hnu[j] = fmax(W.hnu[j], 1.0e-99);
-            }
-        }
-        wsync();
-        if (ln == 0) hnu1[k] = W.hnu[0];
-        // TCORR(2): d(abtot)/d(rhox), then the diagonal Lambda terms
-        deriv_wave(W.rhox, W.abtot, W.pb, n);
-        for (int j = ln; j < n; j += 64) {
-            double d = (j < n - 1 ? W.taunu[j + 1] - W.taunu[j] : W.taunu[n - 1] - W.taunu[n - 2]);
-            d = fmax(1.0e-10, d);
-            double t2;
-            if (d <= 0.01) {
-                t2 = (euler_m1 - log(d)) * d / 4.0 + d * d / 12.0 - d * d * d / 96.0 +
-                     d * d * d * d / 720.0;
-            } else {
-                double ex = 0.0;
-                if (d < 10.0) ex = expi_dev(3, d);
-                if (teff <= 4250.0 && d > 0.005 && d < 0.02) ex = 0.0;
-                t2 = 0.5 * (d + ex - 0.5) / d;
-            }
-            term2[j] = t2;
-        }
-        wsync();
-        for (int j = ln; j < n; j += 64) {
-            const double ab = W.abtot[j], al = W.alpha[j], h = W.hnu[j];
-            const double bnu = W.bnu[j], stim = s_stim[j];
-            pp[0 * n + j] = pp[0 * n + j] + ab * W.jmins[j] * rcowt;
-            pp[1 * n + j] = pp[1 * n + j] + W.pb[j] / ab * h * rcowt;
-            const double diagj = (j > 0 ? term2[j - 1] : 0.0) + term2[j];
-            const double dbdt_t = bnu * freq * hkt[j] / t[j] / fmax(stim, 1.0e-30);
-            pp[2 * n + j] = pp[2 * n + j] + ab * (diagj - 1.0) / (1.0 - al * diagj) * (1.0 - al) * dbdt_t * rcowt;
-            pp[3 * n + j] = pp[3 * n + j] + h * rcowt;
-            pp[4 * n + j] = pp[4 * n + j] + W.jnu[j] * rcowt;
-            pp[5 * n + j] = pp[5 * n + j] + h * rcowt;
-            pp[6 * n + j] = pp[6 * n + j] + ab * h * rcowt;
-            const double dbdt_r = bnu * freq * hkt[j] / t[j] / stim;
-            pp[7 * n + j] = pp[7 * n + j] + dbdt_r / ab * rcowt;
-        }
-        pradk0 = pradk0 + r.knu1 * rcowt;
-        wsync();
+        rt_freq_accumulate(W, term2, s_stim, n, freq, rcoset[k], teff, t, hkt, coefj, coefh,
+                           hnu1 + k, pp, pradk0);
     }
     if (ln == 0) pp[8 * n] = pradk0;
 }
@@ -867,15 +885,12 @@ struct TcorrLDS {
     double gflux[NR], dtau[NR], dum[NR], tinteg[NR], xs[2], ys[2];
 };
 
-__global__ __launch_bounds__(64) void k_tcorr3(int n, int ifconv, int iter, double flux,
-                                               double teff, double mixlth,
-                                               const double *__restrict__ in,
-                                               double *__restrict__ rdabh,
-                                               double *__restrict__ oldt1, double *__restrict__ out)
+__device__ void tcorr3_wave(JoshLDS &W, TcorrLDS &L, int n, int ifconv, int iter, double flux,
+                            double teff, double mixlth, const double *__restrict__ in,
+                            double *__restrict__ rdabh, double *__restrict__ oldt1,
+                            double *__restrict__ out)
 {
 #pragma clang fp contract(off)
-    __shared__ JoshLDS W;
-    __shared__ TcorrLDS L;
     const int ln = lane_id();
     const double fourpi = 4.0 * 3.141592653589793, sigma_sb = 5.670374419e-5;
     const double abfl = 1.0e-30, rdfl = 1.0e-30, delfl = 1.0e-10, hoff = 1.0e-3;
@@ -980,6 +995,422 @@ __global__ __launch_bounds__(64) void k_tcorr3(int n, int ifconv, int iter, doub
             oldt1[j] = x;
         }
     }
+}
+
+__global__ __launch_bounds__(64) void k_tcorr3(int n, int ifconv, int iter, double flux,
+                                               double teff, double mixlth,
+                                               const double *__restrict__ in,
+                                               double *__restrict__ rdabh,
+                                               double *__restrict__ oldt1, double *__restrict__ out)
+{
+    __shared__ JoshLDS W;
+    __shared__ TcorrLDS L;
+    tcorr3_wave(W, L, n, ifconv, iter, flux, teff, mixlth, in, rdabh, oldt1, out);
+}
+
+// --------------------------------------------------------------------------
+// Model iteration: the ATLAS12 loop (src/atlas12c.f90:496-606) for many
+// models at once on a fixed Rosseland grid TAUSTD(j) = 10**(-6.875 +
+// 0.125 (j-1)), j = 1..72, with KAPP and the EOS replaced by closed forms
+// (KAPP and POPS are not part of this library):
+//   EOS    H, He (0.085 by number) and one metal donor (1e-4 10**[M/H],
+//          ionisation 88,194 K); Saha for H (157,807.7 K) and the donor, n_e
+//          by bisection in log n_e (64 halvings); H- by Saha (8,753.3 K);
+//          rho = n_H m_H (1 + 4 x 0.085 + 20 A_M);
+//   KAPP   H- bound-free (Wishart's fit) and free-free (Gray's fit), H I
+//          bound-free (levels 1-8, Kramers, g = 1) and free-free, the donor's
+//          bound-free (1e-18 cm^2 (nu_M/nu)^2 above 1.839e15 Hz), all times
+//          (1 - e^{-h nu/kT}); electron and H Rayleigh scattering; SCONT =
+//          SLINE = B_nu, no lines;
+//   hydrostatics  dP/dtau = g / kappa_Ross(T, P) on the grid (trapezoid in
+//          ln tau, 4 corrector passes per layer; 30 fixed-point passes at the
+//          top), P = P_gas + rho vturb^2 / 2;
+// and, on the device as in the library's ATLAS12 path: the RUN_RT_PASS
+// frequency loop (JOSH(1,0) + TCORR/RADIAP/ROSS 1-2, rt_freq_accumulate) on
+// 1,200 frequencies (the driver's grid at every 25th point from
+// NUSTART 9599, 91.2 nm - 91.2 um, with its RCOSET rule), ROSS(3), TCORR(3)
+// (tcorr3_wave, no convection) and T = max(T + T1, 1500 K).  A model is
+// converged at the first iteration whose TCORR(3) table has max |ERROR| <=
+// 1 %, max |DERIV| <= 10 % and max |T1| <= 1 K; its structure is then kept.
+// --------------------------------------------------------------------------
+constexpr int kMdN = 72;        // layers
+constexpr int kMdNf = 1200;     // frequencies
+constexpr int kMdChunk = 4;     // frequencies per RT wavefront
+constexpr int kMdFt = 16;       // doubles per frequency in the table
+constexpr int kMdDep = 24;      // per-layer arrays per model
+constexpr double kMdKb = 1.380649e-16, kMdMh = 1.6735575e-24, kMdSigma = 5.670374419e-5;
+constexpr double kMdSigT = 6.6524587e-25, kMdChiH = 157807.7, kMdChiM = 88194.0;
+constexpr double kMdSaha = 2.4147e15, kMdChiHm = 8753.3, kMdHe = 0.085;
+// per-layer array slots in dep (kMdDep x n per model)
+enum { D_T, D_HKT, D_RHOX, D_PTOT, D_PG, D_RHO, D_NE, D_NH0, D_NHP, D_NHM, D_NM0, D_LTH,
+       D_PE, D_HFF, D_S1,           // D_S1 .. D_S1 + 7: sum over n' >= n of pop_n' / n'^5
+       D_ABR = D_S1 + 8 };          // ABROSS of the last RT pass (ROSS(3))
+
+struct MdPop {
+    double ne, nh0, nhp, nhm, nm0, rho;
+};
+
+// Saha/charge balance at (T, P_gas) for metal abundance am.
+__device__ MdPop md_eos(double T, double pg, double am)
+{
+#pragma clang fp contract(off)
+    const double ntot = pg / (kMdKb * T);
+    const double t15 = T * sqrt(T);
+    const double sh = kMdSaha * t15 * exp(-kMdChiH / T);
+    const double sm = kMdSaha * t15 * exp(-kMdChiM / T);
+    double lo = 1.0e-30, hi = ntot;
+    for (int i = 0; i < 64; ++i) {
+        const double ne = sqrt(lo * hi);
+        const double nh = (ntot - ne) / (1.0 + kMdHe + am);
+        const double f = ne - nh * (sh / (ne + sh) + am * sm / (ne + sm));
+        if (f > 0.0) hi = ne;
+        else lo = ne;
+    }
+    MdPop p;
+    p.ne = sqrt(lo * hi);
+    const double nh = (ntot - p.ne) / (1.0 + kMdHe + am);
+    const double fh = sh / (p.ne + sh);
+    p.nh0 = nh * (1.0 - fh);
+    p.nhp = nh * fh;
+    p.rho = nh * kMdMh * (1.0 + 4.0 * kMdHe + 20.0 * am);
+    p.nhm = p.nh0 * p.ne / (4.0 * kMdSaha * t15) * exp(kMdChiHm / T);
+    p.nm0 = am * nh * p.ne / (p.ne + sm);
+    return p;
+}
+
+// Per-layer quantities the opacity reads, from T and the populations.
+struct MdLayer {
+    double T, rho, ne, nh0, nhm, nm0, lth, pe, hff, s[9];
+};
+
+__device__ MdLayer md_layer(double T, const MdPop &p)
+{
+#pragma clang fp contract(off)
+    MdLayer L;
+    L.T = T;
+    L.rho = p.rho;
+    L.ne = p.ne;
+    L.nh0 = p.nh0;
+    L.nhm = p.nhm;
+    L.nm0 = p.nm0;
+    L.lth = log10(5040.0 / T);
+    L.pe = p.ne * kMdKb * T;
+    L.hff = p.ne * p.nhp / sqrt(T);
+    L.s[8] = 0.0;
+    for (int lv = 8; lv >= 1; --lv) {
+        const double n2 = (double)(lv * lv);
+        const double pop = p.nh0 * n2 * exp(-kMdChiH * (1.0 - 1.0 / n2) / T);
+        L.s[lv - 1] = L.s[lv] + pop / (n2 * n2 * (double)lv);
+    }
+    return L;
+}
+
+// Absorption and scattering per gram at frequency table row f, and the
+// stimulated-emission factor and Planck function.
+__device__ __forceinline__ void md_kappa(const double *__restrict__ f, const MdLayer &L,
+                                         double *acont, double *sigmac, double *stim, double *bnu)
+{
+#pragma clang fp contract(off)
+    const double bnu_prefac = 2.0 * A12_HPLANCK / (A12_CLIGHT * A12_CLIGHT) * 1.0e45;
+    const double ehv = exp(-f[2] / L.T);
+    const double st = 1.0 - ehv;
+    const double hmbf = f[3] * L.nhm;
+    const double hmff = 1.0e-26 * L.pe * exp10(f[4] + f[5] * L.lth + f[6] * L.lth * L.lth) * L.nh0;
+    const int nmin = (int)f[9];
+    const double hbf = f[8] * L.s[nmin - 1];
+    const double hff = f[10] * L.hff;
+    const double mbf = f[11] * L.nm0;
+    *acont = ((hmbf + hbf + hff + mbf) * st + hmff) / L.rho;
+    *sigmac = (kMdSigT * L.ne + f[7] * L.nh0) / L.rho;
+    *stim = st;
+    *bnu = bnu_prefac * f[12] * ehv / st;
+}
+
+// Wave-wide FP64 sum, butterfly order (every lane gets the same bits).
+__device__ __forceinline__ double md_wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o);
+    return v;
+}
+
+// kappa_Ross at (T, P_tot): P_gas from the turbulent-pressure ratio r =
+// rho / P_gas (updated), ROSS's mean over the frequency table.
+__device__ double md_kross(const double *__restrict__ ft, double T, double ptot, double am,
+                           double vt2, double *r, MdPop *pop_out)
+{
+#pragma clang fp contract(off)
+    const double pg = ptot / (1.0 + 0.5 * vt2 * *r);
+    const MdPop p = md_eos(T, pg, am);
+    *r = p.rho / pg;
+    const MdLayer L = md_layer(T, p);
+    double s = 0.0;
+    for (int k = lane_id(); k < kMdNf; k += 64) {
+        const double *f = ft + (size_t)k * kMdFt;
+        double ac, sg, st, b;
+        md_kappa(f, L, &ac, &sg, &st, &b);
+        const double dbdt = b * f[0] * (A12_HPLANCK / kMdKb / T) / T / st;
+        s = s + dbdt / (ac + sg) * f[1];
+    }
+    s = md_wave_sum(s);
+    if (pop_out) *pop_out = p;
+    return 4.0 * kMdSigma / 3.141592653589793 * ((T * T) * T) / s;
+}
+
+__device__ __forceinline__ void md_store(double *__restrict__ dep, int n, int j, double T,
+                                         double ptot, double pg, double g, const MdPop &p)
+{
+    if (lane_id() != 0) return;
+    const MdLayer L = md_layer(T, p);
+    dep[D_T * n + j] = T;
+    dep[D_HKT * n + j] = A12_HPLANCK / (kMdKb * T);
+    dep[D_RHOX * n + j] = ptot / g;
+    dep[D_PTOT * n + j] = ptot;
+    dep[D_PG * n + j] = pg;
+    dep[D_RHO * n + j] = p.rho;
+    dep[D_NE * n + j] = p.ne;
+    dep[D_NH0 * n + j] = p.nh0;
+    dep[D_NHP * n + j] = p.nhp;
+    dep[D_NHM * n + j] = p.nhm;
+    dep[D_NM0 * n + j] = p.nm0;
+    dep[D_LTH * n + j] = L.lth;
+    dep[D_PE * n + j] = L.pe;
+    dep[D_HFF * n + j] = L.hff;
+    for (int lv = 0; lv < 8; ++lv) dep[(D_S1 + lv) * n + j] = L.s[lv];
+}
+
+__device__ __forceinline__ MdLayer md_layer_of(const double *__restrict__ dep, int n, int j)
+{
+    MdLayer L;
+    L.T = dep[D_T * n + j];
+    L.rho = dep[D_RHO * n + j];
+    L.ne = dep[D_NE * n + j];
+    L.nh0 = dep[D_NH0 * n + j];
+    L.nhm = dep[D_NHM * n + j];
+    L.nm0 = dep[D_NM0 * n + j];
+    L.lth = dep[D_LTH * n + j];
+    L.pe = dep[D_PE * n + j];
+    L.hff = dep[D_HFF * n + j];
+    for (int lv = 0; lv < 8; ++lv) L.s[lv] = dep[(D_S1 + lv) * n + j];
+    L.s[8] = 0.0;
+    return L;
+}
+
+// Hydrostatic structure of each unconverged model for its current T
+// (one wavefront per model; lanes share the frequency sums).
+__global__ __launch_bounds__(64) void k_md_hydro(int nmodel, const double *__restrict__ par,
+                                                 const double *__restrict__ ft,
+                                                 const double *__restrict__ taustd,
+                                                 const double *__restrict__ tstate,
+                                                 const int *__restrict__ conv,
+                                                 double *__restrict__ dep)
+{
+#pragma clang fp contract(off)
+    const int m = blockIdx.x;
+    if (m >= nmodel || conv[m] != 0) return;
+    const int n = kMdN;
+    const double g = exp10(par[4 * m + 1]), am = 1.0e-4 * exp10(par[4 * m + 2]);
+    const double vt = par[4 * m + 3] * 1.0e5, vt2 = vt * vt;
+    const double *T = tstate + (size_t)m * n;
+    double *d = dep + (size_t)m * kMdDep * n;
+    double r = 0.0;
+    MdPop pop;
+    double p = g * taustd[0] / 1.0e-2;  // any positive start: the top fixed point converges
+    for (int it = 0; it < 30; ++it) p = g * taustd[0] / md_kross(ft, T[0], p, am, vt2, &r, &pop);
+    md_kross(ft, T[0], p, am, vt2, &r, &pop);
+    md_store(d, n, 0, T[0], p, p / (1.0 + 0.5 * vt2 * r), g, pop);
+    double pprev = p;
+    for (int j = 1; j < n; ++j) {
+        const double rprev = r;
+        const double k0 = md_kross(ft, T[j - 1], pprev, am, vt2, &r, nullptr);
+        r = rprev;
+        const double dl = log(taustd[j] / taustd[j - 1]);
+        const double f0 = g * taustd[j - 1] / k0;
+        double pj = pprev + f0 * dl;
+        for (int c = 0; c < 4; ++c) {
+            const double k1 = md_kross(ft, T[j], pj, am, vt2, &r, nullptr);
+            pj = pprev + 0.5 * dl * (f0 + g * taustd[j] / k1);
+        }
+        md_kross(ft, T[j], pj, am, vt2, &r, &pop);
+        md_store(d, n, j, T[j], pj, pj / (1.0 + 0.5 * vt2 * r), g, pop);
+        pprev = pj;
+    }
+}
+
+// RUN_RT_PASS on the closed-form KAPP: grid (chunks, models).
+__global__ __launch_bounds__(64) void k_md_rt(int nmodel, int nchunk, const double *__restrict__ par,
+                                              const double *__restrict__ ft,
+                                              const int *__restrict__ conv,
+                                              const double *__restrict__ dep,
+                                              const double *__restrict__ coefj,
+                                              const double *__restrict__ coefh,
+                                              double *__restrict__ hnu1,
+                                              double *__restrict__ part)
+{
+#pragma clang fp contract(off)
+    __shared__ JoshLDS W;
+    __shared__ double term2[NR], s_stim[NR];
+    const int m = blockIdx.y, c = blockIdx.x;
+    if (m >= nmodel || c >= nchunk || conv[m] != 0) return;
+    const int n = kMdN, ln = lane_id();
+    const double *d = dep + (size_t)m * kMdDep * n;
+    const double teff = par[4 * m];
+    for (int j = ln; j < n; j += 64) W.rhox[j] = d[D_RHOX * n + j];
+    double *pp = part + ((size_t)m * nchunk + c) * 9 * n;
+    for (int i = ln; i < 9 * n; i += 64) pp[i] = 0.0;
+    wsync();
+    double pradk0 = 0.0;
+    const int k0 = c * kMdChunk, k1 = min(kMdNf, k0 + kMdChunk);
+    for (int k = k0; k < k1; ++k) {
+        const double *f = ft + (size_t)k * kMdFt;
+        for (int j = ln; j < n; j += 64) {
+            const MdLayer L = md_layer_of(d, n, j);
+            double ac, sg, st, b;
+            md_kappa(f, L, &ac, &sg, &st, &b);
+            s_stim[j] = st;
+            W.bnu[j] = b;
+            josh_prepare(W, j, ac, 0.0, sg, 0.0, b, b);
+            W.snu[j] = 0.0;
+            W.jnu[j] = 0.0;
+            W.hnu[j] = 0.0;
+            W.jmins[j] = 0.0;
+        }
+        rt_freq_accumulate(W, term2, s_stim, n, f[0], f[1], teff, d + D_T * n, d + D_HKT * n,
+                           coefj, coefh, hnu1 + (size_t)m * kMdNf + k, pp, pradk0);
+    }
+    if (ln == 0) pp[8 * n] = pradk0;
+}
+
+// Per-model fixed-order chunk sums (as k_rt_reduce): grid (9 n, models).
+__global__ __launch_bounds__(256) void k_md_reduce(int nmodel, int nchunk,
+                                                   const int *__restrict__ conv,
+                                                   const double *__restrict__ part,
+                                                   double *__restrict__ acc)
+{
+    __shared__ double red[256];
+    const int n = kMdN, i = blockIdx.x, m = blockIdx.y, t = threadIdx.x;
+    if (m >= nmodel || conv[m] != 0) return;
+    const double *pm = part + (size_t)m * nchunk * 9 * n;
+    double s = 0.0;
+    for (int c = t; c < nchunk; c += 256) s += pm[(size_t)c * 9 * n + i];
+    red[t] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (t < k) red[t] += red[t + k];
+        __syncthreads();
+    }
+    if (t == 0) acc[(size_t)m * 9 * n + i] = red[0];
+}
+
+// ROSS(3), TCORR(3), the convergence test and T + T1, one wavefront per
+// model.  *iter_dev is the iteration number (1-based); hist (nmodel, itmax,
+// 3) gets max |ERROR|, max |DERIV|, max |T1| of this iteration.
+__global__ __launch_bounds__(64) void k_md_update(int nmodel, int itmax,
+                                                  const int *__restrict__ iter_dev,
+                                                  const double *__restrict__ par,
+                                                  const double *__restrict__ acc,
+                                                  double *__restrict__ dep,
+                                                  double *__restrict__ tstate,
+                                                  double *__restrict__ oldt1,
+                                                  double *__restrict__ rdabh,
+                                                  double *__restrict__ tin, double *__restrict__ tout,
+                                                  int *__restrict__ conv, double *__restrict__ hist)
+{
+#pragma clang fp contract(off)
+    __shared__ JoshLDS W;
+    __shared__ TcorrLDS L;
+    __shared__ double s_ab[NR];
+    const int m = blockIdx.x;
+    if (m >= nmodel || conv[m] != 0) return;
+    const int n = kMdN, ln = lane_id(), it = *iter_dev;
+    const double *a = acc + (size_t)m * 9 * n, *d = dep + (size_t)m * kMdDep * n;
+    double *T = tstate + (size_t)m * n, *ot = oldt1 + (size_t)m * n, *rd = rdabh + (size_t)m * n;
+    double *in = tin + (size_t)m * 16 * n, *out = tout + (size_t)m * 7 * n;
+    const double teff = par[4 * m];
+    const double flux = kMdSigma / (4.0 * 3.141592653589793) * (((teff * teff) * teff) * teff);
+    for (int j = ln; j < n; j += 64) {  // ROSS(3)
+        const double tj = T[j];
+        s_ab[j] = 4.0 * kMdSigma / 3.141592653589793 * ((tj * tj) * tj) / a[7 * n + j];
+        dep[((size_t)m * kMdDep + D_ABR) * n + j] = s_ab[j];
+        rd[j] = a[n + j];
+    }
+    wsync();
+    integ_wave(W, d + D_RHOX * n, s_ab, in, n, s_ab[0] * d[D_RHOX * n]);  // TAUROS
+    for (int j = ln; j < n; j += 64) {
+        in[n + j] = d[D_RHOX * n + j];
+        in[2 * n + j] = T[j];
+        in[3 * n + j] = 0.0;
+        in[4 * n + j] = s_ab[j];
+        in[5 * n + j] = 0.0;
+        in[6 * n + j] = 0.0;
+        in[7 * n + j] = 0.0;
+        in[8 * n + j] = d[D_PTOT * n + j];
+        in[9 * n + j] = d[D_RHO * n + j];
+        in[10 * n + j] = 0.0;
+        in[11 * n + j] = 0.0;
+        in[12 * n + j] = 0.0;
+        in[13 * n + j] = a[3 * n + j];
+        in[14 * n + j] = a[j];
+        in[15 * n + j] = a[2 * n + j];
+    }
+    wsync();
+    tcorr3_wave(W, L, n, 0, it, flux, teff, 1.25, in, rd, ot, out);
+    wsync();
+    double e = 0.0, dv = 0.0, dt = 0.0;
+    bool bad = false;
+    for (int j = ln; j < n; j += 64) {
+        e = fmax(e, fabs(out[4 * n + j]));
+        dv = fmax(dv, fabs(out[5 * n + j]));
+        dt = fmax(dt, fabs(out[j]));
+        if (!isfinite(out[j]) || !isfinite(out[4 * n + j])) bad = true;
+    }
+    e = wave_max(e);
+    dv = wave_max(dv);
+    dt = wave_max(dt);
+    const bool anybad = __any(bad);
+    if (ln == 0 && it <= itmax) {
+        double *h = hist + ((size_t)m * itmax + (it - 1)) * 3;
+        h[0] = e;
+        h[1] = dv;
+        h[2] = dt;
+    }
+    if (anybad) {
+        if (ln == 0) conv[m] = -it;
+        return;
+    }
+    if (e <= 1.0 && dv <= 10.0 && dt <= 1.0) {
+        if (ln == 0) conv[m] = it;
+        return;
+    }
+    for (int j = ln; j < n; j += 64) T[j] = fmax(T[j] + out[j], 1500.0);
+}
+
+// Advances the iteration number and counts the models still iterating.
+__global__ __launch_bounds__(256) void k_md_tick(int nmodel, const int *__restrict__ conv,
+                                                 int *__restrict__ iter_dev,
+                                                 int *__restrict__ left)
+{
+    __shared__ int red[256];
+    int c = 0;
+    for (int m = threadIdx.x; m < nmodel; m += 256) c += conv[m] == 0 ? 1 : 0;
+    red[threadIdx.x] = c;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *left = red[0];
+        *iter_dev = *iter_dev + 1;
+    }
+}
+
+// Sets the iteration counter and clears the convergence flags (fill kernel:
+// no memset node in a captured sequence).
+__global__ void k_md_init(int nmodel, int *__restrict__ conv, int *__restrict__ iter_dev)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < nmodel) conv[m] = 0;
+    if (m == 0) *iter_dev = 1;
 }
 
 }  // namespace
@@ -1325,5 +1756,201 @@ int a12i_atlas_tcorr3(a12_ctx *ctx, const a12_tcorr_in *in, double *rdabh, doubl
     A12_HIP(hipMemcpyAsync(oldt1, dold, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     if (rdabh) A12_HIP(hipMemcpyAsync(rdabh, drd, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     A12_HIP(hipStreamSynchronize(s));
+    return A12_OK;
+}
+
+// ---- model iteration (host) ------------------------------------------------
+namespace {
+// The frequency table of the model iteration (kMdFt doubles per frequency):
+// freq, RCOSET, h nu / k, H- bound-free cross-section, the three H- free-free
+// coefficients, H Rayleigh cross-section, 2.815e29 / nu^3, the lowest H I
+// level ionised at nu (9: none), 3.69e8 / nu^3, the donor's cross-section,
+// (nu / 1e15)^3.
+std::vector<double> md_freq_table()
+{
+    const int nf = kMdNf;
+    std::vector<double> w(nf), f(nf), ft((size_t)nf * kMdFt, 0.0);
+    for (int k = 0; k < nf; ++k) {
+        const int nu = 1 + 25 * k;  // WAVESET(NU) with NUSTART 9599 (src/atlas12c.f90:470-472)
+        w[k] = std::pow(10.0, 1.0 + 0.0001 * (double)(nu + 9599 - 1));
+        f[k] = A12_CLIGHT_NMS / w[k];
+    }
+    const double hm[7] = {1.99654, -1.18267e-5, 2.64243e-6, -4.40524e-10, 3.23992e-14,
+                          -1.39568e-18, 2.78701e-23};
+    for (int k = 0; k < nf; ++k) {
+        double *q = &ft[(size_t)k * kMdFt];
+        const double fr = f[k], lam = w[k] * 10.0;  // Angstrom
+        q[0] = fr;
+        // RCOSET's trapezoid weights with zero flux beyond both ends (:475-490)
+        if (k == 0) q[1] = (f[0] - f[1]) * 1.5;
+        else if (k == nf - 1) q[1] = (f[nf - 2] + f[nf - 1]) * 0.25;
+        else q[1] = (f[k - 1] - f[k + 1]) * 0.5;
+        q[2] = A12_HPLANCK * fr / kMdKb;
+        const double lc = std::min(std::max(lam, 2250.0), 16000.0);
+        double a = 0.0, p = 1.0;
+        for (int i = 0; i < 7; ++i) {
+            a += hm[i] * p;
+            p *= lc;
+        }
+        q[3] = lam < 16419.0 ? std::max(a, 0.0) * 1.0e-18 : 0.0;
+        const double ll = std::log10(std::min(std::max(lam, 2600.0), 113900.0));
+        q[4] = -2.2763 - 1.6850 * ll + 0.76661 * ll * ll - 0.053346 * ll * ll * ll;
+        q[5] = 15.2827 - 9.2846 * ll + 1.99381 * ll * ll - 0.142631 * ll * ll * ll;
+        q[6] = -197.789 + 190.266 * ll - 67.9775 * ll * ll + 10.6913 * ll * ll * ll -
+               0.625151 * ll * ll * ll * ll;
+        const double lr = std::max(lam, 1300.0), l2 = lr * lr, l4 = l2 * l2;
+        q[7] = 5.799e-13 / l4 + 1.422e-6 / (l4 * l2) + 2.784 / (l4 * l4);
+        const double f3 = fr * fr * fr;
+        q[8] = 2.815e29 / f3;
+        int nmin = 9;
+        for (int lv = 8; lv >= 1; --lv)
+            if (fr >= 3.2898e15 / (double)(lv * lv)) nmin = lv;
+        q[9] = (double)nmin;
+        q[10] = 3.69e8 / f3;
+        q[11] = fr >= 1.839e15 ? 1.0e-18 * (1.839e15 / fr) * (1.839e15 / fr) : 0.0;
+        const double f15 = fr / 1.0e15;
+        q[12] = f15 * f15 * f15;
+    }
+    return ft;
+}
+}  // namespace
+
+int a12i_model_converge(a12_ctx *ctx, int nmodel, const a12_model_par *par,
+                        const double *t_start, int itmax, double *t_out, double *s_out,
+                        int32_t *iters, double *hist)
+{
+    if (nmodel < 1 || nmodel > 65535 || itmax < 1 || itmax > 1000 || !par || !t_start ||
+        !t_out || !s_out || !iters) {
+        ctx->err = "a12_model_converge: bad dimensions or null array";
+        return A12_ERR_ARG;
+    }
+    if (!ctx->have_ops) {
+        ctx->err = "a12_model_converge: call a12_set_operators first";
+        return A12_ERR_STATE;
+    }
+    const int n = kMdN;
+    for (int m = 0; m < nmodel; ++m) {
+        const a12_model_par &q = par[m];
+        bool ok = std::isfinite(q.teff) && q.teff > 1000.0 && q.teff < 1.0e5 &&
+                  std::isfinite(q.logg) && q.logg > -1.0 && q.logg < 9.0 && std::isfinite(q.mh) &&
+                  std::fabs(q.mh) < 5.0 && std::isfinite(q.vturb) && q.vturb >= 0.0 &&
+                  q.vturb < 50.0;
+        for (int j = 0; j < n && ok; ++j) ok = std::isfinite(t_start[(size_t)m * n + j]) &&
+                                            t_start[(size_t)m * n + j] > 0.0;
+        if (!ok) {
+            ctx->err = "a12_model_converge: model " + std::to_string(m) +
+                       " has a parameter or starting T out of range";
+            return A12_ERR_ARG;
+        }
+    }
+    hipStream_t st = ctx->stream;
+    ModelDev &M = ctx->mdl;
+    const int nchunk = (kMdNf + kMdChunk - 1) / kMdChunk;
+    const size_t nm = (size_t)nmodel;
+    A12_HIP(M.ft.reserve(sizeof(double) * kMdNf * kMdFt));
+    A12_HIP(M.taustd.reserve(sizeof(double) * n));
+    A12_HIP(M.par.reserve(sizeof(double) * 4 * nm));
+    A12_HIP(M.t.reserve(sizeof(double) * n * nm));
+    A12_HIP(M.ot.reserve(sizeof(double) * n * nm));
+    A12_HIP(M.rd.reserve(sizeof(double) * n * nm));
+    A12_HIP(M.dep.reserve(sizeof(double) * kMdDep * n * nm));
+    A12_HIP(M.part.reserve(sizeof(double) * 9 * n * nchunk * nm));
+    A12_HIP(M.acc.reserve(sizeof(double) * 9 * n * nm));
+    A12_HIP(M.hnu.reserve(sizeof(double) * kMdNf * nm));
+    A12_HIP(M.tin.reserve(sizeof(double) * 16 * n * nm));
+    A12_HIP(M.tout.reserve(sizeof(double) * 7 * n * nm));
+    A12_HIP(M.conv.reserve(sizeof(int) * nm));
+    A12_HIP(M.hist.reserve(sizeof(double) * 3 * (size_t)itmax * nm));
+    A12_HIP(M.ctl.reserve(sizeof(int) * 2));
+    const std::vector<double> ft = md_freq_table();
+    std::vector<double> tau(n), pv(4 * nm), zero(n * nm, 0.0);
+    for (int j = 0; j < n; ++j) tau[j] = std::pow(10.0, -6.875 + 0.125 * (double)j);
+    for (int m = 0; m < nmodel; ++m) {
+        pv[4 * m] = par[m].teff;
+        pv[4 * m + 1] = par[m].logg;
+        pv[4 * m + 2] = par[m].mh;
+        pv[4 * m + 3] = par[m].vturb;
+    }
+    A12_HIP(hipMemcpyAsync(M.ft.p, ft.data(), sizeof(double) * ft.size(), hipMemcpyHostToDevice, st));
+    A12_HIP(hipMemcpyAsync(M.taustd.p, tau.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+    A12_HIP(hipMemcpyAsync(M.par.p, pv.data(), sizeof(double) * pv.size(), hipMemcpyHostToDevice, st));
+    A12_HIP(hipMemcpyAsync(M.t.p, t_start, sizeof(double) * n * nm, hipMemcpyHostToDevice, st));
+    A12_HIP(hipMemcpyAsync(M.ot.p, zero.data(), sizeof(double) * n * nm, hipMemcpyHostToDevice, st));
+    int *conv = M.conv.as<int>(), *iter_dev = M.ctl.as<int>(), *left = iter_dev + 1;
+    hipLaunchKernelGGL(k_md_init, dim3((nmodel + 255) / 256), dim3(256), 0, st, nmodel, conv,
+                       iter_dev);
+    A12_CHECK_LAUNCH();
+    // one iteration, captured once and replayed
+    auto iteration = [&]() -> int {
+        hipLaunchKernelGGL(k_md_hydro, dim3(nmodel), dim3(64), 0, st, nmodel, M.par.as<double>(),
+                           M.ft.as<double>(), M.taustd.as<double>(), M.t.as<double>(), conv,
+                           M.dep.as<double>());
+        A12_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_md_rt, dim3(nchunk, nmodel), dim3(64), 0, st, nmodel, nchunk,
+                           M.par.as<double>(), M.ft.as<double>(), conv, M.dep.as<double>(),
+                           ctx->coefj.as<double>(), ctx->coefh.as<double>(), M.hnu.as<double>(),
+                           M.part.as<double>());
+        A12_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_md_reduce, dim3(9 * n, nmodel), dim3(256), 0, st, nmodel, nchunk,
+                           conv, M.part.as<double>(), M.acc.as<double>());
+        A12_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_md_update, dim3(nmodel), dim3(64), 0, st, nmodel, itmax, iter_dev,
+                           M.par.as<double>(), M.acc.as<double>(), M.dep.as<double>(),
+                           M.t.as<double>(), M.ot.as<double>(), M.rd.as<double>(),
+                           M.tin.as<double>(), M.tout.as<double>(), conv, M.hist.as<double>());
+        A12_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_md_tick, dim3(1), dim3(256), 0, st, nmodel, conv, iter_dev, left);
+        A12_CHECK_LAUNCH();
+        return A12_OK;
+    };
+    A12_HIP(hipStreamSynchronize(st));
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ex = nullptr;
+    A12_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeGlobal));
+    int rc = iteration();
+    const hipError_t ec = hipStreamEndCapture(st, &g);
+    if (rc == A12_OK && ec != hipSuccess) {
+        ctx->err = std::string("a12_model_converge: graph capture: ") + hipGetErrorString(ec);
+        rc = A12_ERR_HIP;
+    }
+    if (rc == A12_OK && hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) {
+        ctx->err = "a12_model_converge: graph instantiation failed";
+        rc = A12_ERR_HIP;
+    }
+    int nleft = nmodel;
+    for (int it = 1; it <= itmax && rc == A12_OK && nleft > 0; ++it) {
+        if (hipGraphLaunch(ex, st) != hipSuccess) {
+            ctx->err = "a12_model_converge: graph launch failed";
+            rc = A12_ERR_HIP;
+            break;
+        }
+        if (it % 4 == 0 || it == itmax) {  // every few iterations: any model left?
+            if (hipMemcpyAsync(&nleft, left, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess) {
+                ctx->err = "a12_model_converge: synchronisation failed";
+                rc = A12_ERR_HIP;
+            }
+        }
+    }
+    if (ex) (void)hipGraphExecDestroy(ex);
+    if (g) (void)hipGraphDestroy(g);
+    if (rc != A12_OK) return rc;
+    // results: T, the structure of the last hydrostatic pass and RT pass, flags
+    std::vector<double> dep(kMdDep * n * nm);
+    std::vector<int> cv(nm);
+    A12_HIP(hipMemcpyAsync(t_out, M.t.p, sizeof(double) * n * nm, hipMemcpyDeviceToHost, st));
+    A12_HIP(hipMemcpyAsync(dep.data(), M.dep.p, sizeof(double) * dep.size(), hipMemcpyDeviceToHost, st));
+    A12_HIP(hipMemcpyAsync(cv.data(), conv, sizeof(int) * nm, hipMemcpyDeviceToHost, st));
+    if (hist)
+        A12_HIP(hipMemcpyAsync(hist, M.hist.p, sizeof(double) * 3 * (size_t)itmax * nm,
+                               hipMemcpyDeviceToHost, st));
+    A12_HIP(hipStreamSynchronize(st));
+    const int rows[5] = {D_PG, D_PTOT, D_RHOX, D_ABR, D_NE};
+    for (int m = 0; m < nmodel; ++m) {
+        iters[m] = cv[m];
+        for (int q = 0; q < 5; ++q)
+            std::memcpy(s_out + ((size_t)m * 5 + q) * n, &dep[((size_t)m * kMdDep + rows[q]) * n],
+                        sizeof(double) * n);
+    }
     return A12_OK;
 }

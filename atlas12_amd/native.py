@@ -40,7 +40,7 @@ EXPORTS = ["a12_ctx_create", "a12_ctx_destroy", "a12_last_error", "a12_stream",
            "a12_synthe_shape_info", "a12_synthe_set_shape", "a12_synthe_batch_load",
            "a12_synthe_batch_set_budget", "a12_synthe_batch_run", "a12_synthe_batch_fetch",
            "a12_synthe_batch_info", "a12_solvit_batched", "a12_atlas_load_lines",
-           "a12_atlas_set_state", "a12_atlas_rt_pass_noalloc"]
+           "a12_atlas_set_state", "a12_atlas_rt_pass_noalloc", "a12_model_converge"]
 
 
 class A12Error(RuntimeError):
@@ -725,3 +725,41 @@ def atlas_rt_pass_noalloc(ctx: Context, n: int, nf: int, teff: float, d_waveset:
         ctx.h, C.c_int32(n), C.c_int32(nf), C.c_double(teff), vp(d_waveset), vp(d_rcoset),
         vp(d_depth3), vp(d_kapp), vp(d_xlines), vp(d_hnu1), vp(d_acc)),
         "a12_atlas_rt_pass_noalloc")
+
+
+class ModelPar(C.Structure):
+    _fields_ = [("teff", C.c_double), ("logg", C.c_double), ("mh", C.c_double),
+                ("vturb", C.c_double)]
+
+
+MODEL_NRHOX = 72
+MODEL_NFREQ = 1200
+MODEL_TAUSTD = 10.0 ** (-6.875 + 0.125 * np.arange(MODEL_NRHOX))
+
+
+def model_converge(ctx: Context, par, t_start, itmax: int = 60, history: bool = False):
+    """a12_model_converge: the ATLAS12 iteration (reduced-physics KAPP and
+    EOS, device RT pass and TCORR(3)) for many models.  par (nmodel, 4):
+    Teff, log g, [M/H], vturb [km/s]; t_start (nmodel, 72) on MODEL_TAUSTD.
+    Returns (iters (nmodel,), T (nmodel, 72), structure dict of (nmodel, 72)
+    arrays pg, ptot, rhox, abross, xne[, hist (nmodel, itmax, 3)])."""
+    pa = np.ascontiguousarray(par, np.float64)
+    if pa.ndim != 2 or pa.shape[1] != 4:
+        raise ValueError("par must have shape (nmodel, 4)")
+    nm = len(pa)
+    ts = np.ascontiguousarray(t_start, np.float64)
+    if ts.shape != (nm, MODEL_NRHOX):
+        raise ValueError(f"t_start must have shape ({nm}, {MODEL_NRHOX})")
+    ps = (ModelPar * nm)(*[ModelPar(*row) for row in pa])
+    t_out = np.zeros((nm, MODEL_NRHOX))
+    s_out = np.zeros((nm, 5, MODEL_NRHOX))
+    iters = np.zeros(nm, np.int32)
+    hist = np.zeros((nm, itmax, 3)) if history else None
+    ctx._check(ctx.lib.a12_model_converge(
+        ctx.h, C.c_int32(nm), ps, _ptr(ts), C.c_int32(itmax), _ptr(t_out), _ptr(s_out),
+        iters.ctypes.data_as(C.c_void_p), None if hist is None else _ptr(hist)),
+        "a12_model_converge")
+    st = {k: s_out[:, i] for i, k in enumerate(("pg", "ptot", "rhox", "abross", "xne"))}
+    if history:
+        return iters, t_out, st, hist
+    return iters, t_out, st

@@ -380,6 +380,45 @@ int a12_scale_model(a12_ctx *ctx, int32_t nmodel, int32_t nrhox, int32_t kw, dou
                     double steplg, const double *sc, const double *fields, double *out,
                     double *so, double *taustd);
 
+/* ---- model iteration (reduced physics) ------------------------------------ */
+/* The ATLAS12 iteration loop (src/atlas12c.f90:496-606) for nmodel models in
+ * one launch sequence per iteration (captured once in a hipGraph and
+ * replayed), on the fixed Rosseland grid TAUSTD(j) = 10**(-6.875 +
+ * 0.125 (j-1)), j = 1..A12_MODEL_NRHOX, with KAPP and the EOS replaced by
+ * closed forms (H, He and one metal donor in Saha/charge balance; H-
+ * bound-free and free-free, H I bound-free (n <= 8) and free-free, the
+ * donor's bound-free, electron and H Rayleigh scattering; no lines) and the
+ * structure put in hydrostatic equilibrium on the grid each iteration.  On
+ * the device as in the ATLAS12 path above: the RUN_RT_PASS loop (JOSH(1,0)
+ * with the TCORR/RADIAP/ROSS sums) on A12_MODEL_NFREQ frequencies (the
+ * driver's 1e-4 dex grid from NUSTART 9599 at every 25th point, RCOSET's
+ * rule), ROSS(3) and TCORR(3) without convection, then T = max(T + T1,
+ * 1500 K).  No counterpart in the reference as a whole: KAPP and POPS are
+ * outside this library; this is the loop that converges the models the
+ * starting-structure network is trained on and compared with.
+ *   par     (nmodel): Teff [K], log g, [M/H], vturb [km/s];
+ *   t_start (nmodel, A12_MODEL_NRHOX): starting T on the grid;
+ *   itmax:  iterations at most (1..1000);
+ *   t_out   (nmodel, A12_MODEL_NRHOX): T at the end (the converged
+ *           structure, or the last one);
+ *   s_out   (nmodel, 5, A12_MODEL_NRHOX): P_gas, P_total, RHOX, n_e of the
+ *           last hydrostatic pass and ABROSS (ROSS(3)) of the last RT pass;
+ *   iters   (nmodel): the first iteration whose TCORR(3) table has
+ *           max |ERROR| <= 1 %, max |DERIV| <= 10 % and max |T1| <= 1 K
+ *           (the model is kept as it was at that iteration's start), 0 if
+ *           none within itmax, -k if iteration k produced a non-finite T1;
+ *   hist    (nmodel, itmax, 3), may be NULL: max |ERROR| %, max |DERIV| %,
+ *           max |T1| K of every iteration run (untouched after convergence).
+ * Host pointers, synchronous. */
+#define A12_MODEL_NRHOX 72
+#define A12_MODEL_NFREQ 1200
+typedef struct {
+    double teff, logg, mh, vturb;
+} a12_model_par;
+int a12_model_converge(a12_ctx *ctx, int32_t nmodel, const a12_model_par *par,
+                       const double *t_start, int32_t itmax, double *t_out, double *s_out,
+                       int32_t *iters, double *hist);
+
 /* ---- starting-structure network (no counterpart in the reference) ------- */
 /* A 4 -> 256 -> 256 -> 256 -> 256 -> 144 MLP (SiLU hidden layers, 235,664
  * parameters) that maps (Teff, log g, [M/H], vturb) -- standardised by the
