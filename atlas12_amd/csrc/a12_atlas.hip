@@ -855,114 +855,9 @@ __global__ __launch_bounds__(kLoBlock) void k_lo_gather_w(
         }
 }
 
-// Point-lane form of the gather: a wavefront owns 64 consecutive grid points
-// (one per lane) and G consecutive depths, with its sums in registers.  The
-// candidate lines (NU within 100 points of the window) are taken 64 at a
-// time: each lane tests one candidate's G per-depth states against the
-// window, and the wave ballots those with any reaching depth.  It then takes
-// them in store order (lowest bit first) and, for each reaching depth (a
-// wave-uniform branch on the broadcast depth mask, the line's state and
-// parameters read as wave-uniform loads), every lane adds its own point's
-// contribution.  A (line, depth)'s parameters are thus read once per 64
-// points instead of once per lane tile, and a line's items are spread over
-// the lanes of its points.  Per point and depth the lines are added in store
-// order with the same per-item value and summation as the other forms: the
-// same sums and bits.  ngrp = ceil(n / G) depth groups; one wavefront per
-// (window, group).
-template <bool F64, int G>
-__global__ __launch_bounds__(64) void k_lo_gather_p(
-    int n, int numnu, int ngrp, const int *__restrict__ seg, const int *__restrict__ l_nu,
-    const int *__restrict__ l_reach, const int *__restrict__ state, const double *__restrict__ par,
-    const double *__restrict__ l_const, const double *__restrict__ waveset,
-    const double *__restrict__ h0, const double *__restrict__ h1, const double *__restrict__ h2,
-    const float4 *__restrict__ f, double *__restrict__ xlines)
-{
-#pragma clang fp contract(off)
-    static_assert(G >= 1 && G <= 32, "depth group");
-    const int win = blockIdx.x / ngrp, grp = blockIdx.x - win * ngrp;
-    const int lane = threadIdx.x & 63;
-    const int p0 = win * 64, p = p0 + lane;
-    const bool live = p < numnu;
-    const int j0 = grp * G, gd = min(G, n - j0);
-    if (gd <= 0 || p0 >= numnu) return;  // uniform
-    const double wp = waveset[min(p, numnu - 1)];
-    const int wfirst = p0, wlast = min(p0 + 63, numnu - 1);
-    double accd[G];
-    float accs[G], accc[G];
-#pragma unroll
-    for (int d = 0; d < G; ++d) {
-        accd[d] = 0.0;
-        accs[d] = 0.0f;
-        accc[d] = 0.0f;
-    }
-    const int lbeg = seg[max(0, wfirst - kMaxWing)];
-    const int lend = seg[min(numnu, wlast + kMaxWing) + 1];
-    for (int c0 = lbeg; c0 < lend; c0 += 64) {
-        const int q = c0 + lane;
-        unsigned dmask = 0;  // depths of candidate q whose reach meets the window
-        int kq = 0;
-        if (q < lend) {
-            kq = l_nu[q];
-            const int rr = l_reach[q], rmax = rr & 0xff, bmax = (rr >> 8) & 0xff;
-            const bool any = (rmax > 0 && kq <= wlast && kq + rmax - 1 >= wfirst) ||
-                             (bmax > 0 && kq - 1 >= wfirst && kq - bmax <= wlast);
-            if (any) {
-                const int *sq = state + (size_t)q * n + j0;
-#pragma unroll
-                for (int d = 0; d < G; ++d) {
-                    if (d < gd) {
-                        const int st = sq[d];
-                        const int rn = st & 0xff, bn = (st >> 8) & 0xff;
-                        if (st < 0 && kq + rn - 1 >= wfirst && kq - bn <= wlast && rn + bn > 0)
-                            dmask |= 1u << d;
-                    }
-                }
-            }
-        }
-        unsigned long long m = __ballot(dmask != 0);
-        while (m) {
-            const int src = __builtin_ctzll(m);
-            m &= m - 1;
-            const int row = c0 + src;
-            const unsigned dm = (unsigned)__builtin_amdgcn_readlane((int)dmask, src);
-            const int k = __builtin_amdgcn_readlane(kq, src);
-            const double wl = l_const[5 * (size_t)row];
-            const double d_red = wp - wl, d_blue = wl - wp;
-#pragma unroll
-            for (int d = 0; d < G; ++d) {
-                if (!(dm & (1u << d))) continue;
-                const size_t o = (size_t)row * n + j0 + d;
-                const int st = state[o];
-                const int rn = st & 0xff, bn = (st >> 8) & 0xff;
-                const LoPar qp = {par[3 * o], par[3 * o + 1], par[3 * o + 2]};
-                const bool red = p >= k && p <= k + rn - 1;
-                const bool blue = p < k && p >= k - bn;
-                if (live && (red || blue)) {
-                    const double dd = red ? d_red : d_blue;
-                    if (F64) {
-                        accd[d] = accd[d] + lo_cv<double>(qp.p0, dd / qp.p2, qp.p1, h0, h1, h2);
-                    } else {
-                        const float x = lo_item_mixed(qp, dd, f);
-                        const float s0 = accs[d], y = x - accc[d], tt = s0 + y;
-                        accc[d] = (tt - s0) - y;
-                        accs[d] = tt;
-                    }
-                }
-            }
-        }
-    }
-    if (live) {
-#pragma unroll
-        for (int d = 0; d < G; ++d)
-            if (d < gd)
-                xlines[(size_t)p * n + j0 + d] = F64 ? accd[d] : (double)accs[d] - (double)accc[d];
-    }
-}
-
 // Tile shapes (points per tile, lanes per block, lines per LDS batch, form:
-// 0 staged, 1 flat, 2 barrier-free per wave; 3 point lanes, with 64-point
-// windows and `batch` depths per wavefront) the autotuner chooses from;
-// LDS = 8 B x points x lanes (+ 4 B x 128 x batch when staged; none for 3).
+// 0 staged, 1 flat, 2 barrier-free per wave) the autotuner chooses from;
+// LDS = 8 B x points x lanes (+ 4 B x 128 x batch when staged).
 struct LoShape {
     int pts, block, batch;
     int flat;
@@ -976,7 +871,6 @@ struct LoShape {
 #define A12_LO_SHAPE(P, B, T, F) \
     {P, B, T, F ? 1 : 0, k_lo_gather<false, P, B, T, F>, k_lo_gather<true, P, B, T, F>}
 #define A12_LO_WAVE(P, B) {P, B, 0, 2, k_lo_gather_w<false, P, B>, k_lo_gather_w<true, P, B>}
-#define A12_LO_POINT(G) {64, 64, G, 3, k_lo_gather_p<false, G>, k_lo_gather_p<true, G>}
 const LoShape kLoShapes[] = {
     A12_LO_SHAPE(8, 512, 32, false),  A12_LO_SHAPE(4, 512, 64, false),
     A12_LO_SHAPE(4, 256, 32, false),  A12_LO_SHAPE(8, 256, 32, false),
@@ -985,12 +879,9 @@ const LoShape kLoShapes[] = {
     A12_LO_SHAPE(2, 256, 16, false),  A12_LO_SHAPE(8, 256, 16, false),
     A12_LO_WAVE(4, 256),              A12_LO_WAVE(2, 256),
     A12_LO_WAVE(8, 256),              A12_LO_WAVE(4, 512),
-    A12_LO_POINT(4),                  A12_LO_POINT(8),
-    A12_LO_POINT(12),                 A12_LO_POINT(18),
 };
 #undef A12_LO_SHAPE
 #undef A12_LO_WAVE
-#undef A12_LO_POINT
 constexpr int kNumLoShapes = sizeof(kLoShapes) / sizeof(kLoShapes[0]);
 
 template <class T>
@@ -1230,16 +1121,6 @@ static int lo_gather_launch(a12_ctx *ctx, int prec, int shape)
     const float4 *tabs32 = A.tabs32.as<float4>();
     const LoShape &sh = kLoShapes[shape];
     auto g = prec == A12_PREC_FP64 ? sh.fp64 : sh.mixed;
-    if (sh.flat == 3) {  // point lanes: one wavefront per (64-point window, depth group)
-        const int nwin = (numnu + 63) / 64, ngrp = (n + sh.batch - 1) / sh.batch;
-        hipLaunchKernelGGL(g, dim3(nwin * ngrp), dim3(64), 0, ctx->stream, n, numnu, ngrp,
-                           A.seg.as<int>(), A.l_nu.as<int>(), A.reach.as<int>(),
-                           A.state.as<int>(), A.par.as<double>(), A.l_const.as<double>(),
-                           A.waveset.as<double>(), tabs, tabs + kTabN, tabs + 2 * kTabN, tabs32,
-                           A.xlines.as<double>());
-        A12_CHECK_LAUNCH();
-        return A12_OK;
-    }
     const int tiles = (numnu + sh.pts - 1) / sh.pts, per = sh.block / n;
     if (per < 1) {
         ctx->err = "LINOP1 gather: block smaller than the layer count";
@@ -1343,7 +1224,7 @@ int a12i_atlas_autotune(a12_ctx *ctx, int prec, float *ms, int32_t *best)
     int rc = A12_OK;
     for (int k = 0; k < kNumLoShapes && rc == A12_OK; ++k) {
         float t = -1.0f;
-        if (kLoShapes[k].flat == 3 || A.n <= kLoShapes[k].block) {
+        if (A.n <= kLoShapes[k].block) {
             if ((rc = lo_gather_launch(ctx, prec, k)) != A12_OK) break;  // warm
             if (hipEventRecord(e0, ctx->stream) != hipSuccess) rc = A12_ERR_HIP;
             if (rc == A12_OK) rc = lo_gather_launch(ctx, prec, k);

@@ -235,9 +235,7 @@ int a12_atlas_linop1_run(a12_ctx *ctx, int32_t prec);
  * and, for 0 <= idx < that number, writes {points per tile, lanes per
  * block, lines per LDS batch, form} (form 0: staged; 1: a line's items
  * dealt out over the wave instead of walked per lane; 2: per wavefront
- * without barriers; 3: one lane per grid point of a 64-point window, with
- * `lines per LDS batch` read as the depths per wavefront).
- * a12_atlas_set_shape picks one (default 0);
+ * without barriers).  a12_atlas_set_shape picks one (default 0);
  * a12_atlas_autotune runs LINOP1 once, times the gather in every shape on
  * the loaded store (ms[shape], -1 where not applicable) and keeps the
  * fastest.  XLINES is bitwise the same in every shape. */
