@@ -68,6 +68,11 @@ MODULE a12gpu_iface
     TYPE(C_PTR)        :: flxrad, rjmins, rdiagj
   END TYPE a12_tcorr_in
 
+  ! One model of the model iteration (a12_model_converge)
+  TYPE, BIND(C) :: a12_model_par
+    REAL(C_DOUBLE)     :: teff, logg, mh, vturb
+  END TYPE a12_model_par
+
   ! READMOL's molecule table for NMOLEC's Newton solve (1-based arrays)
   TYPE, BIND(C) :: a12_moltab
     INTEGER(C_INT32_T) :: nequa, nummol
@@ -191,6 +196,19 @@ MODULE a12gpu_iface
       INTEGER(C_INT32_T), INTENT(IN) :: linedata(*)
       INTEGER(C_INT32_T), VALUE :: nlines
     END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_atlas_load_lines(ctx, atm, linedata, nlines) &
+        BIND(C, NAME='a12_atlas_load_lines')
+      IMPORT :: C_INT32_T, C_PTR, a12_atlas_atm
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(a12_atlas_atm), INTENT(IN) :: atm
+      INTEGER(C_INT32_T), INTENT(IN) :: linedata(*)
+      INTEGER(C_INT32_T), VALUE :: nlines
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_atlas_set_state(ctx, atm) BIND(C, NAME='a12_atlas_set_state')
+      IMPORT :: C_INT32_T, C_PTR, a12_atlas_atm
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(a12_atlas_atm), INTENT(IN) :: atm
+    END FUNCTION
     INTEGER(C_INT32_T) FUNCTION a12_atlas_linop1_run(ctx, prec) &
         BIND(C, NAME='a12_atlas_linop1_run')
       IMPORT :: C_INT32_T, C_PTR
@@ -219,6 +237,30 @@ MODULE a12gpu_iface
       IMPORT :: C_INT32_T, C_PTR, C_DOUBLE
       TYPE(C_PTR), VALUE :: ctx
       REAL(C_DOUBLE), INTENT(OUT) :: acc(*), hnu1(*)
+    END FUNCTION
+
+    ! device arrays the caller owns (C_PTR from its own hipMalloc)
+    INTEGER(C_INT32_T) FUNCTION a12_atlas_rt_pass_noalloc(ctx, nrhox, numnu, teff, d_waveset, &
+        d_rcoset, d_depth3, d_kapp, d_xlines, d_hnu1, d_acc) &
+        BIND(C, NAME='a12_atlas_rt_pass_noalloc')
+      IMPORT :: C_INT32_T, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT32_T), VALUE :: nrhox, numnu
+      REAL(C_DOUBLE), VALUE :: teff
+      TYPE(C_PTR), VALUE :: d_waveset, d_rcoset, d_depth3, d_kapp, d_xlines, d_hnu1, d_acc
+    END FUNCTION
+
+    ! ---- model iteration ---------------------------------------------------
+    INTEGER(C_INT32_T) FUNCTION a12_model_converge(ctx, nmodel, par, t_start, itmax, t_out, &
+        s_out, iters, hist) BIND(C, NAME='a12_model_converge')
+      IMPORT :: C_INT32_T, C_PTR, C_DOUBLE, a12_model_par
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT32_T), VALUE :: nmodel, itmax
+      TYPE(a12_model_par), INTENT(IN) :: par(*)
+      REAL(C_DOUBLE), INTENT(IN) :: t_start(*)
+      REAL(C_DOUBLE), INTENT(OUT) :: t_out(*), s_out(*)
+      INTEGER(C_INT32_T), INTENT(OUT) :: iters(*)
+      TYPE(C_PTR), VALUE :: hist             ! C_NULL_PTR or C_LOC of (3, itmax, nmodel)
     END FUNCTION
 
     ! ---- TCORR(3), SCALE_MODEL, NMOLEC -----------------------------------

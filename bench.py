@@ -66,8 +66,12 @@ def parse():
     ap.add_argument("--large-lines", type=int, default=50_000_000)
     ap.add_argument("--profile-only", action="store_true",
                     help="warmup+steps iterations of the ATLAS12 and plain-SYNTHE workloads at "
-                         "both precisions, 25 network training steps, a 4-model batch, and a "
-                         "device copy of --calib-mib MiB (for rocprofv3 runs); no JSON")
+                         "both precisions, 25 network training steps, the large-SYNTHE workload "
+                         "on --profile-large-lines lines, 64 models x 20 iterations of the "
+                         "model iteration, a 4-model batch, and a device copy of --calib-mib "
+                         "MiB (for rocprofv3 runs); no JSON")
+    ap.add_argument("--profile-large-lines", type=int, default=5_000_000,
+                    help="lines of the large-SYNTHE workload in --profile-only runs")
     ap.add_argument("--calib-mib", type=int, default=256,
                     help="size of the plain device-to-device copy used to scale HBM counters")
     return ap.parse_args()
@@ -521,6 +525,28 @@ def starting_iterations(ctx):
             "note": "non-converged models count as itmax"}
 
 
+def profile_extra(args, ctx, native, wl):
+    """For profiler runs only: the large SYNTHE workload on a reduced list
+    (--profile-large-lines lines, same generator, grid and structure; mixed
+    and FP64 opacity and transfer in the tuned shape) and 64 models of the
+    model iteration (grey starts, 20 iterations)."""
+    from atlas12_amd import mlp
+    from atlas12_amd.state import SyntheState
+    base = SyntheState.load(os.path.join(GOLDEN, "synthe_plain_state.npz"))
+    grid = wl.Grid.make(300.0, 1000.0, 500000.0)
+    st = wl.cool_dwarf_state(base, grid)
+    lines = wl.large_synthe_lines(grid, args.profile_large_lines)
+    ctx.synthe_load(st, lines)
+    for prec in (native.PREC_MIXED, native.PREC_FP64):
+        ctx.synthe_opacity_run(prec)
+        ctx.synthe_transfer_run()
+        ctx.synthe_transfer_fetch()
+    log(f"profile: large SYNTHE workload on {len(lines):,} lines")
+    x = mlp.draw_inputs(64, 10)
+    native.model_converge(ctx, x, np.stack([mlp.grey_t(te) for te in x[:, 0]]), 20)
+    log("profile: 64 models x 20 iterations of the model iteration")
+
+
 # ------------------------------------------------------------ CPU legs
 def host_cpu():
     """The host the CPU legs run on: logical CPUs of the machine, those this
@@ -624,6 +650,8 @@ def main():
         big = large_run(args, ctx, native, wl, syn[0]["_shape"])
     net = mlp_run(args, ctx, native)
     mol = None if args.profile_only else molec_run(args, ctx, native)
+    if args.profile_only:
+        profile_extra(args, ctx, native, wl)
     if args.profile_only and not args.no_batch:
         args.batch_models, args.batch_lines = 4, 1_000_000
     bat = None if args.no_batch else batch_run(args, ctx, native, wl)
