@@ -474,6 +474,36 @@ def golden_tcorr():
                       "rows": int(len(tab["j"]))}}
 
 
+def golden_tcorr_conv():
+    """TCORR(3) on a deck whose deep layers convect: the plain-SYNTHE RHOX
+    grid with T rising from 4,000 to 16,000 K, after a full 30,000-frequency
+    pass (20,000 raw lines through SELECTLINES).  The reference's CONVEC gives
+    convective flux at 20 layers (CONV/TOT up to 0.59), so the convective
+    branches of sections A-F are exercised: the DDEL factor, the convective
+    terms of the gradient, the DTLAMB halving above convective layers and no
+    damping where CONV/TOT > 0."""
+    from oracle import oracle as orc
+    atm = wl.scurve_atmosphere(t_top=4000.0, t_bot=16000.0)
+    st = np.load(os.path.join(HERE, "atlas_plain_state.npz"))
+    mols = tuple(int(m) for m in st["mol_nelion"])
+    pred, tio = wl.atlas_iteration_lists(20_000, 95.0, 90_000.0, seed=41, molecules=mols)
+    d = atlas_case(atm, 9599, 30000, None, raw=(pred, tio), tcorr=True)
+    orc.build()
+    acc, _ = orc.rt_pass(d, d["xlines"], d["coefj"], d["coefh"])
+    for k in ("rjmins", "flxrad", "raden", "accrad", "abross"):
+        assert np.array_equal(acc[k], d["acc_" + k]), k
+    ti, tab = d["tcorr_in"], d["tcorr_iter"]
+    assert ti["ifconv"] == 1 and np.sum(ti["flxcnv"] > 0) > 5, "no convective layers"
+    keep = {k: ti[k] for k in TCORR_FIELDS}
+    keep.update({"iter_" + k: v for k, v in tab.items()})
+    np.savez_compressed(os.path.join(HERE, "tcorr_conv_ref.npz"), n=ti["n"],
+                        ifconv=ti["ifconv"], iter=ti["iter"], flux=ti["flux"], teff=ti["teff"],
+                        mixlth=ti["mixlth"], rdabh=acc["rdabh"], rdiagj=acc["rdiagj"], **keep)
+    return {"tcorr_conv": {"selected_lines": int(d["nlines"]),
+                           "convective_layers": int(np.sum(ti["flxcnv"] > 0)),
+                           "max_conv_tot": float(tab["hratio"].max())}}
+
+
 def golden_atlas_small():
     """One ATLAS12 opacity-sampling pass on the plain-SYNTHE atmosphere:
     240 frequencies from 500 nm (1e-4 dex steps), 3000 lines with the
@@ -673,7 +703,8 @@ def golden_molec():
 
 STEPS = {"synthe_small": golden_synthe_small, "atlas_small": golden_atlas_small,
          "state_atlas": golden_state_atlas, "select": golden_select, "scale": golden_scale,
-         "tcorr": golden_tcorr, "state_plain": golden_state_plain,
+         "tcorr": golden_tcorr, "tcorr_conv": golden_tcorr_conv,
+         "state_plain": golden_state_plain,
          "numerics": golden_numerics, "josh": golden_josh, "molec": golden_molec}
 
 

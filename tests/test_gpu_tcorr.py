@@ -85,3 +85,21 @@ def test_tcorr3_rejects_bad_input(gpu_ctx, tz):
     bad["n"] = 200
     with pytest.raises(ValueError):
         native.atlas_tcorr3(gpu_ctx, bad, tz["rdabh"])
+
+
+def test_tcorr3_convective_matches_oracle_and_table(gpu_ctx, golden, orc):
+    """The convective case (CONVEC's flux at 20 layers): the device equals
+    the oracle (bit for bit except where exp enters, 1e-14) and the
+    reference's printed table at every printed digit."""
+    from atlas12_amd import native
+    z = golden("tcorr_conv_ref.npz")
+    out, rd, ot = native.atlas_tcorr3(gpu_ctx, z, z["rdabh"])
+    ref, rrd, _ = orc.tcorr3(z, z["rdabh"])
+    assert np.array_equal(rd, rrd)
+    for k in NAMES:
+        assert _close(out[k], ref[k]), k
+    for k in ("dtlamb", "dtsurf", "dtflux", "t1"):
+        assert [f"{v:.1f}" for v in out[k]] == [f"{v:.1f}" for v in z["iter_" + k]], k
+    for k in ("flxerr", "flxdrv", "hratio"):
+        assert [f"{v:.2E}" for v in out[k]] == [f"{v:.2E}" for v in z["iter_" + k]], k
+    assert np.max(out["hratio"]) > 0.5

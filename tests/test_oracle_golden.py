@@ -180,6 +180,23 @@ def test_tcorr3_matches_reference_table(orc, golden):
     assert np.array_equal(_printed(out["hratio"], "%.3f"), z["iter_hratio"])
 
 
+def test_tcorr3_convective_matches_reference_table(orc, golden):
+    """TCORR(3) on a deck whose deep layers convect (CONVEC's flux at 20
+    layers, CONV/TOT up to 0.59): the convective branches -- the DDEL
+    factor, the convective gradient terms, the DTLAMB halving of the five
+    layers above a convective one, no damping where CONV/TOT > 0 -- give the
+    reference's printed DTLAMB, DTSURF, DTFLUX, T1 (F8.1), ERROR, DERIV and
+    CONV/TOT (ES11.2) at every printed digit."""
+    z = golden("tcorr_conv_ref.npz")
+    assert int(z["ifconv"]) == 1 and np.sum(z["flxcnv"] > 0) >= 10
+    assert np.max(z["iter_hratio"]) > 0.5
+    out, _, _ = orc.tcorr3(z, z["rdabh"])
+    for k in ("dtlamb", "dtsurf", "dtflux", "t1"):
+        assert np.array_equal(_printed(out[k], "%.1f") + 0.0, z["iter_" + k] + 0.0), k
+    for k in ("flxerr", "flxdrv", "hratio"):
+        assert np.array_equal(_printed(out[k], "%.2E"), z["iter_" + k]), k
+
+
 def test_tcorr3_damping_rules(orc, golden):
     """Iteration 2 against a planted previous correction: same sign and
     shrinking -> x1.25, sign flip -> x0.5, same sign growing -> unchanged
