@@ -1020,6 +1020,7 @@ int a12i_atlas_load_lines(a12_ctx *ctx, const a12_atlas_atm *a, const int32_t *l
     A12_HIP(up(A.seg, seg.data(), seg.size(), s));
     A12_HIP(hipStreamSynchronize(s));
     A.lines_ok = true;
+    ++A.grid_gen;
     return A12_OK;
 }
 
@@ -1057,6 +1058,7 @@ int a12i_atlas_set_state(a12_ctx *ctx, const a12_atlas_atm *a)
     const int n = a->nrhox;
     A.n = n;
     A.teff = a->teff;
+    A.t_h.assign(a->t, a->t + n);
     // depth vectors: t hkt rhox | xne hckt txnxn (TXNXN as LINOP1 forms it, :18164)
     std::vector<double> dep((size_t)6 * n);
     for (int j = 0; j < n; ++j) {

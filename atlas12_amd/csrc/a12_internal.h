@@ -72,10 +72,12 @@ struct AtlasDev {
     // host staging of set_state
     std::vector<double> waveset_h, stage_h;
     std::vector<int32_t> iwavetab_h;
+    std::vector<double> t_h;    // T of the state set on it (a12_atlas_kapp_run checks its state)
+    uint64_t grid_gen = 0;      // counts line-store loads (a new grid)
     size_t kapp_plane = 0;  // nrhox * numnu of the KAPP tables on the device
     // lines_ok: a store is loaded; loaded: a state is set on it
     bool lines_ok = false, loaded = false, xl_ready = false, kapp_ready = false, rt_done = false;
-    bool prof_lo = false, prof_rt = false;
+    bool prof_lo = false, prof_rt = false, prof_kapp = false;
     int shape = 0;  // LINOP1 gather tile shape (a12_atlas_set_shape / a12_atlas_autotune)
 };
 
@@ -97,6 +99,7 @@ struct ModelDev {
 };
 
 struct MlpDev;    // a12_mlp.hip
+struct KappDev;   // a12_kapp.hip
 struct SynBatch;  // a12_synthe.hip
 
 struct a12_ctx {
@@ -111,13 +114,14 @@ struct a12_ctx {
     MolDev mol;
     ModelDev mdl;
     MlpDev *mlp = nullptr;
+    KappDev *kapp = nullptr;
     SynBatch *sbatch = nullptr;
     DevBuf rt_part;
     // scratch for a12_josh_batch and a12_solvit_batched
     DevBuf jb_in, jb_out, sv_a, sv_b;
     // stage timing (a12_set_profiling): events on the context stream
     bool prof = false;
-    hipEvent_t ev[16] = {};  // 0-6 SYNTHE stages, 8-11 LINOP1 stages, 12-13 RT pass
+    hipEvent_t ev[16] = {};  // 0-6 SYNTHE stages, 8-11 LINOP1 stages, 12-13 RT pass, 14-15 KAPP
 };
 
 #define A12_HIP(call)                                                              \
@@ -182,6 +186,7 @@ int a12i_mlp_infer(a12_ctx *ctx, const float *x, int n, float *y);
 int a12i_mlp_gemm_check(a12_ctx *ctx, int m, int n, int k, const uint16_t *a, const uint16_t *b,
                         float *c, int split);
 void a12i_mlp_free(a12_ctx *ctx);
+void a12i_kapp_free(a12_ctx *ctx);
 // Implemented in a12_josh.hip
 int a12i_atlas_rt_pass(a12_ctx *ctx, int n, int nf, double teff, const double *d_wave,
                        const double *d_rcoset, const double *d_depth3, const double *d_kapp,

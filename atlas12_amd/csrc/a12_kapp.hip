@@ -1,0 +1,311 @@
+// a12_kapp.hip -- KAPP, the continuum opacity (src/atlas12_modules.f90:11202-11324
+// and every source it calls), on gfx950.
+//
+// Work split (a12_kapp_core.h):
+//   k_kapp_freq  one thread per frequency: the frequency-only row (Karzas &
+//                Latter cross sections of 80 H levels, He I levels and
+//                edges, H- tables, Rayleigh fits, CH/OH rows, Fe I/II cross
+//                sections, HLINOP's levels).  Rows of the ATLAS12 grid are
+//                formed once per line-store grid and kept.
+//   k_kapp_state one thread per depth: the per-state factors the sources
+//                cache between calls; k_kapp_tb: TOPbase Boltzmann factors,
+//                one thread per (level, depth).
+//   k_kapp_cells one thread per (frequency, depth), consecutive threads
+//                walk the depths of a frequency: every source and KAPP's
+//                totals, written as the six (nrhox, numnu) tables the RT
+//                pass reads (or all 23 per-source arrays for a12_kapp_eval).
+// The per-cell work is VALU FP64 (no HBM-heavy streams: a frequency row is
+// read by the 72 threads of its depths from cache); TOPbase's ~2,000 level
+// loop dominates it.
+#include "a12_internal.h"
+
+#pragma clang fp contract(off)
+
+#include "a12_kapp_core.h"
+#include "a12_kapp_host.h"
+
+#include <cstring>
+#include <vector>
+
+using namespace kapp;
+
+struct KappDev {
+    KPack pack;
+    DevBuf td, ti, tf;           // packed tables
+    bool tables = false;
+    int n = 0, mion = 0;
+    std::vector<double> t_h;     // T of the KAPP state
+    DevBuf st, planes;           // packed state, per-state planes
+    bool state = false;
+    DevBuf fq;                   // frequency rows of the ATLAS12 grid
+    uint64_t fq_gen = 0;
+    int fq_numnu = 0;
+    DevBuf ew, eout, erow;       // a12_kapp_eval scratch
+    std::vector<double> stage;
+};
+
+namespace {
+
+KTab dev_tab(const KappDev &K)
+{
+    return kapp_tab(K.pack, K.td.as<double>(), K.ti.as<int>(), K.tf.as<float>());
+}
+
+__global__ void k_kapp_freq(KTab T, int nw, const double *__restrict__ waves, double *__restrict__ rows)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nw) return;
+    kp_freq(T, waves[w], rows + (size_t)w * KF_N);
+}
+
+__global__ void k_kapp_state(KTab T, KState S, double *__restrict__ planes)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= S.n) return;
+    kp_state(T, S, Q_HOLTSMARK, j, planes);
+}
+
+__global__ void k_kapp_tb(KTab T, KState S, double *__restrict__ planes)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= T.tb_nlev_tot * S.n) return;
+    const int l = idx / S.n, j = idx - l * S.n;
+    planes[(size_t)(KS_TB + l) * S.n + j] = kp_state_tb(T, S, l, j);
+}
+
+// mode 0: out = six tables (nrhox, nw) in the order ACONT SIGMAC SCONT SLINE
+// ALINE SIGMAL; mode 1: out (nw, 23, nrhox)
+__global__ void __launch_bounds__(256) k_kapp_cells(KTab T, KState S, const double *__restrict__ planes,
+                                                    const double *__restrict__ rows,
+                                                    const double *__restrict__ waves, int nw, int mode,
+                                                    double *__restrict__ out)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = S.n;
+    if (idx >= (long)nw * n) return;
+    const int w = (int)(idx / n), j = (int)(idx - (long)w * n);
+    double cell[A12_KAPP_NSRC];
+    kp_cell(T, S, planes, rows + (size_t)w * KF_N, waves[w], j, cell);
+    if (mode == 0) {
+        const size_t plane = (size_t)nw * n;
+        for (int k = 0; k < 6; ++k) out[k * plane + (size_t)w * n + j] = cell[17 + k];
+    } else {
+        for (int k = 0; k < A12_KAPP_NSRC; ++k) out[((size_t)w * A12_KAPP_NSRC + k) * n + j] = cell[k];
+    }
+}
+
+KappDev &kdev(a12_ctx *ctx)
+{
+    if (!ctx->kapp) ctx->kapp = new KappDev();
+    return *ctx->kapp;
+}
+
+int launch_rows(a12_ctx *ctx, KappDev &K, int nw, const double *d_waves, double *d_rows)
+{
+    k_kapp_freq<<<(nw + 63) / 64, 64, 0, ctx->stream>>>(dev_tab(K), nw, d_waves, d_rows);
+    A12_HIP(hipGetLastError());
+    return A12_OK;
+}
+
+int launch_cells(a12_ctx *ctx, KappDev &K, int nw, const double *d_waves, const double *d_rows, int mode,
+                 double *d_out)
+{
+    const long cells = (long)nw * K.n;
+    const KState S = kapp_state_view(K.st.as<double>(), K.n, K.mion);
+    k_kapp_cells<<<(unsigned)((cells + 255) / 256), 256, 0, ctx->stream>>>(
+        dev_tab(K), S, K.planes.as<double>(), d_rows, d_waves, nw, mode, d_out);
+    A12_HIP(hipGetLastError());
+    return A12_OK;
+}
+
+}  // namespace
+
+void a12i_kapp_free(a12_ctx *ctx)
+{
+    if (!ctx->kapp) return;
+    KappDev &K = *ctx->kapp;
+    DevBuf *b[] = {&K.td, &K.ti, &K.tf, &K.st, &K.planes, &K.fq, &K.ew, &K.eout, &K.erow};
+    for (DevBuf *p : b) p->release();
+    delete ctx->kapp;
+    ctx->kapp = nullptr;
+}
+
+extern "C" {
+
+int a12_kapp_constants(double *out12)
+{
+    if (!out12) return A12_ERR_ARG;
+    const double v[12] = {HCK, HOVERK, KBOL_EV, THETA_COEFF, BNU_PREFAC, saha_prefac(), FREQ_RYDH,
+                          RYD_ANG, LN10, COEFF_FF, SIGMA_THOMSON, CLIGHT_ANGS};
+    std::memcpy(out12, v, sizeof(v));
+    return A12_OK;
+}
+
+int a12_kapp_set_tables(a12_ctx *ctx, const a12_kapp_tables *tables)
+{
+    if (!ctx) return A12_ERR_ARG;
+    KappDev &K = kdev(ctx);
+    KPack P;
+    const std::string e = kapp_pack(tables, P);
+    if (!e.empty()) {
+        ctx->err = "a12_kapp_set_tables: " + e;
+        return A12_ERR_ARG;
+    }
+    A12_HIP(hipSetDevice(ctx->device));
+    K.tables = K.state = false;
+    K.fq_gen = 0;
+    K.pack = std::move(P);
+    A12_HIP(K.td.reserve(sizeof(double) * K.pack.d.size()));
+    A12_HIP(K.ti.reserve(sizeof(int) * K.pack.i.size()));
+    A12_HIP(K.tf.reserve(sizeof(float) * K.pack.f.size()));
+    A12_HIP(hipMemcpyAsync(K.td.p, K.pack.d.data(), sizeof(double) * K.pack.d.size(), hipMemcpyHostToDevice,
+                           ctx->stream));
+    A12_HIP(hipMemcpyAsync(K.ti.p, K.pack.i.data(), sizeof(int) * K.pack.i.size(), hipMemcpyHostToDevice,
+                           ctx->stream));
+    A12_HIP(hipMemcpyAsync(K.tf.p, K.pack.f.data(), sizeof(float) * K.pack.f.size(), hipMemcpyHostToDevice,
+                           ctx->stream));
+    A12_HIP(hipStreamSynchronize(ctx->stream));
+    K.tables = true;
+    return A12_OK;
+}
+
+int a12_kapp_set_state(a12_ctx *ctx, const a12_kapp_state *state)
+{
+    if (!ctx) return A12_ERR_ARG;
+    KappDev &K = kdev(ctx);
+    if (!K.tables) {
+        ctx->err = "a12_kapp_set_state: call a12_kapp_set_tables first";
+        return A12_ERR_STATE;
+    }
+    const std::string e = kapp_check_state(state);
+    if (!e.empty()) {
+        ctx->err = "a12_kapp_set_state: " + e;
+        return A12_ERR_ARG;
+    }
+    if (state->mion < kapp_max_species(K.pack)) {
+        ctx->err = "a12_kapp_set_state: mion is smaller than a species index the tables use";
+        return A12_ERR_ARG;
+    }
+    A12_HIP(hipSetDevice(ctx->device));
+    K.state = false;
+    K.n = state->nrhox;
+    K.mion = state->mion;
+    K.t_h.assign(state->t, state->t + K.n);
+    kapp_pack_state(state, K.stage);
+    A12_HIP(K.st.reserve(sizeof(double) * K.stage.size()));
+    A12_HIP(hipMemcpyAsync(K.st.p, K.stage.data(), sizeof(double) * K.stage.size(), hipMemcpyHostToDevice,
+                           ctx->stream));
+    const KTab T = dev_tab(K);
+    const size_t np = (size_t)kapp_nplanes(T) * K.n;
+    A12_HIP(K.planes.reserve(sizeof(double) * np));
+    const KState S = kapp_state_view(K.st.as<double>(), K.n, K.mion);
+    k_kapp_state<<<1, 128, 0, ctx->stream>>>(T, S, K.planes.as<double>());
+    A12_HIP(hipGetLastError());
+    const int ntb = T.tb_nlev_tot * K.n;
+    if (ntb > 0) {
+        k_kapp_tb<<<(ntb + 255) / 256, 256, 0, ctx->stream>>>(T, S, K.planes.as<double>());
+        A12_HIP(hipGetLastError());
+    }
+    A12_HIP(hipStreamSynchronize(ctx->stream));
+    K.state = true;
+    return A12_OK;
+}
+
+int a12_kapp_eval(a12_ctx *ctx, int32_t nwave, const double *wave_nm, double *out)
+{
+    if (!ctx) return A12_ERR_ARG;
+    KappDev &K = kdev(ctx);
+    if (!K.tables || !K.state) {
+        ctx->err = "a12_kapp_eval: call a12_kapp_set_tables and a12_kapp_set_state first";
+        return A12_ERR_STATE;
+    }
+    if (nwave < 1 || nwave > (1 << 22) || !wave_nm || !out) {
+        ctx->err = "a12_kapp_eval: bad nwave or null array";
+        return A12_ERR_ARG;
+    }
+    for (int w = 0; w < nwave; ++w)
+        if (!(wave_nm[w] > 0.0) || !std::isfinite(wave_nm[w])) {
+            ctx->err = "a12_kapp_eval: wavelengths must be positive and finite";
+            return A12_ERR_ARG;
+        }
+    A12_HIP(hipSetDevice(ctx->device));
+    const size_t nout = (size_t)nwave * A12_KAPP_NSRC * K.n;
+    A12_HIP(K.ew.reserve(sizeof(double) * nwave));
+    A12_HIP(K.erow.reserve(sizeof(double) * (size_t)nwave * KF_N));
+    A12_HIP(K.eout.reserve(sizeof(double) * nout));
+    A12_HIP(hipMemcpyAsync(K.ew.p, wave_nm, sizeof(double) * nwave, hipMemcpyHostToDevice, ctx->stream));
+    int rc = launch_rows(ctx, K, nwave, K.ew.as<double>(), K.erow.as<double>());
+    if (rc) return rc;
+    rc = launch_cells(ctx, K, nwave, K.ew.as<double>(), K.erow.as<double>(), 1, K.eout.as<double>());
+    if (rc) return rc;
+    A12_HIP(hipMemcpyAsync(out, K.eout.p, sizeof(double) * nout, hipMemcpyDeviceToHost, ctx->stream));
+    A12_HIP(hipStreamSynchronize(ctx->stream));
+    return A12_OK;
+}
+
+int a12_atlas_kapp_run(a12_ctx *ctx)
+{
+    if (!ctx) return A12_ERR_ARG;
+    AtlasDev &A = ctx->atl;
+    KappDev &K = kdev(ctx);
+    if (!A.loaded) {
+        ctx->err = "a12_atlas_kapp_run: call a12_atlas_set_state (or a12_atlas_load) first";
+        return A12_ERR_STATE;
+    }
+    if (!K.tables || !K.state) {
+        ctx->err = "a12_atlas_kapp_run: call a12_kapp_set_tables and a12_kapp_set_state first";
+        return A12_ERR_STATE;
+    }
+    if (K.n != A.n || std::memcmp(K.t_h.data(), A.t_h.data(), sizeof(double) * K.n) != 0) {
+        ctx->err = "a12_atlas_kapp_run: the KAPP state is not the ATLAS12 state (NRHOX or T differ)";
+        return A12_ERR_STATE;
+    }
+    A12_HIP(hipSetDevice(ctx->device));
+    int rc;
+    if (K.fq_gen != A.grid_gen || K.fq_numnu != A.numnu) {
+        A12_HIP(K.fq.reserve(sizeof(double) * (size_t)A.numnu * KF_N));
+        rc = launch_rows(ctx, K, A.numnu, A.waveset.as<double>(), K.fq.as<double>());
+        if (rc) return rc;
+        K.fq_gen = A.grid_gen;
+        K.fq_numnu = A.numnu;
+    }
+    const size_t plane = (size_t)A.n * A.numnu;
+    A12_HIP(A.kapp.reserve(sizeof(double) * 6 * plane));
+    if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[14], ctx->stream));
+    rc = launch_cells(ctx, K, A.numnu, A.waveset.as<double>(), K.fq.as<double>(), 0, A.kapp.as<double>());
+    if (rc) return rc;
+    if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[15], ctx->stream));
+    A12_HIP(hipStreamSynchronize(ctx->stream));
+    A.kapp_ready = true;
+    A.kapp_plane = plane;
+    A.prof_kapp = ctx->prof;
+    return A12_OK;
+}
+
+int a12_atlas_kapp_fetch(a12_ctx *ctx, double *kapp6)
+{
+    if (!ctx || !kapp6) return A12_ERR_ARG;
+    AtlasDev &A = ctx->atl;
+    if (!A.kapp_ready) {
+        ctx->err = "a12_atlas_kapp_fetch: no KAPP tables on the device";
+        return A12_ERR_STATE;
+    }
+    A12_HIP(hipSetDevice(ctx->device));
+    A12_HIP(hipMemcpyAsync(kapp6, A.kapp.p, sizeof(double) * 6 * A.kapp_plane, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    A12_HIP(hipStreamSynchronize(ctx->stream));
+    return A12_OK;
+}
+
+int a12_kapp_ms(a12_ctx *ctx, float *ms)
+{
+    if (!ctx || !ms) return A12_ERR_ARG;
+    *ms = -1.0f;
+    if (ctx->atl.prof_kapp && ctx->atl.kapp_ready) {
+        A12_HIP(hipEventSynchronize(ctx->ev[15]));
+        A12_HIP(hipEventElapsedTime(ms, ctx->ev[14], ctx->ev[15]));
+    }
+    return A12_OK;
+}
+
+}  // extern "C"

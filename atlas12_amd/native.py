@@ -40,7 +40,9 @@ EXPORTS = ["a12_ctx_create", "a12_ctx_destroy", "a12_last_error", "a12_stream",
            "a12_synthe_shape_info", "a12_synthe_set_shape", "a12_synthe_batch_load",
            "a12_synthe_batch_set_budget", "a12_synthe_batch_run", "a12_synthe_batch_fetch",
            "a12_synthe_batch_info", "a12_solvit_batched", "a12_atlas_load_lines",
-           "a12_atlas_set_state", "a12_atlas_rt_pass_noalloc", "a12_model_converge"]
+           "a12_atlas_set_state", "a12_atlas_rt_pass_noalloc", "a12_model_converge",
+           "a12_kapp_set_tables", "a12_kapp_set_state", "a12_atlas_kapp_run", "a12_kapp_eval",
+           "a12_kapp_constants", "a12_kapp_ms", "a12_atlas_kapp_fetch"]
 
 
 class A12Error(RuntimeError):
@@ -763,3 +765,173 @@ def model_converge(ctx: Context, par, t_start, itmax: int = 60, history: bool = 
     if history:
         return iters, t_out, st, hist
     return iters, t_out, st
+
+
+# ---- KAPP: continuum opacity ---------------------------------------------------
+KAPP_SRC = ("ahyd", "shyd", "ah2p", "ahmin", "shmin", "sigh", "ahe1", "ahe2", "ahemin", "sighe",
+            "ah2coll", "ah2min", "acont_metal", "sigel", "sigh2", "ahline", "shline",
+            "acont", "sigmac", "scont", "sline", "aline", "sigmal")
+KAPP_CONST = ("hck", "hoverk", "kbol_ev", "theta_coeff", "bnu_prefac", "saha_prefac", "freq_rydh",
+              "ryd_ang", "ln10", "coeff_ff", "sigma_thomson", "clight_angs")
+
+_vp = C.c_void_p
+
+
+class KappStark(C.Structure):
+    _fields_ = [("loaded", C.c_int32), ("n_upper_min", C.c_int32), ("n_upper_max", C.c_int32),
+                ("n_dens", C.c_int32), ("dens", _vp), ("temp", _vp), ("log_dalpha", _vp),
+                ("max_dens_idx", _vp), ("k_alpha", _vp), ("profiles", _vp)]
+
+
+class KappTables(C.Structure):
+    _fields_ = [("k_freqn", _vp), ("k_xn", _vp), ("k_xl", _vp), ("k_ekarzas", _vp),
+                ("gff_logtab", _vp),
+                ("cia_npair", C.c_int32), ("cia_ntmax", C.c_int32), ("cia_nnu", C.c_int32),
+                ("cia_nu0", C.c_double), ("cia_dnu", C.c_double), ("cia_nt", _vp),
+                ("cia_temp", _vp), ("cia_tab", _vp), ("cia_use", _vp),
+                ("cross_ch", _vp), ("cross_oh", _vp),
+                ("hi_nedge", C.c_int32), ("hi_edge", _vp),
+                ("felo_nlev", C.c_int32 * 2), ("felo_xnfp_idx", C.c_int32 * 2),
+                ("felo_nuth", _vp), ("felo_g", _vp), ("felo_e", _vp), ("felo_ifancy", _vp),
+                ("felo_npts", _vp), ("felo_x", _vp), ("felo_logsig", _vp), ("felo_seaton", _vp),
+                ("tb_nspecies", C.c_int32), ("tb_nlevels", C.c_int32), ("tb_npoints", C.c_int32),
+                ("tb_xnfp_idx", _vp), ("tb_nlev", _vp), ("tb_lowest_thr", _vp), ("tb_g", _vp),
+                ("tb_np", _vp), ("tb_ebind", _vp), ("tb_eexc", _vp), ("tb_logemin", _vp),
+                ("tb_logestride", _vp), ("tb_ephot", _vp), ("tb_sigma", _vp),
+                ("stark", KappStark * 4)]
+
+
+class KappState(C.Structure):
+    _fields_ = [("nrhox", C.c_int32), ("ld", C.c_int32), ("mion", C.c_int32),
+                ("t", _vp), ("tkev", _vp), ("tlog", _vp), ("hkt", _vp), ("hckt", _vp),
+                ("xne", _vp), ("rho", _vp), ("bmin", _vp), ("bhyd", _vp), ("equilh2", _vp),
+                ("xnf", _vp), ("xnfp", _vp)]
+
+
+class _Keep:
+    """A ctypes struct together with the numpy arrays its pointers point into."""
+
+    def __init__(self, struct, arrays):
+        self.struct = struct
+        self.arrays = arrays
+
+
+def _arr(keep, a, dtype):
+    a = np.ascontiguousarray(a, dtype)
+    keep.append(a)
+    return a.ctypes.data_as(_vp)
+
+
+def kapp_tables(t) -> _Keep:
+    """a12_kapp_tables from a mapping of arrays in the layout the C-ABI
+    documents (the keys of tests/golden/kapp_tables.npz)."""
+    keep = []
+    T = KappTables()
+    T.k_freqn = _arr(keep, t["k_freqn"], np.float32)
+    T.k_xn = _arr(keep, t["k_xn"], np.float32)
+    T.k_xl = _arr(keep, t["k_xl"], np.float32)
+    T.k_ekarzas = _arr(keep, t["k_ekarzas"], np.float32)
+    T.gff_logtab = _arr(keep, t["gff_logtab"], np.float64)
+    nt = np.asarray(t["cia_nt"] if "cia_nt" in t else t["cia_nciat"], np.int32)
+    temp = np.asarray(t["cia_temp"], np.float64)
+    tab = np.asarray(t["cia_tab"], np.float64)
+    T.cia_npair, T.cia_ntmax, T.cia_nnu = len(nt), temp.shape[1], tab.shape[2]
+    T.cia_nu0, T.cia_dnu = (float(x) for x in t["cia_nu"])
+    T.cia_nt = _arr(keep, nt, np.int32)
+    T.cia_temp = _arr(keep, temp, np.float64)
+    T.cia_tab = _arr(keep, tab, np.float64)
+    T.cia_use = None
+    T.cross_ch = _arr(keep, t["cross_ch"], np.float64)
+    T.cross_oh = _arr(keep, t["cross_oh"], np.float64)
+    hi = np.asarray(t["hi_edge"], np.float64)
+    T.hi_nedge = len(hi)
+    T.hi_edge = _arr(keep, hi, np.float64)
+    for k in range(2):
+        T.felo_nlev[k] = int(t["felo_nlev"][k])
+        T.felo_xnfp_idx[k] = int(t["felo_xnfp_idx"][k])
+    for k in ("nuth", "g", "e", "x", "logsig", "seaton"):
+        setattr(T, "felo_" + k, _arr(keep, t["felo_" + k], np.float64))
+    T.felo_ifancy = _arr(keep, t["felo_ifancy"], np.int32)
+    T.felo_npts = _arr(keep, t["felo_npts"], np.int32)
+    T.tb_nspecies = len(t["tb_nlev"])
+    T.tb_nlevels = len(t["tb_g"])
+    T.tb_npoints = len(t["tb_sigma"])
+    T.tb_xnfp_idx = _arr(keep, t["tb_xnfp_idx"], np.int32)
+    T.tb_nlev = _arr(keep, t["tb_nlev"], np.int32)
+    T.tb_lowest_thr = _arr(keep, t["tb_lowest_thr"], np.float64)
+    T.tb_g = _arr(keep, t["tb_g"], np.int32)
+    T.tb_np = _arr(keep, t["tb_np"], np.int32)
+    for k in ("ebind", "eexc", "logemin", "logestride", "ephot", "sigma"):
+        setattr(T, "tb_" + k, _arr(keep, t["tb_" + k], np.float64))
+    for s in range(4):
+        hdr = np.asarray(t[f"st{s}_hdr"])
+        S = T.stark[s]
+        S.loaded, S.n_upper_min, S.n_upper_max, S.n_dens = (int(x) for x in hdr[[0, 2, 3, 4]])
+        if S.loaded:
+            S.dens = _arr(keep, t[f"st{s}_dens"], np.float64)
+            S.temp = _arr(keep, t[f"st{s}_temp"], np.float64)
+            S.log_dalpha = _arr(keep, t[f"st{s}_ldalpha"], np.float64)
+            S.max_dens_idx = _arr(keep, t[f"st{s}_maxdens"], np.int32)
+            S.k_alpha = _arr(keep, t[f"st{s}_kalpha"], np.float64)
+            S.profiles = _arr(keep, t[f"st{s}_prof"], np.float64)
+    return _Keep(T, keep)
+
+
+def kapp_state(st, mion: int = 1006) -> _Keep:
+    """a12_kapp_state from depth arrays t, tkev, tlog, hkt, hckt, xne, rho,
+    bmin, equilh2 (nrhox), bhyd (6, nrhox) and xnf, xnfp (mion, nrhox)."""
+    keep = []
+    n = len(st["t"])
+    S = KappState()
+    S.nrhox, S.ld, S.mion = n, n, mion
+    for k in ("t", "tkev", "tlog", "hkt", "hckt", "xne", "rho", "bmin", "equilh2"):
+        setattr(S, k, _arr(keep, st[k], np.float64))
+    S.bhyd = _arr(keep, st["bhyd"], np.float64)
+    S.xnf = _arr(keep, st["xnf"], np.float64)
+    S.xnfp = _arr(keep, st["xnfp"], np.float64)
+    return _Keep(S, keep)
+
+
+def kapp_constants() -> dict:
+    """a12_kapp_constants (no GPU needed)."""
+    out = np.zeros(12)
+    L = load_library()
+    if L.a12_kapp_constants(out.ctypes.data_as(_vp)) != 0:
+        raise A12Error("a12_kapp_constants failed")
+    return dict(zip(KAPP_CONST, out))
+
+
+def kapp_set_tables(ctx: Context, tables: _Keep):
+    ctx._check(ctx.lib.a12_kapp_set_tables(ctx.h, C.byref(tables.struct)), "a12_kapp_set_tables")
+
+
+def kapp_set_state(ctx: Context, state: _Keep):
+    ctx._check(ctx.lib.a12_kapp_set_state(ctx.h, C.byref(state.struct)), "a12_kapp_set_state")
+
+
+def kapp_eval(ctx: Context, waves_nm, nrhox: int) -> dict:
+    """a12_kapp_eval: every per-source array of KAPP at the given
+    wavelengths; returns {name: (nwave, nrhox)} in KAPP_SRC order."""
+    w = np.ascontiguousarray(waves_nm, np.float64)
+    out = np.zeros((len(w), len(KAPP_SRC), nrhox))
+    ctx._check(ctx.lib.a12_kapp_eval(ctx.h, C.c_int32(len(w)), _ptr(w), _ptr(out)), "a12_kapp_eval")
+    return {k: np.ascontiguousarray(out[:, i]) for i, k in enumerate(KAPP_SRC)}
+
+
+def atlas_kapp_run(ctx: Context):
+    ctx._check(ctx.lib.a12_atlas_kapp_run(ctx.h), "a12_atlas_kapp_run")
+
+
+def atlas_kapp_fetch(ctx: Context) -> dict:
+    """The six KAPP tables on the device: {acont, sigmac, scont, sline, aline,
+    sigmal} each (numnu, nrhox)."""
+    n, nf = _atl_dims(ctx)
+    out = np.zeros((6, nf, n))
+    ctx._check(ctx.lib.a12_atlas_kapp_fetch(ctx.h, _ptr(out)), "a12_atlas_kapp_fetch")
+    return dict(zip(("acont", "sigmac", "scont", "sline", "aline", "sigmal"), out))
+
+
+def kapp_ms(ctx: Context) -> float:
+    ms = C.c_float(-1.0)
+    ctx._check(ctx.lib.a12_kapp_ms(ctx.h, C.byref(ms)), "a12_kapp_ms")
+    return ms.value

@@ -362,6 +362,112 @@ int a12_atlas_select_lines(a12_ctx *ctx, const a12_atlas_atm *atm, const double 
                            const int32_t *pred, int64_t npred, const a12_mol2_rec *mol2,
                            int64_t nmol2, int32_t *linedata_out, int64_t cap, int64_t *nkept);
 
+/* ---- KAPP: continuum opacity ----------------------------------------------- */
+/* KAPP (src/atlas12_modules.f90:11202-11324) with every source it calls:
+ * HOP (:11359), H2PLOP (:12224), HMINOP (:12360), H2MINOP (:12611),
+ * H2COLLOP (:16371), HRAYOP (:12740), HE1OP (:12939), HE2OP (:13569),
+ * HEMIOP (:13716), HERAOP (:13918), CONT_METAL_OPACITY_TOPBASE (:14147:
+ * MBF_TOPBASE, MBF_HIGH_ION, FELO_OPACITY for Fe I and Fe II, CHOP and
+ * OHOP), ELECOP (:17212), H2RAOP (:17289) and HLINOP (:17381, with
+ * STARK_MMM :17854 and STARK :17528).  Replaces the 30,000 KAPP calls per
+ * RUN_RT_PASS (:3918): a12_atlas_kapp_run fills the tables that
+ * a12_atlas_set_kapp would upload.
+ *
+ * Tables: what the sources read from DATADIR or hold after their first
+ * call, in the reference's units.  Arrays are C order with the last index
+ * fastest; the Fortran (first-index-fastest) arrays map as noted. */
+typedef struct {
+    int32_t loaded;                  /* STEHLE_DATA(s)%loaded */
+    int32_t n_upper_min, n_upper_max, n_dens;
+    const double *dens;              /* (20) density_grid */
+    const double *temp;              /* (10) temp_grid */
+    const double *log_dalpha;        /* (60) log_dalpha_grid */
+    const int32_t *max_dens_idx;     /* (n_trans) */
+    const double *k_alpha;           /* (n_trans) */
+    const double *profiles;          /* (n_trans, n_dens, 10, 60) = profiles(60,10,n_dens,n_trans) */
+} a12_kapp_stark;
+
+typedef struct {
+    /* XKARZAS (:11635), REAL(4) as read from karzas_*.dat */
+    const float *k_freqn;            /* (15, 29) = FREQN(29, 15) */
+    const float *k_xn;               /* (15, 29) = XN(29, 15) */
+    const float *k_xl;               /* (6, 6, 29) = XL(29, 6, 6): [n][l][point] */
+    const float *k_ekarzas;          /* (29) */
+    /* COULFF (:11995): log10 g_ff of gauntff_vanhoof.dat */
+    const double *gff_logtab;        /* (146, 81) = GFF_LOGTAB(81, 146) */
+    /* H2COLLOP (:16371): h2collop.dat */
+    int32_t cia_npair, cia_ntmax, cia_nnu;
+    double cia_nu0, cia_dnu;
+    const int32_t *cia_nt;           /* (npair) */
+    const double *cia_temp;          /* (npair, ntmax) */
+    const double *cia_tab;           /* (npair, ntmax, nnu) = CIATAB(nnu, ntmax, npair) */
+    const int32_t *cia_use;          /* (npair) USE_CIA_PAIR, may be NULL (all on) */
+    /* CHOP (:16103), OHOP (:16228): crossch.dat, crossoh.dat */
+    const double *cross_ch;          /* (105, 15) = CROSSCH(15, 105) */
+    const double *cross_oh;          /* (130, 15) = CROSSOH(15, 130) */
+    /* MBF_HIGH_ION (:14338): HI_EDGE_DATA after INIT_MBF_HIGH_ION */
+    int32_t hi_nedge;
+    const double *hi_edge;           /* (nedge, 7) = HI_EDGE_DATA(7, nedge) */
+    /* FELO_OPACITY (:14710): Fe I, Fe II after INIT_FELO */
+    int32_t felo_nlev[2], felo_xnfp_idx[2];
+    const double *felo_nuth, *felo_g, *felo_e;   /* (2, 60) */
+    const int32_t *felo_ifancy, *felo_npts;      /* (2, 60) */
+    const double *felo_x, *felo_logsig;          /* (2, 60, 200) */
+    const double *felo_seaton;                   /* (2, 60, 4) */
+    /* MBF_TOPBASE (:14226): TB_SPECIES after INIT_MBF_TOPBASE, levels of all
+     * species concatenated in species order, their points likewise */
+    int32_t tb_nspecies, tb_nlevels, tb_npoints;
+    const int32_t *tb_xnfp_idx, *tb_nlev;        /* (nspecies) */
+    const double *tb_lowest_thr;                 /* (nspecies) LOWEST_THR_RY */
+    const int32_t *tb_g, *tb_np;                 /* (nlevels) G_STAT, NP */
+    const double *tb_ebind, *tb_eexc;            /* (nlevels) E_BIND_RY, E_EXC_RY */
+    const double *tb_logemin, *tb_logestride;    /* (nlevels) */
+    const double *tb_ephot, *tb_sigma;           /* (npoints) E_PHOTON_RY, SIGMA_MB */
+    /* STARK_MMM (:17854): STEHLE_DATA after INIT_STARK_TABLES */
+    a12_kapp_stark stark[4];
+} a12_kapp_tables;
+
+/* The depth state KAPP reads (module arrays after POPS): Fortran arrays with
+ * leading dimension ld (kw); XNF/XNFP (ld, mion), BHYD (ld, 6).  equilh2 is
+ * EQUILH2(T(J)) (:22437), the EOS's H2 equilibrium constant. */
+typedef struct {
+    int32_t nrhox, ld, mion;
+    const double *t, *tkev, *tlog, *hkt, *hckt, *xne, *rho, *bmin;
+    const double *bhyd, *equilh2, *xnf, *xnfp;
+} a12_kapp_state;
+
+/* Copies the tables to the device (once per run; A12_ERR_ARG on
+ * inconsistent counts or a missing array). */
+int a12_kapp_set_tables(a12_ctx *ctx, const a12_kapp_tables *tables);
+/* Uploads the depth state and forms the per-state factors the sources keep
+ * between calls (the ITEMP/TB_T_LAST caches: occupation probabilities,
+ * Boltzmann factors, CIA weights, H- and H2 densities). */
+int a12_kapp_set_state(a12_ctx *ctx, const a12_kapp_state *state);
+/* KAPP at every frequency of the loaded ATLAS12 grid (WAVESET of
+ * a12_atlas_load_lines), written as the six device tables a12_atlas_rt_run
+ * reads; same layout and effect as a12_atlas_set_kapp.  The state's nrhox
+ * must equal the grid's. */
+int a12_atlas_kapp_run(a12_ctx *ctx);
+/* Copies the six KAPP tables on the device (from a12_atlas_kapp_run or
+ * a12_atlas_set_kapp) to kapp6 (6, numnu, nrhox), in a12_atlas_set_kapp's
+ * order and layout. */
+int a12_atlas_kapp_fetch(a12_ctx *ctx, double *kapp6);
+/* KAPP at nwave wavelengths (nm), with RUN_RT_PASS's frequency set-up
+ * (:3905-3915); out (nwave, 23, nrhox), host memory, per wavelength:
+ * AHYD SHYD AH2P AHMIN SHMIN SIGH AHE1 AHE2 AHEMIN SIGHE AH2COLL AH2MIN
+ * ACONT_METAL SIGEL SIGH2 AHLINE SHLINE ACONT SIGMAC SCONT SLINE ALINE
+ * SIGMAL. */
+#define A12_KAPP_NSRC 23
+int a12_kapp_eval(a12_ctx *ctx, int32_t nwave, const double *wave_nm, double *out);
+/* Milliseconds of the last a12_atlas_kapp_run (with a12_set_profiling on),
+ * -1 when not measured. */
+int a12_kapp_ms(a12_ctx *ctx, float *ms);
+/* The physical constants the library's KAPP uses, in this order: HCK,
+ * HOVERK, KBOL_EV, THETA_COEFF, BNU_PREFAC, SAHA_PREFAC, FREQ_RYDH,
+ * RYD_ANG, LN10, COEFF_FF, SIGMA_THOMSON, CLIGHT_ANGS (mod_constants,
+ * :318-407).  No GPU needed. */
+int a12_kapp_constants(double *out12);
+
 /* ---- model start ---------------------------------------------------------- */
 /* SCALE_MODEL(TEFF_NEW, LOGG_NEW), src/atlas12_modules.f90:6967-7060, for
  * nmodel models at once (one wavefront each): regrid onto TAUSTD(i) =

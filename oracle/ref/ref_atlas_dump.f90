@@ -181,4 +181,91 @@ PROGRAM ref_atlas_dump
     CALL TCORR(3, 0.0D0)
     CLOSE(66)
   END IF
+
+  ! With A12_KAPP_DUMP and A12_KAPP_WAVES set: KAPP's inputs (the depth
+  ! state and the tables its sources load on first use), then, at every
+  ! wavelength of A12_KAPP_WAVES (int32 count, then REAL(8) nm), the
+  ! frequency set-up of RUN_RT_PASS, KAPP, and every per-source array.
+  CALL GET_ENVIRONMENT_VARIABLE('A12_KAPP_DUMP', envval, envlen, envstat)
+  IF (envstat .EQ. 0 .AND. envlen .GT. 0) CALL dump_kapp(TRIM(envval))
+
+CONTAINS
+
+  SUBROUTINE dump_kapp(path)
+    CHARACTER(LEN=*), INTENT(IN) :: path
+    CHARACTER(LEN=512) :: wpath
+    INTEGER :: uo, uw, nw, iw, is, il, s, nt, k
+    REAL(8), ALLOCATABLE :: waves(:)
+    REAL(8) :: eqh2(kw)
+
+    CALL GET_ENVIRONMENT_VARIABLE('A12_KAPP_WAVES', wpath)
+    OPEN(NEWUNIT=uw, FILE=TRIM(wpath), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
+    READ(uw) nw
+    ALLOCATE(waves(nw))
+    READ(uw) waves
+    CLOSE(uw)
+    DO k = 1, NRHOX
+      eqh2(k) = EQUILH2(T(k))
+    END DO
+
+    OPEN(NEWUNIT=uo, FILE=path, ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+    WRITE(uo) NRHOX, MION, kw, nw
+    WRITE(uo) HCK, HOVERK, KBOL_EV, THETA_COEFF, BNU_PREFAC, SAHA_PREFAC, FREQ_RYDH, RYD_ANG, &
+              LN10, COEFF_FF, SIGMA_THOMSON, CLIGHT_ANGS
+    WRITE(uo) T(1:NRHOX), TKEV(1:NRHOX), TLOG(1:NRHOX), HKT(1:NRHOX), HCKT(1:NRHOX), &
+              XNE(1:NRHOX), RHO(1:NRHOX), BMIN(1:NRHOX), BHYD(1:NRHOX, 1:6), eqh2(1:NRHOX)
+    WRITE(uo) XNF(1:NRHOX, 1:MION), XNFP(1:NRHOX, 1:MION)
+    WRITE(uo) Q_HOLTSMARK, LOG_BETA_MIN, LOG_BETA_STEP
+    ! TOPbase levels as INIT_MBF_TOPBASE left them
+    WRITE(uo) N_TB_SPECIES
+    DO is = 1, N_TB_SPECIES
+      WRITE(uo) TB_SPECIES(is)%XNFP_INDEX, TB_SPECIES(is)%NLEVELS, TB_SPECIES(is)%LOWEST_THR_RY
+      DO il = 1, TB_SPECIES(is)%NLEVELS
+        WRITE(uo) TB_SPECIES(is)%LEVELS(il)%G_STAT, TB_SPECIES(is)%LEVELS(il)%NP, &
+                  TB_SPECIES(is)%LEVELS(il)%E_BIND_RY, TB_SPECIES(is)%LEVELS(il)%E_EXC_RY, &
+                  TB_SPECIES(is)%LEVELS(il)%LOG_E_MIN, TB_SPECIES(is)%LEVELS(il)%LOG_E_STRIDE
+        WRITE(uo) TB_SPECIES(is)%LEVELS(il)%E_PHOTON_RY, TB_SPECIES(is)%LEVELS(il)%SIGMA_MB
+      END DO
+    END DO
+    ! hotop.dat edges kept by INIT_MBF_HIGH_ION
+    WRITE(uo) N_HIGH_ION_EDGES
+    WRITE(uo) HI_EDGE_DATA(1:7, 1:N_HIGH_ION_EDGES)
+    ! Fe I / Fe II (INIT_FELO)
+    WRITE(uo) FELO_NLEV, FELO_XNFP_IDX, FELO_NUTH, FELO_G, FELO_E, FELO_IFANCY, FELO_NPTS, &
+              FELO_X, FELO_LOGSIG, FELO_SEATON
+    ! Stehle hydrogen Stark tables (INIT_STARK_TABLES)
+    DO s = 1, NSTARK_SERIES
+      k = 0
+      IF (STEHLE_DATA(s)%loaded) k = 1
+      WRITE(uo) k, STEHLE_DATA(s)%n_lower, STEHLE_DATA(s)%n_upper_min, &
+                STEHLE_DATA(s)%n_upper_max, STEHLE_DATA(s)%n_dens, &
+                STEHLE_DATA(s)%density_grid, STEHLE_DATA(s)%temp_grid, &
+                STEHLE_DATA(s)%log_dalpha_grid
+      IF (k .EQ. 1) THEN
+        nt = STEHLE_DATA(s)%n_upper_max - STEHLE_DATA(s)%n_upper_min + 1
+        WRITE(uo) STEHLE_DATA(s)%max_dens_idx(1:nt), STEHLE_DATA(s)%k_alpha(1:nt)
+        WRITE(uo) STEHLE_DATA(s)%profiles
+      END IF
+    END DO
+
+    DO iw = 1, nw
+      WAVE = waves(iw)
+      FREQ = CLIGHT_NMS / WAVE
+      WAVENO = 1.0D7 / WAVE
+      FREQLG = LOG(FREQ)
+      freq15_l = FREQ / 1.0D15
+      EHVKT = EXP(-FREQ * HKT)
+      STIM = 1.0D0 - EHVKT
+      BNU = BNU_PREFAC * freq15_l**3 * EHVKT / STIM
+      CALL KAPP
+      WRITE(uo) AHYD(1:NRHOX), SHYD(1:NRHOX), AH2P(1:NRHOX), AHMIN(1:NRHOX), SHMIN(1:NRHOX), &
+                SIGH(1:NRHOX), AHE1(1:NRHOX), AHE2(1:NRHOX), AHEMIN(1:NRHOX), SIGHE(1:NRHOX), &
+                AH2COLL(1:NRHOX), AH2MIN(1:NRHOX), ACONT_METAL(1:NRHOX), SIGEL(1:NRHOX), &
+                SIGH2(1:NRHOX), AHLINE(1:NRHOX), SHLINE(1:NRHOX), &
+                ACONT(1:NRHOX), SIGMAC(1:NRHOX), SCONT(1:NRHOX), SLINE(1:NRHOX), &
+                ALINE(1:NRHOX), SIGMAL(1:NRHOX)
+    END DO
+    CLOSE(uo)
+  END SUBROUTINE dump_kapp
+
 END PROGRAM ref_atlas_dump

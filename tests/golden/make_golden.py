@@ -24,6 +24,7 @@ import hashlib
 import json
 import os
 import shutil
+import math
 import subprocess
 import sys
 import tempfile
@@ -364,7 +365,98 @@ def read_atlas_dump(path: str) -> dict:
     return d
 
 
-def atlas_case(atm, nustart, nuhi, linedata, raw=None, tcorr=False):
+KAPP_SRC = ("ahyd", "shyd", "ah2p", "ahmin", "shmin", "sigh", "ahe1", "ahe2", "ahemin", "sighe",
+            "ah2coll", "ah2min", "acont_metal", "sigel", "sigh2", "ahline", "shline",
+            "acont", "sigmac", "scont", "sline", "aline", "sigmal")
+KAPP_CONST = ("hck", "hoverk", "kbol_ev", "theta_coeff", "bnu_prefac", "saha_prefac", "freq_rydh",
+              "ryd_ang", "ln10", "coeff_ff", "sigma_thomson", "clight_angs")
+
+
+def read_kapp_dump(path):
+    """ref_atlas_dump's A12_KAPP_DUMP file: KAPP's depth inputs, the tables
+    its sources hold in module memory after first use, and the per-source
+    arrays at every requested wavelength."""
+    b = open(path, "rb").read()
+    off = 0
+
+    def take(dt, n):
+        nonlocal off
+        a = np.frombuffer(b, dtype=dt, count=n, offset=off)
+        off += a.nbytes
+        return a.copy()
+    n, mion, kw, nw = (int(x) for x in take("<i4", 4))
+    d = dict(nrhox=n, mion=mion)
+    d["const"] = dict(zip(KAPP_CONST, take("<f8", len(KAPP_CONST))))
+    st = {}
+    for k in ("t", "tkev", "tlog", "hkt", "hckt", "xne", "rho", "bmin"):
+        st[k] = take("<f8", n)
+    st["bhyd"] = take("<f8", 6 * n).reshape(6, n)
+    st["equilh2"] = take("<f8", n)
+    st["xnf"] = take("<f8", n * mion).reshape(mion, n)
+    st["xnfp"] = take("<f8", n * mion).reshape(mion, n)
+    d["state"] = st
+    tb = {}
+    tb["q_holtsmark"] = take("<f8", 150)
+    tb["log_beta_min"], tb["log_beta_step"] = take("<f8", 2)
+    nsp = int(take("<i4", 1)[0])
+    sp_idx, sp_nlev, sp_thr = [], [], []
+    lv = {k: [] for k in ("g", "np", "ebind", "eexc", "logemin", "logestride")}
+    ephot, sigma = [], []
+    for _ in range(nsp):
+        sp_idx.append(int(take("<i4", 1)[0]))
+        nl = int(take("<i4", 1)[0])
+        sp_nlev.append(nl)
+        sp_thr.append(float(take("<f8", 1)[0]))
+        for _ in range(nl):
+            g, npt = (int(x) for x in take("<i4", 2))
+            eb, ee, lmin, lstr = take("<f8", 4)
+            lv["g"].append(g)
+            lv["np"].append(npt)
+            lv["ebind"].append(eb)
+            lv["eexc"].append(ee)
+            lv["logemin"].append(lmin)
+            lv["logestride"].append(lstr)
+            ephot.append(take("<f8", npt))
+            sigma.append(take("<f8", npt))
+    tb["tb_xnfp_idx"] = np.array(sp_idx, np.int32)
+    tb["tb_nlev"] = np.array(sp_nlev, np.int32)
+    tb["tb_lowest_thr"] = np.array(sp_thr)
+    tb["tb_g"] = np.array(lv["g"], np.int32)
+    tb["tb_np"] = np.array(lv["np"], np.int32)
+    for k in ("ebind", "eexc", "logemin", "logestride"):
+        tb["tb_" + k] = np.array(lv[k])
+    tb["tb_ephot"] = np.concatenate(ephot)
+    tb["tb_sigma"] = np.concatenate(sigma)
+    nhi = int(take("<i4", 1)[0])
+    tb["hi_edge"] = take("<f8", 7 * nhi).reshape(nhi, 7)
+    tb["felo_nlev"] = take("<i4", 2)
+    tb["felo_xnfp_idx"] = take("<i4", 2)
+    for k in ("nuth", "g", "e"):
+        tb["felo_" + k] = take("<f8", 120).reshape(2, 60)
+    tb["felo_ifancy"] = take("<i4", 120).reshape(2, 60)
+    tb["felo_npts"] = take("<i4", 120).reshape(2, 60)
+    tb["felo_x"] = take("<f8", 200 * 120).reshape(2, 60, 200)
+    tb["felo_logsig"] = take("<f8", 200 * 120).reshape(2, 60, 200)
+    tb["felo_seaton"] = take("<f8", 4 * 120).reshape(2, 60, 4)
+    for s in range(4):
+        loaded, nlo, numin, numax, ndens = (int(x) for x in take("<i4", 5))
+        tb[f"st{s}_hdr"] = np.array([loaded, nlo, numin, numax, ndens], np.int32)
+        tb[f"st{s}_dens"] = take("<f8", 20)
+        tb[f"st{s}_temp"] = take("<f8", 10)
+        tb[f"st{s}_ldalpha"] = take("<f8", 60)
+        if loaded:
+            nt = numax - numin + 1
+            tb[f"st{s}_maxdens"] = take("<i4", nt)
+            tb[f"st{s}_kalpha"] = take("<f8", nt)
+            tb[f"st{s}_prof"] = take("<f8", 60 * 10 * ndens * nt).reshape(nt, ndens, 10, 60)
+    d["tables"] = tb
+    src = take("<f8", nw * len(KAPP_SRC) * n).reshape(nw, len(KAPP_SRC), n)
+    d["src"] = {k: np.ascontiguousarray(src[:, i, :]) for i, k in enumerate(KAPP_SRC)}
+    assert off == len(b), (off, len(b))
+    return d
+
+
+def atlas_case(atm, nustart, nuhi, linedata, raw=None, tcorr=False, kapp_waves=None):
     """Runs ref_atlas_dump.  raw = (predicted records, TiO MOL2 records):
     the store is then SELECTLINES' selection from these lists, returned
     as d["selected"] with d["wavetab"]."""
@@ -394,9 +486,17 @@ def atlas_case(atm, nustart, nuhi, linedata, raw=None, tcorr=False):
     tco = os.path.join(tmp, "tcorr.bin")
     if tcorr:
         env["A12_TCORR_OUT"] = tco
+    if kapp_waves is not None:
+        kw_path = os.path.join(tmp, "kapp_waves.bin")
+        with open(kw_path, "wb") as fh:
+            fh.write(np.int32(len(kapp_waves)).tobytes() + np.asarray(kapp_waves, "<f8").tobytes())
+        env["A12_KAPP_WAVES"] = kw_path
+        env["A12_KAPP_DUMP"] = os.path.join(tmp, "kapp.bin")
     run([os.path.join(REFBIN, "ref_atlas_dump"), deck, str(nustart), str(nuhi), lp, out, *extra],
         tmp, env)
     d = read_atlas_dump(out)
+    if kapp_waves is not None:
+        d["kapp"] = read_kapp_dump(env["A12_KAPP_DUMP"])
     if tcorr:
         d["tcorr_in"] = read_tcorr_in(tco)
         d["tcorr_iter"] = read_iter_table(tco + ".iter", d["tcorr_in"]["n"])
@@ -502,6 +602,140 @@ def golden_tcorr_conv():
     return {"tcorr_conv": {"selected_lines": int(d["nlines"]),
                            "convective_layers": int(np.sum(ti["flxcnv"] > 0)),
                            "max_conv_tot": float(tab["hratio"].max())}}
+
+
+class _Records:
+    """Fortran list-directed reading over the records (lines) of a text
+    file: read(k) takes k values from as many records as needed and
+    drops the rest of the last one, as READ(u, *) does."""
+
+    def __init__(self, path, skip=0):
+        self.lines = open(path).read().splitlines()[skip:]
+        self.i = 0
+
+    def line(self):
+        s = self.lines[self.i]
+        self.i += 1
+        return s
+
+    def read(self, k, conv=float):
+        out = []
+        while len(out) < k:
+            out += [conv(t.replace("D", "E")) for t in self.line().split()]
+        return out[:k]
+
+
+def read_kapp_files(datadir):
+    """The KAPP tables the reference keeps in SAVE variables of its source
+    functions, read from the data files as those functions do: the Karzas &
+    Latter tables (REAL(4)), the van Hoof free-free Gaunt table (stored as
+    log10 g), the CIA tables and the CH/OH cross sections."""
+    t = {}
+    r = _Records(os.path.join(datadir, "karzas_ekarzas.dat"), 2)
+    t["k_ekarzas"] = np.array(r.read(29), np.float32)
+    r = _Records(os.path.join(datadir, "karzas_xn.dat"), 3)
+    t["k_xn"] = np.array([r.read(1)[0] for _ in range(15 * 29)], np.float32).reshape(15, 29)
+    r = _Records(os.path.join(datadir, "karzas_freqn.dat"), 2)
+    t["k_freqn"] = np.array([r.read(1)[0] for _ in range(15 * 29)], np.float32).reshape(15, 29)
+    r = _Records(os.path.join(datadir, "karzas_xl.dat"), 3)
+    t["k_xl"] = np.array([r.read(1)[0] for _ in range(6 * 6 * 29)], np.float32).reshape(6, 6, 29)
+    rows = []
+    for line in open(os.path.join(datadir, "gauntff_vanhoof.dat")):
+        line = line.split("#")[0].strip()
+        if line:
+            rows.append(line)
+    assert int(rows[0]) == 20140210 and rows[1].split()[:2] == ["81", "146"]
+    g = np.array([[float(v) for v in rows[5 + j].split()] for j in range(146)])
+    t["gff_logtab"] = np.array([[math.log10(v) for v in row] for row in g])  # (146 u, 81 gam2)
+    r = _Records(os.path.join(datadir, "h2collop.dat"))
+    line = r.line()
+    while line.startswith("#"):
+        line = r.line()
+    npair, ntmax = (int(v) for v in line.split()[:2])
+    ncianu, nu0, dnu = r.read(3)
+    ncianu = int(ncianu)
+    nciat = np.zeros(npair, np.int32)
+    temps = np.zeros((npair, ntmax))
+    tab = np.full((npair, ntmax, ncianu), -99.0)
+    for ip in range(npair):
+        nciat[ip] = int(r.line().split()[1])
+        temps[ip, :nciat[ip]] = r.read(nciat[ip])
+        for it in range(nciat[ip]):
+            tab[ip, it] = r.read(ncianu)
+    t.update(cia_nciat=nciat, cia_temp=temps, cia_tab=tab, cia_nu=np.array([nu0, dnu]))
+    for name, nb in (("ch", 105), ("oh", 130)):
+        r = _Records(os.path.join(datadir, f"cross{name}.dat"), 5)
+        t[f"cross_{name}"] = np.array([r.read(15) for _ in range(nb)])  # (energy bin, T)
+    return t
+
+
+def kapp_waves():
+    """Wavelengths (nm) of the per-source KAPP fixture: 240 log-spaced
+    points from 8 nm to 300 um, and clusters across the edges and lines the
+    sources switch on: the H I series limits n = 1..15, the He I and He II
+    edges, the H- threshold, the bands of the H Rayleigh fit, the H I lines
+    HLINOP covers near Ly-a, H-a/b/g, Pa-a/b and Br-a."""
+    w = list(np.geomspace(8.0, 300000.0, 240))
+    limits = [109678.764, 27419.659, 12186.462, 6854.871, 4387.113, 3046.604, 2238.320,
+              1713.713, 1354.044, 1096.776, 906.426, 761.650, 648.980, 559.579, 487.456]
+    edges = [1e7 / x for x in limits] + [22.79, 50.43, 260.0, 312.0, 342.2, 368.0, 1643.9]
+    nu_l = 109677.583403 * 2.99792458e10
+    for f in (0.01, 0.74, 0.77, 0.885, 0.890, 0.936, 0.938, 0.959, 0.961, 1.0):
+        edges.append(2.99792458e17 / (f * nu_l))
+    lines = [121.567, 102.572, 656.28, 486.13, 434.05, 1875.1, 1281.8, 4051.2, 2625.1]
+    for e in edges:
+        w += list(e * (1.0 + np.array([-1e-5, 1e-5])))
+    for c in lines:
+        w += list(c * (1.0 + np.array([-2e-3, -2e-4, 0.0, 1e-4, 1e-3])))
+    return np.array(sorted(w))
+
+
+KAPP_DECKS = {"plain": dict(), "hot": dict(t_top=6000.0, t_bot=16000.0),
+              "cool": dict(t_top=2600.0, t_bot=6000.0)}
+
+
+KAPP_DEPTHS = np.r_[0:72:3, 71]
+KAPP_XNFP = np.r_[1:6, 6, 7, 10, 11, 15, 16, 21, 22, 28, 29, 36, 37, 45, 46, 55, 56, 66, 67, 78, 79,
+                  91, 92, 105, 106, 136, 137, 171, 172, 210, 211, 351, 352, 846, 848]
+KAPP_XNF = np.r_[1, 3, 4, 22:27, 29:34, 37:42, 56:61, 79:84, 106:111, 137:142, 352:357]
+
+
+def golden_kapp():
+    """KAPP (src/atlas12_modules.f90:11202-11324) and every source it calls,
+    at 360 wavelengths, on three decks: the plain-SYNTHE atmosphere
+    (4,400-9,500 K), a hot one (6,000-16,000 K: He I/II, H I n = 1) and a
+    cool one (2,600-6,000 K: H2, CIA, H2-, CH/OH).  Writes the tables the
+    sources hold (module tables from the reference's own memory after first
+    use, SAVE tables read from the data files as the sources read them) to
+    kapp_tables.npz, and per deck the depth inputs and per-source outputs to
+    kapp_ref.npz."""
+    waves = kapp_waves()
+    out = {"waves": waves, "depths": KAPP_DEPTHS.astype(np.int32)}
+    tables = None
+    for name, kwargs in KAPP_DECKS.items():
+        atm = wl.scurve_atmosphere(**kwargs)
+        d = atlas_case(atm, 9599, 20, np.zeros((0, 4), np.int32), kapp_waves=waves)
+        k = d["kapp"]
+        if tables is None:
+            tables = dict(k["tables"])
+            for c, v in k["const"].items():
+                tables["const_" + c] = np.float64(v)
+        hi_ids = np.unique(k["tables"]["hi_edge"][:, 6].astype(int))
+        xnfp_ids = np.unique(np.concatenate([KAPP_XNFP, hi_ids]))
+        out["xnfp_ids"], out["xnf_ids"] = xnfp_ids.astype(np.int32), KAPP_XNF.astype(np.int32)
+        for sk, sv in k["state"].items():
+            if sk == "xnfp":
+                sv = sv[xnfp_ids - 1]
+            elif sk == "xnf":
+                sv = sv[KAPP_XNF - 1]
+            out[f"{name}_{sk}"] = sv
+        for sk, sv in k["src"].items():
+            out[f"{name}_{sk}"] = sv[:, KAPP_DEPTHS]
+    tables.update(read_kapp_files("/root/reference/data"))
+    np.savez_compressed(os.path.join(HERE, "kapp_tables.npz"), **tables)
+    np.savez_compressed(os.path.join(HERE, "kapp_ref.npz"), **out)
+    return {"kapp": {"waves": len(waves), "decks": list(KAPP_DECKS),
+                     "tb_levels": int(len(tables["tb_g"])), "hi_edges": int(len(tables["hi_edge"]))}}
 
 
 def golden_atlas_small():
@@ -705,7 +939,8 @@ STEPS = {"synthe_small": golden_synthe_small, "atlas_small": golden_atlas_small,
          "state_atlas": golden_state_atlas, "select": golden_select, "scale": golden_scale,
          "tcorr": golden_tcorr, "tcorr_conv": golden_tcorr_conv,
          "state_plain": golden_state_plain,
-         "numerics": golden_numerics, "josh": golden_josh, "molec": golden_molec}
+         "numerics": golden_numerics, "josh": golden_josh, "molec": golden_molec,
+         "kapp": golden_kapp}
 
 
 def main():
