@@ -41,6 +41,9 @@ struct KappDev {
     uint64_t fq_gen = 0;
     int fq_numnu = 0;
     DevBuf ew, eout, erow;       // a12_kapp_eval scratch
+    DevBuf tbsum;                // MBF_TOPBASE's sum per cell
+    DevBuf cbuf;                 // the 17 per-source values per cell between the kernels
+    DevBuf tbs, etbs;            // TOPbase cross sections (nlev, nw) of the grid / of an eval
     std::vector<double> stage;
 };
 
@@ -73,23 +76,124 @@ __global__ void k_kapp_tb(KTab T, KState S, double *__restrict__ planes)
     planes[(size_t)(KS_TB + l) * S.n + j] = kp_state_tb(T, S, l, j);
 }
 
-// mode 0: out = six tables (nrhox, nw) in the order ACONT SIGMAC SCONT SLINE
-// ALINE SIGMAL; mode 1: out (nw, 23, nrhox)
-__global__ void __launch_bounds__(256) k_kapp_cells(KTab T, KState S, const double *__restrict__ planes,
-                                                    const double *__restrict__ rows,
-                                                    const double *__restrict__ waves, int nw, int mode,
-                                                    double *__restrict__ out)
+// SIGMA_NU of every TOPbase level at every wavelength: sig (nlev, nw)
+__global__ void k_kapp_tbsig(KTab T, int nw, const double *__restrict__ waves, double *__restrict__ sig)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.y;
+    if (w >= nw) return;
+    sig[(size_t)l * nw + w] = kp_tb_sigma(T, l, waves[w]);
+}
+
+// MBF_TOPBASE's sum, the first part of ACONT_METAL: one lane per wavelength,
+// a workgroup per (64 wavelengths, 8 depths), so a level's weights for the
+// 8 depths are the same for every lane; each depth's sum runs over the
+// levels in the reference's order (kp_cell_tb).
+constexpr int TB_DEPTHS = 8;
+__global__ void __launch_bounds__(64) k_kapp_tbsum(KTab T, KState S, const double *__restrict__ planes,
+                                                   const double *__restrict__ sig,
+                                                   const double *__restrict__ waves, int nw,
+                                                   double *__restrict__ tbsum)
+{
+    const int w = blockIdx.x * 64 + threadIdx.x;
+    const int n = S.n, j0 = blockIdx.y * TB_DEPTHS;
+    if (w >= nw) return;
+    const double freq = CLIGHT_NMS / waves[w];
+    double sor[TB_DEPTHS], acc[TB_DEPTHS];
+#pragma unroll
+    for (int d = 0; d < TB_DEPTHS; ++d) {
+        const int j = j0 + d < n ? j0 + d : n - 1;
+        sor[d] = (1.0 - exp(-freq * S.hkt[j])) / S.rho[j];
+        acc[d] = 0.0;
+    }
+    const double *W = planes + (size_t)KS_TB * n + j0;
+    const int nd = n - j0 < TB_DEPTHS ? n - j0 : TB_DEPTHS;
+    for (int l = 0; l < T.tb_nlev_tot; ++l) {
+        const double sg = sig[(size_t)l * nw + w];
+        if (sg == 0.0) continue;
+        const double *wl = W + (size_t)l * n;
+        if (nd == TB_DEPTHS) {
+#pragma unroll
+            for (int d = 0; d < TB_DEPTHS; ++d) acc[d] = acc[d] + wl[d] * sg * sor[d];
+        } else {
+            for (int d = 0; d < nd; ++d) acc[d] = acc[d] + wl[d] * sg * sor[d];
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < TB_DEPTHS; ++d)
+        if (d < nd) tbsum[(size_t)w * n + j0 + d] = acc[d];
+}
+
+// The three parts of the rest of KAPP, one thread per cell; o (17, cells)
+// holds the per-source values between them.
+__global__ void __launch_bounds__(256) k_kapp_ca(KTab T, KState S, const double *__restrict__ planes,
+                                                 const double *__restrict__ rows, const double *__restrict__ waves,
+                                                 int nw, double *__restrict__ o)
 {
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int n = S.n;
-    if (idx >= (long)nw * n) return;
+    const long cells = (long)nw * n;
+    if (idx >= cells) return;
     const int w = (int)(idx / n), j = (int)(idx - (long)w * n);
-    double cell[A12_KAPP_NSRC];
-    kp_cell(T, S, planes, rows + (size_t)w * KF_N, waves[w], j, cell);
+    double v[17];
+    kp_cell_a(T, S, planes, rows + (size_t)w * KF_N, waves[w], j, v);
+    const int ks[8] = {0, 1, 2, 3, 4, 5, 10, 11};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) o[ks[q] * cells + idx] = v[ks[q]];
+}
+
+__global__ void __launch_bounds__(256) k_kapp_cb(KTab T, KState S, const double *__restrict__ planes,
+                                                 const double *__restrict__ rows, const double *__restrict__ waves,
+                                                 int nw, const double *__restrict__ tbsum, double *__restrict__ o)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = S.n;
+    const long cells = (long)nw * n;
+    if (idx >= cells) return;
+    const int w = (int)(idx / n), j = (int)(idx - (long)w * n);
+    double v[17];
+    kp_cell_b(T, S, planes, rows + (size_t)w * KF_N, waves[w], j, tbsum[idx], v);
+#pragma unroll
+    for (int q = 6; q <= 14; ++q)
+        if (q <= 9 || q >= 12) o[q * cells + idx] = v[q];
+}
+
+__global__ void __launch_bounds__(256) k_kapp_cc(KTab T, KState S, const double *__restrict__ planes,
+                                                 const double *__restrict__ rows, const double *__restrict__ waves,
+                                                 int nw, double *__restrict__ o)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = S.n;
+    const long cells = (long)nw * n;
+    if (idx >= cells) return;
+    const int w = (int)(idx / n), j = (int)(idx - (long)w * n);
+    double v[17];
+    v[15] = v[16] = 0.0;
+    if (rows[(size_t)w * KF_N + KF_HLN] > 0.0) kp_cell_c(T, S, planes, rows + (size_t)w * KF_N, waves[w], j, v);
+    o[15 * cells + idx] = v[15];
+    o[16 * cells + idx] = v[16];
+}
+
+// KAPP's totals.  mode 0: out = six tables (nrhox, nw) in the order ACONT
+// SIGMAC SCONT SLINE ALINE SIGMAL; mode 1: out (nw, 23, nrhox)
+__global__ void __launch_bounds__(256) k_kapp_tot(KState S, const double *__restrict__ waves, int nw,
+                                                  const double *__restrict__ o, int mode,
+                                                  double *__restrict__ out)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = S.n;
+    const long cells = (long)nw * n;
+    if (idx >= cells) return;
+    const int w = (int)(idx / n), j = (int)(idx - (long)w * n);
+    double v[17], cell[A12_KAPP_NSRC];
+#pragma unroll
+    for (int q = 0; q < 17; ++q) v[q] = o[q * cells + idx];
+    kp_totals(v, waves[w], S.hkt[j], cell);
     if (mode == 0) {
-        const size_t plane = (size_t)nw * n;
-        for (int k = 0; k < 6; ++k) out[k * plane + (size_t)w * n + j] = cell[17 + k];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) out[k * cells + idx] = cell[17 + k];
     } else {
+#pragma unroll
         for (int k = 0; k < A12_KAPP_NSRC; ++k) out[((size_t)w * A12_KAPP_NSRC + k) * n + j] = cell[k];
     }
 }
@@ -107,13 +211,37 @@ int launch_rows(a12_ctx *ctx, KappDev &K, int nw, const double *d_waves, double 
     return A12_OK;
 }
 
-int launch_cells(a12_ctx *ctx, KappDev &K, int nw, const double *d_waves, const double *d_rows, int mode,
-                 double *d_out)
+int launch_tbsig(a12_ctx *ctx, KappDev &K, int nw, const double *d_waves, double *d_sig)
+{
+    const KTab T = dev_tab(K);
+    if (T.tb_nlev_tot < 1) return A12_OK;
+    k_kapp_tbsig<<<dim3((nw + 255) / 256, T.tb_nlev_tot), 256, 0, ctx->stream>>>(T, nw, d_waves, d_sig);
+    A12_HIP(hipGetLastError());
+    return A12_OK;
+}
+
+// d_sig: the TOPbase cross sections of these wavelengths (launch_tbsig)
+int launch_cells(a12_ctx *ctx, KappDev &K, int nw, const double *d_waves, const double *d_rows,
+                 const double *d_sig, int mode, double *d_out)
 {
     const long cells = (long)nw * K.n;
     const KState S = kapp_state_view(K.st.as<double>(), K.n, K.mion);
-    k_kapp_cells<<<(unsigned)((cells + 255) / 256), 256, 0, ctx->stream>>>(
-        dev_tab(K), S, K.planes.as<double>(), d_rows, d_waves, nw, mode, d_out);
+    A12_HIP(K.tbsum.reserve(sizeof(double) * cells));
+    const unsigned nb = (unsigned)((cells + 255) / 256);
+    k_kapp_tbsum<<<dim3((nw + 63) / 64, (K.n + TB_DEPTHS - 1) / TB_DEPTHS), 64, 0, ctx->stream>>>(
+        dev_tab(K), S, K.planes.as<double>(), d_sig, d_waves, nw, K.tbsum.as<double>());
+    A12_HIP(hipGetLastError());
+    A12_HIP(K.cbuf.reserve(sizeof(double) * 17 * cells));
+    const KTab T = dev_tab(K);
+    double *o = K.cbuf.as<double>();
+    k_kapp_ca<<<nb, 256, 0, ctx->stream>>>(T, S, K.planes.as<double>(), d_rows, d_waves, nw, o);
+    A12_HIP(hipGetLastError());
+    k_kapp_cb<<<nb, 256, 0, ctx->stream>>>(T, S, K.planes.as<double>(), d_rows, d_waves, nw,
+                                           K.tbsum.as<double>(), o);
+    A12_HIP(hipGetLastError());
+    k_kapp_cc<<<nb, 256, 0, ctx->stream>>>(T, S, K.planes.as<double>(), d_rows, d_waves, nw, o);
+    A12_HIP(hipGetLastError());
+    k_kapp_tot<<<nb, 256, 0, ctx->stream>>>(S, d_waves, nw, o, mode, d_out);
     A12_HIP(hipGetLastError());
     return A12_OK;
 }
@@ -124,7 +252,7 @@ void a12i_kapp_free(a12_ctx *ctx)
 {
     if (!ctx->kapp) return;
     KappDev &K = *ctx->kapp;
-    DevBuf *b[] = {&K.td, &K.ti, &K.tf, &K.st, &K.planes, &K.fq, &K.ew, &K.eout, &K.erow};
+    DevBuf *b[] = {&K.td, &K.ti, &K.tf, &K.st, &K.planes, &K.fq, &K.ew, &K.eout, &K.erow, &K.tbsum, &K.tbs, &K.etbs, &K.cbuf};
     for (DevBuf *p : b) p->release();
     delete ctx->kapp;
     ctx->kapp = nullptr;
@@ -236,7 +364,11 @@ int a12_kapp_eval(a12_ctx *ctx, int32_t nwave, const double *wave_nm, double *ou
     A12_HIP(hipMemcpyAsync(K.ew.p, wave_nm, sizeof(double) * nwave, hipMemcpyHostToDevice, ctx->stream));
     int rc = launch_rows(ctx, K, nwave, K.ew.as<double>(), K.erow.as<double>());
     if (rc) return rc;
-    rc = launch_cells(ctx, K, nwave, K.ew.as<double>(), K.erow.as<double>(), 1, K.eout.as<double>());
+    A12_HIP(K.etbs.reserve(sizeof(double) * ((size_t)K.pack.scal.tb_nlev_tot * nwave + 1)));
+    rc = launch_tbsig(ctx, K, nwave, K.ew.as<double>(), K.etbs.as<double>());
+    if (rc) return rc;
+    rc = launch_cells(ctx, K, nwave, K.ew.as<double>(), K.erow.as<double>(), K.etbs.as<double>(), 1,
+                      K.eout.as<double>());
     if (rc) return rc;
     A12_HIP(hipMemcpyAsync(out, K.eout.p, sizeof(double) * nout, hipMemcpyDeviceToHost, ctx->stream));
     A12_HIP(hipStreamSynchronize(ctx->stream));
@@ -266,13 +398,17 @@ int a12_atlas_kapp_run(a12_ctx *ctx)
         A12_HIP(K.fq.reserve(sizeof(double) * (size_t)A.numnu * KF_N));
         rc = launch_rows(ctx, K, A.numnu, A.waveset.as<double>(), K.fq.as<double>());
         if (rc) return rc;
+        A12_HIP(K.tbs.reserve(sizeof(double) * ((size_t)K.pack.scal.tb_nlev_tot * A.numnu + 1)));
+        rc = launch_tbsig(ctx, K, A.numnu, A.waveset.as<double>(), K.tbs.as<double>());
+        if (rc) return rc;
         K.fq_gen = A.grid_gen;
         K.fq_numnu = A.numnu;
     }
     const size_t plane = (size_t)A.n * A.numnu;
     A12_HIP(A.kapp.reserve(sizeof(double) * 6 * plane));
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[14], ctx->stream));
-    rc = launch_cells(ctx, K, A.numnu, A.waveset.as<double>(), K.fq.as<double>(), 0, A.kapp.as<double>());
+    rc = launch_cells(ctx, K, A.numnu, A.waveset.as<double>(), K.fq.as<double>(), K.tbs.as<double>(), 0,
+                      A.kapp.as<double>());
     if (rc) return rc;
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[15], ctx->stream));
     A12_HIP(hipStreamSynchronize(ctx->stream));

@@ -88,7 +88,8 @@ enum {
     KF_FELO = 183,      // FELO_SIGMA(ion, i, WAVENO) at KF_FELO + 60 (ion - 1) + i - 1
     KF_HLN = 303,       // HLINOP lower level N (0: no line opacity at this frequency)
     KF_HLMF = 304,      // HLINOP MFREQ
-    KF_N = 305
+    KF_HI = 305,        // MBF_HIGH_ION XSECT of edge e at KF_HI + e (0 below the edge), 60 edges
+    KF_N = 365
 };
 
 // ---- per-depth planes (kp_state), each n values ----------------------------
@@ -112,7 +113,8 @@ enum {
     KS_HLBOLT = KS_HE2E + 9,      // HLINOP BOLT(J, 1..4) at +n-1 (4)
     KS_F0 = KS_HLBOLT + 4,        // STARK F0
     KS_FELOE,                     // FELO EXP(-FELO_E HCKT): 2 x 60 planes
-    KS_TB = KS_FELOE + 120,       // TB_BOLTZ(level, J): one plane per TOPbase level
+    KS_HIEXP = KS_FELOE + 120,    // MBF_HIGH_ION exp(E_exc / TKEV) per edge (60)
+    KS_TB = KS_HIEXP + 60,        // XNFP(J, species) * TB_BOLTZ(level, J), one plane per TOPbase level
 };
 
 // The tables of a12_kapp_tables on one side (host or device), plus the
@@ -132,6 +134,7 @@ struct KTab {
     const int *felo_ifancy, *felo_npts;
     int tb_nsp, tb_nlev_tot;
     const int *tb_idx, *tb_nlev, *tb_g, *tb_np, *tb_off;   // tb_off: first point of each level
+    const int *tb_sp;                                        // species of each level
     const double *tb_thr, *tb_ebind, *tb_eexc, *tb_logemin, *tb_logestride, *tb_ephot, *tb_sigma;
     int st_loaded[4], st_numin[4], st_numax[4], st_ndens[4];
     const double *st_dens[4], *st_temp[4], *st_ldalpha[4], *st_kalpha[4], *st_prof[4];
@@ -734,6 +737,13 @@ A12_HD inline void kp_freq(const KTab &T, double wave, double *row)
     // FELO_SIGMA of every Fe I / Fe II level (:14725)
     for (int ion = 0; ion < 2; ++ion)
         for (int i = 0; i < T.felo_nlev[ion]; ++i) row[KF_FELO + 60 * ion + i] = felo_sigma(T, ion, i, waveno);
+    // MBF_HIGH_ION's edge cross sections (:14374-14379)
+    for (int e = 0; e < T.hi_nedge; ++e) {
+        const double *h = T.hi_edge + 7 * e;
+        if (freq < h[0]) continue;
+        const double fr = h[0] / freq;
+        row[KF_HI + e] = h[1] * (h[2] + fr - h[2] * fr) * sqrt(powi_rt(fr, (int)h[3]));
+    }
     // HLINOP (:17420-17439): the lower level and the nearest upper level
     {
         const double NU_LYMAN = FREQ_RYDH;
@@ -818,20 +828,69 @@ A12_HD inline void kp_state(const KTab &T, const KState &S, const double *qh, in
     for (int ion = 0; ion < 2; ++ion)
         for (int i = 0; i < 60; ++i)
             PL(KS_FELOE + ion * 60 + i) = i < T.felo_nlev[ion] ? exp(-T.felo_e[ion * 60 + i] * hckt) : 0.0;
+    // MBF_HIGH_ION's Boltzmann divisors (:14383)
+    for (int e = 0; e < 60; ++e) PL(KS_HIEXP + e) = e < T.hi_nedge ? exp(T.hi_edge[7 * e + 5] / tkev) : 1.0;
 #undef PL
 }
 
-// TB_BOLTZ(level, J) of MBF_TOPBASE (:14249-14262)
+// XNFP(J, species) * TB_BOLTZ(level, J) of MBF_TOPBASE (:14249-14262, the
+// first product of its sum at :14317)
 A12_HD inline double kp_state_tb(const KTab &T, const KState &S, int level, int j)
 {
     const double kt_ry = S.t[j] * 8.617333e-5 / 13.6056923;
-    return (double)T.tb_g[level] * exp(-T.tb_eexc[level] / kt_ry);
+    const double boltz = (double)T.tb_g[level] * exp(-T.tb_eexc[level] / kt_ry);
+    return xnfp(S, T.tb_idx[T.tb_sp[level]], j) * boltz;
 }
 
-// ---- one (frequency, depth) cell ----------------------------------------------
-// out: the 23 arrays of a12_kapp_eval's order at this cell.
-A12_HD inline void kp_cell(const KTab &T, const KState &S, const double *P, const double *row,
-                           double wave, int j, double *out)
+// SIGMA_NU of one TOPbase level at one wavelength (:14280-14310); 0 where
+// MBF_TOPBASE skips the level (below its species' or its own threshold,
+// or before its first tabulated energy).
+A12_HD inline double kp_tb_sigma(const KTab &T, int lvl, double wave)
+{
+    const double freq = CLIGHT_NMS / wave;
+    const double eph = freq / 3.289842e15;
+    if (eph < T.tb_thr[T.tb_sp[lvl]]) return 0.0;
+    if (eph < fabs(T.tb_ebind[lvl])) return 0.0;
+    const double log_e = log10(eph);
+    const double xi = (log_e - T.tb_logemin[lvl]) / T.tb_logestride[lvl];
+    const int k = (int)xi + 1;
+    if (k < 1) return 0.0;
+    const int np = T.tb_np[lvl];
+    const double *sg = T.tb_sigma + T.tb_off[lvl];
+    double sig;
+    if (k >= np) {
+        const double r = T.tb_ephot[T.tb_off[lvl] + np - 1] / eph;
+        sig = sg[np - 1] * ((r * r) * r);
+    } else {
+        sig = sg[k - 1] + (sg[k] - sg[k - 1]) * (xi - (double)(k - 1));
+    }
+    return sig * 1.0e-18;
+}
+
+// MBF_TOPBASE's sum at one cell over the levels in the reference's order,
+// sigma(level) = sig(level * nw + w): the first part of ACONT_METAL.
+A12_HD inline double kp_cell_tb(const KTab &T, const KState &S, const double *P, const double *sig, int nw,
+                                int w, double wave, int j)
+{
+    const int n = S.n;
+    const double freq = CLIGHT_NMS / wave;
+    const double stim = 1.0 - exp(-freq * S.hkt[j]);
+    const double stim_over_rho = stim / S.rho[j];
+    double acm = 0.0;
+    for (int l = 0; l < T.tb_nlev_tot; ++l) {
+        const double sg = sig[(size_t)l * nw + w];
+        if (sg == 0.0) continue;
+        acm = acm + P[(size_t)(KS_TB + l) * n + j] * sg * stim_over_rho;
+    }
+    return acm;
+}
+
+// The rest of KAPP at one cell, in three parts that keep fewer values live
+// (each a kernel of its own on the device), and KAPP's totals.  o[] holds
+// the 17 per-source values in a12_kapp_eval's order.
+// Part A: HOP, H2PLOP, HMINOP, H2MINOP, H2COLLOP, HRAYOP.
+A12_HD inline void kp_cell_a(const KTab &T, const KState &S, const double *P, const double *row,
+                             double wave, int j, double *o)
 {
     const int n = S.n;
 #define PL(k) P[(size_t)(k) * n + j]
@@ -849,6 +908,8 @@ A12_HD inline void kp_cell(const KTab &T, const KState &S, const double *P, cons
     const double b1 = bhyd(S, 1, j);
     const double g1 = coulff(T, tlog, freqlg, 1);
 
+    (void)waveno; (void)freqlg; (void)t; (void)tkev; (void)tlog; (void)hckt; (void)hkt; (void)xne;
+    (void)rho; (void)x1; (void)x2; (void)x3; (void)b1; (void)g1; (void)bnu; (void)stim; (void)ehvkt;
     // HOP (:11384-11604)
     double ahyd, shyd = 0.0;
     {
@@ -955,6 +1016,34 @@ A12_HD inline void kp_cell(const KTab &T, const KState &S, const double *P, cons
     }
     // HRAYOP (:12899)
     const double sigh = row[KF_HRAY] * x1 * 2.0 * b1 / rho;
+    o[0] = ahyd; o[1] = shyd; o[2] = ah2p; o[3] = ahmin; o[4] = shmin; o[5] = sigh;
+    o[10] = ah2coll; o[11] = ah2min;
+#undef PL
+}
+
+// Part B: HE1OP, HE2OP, HEMIOP, HERAOP, the rest of the metal continuum
+// (acm_tb is MBF_TOPBASE's sum), ELECOP, H2RAOP.
+A12_HD inline void kp_cell_b(const KTab &T, const KState &S, const double *P, const double *row,
+                             double wave, int j, double acm_tb, double *o)
+{
+    const int n = S.n;
+#define PL(k) P[(size_t)(k) * n + j]
+    // RUN_RT_PASS's frequency set-up (:3905-3915)
+    const double freq = CLIGHT_NMS / wave;
+    const double waveno = 1.0e7 / wave;
+    const double freqlg = log(freq);
+    const double freq15 = freq / 1.0e15;
+    const double ehvkt = exp(-freq * S.hkt[j]);
+    const double stim = 1.0 - ehvkt;
+    const double bnu = BNU_PREFAC * ((freq15 * freq15) * freq15) * ehvkt / stim;
+    const double t = S.t[j], tkev = S.tkev[j], tlog = S.tlog[j], hckt = S.hckt[j], hkt = S.hkt[j];
+    const double xne = S.xne[j], rho = S.rho[j];
+    const double x1 = xnfp(S, 1, j), x2 = xnfp(S, 2, j), x3 = xnfp(S, 3, j);
+    const double b1 = bhyd(S, 1, j);
+    const double g1 = coulff(T, tlog, freqlg, 1);
+
+    (void)waveno; (void)freqlg; (void)t; (void)tkev; (void)tlog; (void)hckt; (void)hkt; (void)xne;
+    (void)rho; (void)x1; (void)x2; (void)x3; (void)b1; (void)g1; (void)bnu; (void)stim; (void)ehvkt;
     // HE1OP (:13114-13139)
     double ahe1;
     {
@@ -1031,34 +1120,8 @@ A12_HD inline void kp_cell(const KTab &T, const KState &S, const double *P, cons
     // HERAOP (:13939)
     const double sighe = row[KF_HERA] * x3 / rho;
     // CONT_METAL_OPACITY_TOPBASE (:14160-14194)
-    double acm = 0.0;
+    double acm = acm_tb;
     {
-        const double stim_over_rho = stim / rho;
-        // MBF_TOPBASE (:14226-14326)
-        const double eph = freq / 3.289842e15;
-        const double log_e = log10(eph);
-        int lvl = 0;
-        for (int is = 0; is < T.tb_nsp; ++is) {
-            const int nls = T.tb_nlev[is];
-            if (eph < T.tb_thr[is]) { lvl += nls; continue; }
-            const double xs = xnfp(S, T.tb_idx[is], j);
-            for (int il = 0; il < nls; ++il, ++lvl) {
-                if (eph < fabs(T.tb_ebind[lvl])) continue;
-                const double xi = (log_e - T.tb_logemin[lvl]) / T.tb_logestride[lvl];
-                const int k = (int)xi + 1;
-                if (k < 1) continue;
-                const int np = T.tb_np[lvl];
-                const double *sg = T.tb_sigma + T.tb_off[lvl];
-                double sig;
-                if (k >= np) {
-                    const double r = T.tb_ephot[T.tb_off[lvl] + np - 1] / eph;
-                    sig = sg[np - 1] * ((r * r) * r);
-                } else {
-                    sig = sg[k - 1] + (sg[k] - sg[k - 1]) * (xi - (double)(k - 1));
-                }
-                acm = acm + xs * PL(KS_TB + lvl) * (sig * 1.0e-18) * stim_over_rho;
-            }
-        }
         // MBF_HIGH_ION (:14338-14391)
         {
             double xnfs[5] = {0, 0, 0, 0, 0};
@@ -1076,10 +1139,8 @@ A12_HD inline void kp_cell(const KTab &T, const KState &S, const double *P, cons
             for (int e = 0; e < T.hi_nedge; ++e) {
                 const double *h = T.hi_edge + 7 * e;
                 if (freq < h[0]) continue;
-                const double fr = h[0] / freq;
-                const double xsect = h[1] * (h[2] + fr - h[2] * fr) * sqrt(powi_rt(fr, (int)h[3]));
-                const double xx = xsect * xnfp(S, (int)h[6], j) * h[4];
-                if (xx > ahot / 100.0) ahot = ahot + xx / exp(h[5] / tkev);
+                const double xx = row[KF_HI + e] * xnfp(S, (int)h[6], j) * h[4];
+                if (xx > ahot / 100.0) ahot = ahot + xx / PL(KS_HIEXP + e);
             }
             acm = acm + ahot * stim / rho;
         }
@@ -1125,6 +1186,33 @@ A12_HD inline void kp_cell(const KTab &T, const KState &S, const double *P, cons
     // ELECOP (:17220), H2RAOP (:17336)
     const double sigel = SIGMA_THOMSON * xne / rho;
     const double sigh2 = row[KF_H2RA] * PL(KS_XNH2) / rho;
+    o[6] = ahe1; o[7] = ahe2; o[8] = ahemin; o[9] = sighe;
+    o[12] = acm; o[13] = sigel; o[14] = sigh2;
+#undef PL
+}
+
+// Part C: HLINOP.
+A12_HD inline void kp_cell_c(const KTab &T, const KState &S, const double *P, const double *row,
+                             double wave, int j, double *o)
+{
+    const int n = S.n;
+#define PL(k) P[(size_t)(k) * n + j]
+    // RUN_RT_PASS's frequency set-up (:3905-3915)
+    const double freq = CLIGHT_NMS / wave;
+    const double waveno = 1.0e7 / wave;
+    const double freqlg = log(freq);
+    const double freq15 = freq / 1.0e15;
+    const double ehvkt = exp(-freq * S.hkt[j]);
+    const double stim = 1.0 - ehvkt;
+    const double bnu = BNU_PREFAC * ((freq15 * freq15) * freq15) * ehvkt / stim;
+    const double t = S.t[j], tkev = S.tkev[j], tlog = S.tlog[j], hckt = S.hckt[j], hkt = S.hkt[j];
+    const double xne = S.xne[j], rho = S.rho[j];
+    const double x1 = xnfp(S, 1, j), x2 = xnfp(S, 2, j), x3 = xnfp(S, 3, j);
+    const double b1 = bhyd(S, 1, j);
+    const double g1 = 0.0;
+
+    (void)waveno; (void)freqlg; (void)t; (void)tkev; (void)tlog; (void)hckt; (void)hkt; (void)xne;
+    (void)rho; (void)x1; (void)x2; (void)x3; (void)b1; (void)g1; (void)bnu; (void)stim; (void)ehvkt;
     // HLINOP (:17419-17500)
     double ahline = 0.0, shline = 0.0;
     {
@@ -1172,7 +1260,22 @@ A12_HD inline void kp_cell(const KTab &T, const KState &S, const double *P, cons
             }
         }
     }
-    // KAPP's totals (:11296-11320)
+    o[15] = ahline; o[16] = shline;
+#undef PL
+}
+
+// KAPP's totals (:11296-11320) from the 17 per-source values; out[0..22]
+// in a12_kapp_eval's order.
+A12_HD inline void kp_totals(const double *o, double wave, double hkt, double *out)
+{
+    const double freq = CLIGHT_NMS / wave;
+    const double freq15 = freq / 1.0e15;
+    const double ehvkt = exp(-freq * hkt);
+    const double stim = 1.0 - ehvkt;
+    const double bnu = BNU_PREFAC * ((freq15 * freq15) * freq15) * ehvkt / stim;
+    const double ahyd = o[0], shyd = o[1], ah2p = o[2], ahmin = o[3], shmin = o[4], sigh = o[5];
+    const double ahe1 = o[6], ahe2 = o[7], ahemin = o[8], sighe = o[9], ah2coll = o[10], ah2min = o[11];
+    const double acm = o[12], sigel = o[13], sigh2 = o[14], ahline = o[15], shline = o[16];
     const double A = ah2p + ahe1 + ahe2 + ahemin + ah2min + ah2coll + acm;
     const double acont = A + ahyd + ahmin;
     double scont = bnu;
@@ -1181,11 +1284,20 @@ A12_HD inline void kp_cell(const KTab &T, const KState &S, const double *P, cons
     double sline = bnu;
     if (aline > 0.0) sline = (ahline * shline + 0.0 * bnu + 0.0 * 0.0) / aline;
     const double sigmac = sigh + sighe + sigel + sigh2 + 0.0;
-    out[0] = ahyd; out[1] = shyd; out[2] = ah2p; out[3] = ahmin; out[4] = shmin; out[5] = sigh;
-    out[6] = ahe1; out[7] = ahe2; out[8] = ahemin; out[9] = sighe; out[10] = ah2coll; out[11] = ah2min;
-    out[12] = acm; out[13] = sigel; out[14] = sigh2; out[15] = ahline; out[16] = shline;
+    for (int k = 0; k < 17; ++k) out[k] = o[k];
     out[17] = acont; out[18] = sigmac; out[19] = scont; out[20] = sline; out[21] = aline; out[22] = 0.0;
-#undef PL
+}
+
+// All of KAPP at one cell (the host build's path; the device runs the parts
+// as separate kernels).
+A12_HD inline void kp_cell(const KTab &T, const KState &S, const double *P, const double *row,
+                           double wave, int j, double acm_tb, double *out)
+{
+    double o[17];
+    kp_cell_a(T, S, P, row, wave, j, o);
+    kp_cell_b(T, S, P, row, wave, j, acm_tb, o);
+    kp_cell_c(T, S, P, row, wave, j, o);
+    kp_totals(o, wave, S.hkt[j], out);
 }
 
 }  // namespace kapp

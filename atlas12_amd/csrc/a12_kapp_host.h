@@ -24,7 +24,7 @@ struct KPack {
         felo_logsig, felo_seaton, tb_thr, tb_ebind, tb_eexc, tb_logemin, tb_logestride, tb_ephot,
         tb_sigma, st_dens[4], st_temp[4], st_ldalpha[4], st_kalpha[4], st_prof[4], hm_wfflog, hm_fflog,
         h2m_logw, h2m_logt, h2m_logk, hem_logw, hem_logt, hem_logk;
-    size_t cia_nt, cia_use, felo_ifancy, felo_npts, tb_idx, tb_nlev, tb_g, tb_np, tb_off, st_maxdens[4];
+    size_t cia_nt, cia_use, felo_ifancy, felo_npts, tb_idx, tb_nlev, tb_g, tb_np, tb_off, tb_sp, st_maxdens[4];
     size_t k_freqn, k_xn, k_xl, k_ekarzas;
     KTab scal;   // the scalar members (counts, constants); pointers unset
 };
@@ -64,7 +64,8 @@ inline std::string kapp_pack(const a12_kapp_tables *t, KPack &P)
     for (int p = 0; p < t->cia_npair; ++p)
         if (t->cia_nt[p] < 2 || t->cia_nt[p] > t->cia_ntmax) return "CIA temperature count out of range";
     if (!t->cross_ch || !t->cross_oh) return "CH/OH cross sections missing";
-    if (t->hi_nedge < 0 || (t->hi_nedge > 0 && !t->hi_edge)) return "hotop edges missing";
+    if (t->hi_nedge < 0 || t->hi_nedge > 60 || (t->hi_nedge > 0 && !t->hi_edge))
+        return "hotop edges missing or more than 60";
     for (int e = 0; e < t->hi_nedge; ++e) {
         const int id = (int)t->hi_edge[7 * e + 6];
         if (id < 1 || id > 1006) return "hotop species index out of range";
@@ -156,6 +157,10 @@ inline std::string kapp_pack(const a12_kapp_tables *t, KPack &P)
     std::vector<int32_t> off(tb ? t->tb_nlevels : 1, 0);
     for (int l = 1; l < t->tb_nlevels; ++l) off[l] = off[l - 1] + t->tb_np[l - 1];
     P.tb_off = put(P.i, off.data(), off.size());
+    std::vector<int32_t> sp(tb ? t->tb_nlevels : 1, 0);
+    for (int q = 0, l = 0; tb && q < t->tb_nspecies; ++q)
+        for (int m = 0; m < t->tb_nlev[q]; ++m) sp[l++] = q;
+    P.tb_sp = put(P.i, sp.data(), sp.size());
     P.tb_ebind = tb ? put(P.d, t->tb_ebind, t->tb_nlevels) : put(P.d, &dzero, 1);
     P.tb_eexc = tb ? put(P.d, t->tb_eexc, t->tb_nlevels) : put(P.d, &dzero, 1);
     P.tb_logemin = tb ? put(P.d, t->tb_logemin, t->tb_nlevels) : put(P.d, &dzero, 1);
@@ -231,6 +236,7 @@ inline KTab kapp_tab(const KPack &P, const double *d, const int *i, const float 
     T.tb_g = i + P.tb_g;
     T.tb_np = i + P.tb_np;
     T.tb_off = i + P.tb_off;
+    T.tb_sp = i + P.tb_sp;
     T.tb_thr = d + P.tb_thr;
     T.tb_ebind = d + P.tb_ebind;
     T.tb_eexc = d + P.tb_eexc;

@@ -577,3 +577,19 @@ def kapp_standin(z, numnu):
             f = np.stack([np.interp(x, sub, s[:, j]) for j in range(s.shape[1])], 1)
         out.append(np.ascontiguousarray(f))
     return out
+
+
+def kapp_deck_state(ref, deck: str, mion: int = 1006) -> dict:
+    """KAPP's depth inputs of one deck of a KAPP fixture (the layout of
+    tests/golden/kapp_ref.npz: per-depth arrays, BHYD (6, nrhox), and the
+    XNF/XNFP columns the sources read, listed in xnf_ids/xnfp_ids) as full
+    (mion, nrhox) arrays, zero in the columns the sources do not read."""
+    n = len(ref[deck + "_t"])
+    st = {k: np.asarray(ref[f"{deck}_{k}"]) for k in ("t", "tkev", "tlog", "hkt", "hckt", "xne", "rho",
+                                                      "bmin", "equilh2", "bhyd")}
+    xnf = np.zeros((mion, n))
+    xnf[np.asarray(ref["xnf_ids"]) - 1] = ref[deck + "_xnf"]
+    xnfp = np.zeros((mion, n))
+    xnfp[np.asarray(ref["xnfp_ids"]) - 1] = ref[deck + "_xnfp"]
+    st["xnf"], st["xnfp"] = xnf, xnfp
+    return st

@@ -115,8 +115,16 @@ def atlas_run(args, ctx, native, z, d, pred, tio):
     t_lines = time.perf_counter() - t0
     log(f"atlas: line store loaded in {t_lines:.2f}s (once per store)")
     native.atlas_set_state(ctx, d)
-    kapp = wl.kapp_standin(z, len(z["waveset"]))
-    native.atlas_set_kapp(ctx, *kapp)
+    # KAPP on the device: its tables once per run, its depth state (the
+    # deck's populations from the reference's set-up) per iteration
+    kref = np.load(os.path.join(GOLDEN, "kapp_ref.npz"))
+    kst = wl.kapp_deck_state(kref, "plain")
+    if not (np.array_equal(kst["t"], z["t"]) and np.array_equal(kst["xne"], z["xne"])):
+        raise SystemExit("KAPP state fixture is not the ATLAS12 workload's atmosphere")
+    native.kapp_set_tables(ctx, native.kapp_tables(np.load(os.path.join(GOLDEN, "kapp_tables.npz"))))
+    kstate = native.kapp_state(kst)
+    native.kapp_set_state(ctx, kstate)
+    native.atlas_kapp_run(ctx)
     shapes = native.atlas_shapes(ctx)
     best, tune_ms = native.atlas_autotune(ctx, native.PREC_MIXED)
     log(f"atlas: autotuned gather shape {shapes[best]} (pts, lanes, batch, form 0 staged / 1 flat "
@@ -131,18 +139,21 @@ def atlas_run(args, ctx, native, z, d, pred, tio):
 
     def steps(prec, k):
         """k iterations of the driver's per-iteration call sequence: the
-        state, LINOP1, KAPP's tables, the RT pass, TCORR(3) on the device
-        sums (synchronous: it returns T1 to the host)."""
-        st = np.zeros(4)
+        state, LINOP1, KAPP (its depth state, then every frequency on the
+        device), the RT pass, TCORR(3) on the device sums (synchronous: it
+        returns T1 to the host)."""
+        st = np.zeros(5)
         torch.cuda.synchronize()
         t = time.perf_counter()
         for _ in range(k):
             native.atlas_set_state(ctx, d)
             native.atlas_linop1(ctx, prec, fetch=False)
-            native.atlas_set_kapp(ctx, *kapp)
+            native.kapp_set_state(ctx, kstate)
+            native.atlas_kapp_run(ctx)
             native.atlas_rt_pass(ctx, fetch=False)
             native.atlas_tcorr3(ctx, tz, device_sums=True)
-            st += native.atlas_stage_ms(ctx)  # waits on this step's events
+            st[:4] += native.atlas_stage_ms(ctx)  # waits on this step's events
+            st[4] += native.kapp_ms(ctx)
         torch.cuda.synchronize()
         return time.perf_counter() - t, st / max(k, 1)
 
@@ -163,7 +174,8 @@ def atlas_run(args, ctx, native, z, d, pred, tio):
         return None
     parts = {"set_state": host_ms(lambda: native.atlas_set_state(ctx, d))}
     native.atlas_linop1(ctx, native.PREC_MIXED, fetch=False)
-    parts["set_kapp_upload"] = host_ms(lambda: native.atlas_set_kapp(ctx, *kapp))
+    parts["kapp_set_state"] = host_ms(lambda: native.kapp_set_state(ctx, kstate))
+    parts["kapp_run"] = host_ms(lambda: native.atlas_kapp_run(ctx))
     native.atlas_rt_pass(ctx, fetch=False)
     parts["tcorr3"] = host_ms(lambda: native.atlas_tcorr3(ctx, tz, device_sums=True))
     log("atlas: timed mixed")
@@ -176,6 +188,12 @@ def atlas_run(args, ctx, native, z, d, pred, tio):
     t_64, st_64 = steps(native.PREC_FP64, k64)
     _, ev_64 = native.atlas_linop1(ctx, native.PREC_FP64)
     acc_64, h_64 = native.atlas_rt_pass(ctx)
+    # KAPP of the last step against the reference's KAPP at every 100th frequency
+    kdev = native.atlas_kapp_fetch(ctx)
+    sub = z["kapp_sub"]
+    kapp_err = max(float(np.max(np.abs(kdev[k][sub] - z[k + "_sub"]) /
+                                np.maximum(np.abs(z[k + "_sub"]), 1e-300)))
+                   for k in ("acont", "sigmac", "scont", "sline", "aline", "sigmal"))
     flux_err = float(np.max(np.abs(h_mix - h_64) / np.abs(h_64)))
     flx_err = float(np.max(np.abs(acc_mix["flxrad"] - acc_64["flxrad"]) / np.abs(acc_64["flxrad"])))
     lo_mix = float(st_mix[:3].sum())
@@ -184,7 +202,7 @@ def atlas_run(args, ctx, native, z, d, pred, tio):
     return dict(shape=shapes[best], tune={str(k): float(m) for k, m in zip(shapes, tune_ms)},
                 store=len(store), t_sel=t_sel, t_lines=t_lines, parts=parts, t_mix=t_mix / args.steps, t_64=t_64 / k64,
                 st_mix=st_mix, st_64=st_64, ev_mix=ev_mix, ev_64=ev_64, lo_mix=lo_mix, lo_64=lo_64,
-                flux_err=flux_err, flx_err=flx_err, nf=nf, store_lines=store)
+                flux_err=flux_err, flx_err=flx_err, nf=nf, store_lines=store, kapp_err=kapp_err)
 
 
 # ----------------------------------------------------------- plain SYNTHE
@@ -690,9 +708,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
-        "data": "synthetic (seeded raw line lists; atmosphere state and KAPP tables from the "
-                "reference's own first-iteration set-up on a seeded deck, KAPP interpolated "
-                "between every 100th frequency)",
+        "data": "synthetic (seeded raw line lists; atmosphere state and populations from the "
+                "reference's own first-iteration set-up on a seeded deck; KAPP computed on the "
+                "device from the reference's data tables)",
         "config": {"workload": "atlas12_iteration: one ATLAS12 opacity-sampling iteration, "
                                "72 layers, 30,000 frequencies (1e-4 dex from 91.2 nm), "
                                f"{args.lines:,} raw lines -> SELECTLINES -> LINOP1 + RUN_RT_PASS",
@@ -701,10 +719,16 @@ def main():
                    "evals_per_step": int(ev)},
         "s_per_iter": a["t_mix"],
         "s_per_iter_covers": "per iteration: a12_atlas_set_state (host arrays -> device), "
-                             "LINOP1, a12_atlas_set_kapp (upload of KAPP's six tables, computed "
-                             "on the host), the RUN_RT_PASS loop, TCORR(3) on the device sums; "
-                             "not KAPP itself",
+                             "LINOP1, a12_kapp_set_state (populations -> device) and KAPP at "
+                             "all 30,000 frequencies on the device, the RUN_RT_PASS loop, "
+                             "TCORR(3) on the device sums",
         "host_ms_per_call": a["parts"],
+        "kapp": {"ms": a["st_mix"][4], "cells": int(a["nf"] * len(z["t"])),
+                 "cells_per_s": a["nf"] * len(z["t"]) / (a["st_mix"][4] * 1e-3),
+                 "max_rel_err_vs_reference_every_100th_frequency": a["kapp_err"],
+                 "note": "KAPP and its sources (H, H-, H2+, H2-, CIA, He I/II/He-, Rayleigh, "
+                         "TOPbase metals, hot ions, Fe I/II, CH/OH, H lines) at every (frequency, "
+                         "layer) of the grid, HIP events around a12_atlas_kapp_run"},
         "line_store_load_s": a["t_lines"],
         "line_store_load_note": "once per store (decode, bin and sort of the selected lines); "
                                 "before the store/state split it was paid on every state",
@@ -715,9 +739,11 @@ def main():
         "flux_max_rel_err_mixed_vs_fp64": a["flux_err"],
         "flxrad_max_rel_err_mixed_vs_fp64": a["flx_err"],
         "stage_ms_mixed": {"linop1_setup": a["st_mix"][0], "linop1_reach": a["st_mix"][1],
-                           "linop1_gather": a["st_mix"][2], "rt_pass": a["st_mix"][3]},
+                           "linop1_gather": a["st_mix"][2], "rt_pass": a["st_mix"][3],
+                           "kapp": a["st_mix"][4]},
         "stage_ms_fp64": {"linop1_setup": a["st_64"][0], "linop1_reach": a["st_64"][1],
-                          "linop1_gather": a["st_64"][2], "rt_pass": a["st_64"][3]},
+                          "linop1_gather": a["st_64"][2], "rt_pass": a["st_64"][3],
+                          "kapp": a["st_64"][4]},
         "selectlines_s": a["t_sel"],
         "autotune": {"shape_pts_lanes_batch_form": list(a["shape"]),
                      "gather_ms_per_shape": a["tune"]},

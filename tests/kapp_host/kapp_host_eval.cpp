@@ -26,12 +26,15 @@ extern "C" int kcpu_eval(const a12_kapp_tables *t, const a12_kapp_state *s, int 
         kp_state(T, S, Q_HOLTSMARK, j, pl.data());
         for (int l = 0; l < T.tb_nlev_tot; ++l) pl[(size_t)(KS_TB + l) * n + j] = kp_state_tb(T, S, l, j);
     }
-    std::vector<double> row(KF_N);
+    std::vector<double> row(KF_N), sig((size_t)T.tb_nlev_tot * nw + 1);
+    for (int l = 0; l < T.tb_nlev_tot; ++l)
+        for (int w = 0; w < nw; ++w) sig[(size_t)l * nw + w] = kp_tb_sigma(T, l, waves[w]);
     double cell[23];
     for (int w = 0; w < nw; ++w) {
         kp_freq(T, waves[w], row.data());
         for (int j = 0; j < n; ++j) {
-            kp_cell(T, S, pl.data(), row.data(), waves[w], j, cell);
+            const double tb = kp_cell_tb(T, S, pl.data(), sig.data(), nw, w, waves[w], j);
+            kp_cell(T, S, pl.data(), row.data(), waves[w], j, tb, cell);
             for (int k = 0; k < 23; ++k) out[((size_t)w * 23 + k) * n + j] = cell[k];
         }
     }

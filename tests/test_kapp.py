@@ -70,15 +70,8 @@ def tables():
 def deck_state(ref, deck, mion=1006):
     """The deck's KAPP inputs with XNF/XNFP rebuilt from the stored columns
     (the species the sources read; the rest zero)."""
-    n = len(ref[deck + "_t"])
-    st = {k: ref[f"{deck}_{k}"] for k in ("t", "tkev", "tlog", "hkt", "hckt", "xne", "rho", "bmin",
-                                          "equilh2", "bhyd")}
-    xnf = np.zeros((mion, n))
-    xnf[ref["xnf_ids"] - 1] = ref[deck + "_xnf"]
-    xnfp = np.zeros((mion, n))
-    xnfp[ref["xnfp_ids"] - 1] = ref[deck + "_xnfp"]
-    st["xnf"], st["xnfp"] = xnf, xnfp
-    return st
+    from atlas12_amd.workloads import kapp_deck_state
+    return kapp_deck_state(ref, deck, mion)
 
 
 @pytest.mark.parametrize("deck", DECKS)
