@@ -79,6 +79,35 @@ MODULE a12gpu_iface
     TYPE(C_PTR)        :: locj, kcomps, idequa
   END TYPE a12_moltab
 
+  ! KAPP's tables (a12_kapp_tables): the Stehle Stark tables of one series
+  TYPE, BIND(C) :: a12_kapp_stark
+    INTEGER(C_INT32_T) :: loaded, n_upper_min, n_upper_max, n_dens
+    TYPE(C_PTR)        :: dens, temp, log_dalpha, max_dens_idx, k_alpha, profiles
+  END TYPE a12_kapp_stark
+
+  ! KAPP's data tables (C_LOC of the arrays the sources hold; see a12gpu.h)
+  TYPE, BIND(C) :: a12_kapp_tables
+    TYPE(C_PTR)        :: k_freqn, k_xn, k_xl, k_ekarzas, gff_logtab
+    INTEGER(C_INT32_T) :: cia_npair, cia_ntmax, cia_nnu
+    REAL(C_DOUBLE)     :: cia_nu0, cia_dnu
+    TYPE(C_PTR)        :: cia_nt, cia_temp, cia_tab, cia_use, cross_ch, cross_oh
+    INTEGER(C_INT32_T) :: hi_nedge
+    TYPE(C_PTR)        :: hi_edge
+    INTEGER(C_INT32_T) :: felo_nlev(2), felo_xnfp_idx(2)
+    TYPE(C_PTR)        :: felo_nuth, felo_g, felo_e, felo_ifancy, felo_npts, felo_x, felo_logsig, &
+                          felo_seaton
+    INTEGER(C_INT32_T) :: tb_nspecies, tb_nlevels, tb_npoints
+    TYPE(C_PTR)        :: tb_xnfp_idx, tb_nlev, tb_lowest_thr, tb_g, tb_np, tb_ebind, tb_eexc, &
+                          tb_logemin, tb_logestride, tb_ephot, tb_sigma
+    TYPE(a12_kapp_stark) :: stark(4)
+  END TYPE a12_kapp_tables
+
+  ! KAPP's depth state: C_LOC of T, TKEV, ... (kw), BHYD (kw,6), XNF/XNFP (kw,mion)
+  TYPE, BIND(C) :: a12_kapp_state
+    INTEGER(C_INT32_T) :: nrhox, ld, mion
+    TYPE(C_PTR)        :: t, tkev, tlog, hkt, hckt, xne, rho, bmin, bhyd, equilh2, xnf, xnfp
+  END TYPE a12_kapp_state
+
   INTERFACE
     INTEGER(C_INT32_T) FUNCTION a12_ctx_create(device, ctx) BIND(C, NAME='a12_ctx_create')
       IMPORT :: C_INT32_T, C_PTR
@@ -261,6 +290,43 @@ MODULE a12gpu_iface
       REAL(C_DOUBLE), INTENT(OUT) :: t_out(*), s_out(*)
       INTEGER(C_INT32_T), INTENT(OUT) :: iters(*)
       TYPE(C_PTR), VALUE :: hist             ! C_NULL_PTR or C_LOC of (3, itmax, nmodel)
+    END FUNCTION
+
+    ! ---- KAPP ---------------------------------------------------------------
+    INTEGER(C_INT32_T) FUNCTION a12_kapp_set_tables(ctx, tables) BIND(C, NAME='a12_kapp_set_tables')
+      IMPORT :: C_INT32_T, C_PTR, a12_kapp_tables
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(a12_kapp_tables), INTENT(IN) :: tables
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_kapp_set_state(ctx, state) BIND(C, NAME='a12_kapp_set_state')
+      IMPORT :: C_INT32_T, C_PTR, a12_kapp_state
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(a12_kapp_state), INTENT(IN) :: state
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_atlas_kapp_run(ctx) BIND(C, NAME='a12_atlas_kapp_run')
+      IMPORT :: C_INT32_T, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_atlas_kapp_fetch(ctx, kapp6) BIND(C, NAME='a12_atlas_kapp_fetch')
+      IMPORT :: C_INT32_T, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), INTENT(OUT) :: kapp6(*)        ! (nrhox, numnu, 6)
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_kapp_eval(ctx, nwave, wave_nm, out) BIND(C, NAME='a12_kapp_eval')
+      IMPORT :: C_INT32_T, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT32_T), VALUE :: nwave
+      REAL(C_DOUBLE), INTENT(IN) :: wave_nm(*)
+      REAL(C_DOUBLE), INTENT(OUT) :: out(*)           ! (nrhox, 23, nwave)
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_kapp_constants(out12) BIND(C, NAME='a12_kapp_constants')
+      IMPORT :: C_INT32_T, C_DOUBLE
+      REAL(C_DOUBLE), INTENT(OUT) :: out12(12)
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_kapp_ms(ctx, ms) BIND(C, NAME='a12_kapp_ms')
+      IMPORT :: C_INT32_T, C_PTR, C_FLOAT
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_FLOAT), INTENT(OUT) :: ms
     END FUNCTION
 
     ! ---- TCORR(3), SCALE_MODEL, NMOLEC -----------------------------------
