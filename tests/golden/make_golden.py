@@ -738,6 +738,145 @@ def golden_kapp():
                      "tb_levels": int(len(tables["tb_g"])), "hi_edges": int(len(tables["hi_edge"]))}}
 
 
+def gfall_line(wl_nm, loggf, code, e, xj, ep, xjp, gr=0.0, gs=0.0, gw=0.0, nblo=0, nbup=0,
+               iso1=0, iso2=0, labelp="SYNTH   ", auto="   "):
+    """One gfall record with the fields the complex-line routing reads
+    (src/mod_mklinelist.f90:861-1060): NBLO/NBUP, ISO1/ISO2 (NCON), the
+    CONTINUU label and the AUT/PRD flag in the last OTHER field."""
+    other2 = f"{'':6s}{0:1d}{auto:3s}"
+    return (f"{wl_nm:11.4f}{loggf:7.3f}{code:6.2f}{e:12.3f}{xj:5.1f} {'SYNTH':8s}{'':2s}"
+            f"{ep:12.3f}{xjp:5.1f} {labelp:8s}{'':2s}{gr:6.2f}{gs:6.2f}{gw:6.2f}"
+            f"SYN {nblo:2d}{nbup:2d}{iso1:3d}{0.0:6.3f}{iso2:3d}{0.0:6.3f}{'':10s}{other2:10s}"
+            f"{0:5d}{0:5d}{0:6d}\n")
+
+
+def cplx_gfall():
+    """Complex-profile lines of every type compute_line_opacity handles
+    (src/synthe_module.f90:3167-3702): H I Balmer lines n = 2 -> 3..30
+    (Halpha as m = n+1, Hbeta as m = n+2, the rest merging into the Balmer
+    continuum, NCON = 2), He I lines with and without a continuum edge,
+    NLTE-flagged metal lines (Mg I, Ca II, Na I), a PRD line, an
+    autoionising Ca I line and a merged-continuum (CONTINUU) Mg I record;
+    sorted by wavelength as gfall is."""
+    ryd, elim = 109677.583403, 109678.764
+    e2 = elim - ryd / 4.0
+    out = []
+    for m in range(3, 31):
+        eu = elim - ryd / m ** 2
+        w = 1.0e7 / (eu - e2)
+        f = 0.64 / m ** 3 * (1.0 / 4.0 - 1.0 / m ** 2) ** -3 / 8.0
+        out.append((w, gfall_line(w, float(np.log10(8.0 * f)), 1.00, e2, 0.5, eu, 0.5,
+                                  nblo=2, nbup=m, iso2=(2 if m >= 5 else 0))))
+    # He I (code 2.00): 447.1 (no edge) and 492.2 (NCON 1)
+    out.append((447.1480, gfall_line(447.1480, 0.052, 2.00, 169086.766, 5.0, 191444.482, 7.0,
+                                     nblo=4, nbup=9)))
+    out.append((492.1931, gfall_line(492.1931, -0.435, 2.00, 171134.897, 1.0, 191446.456, 2.0,
+                                     nblo=5, nbup=10, iso2=1)))
+    # NLTE-flagged metals: Mg I b (518.36), Ca II H (396.85), Na I D2 (589.16)
+    out.append((518.3604, gfall_line(518.3604, -0.167, 12.00, 21911.178, 1.0, 41197.403, 1.0,
+                                     nblo=3, nbup=6)))
+    out.append((396.9591, gfall_line(396.9591, -0.166, 20.01, 0.0, 0.5, 25191.51, 0.5,
+                                     nblo=1, nbup=2)))
+    out.append((589.1583, gfall_line(589.1583, 0.108, 11.00, 0.0, 0.5, 16973.366, 1.5,
+                                     nblo=1, nbup=3, iso2=1)))
+    # a PRD line (Fe II), an autoionising Ca I line, a merged continuum (Mg I)
+    out.append((455.0, gfall_line(455.0, -1.2, 26.01, 20000.0, 2.5, 41978.0, 3.5, auto="PRD")))
+    out.append((520.5, gfall_line(520.5, -2.0, 20.00, 20000.0, 1.0, 39212.4, 2.0, gr=13.0, gs=1.5,
+                                  gw=2.0, auto="AUT")))
+    out.append((430.0, gfall_line(430.0, -3.0, 12.00, 35000.0, 0.5, 58255.8, 6.0,
+                                  labelp="CONTINUU")))
+    out.sort()
+    return "".join(r for _, r in out)
+
+
+def read_cplx_dump(path):
+    """ref_synthe_dump's A12_CPLX_DUMP file."""
+    b = open(path, "rb").read()
+    off = 0
+
+    def take(dt, n):
+        nonlocal off
+        a = np.frombuffer(b, dtype=dt, count=n, offset=off)
+        off += a.nbytes
+        return a.copy()
+    n, length, n19, mw6 = (int(x) for x in take("<i4", 4))
+    d = dict(nrhox=n, length=length, nlines=n19, mw6=mw6, cutoff=float(take("<f8", 1)[0]))
+    recs = []
+    for _ in range(n19):
+        wlv = float(take("<f8", 1)[0])
+        elo, gf = (float(x) for x in take("<f4", 2))
+        nblo, nbup, nelion, itype, ncon, nelionx = (int(x) for x in take("<i4", 6))
+        gr, gs, gw = (float(x) for x in take("<f4", 3))
+        nbuff, lim = (int(x) for x in take("<i4", 2))
+        recs.append((wlv, elo, gf, nblo, nbup, nelion, itype, ncon, nelionx, gr, gs, gw, nbuff, lim))
+    d["lines"] = np.array(recs, dtype=CPLX_DTYPE)
+    for k in ("t", "xne", "rho", "hckt", "xnfhe1", "xnfh2"):
+        d[k] = take("<f8", n)
+    per = {k: [] for k in ("txnxn", "xnfph1", "xnfph2", "xnfpel", "dopple", "xnfdop", "continuum", "buffer")}
+    for _ in range(n):
+        a = take("<f4", 1)
+        per["txnxn"].append(float(a[0]))
+        x = take("<f8", 2)
+        per["xnfph1"].append(x[0])
+        per["xnfph2"].append(x[1])
+        per["xnfpel"].append(take("<f8", mw6))
+        per["dopple"].append(take("<f8", mw6))
+        per["xnfdop"].append(take("<f8", mw6))
+        per["continuum"].append(take("<f4", length))
+        per["buffer"].append(take("<f4", length))
+    for k, v in per.items():
+        d[k] = np.array(v, dtype=np.float32 if k in ("txnxn", "continuum", "buffer") else np.float64)
+    assert off == len(b), (off, len(b))
+    return d
+
+
+CPLX_DTYPE = np.dtype([("wlvac", "<f8"), ("elo", "<f4"), ("gf", "<f4"), ("nblo", "<i4"),
+                       ("nbup", "<i4"), ("nelion", "<i4"), ("itype", "<i4"), ("ncon", "<i4"),
+                       ("nelionx", "<i4"), ("gammar", "<f4"), ("gammas", "<f4"), ("gammaw", "<f4"),
+                       ("nbuff", "<i4"), ("lim", "<i4")])
+
+
+CPLX_DEPTHS = np.r_[0:72:6, 71]
+
+
+def golden_cplx():
+    """compute_line_opacity (src/synthe_module.f90:3167-3702) on the
+    plain-SYNTHE atmosphere, 400-660 nm at R = 30,000, for the complex lines
+    of cplx_gfall(): the line records as run_mklinelist builds them, the
+    per-depth inputs the driver forms (src/synthe.f90:476-551) and the
+    CONTINUUM and BUFFER arrays after the call, at 13 of the 72 depths; the
+    SYNTHE state of the window for the device path."""
+    atm = wl.scurve_atmosphere()
+    tmp = tempfile.mkdtemp(prefix="a12c_")
+    gpath = os.path.join(tmp, "gfall.dat")
+    with open(gpath, "w") as fh:
+        fh.write(cplx_gfall())
+    root = data_root(tmp, gpath)
+    deck = os.path.join(tmp, "cplx.dat")
+    with open(deck, "w") as fh:
+        fh.write(atm.deck())
+    env = dict(os.environ, ATLAS12=root, A12_CPLX_DUMP=os.path.join(tmp, "cplx.bin"))
+    dump = os.path.join(tmp, "dump.bin")
+    run([os.path.join(REFBIN, "ref_synthe_dump"), deck, "400.0", "660.0", "30000.0", dump], tmp, env)
+    d = trim_edges(read_dump(dump))
+    c = read_cplx_dump(env["A12_CPLX_DUMP"])
+    shutil.rmtree(tmp)
+    nel = sorted(set(int(x) for x in c["lines"]["nelion"]) | {1, 2, 7, 8, 9})
+    save_state(os.path.join(HERE, "cplx_state.npz"), d, nel)
+    js = CPLX_DEPTHS
+    keep = {k: c[k] for k in ("t", "xne", "rho", "hckt", "xnfhe1", "xnfh2")}
+    for k in ("txnxn", "xnfph1", "xnfph2", "continuum", "buffer"):
+        keep[k] = c[k][js]
+    idx = np.array([i - 1 for i in nel], np.int32)
+    for k in ("xnfpel", "dopple", "xnfdop"):
+        keep[k] = c[k][js][:, idx]
+    np.savez_compressed(os.path.join(HERE, "cplx_ref.npz"), lines=c["lines"], depths=js.astype(np.int32),
+                        ion_idx=idx, length=c["length"], cutoff=c["cutoff"], **keep)
+    types = {int(t): int(np.sum(c["lines"]["itype"] == t)) for t in np.unique(c["lines"]["itype"])}
+    return {"cplx": {"nlines": int(c["nlines"]), "length": int(c["length"]), "itypes": types,
+                     "nonzero_points": int(np.count_nonzero(c["buffer"]))}}
+
+
 def golden_atlas_small():
     """One ATLAS12 opacity-sampling pass on the plain-SYNTHE atmosphere:
     240 frequencies from 500 nm (1e-4 dex steps), 3000 lines with the
@@ -940,7 +1079,7 @@ STEPS = {"synthe_small": golden_synthe_small, "atlas_small": golden_atlas_small,
          "tcorr": golden_tcorr, "tcorr_conv": golden_tcorr_conv,
          "state_plain": golden_state_plain,
          "numerics": golden_numerics, "josh": golden_josh, "molec": golden_molec,
-         "kapp": golden_kapp}
+         "kapp": golden_kapp, "cplx": golden_cplx}
 
 
 def main():
