@@ -21,7 +21,7 @@ ARCH = os.environ.get("A12_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["a12_capi.hip", "a12_synthe.hip", "a12_josh.hip", "a12_atlas.hip", "a12_select.hip",
-           "a12_mlp.hip", "a12_molec.hip", "a12_kapp.hip"]
+           "a12_mlp.hip", "a12_molec.hip", "a12_kapp.hip", "a12_cplx.hip"]
 CFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wall", "-Wno-unused-function", "-I", os.path.join(REPO, "include")]

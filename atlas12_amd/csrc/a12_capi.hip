@@ -67,6 +67,7 @@ int a12_ctx_destroy(a12_ctx *ctx)
     for (DevBuf *b : bufs) b->release();
     a12i_mlp_free(ctx);
     a12i_kapp_free(ctx);
+    a12i_cplx_free(ctx);
     a12i_synthe_batch_free(ctx);
     for (hipEvent_t e : ctx->ev)
         if (e) (void)hipEventDestroy(e);

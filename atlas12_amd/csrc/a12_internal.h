@@ -49,6 +49,7 @@ struct SyntheDev {
     DevBuf work;
     long long *evals_dev = nullptr;  // [all, Voigt core] of the last run, in work
     int shape = 0;                   // gather tile shape (a12_synthe_set_shape)
+    uint64_t gen = 0;                // counts a12_synthe_load calls (a new atmosphere)
     // outputs
     DevBuf opacity;  // (nrhox x L) float or double
     int opac_prec = -1;
@@ -100,6 +101,7 @@ struct ModelDev {
 
 struct MlpDev;    // a12_mlp.hip
 struct KappDev;   // a12_kapp.hip
+struct CplxDev;   // a12_cplx.hip
 struct SynBatch;  // a12_synthe.hip
 
 struct a12_ctx {
@@ -115,6 +117,7 @@ struct a12_ctx {
     ModelDev mdl;
     MlpDev *mlp = nullptr;
     KappDev *kapp = nullptr;
+    CplxDev *cplx = nullptr;
     SynBatch *sbatch = nullptr;
     DevBuf rt_part;
     // scratch for a12_josh_batch and a12_solvit_batched
@@ -158,6 +161,16 @@ int a12i_synthe_batch_fetch(a12_ctx *ctx, int32_t model, void *opacity_out, doub
                             double *hcont, int64_t *evals);
 int a12i_synthe_batch_info(a12_ctx *ctx, int64_t *info);
 void a12i_synthe_batch_free(a12_ctx *ctx);
+// The per-depth prep and continuum of the loaded SYNTHE state (k_depth_prep,
+// k_continuum) into caller arrays: xnfdop (nrhox, mw6), dscal (nrhox, 4),
+// cont32 (nrhox, length).
+int a12i_synthe_prep(a12_ctx *ctx, double *xnfdop, double *dscal, float *cont32);
+// Implemented in a12_cplx.hip: the complex lines of the loaded SYNTHE state
+// from the prep's xnfdop and cont32, added into the opacity (float, or
+// double when f64), on the context stream.
+bool a12i_cplx_active(const a12_ctx *ctx);
+int a12i_cplx_add(a12_ctx *ctx, const double *xnfdop, const float *cont32, void *opacity, bool f64);
+void a12i_cplx_free(a12_ctx *ctx);
 // Implemented in a12_atlas.hip
 int a12i_atlas_load(a12_ctx *ctx, const a12_atlas_atm *a, const int32_t *linedata, int32_t nlines);
 int a12i_atlas_load_lines(a12_ctx *ctx, const a12_atlas_atm *a, const int32_t *linedata,

@@ -891,8 +891,29 @@ int a12i_synthe_load(a12_ctx *ctx, const a12_synthe_atm *atm, const a12_lte_line
     A12_HIP(upload(S.l_gw, g3.data(), nl * (nlines > 0), st));
     A12_HIP(hipStreamSynchronize(st));
     S.loaded = true;
+    ++S.gen;
+    a12_cplx_clear(ctx);  // complex lines belong to the atmosphere they were loaded with
     S.opac_prec = -1;
     S.transferred = false;
+    return A12_OK;
+}
+
+int a12i_synthe_prep(a12_ctx *ctx, double *xnfdop, double *dscal, float *cont32)
+{
+    SyntheDev &S = ctx->syn;
+    if (!S.loaded) {
+        ctx->err = "call a12_synthe_load first";
+        return A12_ERR_STATE;
+    }
+    hipStream_t st = ctx->stream;
+    const int n = S.nrhox, L = S.L;
+    hipLaunchKernelGGL(k_depth_prep, dim3(n), dim3(256), 0, st, n, S.mw6, S.xnfpel.as<double>(),
+                       S.dopple.as<double>(), S.depth.as<double>(), xnfdop, dscal);
+    A12_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_continuum, dim3((L + 255) / 256, n), dim3(256), 0, st, L, n, S.wbegin, S.ratio,
+                       S.nedge, S.wledge.as<double>(), S.nf, S.continall.as<double>(), cont32,
+                       (double *)nullptr);
+    A12_CHECK_LAUNCH();
     return A12_OK;
 }
 
@@ -1104,6 +1125,12 @@ int a12i_synthe_run(a12_ctx *ctx, int prec)
     rc = launch_gather(ctx, M, f64, shape, st);
     if (rc != A12_OK) return rc;
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[4], st));
+    // complex-profile lines, when loaded: compute_line_opacity's buffer is
+    // the depth's starting BUFFER in the reference (src/synthe.f90:552)
+    if (a12i_cplx_active(ctx)) {
+        rc = a12i_cplx_add(ctx, M.xnfdop, M.cont32, M.opacity, f64);
+        if (rc != A12_OK) return rc;
+    }
     S.prof_valid = ctx->prof && N > 0;
     S.evals_dev = M.evals_out;
     S.opac_prec = prec;

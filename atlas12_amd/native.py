@@ -42,7 +42,8 @@ EXPORTS = ["a12_ctx_create", "a12_ctx_destroy", "a12_last_error", "a12_stream",
            "a12_synthe_batch_info", "a12_solvit_batched", "a12_atlas_load_lines",
            "a12_atlas_set_state", "a12_atlas_rt_pass_noalloc", "a12_model_converge",
            "a12_kapp_set_tables", "a12_kapp_set_state", "a12_atlas_kapp_run", "a12_kapp_eval",
-           "a12_kapp_constants", "a12_kapp_ms", "a12_atlas_kapp_fetch"]
+           "a12_kapp_constants", "a12_kapp_ms", "a12_atlas_kapp_fetch", "a12_cplx_load",
+           "a12_cplx_run", "a12_cplx_fetch", "a12_cplx_clear", "a12_cplx_ms"]
 
 
 class A12Error(RuntimeError):
@@ -935,3 +936,85 @@ def kapp_ms(ctx: Context) -> float:
     ms = C.c_float(-1.0)
     ctx._check(ctx.lib.a12_kapp_ms(ctx.h, C.byref(ms)), "a12_kapp_ms")
     return ms.value
+
+
+# ---- SYNTHE complex-profile lines (a12_cplx_*) ------------------------------
+# a12_cplx_line (nlte_line_t, src/mod_mklinelist.f90:43-62) with C's
+# alignment: 60 bytes of fields, 64 per record.
+CPLX_DTYPE = np.dtype([("wlvac", "<f8"), ("elo", "<f4"), ("gf", "<f4"), ("nblo", "<i4"),
+                       ("nbup", "<i4"), ("nelion", "<i4"), ("itype", "<i4"), ("ncon", "<i4"),
+                       ("nelionx", "<i4"), ("gammar", "<f4"), ("gammas", "<f4"), ("gammaw", "<f4"),
+                       ("nbuff", "<i4"), ("lim", "<i4")], align=True)
+
+
+class CplxLine(C.Structure):
+    _fields_ = [("wlvac", C.c_double), ("elo", C.c_float), ("gf", C.c_float), ("nblo", C.c_int32),
+                ("nbup", C.c_int32), ("nelion", C.c_int32), ("itype", C.c_int32), ("ncon", C.c_int32),
+                ("nelionx", C.c_int32), ("gammar", C.c_float), ("gammas", C.c_float),
+                ("gammaw", C.c_float), ("nbuff", C.c_int32), ("lim", C.c_int32)]
+
+
+def cplx_lines(recs) -> np.ndarray:
+    """Line records (any structured array with the nlte_line_t field
+    names) in the C-ABI layout."""
+    out = np.zeros(len(recs), CPLX_DTYPE)
+    for k in CPLX_DTYPE.names:
+        out[k] = recs[k]
+    return out
+
+
+def kapp_stark(t) -> _Keep:
+    """The four Stehle series (a12_kapp_stark[4]) from a mapping of arrays in
+    the layout of tests/golden/kapp_tables.npz (st{s}_* keys)."""
+    keep = []
+    S = (KappStark * 4)()
+    for s in range(4):
+        hdr = np.asarray(t[f"st{s}_hdr"])
+        x = S[s]
+        x.loaded, x.n_upper_min, x.n_upper_max, x.n_dens = (int(v) for v in hdr[[0, 2, 3, 4]])
+        if x.loaded:
+            x.dens = _arr(keep, t[f"st{s}_dens"], np.float64)
+            x.temp = _arr(keep, t[f"st{s}_temp"], np.float64)
+            x.log_dalpha = _arr(keep, t[f"st{s}_ldalpha"], np.float64)
+            x.max_dens_idx = _arr(keep, t[f"st{s}_maxdens"], np.int32)
+            x.k_alpha = _arr(keep, t[f"st{s}_kalpha"], np.float64)
+            x.profiles = _arr(keep, t[f"st{s}_prof"], np.float64)
+    return _Keep(S, keep)
+
+
+def cplx_load(ctx: Context, st: SyntheState, lines: np.ndarray, stark: _Keep, wlend: float,
+              cutoff: float, ifvac: int = 1, velshift: float = 0.0):
+    """a12_cplx_load: the complex lines of the SYNTHE state loaded on ctx
+    (a12_synthe_load must have been called with the same state)."""
+    lines = np.ascontiguousarray(lines, dtype=CPLX_DTYPE)
+    a = ctx._atm(st)
+    ctx._check(ctx.lib.a12_cplx_load(ctx.h, C.byref(a), lines.ctypes.data_as(_vp), C.c_int32(len(lines)),
+                                     C.cast(stark.struct, _vp), C.c_double(wlend), C.c_double(cutoff),
+                                     C.c_int32(ifvac), C.c_float(velshift)), "a12_cplx_load")
+    ctx._cplx = (st.nrhox, st.length, len(lines))
+
+
+def cplx_run(ctx: Context):
+    ctx._check(ctx.lib.a12_cplx_run(ctx.h), "a12_cplx_run")
+
+
+def cplx_fetch(ctx: Context, extents: bool = False):
+    """The last run's buffer (nrhox, length) float32 and, with extents, the
+    (nrhox, nlines, 4) first/last red and blue points."""
+    n, L, nl = ctx._cplx
+    buf = np.zeros((n, L), np.float32)
+    ext = np.zeros((n, max(nl, 1), 4), np.int32) if extents else None
+    ctx._check(ctx.lib.a12_cplx_fetch(ctx.h, buf.ctypes.data_as(_vp),
+                                      None if ext is None else ext.ctypes.data_as(_vp)), "a12_cplx_fetch")
+    return (buf, ext[:, :nl]) if extents else buf
+
+
+def cplx_clear(ctx: Context):
+    ctx._check(ctx.lib.a12_cplx_clear(ctx.h), "a12_cplx_clear")
+
+
+def cplx_ms(ctx: Context):
+    """[plans, convolutions, wing extents, accumulation] of the last run (ms)."""
+    ms = (C.c_float * 4)()
+    ctx._check(ctx.lib.a12_cplx_ms(ctx.h, ms), "a12_cplx_ms")
+    return [float(v) for v in ms]

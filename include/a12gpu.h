@@ -468,6 +468,58 @@ int a12_kapp_ms(a12_ctx *ctx, float *ms);
  * :318-407).  No GPU needed. */
 int a12_kapp_constants(double *out12);
 
+/* ---- SYNTHE complex-profile lines ----------------------------------------- */
+/* nlte_line_t, src/mod_mklinelist.f90:43-62: one complex-profile line as
+ * run_mklinelist keeps it for compute_line_opacity (the fort.19 record;
+ * 60 bytes of fields, 64 with C's trailing alignment). */
+typedef struct {
+    double  wlvac;          /* vacuum wavelength [nm]                           */
+    float   elo;            /* lower-level energy [cm^-1]                       */
+    float   gf;             /* CGF; raw gf for types 1 and >= 4                 */
+    int32_t nblo, nbup;     /* H I: lower / upper principal quantum numbers     */
+    int32_t nelion;         /* species slot (nelem-1)*6 + ion stage, 1-based    */
+    int32_t itype;          /* 2 coronal; 0, 3, <= -3 Voigt; -1 H I, -2 D I;
+                               1 autoionising; >= 4 merged continuum (NLAST)   */
+    int32_t ncon;           /* continuum edge (CONTX row, CONTH for H I)        */
+    int32_t nelionx;        /* CONTX column                                     */
+    float   gammar, gammas, gammaw;  /* damping / (4 pi nu); autoionising:
+                                        width, Shore q, Shore w                 */
+    int32_t nbuff;          /* grid index of the centre (1-based)               */
+    int32_t lim;            /* wing class (not read by the opacity)             */
+} a12_cplx_line;
+
+/* Replaces compute_line_opacity (src/synthe_module.f90:3167-3702, with
+ * hydrogen_line_profile :1032-1980), which SYNTHE's depth loop calls before
+ * the LTE lines (src/synthe.f90:552).  atm: the atmosphere given to
+ * a12_synthe_load (its window, nrhox and mw6 must match the loaded one);
+ * lines in the reference's order; stark: the four Stehle series
+ * (STEHLE_DATA, as for a12_kapp_set_tables); wlend: the window's WLEND;
+ * cutoff: LINE_CUTOFF; ifvac: IFVAC; velshift: the call's velocity shift
+ * (km/s, 0 in the reference's driver).  Copies everything to the device.
+ * A12_ERR_ARG for a hydrogen line with n outside 1..99 or m outside 2..100
+ * (where the reference stops), a merged hydrogen line whose NCON is outside
+ * 1..15, a Voigt line with NCON > 0 outside CONTX (1..26, 1..17), or a
+ * species slot outside 1..mw6.  The lines stay until a12_cplx_clear or the
+ * next a12_synthe_load. */
+int a12_cplx_load(a12_ctx *ctx, const a12_synthe_atm *atm, const a12_cplx_line *lines,
+                  int32_t nlines, const a12_kapp_stark *stark, double wlend, double cutoff,
+                  int32_t ifvac, float velshift);
+/* The complex-line opacity of every depth on the device: BUFFER after the
+ * call at each depth, (nrhox, length) FP32 in the context.  Once loaded,
+ * every later a12_synthe_opacity_run also adds it to the opacity. */
+int a12_cplx_run(a12_ctx *ctx);
+/* Copies the last run's buffer (nrhox, length) and, when extents is not
+ * NULL, the points each (depth, line) reached: (nrhox, nlines, 4) =
+ * first/last red, first/last blue (1-based; empty when first > last; a
+ * merged-continuum edge reports its range as red). */
+int a12_cplx_fetch(a12_ctx *ctx, float *buffer, int32_t *extents);
+/* Drops the loaded lines; later SYNTHE runs hold the LTE lines only. */
+int a12_cplx_clear(a12_ctx *ctx);
+/* Milliseconds of the parts of the last a12_cplx_run (with
+ * a12_set_profiling on): set-up and plans, convolutions, wing extents,
+ * accumulation; -1 when not measured. */
+int a12_cplx_ms(a12_ctx *ctx, float ms[4]);
+
 /* ---- model start ---------------------------------------------------------- */
 /* SCALE_MODEL(TEFF_NEW, LOGG_NEW), src/atlas12_modules.f90:6967-7060, for
  * nmodel models at once (one wavefront each): regrid onto TAUSTD(i) =

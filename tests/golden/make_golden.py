@@ -750,41 +750,80 @@ def gfall_line(wl_nm, loggf, code, e, xj, ep, xjp, gr=0.0, gs=0.0, gw=0.0, nblo=
             f"{0:5d}{0:5d}{0:6d}\n")
 
 
-def cplx_gfall():
-    """Complex-profile lines of every type compute_line_opacity handles
-    (src/synthe_module.f90:3167-3702): H I Balmer lines n = 2 -> 3..30
-    (Halpha as m = n+1, Hbeta as m = n+2, the rest merging into the Balmer
-    continuum, NCON = 2), He I lines with and without a continuum edge,
-    NLTE-flagged metal lines (Mg I, Ca II, Na I), a PRD line, an
-    autoionising Ca I line and a merged-continuum (CONTINUU) Mg I record;
-    sorted by wavelength as gfall is."""
+def _hseries(n, ms, ncon_from, ncon):
+    """H I lines n -> m (gfall code 1.00) with NCON (ISO2) from m >= ncon_from."""
     ryd, elim = 109677.583403, 109678.764
-    e2 = elim - ryd / 4.0
+    el = elim - ryd / n ** 2
     out = []
-    for m in range(3, 31):
+    for m in ms:
         eu = elim - ryd / m ** 2
-        w = 1.0e7 / (eu - e2)
-        f = 0.64 / m ** 3 * (1.0 / 4.0 - 1.0 / m ** 2) ** -3 / 8.0
-        out.append((w, gfall_line(w, float(np.log10(8.0 * f)), 1.00, e2, 0.5, eu, 0.5,
-                                  nblo=2, nbup=m, iso2=(2 if m >= 5 else 0))))
-    # He I (code 2.00): 447.1 (no edge) and 492.2 (NCON 1)
-    out.append((447.1480, gfall_line(447.1480, 0.052, 2.00, 169086.766, 5.0, 191444.482, 7.0,
-                                     nblo=4, nbup=9)))
-    out.append((492.1931, gfall_line(492.1931, -0.435, 2.00, 171134.897, 1.0, 191446.456, 2.0,
-                                     nblo=5, nbup=10, iso2=1)))
-    # NLTE-flagged metals: Mg I b (518.36), Ca II H (396.85), Na I D2 (589.16)
-    out.append((518.3604, gfall_line(518.3604, -0.167, 12.00, 21911.178, 1.0, 41197.403, 1.0,
-                                     nblo=3, nbup=6)))
-    out.append((396.9591, gfall_line(396.9591, -0.166, 20.01, 0.0, 0.5, 25191.51, 0.5,
-                                     nblo=1, nbup=2)))
-    out.append((589.1583, gfall_line(589.1583, 0.108, 11.00, 0.0, 0.5, 16973.366, 1.5,
-                                     nblo=1, nbup=3, iso2=1)))
-    # a PRD line (Fe II), an autoionising Ca I line, a merged continuum (Mg I)
-    out.append((455.0, gfall_line(455.0, -1.2, 26.01, 20000.0, 2.5, 41978.0, 3.5, auto="PRD")))
-    out.append((520.5, gfall_line(520.5, -2.0, 20.00, 20000.0, 1.0, 39212.4, 2.0, gr=13.0, gs=1.5,
-                                  gw=2.0, auto="AUT")))
-    out.append((430.0, gfall_line(430.0, -3.0, 12.00, 35000.0, 0.5, 58255.8, 6.0,
-                                  labelp="CONTINUU")))
+        w = 1.0e7 / (eu - el)
+        # Kramers-like f_nm, large enough that the wings reach the cutoff
+        f = 0.64 / m ** 3 * (1.0 / n ** 2 - 1.0 / m ** 2) ** -3 / n ** 5
+        out.append((w, gfall_line(w, float(np.log10(2 * n * n * f)), 1.00, el, 0.5, eu, 0.5,
+                                  nblo=n, nbup=m, iso2=(ncon if m >= ncon_from else 0))))
+    return out
+
+
+def cplx_gfall(window="vis"):
+    """Complex-profile lines of every type compute_line_opacity handles
+    (src/synthe_module.f90:3167-3702), sorted by wavelength as gfall is.
+    vis (365-660 nm): H I Balmer n = 2 -> 3..30 (Halpha alone, Hbeta with
+    the alpha limit, the rest merging into the Balmer continuum, NCON = 2),
+    D I alpha just before Halpha (the same transition: the reference's
+    per-transition cache serves Halpha with D's convolution), He I lines
+    without an edge, with NCON 1 and inside the tapered stretch below the
+    NCON 5 edge, NLTE-flagged metals (Mg I, Ca II, Na I), a PRD line, an
+    autoionising Ca I line and CONTINUU records: Mg I (NLAST 6), H I (NLAST
+    8) and one of type 2 (coronal).  ir (1450-2400 nm): Brackett n = 4 ->
+    6..30 and Pfund n = 5 -> 13..30 -- the Stehle tables stop at Brackett
+    m = 7, so most take the Peterson/Kurucz path -- and a Si I line.  uv
+    (118-180 nm): Lyman n = 1 -> 2..40 (Lyalpha with its H2 and H2+
+    satellites, Lybeta with the alpha limit, m > 30 past the Stehle tables)
+    and a C I line."""
+    out = []
+    if window == "vis":
+        out += _hseries(2, range(3, 31), 4, 2)
+        # D I alpha: ISO1 = 2 (itype -2); 2 dex weaker than H alpha
+        ryd, elim = 109677.583403, 109678.764
+        e2, e3 = elim - ryd / 4.0, elim - ryd / 9.0
+        f = 0.64 / 27 * (1.0 / 4.0 - 1.0 / 9.0) ** -3 / 32.0
+        out.append((656.2905, gfall_line(656.2905, float(np.log10(8 * f)) - 2.0, 1.00, e2, 0.5, e3, 0.5,
+                                         nblo=2, nbup=3, iso1=2)))
+        # He I (code 2.00): 447.1 (no edge), 492.2 (NCON 1), 376.0 (NCON 5:
+        # the edge at 27175.76 cm^-1 tapers it between wcon and wtail)
+        out.append((447.1480, gfall_line(447.1480, 0.052, 2.00, 169086.766, 5.0, 191444.482, 7.0,
+                                         nblo=4, nbup=9)))
+        out.append((492.1931, gfall_line(492.1931, -0.435, 2.00, 171134.897, 1.0, 191446.456, 2.0,
+                                         nblo=5, nbup=10, iso2=1)))
+        out.append((376.0000, gfall_line(376.0000, 1.5, 2.00, 169086.766, 1.0, 195682.0, 2.0,
+                                         nblo=4, nbup=12, iso2=5)))
+        # NLTE-flagged metals: Mg I b (518.36), Ca II H (396.96), Na I D2 (589.16)
+        out.append((518.3604, gfall_line(518.3604, -0.167, 12.00, 21911.178, 1.0, 41197.403, 1.0,
+                                         nblo=3, nbup=6)))
+        out.append((396.9591, gfall_line(396.9591, -0.166, 20.01, 0.0, 0.5, 25191.51, 0.5,
+                                         nblo=1, nbup=2)))
+        out.append((589.1583, gfall_line(589.1583, 0.108, 11.00, 0.0, 0.5, 16973.366, 1.5,
+                                         nblo=1, nbup=3, iso2=1)))
+        # a PRD line (Fe II), an autoionising Ca I line, CONTINUU records
+        out.append((455.0, gfall_line(455.0, -1.2, 26.01, 20000.0, 2.5, 41978.0, 3.5, auto="PRD")))
+        out.append((520.5, gfall_line(520.5, -2.0, 20.00, 20000.0, 1.0, 39212.4, 2.0, gr=13.0, gs=1.5,
+                                      gw=2.0, auto="AUT")))
+        out.append((430.0, gfall_line(430.0, -3.0, 12.00, 35000.0, 0.5, 58255.8, 6.0,
+                                      labelp="CONTINUU")))
+        out.append((500.0, gfall_line(500.0, -2.5, 1.00, 97492.302, 0.5, 117492.302, 8.0,
+                                      labelp="CONTINUU")))
+        out.append((480.0, gfall_line(480.0, -3.0, 12.00, 35000.0, 0.5, 55833.333, 2.0,
+                                      labelp="CONTINUU")))
+    elif window == "ir":
+        out += _hseries(4, range(6, 31), 6, 4)
+        out += _hseries(5, range(13, 31), 7, 5)
+        out.append((1588.8, gfall_line(1588.8, 0.2, 14.00, 47351.0, 2.0, 53645.0, 3.0, nblo=2, nbup=5)))
+    elif window == "uv":
+        out += _hseries(1, range(2, 41), 3, 1)
+        out.append((165.7, gfall_line(165.7, -0.3, 6.00, 16.4, 1.0, 60369.0, 1.0, nblo=1, nbup=4)))
+    else:
+        raise ValueError(window)
     out.sort()
     return "".join(r for _, r in out)
 
@@ -837,44 +876,51 @@ CPLX_DTYPE = np.dtype([("wlvac", "<f8"), ("elo", "<f4"), ("gf", "<f4"), ("nblo",
 
 
 CPLX_DEPTHS = np.r_[0:72:6, 71]
+# window: (wlbeg, wlend, resolving power, depths kept, file suffix)
+CPLX_WINDOWS = {"vis": (365.0, 660.0, 30000.0, CPLX_DEPTHS, ""),
+                "ir": (1450.0, 2400.0, 30000.0, np.r_[0:72:12, 71], "_ir"),
+                "uv": (118.0, 180.0, 30000.0, np.r_[0:72:12, 71], "_uv")}
 
 
 def golden_cplx():
     """compute_line_opacity (src/synthe_module.f90:3167-3702) on the
-    plain-SYNTHE atmosphere, 400-660 nm at R = 30,000, for the complex lines
-    of cplx_gfall(): the line records as run_mklinelist builds them, the
-    per-depth inputs the driver forms (src/synthe.f90:476-551) and the
-    CONTINUUM and BUFFER arrays after the call, at 13 of the 72 depths; the
-    SYNTHE state of the window for the device path."""
+    plain-SYNTHE atmosphere in three windows at R = 30,000, for the complex
+    lines of cplx_gfall(window): the line records as run_mklinelist builds
+    them, the per-depth inputs the driver forms (src/synthe.f90:476-551) and
+    the CONTINUUM and BUFFER arrays after the call at the kept depths; the
+    SYNTHE state of each window for the device path."""
     atm = wl.scurve_atmosphere()
-    tmp = tempfile.mkdtemp(prefix="a12c_")
-    gpath = os.path.join(tmp, "gfall.dat")
-    with open(gpath, "w") as fh:
-        fh.write(cplx_gfall())
-    root = data_root(tmp, gpath)
-    deck = os.path.join(tmp, "cplx.dat")
-    with open(deck, "w") as fh:
-        fh.write(atm.deck())
-    env = dict(os.environ, ATLAS12=root, A12_CPLX_DUMP=os.path.join(tmp, "cplx.bin"))
-    dump = os.path.join(tmp, "dump.bin")
-    run([os.path.join(REFBIN, "ref_synthe_dump"), deck, "400.0", "660.0", "30000.0", dump], tmp, env)
-    d = trim_edges(read_dump(dump))
-    c = read_cplx_dump(env["A12_CPLX_DUMP"])
-    shutil.rmtree(tmp)
-    nel = sorted(set(int(x) for x in c["lines"]["nelion"]) | {1, 2, 7, 8, 9})
-    save_state(os.path.join(HERE, "cplx_state.npz"), d, nel)
-    js = CPLX_DEPTHS
-    keep = {k: c[k] for k in ("t", "xne", "rho", "hckt", "xnfhe1", "xnfh2")}
-    for k in ("txnxn", "xnfph1", "xnfph2", "continuum", "buffer"):
-        keep[k] = c[k][js]
-    idx = np.array([i - 1 for i in nel], np.int32)
-    for k in ("xnfpel", "dopple", "xnfdop"):
-        keep[k] = c[k][js][:, idx]
-    np.savez_compressed(os.path.join(HERE, "cplx_ref.npz"), lines=c["lines"], depths=js.astype(np.int32),
-                        ion_idx=idx, length=c["length"], cutoff=c["cutoff"], **keep)
-    types = {int(t): int(np.sum(c["lines"]["itype"] == t)) for t in np.unique(c["lines"]["itype"])}
-    return {"cplx": {"nlines": int(c["nlines"]), "length": int(c["length"]), "itypes": types,
-                     "nonzero_points": int(np.count_nonzero(c["buffer"]))}}
+    info = {}
+    for name, (wlb, wle, res, js, suf) in CPLX_WINDOWS.items():
+        tmp = tempfile.mkdtemp(prefix="a12c_")
+        gpath = os.path.join(tmp, "gfall.dat")
+        with open(gpath, "w") as fh:
+            fh.write(cplx_gfall(name))
+        root = data_root(tmp, gpath)
+        deck = os.path.join(tmp, "cplx.dat")
+        with open(deck, "w") as fh:
+            fh.write(atm.deck())
+        env = dict(os.environ, ATLAS12=root, A12_CPLX_DUMP=os.path.join(tmp, "cplx.bin"))
+        dump = os.path.join(tmp, "dump.bin")
+        run([os.path.join(REFBIN, "ref_synthe_dump"), deck, repr(wlb), repr(wle), repr(res), dump], tmp, env)
+        d = trim_edges(read_dump(dump))
+        c = read_cplx_dump(env["A12_CPLX_DUMP"])
+        shutil.rmtree(tmp)
+        nel = sorted(set(int(x) for x in c["lines"]["nelion"]) | {1, 2, 7, 8, 9})
+        save_state(os.path.join(HERE, f"cplx{suf}_state.npz"), d, nel)
+        keep = {k: c[k] for k in ("t", "xne", "rho", "hckt", "xnfhe1", "xnfh2")}
+        for k in ("txnxn", "xnfph1", "xnfph2", "continuum", "buffer"):
+            keep[k] = c[k][js]
+        idx = np.array([i - 1 for i in nel], np.int32)
+        for k in ("xnfpel", "dopple", "xnfdop"):
+            keep[k] = c[k][js][:, idx]
+        np.savez_compressed(os.path.join(HERE, f"cplx{suf}_ref.npz"), lines=c["lines"],
+                            depths=np.asarray(js, np.int32), ion_idx=idx, length=c["length"],
+                            cutoff=c["cutoff"], wlbeg=wlb, wlend=wle, resolu=res, **keep)
+        types = {int(t): int(np.sum(c["lines"]["itype"] == t)) for t in np.unique(c["lines"]["itype"])}
+        info[name] = {"nlines": int(c["nlines"]), "length": int(c["length"]), "itypes": types,
+                      "nonzero_points": int(np.count_nonzero(c["buffer"]))}
+    return {"cplx": info}
 
 
 def golden_atlas_small():
