@@ -102,6 +102,60 @@ typedef struct {
 } or_tcorr_in;
 void or_tcorr3(const or_tcorr_in *in, double *rdabh, double *oldt1, double *out);
 
+/* ROSSTAB's table (module state ROSSTAB_*, src/atlas12_modules.f90:
+ * 1779-1785; maxtab = kw * 60).  Arrays of maxtab entries. */
+typedef struct {
+    int nross, maxtab;
+    double zerot, slopet, zerop, slopep;
+    double *tabt, *tabp, *ross;
+} or_rosstab;
+/* ROSSTAB(0, 0, 0), the store (:6003-6042); returns the entries added. */
+int or_rosstab_store(or_rosstab *tb, int n, const double *t, const double *p,
+                     const double *abross);
+/* ROSSTAB(T, P, V), the lookup (:6047-6268) in mode irosstab (1, 2, 3). */
+double or_rosstab_lookup(const or_rosstab *tb, int irosstab, double temp, double pressure);
+/* TTAUP (:6303-6523); returns the number of depths whose corrector hit
+ * its cap (the best iterate is used there). */
+int or_ttaup(const or_rosstab *tb, int irosstab, const double *t_in, const double *tau,
+             double *abstd, double *ptotal, double *p_out, const double *prad_in,
+             const double *pturb_in, double grav, int numtau);
+
+/* CZ_CONSTRUCT's SAVE state (:3054-3069) and the module flags it sets. */
+typedef struct {
+    int prev_engaged, jc_sticky, nbot_bad, run_engaged, run_jc;
+    double last_dtmax, rel_err;
+} or_czc_state;
+/* The module arrays CZ_CONSTRUCT and CZC_DELREQ_LAYER read (n each). */
+typedef struct {
+    int n, ifconv, iter, use4pt, use_constructor, irosstab;
+    double flux, mixlth, grav;
+    const double *flxcnv, *flxrad, *dltdlp, *grdadb, *flxerr, *t, *rhox, *ptotal, *rho,
+        *heatcp, *dlrdlt, *p, *vturb, *abross, *hscale;
+    const or_rosstab *tab;
+} or_czc_in;
+/* CZ_CONSTRUCT(T1), :3012-3287; returns 1 when engaged (T1 changed). */
+int or_cz_construct(const or_czc_in *in, or_czc_state *st, double *t1);
+
+/* TCORR(3) sections (G2)-(K), :2816-2967.  Read-only inputs (n each) and
+ * the module arrays the sections update in place. */
+typedef struct {
+    int n, ifconv, iter, use4pt, use_constructor, irosstab, j1smooth, j2smooth;
+    double flux, mixlth, grav, tau1lg, steplg, pradk0, tfloor, wtjm1, wtj, wtjp1;
+    const double *flxcnv, *flxrad, *dltdlp, *grdadb, *flxerr, *ptotal, *rho, *heatcp,
+        *dlrdlt, *hscale;
+    const or_rosstab *tab;
+} or_tcorr_fin_in;
+typedef struct {
+    double *tauros, *rhox, *t, *p, *xne, *abross, *prad, *vturb, *bmin, *pturb;
+    const double *accrad;                 /* remapped by (K) but not kept  */
+    double *bhyd;                         /* (n, 6) BHYD, J fastest        */
+    double *tk, *hkt, *hckt, *tkev, *tlog, *pradk;  /* out                 */
+} or_tcorr_fin_io;
+/* t1: TCORR's T1 after section (F) in, after (G2) out; drhox_out (n) or
+ * NULL.  Returns the TTAUP depths that did not converge. */
+int or_tcorr3_finish(const or_tcorr_fin_in *in, or_tcorr_fin_io *io, or_czc_state *cz,
+                     double *t1, double *drhox_out);
+
 /* NMOLEC's molecule table (READMOL): 1-based as in the reference.  locj
  * (nummol + 1) start of each entry's components in kcomps; kcomps the
  * equation index of each component (nequa + 1 = electron, divides);

@@ -464,6 +464,158 @@ def tcorr3(z, rdabh, oldt1=None, iter_=None):
     return {k: out[i] for i, k in enumerate(names)}, rd, ot
 
 
+ROSSTAB_MAXTAB = 80 * 60  # kw * 60, src/atlas12_modules.f90:1779
+
+
+class OrRosstab(C.Structure):
+    _fields_ = [("nross", C.c_int), ("maxtab", C.c_int), ("zerot", C.c_double),
+                ("slopet", C.c_double), ("zerop", C.c_double), ("slopep", C.c_double),
+                ("tabt", P_D), ("tabp", P_D), ("ross", P_D)]
+
+
+class Rosstab:
+    """ROSSTAB's table (the module state ROSSTAB_*): store and lookup."""
+
+    def __init__(self, zero=None, tab=None, maxtab=ROSSTAB_MAXTAB):
+        self.a = np.zeros((3, maxtab))
+        self.s = OrRosstab()
+        self.s.maxtab = maxtab
+        self.s.nross = 0
+        if tab is not None:
+            tab = np.asarray(tab, np.float64)
+            self.a[:, :tab.shape[1]] = tab
+            self.s.nross = tab.shape[1]
+            self.s.zerot, self.s.slopet, self.s.zerop, self.s.slopep = (float(x) for x in zero)
+        self.s.tabt, self.s.tabp, self.s.ross = (_p(self.a[i]) for i in range(3))
+
+    @property
+    def nross(self):
+        return int(self.s.nross)
+
+    def table(self):
+        return self.a[:, :self.nross].copy()
+
+    def zero(self):
+        return np.array([self.s.zerot, self.s.slopet, self.s.zerop, self.s.slopep])
+
+    def store(self, t, p, abross):
+        t, p, ab = _f64(t), _f64(p), _f64(abross)
+        return lib().or_rosstab_store(C.byref(self.s), len(t), _p(t), _p(p), _p(ab))
+
+    def lookup(self, temp, pressure, mode=1):
+        L = lib()
+        L.or_rosstab_lookup.restype = C.c_double
+        L.or_rosstab_lookup.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
+        return np.array([L.or_rosstab_lookup(C.addressof(self.s), int(mode), float(t), float(q))
+                         for t, q in zip(np.atleast_1d(temp), np.atleast_1d(pressure))])
+
+
+def ttaup(tab, t, tau, prad, pturb, grav, mode=1):
+    """TTAUP on a Rosstab.  Returns (abstd, ptotal, p_gas, n_unconverged)."""
+    n = len(t)
+    t, tau, prad, pturb = _f64(t), _f64(tau), _f64(prad), _f64(pturb)
+    ab, pt, pg = np.zeros(n), np.zeros(n), np.zeros(n)
+    L = lib()
+    L.or_ttaup.argtypes = [C.c_void_p, C.c_int] + [P_D] * 7 + [C.c_double, C.c_int]
+    nf = L.or_ttaup(C.addressof(tab.s), int(mode), _p(t), _p(tau), _p(ab), _p(pt), _p(pg),
+                    _p(prad), _p(pturb), float(grav), n)
+    return ab, pt, pg, nf
+
+
+class OrCzcState(C.Structure):
+    _fields_ = [("prev_engaged", C.c_int), ("jc_sticky", C.c_int), ("nbot_bad", C.c_int),
+                ("run_engaged", C.c_int), ("run_jc", C.c_int), ("last_dtmax", C.c_double),
+                ("rel_err", C.c_double)]
+
+
+CZ_FIELDS = ("flxcnv", "flxrad", "dltdlp", "grdadb", "flxerr", "t", "rhox", "ptotal", "rho",
+             "heatcp", "dlrdlt", "p", "vturb", "abross", "hscale")
+
+
+class OrCzcIn(C.Structure):
+    _fields_ = [(k, C.c_int) for k in ("n", "ifconv", "iter", "use4pt", "use_constructor",
+                                       "irosstab")] + \
+               [(k, C.c_double) for k in ("flux", "mixlth", "grav")] + \
+               [(k, P_D) for k in CZ_FIELDS] + [("tab", C.c_void_p)]
+
+
+def czc_state():
+    st = OrCzcState()
+    st.last_dtmax = 1.0e30
+    return st
+
+
+def cz_construct(tab, z, t1, state, it, ifconv=1, use4pt=1, mode=1):
+    """CZ_CONSTRUCT(T1) on arrays z[k] for k in CZ_FIELDS and scalars
+    z["flux"], z["mixlth"], z["grav"]; state (czc_state()) is its SAVE
+    state, updated.  Returns (t1 out, engaged)."""
+    keep = {k: _f64(z[k]) for k in CZ_FIELDS}
+    s = OrCzcIn()
+    s.n, s.ifconv, s.iter, s.use4pt, s.use_constructor, s.irosstab = (
+        len(keep["t"]), int(ifconv), int(it), int(use4pt), 1, int(mode))
+    s.flux, s.mixlth, s.grav = float(z["flux"]), float(z["mixlth"]), float(z["grav"])
+    for k in CZ_FIELDS:
+        setattr(s, k, _p(keep[k]))
+    s.tab = C.addressof(tab.s)
+    out = _f64(np.array(t1, np.float64).copy())
+    L = lib()
+    L.or_cz_construct.argtypes = [C.c_void_p, C.c_void_p, P_D]
+    e = L.or_cz_construct(C.byref(s), C.byref(state), _p(out))
+    return out, e
+
+
+FIN_IN = ("flxcnv", "flxrad", "dltdlp", "grdadb", "flxerr", "ptotal", "rho", "heatcp",
+          "dlrdlt", "hscale")
+FIN_IO = ("tauros", "rhox", "t", "p", "xne", "abross", "prad", "vturb", "bmin", "pturb")
+FIN_OUT = ("tk", "hkt", "hckt", "tkev", "tlog", "pradk")
+
+
+class OrTcorrFinIn(C.Structure):
+    _fields_ = [(k, C.c_int) for k in ("n", "ifconv", "iter", "use4pt", "use_constructor",
+                                       "irosstab", "j1smooth", "j2smooth")] + \
+               [(k, C.c_double) for k in ("flux", "mixlth", "grav", "tau1lg", "steplg", "pradk0",
+                                          "tfloor", "wtjm1", "wtj", "wtjp1")] + \
+               [(k, P_D) for k in FIN_IN] + [("tab", C.c_void_p)]
+
+
+class OrTcorrFinIo(C.Structure):
+    _fields_ = [(k, P_D) for k in FIN_IO] + [("accrad", P_D), ("bhyd", P_D)] + \
+               [(k, P_D) for k in FIN_OUT]
+
+
+def tcorr3_finish(tab, sc, arr, t1, state, mode=1):
+    """TCORR(3) sections (G2)-(K).  sc: dict of the scalars of OrTcorrFinIn;
+    arr: dict with FIN_IN, FIN_IO, "accrad" (n) and "bhyd" (6, n).  Returns
+    (out dict: FIN_IO + FIN_OUT + bhyd after the remap, t1 after (G2),
+    drhox, TTAUP depths not converged)."""
+    n = len(arr["t"])
+    s = OrTcorrFinIn()
+    for k in ("n", "ifconv", "iter", "use4pt", "use_constructor", "j1smooth", "j2smooth"):
+        setattr(s, k, int(sc.get(k, n if k == "n" else 0)))
+    s.irosstab = int(mode)
+    for k in ("flux", "mixlth", "grav", "tau1lg", "steplg", "pradk0", "tfloor", "wtjm1", "wtj",
+              "wtjp1"):
+        setattr(s, k, float(sc[k]))
+    keep = {k: _f64(arr[k]) for k in FIN_IN}
+    for k in FIN_IN:
+        setattr(s, k, _p(keep[k]))
+    s.tab = C.addressof(tab.s)
+    io = OrTcorrFinIo()
+    out = {k: _f64(np.array(arr[k], np.float64).copy()) for k in FIN_IO}
+    out.update({k: np.zeros(n) for k in FIN_OUT})
+    acc = _f64(arr["accrad"])
+    out["bhyd"] = _f64(np.array(arr["bhyd"], np.float64).reshape(6, n).copy())
+    for k in FIN_IO + FIN_OUT:
+        setattr(io, k, _p(out[k]))
+    io.accrad, io.bhyd = _p(acc), _p(out["bhyd"])
+    t1o = _f64(np.array(t1, np.float64).copy())
+    drhox = np.zeros(n)
+    L = lib()
+    L.or_tcorr3_finish.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, P_D, P_D]
+    nf = L.or_tcorr3_finish(C.byref(s), C.byref(io), C.byref(state), _p(t1o), _p(drhox))
+    return out, t1o, drhox, nf
+
+
 class OrMoltab(C.Structure):
     _fields_ = [("nequa", C.c_int), ("nummol", C.c_int), ("locj", C.POINTER(C.c_int)),
                 ("kcomps", C.POINTER(C.c_int)), ("idequa", C.POINTER(C.c_int))]

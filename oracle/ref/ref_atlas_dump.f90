@@ -20,6 +20,10 @@ PROGRAM ref_atlas_dump
   CHARACTER(LEN=512) :: deck, fin, fout, arg, envval, seldir
   INTEGER :: envlen, envstat, u, nustart_in, nuhi_in, nl, nu, j, nel, i
   REAL(8) :: freq15_l, xmax_l
+  ! TCORR(3) sections G2-K, TTAUP, ROSSTAB and CZ_CONSTRUCT probes
+  INTEGER :: uf, idum_l, k, irt, nq, iq
+  REAL(8) :: rx_l, taustd_l(kw), tn_l(kw), pr_l(kw), ab_l(kw), pt_l(kw), pg_l(kw), t1_l(kw)
+  REAL(8) :: t0_l(kw), p0_l(kw), qt_l(kw * 12), qp_l(kw * 12), qr_l(kw * 12)
 
   CALL GET_COMMAND_ARGUMENT(1, deck)
   CALL GET_COMMAND_ARGUMENT(2, arg); READ(arg, *) nustart_in
@@ -159,11 +163,17 @@ PROGRAM ref_atlas_dump
   CLOSE(u)
 
   ! With A12_TCORR_OUT set: the driver's end of iteration up to the
-  ! temperature correction (src/atlas12c.f90:582-587), the state TCORR(3)
-  ! reads, and TCORR(3) itself with its per-iteration table on unit 66.
+  ! temperature correction (src/atlas12c.f90:581-587: ROSS(3), the ROSSTAB
+  ! store, RADIAP(3), COMPUTE_HEIGHT, CONVEC), the state TCORR(3) reads,
+  ! and TCORR(3) itself with its per-iteration table on unit 66.  The file
+  ! <A12_TCORR_OUT>.fin gets what the sections after the table (G2-K) read
+  ! and what TCORR(3) leaves in the module arrays, TTAUP called on its own
+  ! on the same state, ROSSTAB lookups in its three modes, and, on a deck
+  ! that convects, CZ_CONSTRUCT on its own on inputs planted to engage it.
   CALL GET_ENVIRONMENT_VARIABLE('A12_TCORR_OUT', envval, envlen, envstat)
   IF (envstat .EQ. 0 .AND. envlen .GT. 0) THEN
     CALL ROSS(3, 0.0D0)
+    rx_l = ROSSTAB(0.0D0, 0.0D0, 0.0D0)
     CALL RADIAP(3, 0.0D0)
     CALL COMPUTE_HEIGHT
     IF (IFPRES .EQ. 1 .AND. IFCONV .EQ. 1) CALL CONVEC(.FALSE.)
@@ -175,11 +185,83 @@ PROGRAM ref_atlas_dump
              RHO(1:NRHOX), DLRDLT(1:NRHOX), HEATCP(1:NRHOX), HSCALE(1:NRHOX), &
              FLXRAD(1:NRHOX), RJMINS(1:NRHOX)
     CLOSE(u)
+    OPEN(NEWUNIT=uf, FILE=TRIM(envval)//'.fin', ACCESS='STREAM', FORM='UNFORMATTED', &
+         STATUS='REPLACE')
+    WRITE(uf) ROSSTAB_NROSS, IROSSTAB, J1SMOOTH, J2SMOOTH, NUMITS, ITEMP
+    WRITE(uf) GRAV, TAU1LG, STEPLG, PRADK0, TFLOOR_ATM, WTJM1, WTJ, WTJP1
+    WRITE(uf) ROSSTAB_ZEROT, ROSSTAB_SLOPET, ROSSTAB_ZEROP, ROSSTAB_SLOPEP
+    WRITE(uf) ROSSTAB_TABT(1:ROSSTAB_NROSS), ROSSTAB_TABP(1:ROSSTAB_NROSS), &
+              ROSSTAB_ROSS(1:ROSSTAB_NROSS)
+    WRITE(uf) P(1:NRHOX), XNE(1:NRHOX), PRAD(1:NRHOX), PTURB(1:NRHOX), VTURB(1:NRHOX), &
+              BMIN(1:NRHOX), ACCRAD(1:NRHOX), BHYD(1:NRHOX, 1:6)
+    ! TTAUP on its own: T and PRAD mapped onto TAUSTD as section (H) maps them
+    DO j = 1, NRHOX
+      taustd_l(j) = 10.0D0**(TAU1LG + (j - 1) * STEPLG)
+    END DO
+    idum_l = MAP1(TAUROS, T, NRHOX, taustd_l, tn_l, NRHOX)
+    idum_l = MAP1(TAUROS, PRAD, NRHOX, taustd_l, pr_l, NRHOX)
+    CALL TTAUP(tn_l, taustd_l, ab_l, pt_l, pg_l, pr_l, PTURB, VTURB, GRAV, NRHOX)
+    WRITE(uf) tn_l(1:NRHOX), pr_l(1:NRHOX), ab_l(1:NRHOX), pt_l(1:NRHOX), pg_l(1:NRHOX)
     OPEN(UNIT=66, FILE=TRIM(envval)//'.iter', STATUS='REPLACE')
     IFCORR = 1
     IFPRNT(ITER) = 1
     CALL TCORR(3, 0.0D0)
     CLOSE(66)
+    ! What TCORR(3) leaves behind (sections I-K)
+    WRITE(uf) RHOX(1:NRHOX), T(1:NRHOX), TK(1:NRHOX), HKT(1:NRHOX), HCKT(1:NRHOX), &
+              TKEV(1:NRHOX), TLOG(1:NRHOX), P(1:NRHOX), XNE(1:NRHOX), ABROSS(1:NRHOX), &
+              PRAD(1:NRHOX), PRADK(1:NRHOX), VTURB(1:NRHOX), BMIN(1:NRHOX), PTURB(1:NRHOX), &
+              TAUROS(1:NRHOX), BHYD(1:NRHOX, 1:6), FLXERR(1:NRHOX)
+    WRITE(uf) ITEMP, MERGE(1, 0, CZC_RUN_ENGAGED), CZC_RUN_JC
+    ! ROSSTAB lookups: a second profile stored (the corrected structure), then
+    ! queries around every layer of the first, in modes 1, 2 and 3
+    t0_l(1:NRHOX) = tn_l(1:NRHOX)
+    p0_l(1:NRHOX) = pg_l(1:NRHOX)
+    rx_l = ROSSTAB(0.0D0, 0.0D0, 0.0D0)
+    nq = 0
+    DO j = 1, NRHOX
+      DO k = 1, 12
+        nq = nq + 1
+        qt_l(nq) = t0_l(j) * (0.94D0 + 0.01D0 * k)
+        qp_l(nq) = p0_l(j) * 10.0D0**(0.3D0 * MOD(k * 7, 5) - 0.6D0)
+      END DO
+    END DO
+    WRITE(uf) ROSSTAB_NROSS, nq
+    WRITE(uf) ROSSTAB_TABT(1:ROSSTAB_NROSS), ROSSTAB_TABP(1:ROSSTAB_NROSS), &
+              ROSSTAB_ROSS(1:ROSSTAB_NROSS)
+    WRITE(uf) qt_l(1:nq), qp_l(1:nq)
+    DO irt = 1, 3
+      IROSSTAB = irt
+      DO iq = 1, nq
+        qr_l(iq) = ROSSTAB(qt_l(iq), qp_l(iq), 0.0D0)
+      END DO
+      WRITE(uf) qr_l(1:nq)
+    END DO
+    IROSSTAB = 1
+    ! CZ_CONSTRUCT on its own (IFCONV = 1), on planted inputs that engage
+    ! it: convective flux 0.9 FLUX in the deepest 24 layers (they pass the
+    ! efficiency gate; their nabla - nabla_ad is CONVEC's), and a flux
+    ! error of 8, 4, then 0.5 % over three calls (engage, stay engaged,
+    ! release), T1 = 0 on entry to each.
+    IF (IFCONV .EQ. 1) THEN
+      FLXCNV(NRHOX - 23:NRHOX) = 0.9D0 * FLUX
+      WRITE(uf) 1
+      WRITE(uf) FLXCNV(1:NRHOX), FLXRAD(1:NRHOX), DLTDLP(1:NRHOX), GRDADB(1:NRHOX), &
+                T(1:NRHOX), RHOX(1:NRHOX), PTOTAL(1:NRHOX), RHO(1:NRHOX), HEATCP(1:NRHOX), &
+                DLRDLT(1:NRHOX), P(1:NRHOX), VTURB(1:NRHOX), ABROSS(1:NRHOX), &
+                HSCALE(1:NRHOX)
+      WRITE(uf) GRAV, FLUX, MIXLTH, MERGE(1, 0, USE_4POINT_MLT)
+      DO k = 1, 3
+        ITER = k
+        FLXERR(1:NRHOX) = 8.0D0 / 2.0D0**(2 * (k - 1) - MERGE(1, 0, k .EQ. 2))
+        t1_l(1:NRHOX) = 0.0D0
+        CALL CZ_CONSTRUCT(t1_l)
+        WRITE(uf) FLXERR(1:NRHOX), t1_l(1:NRHOX), MERGE(1, 0, CZC_RUN_ENGAGED), CZC_RUN_JC
+      END DO
+    ELSE
+      WRITE(uf) 0
+    END IF
+    CLOSE(uf)
   END IF
 
   ! With A12_KAPP_DUMP and A12_KAPP_WAVES set: KAPP's inputs (the depth

@@ -500,6 +500,7 @@ def atlas_case(atm, nustart, nuhi, linedata, raw=None, tcorr=False, kapp_waves=N
     if tcorr:
         d["tcorr_in"] = read_tcorr_in(tco)
         d["tcorr_iter"] = read_iter_table(tco + ".iter", d["tcorr_in"]["n"])
+        d["tcorr_fin"] = read_tcorr_fin(tco + ".fin", d["tcorr_in"]["n"])
     if raw is not None:
         with open(out + ".sel", "rb") as fh:
             buf = fh.read()
@@ -546,6 +547,77 @@ def read_iter_table(path, n):
     return {k: a[:, i] for i, k in enumerate(ITER_COLS)}
 
 
+class _Stream:
+    """Sequential reader of a Fortran stream-access file."""
+
+    def __init__(self, path):
+        self.b = open(path, "rb").read()
+        self.o = 0
+
+    def i4(self, cnt=1):
+        a = np.frombuffer(self.b, "<i4", cnt, self.o).copy()
+        self.o += 4 * cnt
+        return a
+
+    def f8(self, cnt=1):
+        a = np.frombuffer(self.b, "<f8", cnt, self.o).copy()
+        self.o += 8 * cnt
+        return a
+
+
+FIN_BEFORE = ("p", "xne", "prad", "pturb", "vturb", "bmin", "accrad")
+FIN_AFTER = ("rhox", "t", "tk", "hkt", "hckt", "tkev", "tlog", "p", "xne", "abross", "prad",
+             "pradk", "vturb", "bmin", "pturb", "tauros")
+CZ_IN = ("flxcnv", "flxrad", "dltdlp", "grdadb", "t", "rhox", "ptotal", "rho", "heatcp",
+         "dlrdlt", "p", "vturb", "abross", "hscale")
+
+
+def read_tcorr_fin(path, n):
+    """ref_atlas_dump's <A12_TCORR_OUT>.fin: the inputs of TCORR(3)'s sections
+    G2-K, TTAUP on its own, the module state TCORR(3) leaves, ROSSTAB lookups
+    in modes 1-3 and (decks that convect) three CZ_CONSTRUCT calls."""
+    r = _Stream(path)
+    d = {}
+    nross, irosstab, j1s, j2s, numits, itemp = (int(x) for x in r.i4(6))
+    d.update(nross=nross, irosstab=irosstab, j1smooth=j1s, j2smooth=j2s, numits=numits,
+             itemp=itemp)
+    (d["grav"], d["tau1lg"], d["steplg"], d["pradk0"], d["tfloor"], d["wtjm1"], d["wtj"],
+     d["wtjp1"]) = r.f8(8)
+    d["rt_zero"] = r.f8(4)  # ZEROT, SLOPET, ZEROP, SLOPEP
+    d["rt_tab"] = r.f8(3 * nross).reshape(3, nross)  # TABT, TABP, ROSS
+    for k in FIN_BEFORE:
+        d["in_" + k] = r.f8(n)
+    d["in_bhyd"] = r.f8(6 * n).reshape(6, n)
+    for k in ("tt_t", "tt_prad", "tt_ab", "tt_ptot", "tt_p"):
+        d[k] = r.f8(n)
+    for k in FIN_AFTER:
+        d["out_" + k] = r.f8(n)
+    d["out_bhyd"] = r.f8(6 * n).reshape(6, n)
+    d["out_flxerr"] = r.f8(n)
+    d["out_itemp"], d["out_czc_engaged"], d["out_czc_jc"] = (int(x) for x in r.i4(3))
+    nross2, nq = (int(x) for x in r.i4(2))
+    d["lk_tab"] = r.f8(3 * nross2).reshape(3, nross2)
+    d["lk_q"] = r.f8(2 * nq).reshape(2, nq)
+    d["lk_r"] = r.f8(3 * nq).reshape(3, nq)
+    d["cz"] = int(r.i4(1)[0])
+    if d["cz"]:
+        for k in CZ_IN:
+            d["cz_" + k] = r.f8(n)
+        d["cz_grav"], d["cz_flux"], d["cz_mixlth"] = r.f8(3)
+        d["cz_4pt"] = int(r.i4(1)[0])
+        errs, t1s, eng, jcs = [], [], [], []
+        for _ in range(3):
+            errs.append(r.f8(n))
+            t1s.append(r.f8(n))
+            e, jc = r.i4(2)
+            eng.append(int(e))
+            jcs.append(int(jc))
+        d["cz_flxerr"], d["cz_t1"] = np.array(errs), np.array(t1s)
+        d["cz_engaged"], d["cz_jc"] = np.array(eng, np.int32), np.array(jcs, np.int32)
+    assert r.o == len(r.b), (r.o, len(r.b))
+    return d
+
+
 def golden_tcorr():
     """TCORR(3) after a full 30,000-frequency pass (100,000 raw lines through
     SELECTLINES) on the plain-SYNTHE atmosphere: the state it reads, RDABH
@@ -567,6 +639,7 @@ def golden_tcorr():
     ti, tab = d["tcorr_in"], d["tcorr_iter"]
     keep = {k: ti[k] for k in TCORR_FIELDS}
     keep.update({"iter_" + k: v for k, v in tab.items()})
+    keep.update({"fin_" + k: v for k, v in d["tcorr_fin"].items()})
     np.savez_compressed(os.path.join(HERE, "tcorr_ref.npz"), n=ti["n"], ifconv=ti["ifconv"],
                         iter=ti["iter"], flux=ti["flux"], teff=ti["teff"], mixlth=ti["mixlth"],
                         rdabh=acc["rdabh"], rdiagj=acc["rdiagj"], **keep)
@@ -596,6 +669,7 @@ def golden_tcorr_conv():
     assert ti["ifconv"] == 1 and np.sum(ti["flxcnv"] > 0) > 5, "no convective layers"
     keep = {k: ti[k] for k in TCORR_FIELDS}
     keep.update({"iter_" + k: v for k, v in tab.items()})
+    keep.update({"fin_" + k: v for k, v in d["tcorr_fin"].items()})
     np.savez_compressed(os.path.join(HERE, "tcorr_conv_ref.npz"), n=ti["n"],
                         ifconv=ti["ifconv"], iter=ti["iter"], flux=ti["flux"], teff=ti["teff"],
                         mixlth=ti["mixlth"], rdabh=acc["rdabh"], rdiagj=acc["rdiagj"], **keep)
