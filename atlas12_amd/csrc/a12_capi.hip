@@ -63,6 +63,9 @@ int a12_ctx_destroy(a12_ctx *ctx)
     for (DevBuf *b : mbufs) b->release();
     ctx->mol.tab.release();
     ctx->mol.io.release();
+    ctx->ross.tab.release();
+    ctx->ross.zp.release();
+    ctx->ross.io.release();
     for (DevBuf *b : abufs) b->release();
     for (DevBuf *b : bufs) b->release();
     a12i_mlp_free(ctx);
@@ -509,6 +512,67 @@ int a12_atlas_tcorr3(a12_ctx *ctx, const a12_tcorr_in *in, double *rdabh, double
     if (!ctx_ok(ctx)) return A12_ERR_ARG;
     A12_HIP(hipSetDevice(ctx->device));
     return a12i_atlas_tcorr3(ctx, in, rdabh, oldt1, out);
+}
+
+int a12_rosstab_reset(a12_ctx *ctx)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    return a12i_rosstab_reset(ctx);
+}
+
+int a12_rosstab_store(a12_ctx *ctx, int32_t nrhox, const double *t, const double *p,
+                      const double *abross)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_rosstab_store(ctx, nrhox, t, p, abross);
+}
+
+int a12_rosstab_load(a12_ctx *ctx, int32_t nross, const double *zero, const double *tab)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_rosstab_load(ctx, nross, zero, tab);
+}
+
+int a12_rosstab_fetch(a12_ctx *ctx, int32_t *nross, double *zero, double *tab)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_rosstab_fetch(ctx, nross, zero, tab);
+}
+
+int a12_rosstab_lookup(a12_ctx *ctx, int32_t mode, int32_t nq, const double *temp,
+                       const double *pressure, double *kappa)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_rosstab_lookup(ctx, mode, nq, temp, pressure, kappa);
+}
+
+int a12_ttaup(a12_ctx *ctx, int32_t irosstab, int32_t n, const double *t, const double *tau,
+              const double *prad, const double *pturb, double grav, double *abstd,
+              double *ptotal, double *p, int32_t *nfail)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_ttaup(ctx, irosstab, n, t, tau, prad, pturb, grav, abstd, ptotal, p, nfail);
+}
+
+int a12_cz_construct(a12_ctx *ctx, const a12_czc_in *in, a12_czc_state *st, double *t1,
+                     int32_t *engaged)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_cz_construct(ctx, in, st, t1, engaged);
+}
+
+int a12_atlas_tcorr3_finish(a12_ctx *ctx, const a12_tcorr_fin_in *in, a12_tcorr_fin_io *io,
+                            a12_czc_state *st, double *t1, double *drhox, int32_t *nfail)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_tcorr3_finish(ctx, in, io, st, t1, drhox, nfail);
 }
 
 int a12_molec_set_table(a12_ctx *ctx, const a12_moltab *tab)

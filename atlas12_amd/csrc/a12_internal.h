@@ -99,6 +99,15 @@ struct ModelDev {
     DevBuf ft, taustd, par, t, ot, rd, dep, part, acc, hnu, tin, tout, conv, hist, ctl;
 };
 
+// ROSSTAB's table (a12_rosstab_*): tab (3, A12_ROSSTAB_MAXTAB) TABT, TABP,
+// ROSS and zp (4) ZEROT, SLOPET, ZEROP, SLOPEP on the device; nross on the
+// host (every store's count is known there); io: staging of the TTAUP,
+// CZ_CONSTRUCT and TCORR(3)-finish calls.
+struct RossDev {
+    DevBuf tab, zp, io;
+    int nross = 0;
+};
+
 struct MlpDev;    // a12_mlp.hip
 struct KappDev;   // a12_kapp.hip
 struct CplxDev;   // a12_cplx.hip
@@ -115,6 +124,7 @@ struct a12_ctx {
     SelectDev sel;
     MolDev mol;
     ModelDev mdl;
+    RossDev ross;
     MlpDev *mlp = nullptr;
     KappDev *kapp = nullptr;
     CplxDev *cplx = nullptr;
@@ -220,6 +230,20 @@ int a12i_synthe_transfer_launch(a12_ctx *ctx, hipStream_t st, bool f64, int L, i
                                 double *hcont);
 int a12i_atlas_tcorr3(a12_ctx *ctx, const a12_tcorr_in *in, double *rdabh, double *oldt1,
                       double *out);
+int a12i_rosstab_reset(a12_ctx *ctx);
+int a12i_rosstab_store(a12_ctx *ctx, int n, const double *t, const double *p,
+                       const double *abross);
+int a12i_rosstab_load(a12_ctx *ctx, int nross, const double *zero, const double *tab);
+int a12i_rosstab_fetch(a12_ctx *ctx, int32_t *nross, double *zero, double *tab);
+int a12i_rosstab_lookup(a12_ctx *ctx, int mode, int nq, const double *temp,
+                        const double *pressure, double *kappa);
+int a12i_ttaup(a12_ctx *ctx, int mode, int n, const double *t, const double *tau,
+               const double *prad, const double *pturb, double grav, double *abstd,
+               double *ptotal, double *p, int32_t *nfail);
+int a12i_cz_construct(a12_ctx *ctx, const a12_czc_in *in, a12_czc_state *st, double *t1,
+                      int32_t *engaged);
+int a12i_tcorr3_finish(a12_ctx *ctx, const a12_tcorr_fin_in *in, a12_tcorr_fin_io *io,
+                       a12_czc_state *st, double *t1, double *drhox, int32_t *nfail);
 int a12i_solvit_batched(a12_ctx *ctx, int n, int nsys, const double *a, const double *b,
                         double *x);
 int a12i_model_converge(a12_ctx *ctx, int nmodel, const a12_model_par *par,

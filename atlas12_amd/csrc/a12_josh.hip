@@ -1009,6 +1009,680 @@ __global__ __launch_bounds__(64) void k_tcorr3(int n, int ifconv, int iter, doub
 }
 
 // --------------------------------------------------------------------------
+// The end of TCORR(3): ROSSTAB (src/atlas12_modules.f90:5937-6268), TTAUP
+// (:6303-6523), CZ_CONSTRUCT (:3012-3358) and sections (G2)-(K) (:2816-2967).
+// One wavefront.  The sequential parts (TTAUP's march, CZ_CONSTRUCT's
+// interval recurrence, the monotonicity sweep) run wave-uniform; ROSSTAB's
+// mode-1 search over the table and every per-layer step (MAP1, DERIV, the
+// gates, the remap) are spread over the lanes.  Every expression keeps the
+// reference's order of operations.
+
+// ROSSTAB's table on the device: tab (3, maxtab) TABT, TABP, ROSS; zp
+// (4) ZEROT, SLOPET, ZEROP, SLOPEP (set by the first store on the device).
+struct RossTab {
+    const double *tab;
+    const double *zp;
+    int nross, maxtab;
+};
+
+// MODE 2, LOOKUP.  Mode 1: the nearest entry of each (T, P) quadrant is
+// the minimum of (radius^2, index) over the table -- the serial scan's
+// strict "<" keeps the first of equal radii, so ties go to the lower index
+// -- taken per lane over a strided share, then over the wave.  Modes 2 and
+// 3 keep the serial scan's replacement order (their sums run over the
+// slots in that order), in lane 0.
+__device__ double rosstab_lookup_wave(const RossTab &tb, int mode, double temp, double pressure)
+{
+#pragma clang fp contract(off)
+    const int ln = lane_id();
+    const double *tabt = tb.tab, *tabp = tb.tab + tb.maxtab, *ross = tb.tab + 2 * tb.maxtab;
+    const double templog = (log10(temp) - tb.zp[0]) / tb.zp[1];
+    const double presslog = (log10(pressure) - tb.zp[2]) / tb.zp[3];
+    double r = 0.0;
+    if (mode == 1) {
+        double rm[4] = {1.0e30, 1.0e30, 1.0e30, 1.0e30};  // pp, pm, mp, mm
+        int ix[4] = {0, 0, 0, 0};
+        for (int i = ln; i < tb.nross; i += 64) {
+            const double dt = tabt[i] - templog, dp = tabp[i] - presslog;
+            const double r2 = dt * dt + dp * dp;
+            const int q = (dt >= 0.0 ? 0 : 2) + (dp >= 0.0 ? 0 : 1);
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (k == q && r2 < rm[k]) {
+                    rm[k] = r2;
+                    ix[k] = i + 1;
+                }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            for (int o = 32; o > 0; o >>= 1) {
+                const double r2o = __shfl_xor(rm[k], o, 64);
+                const int io = __shfl_xor(ix[k], o, 64);
+                if (io != 0 && (ix[k] == 0 || r2o < rm[k] || (r2o == rm[k] && io < ix[k]))) {
+                    rm[k] = r2o;
+                    ix[k] = io;
+                }
+            }
+        if (ix[0] != 0 && ix[1] != 0 && ix[2] != 0 && ix[3] != 0) {
+            const double tpp = tabt[ix[0] - 1], ppp = tabp[ix[0] - 1], rpp = ross[ix[0] - 1];
+            const double tpm = tabt[ix[1] - 1], ppm = tabp[ix[1] - 1], rpm = ross[ix[1] - 1];
+            const double tmp = tabt[ix[2] - 1], pmp = tabp[ix[2] - 1], rmp = ross[ix[2] - 1];
+            const double tmm = tabt[ix[3] - 1], pmm = tabp[ix[3] - 1], rmm = ross[ix[3] - 1];
+            const double r_hi = ((templog - tmp) * rpp + (tpp - templog) * rmp) / (tpp - tmp);
+            const double p_hi = ((templog - tmp) * ppp + (tpp - templog) * pmp) / (tpp - tmp);
+            const double r_lo = ((templog - tmm) * rpm + (tpm - templog) * rmm) / (tpm - tmm);
+            const double p_lo = ((templog - tmm) * ppm + (tpm - templog) * pmm) / (tpm - tmm);
+            r = ((presslog - p_lo) * r_hi + (p_hi - presslog) * r_lo) / (p_hi - p_lo);
+        } else {
+            const double d_pp = sqrt(rm[0]) + 1.0e-5, d_pm = sqrt(rm[1]) + 1.0e-5;
+            const double d_mp = sqrt(rm[2]) + 1.0e-5, d_mm = sqrt(rm[3]) + 1.0e-5;
+            const double rwt = 1.0 / d_pp + 1.0 / d_pm + 1.0 / d_mp + 1.0 / d_mm;
+            r = (ross[max(ix[0], 1) - 1] / d_pp + ross[max(ix[1], 1) - 1] / d_pm +
+                 ross[max(ix[2], 1) - 1] / d_mp + ross[max(ix[3], 1) - 1] / d_mm) / rwt;
+        }
+    } else {
+        if (ln == 0) {
+            constexpr int KFIT = 12;
+            double dist2[KFIT];
+            int kidx[KFIT];
+            int kfound = 0, kmax = 1;
+            double dmax2 = 0.0;
+            for (int i = 1; i <= tb.nross; ++i) {
+                const double dt = tabt[i - 1] - templog, dp = tabp[i - 1] - presslog;
+                const double radius2 = dt * dt + dp * dp;
+                if (kfound < KFIT) {
+                    kfound = kfound + 1;
+                    dist2[kfound - 1] = radius2;
+                    kidx[kfound - 1] = i;
+                    if (radius2 > dmax2) {
+                        dmax2 = radius2;
+                        kmax = kfound;
+                    }
+                } else if (radius2 < dmax2) {
+                    dist2[kmax - 1] = radius2;
+                    kidx[kmax - 1] = i;
+                    dmax2 = dist2[0];
+                    kmax = 1;
+                    for (int k = 2; k <= KFIT; ++k)
+                        if (dist2[k - 1] > dmax2) {
+                            dmax2 = dist2[k - 1];
+                            kmax = k;
+                        }
+                }
+            }
+            if (mode == 2) {
+                double rs = 0.0, rwt = 0.0;
+                for (int k = 0; k < kfound; ++k) {
+                    const double w = 1.0 / (dist2[k] + 1.0e-4);
+                    rs = rs + w * ross[kidx[k] - 1];
+                    rwt = rwt + w;
+                }
+                r = rs / rwt;
+            } else {
+                const double d2sup = dmax2 * 1.1025;
+                double a11 = 0.0, a12 = 0.0, a13 = 0.0, a22 = 0.0, a23 = 0.0, a33 = 0.0;
+                double b1 = 0.0, b2 = 0.0, b3 = 0.0;
+                for (int k = 0; k < kfound; ++k) {
+                    const double dnrm = sqrt(dist2[k] / d2sup);
+                    const double om = 1.0 - dnrm;
+                    const double w = ((om * om) * om) * om * (1.0 + 4.0 * dnrm);
+                    const double dt = tabt[kidx[k] - 1] - templog;
+                    const double dp = tabp[kidx[k] - 1] - presslog;
+                    const double rr = ross[kidx[k] - 1];
+                    a11 = a11 + w;
+                    a12 = a12 + w * dt;
+                    a13 = a13 + w * dp;
+                    a22 = a22 + w * dt * dt;
+                    a23 = a23 + w * dt * dp;
+                    a33 = a33 + w * dp * dp;
+                    b1 = b1 + w * rr;
+                    b2 = b2 + w * rr * dt;
+                    b3 = b3 + w * rr * dp;
+                }
+                const double ridge = 1.0e-6 * (a22 + a33) + 1.0e-30;
+                a22 = a22 + ridge;
+                a33 = a33 + ridge;
+                const double det = a11 * (a22 * a33 - a23 * a23) - a12 * (a12 * a33 - a23 * a13) +
+                                   a13 * (a12 * a23 - a22 * a13);
+                const double det1 = b1 * (a22 * a33 - a23 * a23) - a12 * (b2 * a33 - a23 * b3) +
+                                    a13 * (b2 * a23 - a22 * b3);
+                const double deta = fmax(fabs(a11 * a22 * a33), 1.0e-300);
+                if (fabs(det) > 1.0e-10 * deta && a11 > 0.0) r = det1 / det;
+                else r = b1 / fmax(a11, 1.0e-300);
+            }
+        }
+        r = __shfl(r, 0, 64);
+    }
+    return pow(10.0, r);
+}
+
+// MODE 1, STORE: the layers' (T, P, ABROSS) appended at entry nross0
+// (0-based) for cnt layers; nross0 == 0 also sets the normalisation.
+__global__ __launch_bounds__(64) void k_rosstab_store(int n, int nross0, int cnt, int maxtab,
+                                                      const double *__restrict__ tpa,
+                                                      double *__restrict__ tab,
+                                                      double *__restrict__ zp)
+{
+#pragma clang fp contract(off)
+    const double *t = tpa, *p = tpa + n, *ab = tpa + 2 * n;
+    const int ln = threadIdx.x;
+    if (nross0 == 0 && ln == 0) {
+        zp[0] = log10(t[0]);
+        zp[2] = log10(p[0]);
+        zp[1] = log10(t[n - 1]) - zp[0];
+        zp[3] = log10(p[n - 1]) - zp[2];
+    }
+    __syncthreads();
+    const double zt = zp[0], st = zp[1], zq = zp[2], sq = zp[3];
+    for (int j = ln; j < cnt; j += 64) {
+        tab[nross0 + j] = (log10(t[j]) - zt) / st;
+        tab[maxtab + nross0 + j] = (log10(p[j]) - zq) / sq;
+        tab[2 * maxtab + nross0 + j] = log10(ab[j]);
+    }
+}
+
+__global__ __launch_bounds__(64) void k_rosstab_lookup(RossTab tb, int mode,
+                                                       const double *__restrict__ q, int nq,
+                                                       double *__restrict__ out)
+{
+    const int i = blockIdx.x;
+    const double r = rosstab_lookup_wave(tb, mode, q[i], q[nq + i]);
+    if (threadIdx.x == 0) out[i] = r;
+}
+
+// TTAUP: the march over depths, wave-uniform (every lane holds the same
+// iterate; lane 0 stores), ROSSTAB looked up by the whole wave.
+__device__ int ttaup_wave(const RossTab &tb, int mode, const double *t_in, const double *tau,
+                          double *abstd, double *ptotal, double *p_out, const double *prad_in,
+                          const double *pturb_in, double grav, int numtau)
+{
+#pragma clang fp contract(off)
+    const bool w0 = lane_id() == 0;
+    const double relax_ladder[4] = {0.5, 0.3, 0.1, 0.05};
+    const double dlgtau = log(tau[1] / tau[0]);
+    double plog1 = 0.0, plog2 = 0.0, plog3 = 0.0, plog4 = 0.0;
+    double dplog1 = 0.0, dplog2 = 0.0, dplog3 = 0.0;
+    double plog = 0.0, pnew = 0.0, dplog = 0.0;
+    int nfail = 0;
+    double ab0 = 0.1;
+    if (prad_in[0] > 0.0) ab0 = fmin(0.1, grav * tau[0] / prad_in[0] / 2.0);
+    double ab1_init = ab0;
+    for (int J = 1; J <= numtau; ++J) {
+        const int j = J - 1;
+        if (J == 1) plog = log(grav / ab0 * tau[0]);
+        else if (J <= 4) plog = plog1 + dplog1;
+        else plog = (3.0 * plog4 + 8.0 * dplog1 - 4.0 * dplog2 + 8.0 * dplog3) / 3.0;
+        double error = 1.0, prev_error = 1.0, best_error = 1.7976931348623157e308,
+               best_plog = plog;
+        int n = 0, n_stall = 0, relax_level = 1;
+        bool converged = false;
+        double ptj = 0.0, pj = 0.0, abj = 0.0;
+        const double dprad = prad_in[0] - prad_in[j], dpturb = pturb_in[0] - pturb_in[j];
+        for (;;) {
+            ptj = exp(plog);
+            pj = ptj + dprad + dpturb;
+            if (pj <= 0.0) {
+                pj = ptj * 1.0e-4;
+                if (pj <= 0.0) pj = 1.0e-10;
+            }
+            abj = rosstab_lookup_wave(tb, mode, t_in[j], pj);
+            if (J == 1) ab0 = abj;
+            dplog = grav / abj * tau[j] / ptj * dlgtau;
+            dplog = fmax(0.0, fmin(10.0, dplog));
+            n = n + 1;
+            if (n == 1) continue;
+            if (J == 1) pnew = log(grav / ab0 * tau[0]);
+            else if (J <= 4) pnew = (plog + 2.0 * plog1 + dplog + dplog1) / 3.0;
+            else
+                pnew = (126.0 * plog1 - 14.0 * plog3 + 9.0 * plog4 + 42.0 * dplog +
+                        108.0 * dplog1 - 54.0 * dplog2 + 24.0 * dplog3) / 121.0;
+            prev_error = error;
+            error = fabs(pnew - plog);
+            if (error < best_error) {
+                best_error = error;
+                best_plog = pnew;
+            }
+            if (n > 3 && error > 0.9 * prev_error) {
+                n_stall = n_stall + 1;
+                if (n_stall >= 4 && relax_level < 4) {
+                    relax_level = relax_level + 1;
+                    n_stall = 0;
+                }
+            } else {
+                n_stall = max(n_stall - 1, 0);
+            }
+            const double relax = relax_ladder[relax_level - 1];
+            plog = (1.0 - relax) * plog + relax * pnew;
+            if (error <= 1.0e-4) {
+                converged = true;
+                break;
+            }
+            if (n >= 50) break;
+        }
+        if (!converged) {
+            ++nfail;
+            plog = best_plog;
+            ptj = exp(plog);
+            pj = ptj + dprad + dpturb;
+            if (pj <= 0.0) pj = fmax(ptj * 1.0e-4, 1.0e-10);
+            abj = rosstab_lookup_wave(tb, mode, t_in[j], pj);
+            if (J == 1) ab0 = abj;
+            dplog = grav / abj * tau[j] / ptj * dlgtau;
+            dplog = fmax(0.0, fmin(10.0, dplog));
+        }
+        if (J == 1 && (ab0 / ab1_init > 2.0 || ab1_init / ab0 > 2.0)) {
+            ab1_init = ab0;
+            plog = log(grav / ab0 * tau[0]);
+            ptj = exp(plog);
+            pj = ptj;
+            if (pj > 0.0) {
+                ab0 = rosstab_lookup_wave(tb, mode, t_in[0], pj);
+                plog = log(grav / ab0 * tau[0]);
+                ptj = exp(plog);
+                pj = ptj;
+            }
+            abj = ab0;
+            dplog = grav / ab0 * tau[0] / ptj * dlgtau;
+            dplog = fmax(0.0, fmin(10.0, dplog));
+        }
+        if (w0) {
+            abstd[j] = abj;
+            ptotal[j] = ptj;
+            p_out[j] = pj;
+        }
+        plog4 = plog3;
+        plog3 = plog2;
+        plog2 = plog1;
+        plog1 = plog;
+        dplog3 = dplog2;
+        dplog2 = dplog1;
+        dplog1 = dplog;
+    }
+    wsync();
+    return nfail;
+}
+
+__global__ __launch_bounds__(64) void k_ttaup(RossTab tb, int mode, int n, double grav,
+                                              const double *__restrict__ in,
+                                              double *__restrict__ out, int *__restrict__ nfail)
+{
+    // in (4, n): T, TAU, PRAD, PTURB; out (3, n): ABSTD, PTOTAL, P
+    const int nf = ttaup_wave(tb, mode, in, in + n, out, out + n, out + 2 * n, in + 2 * n,
+                              in + 3 * n, grav, n);
+    if (threadIdx.x == 0) *nfail = nf;
+}
+
+// CZ_CONSTRUCT's SAVE state on the device (the layout of a12_czc_state).
+struct CzcState {
+    int prev_engaged, jc_sticky, nbot_bad, run_engaged, run_jc;
+    double last_dtmax, rel_err;
+};
+
+struct CzcPar {
+    int n, ifconv, iter, use_constructor, use4pt, irosstab;
+    double flux, mixlth, grav;
+};
+
+// The arrays CZ_CONSTRUCT reads (n each): the convective state and the
+// structure it corrects.
+struct CzcArr {
+    const double *flxcnv, *flxrad, *dltdlp, *grdadb, *flxerr, *t, *rhox, *ptotal, *rho,
+        *heatcp, *dlrdlt, *p, *vturb, *abross, *hscale;
+};
+
+struct CzcLDS {
+    double hrat[NR], delin[NR], delreq[NR], tkur[NR], tw[NR], wblnd[NR], gin[NR], dtk[NR];
+    double gadj[NR], dtw[NR], errg[NR];
+    int goodj[NR];
+};
+
+// CZC_DELREQ_LAYER for layer j (0-based), wave-uniform.
+__device__ double czc_delreq_wave(const CzcPar &c, const CzcArr &a, const RossTab &tb, int j,
+                                  double delin_j)
+{
+#pragma clang fp contract(off)
+    const double fourpi = 4.0 * 3.141592653589793, sigma_sb = 5.670374419e-5;
+    const double f_req = c.flux - fmin(a.flxrad[j], 0.999 * c.flux);
+    const double fluxco = 0.5 * a.rho[j] * a.heatcp[j] * a.t[j] * c.mixlth / fourpi;
+    const double vco =
+        0.5 * c.mixlth * sqrt(fmax(-0.5 * a.ptotal[j] / a.rho[j] * a.dlrdlt[j], 0.0));
+    if (fluxco <= 0.0 || vco <= 0.0) return delin_j;
+    double delta = pow(f_req / (fluxco * vco), 2.0 / 3.0);
+    delta = fmin(delta, 0.15 / c.mixlth);
+    const double tj = a.t[j], pj = a.p[j];
+    const double deltat_b = tj * c.mixlth * delta;
+    const double rosst0 = rosstab_lookup_wave(tb, c.irosstab, tj, pj);
+    const double dplus = rosstab_lookup_wave(tb, c.irosstab, tj + deltat_b, pj) / rosst0;
+    const double dminus = rosstab_lookup_wave(tb, c.irosstab, tj - deltat_b, pj) / rosst0;
+    double abconv;
+    if (c.use4pt) {
+        const double dplus_i = rosstab_lookup_wave(tb, c.irosstab, tj + deltat_b / 3.0, pj) / rosst0;
+        const double dminus_i = rosstab_lookup_wave(tb, c.irosstab, tj - deltat_b / 3.0, pj) / rosst0;
+        abconv = 4.0 / (1.0 / dplus + 1.0 / dplus_i + 1.0 / dminus + 1.0 / dminus_i) * a.abross[j];
+    } else {
+        abconv = 2.0 / (1.0 / dplus + 1.0 / dminus) * a.abross[j];
+    }
+    const double t4 = ((tj * tj) * tj) * tj;
+    double dresp = 8.0 * sigma_sb * t4 / (abconv * a.hscale[j] * a.rho[j]) / (fluxco * fourpi) / vco;
+    const double taub = abconv * a.rho[j] * c.mixlth * a.hscale[j];
+    dresp = dresp * (taub * taub) / (2.0 + taub * taub);
+    dresp = dresp * dresp / 2.0;
+    return delta + sqrt(2.0 * dresp * delta);
+}
+
+// CZ_CONSTRUCT(T1): returns 1 when engaged.  st (device) is read and
+// written by lane 0 only; t1 (n) updated in place.
+__device__ int cz_construct_wave(const CzcPar &c, const CzcArr &a, const RossTab &tb,
+                                 CzcState *st, double *t1, CzcLDS &L)
+{
+#pragma clang fp contract(off)
+    const double hr_min = 0.60, hr_w_lo = 0.50, hr_w_hi = 0.85, del_max = 0.05;
+    const double err_on = 5.0, err_off = 2.0, dt_release = 1.5, reeng_fac = 2.0, dt_max = 25.0;
+    const int gap_max = 2, j_min = 4, nrefine = 2;
+    const int ln = lane_id(), n = c.n;
+    const bool w0 = ln == 0;
+    CzcState s = *st;
+    wsync();
+    int engaged = 0;
+    if (c.iter <= 1) {
+        s.prev_engaged = 0;
+        s.jc_sticky = 0;
+        s.nbot_bad = 0;
+        s.last_dtmax = 1.0e30;
+        s.rel_err = 0.0;
+        s.run_engaged = 0;
+        s.run_jc = 0;
+    }
+    // early returns leave the state as updated so far
+    do {
+        if (!c.use_constructor || c.ifconv != 1 || c.mixlth <= 0.0 || n < j_min + 5) break;
+        for (int j = ln; j < n; j += 64) {
+            L.hrat[j] = a.flxcnv[j] / (a.flxcnv[j] + fmax(a.flxrad[j], 1.0e-30));
+            L.delin[j] = a.dltdlp[j] - a.grdadb[j];
+            L.goodj[j] = (L.hrat[j] > hr_min) && (L.delin[j] < del_max);
+        }
+        wsync();
+        bool good_bottom = L.goodj[n - 1] != 0;
+        if (!good_bottom) {
+            if (s.prev_engaged && s.nbot_bad < 3) {
+                s.nbot_bad = s.nbot_bad + 1;
+                good_bottom = true;
+            } else {
+                s.prev_engaged = 0;
+                s.jc_sticky = 0;
+                s.nbot_bad = 0;
+                break;
+            }
+        } else {
+            s.nbot_bad = 0;
+        }
+        // block finder (J = n is gate-passing by now)
+        int jc = n, ngap = 0;
+        for (int J = n - 1; J >= j_min; --J) {
+            if (L.goodj[J - 1]) {
+                jc = J;
+                ngap = 0;
+            } else {
+                ngap = ngap + 1;
+                if (ngap > gap_max) break;
+            }
+        }
+        const int jc_found = jc;
+        if (s.jc_sticky > 0) jc = min(jc, max(s.jc_sticky, j_min));
+        for (int J = jc; J <= jc_found - 1; ++J)
+            if (L.delin[J - 1] > del_max) jc = J + 1;
+        if (n - jc < 4) {
+            s.prev_engaged = 0;
+            s.jc_sticky = 0;
+            break;
+        }
+        double maxerr = 0.0;
+        for (int J = jc; J <= n; ++J) maxerr = fmax(maxerr, fabs(a.flxerr[J - 1]));
+        bool engage;
+        if (s.prev_engaged) {
+            engage = maxerr > err_off;
+            if (s.last_dtmax < dt_release) {
+                engage = false;
+                s.rel_err = maxerr;
+            }
+        } else {
+            engage = maxerr > fmax(err_on, reeng_fac * s.rel_err);
+        }
+        s.prev_engaged = engage;
+        if (!engage) {
+            s.jc_sticky = 0;
+            s.last_dtmax = 1.0e30;
+            break;
+        }
+        s.jc_sticky = jc;
+        s.rel_err = 0.0;
+        s.run_engaged = 1;
+        s.run_jc = jc;
+        for (int J = jc; J <= n; ++J) {
+            const double d = czc_delreq_wave(c, a, tb, J - 1, L.delin[J - 1]);
+            if (w0) L.delreq[J - 1] = d;
+        }
+        for (int j = ln; j < n; j += 64) {
+            L.tkur[j] = a.t[j] + t1[j];
+            L.tw[j] = L.tkur[j];
+            L.gadj[j] = 0.0;
+        }
+        for (int J = max(jc - 1, 1) + ln; J <= n; J += 64) {
+            double wint = (L.hrat[J - 1] - hr_w_lo) / (hr_w_hi - hr_w_lo);
+            wint = fmax(0.0, fmin(1.0, wint));
+            L.wblnd[J - 1] = wint * wint * (3.0 - 2.0 * wint);
+        }
+        wsync();
+        deriv_wave(a.rhox, L.tkur, L.dtk, n);
+        for (int J = jc + ln; J <= n - 1; J += 64)
+            L.gin[J - 1] = a.ptotal[J - 1] / L.tkur[J - 1] / c.grav * L.dtk[J - 1];
+        if (w0)
+            L.gin[n - 1] = log(L.tkur[n - 1] / L.tkur[n - 2]) / log(a.ptotal[n - 1] / a.ptotal[n - 2]);
+        wsync();
+        for (int K = 0; K <= nrefine; ++K) {
+            // interval recurrence, wave-uniform; lane 0 stores
+            double twp = L.tw[jc - 2];
+            for (int J = jc; J <= n; ++J) {
+                const double lnpr = log(a.ptotal[J - 1] / a.ptotal[J - 2]);
+                const double slopin = log(L.tkur[J - 1] / L.tkur[J - 2]) / lnpr;
+                double gtarg;
+                if (J == jc)
+                    gtarg = 0.5 * (a.grdadb[J - 2] + a.grdadb[J - 1]) + L.delreq[J - 1];
+                else
+                    gtarg = 0.5 * (a.grdadb[J - 2] + a.grdadb[J - 1]) +
+                            0.5 * (L.delreq[J - 2] + L.delreq[J - 1]);
+                const double wint = 0.5 * (L.wblnd[J - 2] + L.wblnd[J - 1]);
+                const double gbari = (gtarg - slopin) * wint + L.gadj[J - 1];
+                double tw = twp * (L.tkur[J - 1] / L.tkur[J - 2]) * exp(gbari * lnpr);
+                tw = fmax(tw, twp);
+                if (w0) L.tw[J - 1] = tw;
+                twp = tw;
+            }
+            wsync();
+            if (K == nrefine) break;
+            deriv_wave(a.rhox, L.tw, L.dtw, n);
+            for (int J = jc + ln; J <= n - 1; J += 64)
+                L.errg[J - 1] = L.wblnd[J - 1] * (a.grdadb[J - 1] + L.delreq[J - 1]) +
+                                (1.0 - L.wblnd[J - 1]) * L.gin[J - 1] -
+                                a.ptotal[J - 1] / L.tw[J - 1] / c.grav * L.dtw[J - 1];
+            if (w0)
+                L.errg[n - 1] = L.wblnd[n - 1] * (a.grdadb[n - 1] + L.delreq[n - 1]) +
+                                (1.0 - L.wblnd[n - 1]) * L.gin[n - 1] -
+                                log(L.tw[n - 1] / L.tw[n - 2]) / log(a.ptotal[n - 1] / a.ptotal[n - 2]);
+            wsync();
+            for (int J = jc + ln; J <= n; J += 64) {
+                if (J == jc) L.gadj[J - 1] = L.gadj[J - 1] + 0.70 * L.errg[J - 1];
+                else L.gadj[J - 1] = L.gadj[J - 1] + 0.35 * (L.errg[J - 2] + L.errg[J - 1]);
+            }
+            wsync();
+        }
+        double dtmax_blk = 0.0;
+        for (int J = jc; J <= n; ++J) dtmax_blk = fmax(dtmax_blk, fabs(L.tw[J - 1] - L.tkur[J - 1]));
+        double scl = 1.0;
+        if (dtmax_blk > dt_max) scl = dt_max / dtmax_blk;
+        for (int J = jc + ln; J <= n; J += 64)
+            t1[J - 1] = t1[J - 1] + (L.tw[J - 1] - L.tkur[J - 1]) * scl;
+        s.last_dtmax = dtmax_blk * scl;
+        engaged = 1;
+    } while (false);
+    wsync();
+    if (w0) *st = s;
+    wsync();
+    return engaged;
+}
+
+__global__ __launch_bounds__(64) void k_cz_construct(RossTab tb, CzcPar c,
+                                                     const double *__restrict__ in,
+                                                     CzcState *__restrict__ st,
+                                                     double *__restrict__ t1,
+                                                     int *__restrict__ engaged)
+{
+    __shared__ CzcLDS L;
+    const int n = c.n;
+    CzcArr a;
+    const double **f[15] = {&a.flxcnv, &a.flxrad, &a.dltdlp, &a.grdadb, &a.flxerr, &a.t,
+                            &a.rhox, &a.ptotal, &a.rho, &a.heatcp, &a.dlrdlt, &a.p,
+                            &a.vturb, &a.abross, &a.hscale};
+    for (int k = 0; k < 15; ++k) *f[k] = in + (size_t)k * n;
+    const int e = cz_construct_wave(c, a, tb, st, t1, L);
+    if (threadIdx.x == 0) *engaged = e;
+}
+
+// TCORR(3) sections (G2)-(K).  in (10, n): FLXCNV FLXRAD DLTDLP GRDADB
+// FLXERR PTOTAL RHO HEATCP DLRDLT HSCALE; io (17 + 6, n): TAUROS RHOX T P
+// XNE ABROSS PRAD VTURB BMIN PTURB ACCRAD, BHYD(:, 1:6), then out TK HKT
+// HCKT TKEV TLOG PRADK; t1 (n) in/out; drhox (n) out; *nfail out.
+struct FinPar {
+    CzcPar c;
+    int j1smooth, j2smooth;
+    double tau1lg, steplg, pradk0, tfloor, wtjm1, wtj, wtjp1;
+};
+
+struct FinLDS {
+    double taustd[NR], tplus[NR], tnew1[NR], prdnew[NR], tnew2[NR];
+    double ab1[NR], ptot1[NR], p1[NR], ab2[NR], ptot2[NR], p2[NR], ppp[NR], rrr[NR];
+    double drhox[NR], tsm[NR], remap[10][NR];
+};
+
+__global__ __launch_bounds__(64) void k_tcorr3_fin(RossTab tb, FinPar fp,
+                                                   const double *__restrict__ in,
+                                                   double *__restrict__ io,
+                                                   CzcState *__restrict__ st,
+                                                   double *__restrict__ t1,
+                                                   double *__restrict__ drhox_out,
+                                                   int *__restrict__ nfail)
+{
+#pragma clang fp contract(off)
+    __shared__ CzcLDS C;
+    __shared__ FinLDS F;
+    const int n = fp.c.n, ln = lane_id();
+    double *tauros = io, *rhox = io + n, *t = io + 2 * n, *p = io + 3 * n, *xne = io + 4 * n,
+           *abross = io + 5 * n, *prad = io + 6 * n, *vturb = io + 7 * n, *bmin = io + 8 * n,
+           *pturb = io + 9 * n, *accrad = io + 10 * n, *bhyd = io + 11 * n;
+    double *o = io + 17 * n;  // TK HKT HCKT TKEV TLOG PRADK
+    // (G2) on the module arrays as they stand
+    CzcArr a;
+    a.flxcnv = in;
+    a.flxrad = in + n;
+    a.dltdlp = in + 2 * n;
+    a.grdadb = in + 3 * n;
+    a.flxerr = in + 4 * n;
+    a.ptotal = in + 5 * n;
+    a.rho = in + 6 * n;
+    a.heatcp = in + 7 * n;
+    a.dlrdlt = in + 8 * n;
+    a.hscale = in + 9 * n;
+    a.t = t;
+    a.rhox = rhox;
+    a.p = p;
+    a.vturb = vturb;
+    a.abross = abross;
+    (void)cz_construct_wave(fp.c, a, tb, st, t1, C);
+    // (H)
+    for (int j = ln; j < n; j += 64) {
+        F.tplus[j] = t[j] + t1[j];
+        F.taustd[j] = pow(10.0, fp.tau1lg + (double)j * fp.steplg);
+    }
+    wsync();
+    (void)map1_wave(tauros, t, n, F.taustd, F.tnew1, n);
+    wsync();
+    (void)map1_wave(tauros, prad, n, F.taustd, F.prdnew, n);
+    wsync();
+    int nf = ttaup_wave(tb, fp.c.irosstab, F.tnew1, F.taustd, F.ab1, F.ptot1, F.p1, F.prdnew,
+                        pturb, fp.c.grav, n);
+    (void)map1_wave(tauros, F.tplus, n, F.taustd, F.tnew2, n);
+    wsync();
+    nf += ttaup_wave(tb, fp.c.irosstab, F.tnew2, F.taustd, F.ab2, F.ptot2, F.p2, F.prdnew,
+                     pturb, fp.c.grav, n);
+    for (int j = ln; j < n; j += 64) F.ppp[j] = (F.ptot2[j] - F.ptot1[j]) / F.ptot1[j];
+    wsync();
+    (void)map1_wave(F.taustd, F.ppp, n, tauros, F.rrr, n);
+    wsync();
+    // (I)
+    for (int j = ln; j < n; j += 64) {
+        F.drhox[j] = F.rrr[j] * rhox[j];
+        drhox_out[j] = F.drhox[j];
+        t[j] = t[j] + t1[j];
+    }
+    wsync();
+    if (fp.j1smooth > 0) {
+        for (int J = fp.j1smooth + ln; J <= fp.j2smooth; J += 64)
+            F.tsm[J - 1] = fp.wtjm1 * t[J - 2] + fp.wtj * t[J - 1] + fp.wtjp1 * t[J];
+        wsync();
+        for (int J = fp.j1smooth + ln; J <= fp.j2smooth; J += 64) t[J - 1] = F.tsm[J - 1];
+        wsync();
+    }
+    if (ln == 0)
+        for (int I = 2; I <= n; ++I) {
+            const int J = n + 1 - I;
+            t[J - 1] = fmin(t[J - 1], t[J] - 1.0);
+        }
+    wsync();
+    for (int j = ln; j < n; j += 64) {
+        t[j] = fmax(t[j], fp.tfloor);
+        rhox[j] = rhox[j] + F.drhox[j];  // (K); (J)'s vectors are set again below
+    }
+    wsync();
+    const double *src[10] = {rhox, t, p, xne, abross, prad, vturb, bmin, pturb, accrad};
+    for (int k = 0; k < 10; ++k) {
+        (void)map1_wave(tauros, src[k], n, F.taustd, F.remap[k], n);
+        wsync();
+    }
+    const double kbol = 1.380649e-16, hplanck = 6.62607015e-27, clight = 2.99792458e10;
+    const double kbol_ev = 1.380649e-16 / 1.602176634e-12;
+    const double ab_top = abross[0], tr_top = tauros[0];
+    for (int j = ln; j < n; j += 64) {
+        if (tr_top > F.taustd[j] && F.remap[4][j] < 0.0) F.remap[4][j] = ab_top;
+        rhox[j] = F.remap[0][j];
+        t[j] = F.remap[1][j];
+        const double tk = kbol * t[j];
+        const double hkt = hplanck / tk;
+        o[j] = tk;
+        o[n + j] = hkt;
+        o[2 * n + j] = hkt * clight;
+        o[3 * n + j] = kbol_ev * t[j];
+        o[4 * n + j] = log(t[j]);
+        p[j] = F.remap[2][j];
+        xne[j] = F.remap[3][j];
+        abross[j] = F.remap[4][j];
+        prad[j] = F.remap[5][j];
+        o[5 * n + j] = prad[j] + fp.pradk0;
+        vturb[j] = F.remap[6][j];
+        bmin[j] = F.remap[7][j];
+        pturb[j] = F.remap[8][j];
+    }
+    wsync();
+    for (int i = 0; i < 6; ++i) {
+        (void)map1_wave(tauros, bhyd + (size_t)i * n, n, F.taustd, F.remap[0], n);
+        wsync();
+        for (int j = ln; j < n; j += 64) bhyd[(size_t)i * n + j] = F.remap[0][j];
+        wsync();
+    }
+    for (int j = ln; j < n; j += 64) tauros[j] = F.taustd[j];
+    if (ln == 0) *nfail = nf;
+}
+
+// --------------------------------------------------------------------------
 // Model iteration: the ATLAS12 loop (src/atlas12c.f90:496-606) for many
 // models at once on a fixed Rosseland grid TAUSTD(j) = 10**(-6.875 +
 // 0.125 (j-1)), j = 1..72, with KAPP and the EOS replaced by closed forms
@@ -1756,6 +2430,285 @@ int a12i_atlas_tcorr3(a12_ctx *ctx, const a12_tcorr_in *in, double *rdabh, doubl
     A12_HIP(hipMemcpyAsync(oldt1, dold, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     if (rdabh) A12_HIP(hipMemcpyAsync(rdabh, drd, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     A12_HIP(hipStreamSynchronize(s));
+    return A12_OK;
+}
+
+// ---- ROSSTAB, TTAUP, CZ_CONSTRUCT, TCORR(3) (G2)-(K) (host) ----------------
+namespace {
+constexpr int kMaxTab = A12_ROSSTAB_MAXTAB;
+
+int ross_reserve(a12_ctx *ctx)
+{
+    A12_HIP(ctx->ross.tab.reserve(sizeof(double) * 3 * (size_t)kMaxTab));
+    A12_HIP(ctx->ross.zp.reserve(sizeof(double) * 4));
+    return A12_OK;
+}
+
+RossTab ross_view(const a12_ctx *ctx)
+{
+    RossTab tb;
+    tb.tab = ctx->ross.tab.as<double>();
+    tb.zp = ctx->ross.zp.as<double>();
+    tb.nross = ctx->ross.nross;
+    tb.maxtab = kMaxTab;
+    return tb;
+}
+
+// Lookups need a table: the reference's lookup on an empty one reads
+// entries that were never stored.
+int ross_need(a12_ctx *ctx, int mode, const char *who)
+{
+    if (mode < 1 || mode > 3) {
+        ctx->err = std::string(who) + ": IROSSTAB must be 1, 2 or 3";
+        return A12_ERR_ARG;
+    }
+    if (ctx->ross.nross < 1) {
+        ctx->err = std::string(who) + ": the ROSSTAB table is empty (a12_rosstab_store first)";
+        return A12_ERR_STATE;
+    }
+    return A12_OK;
+}
+
+bool all_set(const double *const *p, int k)
+{
+    for (int i = 0; i < k; ++i)
+        if (!p[i]) return false;
+    return true;
+}
+}  // namespace
+
+int a12i_rosstab_reset(a12_ctx *ctx)
+{
+    ctx->ross.nross = 0;
+    return A12_OK;
+}
+
+int a12i_rosstab_store(a12_ctx *ctx, int n, const double *t, const double *p,
+                       const double *abross)
+{
+    if (n < 2 || n > A12_NRMAX || !t || !p || !abross) {
+        ctx->err = "a12_rosstab_store: bad layer count or null array";
+        return A12_ERR_ARG;
+    }
+    int rc = ross_reserve(ctx);
+    if (rc != A12_OK) return rc;
+    RossDev &R = ctx->ross;
+    // the reference stops appending at ROSSTAB_MAXTAB (:6014-6018)
+    const int cnt = std::max(0, std::min(n, kMaxTab - R.nross));
+    hipStream_t s = ctx->stream;
+    A12_HIP(R.io.reserve(sizeof(double) * 3 * (size_t)n));
+    double *d = R.io.as<double>();
+    A12_HIP(hipMemcpyAsync(d, t, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    A12_HIP(hipMemcpyAsync(d + n, p, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    A12_HIP(hipMemcpyAsync(d + 2 * n, abross, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_rosstab_store, dim3(1), dim3(64), 0, s, n, R.nross, cnt, kMaxTab, d,
+                       R.tab.as<double>(), R.zp.as<double>());
+    A12_CHECK_LAUNCH();
+    A12_HIP(hipStreamSynchronize(s));
+    R.nross += cnt;
+    return A12_OK;
+}
+
+int a12i_rosstab_load(a12_ctx *ctx, int nross, const double *zero, const double *tab)
+{
+    if (nross < 0 || nross > kMaxTab || !zero || (nross > 0 && !tab)) {
+        ctx->err = "a12_rosstab_load: nross outside 0..A12_ROSSTAB_MAXTAB or null array";
+        return A12_ERR_ARG;
+    }
+    int rc = ross_reserve(ctx);
+    if (rc != A12_OK) return rc;
+    hipStream_t s = ctx->stream;
+    double *d = ctx->ross.tab.as<double>();
+    A12_HIP(hipMemcpyAsync(ctx->ross.zp.p, zero, sizeof(double) * 4, hipMemcpyHostToDevice, s));
+    for (int k = 0; k < 3 && nross > 0; ++k)
+        A12_HIP(hipMemcpyAsync(d + (size_t)k * kMaxTab, tab + (size_t)k * nross,
+                               sizeof(double) * nross, hipMemcpyHostToDevice, s));
+    A12_HIP(hipStreamSynchronize(s));
+    ctx->ross.nross = nross;
+    return A12_OK;
+}
+
+int a12i_rosstab_fetch(a12_ctx *ctx, int32_t *nross, double *zero, double *tab)
+{
+    if (!nross) {
+        ctx->err = "a12_rosstab_fetch: null count";
+        return A12_ERR_ARG;
+    }
+    *nross = ctx->ross.nross;
+    if (ctx->ross.nross == 0) return A12_OK;
+    hipStream_t s = ctx->stream;
+    const int nr = ctx->ross.nross;
+    if (zero) A12_HIP(hipMemcpyAsync(zero, ctx->ross.zp.p, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
+    const double *d = ctx->ross.tab.as<double>();
+    for (int k = 0; k < 3 && tab; ++k)
+        A12_HIP(hipMemcpyAsync(tab + (size_t)k * nr, d + (size_t)k * kMaxTab, sizeof(double) * nr,
+                               hipMemcpyDeviceToHost, s));
+    A12_HIP(hipStreamSynchronize(s));
+    return A12_OK;
+}
+
+int a12i_rosstab_lookup(a12_ctx *ctx, int mode, int nq, const double *temp,
+                        const double *pressure, double *kappa)
+{
+    if (nq < 0 || (nq > 0 && (!temp || !pressure || !kappa))) {
+        ctx->err = "a12_rosstab_lookup: bad count or null array";
+        return A12_ERR_ARG;
+    }
+    int rc = ross_need(ctx, mode, "a12_rosstab_lookup");
+    if (rc != A12_OK || nq == 0) return rc;
+    hipStream_t s = ctx->stream;
+    A12_HIP(ctx->ross.io.reserve(sizeof(double) * 3 * (size_t)nq));
+    double *d = ctx->ross.io.as<double>();
+    A12_HIP(hipMemcpyAsync(d, temp, sizeof(double) * nq, hipMemcpyHostToDevice, s));
+    A12_HIP(hipMemcpyAsync(d + nq, pressure, sizeof(double) * nq, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_rosstab_lookup, dim3(nq), dim3(64), 0, s, ross_view(ctx), mode, d, nq,
+                       d + 2 * nq);
+    A12_CHECK_LAUNCH();
+    A12_HIP(hipMemcpyAsync(kappa, d + 2 * nq, sizeof(double) * nq, hipMemcpyDeviceToHost, s));
+    A12_HIP(hipStreamSynchronize(s));
+    return A12_OK;
+}
+
+int a12i_ttaup(a12_ctx *ctx, int mode, int n, const double *t, const double *tau,
+               const double *prad, const double *pturb, double grav, double *abstd,
+               double *ptotal, double *p, int32_t *nfail)
+{
+    const double *ins[4] = {t, tau, prad, pturb};
+    if (n < 2 || n > A12_NRMAX || !all_set(ins, 4) || !abstd || !ptotal || !p) {
+        ctx->err = "a12_ttaup: bad layer count or null array";
+        return A12_ERR_ARG;
+    }
+    int rc = ross_need(ctx, mode, "a12_ttaup");
+    if (rc != A12_OK) return rc;
+    hipStream_t s = ctx->stream;
+    A12_HIP(ctx->ross.io.reserve(sizeof(double) * 7 * (size_t)n + 64));
+    double *d = ctx->ross.io.as<double>();
+    for (int k = 0; k < 4; ++k)
+        A12_HIP(hipMemcpyAsync(d + (size_t)k * n, ins[k], sizeof(double) * n, hipMemcpyHostToDevice, s));
+    int *dnf = reinterpret_cast<int *>(d + 7 * n);
+    hipLaunchKernelGGL(k_ttaup, dim3(1), dim3(64), 0, s, ross_view(ctx), mode, n, grav, d,
+                       d + 4 * n, dnf);
+    A12_CHECK_LAUNCH();
+    double *outs[3] = {abstd, ptotal, p};
+    for (int k = 0; k < 3; ++k)
+        A12_HIP(hipMemcpyAsync(outs[k], d + (size_t)(4 + k) * n, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    int nf = 0;
+    A12_HIP(hipMemcpyAsync(&nf, dnf, sizeof(int), hipMemcpyDeviceToHost, s));
+    A12_HIP(hipStreamSynchronize(s));
+    if (nfail) *nfail = nf;
+    return A12_OK;
+}
+
+int a12i_cz_construct(a12_ctx *ctx, const a12_czc_in *in, a12_czc_state *st, double *t1,
+                      int32_t *engaged)
+{
+    if (!in || !st || !t1 || in->nrhox < 2 || in->nrhox > A12_NRMAX) {
+        ctx->err = "a12_cz_construct: bad layer count or null argument";
+        return A12_ERR_ARG;
+    }
+    const int n = in->nrhox;
+    const double *cols[15] = {in->flxcnv, in->flxrad, in->dltdlp, in->grdadb, in->flxerr,
+                              in->t, in->rhox, in->ptotal, in->rho, in->heatcp, in->dlrdlt,
+                              in->p, in->vturb, in->abross, in->hscale};
+    if (!all_set(cols, 15)) {
+        ctx->err = "a12_cz_construct: null state array";
+        return A12_ERR_ARG;
+    }
+    int rc = ross_need(ctx, in->irosstab, "a12_cz_construct");
+    if (rc != A12_OK) return rc;
+    static_assert(sizeof(CzcState) == sizeof(a12_czc_state), "CZ_CONSTRUCT state layout");
+    hipStream_t s = ctx->stream;
+    A12_HIP(ctx->ross.io.reserve(sizeof(double) * 16 * (size_t)n + 2 * sizeof(CzcState)));
+    double *d = ctx->ross.io.as<double>();
+    for (int k = 0; k < 15; ++k)
+        A12_HIP(hipMemcpyAsync(d + (size_t)k * n, cols[k], sizeof(double) * n, hipMemcpyHostToDevice, s));
+    double *dt1 = d + 15 * n;
+    CzcState *dst = reinterpret_cast<CzcState *>(d + 16 * n);
+    int *deng = reinterpret_cast<int *>(dst + 1);
+    A12_HIP(hipMemcpyAsync(dt1, t1, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    A12_HIP(hipMemcpyAsync(dst, st, sizeof(CzcState), hipMemcpyHostToDevice, s));
+    const CzcPar c = {n, in->ifconv, in->iter, in->use_constructor, in->use4pt, in->irosstab,
+                      in->flux, in->mixlth, in->grav};
+    hipLaunchKernelGGL(k_cz_construct, dim3(1), dim3(64), 0, s, ross_view(ctx), c, d, dst, dt1,
+                       deng);
+    A12_CHECK_LAUNCH();
+    int e = 0;
+    A12_HIP(hipMemcpyAsync(t1, dt1, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    A12_HIP(hipMemcpyAsync(st, dst, sizeof(CzcState), hipMemcpyDeviceToHost, s));
+    A12_HIP(hipMemcpyAsync(&e, deng, sizeof(int), hipMemcpyDeviceToHost, s));
+    A12_HIP(hipStreamSynchronize(s));
+    if (engaged) *engaged = e;
+    return A12_OK;
+}
+
+int a12i_tcorr3_finish(a12_ctx *ctx, const a12_tcorr_fin_in *in, a12_tcorr_fin_io *io,
+                       a12_czc_state *st, double *t1, double *drhox, int32_t *nfail)
+{
+    if (!in || !io || !st || !t1 || in->nrhox < 4 || in->nrhox > A12_NRMAX) {
+        ctx->err = "a12_atlas_tcorr3_finish: bad layer count or null argument";
+        return A12_ERR_ARG;
+    }
+    const int n = in->nrhox;
+    if (in->j1smooth > 0 && (in->j1smooth < 2 || in->j2smooth > n - 1 || in->j2smooth < in->j1smooth)) {
+        ctx->err = "a12_atlas_tcorr3_finish: smoothing range J1SMOOTH..J2SMOOTH outside 2..nrhox-1";
+        return A12_ERR_ARG;
+    }
+    const double *cin[10] = {in->flxcnv, in->flxrad, in->dltdlp, in->grdadb, in->flxerr,
+                             in->ptotal, in->rho, in->heatcp, in->dlrdlt, in->hscale};
+    double *cio[10] = {io->tauros, io->rhox, io->t, io->p, io->xne, io->abross, io->prad,
+                       io->vturb, io->bmin, io->pturb};
+    double *cout[6] = {io->tk, io->hkt, io->hckt, io->tkev, io->tlog, io->pradk};
+    bool ok = all_set(cin, 10) && io->accrad && io->bhyd;
+    for (double *c : cio) ok = ok && c;
+    for (double *c : cout) ok = ok && c;
+    if (!ok) {
+        ctx->err = "a12_atlas_tcorr3_finish: null state array";
+        return A12_ERR_ARG;
+    }
+    int rc = ross_need(ctx, in->irosstab, "a12_atlas_tcorr3_finish");
+    if (rc != A12_OK) return rc;
+    hipStream_t s = ctx->stream;
+    // staging: in (10 n) | io (23 n: 10 + ACCRAD + BHYD 6 + out 6) | t1 | drhox | state, nfail
+    A12_HIP(ctx->ross.io.reserve(sizeof(double) * 35 * (size_t)n + 2 * sizeof(CzcState)));
+    double *d = ctx->ross.io.as<double>();
+    double *dio = d + 10 * n, *dt1 = d + 33 * n, *ddr = d + 34 * n;
+    CzcState *dst = reinterpret_cast<CzcState *>(d + 35 * n);
+    int *dnf = reinterpret_cast<int *>(dst + 1);
+    for (int k = 0; k < 10; ++k)
+        A12_HIP(hipMemcpyAsync(d + (size_t)k * n, cin[k], sizeof(double) * n, hipMemcpyHostToDevice, s));
+    for (int k = 0; k < 10; ++k)
+        A12_HIP(hipMemcpyAsync(dio + (size_t)k * n, cio[k], sizeof(double) * n, hipMemcpyHostToDevice, s));
+    A12_HIP(hipMemcpyAsync(dio + 10 * n, io->accrad, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    A12_HIP(hipMemcpyAsync(dio + 11 * n, io->bhyd, sizeof(double) * 6 * n, hipMemcpyHostToDevice, s));
+    A12_HIP(hipMemcpyAsync(dt1, t1, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    A12_HIP(hipMemcpyAsync(dst, st, sizeof(CzcState), hipMemcpyHostToDevice, s));
+    FinPar fp;
+    fp.c = {n, in->ifconv, in->iter, in->use_constructor, in->use4pt, in->irosstab,
+            in->flux, in->mixlth, in->grav};
+    fp.j1smooth = in->j1smooth;
+    fp.j2smooth = in->j2smooth;
+    fp.tau1lg = in->tau1lg;
+    fp.steplg = in->steplg;
+    fp.pradk0 = in->pradk0;
+    fp.tfloor = in->tfloor;
+    fp.wtjm1 = in->wtjm1;
+    fp.wtj = in->wtj;
+    fp.wtjp1 = in->wtjp1;
+    hipLaunchKernelGGL(k_tcorr3_fin, dim3(1), dim3(64), 0, s, ross_view(ctx), fp, d, dio, dst,
+                       dt1, ddr, dnf);
+    A12_CHECK_LAUNCH();
+    for (int k = 0; k < 10; ++k)
+        A12_HIP(hipMemcpyAsync(cio[k], dio + (size_t)k * n, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    A12_HIP(hipMemcpyAsync(io->bhyd, dio + 11 * n, sizeof(double) * 6 * n, hipMemcpyDeviceToHost, s));
+    for (int k = 0; k < 6; ++k)
+        A12_HIP(hipMemcpyAsync(cout[k], dio + (size_t)(17 + k) * n, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    A12_HIP(hipMemcpyAsync(t1, dt1, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    if (drhox) A12_HIP(hipMemcpyAsync(drhox, ddr, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    A12_HIP(hipMemcpyAsync(st, dst, sizeof(CzcState), hipMemcpyDeviceToHost, s));
+    int nf = 0;
+    A12_HIP(hipMemcpyAsync(&nf, dnf, sizeof(int), hipMemcpyDeviceToHost, s));
+    A12_HIP(hipStreamSynchronize(s));
+    if (nfail) *nfail = nf;
     return A12_OK;
 }
 

@@ -43,7 +43,9 @@ EXPORTS = ["a12_ctx_create", "a12_ctx_destroy", "a12_last_error", "a12_stream",
            "a12_atlas_set_state", "a12_atlas_rt_pass_noalloc", "a12_model_converge",
            "a12_kapp_set_tables", "a12_kapp_set_state", "a12_atlas_kapp_run", "a12_kapp_eval",
            "a12_kapp_constants", "a12_kapp_ms", "a12_atlas_kapp_fetch", "a12_cplx_load",
-           "a12_cplx_run", "a12_cplx_fetch", "a12_cplx_clear", "a12_cplx_ms"]
+           "a12_cplx_run", "a12_cplx_fetch", "a12_cplx_clear", "a12_cplx_ms",
+           "a12_rosstab_reset", "a12_rosstab_store", "a12_rosstab_load", "a12_rosstab_fetch",
+           "a12_rosstab_lookup", "a12_ttaup", "a12_cz_construct", "a12_atlas_tcorr3_finish"]
 
 
 class A12Error(RuntimeError):
@@ -515,6 +517,184 @@ def atlas_tcorr3(ctx: Context, z, rdabh=None, oldt1=None, iter_=None, device_sum
     ctx._check(ctx.lib.a12_atlas_tcorr3(ctx.h, C.byref(s), _ptr(rd), _ptr(ot), _ptr(out)),
                "a12_atlas_tcorr3")
     return {k: out[i] for i, k in enumerate(TCORR_OUT)}, rd, ot
+
+
+ROSSTAB_MAXTAB = 4800  # kw * 60, src/atlas12_modules.f90:1779
+
+
+def rosstab_reset(ctx: Context):
+    """Empty the context's ROSSTAB table (a new model)."""
+    ctx._check(ctx.lib.a12_rosstab_reset(ctx.h), "a12_rosstab_reset")
+
+
+def rosstab_store(ctx: Context, t, p, abross):
+    """ROSSTAB(0, 0, 0): append the layers' (T, P, ABROSS) on the device."""
+    t, p, ab = _f64(t), _f64(p), _f64(abross)
+    n = len(t)
+    if p.shape != (n,) or ab.shape != (n,):
+        raise ValueError("t, p and abross must have the same length")
+    ctx._check(ctx.lib.a12_rosstab_store(ctx.h, n, _ptr(t), _ptr(p), _ptr(ab)),
+               "a12_rosstab_store")
+
+
+def rosstab_load(ctx: Context, zero, tab):
+    """Replace the table: zero (ZEROT, SLOPET, ZEROP, SLOPEP), tab (3, nross)."""
+    z = _f64(np.asarray(zero, np.float64).reshape(4))
+    t = _f64(np.asarray(tab, np.float64))
+    if t.ndim != 2 or t.shape[0] != 3:
+        raise ValueError("tab must have shape (3, nross)")
+    ctx._check(ctx.lib.a12_rosstab_load(ctx.h, t.shape[1], _ptr(z), _ptr(t)), "a12_rosstab_load")
+
+
+def rosstab_fetch(ctx: Context):
+    """(zero (4), tab (3, nross)) of the context's table."""
+    nr = C.c_int32(0)
+    ctx._check(ctx.lib.a12_rosstab_fetch(ctx.h, C.byref(nr), None, None), "a12_rosstab_fetch")
+    z = np.zeros(4)
+    t = np.zeros((3, nr.value))
+    if nr.value:
+        ctx._check(ctx.lib.a12_rosstab_fetch(ctx.h, C.byref(nr), _ptr(z), _ptr(t)),
+                   "a12_rosstab_fetch")
+    return z, t
+
+
+def rosstab_lookup(ctx: Context, temp, pressure, mode: int = 1):
+    """ROSSTAB(T, P, V) for each query (mode = IROSSTAB)."""
+    t, p = _f64(np.atleast_1d(temp)), _f64(np.atleast_1d(pressure))
+    if p.shape != t.shape:
+        raise ValueError("temp and pressure must have the same length")
+    out = np.zeros(len(t))
+    ctx._check(ctx.lib.a12_rosstab_lookup(ctx.h, int(mode), len(t), _ptr(t), _ptr(p), _ptr(out)),
+               "a12_rosstab_lookup")
+    return out
+
+
+def ttaup(ctx: Context, t, tau, prad, pturb, grav: float, mode: int = 1):
+    """TTAUP on the device.  Returns (abstd, ptotal, p_gas, n_unconverged)."""
+    t, tau, prad, pturb = (_f64(a) for a in (t, tau, prad, pturb))
+    n = len(t)
+    for a in (tau, prad, pturb):
+        if a.shape != (n,):
+            raise ValueError(f"every array must have shape ({n},)")
+    ab, pt, pg = np.zeros(n), np.zeros(n), np.zeros(n)
+    nf = C.c_int32(0)
+    fn = ctx.lib.a12_ttaup
+    fn.argtypes = [C.c_void_p, C.c_int32, C.c_int32] + [P_D] * 4 + [C.c_double] + [P_D] * 3 + \
+        [C.POINTER(C.c_int32)]
+    ctx._check(fn(ctx.h, int(mode), n, _ptr(t), _ptr(tau), _ptr(prad), _ptr(pturb), float(grav),
+                  _ptr(ab), _ptr(pt), _ptr(pg), C.byref(nf)), "a12_ttaup")
+    return ab, pt, pg, nf.value
+
+
+class CzcState(C.Structure):
+    _fields_ = [("prev_engaged", C.c_int32), ("jc_sticky", C.c_int32), ("nbot_bad", C.c_int32),
+                ("run_engaged", C.c_int32), ("run_jc", C.c_int32), ("last_dtmax", C.c_double),
+                ("rel_err", C.c_double)]
+
+
+def czc_state() -> CzcState:
+    """CZ_CONSTRUCT's SAVE state at the start of a run."""
+    st = CzcState()
+    st.last_dtmax = 1.0e30
+    return st
+
+
+CZC_ARRAYS = ("flxcnv", "flxrad", "dltdlp", "grdadb", "flxerr", "t", "rhox", "ptotal", "rho",
+              "heatcp", "dlrdlt", "p", "vturb", "abross", "hscale")
+
+
+class CzcIn(C.Structure):
+    _fields_ = [(k, C.c_int32) for k in ("nrhox", "ifconv", "iter", "use_constructor", "use4pt",
+                                         "irosstab")] + \
+               [(k, C.c_double) for k in ("flux", "mixlth", "grav")] + \
+               [(k, P_D) for k in CZC_ARRAYS]
+
+
+def cz_construct(ctx: Context, z, t1, state: CzcState, it: int, ifconv: int = 1,
+                 use4pt: int = 1, mode: int = 1, use_constructor: int = 1):
+    """CZ_CONSTRUCT(T1) on the device: z maps CZC_ARRAYS and flux, mixlth,
+    grav; state (czc_state()) is updated.  Returns (t1 out, engaged)."""
+    keep = {k: _f64(z[k]) for k in CZC_ARRAYS}
+    n = len(keep["t"])
+    for k, a in keep.items():
+        if a.shape != (n,):
+            raise ValueError(f"{k} must have shape ({n},)")
+    s = CzcIn()
+    s.nrhox, s.ifconv, s.iter, s.use_constructor, s.use4pt, s.irosstab = (
+        n, int(ifconv), int(it), int(use_constructor), int(use4pt), int(mode))
+    s.flux, s.mixlth, s.grav = float(z["flux"]), float(z["mixlth"]), float(z["grav"])
+    for k, a in keep.items():
+        setattr(s, k, _ptr(a))
+    out = _f64(np.array(t1, np.float64).copy())
+    if out.shape != (n,):
+        raise ValueError(f"t1 must have shape ({n},)")
+    e = C.c_int32(0)
+    ctx._check(ctx.lib.a12_cz_construct(ctx.h, C.byref(s), C.byref(state), _ptr(out), C.byref(e)),
+               "a12_cz_construct")
+    return out, e.value
+
+
+FIN_IN = ("flxcnv", "flxrad", "dltdlp", "grdadb", "flxerr", "ptotal", "rho", "heatcp",
+          "dlrdlt", "hscale")
+FIN_IO = ("tauros", "rhox", "t", "p", "xne", "abross", "prad", "vturb", "bmin", "pturb")
+FIN_OUT = ("tk", "hkt", "hckt", "tkev", "tlog", "pradk")
+
+
+class TcorrFinIn(C.Structure):
+    _fields_ = [(k, C.c_int32) for k in ("nrhox", "ifconv", "iter", "use_constructor", "use4pt",
+                                         "irosstab", "j1smooth", "j2smooth")] + \
+               [(k, C.c_double) for k in ("flux", "mixlth", "grav", "tau1lg", "steplg", "pradk0",
+                                          "tfloor", "wtjm1", "wtj", "wtjp1")] + \
+               [(k, P_D) for k in FIN_IN]
+
+
+class TcorrFinIo(C.Structure):
+    _fields_ = [(k, P_D) for k in FIN_IO] + [("accrad", P_D), ("bhyd", P_D)] + \
+               [(k, P_D) for k in FIN_OUT]
+
+
+def atlas_tcorr3_finish(ctx: Context, sc, arr, t1, state: CzcState, mode: int = 1):
+    """TCORR(3) sections (G2)-(K) on the device.  sc: the scalars of
+    TcorrFinIn (use_constructor and use4pt default to the reference's 1);
+    arr: FIN_IN, FIN_IO, accrad (n) and bhyd (6, n).  Returns (out dict of
+    FIN_IO + FIN_OUT + bhyd, t1 after (G2), drhox, TTAUP depths not
+    converged)."""
+    n = len(arr["t"])
+    s = TcorrFinIn()
+    s.nrhox = n
+    for k in ("ifconv", "iter", "j1smooth", "j2smooth"):
+        setattr(s, k, int(sc.get(k, 0)))
+    s.use_constructor = int(sc.get("use_constructor", 1))
+    s.use4pt = int(sc.get("use4pt", 1))
+    s.irosstab = int(mode)
+    for k in ("flux", "mixlth", "grav", "tau1lg", "steplg", "pradk0", "tfloor", "wtjm1", "wtj",
+              "wtjp1"):
+        setattr(s, k, float(sc[k]))
+    keep = {k: _f64(arr[k]) for k in FIN_IN}
+    out = {k: _f64(np.array(arr[k], np.float64).copy()) for k in FIN_IO}
+    out.update({k: np.zeros(n) for k in FIN_OUT})
+    out["bhyd"] = _f64(np.array(arr["bhyd"], np.float64).copy())
+    acc = _f64(arr["accrad"])
+    for k, a in list(keep.items()) + [(k, out[k]) for k in FIN_IO] + [("accrad", acc)]:
+        if a.shape != (n,):
+            raise ValueError(f"{k} must have shape ({n},)")
+    if out["bhyd"].shape != (6, n):
+        raise ValueError(f"bhyd must have shape (6, {n})")
+    for k in FIN_IN:
+        setattr(s, k, _ptr(keep[k]))
+    io = TcorrFinIo()
+    for k in FIN_IO + FIN_OUT:
+        setattr(io, k, _ptr(out[k]))
+    io.accrad, io.bhyd = _ptr(acc), _ptr(out["bhyd"])
+    t1o = _f64(np.array(t1, np.float64).copy())
+    if t1o.shape != (n,):
+        raise ValueError(f"t1 must have shape ({n},)")
+    drhox = np.zeros(n)
+    nf = C.c_int32(0)
+    ctx._check(ctx.lib.a12_atlas_tcorr3_finish(ctx.h, C.byref(s), C.byref(io), C.byref(state),
+                                               _ptr(t1o), _ptr(drhox), C.byref(nf)),
+               "a12_atlas_tcorr3_finish")
+    return out, t1o, drhox, nf.value
 
 
 class MolTab(C.Structure):

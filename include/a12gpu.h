@@ -286,7 +286,8 @@ int a12_atlas_rt_pass_noalloc(a12_ctx *ctx, int32_t nrhox, int32_t numnu, double
  * flux-constancy (DTFLUX), Lambda-iteration (DTLAMB) and surface (DTSURF)
  * temperature corrections and the iteration damping, after ROSS(3),
  * RADIAP(3), COMPUTE_HEIGHT and CONVEC (src/atlas12c.f90:582-587).  The
- * driver keeps CZ_CONSTRUCT and the RHOX correction that follow (:2816-).
+ * sections that follow (CZ_CONSTRUCT, the RHOX correction, the remap onto
+ * TAUSTD, :2816-2967) are a12_atlas_tcorr3_finish, on out's T1 and FLXERR.
  * Arrays (nrhox) are the module arrays of those names.  flxrad, rjmins,
  * rdiagj NULL: the sums of the last a12_atlas_rt_run, read on the device
  * (rdabh then optional, receiving the updated sum).  rdabh (TCORR's RDABH)
@@ -302,6 +303,95 @@ typedef struct {
 } a12_tcorr_in;
 int a12_atlas_tcorr3(a12_ctx *ctx, const a12_tcorr_in *in, double *rdabh, double *oldt1,
                      double *out);
+
+/* ROSSTAB's Rosseland-opacity table (module state ROSSTAB_*,
+ * src/atlas12_modules.f90:1779-1785) kept on the device by the context.
+ * a12_rosstab_store is ROSSTAB(0, 0, 0) (:6003-6042; the driver's call at
+ * src/atlas12c.f90:583): it appends the nrhox layers (T, P, ABROSS),
+ * the first store fixing the normalisation, and stops at
+ * A12_ROSSTAB_MAXTAB entries as the reference does.  a12_rosstab_reset
+ * empties it (a new model); a12_rosstab_load replaces it by a given table
+ * (zero: ZEROT, SLOPET, ZEROP, SLOPEP; tab (3, nross): TABT, TABP, ROSS,
+ * entry fastest); a12_rosstab_fetch copies it out (tab may be NULL).
+ * a12_rosstab_lookup is ROSSTAB(T, P, V) (:6047-6268) for nq queries in
+ * mode 1 (the reference's IROSSTAB default: nearest point per (T, P)
+ * quadrant and a bilinear blend), 2 (Shepard) or 3 (moving least squares);
+ * one wavefront per query.  Host pointers, synchronous. */
+#define A12_ROSSTAB_MAXTAB 4800
+int a12_rosstab_reset(a12_ctx *ctx);
+int a12_rosstab_store(a12_ctx *ctx, int32_t nrhox, const double *t, const double *p,
+                      const double *abross);
+int a12_rosstab_load(a12_ctx *ctx, int32_t nross, const double *zero, const double *tab);
+int a12_rosstab_fetch(a12_ctx *ctx, int32_t *nross, double *zero, double *tab);
+int a12_rosstab_lookup(a12_ctx *ctx, int32_t mode, int32_t nq, const double *temp,
+                       const double *pressure, double *kappa);
+
+/* TTAUP(T, TAU, ABSTD, PTOTAL, P, PRAD, PTURB, VTURB, GRAV, NUMTAU),
+ * src/atlas12_modules.f90:6303-6523: the hydrostatic march on a log-uniform
+ * tau grid with the context's ROSSTAB table in mode irosstab (predictor,
+ * corrector with stall-driven relaxation, best-iterate fallback, surface
+ * bootstrap).  One wavefront; *nfail (optional) = depths whose corrector
+ * hit its cap of 50.  Host pointers, synchronous. */
+int a12_ttaup(a12_ctx *ctx, int32_t irosstab, int32_t n, const double *t, const double *tau,
+              const double *prad, const double *pturb, double grav, double *abstd,
+              double *ptotal, double *p, int32_t *nfail);
+
+/* CZ_CONSTRUCT's SAVE state (src/atlas12_modules.f90:3054-3069) and the
+ * module flags CZC_RUN_ENGAGED / CZC_RUN_JC it sets; the caller keeps it
+ * between iterations (start: all 0, last_dtmax 1e30; ITER <= 1 resets it
+ * as the reference does). */
+typedef struct {
+    int32_t prev_engaged, jc_sticky, nbot_bad, run_engaged, run_jc;
+    double last_dtmax, rel_err;
+} a12_czc_state;
+
+/* The module arrays CZ_CONSTRUCT and CZC_DELREQ_LAYER read (nrhox each)
+ * and the switches USE_CZ_CONSTRUCTOR, USE_4POINT_MLT, IROSSTAB. */
+typedef struct {
+    int32_t nrhox, ifconv, iter, use_constructor, use4pt, irosstab;
+    double flux, mixlth, grav;
+    const double *flxcnv, *flxrad, *dltdlp, *grdadb, *flxerr, *t, *rhox, *ptotal, *rho,
+        *heatcp, *dlrdlt, *p, *vturb, *abross, *hscale;
+} a12_czc_in;
+
+/* CZ_CONSTRUCT(T1), src/atlas12_modules.f90:3012-3287 with
+ * CZC_DELREQ_LAYER :3298-3358: the efficient-convection temperature
+ * corrector folded into t1 (nrhox, in place); *engaged (optional) 1 when
+ * it changed t1.  Uses the context's ROSSTAB table.  One wavefront. */
+int a12_cz_construct(a12_ctx *ctx, const a12_czc_in *in, a12_czc_state *st, double *t1,
+                     int32_t *engaged);
+
+/* TCORR(3) after its table, src/atlas12_modules.f90:2816-2967:
+ *   (G2) CZ_CONSTRUCT on t1;
+ *   (H)  TAUSTD = 10**(TAU1LG + (J-1) STEPLG), TTAUP on T and on T + T1
+ *        mapped onto it, DRHOX from the pressure change mapped back;
+ *   (I)  T += T1, the optional 3-point smoothing (J1SMOOTH..J2SMOOTH),
+ *        monotonicity (T(J) <= T(J+1) - 1), the floor TFLOOR_ATM;
+ *   (J)  TK, HKT, HCKT, TKEV, TLOG;
+ *   (K)  RHOX += DRHOX, MAP1 of RHOX, T, P, XNE, ABROSS, PRAD, VTURB, BMIN,
+ *        PTURB and BHYD(:, 1:6) onto TAUSTD, the ABROSS guard, PRADK =
+ *        PRAD + PRADK0, TAUROS = TAUSTD.
+ * The Newton step (G) is off in the reference (USE_DTGRAD = .FALSE.,
+ * :1457), and CZC_POLISH (:2974, final iterations, only after the
+ * constructor engaged: st->run_engaged) re-runs the EOS and CONVEC: both
+ * stay with the driver.  t1 (nrhox): T1 after section (F) in, after (G2)
+ * out.  io arrays are the module arrays, updated in place; drhox (nrhox,
+ * optional) out; *nfail (optional) TTAUP depths not converged.  Uses the
+ * context's ROSSTAB table.  One wavefront, host pointers, synchronous. */
+typedef struct {
+    int32_t nrhox, ifconv, iter, use_constructor, use4pt, irosstab, j1smooth, j2smooth;
+    double flux, mixlth, grav, tau1lg, steplg, pradk0, tfloor, wtjm1, wtj, wtjp1;
+    const double *flxcnv, *flxrad, *dltdlp, *grdadb, *flxerr, *ptotal, *rho, *heatcp,
+        *dlrdlt, *hscale;
+} a12_tcorr_fin_in;
+typedef struct {
+    double *tauros, *rhox, *t, *p, *xne, *abross, *prad, *vturb, *bmin, *pturb; /* in/out */
+    const double *accrad;  /* remapped by (K) into a scratch column, not kept */
+    double *bhyd;          /* (nrhox, 6) BHYD, in/out                          */
+    double *tk, *hkt, *hckt, *tkev, *tlog, *pradk;                             /* out    */
+} a12_tcorr_fin_io;
+int a12_atlas_tcorr3_finish(a12_ctx *ctx, const a12_tcorr_fin_in *in, a12_tcorr_fin_io *io,
+                            a12_czc_state *st, double *t1, double *drhox, int32_t *nfail);
 
 /* NMOLEC's gas-phase Newton solve, src/atlas12_modules.f90:9078-9217, for
  * a batch of systems (depths of one or many models).  The molecule table
