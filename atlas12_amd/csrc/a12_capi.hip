@@ -575,6 +575,13 @@ int a12_atlas_tcorr3_finish(a12_ctx *ctx, const a12_tcorr_fin_in *in, a12_tcorr_
     return a12i_tcorr3_finish(ctx, in, io, st, t1, drhox, nfail);
 }
 
+int a12_convec(a12_ctx *ctx, const a12_convec_in *in, a12_convec_io *io, int32_t mlt_only)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_convec(ctx, in, io, mlt_only);
+}
+
 int a12_molec_set_table(a12_ctx *ctx, const a12_moltab *tab)
 {
     if (!ctx_ok(ctx)) return A12_ERR_ARG;

@@ -244,6 +244,7 @@ int a12i_cz_construct(a12_ctx *ctx, const a12_czc_in *in, a12_czc_state *st, dou
                       int32_t *engaged);
 int a12i_tcorr3_finish(a12_ctx *ctx, const a12_tcorr_fin_in *in, a12_tcorr_fin_io *io,
                        a12_czc_state *st, double *t1, double *drhox, int32_t *nfail);
+int a12i_convec(a12_ctx *ctx, const a12_convec_in *in, a12_convec_io *io, int32_t mlt_only);
 int a12i_solvit_batched(a12_ctx *ctx, int n, int nsys, const double *a, const double *b,
                         double *x);
 int a12i_model_converge(a12_ctx *ctx, int nmodel, const a12_model_par *par,

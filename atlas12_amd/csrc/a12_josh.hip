@@ -1548,6 +1548,317 @@ __global__ __launch_bounds__(64) void k_cz_construct(RossTab tb, CzcPar c,
     if (threadIdx.x == 0) *engaged = e;
 }
 
+// --------------------------------------------------------------------------
+// CONVEC, src/atlas12_modules.f90:10323-10953, given the four EOS
+// evaluations it makes (EDENS and RHO at T x 1.001, T x 0.999, P x 1.001,
+// P x 0.999; the EOS stays with the driver).  k_convec_mlt: one wavefront
+// per depth -- the perturb-and-restore drift of T, TK, HKT, HCKT, TKEV,
+// TLOG, P, the finite-difference thermodynamics and the MLT solve with its
+// bubble-opacity iteration (ROSSTAB looked up by the whole wave);
+// k_convec_post: one wavefront -- gap fill, bottom clamp, 1-2-1 smoothing,
+// overshooting, surface zeroing.
+struct ConvecPar {
+    int n, ifconv, ifcorr, nconv, use4pt, smooth, irosstab;
+    double grav, mixlth, flux, overwt;
+};
+
+// in (n each): PTOTAL RHO PRADK TAUROS RHOX ABROSS VTURB HEIGHT FLXRAD,
+// EDENS(:, 1:4), RHO(:, 1:4), T as CONVEC finds it -> 18 n; io (19 n): T
+// TK HKT HCKT TKEV TLOG P (in/out), DLTDLP HEATCP VELSND DLRDLT GRDADB
+// HSCALE VCONV FLXCNV FLXCNV0 FLXCNV1 ABCONV DELTAT.  The waves of
+// k_convec_mlt read T's neighbours from in (io's T is overwritten).
+enum { CV_PTOT, CV_RHO, CV_PRADK, CV_TAUROS, CV_RHOX, CV_ABROSS, CV_VTURB, CV_HEIGHT, CV_FLXRAD,
+       CV_EDENS, CV_RHO4 = CV_EDENS + 4, CV_T0 = CV_RHO4 + 4, CV_NIN };
+enum { CO_T, CO_TK, CO_HKT, CO_HCKT, CO_TKEV, CO_TLOG, CO_P, CO_DLTDLP, CO_HEATCP, CO_VELSND,
+       CO_DLRDLT, CO_GRDADB, CO_HSCALE, CO_VCONV, CO_FLXCNV, CO_FLXCNV0, CO_FLXCNV1, CO_ABCONV,
+       CO_DELTAT, CO_N };
+
+// One element of DERIV (the expression deriv_wave evaluates per lane).
+__device__ double deriv_at(const double *x, const double *f, int n, int j)
+{
+#pragma clang fp contract(off)
+    if (j == 0) return (f[1] - f[0]) / (x[1] - x[0]);
+    if (j == n - 1) return (f[n - 1] - f[n - 2]) / (x[n - 1] - x[n - 2]);
+    const double s = fabs(x[1] - x[0]) / (x[1] - x[0]);
+    double scale = fmax(fmax(fabs(f[j - 1]), fabs(f[j])), fabs(f[j + 1])) / fabs(x[j]);
+    if (scale == 0.0) scale = 1.0;
+    const double dr = (f[j + 1] - f[j]) / (x[j + 1] - x[j]) / scale;
+    const double dl = (f[j] - f[j - 1]) / (x[j] - x[j - 1]) / scale;
+    const double tr = dr / (s * sqrt(1.0 + dr * dr) + 1.0);
+    const double tl = dl / (s * sqrt(1.0 + dl * dl) + 1.0);
+    return (tr + tl) / (1.0 - tr * tl) * scale;
+}
+
+// The restored T of depth j after CONVEC's perturb-and-restore
+// (:10469-10519): T * 1.001, / 1.001 * 0.999, / 0.999.
+__device__ __forceinline__ double convec_drift_t(double t)
+{
+#pragma clang fp contract(off)
+    t = t * 1.001;
+    t = t / 1.001 * 0.999;
+    return t / 0.999;
+}
+
+// Bubble opacity (:10636-10648): harmonic mean of ROSSTAB over the
+// bubble's cross-section, 2 or 4 points.
+__device__ double convec_abconv(const RossTab &tb, const ConvecPar &c, double tj, double pj,
+                                double dt, double rosst, double abross)
+{
+#pragma clang fp contract(off)
+    const double dplus = rosstab_lookup_wave(tb, c.irosstab, tj + dt, pj) / rosst;
+    const double dminus = rosstab_lookup_wave(tb, c.irosstab, tj - dt, pj) / rosst;
+    if (c.use4pt) {
+        const double dpi = rosstab_lookup_wave(tb, c.irosstab, tj + dt / 3.0, pj) / rosst;
+        const double dmi = rosstab_lookup_wave(tb, c.irosstab, tj - dt / 3.0, pj) / rosst;
+        return 4.0 / (1.0 / dplus + 1.0 / dpi + 1.0 / dminus + 1.0 / dmi) * abross;
+    }
+    return 2.0 / (1.0 / dplus + 1.0 / dminus) * abross;
+}
+
+__global__ __launch_bounds__(64) void k_convec_mlt(RossTab tb, ConvecPar c,
+                                                   const double *__restrict__ in,
+                                                   double *__restrict__ io)
+{
+#pragma clang fp contract(off)
+    const int n = c.n, j = blockIdx.x, J = j + 1;
+    const bool w0 = threadIdx.x == 0;
+    const double *ptotal = in + CV_PTOT * n, *rho = in + CV_RHO * n, *pradk = in + CV_PRADK * n;
+    const double *rhox = in + CV_RHOX * n, *abross = in + CV_ABROSS * n;
+    const double *tin = in + CV_T0 * n;  // T as CONVEC finds it (DERIV reads this one)
+    const double fourpi = 4.0 * 3.141592653589793, sigma_sb = 5.670374419e-5;
+    const double dtdrhx = deriv_at(rhox, tin, n, j);
+    const double dilut = 1.0 - exp(-in[CV_TAUROS * n + j]);
+    const double c4p = ((1.001 * 1.001) * 1.001) * 1.001 - 1.0;
+    const double c4m = ((0.999 * 0.999) * 0.999) * 0.999 - 1.0;
+    const double *ed = in + CV_EDENS * n, *r4 = in + CV_RHO4 * n;
+    const double e1 = ed[j] + 3.0 * pradk[j] / r4[j] * (1.0 + dilut * c4p);
+    const double e2 = ed[n + j] + 3.0 * pradk[j] / r4[n + j] * (1.0 + dilut * c4m);
+    const double e3 = ed[2 * n + j] + 3.0 * pradk[j] / r4[2 * n + j];
+    const double e4 = ed[3 * n + j] + 3.0 * pradk[j] / r4[3 * n + j];
+    // the drift of every perturbed module vector (:10468-10535)
+    double tl = io[CO_TLOG * n + j] + 0.0009995003;
+    tl = tl - 0.0009995003 - 0.0010005003;
+    tl = tl + 0.0010005003;
+    const double t = convec_drift_t(tin[j]);
+    double tk = io[CO_TK * n + j] * 1.001;
+    tk = tk / 1.001 * 0.999;
+    tk = tk / 0.999;
+    double hkt = io[CO_HKT * n + j] / 1.001;
+    hkt = hkt * 1.001 / 0.999;
+    hkt = hkt * 0.999;
+    double hckt = io[CO_HCKT * n + j] / 1.001;
+    hckt = hckt * 1.001 / 0.999;
+    hckt = hckt * 0.999;
+    double tkev = io[CO_TKEV * n + j] * 1.001;
+    tkev = tkev / 1.001 * 0.999;
+    tkev = tkev / 0.999;
+    double p = io[CO_P * n + j] * 1.001;
+    p = p / 1.001 * 0.999;
+    p = p / 0.999;
+    // thermodynamics (:10540-10592)
+    const double dedt = (e1 - e2) / t * 500.0;
+    const double drdt = (r4[j] - r4[n + j]) / t * 500.0;
+    const double dedpg = (e3 - e4) / p * 500.0;
+    const double drdpg = (r4[2 * n + j] - r4[3 * n + j]) / p * 500.0;
+    const double dpdpg = 1.0;
+    const double dpdt = 4.0 * pradk[j] / t * dilut;
+    double dltdlp = ptotal[j] / t / c.grav * dtdrhx;
+    if (J == n && J >= 2) {
+        const double tm = convec_drift_t(tin[j - 1]);
+        if (t > tm && ptotal[j] > ptotal[j - 1]) dltdlp = log(t / tm) / log(ptotal[j] / ptotal[j - 1]);
+    }
+    const double heatcv = dedt - dedpg * drdt / drdpg;
+    const double heatcp =
+        dedt - dedpg * dpdt / dpdpg - ptotal[j] / (rho[j] * rho[j]) * (drdt - drdpg * dpdt / dpdpg);
+    const double velsnd = heatcv <= 0.0 ? 0.0 : sqrt(fmax(heatcp / heatcv * dpdpg / drdpg, 0.0));
+    const double dlrdlt = t / rho[j] * (drdt - drdpg * dpdt / dpdpg);
+    const double grdadb = -ptotal[j] / rho[j] / t * dlrdlt / heatcp;
+    const double hscale = ptotal[j] / rho[j] / c.grav;
+    double vconv = 0.0, flxcnv = 0.0, abconv = abross[j], deltat = 0.0;
+    // the MLT solve (:10594-10797)
+    const double del = dltdlp - grdadb;
+    const double vco = 0.5 * c.mixlth * sqrt(fmax(-0.5 * ptotal[j] / rho[j] * dlrdlt, 0.0));
+    if (c.mixlth != 0.0 && J >= 4 && !(del < 0.0) && vco != 0.0) {
+        const double fluxco = 0.5 * rho[j] * heatcp * t * c.mixlth / fourpi;
+        const double pj = p;
+        const double rosst = rosstab_lookup_wave(tb, c.irosstab, t, pj);
+        double olddelt = 0.0;
+        const int its30 = c.ifconv == 0 ? 1 : 30;
+        const double omega = 0.7;
+        double sa = 0.0, sb = 0.0, sc = 0.0;
+        double hist[12];
+        int shave = 0, hist_n = 0;
+        for (int id = 1; id <= its30; ++id) {
+            abconv = convec_abconv(tb, c, t, pj, deltat, rosst, abross[j]);
+            const double t4 = ((t * t) * t) * t;
+            double d = 8.0 * sigma_sb * t4 / (abconv * hscale * rho[j]) / (fluxco * fourpi) / vco;
+            const double taub = abconv * rho[j] * c.mixlth * hscale;
+            d = d * (taub * taub) / (2.0 + taub * taub);
+            d = d * d / 2.0;
+            const double q = del / (d + del);
+            const double ddd = q * q;
+            double delta;
+            if (ddd >= 0.5) {
+                delta = (1.0 - sqrt(1.0 - ddd)) / ddd;
+            } else {
+                delta = 0.5;
+                double term = 0.5, up = -1.0, down = 2.0;
+                for (;;) {
+                    up = up + 2.0;
+                    down = down + 2.0;
+                    term = up / down * ddd * term;
+                    delta = delta + term;
+                    if (term <= 1.0e-6) break;
+                }
+            }
+            delta = delta * (del * del) / (d + del);
+            vconv = vco * sqrt(delta);
+            flxcnv = fmax(fluxco * vconv * delta, 0.0);
+            double dt = t * c.mixlth * delta;
+            dt = fmin(dt, t * 0.15);
+            dt = dt * omega + olddelt * (1.0 - omega);
+            deltat = dt;
+            const double dstep = dt - olddelt;
+            if (fabs(dstep) < 0.5 || fabs(dstep) < 1.0e-3 * fmax(fabs(dt), 1.0)) break;
+            sa = sb;
+            sb = sc;
+            sc = dt;
+            shave = min(shave + 1, 3);
+            if (shave == 3 && id % 3 == 0) {  // Steffensen (:10707-10728)
+                const double den = sc - 2.0 * sb + sa;
+                if (fabs(den) > 1.0e-6 * fmax(fabs(sc), 1.0)) {
+                    const double num = sc - sb;
+                    const double x = sc - num * num / den;
+                    if (x > 0.0 && fabs(x - sc) < 0.5 * fmax(fabs(sc), 1.0)) {
+                        deltat = x;
+                        olddelt = x;
+                        shave = 0;
+                        continue;
+                    }
+                }
+            }
+            for (int ii = min(hist_n, 11); ii >= 1; --ii) hist[ii] = hist[ii - 1];
+            hist[0] = dt;
+            hist_n = min(hist_n + 1, 12);
+            bool cycle_exit = false;
+            double cmean = 0.0;
+            if (hist_n >= 8) {  // cycle-mean exit (:10738-10768)
+                for (int ii = 2; ii <= 6 && !cycle_exit; ++ii) {
+                    if (hist_n < 2 * ii) continue;
+                    const double tol = 5.0e-4 * fmax(fabs(hist[0]), 1.0);
+                    bool all = true;
+                    for (int k = 0; k < ii; ++k)
+                        if (!(fabs(hist[k] - hist[ii + k]) <= tol)) all = false;
+                    if (all) {
+                        double cmin = hist[0], cmax = hist[0], s = 0.0;
+                        for (int k = 0; k < ii; ++k) {
+                            cmin = fmin(cmin, hist[k]);
+                            cmax = fmax(cmax, hist[k]);
+                            s = s + hist[k];
+                        }
+                        cmean = s / (double)ii;
+                        if (cmax - cmin > 2.0e-3 * fmax(fabs(cmean), 1.0)) cycle_exit = true;
+                    }
+                }
+            }
+            if (cycle_exit) {
+                deltat = cmean;
+                abconv = convec_abconv(tb, c, t, pj, cmean, rosst, abross[j]);
+                break;
+            }
+            olddelt = dt;
+        }
+    }
+    if (w0) {
+        const double v[CO_N] = {t, tk, hkt, hckt, tkev, tl, p, dltdlp, heatcp, velsnd, dlrdlt,
+                                grdadb, hscale, vconv, flxcnv, 0.0, 0.0, abconv, deltat};
+        for (int k = 0; k < CO_N; ++k)
+            if (k != CO_FLXCNV0 && k != CO_FLXCNV1) io[k * n + j] = v[k];
+    }
+}
+
+__global__ __launch_bounds__(64) void k_convec_post(ConvecPar c, const double *__restrict__ in,
+                                                    double *__restrict__ io)
+{
+#pragma clang fp contract(off)
+    __shared__ JoshLDS W;
+    __shared__ double cnvint[NR], delhgt[NR], xs[2], ys[2];
+    const int n = c.n, ln = lane_id();
+    double *f = io + CO_FLXCNV * n, *f0 = io + CO_FLXCNV0 * n, *f1 = io + CO_FLXCNV1 * n;
+    const double *height = in + CV_HEIGHT * n, *flxrad = in + CV_FLXRAD * n;
+    const double *dltdlp = io + CO_DLTDLP * n, *grdadb = io + CO_GRDADB * n,
+                 *hscale = io + CO_HSCALE * n;
+    if (ln == 0) {
+        // radiative gaps inside the zone (:10813-10841), serial as written
+        int jtop = 0, jbot = 0;
+        for (int J = 1; J <= n; ++J)
+            if (f[J - 1] > 0.0) {
+                if (jtop == 0) jtop = J;
+                jbot = J;
+            }
+        if (jtop > 0 && jbot > jtop + 1)
+            for (int J = jtop + 1; J <= jbot - 1; ++J)
+                if (f[J - 1] == 0.0) {
+                    int ja = J - 1, jb = J + 1;
+                    while (ja > jtop && f[ja - 1] == 0.0) ja = ja - 1;
+                    while (jb < jbot && f[jb - 1] == 0.0) jb = jb + 1;
+                    if (f[ja - 1] > 0.0 && f[jb - 1] > 0.0) {
+                        const double w = (double)(J - ja) / (double)(jb - ja);
+                        f[J - 1] = f[ja - 1] * (1.0 - w) + f[jb - 1] * w;
+                    }
+                }
+        if (c.ifcorr == 1) {  // bottom clamp (:10867-10878)
+            jbot = 0;
+            for (int J = 1; J <= n; ++J)
+                if (f[J - 1] > 0.0) jbot = J;
+            if (jbot >= n - 3 && jbot < n)
+                for (int J = jbot + 1; J <= n; ++J) {
+                    if (dltdlp[J - 1] - grdadb[J - 1] > -0.01) f[J - 1] = fmax(c.flux - flxrad[J - 1], 0.0);
+                    else break;
+                }
+        }
+    }
+    wsync();
+    for (int j = ln; j < n; j += 64) f0[j] = f[j];  // smoothing (:10885-10893)
+    wsync();
+    if (c.smooth)
+        for (int J = 2 + ln; J <= n - 1; J += 64)
+            f[J - 1] = 0.25 * f0[J - 2] + 0.50 * f0[J - 1] + 0.25 * f0[J];
+    wsync();
+    for (int j = ln; j < n; j += 64) f0[j] = f[j];
+    wsync();
+    if (c.overwt > 0.0) {  // overshooting (:10901-10927)
+        double wtcnv = 0.0;
+        for (int j = 0; j < n; ++j) wtcnv = fmax(wtcnv, f[j] / c.flux);
+        wtcnv = fmin(wtcnv, 1.0) * c.overwt;
+        for (int j = ln; j < n; j += 64) {
+            delhgt[j] = fmin(fmin(hscale[j] * 0.5e-5 * wtcnv, height[n - 1] - height[j]),
+                             height[j] - height[0]);
+            f0[j] = f[j];
+            f1[j] = 0.0;
+        }
+        wsync();
+        integ_wave(W, height, f, cnvint, n, 0.0);
+        for (int J = n / 2; J <= n - 1; ++J) {
+            if (delhgt[J - 1] == 0.0) continue;
+            if (ln == 0) {
+                xs[0] = height[J - 1] - delhgt[J - 1];
+                xs[1] = height[J - 1] + delhgt[J - 1];
+            }
+            wsync();
+            (void)map1_wave(height, cnvint, n, xs, ys, 1);
+            wsync();
+            (void)map1_wave(height, cnvint, n, xs + 1, ys + 1, 1);
+            wsync();
+            if (ln == 0) f1[J - 1] = f1[J - 1] + (ys[1] - ys[0]) / delhgt[J - 1] / 2.0;
+            wsync();
+        }
+        for (int j = ln; j < n; j += 64) f[j] = fmax(f0[j], f1[j]);
+        wsync();
+    }
+    for (int J = 1 + ln; J <= c.nconv && J <= n; J += 64) f[J - 1] = 0.0;  // (:10930-10932)
+}
+
 // TCORR(3) sections (G2)-(K).  in (10, n): FLXCNV FLXRAD DLTDLP GRDADB
 // FLXERR PTOTAL RHO HEATCP DLRDLT HSCALE; io (17 + 6, n): TAUROS RHOX T P
 // XNE ABROSS PRAD VTURB BMIN PTURB ACCRAD, BHYD(:, 1:6), then out TK HKT
@@ -2638,6 +2949,54 @@ int a12i_cz_construct(a12_ctx *ctx, const a12_czc_in *in, a12_czc_state *st, dou
     A12_HIP(hipMemcpyAsync(&e, deng, sizeof(int), hipMemcpyDeviceToHost, s));
     A12_HIP(hipStreamSynchronize(s));
     if (engaged) *engaged = e;
+    return A12_OK;
+}
+
+int a12i_convec(a12_ctx *ctx, const a12_convec_in *in, a12_convec_io *io, int32_t mlt_only)
+{
+    if (!in || !io || in->nrhox < 4 || in->nrhox > A12_NRMAX) {
+        ctx->err = "a12_convec: bad layer count or null argument";
+        return A12_ERR_ARG;
+    }
+    const int n = in->nrhox;
+    const double *cin[9] = {in->ptotal, in->rho, in->pradk, in->tauros, in->rhox, in->abross,
+                            in->vturb, in->height, in->flxrad};
+    double *cio[CO_N] = {io->t, io->tk, io->hkt, io->hckt, io->tkev, io->tlog, io->p,
+                         io->dltdlp, io->heatcp, io->velsnd, io->dlrdlt, io->grdadb,
+                         io->hscale, io->vconv, io->flxcnv, io->flxcnv0, io->flxcnv1,
+                         io->abconv, io->deltat};
+    bool ok = all_set(cin, 9) && in->edens && in->rho4;
+    for (double *c : cio) ok = ok && c;
+    if (!ok) {
+        ctx->err = "a12_convec: null state array";
+        return A12_ERR_ARG;
+    }
+    int rc = ross_need(ctx, in->irosstab, "a12_convec");
+    if (rc != A12_OK) return rc;
+    hipStream_t s = ctx->stream;
+    A12_HIP(ctx->ross.io.reserve(sizeof(double) * (size_t)(CV_NIN + CO_N) * n));
+    double *d = ctx->ross.io.as<double>(), *dio = d + (size_t)CV_NIN * n;
+    for (int k = 0; k < 9; ++k)
+        A12_HIP(hipMemcpyAsync(d + (size_t)k * n, cin[k], sizeof(double) * n, hipMemcpyHostToDevice, s));
+    A12_HIP(hipMemcpyAsync(d + (size_t)CV_EDENS * n, in->edens, sizeof(double) * 4 * n, hipMemcpyHostToDevice, s));
+    A12_HIP(hipMemcpyAsync(d + (size_t)CV_RHO4 * n, in->rho4, sizeof(double) * 4 * n, hipMemcpyHostToDevice, s));
+    A12_HIP(hipMemcpyAsync(d + (size_t)CV_T0 * n, io->t, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    for (int k = 0; k <= CO_P; ++k)
+        A12_HIP(hipMemcpyAsync(dio + (size_t)k * n, cio[k], sizeof(double) * n, hipMemcpyHostToDevice, s));
+    A12_HIP(hipMemcpyAsync(dio + (size_t)CO_FLXCNV1 * n, io->flxcnv1, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    const ConvecPar c = {n, in->ifconv, in->ifcorr, in->nconv, in->use4pt, in->smooth, in->irosstab,
+                         in->grav, in->mixlth, in->flux, in->overwt};
+    hipLaunchKernelGGL(k_convec_mlt, dim3(n), dim3(64), 0, s, ross_view(ctx), c, d, dio);
+    A12_CHECK_LAUNCH();
+    if (!mlt_only) {
+        hipLaunchKernelGGL(k_convec_post, dim3(1), dim3(64), 0, s, c, d, dio);
+        A12_CHECK_LAUNCH();
+    }
+    for (int k = 0; k < CO_N; ++k) {
+        if (mlt_only && (k == CO_FLXCNV0 || k == CO_FLXCNV1)) continue;  // untouched then
+        A12_HIP(hipMemcpyAsync(cio[k], dio + (size_t)k * n, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    }
+    A12_HIP(hipStreamSynchronize(s));
     return A12_OK;
 }
 

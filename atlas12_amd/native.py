@@ -45,7 +45,8 @@ EXPORTS = ["a12_ctx_create", "a12_ctx_destroy", "a12_last_error", "a12_stream",
            "a12_kapp_constants", "a12_kapp_ms", "a12_atlas_kapp_fetch", "a12_cplx_load",
            "a12_cplx_run", "a12_cplx_fetch", "a12_cplx_clear", "a12_cplx_ms",
            "a12_rosstab_reset", "a12_rosstab_store", "a12_rosstab_load", "a12_rosstab_fetch",
-           "a12_rosstab_lookup", "a12_ttaup", "a12_cz_construct", "a12_atlas_tcorr3_finish"]
+           "a12_rosstab_lookup", "a12_ttaup", "a12_cz_construct", "a12_atlas_tcorr3_finish",
+           "a12_convec"]
 
 
 class A12Error(RuntimeError):
@@ -695,6 +696,62 @@ def atlas_tcorr3_finish(ctx: Context, sc, arr, t1, state: CzcState, mode: int = 
                                                _ptr(t1o), _ptr(drhox), C.byref(nf)),
                "a12_atlas_tcorr3_finish")
     return out, t1o, drhox, nf.value
+
+
+CNV_IN = ("ptotal", "rho", "pradk", "tauros", "rhox", "abross", "vturb", "height", "flxrad")
+CNV_STATE = ("t", "tk", "hkt", "hckt", "tkev", "tlog", "p")
+CNV_OUT = ("dltdlp", "heatcp", "velsnd", "dlrdlt", "grdadb", "hscale", "vconv", "flxcnv",
+           "flxcnv0", "flxcnv1", "abconv", "deltat")
+
+
+class ConvecIn(C.Structure):
+    _fields_ = [(k, C.c_int32) for k in ("nrhox", "ifconv", "ifcorr", "nconv", "use4pt", "smooth",
+                                         "irosstab")] + \
+               [(k, C.c_double) for k in ("grav", "mixlth", "flux", "overwt")] + \
+               [(k, P_D) for k in CNV_IN + ("edens", "rho4")]
+
+
+class ConvecIo(C.Structure):
+    _fields_ = [(k, P_D) for k in CNV_STATE + CNV_OUT]
+
+
+def convec(ctx: Context, sc, arr, edens, rho4, mode: int = 1, mlt_only: bool = False):
+    """CONVEC on the device.  sc: ifconv, ifcorr, nconv, use4pt, smooth,
+    grav, mixlth, flux, overwt; arr: CNV_IN, CNV_STATE and flxcnv1 (n);
+    edens, rho4 (4, n): the EOS's EDENS and RHO at T x 1.001, T x 0.999,
+    P x 1.001, P x 0.999.  Returns a dict of CNV_STATE (after CONVEC's
+    perturb-and-restore) and CNV_OUT."""
+    n = len(arr["t"])
+    s = ConvecIn()
+    s.nrhox = n
+    for k in ("ifconv", "ifcorr", "nconv", "use4pt", "smooth"):
+        setattr(s, k, int(sc[k]))
+    s.irosstab = int(mode)
+    for k in ("grav", "mixlth", "flux", "overwt"):
+        setattr(s, k, float(sc[k]))
+    keep = {k: _f64(arr[k]) for k in CNV_IN}
+    e = _f64(np.asarray(edens, np.float64))
+    r = _f64(np.asarray(rho4, np.float64))
+    for k, a in keep.items():
+        if a.shape != (n,):
+            raise ValueError(f"{k} must have shape ({n},)")
+    if e.shape != (4, n) or r.shape != (4, n):
+        raise ValueError(f"edens and rho4 must have shape (4, {n})")
+    for k in CNV_IN:
+        setattr(s, k, _ptr(keep[k]))
+    s.edens, s.rho4 = _ptr(e), _ptr(r)
+    out = {k: _f64(np.array(arr[k], np.float64).copy()) for k in CNV_STATE}
+    out.update({k: np.zeros(n) for k in CNV_OUT})
+    out["flxcnv1"] = _f64(np.array(arr["flxcnv1"], np.float64).copy())
+    for k in CNV_STATE + ("flxcnv1",):
+        if out[k].shape != (n,):
+            raise ValueError(f"{k} must have shape ({n},)")
+    io = ConvecIo()
+    for k in CNV_STATE + CNV_OUT:
+        setattr(io, k, _ptr(out[k]))
+    ctx._check(ctx.lib.a12_convec(ctx.h, C.byref(s), C.byref(io), int(bool(mlt_only))),
+               "a12_convec")
+    return out
 
 
 class MolTab(C.Structure):

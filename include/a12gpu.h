@@ -393,6 +393,36 @@ typedef struct {
 int a12_atlas_tcorr3_finish(a12_ctx *ctx, const a12_tcorr_fin_in *in, a12_tcorr_fin_io *io,
                             a12_czc_state *st, double *t1, double *drhox, int32_t *nfail);
 
+/* CONVEC(MLT_ONLY), src/atlas12_modules.f90:10323-10953 (the driver's call
+ * src/atlas12c.f90:586, after ROSS(3), the ROSSTAB store, RADIAP(3) and
+ * COMPUTE_HEIGHT).  CONVEC evaluates the EOS four times, at T x 1.001,
+ * T x 0.999 and then (T restored) P x 1.001, P x 0.999 (:10468-10528); the
+ * EOS (COMPUTE_ONE_POP) stays with the driver, which passes the module
+ * EDENS and RHO after each of those evaluations: edens, rho4 (nrhox, 4),
+ * depth fastest, in that order.  On the device: the finite-difference
+ * thermodynamics (DLTDLP with the log-space bottom slope, HEATCP, VELSND,
+ * DLRDLT, GRDADB, HSCALE), the MLT solve of each depth with its bubble-
+ * opacity iteration on the context's ROSSTAB table (Picard, Steffensen,
+ * cycle-mean exit; one wavefront per depth), then, unless mlt_only, the
+ * gap fill, the bottom clamp (IFCORR = 1), the 1-2-1 smoothing
+ * (USE_FLXCNV_SMOOTH), overshooting (OVERWT > 0) and the zeroing of the
+ * top NCONV layers.  io: T, TK, HKT, HCKT, TKEV, TLOG, P in, returned as
+ * CONVEC's perturb-and-restore leaves them (within a few ulps of the
+ * input); FLXCNV1 in/out (kept unless overshooting); the rest out
+ * (ABCONV, DELTAT: CONVEC's locals).  Host pointers, synchronous. */
+typedef struct {
+    int32_t nrhox, ifconv, ifcorr, nconv, use4pt, smooth, irosstab;
+    double grav, mixlth, flux, overwt;
+    const double *ptotal, *rho, *pradk, *tauros, *rhox, *abross, *vturb, *height, *flxrad;
+    const double *edens, *rho4;
+} a12_convec_in;
+typedef struct {
+    double *t, *tk, *hkt, *hckt, *tkev, *tlog, *p;
+    double *dltdlp, *heatcp, *velsnd, *dlrdlt, *grdadb, *hscale, *vconv, *flxcnv, *flxcnv0,
+        *flxcnv1, *abconv, *deltat;
+} a12_convec_io;
+int a12_convec(a12_ctx *ctx, const a12_convec_in *in, a12_convec_io *io, int32_t mlt_only);
+
 /* NMOLEC's gas-phase Newton solve, src/atlas12_modules.f90:9078-9217, for
  * a batch of systems (depths of one or many models).  The molecule table
  * is READMOL's, 1-based as in the reference: locj (nummol + 1) the first

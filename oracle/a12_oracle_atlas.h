@@ -156,6 +156,27 @@ typedef struct {
 int or_tcorr3_finish(const or_tcorr_fin_in *in, or_tcorr_fin_io *io, or_czc_state *cz,
                      double *t1, double *drhox_out);
 
+/* CONVEC (src/atlas12_modules.f90:10323-10953) given the four EOS
+ * evaluations it makes: edens[k], rho4[k] (n) = the module EDENS and RHO
+ * after COMPUTE_ONE_POP at T x 1.001, T x 0.999 (P), then P x 1.001,
+ * P x 0.999 (T), in that order (:10468-10528).  height: COMPUTE_HEIGHT's
+ * HEIGHT; flxrad: the RT pass's FLXRAD. */
+typedef struct {
+    int n, ifconv, ifcorr, nconv, use4pt, smooth, irosstab;
+    double grav, mixlth, flux, overwt;
+    const double *ptotal, *rho, *pradk, *tauros, *rhox, *abross, *vturb, *height, *flxrad;
+    const double *edens[4], *rho4[4];
+    const or_rosstab *tab;
+} or_convec_in;
+/* In/out: T, TK, HKT, HCKT, TKEV, TLOG, P (the drift of CONVEC's perturb-
+ * and-restore), FLXCNV1 (kept unless overshooting); out: the rest. */
+typedef struct {
+    double *t, *tk, *hkt, *hckt, *tkev, *tlog, *p;
+    double *dltdlp, *heatcp, *velsnd, *dlrdlt, *grdadb, *hscale, *vconv, *flxcnv, *flxcnv0,
+        *flxcnv1, *abconv, *deltat;
+} or_convec_io;
+void or_convec(const or_convec_in *in, or_convec_io *io, int mlt_only);
+
 /* NMOLEC's molecule table (READMOL): 1-based as in the reference.  locj
  * (nummol + 1) start of each entry's components in kcomps; kcomps the
  * equation index of each component (nequa + 1 = electron, divides);

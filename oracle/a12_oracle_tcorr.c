@@ -572,3 +572,255 @@ int or_tcorr3_finish(const or_tcorr_fin_in *in, or_tcorr_fin_io *io, or_czc_stat
     memcpy(io->tauros, taustd, sizeof(double) * (size_t)n);
     return nfail;
 }
+
+/* ---- CONVEC, src/atlas12_modules.f90:10323-10953 ---------------------- */
+/* The mixing-length convection of one iteration given the four EOS
+ * evaluations CONVEC makes (EDENS and RHO at T x 1.001, T x 0.999,
+ * P x 1.001, P x 0.999; the EOS itself stays with the driver): the
+ * finite-difference thermodynamic derivatives, the per-depth MLT solve
+ * with its bubble-opacity iteration (Picard, Steffensen every third step,
+ * cycle-mean exit), the radiative-gap fill, the bottom clamp, the 1-2-1
+ * smoothing, overshooting and the surface zeroing.  T, TK, HKT, HCKT,
+ * TKEV, TLOG and P come back as the reference leaves them after its
+ * perturb-and-restore sequence (a few ulps from the input). */
+void or_convec(const or_convec_in *in, or_convec_io *io, int mlt_only)
+{
+    const int n = in->n;
+    const double fourpi = C_FOURPI, sigma_sb = C_SIGMA_SB;
+    double dtdrhx[KW_MAX], dilut[KW_MAX], e1[KW_MAX], e2[KW_MAX], e3[KW_MAX], e4[KW_MAX];
+    double cnvint[KW_MAX], delhgt[KW_MAX];
+    or_deriv(in->rhox, io->t, dtdrhx, n);                      /* :10452 */
+    for (int j = 0; j < n; ++j) dilut[j] = 1.0 - exp(-in->tauros[j]);
+    const double c4p = ((1.001 * 1.001) * 1.001) * 1.001 - 1.0;
+    const double c4m = ((0.999 * 0.999) * 0.999) * 0.999 - 1.0;
+    for (int j = 0; j < n; ++j) {                              /* :10468-10535 */
+        double tl = io->tlog[j] + 0.0009995003, t = io->t[j] * 1.001, tk = io->tk[j] * 1.001;
+        double hkt = io->hkt[j] / 1.001, hckt = io->hckt[j] / 1.001, tkev = io->tkev[j] * 1.001;
+        e1[j] = in->edens[0][j] + 3.0 * in->pradk[j] / in->rho4[0][j] * (1.0 + dilut[j] * c4p);
+        tl = tl - 0.0009995003 - 0.0010005003;
+        t = t / 1.001 * 0.999;
+        tk = tk / 1.001 * 0.999;
+        hkt = hkt * 1.001 / 0.999;
+        hckt = hckt * 1.001 / 0.999;
+        tkev = tkev / 1.001 * 0.999;
+        e2[j] = in->edens[1][j] + 3.0 * in->pradk[j] / in->rho4[1][j] * (1.0 + dilut[j] * c4m);
+        tl = tl + 0.0010005003;
+        t = t / 0.999;
+        tk = tk / 0.999;
+        hkt = hkt * 0.999;
+        hckt = hckt * 0.999;
+        tkev = tkev / 0.999;
+        double p = io->p[j] * 1.001;
+        e3[j] = in->edens[2][j] + 3.0 * in->pradk[j] / in->rho4[2][j];
+        p = p / 1.001 * 0.999;
+        e4[j] = in->edens[3][j] + 3.0 * in->pradk[j] / in->rho4[3][j];
+        p = p / 0.999;
+        io->t[j] = t;
+        io->tk[j] = tk;
+        io->hkt[j] = hkt;
+        io->hckt[j] = hckt;
+        io->tkev[j] = tkev;
+        io->tlog[j] = tl;
+        io->p[j] = p;
+    }
+    const double *t = io->t, *p = io->p, *rho = in->rho, *ptotal = in->ptotal;
+    for (int J = 1; J <= n; ++J) {                             /* :10540-10798 */
+        const int j = J - 1;
+        const double dedt = (e1[j] - e2[j]) / t[j] * 500.0;
+        const double drdt = (in->rho4[0][j] - in->rho4[1][j]) / t[j] * 500.0;
+        const double dedpg = (e3[j] - e4[j]) / p[j] * 500.0;
+        const double drdpg = (in->rho4[2][j] - in->rho4[3][j]) / p[j] * 500.0;
+        const double dpdpg = 1.0;
+        const double dpdt = 4.0 * in->pradk[j] / t[j] * dilut[j];
+        io->dltdlp[j] = ptotal[j] / t[j] / in->grav * dtdrhx[j];
+        if (J == n && J >= 2)
+            if (t[j] > t[j - 1] && ptotal[j] > ptotal[j - 1])
+                io->dltdlp[j] = log(t[j] / t[j - 1]) / log(ptotal[j] / ptotal[j - 1]);
+        const double heatcv = dedt - dedpg * drdt / drdpg;
+        io->heatcp[j] = dedt - dedpg * dpdt / dpdpg -
+                        ptotal[j] / (rho[j] * rho[j]) * (drdt - drdpg * dpdt / dpdpg);
+        if (heatcv <= 0.0) io->velsnd[j] = 0.0;
+        else io->velsnd[j] = sqrt(fmax(io->heatcp[j] / heatcv * dpdpg / drdpg, 0.0));
+        io->dlrdlt[j] = t[j] / rho[j] * (drdt - drdpg * dpdt / dpdpg);
+        io->grdadb[j] = -ptotal[j] / rho[j] / t[j] * io->dlrdlt[j] / io->heatcp[j];
+        io->hscale[j] = ptotal[j] / rho[j] / in->grav;
+        io->vconv[j] = 0.0;
+        io->flxcnv[j] = 0.0;
+        io->abconv[j] = in->abross[j];
+        io->deltat[j] = 0.0;
+        if (in->mixlth == 0.0) continue;
+        if (J < 4) continue;
+        const double del = io->dltdlp[j] - io->grdadb[j];
+        if (del < 0.0) continue;
+        const double vco = 0.5 * in->mixlth * sqrt(fmax(-0.5 * ptotal[j] / rho[j] * io->dlrdlt[j], 0.0));
+        if (vco == 0.0) continue;
+        const double fluxco = 0.5 * rho[j] * io->heatcp[j] * t[j] * in->mixlth / fourpi;
+        const double rosst = or_rosstab_lookup(in->tab, in->irosstab, t[j], p[j]);
+        double olddelt = 0.0;
+        const int its30 = in->ifconv == 0 ? 1 : 30;
+        const double omega = 0.7;
+        double sa = 0.0, sb = 0.0, sc = 0.0, hist[12];
+        int shave = 0, hist_n = 0;
+        for (int id = 1; id <= its30; ++id) {
+            double dt_j = io->deltat[j];
+            const double dplus = or_rosstab_lookup(in->tab, in->irosstab, t[j] + dt_j, p[j]) / rosst;
+            const double dminus = or_rosstab_lookup(in->tab, in->irosstab, t[j] - dt_j, p[j]) / rosst;
+            if (in->use4pt) {
+                const double dpi = or_rosstab_lookup(in->tab, in->irosstab, t[j] + dt_j / 3.0, p[j]) / rosst;
+                const double dmi = or_rosstab_lookup(in->tab, in->irosstab, t[j] - dt_j / 3.0, p[j]) / rosst;
+                io->abconv[j] = 4.0 / (1.0 / dplus + 1.0 / dpi + 1.0 / dminus + 1.0 / dmi) * in->abross[j];
+            } else {
+                io->abconv[j] = 2.0 / (1.0 / dplus + 1.0 / dminus) * in->abross[j];
+            }
+            const double t4 = ((t[j] * t[j]) * t[j]) * t[j];
+            double d = 8.0 * sigma_sb * t4 / (io->abconv[j] * io->hscale[j] * rho[j]) /
+                       (fluxco * fourpi) / vco;
+            const double taub = io->abconv[j] * rho[j] * in->mixlth * io->hscale[j];
+            d = d * (taub * taub) / (2.0 + taub * taub);
+            d = d * d / 2.0;
+            const double q = del / (d + del);
+            const double ddd = q * q;
+            double delta;
+            if (ddd >= 0.5) {
+                delta = (1.0 - sqrt(1.0 - ddd)) / ddd;
+            } else {
+                delta = 0.5;
+                double term = 0.5, up = -1.0, down = 2.0;
+                for (;;) {
+                    up = up + 2.0;
+                    down = down + 2.0;
+                    term = up / down * ddd * term;
+                    delta = delta + term;
+                    if (term <= 1.0e-6) break;
+                }
+            }
+            delta = delta * (del * del) / (d + del);
+            io->vconv[j] = vco * sqrt(delta);
+            io->flxcnv[j] = fluxco * io->vconv[j] * delta;
+            io->flxcnv[j] = fmax(io->flxcnv[j], 0.0);
+            dt_j = t[j] * in->mixlth * delta;
+            dt_j = fmin(dt_j, t[j] * 0.15);
+            dt_j = dt_j * omega + olddelt * (1.0 - omega);
+            io->deltat[j] = dt_j;
+            const double dstep = dt_j - olddelt;
+            if (fabs(dstep) < 0.5 || fabs(dstep) < 1.0e-3 * fmax(fabs(dt_j), 1.0)) break;
+            sa = sb;
+            sb = sc;
+            sc = dt_j;
+            shave = shave + 1 < 3 ? shave + 1 : 3;
+            if (shave == 3 && id % 3 == 0) {
+                const double den = sc - 2.0 * sb + sa;
+                if (fabs(den) > 1.0e-6 * fmax(fabs(sc), 1.0)) {
+                    const double num = sc - sb;
+                    const double x = sc - num * num / den;
+                    if (x > 0.0 && fabs(x - sc) < 0.5 * fmax(fabs(sc), 1.0)) {
+                        io->deltat[j] = x;
+                        olddelt = x;
+                        shave = 0;
+                        continue;
+                    }
+                }
+            }
+            for (int ii = (hist_n < 11 ? hist_n : 11); ii >= 1; --ii) hist[ii] = hist[ii - 1];
+            hist[0] = dt_j;
+            hist_n = hist_n + 1 < 12 ? hist_n + 1 : 12;
+            int cycle_exit = 0;
+            double cmean = 0.0;
+            if (hist_n >= 8) {
+                for (int ii = 2; ii <= 6; ++ii) {
+                    if (hist_n < 2 * ii) continue;
+                    int all = 1;
+                    const double tol = 5.0e-4 * fmax(fabs(hist[0]), 1.0);
+                    for (int k = 0; k < ii; ++k)
+                        if (!(fabs(hist[k] - hist[ii + k]) <= tol)) all = 0;
+                    if (all) {
+                        double cmin = hist[0], cmax = hist[0], s = 0.0;
+                        for (int k = 0; k < ii; ++k) {
+                            cmin = fmin(cmin, hist[k]);
+                            cmax = fmax(cmax, hist[k]);
+                            s = s + hist[k];
+                        }
+                        cmean = s / (double)ii;
+                        if (cmax - cmin > 2.0e-3 * fmax(fabs(cmean), 1.0)) {
+                            cycle_exit = 1;
+                            break;
+                        }
+                    }
+                }
+            }
+            if (cycle_exit) {
+                io->deltat[j] = cmean;
+                const double dp2 = or_rosstab_lookup(in->tab, in->irosstab, t[j] + cmean, p[j]) / rosst;
+                const double dm2 = or_rosstab_lookup(in->tab, in->irosstab, t[j] - cmean, p[j]) / rosst;
+                if (in->use4pt) {
+                    const double dpi = or_rosstab_lookup(in->tab, in->irosstab, t[j] + cmean / 3.0, p[j]) / rosst;
+                    const double dmi = or_rosstab_lookup(in->tab, in->irosstab, t[j] - cmean / 3.0, p[j]) / rosst;
+                    io->abconv[j] = 4.0 / (1.0 / dp2 + 1.0 / dpi + 1.0 / dm2 + 1.0 / dmi) * in->abross[j];
+                } else {
+                    io->abconv[j] = 2.0 / (1.0 / dp2 + 1.0 / dm2) * in->abross[j];
+                }
+                break;
+            }
+            olddelt = dt_j;
+        }
+    }
+    if (mlt_only) return;
+    double *f = io->flxcnv;
+    int jtop = 0, jbot = 0;                                    /* gap fill :10813-10841 */
+    for (int J = 1; J <= n; ++J)
+        if (f[J - 1] > 0.0) {
+            if (jtop == 0) jtop = J;
+            jbot = J;
+        }
+    if (jtop > 0 && jbot > jtop + 1)
+        for (int J = jtop + 1; J <= jbot - 1; ++J)
+            if (f[J - 1] == 0.0) {
+                int ja = J - 1, jb = J + 1;
+                while (ja > jtop && f[ja - 1] == 0.0) ja = ja - 1;
+                while (jb < jbot && f[jb - 1] == 0.0) jb = jb + 1;
+                if (f[ja - 1] > 0.0 && f[jb - 1] > 0.0) {
+                    const double w = (double)(J - ja) / (double)(jb - ja);
+                    f[J - 1] = f[ja - 1] * (1.0 - w) + f[jb - 1] * w;
+                }
+            }
+    if (in->ifcorr == 1) {                                     /* bottom clamp :10867-10878 */
+        jbot = 0;
+        for (int J = 1; J <= n; ++J)
+            if (f[J - 1] > 0.0) jbot = J;
+        if (jbot >= n - 3 && jbot < n)
+            for (int J = jbot + 1; J <= n; ++J) {
+                if (io->dltdlp[J - 1] - io->grdadb[J - 1] > -0.01)
+                    f[J - 1] = fmax(in->flux - in->flxrad[J - 1], 0.0);
+                else
+                    break;
+            }
+    }
+    for (int j = 0; j < n; ++j) io->flxcnv0[j] = f[j];        /* smoothing :10885-10893 */
+    if (in->smooth)
+        for (int J = 2; J <= n - 1; ++J)
+            f[J - 1] = 0.25 * io->flxcnv0[J - 2] + 0.50 * io->flxcnv0[J - 1] + 0.25 * io->flxcnv0[J];
+    for (int j = 0; j < n; ++j) io->flxcnv0[j] = f[j];
+    if (in->overwt > 0.0) {                                    /* overshooting :10901-10927 */
+        double wtcnv = 0.0;
+        for (int j = 0; j < n; ++j) wtcnv = fmax(wtcnv, f[j] / in->flux);
+        wtcnv = fmin(wtcnv, 1.0) * in->overwt;
+        for (int j = 0; j < n; ++j)
+            delhgt[j] = fmin(fmin(io->hscale[j] * 0.5e-5 * wtcnv, in->height[n - 1] - in->height[j]),
+                             in->height[j] - in->height[0]);
+        for (int j = 0; j < n; ++j) {
+            io->flxcnv0[j] = f[j];
+            io->flxcnv1[j] = 0.0;
+        }
+        or_integ(in->height, f, cnvint, n, 0.0);
+        for (int J = n / 2; J <= n - 1; ++J) {
+            if (delhgt[J - 1] == 0.0) continue;
+            double x1 = in->height[J - 1] - delhgt[J - 1], x2 = in->height[J - 1] + delhgt[J - 1];
+            double c1, c2;
+            (void)or_map1(in->height, cnvint, n, &x1, &c1, 1);
+            (void)or_map1(in->height, cnvint, n, &x2, &c2, 1);
+            io->flxcnv1[J - 1] = io->flxcnv1[J - 1] + (c2 - c1) / delhgt[J - 1] / 2.0;
+        }
+        for (int j = 0; j < n; ++j) f[j] = fmax(io->flxcnv0[j], io->flxcnv1[j]);
+    }
+    for (int J = 1; J <= in->nconv && J <= n; ++J) f[J - 1] = 0.0;   /* :10930-10932 */
+}

@@ -616,6 +616,60 @@ def tcorr3_finish(tab, sc, arr, t1, state, mode=1):
     return out, t1o, drhox, nf
 
 
+CNV_IN = ("ptotal", "rho", "pradk", "tauros", "rhox", "abross", "vturb", "height", "flxrad")
+CNV_STATE = ("t", "tk", "hkt", "hckt", "tkev", "tlog", "p")
+CNV_OUT = ("dltdlp", "heatcp", "velsnd", "dlrdlt", "grdadb", "hscale", "vconv", "flxcnv",
+           "flxcnv0", "flxcnv1", "abconv", "deltat")
+
+
+class OrConvecIn(C.Structure):
+    _fields_ = [(k, C.c_int) for k in ("n", "ifconv", "ifcorr", "nconv", "use4pt", "smooth",
+                                       "irosstab")] + \
+               [(k, C.c_double) for k in ("grav", "mixlth", "flux", "overwt")] + \
+               [(k, P_D) for k in CNV_IN] + [("edens", P_D * 4), ("rho4", P_D * 4),
+                                             ("tab", C.c_void_p)]
+
+
+class OrConvecIo(C.Structure):
+    _fields_ = [(k, P_D) for k in CNV_STATE + CNV_OUT]
+
+
+def convec(tab, sc, arr, edens, rho4, mode=1, mlt_only=False):
+    """CONVEC on the EOS evaluations edens, rho4 (4, n).  sc: n, ifconv,
+    ifcorr, nconv, use4pt, smooth, grav, mixlth, flux, overwt; arr: CNV_IN,
+    CNV_STATE and flxcnv1 (n).  Returns a dict of CNV_STATE (after the
+    perturb-and-restore) and CNV_OUT."""
+    n = len(arr["t"])
+    s = OrConvecIn()
+    s.n = n
+    for k in ("ifconv", "ifcorr", "nconv", "use4pt", "smooth"):
+        setattr(s, k, int(sc[k]))
+    s.irosstab = int(mode)
+    for k in ("grav", "mixlth", "flux", "overwt"):
+        setattr(s, k, float(sc[k]))
+    keep = {k: _f64(arr[k]) for k in CNV_IN}
+    for k in CNV_IN:
+        setattr(s, k, _p(keep[k]))
+    e = _f64(np.asarray(edens, np.float64).reshape(4, n))
+    r = _f64(np.asarray(rho4, np.float64).reshape(4, n))
+    for i in range(4):
+        s.edens[i] = _p(e[i])
+        s.rho4[i] = _p(r[i])
+    s.tab = C.addressof(tab.s)
+    out = {k: _f64(np.array(arr[k], np.float64).copy()) for k in CNV_STATE}
+    out.update({k: np.zeros(n) for k in CNV_OUT})
+    out["flxcnv1"] = _f64(np.array(arr["flxcnv1"], np.float64).copy())
+    io = OrConvecIo()
+    for k in CNV_STATE + CNV_OUT:
+        setattr(io, k, _p(out[k]))
+    L = lib()
+    L.or_convec.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+    L.or_convec(C.byref(s), C.byref(io), int(bool(mlt_only)))
+    out["_keep"] = (keep, e, r)
+    del out["_keep"]
+    return out
+
+
 class OrMoltab(C.Structure):
     _fields_ = [("nequa", C.c_int), ("nummol", C.c_int), ("locj", C.POINTER(C.c_int)),
                 ("kcomps", C.POINTER(C.c_int)), ("idequa", C.POINTER(C.c_int))]

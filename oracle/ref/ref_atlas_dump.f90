@@ -24,6 +24,9 @@ PROGRAM ref_atlas_dump
   INTEGER :: uf, idum_l, k, irt, nq, iq
   REAL(8) :: rx_l, taustd_l(kw), tn_l(kw), pr_l(kw), ab_l(kw), pt_l(kw), pg_l(kw), t1_l(kw)
   REAL(8) :: t0_l(kw), p0_l(kw), qt_l(kw * 12), qp_l(kw * 12), qr_l(kw * 12)
+  ! CONVEC probe: the EOS evaluations it makes (EDENS, RHO at four states)
+  INTEGER :: uc
+  REAL(8) :: cv_e(kw, 4), cv_r(kw, 4)
 
   CALL GET_COMMAND_ARGUMENT(1, deck)
   CALL GET_COMMAND_ARGUMENT(2, arg); READ(arg, *) nustart_in
@@ -176,7 +179,82 @@ PROGRAM ref_atlas_dump
     rx_l = ROSSTAB(0.0D0, 0.0D0, 0.0D0)
     CALL RADIAP(3, 0.0D0)
     CALL COMPUTE_HEIGHT
-    IF (IFPRES .EQ. 1 .AND. IFCONV .EQ. 1) CALL CONVEC(.FALSE.)
+    IF (IFPRES .EQ. 1 .AND. IFCONV .EQ. 1) THEN
+      ! <A12_TCORR_OUT>.cnv: CONVEC's inputs, the four EOS evaluations it
+      ! makes (EDENS and RHO at T(1.001), T(0.999), P(1.001), P(0.999),
+      ! src/atlas12_modules.f90:10457-10535, taken here with the same
+      ! sequence of perturbations from a snapshot that NEWTON_SAVE_STATE /
+      ! NEWTON_RESTORE_STATE put back exactly), then what CONVEC leaves.
+      CALL NEWTON_SAVE_STATE
+      IFEDNS = 1
+      TLOG = TLOG + 0.0009995003D0
+      T    = T * 1.001D0
+      TK   = TK * 1.001D0
+      HKT  = HKT / 1.001D0
+      HCKT = HCKT / 1.001D0
+      TKEV = TKEV * 1.001D0
+      ITEMP = ITEMP + 1
+      CALL COMPUTE_ONE_POP(0.0D0, 1, XNE)
+      cv_e(1:NRHOX, 1) = EDENS(1:NRHOX)
+      cv_r(1:NRHOX, 1) = RHO(1:NRHOX)
+      TLOG = TLOG - 0.0009995003D0 - 0.0010005003D0
+      T    = T / 1.001D0 * 0.999D0
+      TK   = TK / 1.001D0 * 0.999D0
+      HKT  = HKT * 1.001D0 / 0.999D0
+      HCKT = HCKT * 1.001D0 / 0.999D0
+      TKEV = TKEV / 1.001D0 * 0.999D0
+      ITEMP = ITEMP + 1
+      CALL COMPUTE_ONE_POP(0.0D0, 1, XNE)
+      cv_e(1:NRHOX, 2) = EDENS(1:NRHOX)
+      cv_r(1:NRHOX, 2) = RHO(1:NRHOX)
+      TLOG = TLOG + 0.0010005003D0
+      T    = T / 0.999D0
+      TK   = TK / 0.999D0
+      HKT  = HKT * 0.999D0
+      HCKT = HCKT * 0.999D0
+      TKEV = TKEV / 0.999D0
+      P    = P * 1.001D0
+      ITEMP = ITEMP + 1
+      CALL COMPUTE_ONE_POP(0.0D0, 1, XNE)
+      cv_e(1:NRHOX, 3) = EDENS(1:NRHOX)
+      cv_r(1:NRHOX, 3) = RHO(1:NRHOX)
+      P = P / 1.001D0 * 0.999D0
+      ITEMP = ITEMP + 1
+      CALL COMPUTE_ONE_POP(0.0D0, 1, XNE)
+      cv_e(1:NRHOX, 4) = EDENS(1:NRHOX)
+      cv_r(1:NRHOX, 4) = RHO(1:NRHOX)
+      CALL NEWTON_RESTORE_STATE
+      OPEN(NEWUNIT=uc, FILE=TRIM(envval)//'.cnv', ACCESS='STREAM', FORM='UNFORMATTED', &
+           STATUS='REPLACE')
+      WRITE(uc) NRHOX, IFCORR, NCONV, MERGE(1, 0, USE_4POINT_MLT), MERGE(1, 0, USE_FLXCNV_SMOOTH)
+      WRITE(uc) GRAV, MIXLTH, FLUX, OVERWT
+      WRITE(uc) T(1:NRHOX), TK(1:NRHOX), HKT(1:NRHOX), HCKT(1:NRHOX), TKEV(1:NRHOX), &
+                TLOG(1:NRHOX), P(1:NRHOX), PTOTAL(1:NRHOX), RHO(1:NRHOX), PRADK(1:NRHOX), &
+                TAUROS(1:NRHOX), RHOX(1:NRHOX), ABROSS(1:NRHOX), VTURB(1:NRHOX), &
+                HEIGHT(1:NRHOX), FLXRAD(1:NRHOX), FLXCNV1(1:NRHOX)
+      WRITE(uc) cv_e(1:NRHOX, 1:4), cv_r(1:NRHOX, 1:4)
+      WRITE(uc) ROSSTAB_NROSS
+      WRITE(uc) ROSSTAB_ZEROT, ROSSTAB_SLOPET, ROSSTAB_ZEROP, ROSSTAB_SLOPEP
+      WRITE(uc) ROSSTAB_TABT(1:ROSSTAB_NROSS), ROSSTAB_TABP(1:ROSSTAB_NROSS), &
+                ROSSTAB_ROSS(1:ROSSTAB_NROSS)
+      CALL CONVEC(.FALSE.)
+      WRITE(uc) T(1:NRHOX), TK(1:NRHOX), HKT(1:NRHOX), HCKT(1:NRHOX), TKEV(1:NRHOX), &
+                TLOG(1:NRHOX), P(1:NRHOX), ITEMP
+      WRITE(uc) DLTDLP(1:NRHOX), HEATCP(1:NRHOX), VELSND(1:NRHOX), DLRDLT(1:NRHOX), &
+                GRDADB(1:NRHOX), HSCALE(1:NRHOX), VCONV(1:NRHOX), FLXCNV(1:NRHOX), &
+                FLXCNV0(1:NRHOX), FLXCNV1(1:NRHOX)
+      ! Overshooting (OVERWT > 0; the reference's default is 0): CONVEC again
+      ! from the same snapshot with OVERWT = 0.5, then once more with the
+      ! default so that TCORR(3) below reads what the first call left.
+      CALL NEWTON_RESTORE_STATE
+      OVERWT = 0.5D0
+      CALL CONVEC(.FALSE.)
+      WRITE(uc) FLXCNV(1:NRHOX), FLXCNV0(1:NRHOX), FLXCNV1(1:NRHOX)
+      CALL NEWTON_RESTORE_STATE
+      OVERWT = 0.0D0
+      CALL CONVEC(.FALSE.)
+      CLOSE(uc)
+    END IF
     OPEN(NEWUNIT=u, FILE=TRIM(envval), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
     WRITE(u) NRHOX, IFCONV, ITER
     WRITE(u) FLUX, TEFF, MIXLTH

@@ -501,6 +501,8 @@ def atlas_case(atm, nustart, nuhi, linedata, raw=None, tcorr=False, kapp_waves=N
         d["tcorr_in"] = read_tcorr_in(tco)
         d["tcorr_iter"] = read_iter_table(tco + ".iter", d["tcorr_in"]["n"])
         d["tcorr_fin"] = read_tcorr_fin(tco + ".fin", d["tcorr_in"]["n"])
+        if os.path.exists(tco + ".cnv"):
+            d["convec"] = read_convec(tco + ".cnv")
     if raw is not None:
         with open(out + ".sel", "rb") as fh:
             buf = fh.read()
@@ -618,6 +620,39 @@ def read_tcorr_fin(path, n):
     return d
 
 
+CNV_IN = ("t", "tk", "hkt", "hckt", "tkev", "tlog", "p", "ptotal", "rho", "pradk", "tauros",
+          "rhox", "abross", "vturb", "height", "flxrad", "flxcnv1")
+CNV_STATE = ("t", "tk", "hkt", "hckt", "tkev", "tlog", "p")
+CNV_OUT = ("dltdlp", "heatcp", "velsnd", "dlrdlt", "grdadb", "hscale", "vconv", "flxcnv",
+           "flxcnv0", "flxcnv1")
+
+
+def read_convec(path):
+    """ref_atlas_dump's <A12_TCORR_OUT>.cnv: CONVEC's inputs, the four EOS
+    evaluations (EDENS, RHO) it makes, the ROSSTAB table, and what CONVEC
+    leaves in the module arrays."""
+    r = _Stream(path)
+    n, ifcorr, nconv, use4pt, smooth = (int(x) for x in r.i4(5))
+    d = dict(n=n, ifcorr=ifcorr, nconv=nconv, use4pt=use4pt, smooth=smooth)
+    d["grav"], d["mixlth"], d["flux"], d["overwt"] = r.f8(4)
+    for k in CNV_IN:
+        d["in_" + k] = r.f8(n)
+    d["edens"] = r.f8(4 * n).reshape(4, n)
+    d["rho4"] = r.f8(4 * n).reshape(4, n)
+    nross = int(r.i4(1)[0])
+    d["rt_zero"] = r.f8(4)
+    d["rt_tab"] = r.f8(3 * nross).reshape(3, nross)
+    for k in CNV_STATE:
+        d["st_" + k] = r.f8(n)
+    d["st_itemp"] = int(r.i4(1)[0])
+    for k in CNV_OUT:
+        d["out_" + k] = r.f8(n)
+    for k in ("flxcnv", "flxcnv0", "flxcnv1"):  # the same call with OVERWT = 0.5
+        d["ovr_" + k] = r.f8(n)
+    assert r.o == len(r.b), (r.o, len(r.b))
+    return d
+
+
 def golden_tcorr():
     """TCORR(3) after a full 30,000-frequency pass (100,000 raw lines through
     SELECTLINES) on the plain-SYNTHE atmosphere: the state it reads, RDABH
@@ -640,6 +675,7 @@ def golden_tcorr():
     keep = {k: ti[k] for k in TCORR_FIELDS}
     keep.update({"iter_" + k: v for k, v in tab.items()})
     keep.update({"fin_" + k: v for k, v in d["tcorr_fin"].items()})
+    keep.update({"cnv_" + k: v for k, v in d["convec"].items()})
     np.savez_compressed(os.path.join(HERE, "tcorr_ref.npz"), n=ti["n"], ifconv=ti["ifconv"],
                         iter=ti["iter"], flux=ti["flux"], teff=ti["teff"], mixlth=ti["mixlth"],
                         rdabh=acc["rdabh"], rdiagj=acc["rdiagj"], **keep)
@@ -670,6 +706,7 @@ def golden_tcorr_conv():
     keep = {k: ti[k] for k in TCORR_FIELDS}
     keep.update({"iter_" + k: v for k, v in tab.items()})
     keep.update({"fin_" + k: v for k, v in d["tcorr_fin"].items()})
+    keep.update({"cnv_" + k: v for k, v in d["convec"].items()})
     np.savez_compressed(os.path.join(HERE, "tcorr_conv_ref.npz"), n=ti["n"],
                         ifconv=ti["ifconv"], iter=ti["iter"], flux=ti["flux"], teff=ti["teff"],
                         mixlth=ti["mixlth"], rdabh=acc["rdabh"], rdiagj=acc["rdiagj"], **keep)

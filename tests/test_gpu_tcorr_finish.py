@@ -196,3 +196,64 @@ def test_tcorr3_finish_rejects_bad_input(gpu_ctx, deck):
     with pytest.raises(ValueError):
         native.atlas_tcorr3_finish(gpu_ctx, sc, dict(arr, p=arr["p"][:-1]), out["t1"],
                                    native.czc_state())
+
+
+# CONVEC's bubble-opacity iteration stops once a step is below 0.5 K (or
+# 1e-3 relative) and takes Steffensen and cycle-mean exits on tests of its
+# iterates: one-ulp changes of its inputs (the EOS energies, the ROSSTAB
+# entries) move the oracle's converged DELTAT by up to 1.8 % and FLXCNV by up
+# to 1.2 % of its largest value on the convective deck (measured).  The
+# device's log10 and pow differ from libm in the last bit, so the iterated
+# quantities are held to that band (5 %); the thermodynamic derivatives,
+# which nothing iterates, to 1e-11.
+CNV_DIRECT = ("dltdlp", "heatcp", "velsnd", "dlrdlt", "grdadb", "hscale")
+CNV_ITER = ("vconv", "flxcnv", "flxcnv0", "flxcnv1")
+
+
+def _convec_inputs(z, overwt=None):
+    g = lambda k: z["cnv_" + k]  # noqa: E731
+    sc = dict(ifconv=1, ifcorr=int(g("ifcorr")), nconv=int(g("nconv")), use4pt=int(g("use4pt")),
+              smooth=int(g("smooth")), grav=float(g("grav")), mixlth=float(g("mixlth")),
+              flux=float(g("flux")), overwt=float(g("overwt") if overwt is None else overwt))
+    arr = {k: g("in_" + k) for k in ("t", "tk", "hkt", "hckt", "tkev", "tlog", "p", "ptotal",
+                                     "rho", "pradk", "tauros", "rhox", "abross", "vturb",
+                                     "height", "flxrad", "flxcnv1")}
+    return sc, arr, g("edens"), g("rho4")
+
+
+@pytest.mark.parametrize("overwt", [None, 0.5])
+def test_convec_on_device_matches_reference(gpu_ctx, deck, overwt):
+    from atlas12_amd import native
+    sc, arr, e, r = _convec_inputs(deck, overwt)
+    native.rosstab_load(gpu_ctx, deck["cnv_rt_zero"], deck["cnv_rt_tab"])
+    out = native.convec(gpu_ctx, sc, arr, e, r)
+    for k in ("t", "tk", "hkt", "hckt", "tkev", "tlog", "p"):
+        assert np.array_equal(out[k], deck["cnv_st_" + k]), k
+    for k in CNV_DIRECT:
+        assert _rel(out[k], deck["cnv_out_" + k]) <= 1e-11, k
+    pre = "cnv_ovr_" if overwt else "cnv_out_"
+    for k in CNV_ITER:
+        want = deck[pre + k] if pre + k in deck.files else deck["cnv_out_" + k]
+        scale = max(np.max(np.abs(want)), 1e-300)
+        assert np.max(np.abs(out[k] - want)) <= 0.05 * scale, k
+    # the same layers convect (overshooting adds flux near zero at the
+    # zone's edges, where the sign of a tiny value is not held)
+    assert np.array_equal(out["vconv"] > 0.0, deck["cnv_out_vconv"] > 0.0)
+    if not overwt:
+        assert np.array_equal(out["flxcnv"] > 0.0, deck["cnv_out_flxcnv"] > 0.0)
+
+
+def test_convec_mlt_only_leaves_history(gpu_ctx, deck):
+    """mlt_only (the Newton probe's call): the MLT solve without the
+    post-processing -- no smoothing, FLXCNV0 and FLXCNV1 untouched."""
+    from atlas12_amd import native
+    sc, arr, e, r = _convec_inputs(deck)
+    native.rosstab_load(gpu_ctx, deck["cnv_rt_zero"], deck["cnv_rt_tab"])
+    arr = dict(arr, flxcnv1=np.full(len(arr["t"]), 7.0))
+    full = native.convec(gpu_ctx, sc, arr, e, r)
+    raw = native.convec(gpu_ctx, sc, arr, e, r, mlt_only=True)
+    assert np.all(raw["flxcnv1"] == 7.0)
+    assert np.all(raw["flxcnv0"] == 0.0)
+    for k in CNV_DIRECT:
+        assert np.array_equal(raw[k], full[k]), k
+    assert np.all(full["flxcnv"][:int(sc["nconv"])] == 0.0)

@@ -204,3 +204,67 @@ def test_ttaup_constant_opacity_closed_form(orc):
     assert np.allclose(ab, 0.37, rtol=1e-13, atol=0)
     assert np.max(np.abs(np.log(pt / (2.75e4 * tau / 0.37)))) < 2e-4
     assert np.array_equal(pg, pt)
+
+
+CNV_IN = ("t", "tk", "hkt", "hckt", "tkev", "tlog", "p", "ptotal", "rho", "pradk", "tauros",
+          "rhox", "abross", "vturb", "height", "flxrad", "flxcnv1")
+CNV_OUT = ("dltdlp", "heatcp", "velsnd", "dlrdlt", "grdadb", "hscale", "vconv", "flxcnv",
+           "flxcnv0", "flxcnv1")
+
+
+def convec_inputs(z, overwt=None):
+    g = lambda k: z["cnv_" + k]  # noqa: E731
+    sc = dict(ifconv=1, ifcorr=int(g("ifcorr")), nconv=int(g("nconv")), use4pt=int(g("use4pt")),
+              smooth=int(g("smooth")), grav=float(g("grav")), mixlth=float(g("mixlth")),
+              flux=float(g("flux")), overwt=float(g("overwt") if overwt is None else overwt))
+    arr = {k: g("in_" + k) for k in CNV_IN}
+    return sc, arr, g("edens"), g("rho4")
+
+
+def test_convec_matches_reference(orc, deck):
+    """CONVEC on the EOS evaluations it made (recorded by the harness from
+    the same snapshot): the perturb-and-restore drift of T, TK, HKT, HCKT,
+    TKEV, TLOG, P, the thermodynamic derivatives, and the MLT solve with its
+    post-processing, bit for bit (26 MLT layers, gap fill, smoothing and
+    surface zeroing on the convective deck)."""
+    sc, arr, e, r = convec_inputs(deck)
+    tab = orc.Rosstab(deck["cnv_rt_zero"], deck["cnv_rt_tab"])
+    out = orc.convec(tab, sc, arr, e, r)
+    for k in ("t", "tk", "hkt", "hckt", "tkev", "tlog", "p"):
+        assert np.array_equal(out[k], deck["cnv_st_" + k]), k
+    for k in CNV_OUT:
+        assert np.array_equal(out[k], deck["cnv_out_" + k]), k
+
+
+def test_convec_overshooting_matches_reference(orc, deck):
+    """The same call with OVERWT = 0.5 (the reference's default is 0): the
+    overshoot flux FLXCNV1 from INTEG and MAP1 over HEIGHT, FLXCNV =
+    max(FLXCNV0, FLXCNV1), bit for bit."""
+    sc, arr, e, r = convec_inputs(deck, overwt=0.5)
+    tab = orc.Rosstab(deck["cnv_rt_zero"], deck["cnv_rt_tab"])
+    out = orc.convec(tab, sc, arr, e, r)
+    for k in ("flxcnv", "flxcnv0", "flxcnv1"):
+        assert np.array_equal(out[k], deck["cnv_ovr_" + k]), k
+
+
+def test_convec_thermodynamics_closed_form(orc, deck):
+    """For an ideal monatomic gas without radiation pressure (E = 3/2 P/rho,
+    rho = P mu / (k T)) the finite differences give the textbook values:
+    d ln rho / d ln T = -1 at constant P, HEATCP = 5/2 P/(rho T), so
+    nabla_ad = -P/(rho T) DLRDLT / HEATCP = 2/5, to the differences'
+    O(1e-6); with MIXLTH = 0 no depth convects."""
+    sc, arr, _, _ = convec_inputs(deck)
+    n = len(arr["t"])
+    t0, p0 = arr["t"], arr["p"]
+    mu_k = 1.3e-8  # mu / k in g/(erg/K) units: any positive constant
+    rho_at = lambda t, p: p * mu_k / t  # noqa: E731
+    # the four states of CONVEC's perturbations, from the T and P it starts at
+    tp, tm, pp, pm = t0 * 1.001, t0 * 0.999, p0 * 1.001, p0 * 0.999
+    r4 = np.array([rho_at(tp, p0), rho_at(tm, p0), rho_at(t0, pp), rho_at(t0, pm)])
+    e4 = np.array([1.5 * p0 / r4[0], 1.5 * p0 / r4[1], 1.5 * pp / r4[2], 1.5 * pm / r4[3]])
+    a = dict(arr, pradk=np.zeros(n), rho=rho_at(t0, p0), ptotal=p0.copy())
+    tab = orc.Rosstab(deck["cnv_rt_zero"], deck["cnv_rt_tab"])
+    out = orc.convec(tab, dict(sc, mixlth=0.0), a, e4, r4)
+    assert np.allclose(out["grdadb"], 0.4, rtol=0, atol=1e-5)
+    assert np.allclose(out["dlrdlt"], -1.0, rtol=0, atol=1e-5)
+    assert np.all(out["flxcnv"] == 0.0)
