@@ -68,6 +68,51 @@ MODULE a12gpu_iface
     TYPE(C_PTR)        :: flxrad, rjmins, rdiagj
   END TYPE a12_tcorr_in
 
+  ! CZ_CONSTRUCT's SAVE state and the run flags CZC_RUN_ENGAGED / _JC
+  ! (src/atlas12_modules.f90:3054-3069); start: all 0, last_dtmax 1D30
+  TYPE, BIND(C) :: a12_czc_state
+    INTEGER(C_INT32_T) :: prev_engaged, jc_sticky, nbot_bad, run_engaged, run_jc
+    REAL(C_DOUBLE)     :: last_dtmax, rel_err
+  END TYPE a12_czc_state
+
+  ! What CZ_CONSTRUCT and CZC_DELREQ_LAYER read (:3012-3358)
+  TYPE, BIND(C) :: a12_czc_in
+    INTEGER(C_INT32_T) :: nrhox, ifconv, iter, use_constructor, use4pt, irosstab
+    REAL(C_DOUBLE)     :: flux, mixlth, grav
+    TYPE(C_PTR)        :: flxcnv, flxrad, dltdlp, grdadb, flxerr, t, rhox, ptotal, rho
+    TYPE(C_PTR)        :: heatcp, dlrdlt, p, vturb, abross, hscale
+  END TYPE a12_czc_in
+
+  ! TCORR(3) sections (G2)-(K) (:2816-2967): what they read, and the
+  ! module arrays they update in place (bhyd: BHYD(kw, 6) with kw = nrhox)
+  TYPE, BIND(C) :: a12_tcorr_fin_in
+    INTEGER(C_INT32_T) :: nrhox, ifconv, iter, use_constructor, use4pt, irosstab
+    INTEGER(C_INT32_T) :: j1smooth, j2smooth
+    REAL(C_DOUBLE)     :: flux, mixlth, grav, tau1lg, steplg, pradk0, tfloor, wtjm1, wtj, wtjp1
+    TYPE(C_PTR)        :: flxcnv, flxrad, dltdlp, grdadb, flxerr, ptotal, rho, heatcp
+    TYPE(C_PTR)        :: dlrdlt, hscale
+  END TYPE a12_tcorr_fin_in
+  TYPE, BIND(C) :: a12_tcorr_fin_io
+    TYPE(C_PTR)        :: tauros, rhox, t, p, xne, abross, prad, vturb, bmin, pturb
+    TYPE(C_PTR)        :: accrad, bhyd
+    TYPE(C_PTR)        :: tk, hkt, hckt, tkev, tlog, pradk
+  END TYPE a12_tcorr_fin_io
+
+  ! CONVEC (:10323-10953) on the four EOS evaluations it makes: edens,
+  ! rho4 are (nrhox, 4) -- EDENS and RHO after COMPUTE_ONE_POP at
+  ! T*1.001, T*0.999, P*1.001, P*0.999
+  TYPE, BIND(C) :: a12_convec_in
+    INTEGER(C_INT32_T) :: nrhox, ifconv, ifcorr, nconv, use4pt, smooth, irosstab
+    REAL(C_DOUBLE)     :: grav, mixlth, flux, overwt
+    TYPE(C_PTR)        :: ptotal, rho, pradk, tauros, rhox, abross, vturb, height, flxrad
+    TYPE(C_PTR)        :: edens, rho4
+  END TYPE a12_convec_in
+  TYPE, BIND(C) :: a12_convec_io
+    TYPE(C_PTR)        :: t, tk, hkt, hckt, tkev, tlog, p
+    TYPE(C_PTR)        :: dltdlp, heatcp, velsnd, dlrdlt, grdadb, hscale, vconv, flxcnv
+    TYPE(C_PTR)        :: flxcnv0, flxcnv1, abconv, deltat
+  END TYPE a12_convec_io
+
   ! One model of the model iteration (a12_model_converge)
   TYPE, BIND(C) :: a12_model_par
     REAL(C_DOUBLE)     :: teff, logg, mh, vturb
@@ -338,6 +383,78 @@ MODULE a12gpu_iface
       REAL(C_DOUBLE), INTENT(INOUT) :: rdabh(*), oldt1(*)
       REAL(C_DOUBLE), INTENT(OUT) :: out(*)
     END FUNCTION
+    ! ---- ROSSTAB, TTAUP, CONVEC, TCORR(3) (G2)-(K) --------------------------
+    INTEGER(C_INT32_T) FUNCTION a12_rosstab_reset(ctx) BIND(C, NAME='a12_rosstab_reset')
+      IMPORT :: C_INT32_T, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_rosstab_store(ctx, nrhox, t, p, abross) &
+        BIND(C, NAME='a12_rosstab_store')
+      IMPORT :: C_INT32_T, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT32_T), VALUE :: nrhox
+      REAL(C_DOUBLE), INTENT(IN) :: t(*), p(*), abross(*)
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_rosstab_load(ctx, nross, zero, tab) &
+        BIND(C, NAME='a12_rosstab_load')
+      IMPORT :: C_INT32_T, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT32_T), VALUE :: nross
+      REAL(C_DOUBLE), INTENT(IN) :: zero(4), tab(*)
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_rosstab_fetch(ctx, nross, zero, tab) &
+        BIND(C, NAME='a12_rosstab_fetch')
+      IMPORT :: C_INT32_T, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT32_T), INTENT(OUT) :: nross
+      REAL(C_DOUBLE), INTENT(OUT) :: zero(4), tab(*)
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_rosstab_lookup(ctx, mode, nq, temp, pressure, kappa) &
+        BIND(C, NAME='a12_rosstab_lookup')
+      IMPORT :: C_INT32_T, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT32_T), VALUE :: mode, nq
+      REAL(C_DOUBLE), INTENT(IN) :: temp(*), pressure(*)
+      REAL(C_DOUBLE), INTENT(OUT) :: kappa(*)
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_ttaup(ctx, irosstab, n, t, tau, prad, pturb, grav, abstd, &
+        ptotal, p, nfail) BIND(C, NAME='a12_ttaup')
+      IMPORT :: C_INT32_T, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT32_T), VALUE :: irosstab, n
+      REAL(C_DOUBLE), INTENT(IN) :: t(*), tau(*), prad(*), pturb(*)
+      REAL(C_DOUBLE), VALUE :: grav
+      REAL(C_DOUBLE), INTENT(OUT) :: abstd(*), ptotal(*), p(*)
+      INTEGER(C_INT32_T), INTENT(OUT) :: nfail
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_cz_construct(ctx, cin, st, t1, engaged) &
+        BIND(C, NAME='a12_cz_construct')
+      IMPORT :: C_INT32_T, C_PTR, C_DOUBLE, a12_czc_in, a12_czc_state
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(a12_czc_in), INTENT(IN) :: cin
+      TYPE(a12_czc_state), INTENT(INOUT) :: st
+      REAL(C_DOUBLE), INTENT(INOUT) :: t1(*)
+      INTEGER(C_INT32_T), INTENT(OUT) :: engaged
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_atlas_tcorr3_finish(ctx, fin, fio, st, t1, drhox, nfail) &
+        BIND(C, NAME='a12_atlas_tcorr3_finish')
+      IMPORT :: C_INT32_T, C_PTR, C_DOUBLE, a12_tcorr_fin_in, a12_tcorr_fin_io, a12_czc_state
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(a12_tcorr_fin_in), INTENT(IN) :: fin
+      TYPE(a12_tcorr_fin_io), INTENT(INOUT) :: fio
+      TYPE(a12_czc_state), INTENT(INOUT) :: st
+      REAL(C_DOUBLE), INTENT(INOUT) :: t1(*)
+      REAL(C_DOUBLE), INTENT(OUT) :: drhox(*)
+      INTEGER(C_INT32_T), INTENT(OUT) :: nfail
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_convec(ctx, cin, cio, mlt_only) BIND(C, NAME='a12_convec')
+      IMPORT :: C_INT32_T, C_PTR, a12_convec_in, a12_convec_io
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(a12_convec_in), INTENT(IN) :: cin
+      TYPE(a12_convec_io), INTENT(INOUT) :: cio
+      INTEGER(C_INT32_T), VALUE :: mlt_only
+    END FUNCTION
+
     INTEGER(C_INT32_T) FUNCTION a12_scale_model(ctx, nmodel, nrhox, kw, tau1lg, steplg, sc, &
         fields, out, so, taustd) BIND(C, NAME='a12_scale_model')
       IMPORT :: C_INT32_T, C_PTR, C_DOUBLE

@@ -134,14 +134,48 @@ def atlas_run(args, ctx, native, z, d, pred, tio):
     # reference's own ROSS(3)/CONVEC outputs); its sums come from the RT pass
     # on the device
     tz = dict(np.load(os.path.join(GOLDEN, "tcorr_ref.npz")))
+    # the end of the iteration on the device: the ROSSTAB store, CONVEC (on
+    # the EOS evaluations it makes for this deck, from the reference's set-up)
+    # and TCORR(3) after its table (CZ_CONSTRUCT, the TTAUP RHOX correction,
+    # the remap onto TAUSTD)
+    cnv_sc = dict(ifconv=1, ifcorr=int(tz["cnv_ifcorr"]), nconv=int(tz["cnv_nconv"]),
+                  use4pt=int(tz["cnv_use4pt"]), smooth=int(tz["cnv_smooth"]),
+                  grav=float(tz["cnv_grav"]), mixlth=float(tz["cnv_mixlth"]),
+                  flux=float(tz["cnv_flux"]), overwt=float(tz["cnv_overwt"]))
+    cnv_arr = {k: tz["cnv_in_" + k] for k in ("t", "tk", "hkt", "hckt", "tkev", "tlog", "p",
+                                              "ptotal", "rho", "pradk", "tauros", "rhox",
+                                              "abross", "vturb", "height", "flxrad", "flxcnv1")}
+    fin_sc = dict(ifconv=int(tz["ifconv"]), iter=int(tz["iter"]), j1smooth=int(tz["fin_j1smooth"]),
+                  j2smooth=int(tz["fin_j2smooth"]), flux=float(tz["flux"]),
+                  mixlth=float(tz["mixlth"]), grav=float(tz["fin_grav"]),
+                  tau1lg=float(tz["fin_tau1lg"]), steplg=float(tz["fin_steplg"]),
+                  pradk0=float(tz["fin_pradk0"]), tfloor=float(tz["fin_tfloor"]),
+                  wtjm1=float(tz["fin_wtjm1"]), wtj=float(tz["fin_wtj"]),
+                  wtjp1=float(tz["fin_wtjp1"]))
+    fin_arr = {k: tz[k] for k in ("flxcnv", "flxrad", "dltdlp", "grdadb", "ptotal", "rho",
+                                  "heatcp", "dlrdlt", "hscale", "tauros", "rhox", "t", "abross")}
+    for k in ("p", "xne", "prad", "pturb", "vturb", "bmin", "accrad"):
+        fin_arr[k] = tz["fin_in_" + k]
+    fin_arr["bhyd"] = tz["fin_in_bhyd"]
+
+    def iteration_end():
+        """ROSS(3)'s table store, CONVEC, TCORR(3) A-F on the device sums,
+        then TCORR(3)'s end (synchronous calls)."""
+        native.rosstab_reset(ctx)
+        native.rosstab_store(ctx, tz["t"], tz["fin_in_p"], tz["abross"])
+        native.convec(ctx, cnv_sc, cnv_arr, tz["cnv_edens"], tz["cnv_rho4"])
+        out, _, _ = native.atlas_tcorr3(ctx, tz, device_sums=True)
+        native.atlas_tcorr3_finish(ctx, fin_sc, dict(fin_arr, flxerr=out["flxerr"]), out["t1"],
+                                   native.czc_state())
 
     import torch
 
     def steps(prec, k):
         """k iterations of the driver's per-iteration call sequence: the
         state, LINOP1, KAPP (its depth state, then every frequency on the
-        device), the RT pass, TCORR(3) on the device sums (synchronous: it
-        returns T1 to the host)."""
+        device), the RT pass, the ROSSTAB store, CONVEC, TCORR(3) on the
+        device sums and its end (synchronous: T1 and the corrected
+        structure return to the host)."""
         st = np.zeros(5)
         torch.cuda.synchronize()
         t = time.perf_counter()
@@ -151,7 +185,7 @@ def atlas_run(args, ctx, native, z, d, pred, tio):
             native.kapp_set_state(ctx, kstate)
             native.atlas_kapp_run(ctx)
             native.atlas_rt_pass(ctx, fetch=False)
-            native.atlas_tcorr3(ctx, tz, device_sums=True)
+            iteration_end()
             st[:4] += native.atlas_stage_ms(ctx)  # waits on this step's events
             st[4] += native.kapp_ms(ctx)
         torch.cuda.synchronize()
@@ -178,6 +212,7 @@ def atlas_run(args, ctx, native, z, d, pred, tio):
     parts["kapp_run"] = host_ms(lambda: native.atlas_kapp_run(ctx))
     native.atlas_rt_pass(ctx, fetch=False)
     parts["tcorr3"] = host_ms(lambda: native.atlas_tcorr3(ctx, tz, device_sums=True))
+    parts["rosstab_convec_tcorr3_finish_with_tcorr3"] = host_ms(iteration_end)
     log("atlas: timed mixed")
     t_mix, st_mix = steps(native.PREC_MIXED, args.steps)
     _, ev_mix = native.atlas_linop1(ctx, native.PREC_MIXED)
@@ -721,7 +756,10 @@ def main():
         "s_per_iter_covers": "per iteration: a12_atlas_set_state (host arrays -> device), "
                              "LINOP1, a12_kapp_set_state (populations -> device) and KAPP at "
                              "all 30,000 frequencies on the device, the RUN_RT_PASS loop, "
-                             "TCORR(3) on the device sums",
+                             "the ROSSTAB store, CONVEC (on the EOS evaluations it makes), "
+                             "TCORR(3) on the device sums and its end (CZ_CONSTRUCT, TTAUP "
+                             "RHOX correction, remap onto TAUSTD); only the EOS (POPS) stays "
+                             "with the driver",
         "host_ms_per_call": a["parts"],
         "kapp": {"ms": a["st_mix"][4], "cells": int(a["nf"] * len(z["t"])),
                  "cells_per_s": a["nf"] * len(z["t"]) / (a["st_mix"][4] * 1e-3),

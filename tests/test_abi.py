@@ -102,11 +102,20 @@ PROGRAM t
   TYPE(a12_mol2_rec) :: m
   TYPE(a12_tcorr_in) :: tc
   TYPE(a12_moltab) :: mt
+  TYPE(a12_czc_state) :: cs
+  TYPE(a12_czc_in) :: ci
+  TYPE(a12_tcorr_fin_in) :: fi
+  TYPE(a12_tcorr_fin_io) :: fo
+  TYPE(a12_convec_in) :: vi
+  TYPE(a12_convec_io) :: vo
   INTEGER(C_INT32_T) :: rc
   PRINT *, C_SIZEOF(l(1)), C_SIZEOF(sa), C_SIZEOF(aa), C_SIZEOF(m), C_SIZEOF(tc), C_SIZEOF(mt)
+  PRINT *, C_SIZEOF(cs), C_SIZEOF(ci), C_SIZEOF(fi), C_SIZEOF(fo), C_SIZEOF(vi), C_SIZEOF(vo)
   rc = a12_synthe_opacity_run(C_NULL_PTR, A12_PREC_MIXED)
   PRINT *, rc
   rc = a12_atlas_linop1_run(C_NULL_PTR, A12_PREC_FP64)
+  PRINT *, rc
+  rc = a12_convec(C_NULL_PTR, vi, vo, 0)
   PRINT *, rc
 END PROGRAM t
 """)
@@ -116,10 +125,13 @@ END PROGRAM t
     subprocess.run([fc, *mod, iface, str(prog), "-o", str(exe), lib._name,
                     "-Wl,-rpath," + os.path.dirname(lib._name)], check=True, cwd=tmp_path)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
-    sizes = [int(x) for x in out[:6]]
+    sizes = [int(x) for x in out[:12]]
     assert sizes == [32, ctypes.sizeof(native.SyntheAtm), ctypes.sizeof(native.AtlasAtm), 32,
-                     ctypes.sizeof(native.TcorrIn), ctypes.sizeof(native.MolTab)]
-    assert [int(x) for x in out[6:8]] == [-1, -1]
+                     ctypes.sizeof(native.TcorrIn), ctypes.sizeof(native.MolTab),
+                     ctypes.sizeof(native.CzcState), ctypes.sizeof(native.CzcIn),
+                     ctypes.sizeof(native.TcorrFinIn), ctypes.sizeof(native.TcorrFinIo),
+                     ctypes.sizeof(native.ConvecIn), ctypes.sizeof(native.ConvecIo)]
+    assert [int(x) for x in out[12:15]] == [-1, -1, -1]
 
 
 def test_atlas_wrappers_reject_wrong_shapes():
