@@ -1231,12 +1231,64 @@ def golden_molec():
     return {"molec": info}
 
 
+def golden_records():
+    """SYNTHE's record preparation for all three binary/ASCII readers on a
+    5 nm window at R = 500,000: a gfall list, a predicted-format list over
+    every old species index (mapped, unmapped and molecular), and a MOL2
+    list over the isotope/molecule codes molec_dispatch knows plus unknown
+    ones, both label flags and classical/tabulated radiative damping.  The
+    reference's run_mklinelist concatenates them (gfall, predict, mol); the
+    fixture keeps the raw lists, the mol_broad.dat rows, and the records."""
+    tmp = tempfile.mkdtemp(prefix="a12r_")
+    grid = wl.Grid.make(500.0, 505.0, 500000.0)
+    g = wl.synthetic_gfall(600, 499.5, 505.5, seed=71)
+    gpath = os.path.join(tmp, "gfall.dat")
+    wl.write_gfall(g, gpath)
+    root = data_root(tmp, gpath)
+    d = os.path.join(root, "data")
+    rng = np.random.default_rng(72)
+    pred = wl.atlas_linedata(6000, 498.0, 507.0, seed=73, species=tuple(range(1, 1006)))
+    with open(os.path.join(d, "pred.bin"), "wb") as fh:
+        fh.write(np.ascontiguousarray(pred, "<i4").tobytes())
+    isos = np.array([1, 2, 12, 13, 14, 15, 16, 17, 18, 23, 24, 25, 26, 28, 29, 30, 33, 39, 40,
+                     41, 42, 43, 44, 46, 47, 48, 49, 50, 51, 52, 53, 54, 56, 57, 58, 3, 60, 99])
+    codes = np.array([606, 608, 106, 607, 813, 814, 812, 820, 120, 124, 156, 822, 101, 108, 126])
+    n = 5000
+    mol = np.zeros(n, wl.MOL2_DTYPE)
+    mol["wl"] = np.sort(np.exp(rng.uniform(np.log(498.0), np.log(507.0), n))).astype(np.float32)
+    mol["loggf"] = np.round(rng.normal(-2.0, 1.5, n), 3)
+    e = np.round(rng.exponential(8000.0, n), 3)
+    mol["e1"] = np.where(rng.random(n) < 0.5, e, e + 2.0e4)
+    mol["e2"] = np.where(mol["e1"] == e, e + 2.0e4, e)
+    mol["code"] = codes[rng.integers(0, len(codes), n)]
+    mol["iso"] = isos[rng.integers(0, len(isos), n)]
+    mol["gr100"] = np.where(rng.random(n) < 0.3, 0, rng.integers(700, 900, n))
+    mol["flag"] = (rng.random(n) < 0.4).astype(np.int32)
+    wl.write_mol2(mol, os.path.join(d, "molx.bin"))
+    with open(os.path.join(d, "lines.list"), "a") as fh:
+        fh.write("predict pred.bin\nmol molx.asc\n")
+    deck = os.path.join(tmp, "rec.dat")
+    with open(deck, "w") as fh:
+        fh.write(wl.scurve_atmosphere().deck())
+    dump = os.path.join(tmp, "dump.bin")
+    run([os.path.join(REFBIN, "ref_synthe_dump"), deck, "500.0", "505.0", "500000.0", dump],
+        tmp, dict(os.environ, ATLAS12=root))
+    lines = read_dump(dump)["lines"]
+    broad = [ln.rstrip("\n") for ln in open(os.path.join(REF, "data", "mol_broad.dat"))]
+    shutil.rmtree(tmp)
+    np.savez_compressed(os.path.join(HERE, "records_ref.npz"), gfall=g, pred=pred,
+                        mol2=mol.view(np.uint8), lines=lines.view(np.uint8),
+                        mol_broad=np.array(broad), grid=np.array([500.0, 505.0, 500000.0]))
+    return {"records": {"gfall_raw": len(g), "pred_raw": len(pred), "mol2_raw": n,
+                        "lte_records": int(len(lines))}}
+
+
 STEPS = {"synthe_small": golden_synthe_small, "atlas_small": golden_atlas_small,
          "state_atlas": golden_state_atlas, "select": golden_select, "scale": golden_scale,
          "tcorr": golden_tcorr, "tcorr_conv": golden_tcorr_conv,
          "state_plain": golden_state_plain,
          "numerics": golden_numerics, "josh": golden_josh, "molec": golden_molec,
-         "kapp": golden_kapp, "cplx": golden_cplx}
+         "kapp": golden_kapp, "cplx": golden_cplx, "records": golden_records}
 
 
 def main():
