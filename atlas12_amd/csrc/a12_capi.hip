@@ -66,6 +66,7 @@ int a12_ctx_destroy(a12_ctx *ctx)
     ctx->ross.tab.release();
     ctx->ross.zp.release();
     ctx->ross.io.release();
+    if (ctx->ross.pin) (void)hipHostFree(ctx->ross.pin);
     for (DevBuf *b : abufs) b->release();
     for (DevBuf *b : bufs) b->release();
     a12i_mlp_free(ctx);

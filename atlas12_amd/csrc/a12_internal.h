@@ -106,6 +106,8 @@ struct ModelDev {
 struct RossDev {
     DevBuf tab, zp, io;
     int nross = 0;
+    double *pin = nullptr;  // pinned host staging of those calls (grow-only)
+    size_t pin_bytes = 0;
 };
 
 struct MlpDev;    // a12_mlp.hip

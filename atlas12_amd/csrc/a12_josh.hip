@@ -1025,6 +1025,8 @@ struct RossTab {
     int nross, maxtab;
 };
 
+extern "C" __device__ double __ockl_wfred_min_f64(double);
+
 // MODE 2, LOOKUP.  Mode 1: the nearest entry of each (T, P) quadrant is
 // the minimum of (radius^2, index) over the table -- the serial scan's
 // strict "<" keeps the first of equal radii, so ties go to the lower index
@@ -1053,16 +1055,16 @@ __device__ double rosstab_lookup_wave(const RossTab &tb, int mode, double temp, 
                     ix[k] = i + 1;
                 }
         }
+        // the smallest radius per quadrant, then the lowest index holding it
+        // (DPP wavefront reductions)
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            for (int o = 32; o > 0; o >>= 1) {
-                const double r2o = __shfl_xor(rm[k], o, 64);
-                const int io = __shfl_xor(ix[k], o, 64);
-                if (io != 0 && (ix[k] == 0 || r2o < rm[k] || (r2o == rm[k] && io < ix[k]))) {
-                    rm[k] = r2o;
-                    ix[k] = io;
-                }
-            }
+        for (int k = 0; k < 4; ++k) {
+            const double mk = __ockl_wfred_min_f64(rm[k]);
+            const int cand = (ix[k] != 0 && rm[k] == mk) ? ix[k] : 0x7fffffff;
+            const int ik = __ockl_wfred_min_i32(cand);
+            rm[k] = mk;
+            ix[k] = ik == 0x7fffffff ? 0 : ik;
+        }
         if (ix[0] != 0 && ix[1] != 0 && ix[2] != 0 && ix[3] != 0) {
             const double tpp = tabt[ix[0] - 1], ppp = tabp[ix[0] - 1], rpp = ross[ix[0] - 1];
             const double tpm = tabt[ix[1] - 1], ppm = tabp[ix[1] - 1], rpm = ross[ix[1] - 1];
@@ -2780,6 +2782,24 @@ int ross_need(a12_ctx *ctx, int mode, const char *who)
     return A12_OK;
 }
 
+// Pinned host staging of the iteration-end calls: each packs its inputs
+// into one contiguous block (one copy in) and reads its results back in
+// one copy.
+double *ross_pin(a12_ctx *ctx, size_t ndoubles)
+{
+    RossDev &R = ctx->ross;
+    const size_t bytes = ndoubles * sizeof(double);
+    if (bytes > R.pin_bytes) {
+        if (R.pin) (void)hipHostFree(R.pin);
+        R.pin = nullptr;
+        R.pin_bytes = 0;
+        if (hipHostMalloc(reinterpret_cast<void **>(&R.pin), bytes, hipHostMallocDefault) != hipSuccess)
+            return nullptr;
+        R.pin_bytes = bytes;
+    }
+    return R.pin;
+}
+
 bool all_set(const double *const *p, int k)
 {
     for (int i = 0; i < k; ++i)
@@ -2974,16 +2994,22 @@ int a12i_convec(a12_ctx *ctx, const a12_convec_in *in, a12_convec_io *io, int32_
     int rc = ross_need(ctx, in->irosstab, "a12_convec");
     if (rc != A12_OK) return rc;
     hipStream_t s = ctx->stream;
-    A12_HIP(ctx->ross.io.reserve(sizeof(double) * (size_t)(CV_NIN + CO_N) * n));
+    const size_t tot = (size_t)(CV_NIN + CO_N) * n;
+    A12_HIP(ctx->ross.io.reserve(sizeof(double) * tot));
+    double *h = ross_pin(ctx, tot);
+    if (!h) {
+        ctx->err = "a12_convec: pinned host staging";
+        return A12_ERR_OOM;
+    }
     double *d = ctx->ross.io.as<double>(), *dio = d + (size_t)CV_NIN * n;
-    for (int k = 0; k < 9; ++k)
-        A12_HIP(hipMemcpyAsync(d + (size_t)k * n, cin[k], sizeof(double) * n, hipMemcpyHostToDevice, s));
-    A12_HIP(hipMemcpyAsync(d + (size_t)CV_EDENS * n, in->edens, sizeof(double) * 4 * n, hipMemcpyHostToDevice, s));
-    A12_HIP(hipMemcpyAsync(d + (size_t)CV_RHO4 * n, in->rho4, sizeof(double) * 4 * n, hipMemcpyHostToDevice, s));
-    A12_HIP(hipMemcpyAsync(d + (size_t)CV_T0 * n, io->t, sizeof(double) * n, hipMemcpyHostToDevice, s));
-    for (int k = 0; k <= CO_P; ++k)
-        A12_HIP(hipMemcpyAsync(dio + (size_t)k * n, cio[k], sizeof(double) * n, hipMemcpyHostToDevice, s));
-    A12_HIP(hipMemcpyAsync(dio + (size_t)CO_FLXCNV1 * n, io->flxcnv1, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    double *hio = h + (size_t)CV_NIN * n;
+    for (int k = 0; k < 9; ++k) std::memcpy(h + (size_t)k * n, cin[k], sizeof(double) * n);
+    std::memcpy(h + (size_t)CV_EDENS * n, in->edens, sizeof(double) * 4 * n);
+    std::memcpy(h + (size_t)CV_RHO4 * n, in->rho4, sizeof(double) * 4 * n);
+    std::memcpy(h + (size_t)CV_T0 * n, io->t, sizeof(double) * n);
+    for (int k = 0; k <= CO_P; ++k) std::memcpy(hio + (size_t)k * n, cio[k], sizeof(double) * n);
+    std::memcpy(hio + (size_t)CO_FLXCNV1 * n, io->flxcnv1, sizeof(double) * n);
+    A12_HIP(hipMemcpyAsync(d, h, sizeof(double) * tot, hipMemcpyHostToDevice, s));
     const ConvecPar c = {n, in->ifconv, in->ifcorr, in->nconv, in->use4pt, in->smooth, in->irosstab,
                          in->grav, in->mixlth, in->flux, in->overwt};
     hipLaunchKernelGGL(k_convec_mlt, dim3(n), dim3(64), 0, s, ross_view(ctx), c, d, dio);
@@ -2992,11 +3018,12 @@ int a12i_convec(a12_ctx *ctx, const a12_convec_in *in, a12_convec_io *io, int32_
         hipLaunchKernelGGL(k_convec_post, dim3(1), dim3(64), 0, s, c, d, dio);
         A12_CHECK_LAUNCH();
     }
+    A12_HIP(hipMemcpyAsync(hio, dio, sizeof(double) * (size_t)CO_N * n, hipMemcpyDeviceToHost, s));
+    A12_HIP(hipStreamSynchronize(s));
     for (int k = 0; k < CO_N; ++k) {
         if (mlt_only && (k == CO_FLXCNV0 || k == CO_FLXCNV1)) continue;  // untouched then
-        A12_HIP(hipMemcpyAsync(cio[k], dio + (size_t)k * n, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        std::memcpy(cio[k], hio + (size_t)k * n, sizeof(double) * n);
     }
-    A12_HIP(hipStreamSynchronize(s));
     return A12_OK;
 }
 
@@ -3028,19 +3055,25 @@ int a12i_tcorr3_finish(a12_ctx *ctx, const a12_tcorr_fin_in *in, a12_tcorr_fin_i
     if (rc != A12_OK) return rc;
     hipStream_t s = ctx->stream;
     // staging: in (10 n) | io (23 n: 10 + ACCRAD + BHYD 6 + out 6) | t1 | drhox | state, nfail
-    A12_HIP(ctx->ross.io.reserve(sizeof(double) * 35 * (size_t)n + 2 * sizeof(CzcState)));
+    const size_t tot = 35 * (size_t)n + 2 * sizeof(CzcState) / sizeof(double);
+    A12_HIP(ctx->ross.io.reserve(sizeof(double) * tot));
+    double *h = ross_pin(ctx, tot);
+    if (!h) {
+        ctx->err = "a12_atlas_tcorr3_finish: pinned host staging";
+        return A12_ERR_OOM;
+    }
     double *d = ctx->ross.io.as<double>();
     double *dio = d + 10 * n, *dt1 = d + 33 * n, *ddr = d + 34 * n;
     CzcState *dst = reinterpret_cast<CzcState *>(d + 35 * n);
     int *dnf = reinterpret_cast<int *>(dst + 1);
-    for (int k = 0; k < 10; ++k)
-        A12_HIP(hipMemcpyAsync(d + (size_t)k * n, cin[k], sizeof(double) * n, hipMemcpyHostToDevice, s));
-    for (int k = 0; k < 10; ++k)
-        A12_HIP(hipMemcpyAsync(dio + (size_t)k * n, cio[k], sizeof(double) * n, hipMemcpyHostToDevice, s));
-    A12_HIP(hipMemcpyAsync(dio + 10 * n, io->accrad, sizeof(double) * n, hipMemcpyHostToDevice, s));
-    A12_HIP(hipMemcpyAsync(dio + 11 * n, io->bhyd, sizeof(double) * 6 * n, hipMemcpyHostToDevice, s));
-    A12_HIP(hipMemcpyAsync(dt1, t1, sizeof(double) * n, hipMemcpyHostToDevice, s));
-    A12_HIP(hipMemcpyAsync(dst, st, sizeof(CzcState), hipMemcpyHostToDevice, s));
+    double *hio = h + 10 * n;
+    for (int k = 0; k < 10; ++k) std::memcpy(h + (size_t)k * n, cin[k], sizeof(double) * n);
+    for (int k = 0; k < 10; ++k) std::memcpy(hio + (size_t)k * n, cio[k], sizeof(double) * n);
+    std::memcpy(hio + 10 * n, io->accrad, sizeof(double) * n);
+    std::memcpy(hio + 11 * n, io->bhyd, sizeof(double) * 6 * n);
+    std::memcpy(h + 33 * n, t1, sizeof(double) * n);
+    std::memcpy(h + 35 * n, st, sizeof(CzcState));
+    A12_HIP(hipMemcpyAsync(d, h, sizeof(double) * tot, hipMemcpyHostToDevice, s));
     FinPar fp;
     fp.c = {n, in->ifconv, in->iter, in->use_constructor, in->use4pt, in->irosstab,
             in->flux, in->mixlth, in->grav};
@@ -3056,17 +3089,17 @@ int a12i_tcorr3_finish(a12_ctx *ctx, const a12_tcorr_fin_in *in, a12_tcorr_fin_i
     hipLaunchKernelGGL(k_tcorr3_fin, dim3(1), dim3(64), 0, s, ross_view(ctx), fp, d, dio, dst,
                        dt1, ddr, dnf);
     A12_CHECK_LAUNCH();
-    for (int k = 0; k < 10; ++k)
-        A12_HIP(hipMemcpyAsync(cio[k], dio + (size_t)k * n, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-    A12_HIP(hipMemcpyAsync(io->bhyd, dio + 11 * n, sizeof(double) * 6 * n, hipMemcpyDeviceToHost, s));
-    for (int k = 0; k < 6; ++k)
-        A12_HIP(hipMemcpyAsync(cout[k], dio + (size_t)(17 + k) * n, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-    A12_HIP(hipMemcpyAsync(t1, dt1, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-    if (drhox) A12_HIP(hipMemcpyAsync(drhox, ddr, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-    A12_HIP(hipMemcpyAsync(st, dst, sizeof(CzcState), hipMemcpyDeviceToHost, s));
-    int nf = 0;
-    A12_HIP(hipMemcpyAsync(&nf, dnf, sizeof(int), hipMemcpyDeviceToHost, s));
+    // one copy back: io, t1, drhox, the state and the count
+    A12_HIP(hipMemcpyAsync(hio, dio, sizeof(double) * (tot - 10 * (size_t)n), hipMemcpyDeviceToHost, s));
     A12_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < 10; ++k) std::memcpy(cio[k], hio + (size_t)k * n, sizeof(double) * n);
+    std::memcpy(io->bhyd, hio + 11 * n, sizeof(double) * 6 * n);
+    for (int k = 0; k < 6; ++k) std::memcpy(cout[k], hio + (size_t)(17 + k) * n, sizeof(double) * n);
+    std::memcpy(t1, h + 33 * n, sizeof(double) * n);
+    if (drhox) std::memcpy(drhox, h + 34 * n, sizeof(double) * n);
+    std::memcpy(st, h + 35 * n, sizeof(CzcState));
+    int nf = 0;
+    std::memcpy(&nf, reinterpret_cast<const char *>(h + 35 * n) + sizeof(CzcState), sizeof(int));
     if (nfail) *nfail = nf;
     return A12_OK;
 }
