@@ -292,6 +292,15 @@ int a12_atlas_select_lines(a12_ctx *ctx, const a12_atlas_atm *atm, const double 
                              nkept);
 }
 
+int a12_atlas_select_all(a12_ctx *ctx, const a12_atlas_atm *atm, const double *wavetab,
+                         const a12_sel_lists *lists, int32_t *linedata_out, int64_t cap,
+                         int64_t *nkept)
+{
+    if (!ctx_ok(ctx)) return A12_ERR_ARG;
+    A12_HIP(hipSetDevice(ctx->device));
+    return a12i_atlas_select_all(ctx, atm, wavetab, lists, linedata_out, cap, nkept);
+}
+
 int a12_atlas_linop1_run(a12_ctx *ctx, int32_t prec)
 {
     if (!ctx_ok(ctx)) return A12_ERR_ARG;

@@ -84,6 +84,7 @@ struct AtlasDev {
 
 struct SelectDev {
     DevBuf tabcont, xnfdop, iwt, wavetab, tablog, xm, cand, flag, cnt, off, kept, mol2;
+    DevBuf dia, dkey, dord, h2o, igr;   // diatomic records, keys, heap order; H2O; H2O widths
     bool tablog_ready = false;
 };
 
@@ -192,6 +193,8 @@ int a12i_atlas_linop1_run(a12_ctx *ctx, int prec);
 int a12i_atlas_shapes(int idx, int32_t *pts_block_batch_flat);
 int a12i_atlas_autotune(a12_ctx *ctx, int prec, float *ms, int32_t *best);
 // Implemented in a12_select.hip
+int a12i_atlas_select_all(a12_ctx *ctx, const a12_atlas_atm *a, const double *wavetab,
+                          const a12_sel_lists *lists, int32_t *out, int64_t cap, int64_t *nkept);
 int a12i_atlas_select(a12_ctx *ctx, const a12_atlas_atm *a, const double *wavetab,
                       const int32_t *pred, int64_t npred, const a12_mol2_rec *mol2, int64_t nmol2,
                       int32_t *out, int64_t cap, int64_t *nkept);

@@ -34,6 +34,37 @@ MODULE a12gpu_iface
     INTEGER(C_INT32_T) :: code, iso, gr100, flag
   END TYPE a12_mol2_rec
 
+  ! diatomic_record_t (src/mod_mklinelist.f90:71-78), 48 bytes: the
+  ! reference's own type carries the same fields in this order
+  TYPE, BIND(C) :: a12_diatomic_rec
+    REAL(C_DOUBLE)     :: wlvac_nm, gflog_dex, elo_cm, gammar_log, gammaw_log
+    INTEGER(C_INT32_T) :: molcode, pad
+  END TYPE a12_diatomic_rec
+
+  ! h2opokazatel.bin record (src/atlas12_modules.f90:19659-19660)
+  TYPE, BIND(C) :: a12_h2o_rec
+    INTEGER(C_INT32_T) :: iwl
+    INTEGER(C_INT16_T) :: ielo, igflog
+  END TYPE a12_h2o_rec
+
+  ! SELECTLINES' seven lists (C_LOC of each array, or C_NULL_PTR with 0)
+  TYPE, BIND(C) :: a12_sel_lists
+    TYPE(C_PTR)        :: pred
+    INTEGER(C_INT64_T) :: npred
+    TYPE(C_PTR)        :: lowobs
+    INTEGER(C_INT64_T) :: nlowobs
+    TYPE(C_PTR)        :: hilines
+    INTEGER(C_INT64_T) :: nhilines
+    TYPE(C_PTR)        :: dia
+    INTEGER(C_INT64_T) :: ndia
+    TYPE(C_PTR)        :: mol2
+    INTEGER(C_INT64_T) :: nmol2
+    TYPE(C_PTR)        :: h2o
+    INTEGER(C_INT64_T) :: nh2o
+    TYPE(C_PTR)        :: h3p
+    INTEGER(C_INT64_T) :: nh3p
+  END TYPE a12_sel_lists
+
   ! State of SYNTHE's two hot loops (src/synthe.f90:476-718)
   TYPE, BIND(C) :: a12_synthe_atm
     INTEGER(C_INT32_T) :: nrhox, length
@@ -258,6 +289,17 @@ MODULE a12gpu_iface
       INTEGER(C_INT64_T), VALUE :: npred
       TYPE(a12_mol2_rec), INTENT(IN) :: mol2(*)
       INTEGER(C_INT64_T), VALUE :: nmol2
+      INTEGER(C_INT32_T), INTENT(OUT) :: linedata_out(*)
+      INTEGER(C_INT64_T), VALUE :: cap
+      INTEGER(C_INT64_T), INTENT(OUT) :: nkept
+    END FUNCTION
+    INTEGER(C_INT32_T) FUNCTION a12_atlas_select_all(ctx, atm, wavetab, lists, linedata_out, &
+        cap, nkept) BIND(C, NAME='a12_atlas_select_all')
+      IMPORT :: C_INT32_T, C_INT64_T, C_PTR, C_DOUBLE, a12_atlas_atm, a12_sel_lists
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(a12_atlas_atm), INTENT(IN) :: atm
+      REAL(C_DOUBLE), INTENT(IN) :: wavetab(*)
+      TYPE(a12_sel_lists), INTENT(IN) :: lists
       INTEGER(C_INT32_T), INTENT(OUT) :: linedata_out(*)
       INTEGER(C_INT64_T), VALUE :: cap
       INTEGER(C_INT64_T), INTENT(OUT) :: nkept

@@ -32,7 +32,7 @@ EXPORTS = ["a12_ctx_create", "a12_ctx_destroy", "a12_last_error", "a12_stream",
            "a12_set_profiling", "a12_stage_ms", "a12_synthe_eval_split",
            "a12_atlas_load", "a12_atlas_linop1_run", "a12_atlas_linop1_fetch",
            "a12_atlas_set_kapp", "a12_atlas_rt_run", "a12_atlas_rt_fetch",
-           "a12_atlas_select_lines", "a12_atlas_stage_ms", "a12_atlas_shape_info",
+           "a12_atlas_select_lines", "a12_atlas_select_all", "a12_atlas_stage_ms", "a12_atlas_shape_info",
            "a12_atlas_set_shape", "a12_atlas_autotune", "a12_scale_model", "a12_mlp_init",
            "a12_mlp_get_params", "a12_mlp_set_params", "a12_mlp_set_data", "a12_mlp_grads",
            "a12_mlp_train", "a12_mlp_infer", "a12_mlp_gemm", "a12_atlas_tcorr3",
@@ -829,6 +829,61 @@ def atlas_select_lines(ctx: Context, z, pred, mol2, mion: int = 1006):
         C.c_int64(len(pr)), mo.ctypes.data_as(C.c_void_p), C.c_int64(len(mo)),
         out.ctypes.data_as(C.c_void_p), C.c_int64(len(out)), C.byref(nk)),
         "a12_atlas_select_lines")
+    return out[:nk.value]
+
+
+class SelLists(C.Structure):
+    """a12_sel_lists"""
+    _fields_ = [("pred", C.c_void_p), ("npred", C.c_int64), ("lowobs", C.c_void_p),
+                ("nlowobs", C.c_int64), ("hilines", C.c_void_p), ("nhilines", C.c_int64),
+                ("dia", C.c_void_p), ("ndia", C.c_int64), ("mol2", C.c_void_p),
+                ("nmol2", C.c_int64), ("h2o", C.c_void_p), ("nh2o", C.c_int64),
+                ("h3p", C.c_void_p), ("nh3p", C.c_int64)]
+
+
+SEL_LISTS = {"pred": 16, "lowobs": 16, "hilines": 16, "dia": 48, "mol2": 32, "h2o": 8, "h3p": 16}
+
+
+def atlas_select_all(ctx: Context, z, lists: dict, mion: int = 1006):
+    """SELECTLINES over all seven lists (src/atlas12_modules.f90:19171-19781)
+    on the GPU.  lists maps SEL_LISTS names to records: the 16-byte lists as
+    (n, 4) int32, dia as records.DIATOMIC_DTYPE (reader order), mol2 as
+    workloads.MOL2_DTYPE, h2o as workloads.H2O_DTYPE; absent ones are empty.
+    z holds tabcont, iwavetab, wavetab, hckt, teff and xnfdop, either
+    (mion, n) or the rows ion_idx.  Returns the kept LINEDATA (m, 4) int32."""
+    n = len(z["hckt"])
+    if "ion_idx" in z:
+        xnfdop = np.zeros((mion, n))
+        xnfdop[z["ion_idx"]] = z["xnfdop"]
+    else:
+        xnfdop = _f64(z["xnfdop"])
+        mion = xnfdop.shape[0]
+    keep = {"tabcont": _f64(z["tabcont"]), "xnfdop": xnfdop, "hckt": _f64(z["hckt"]),
+            "wavetab": _f64(z["wavetab"])}
+    iw = np.ascontiguousarray(z["iwavetab"], np.int32)
+    a = AtlasAtm()
+    a.nrhox, a.numnu, a.mion, a.ldion, a.ldtab, a.nwave1 = n, 0, mion, n, n, len(iw)
+    a.tabcont, a.xnfdop, a.hckt = _ptr(keep["tabcont"]), _ptr(keep["xnfdop"]), _ptr(keep["hckt"])
+    a.iwavetab = iw.ctypes.data_as(C.c_void_p)
+    a.teff = float(z["teff"])
+    sl = SelLists()
+    held, total = [], 0
+    for name, size in SEL_LISTS.items():
+        r = lists.get(name)
+        if r is None or len(r) == 0:
+            continue
+        r = np.ascontiguousarray(r)
+        if r.nbytes != size * len(r):
+            raise ValueError(f"{name} records must be {size} bytes")
+        held.append(r)
+        setattr(sl, name, r.ctypes.data_as(C.c_void_p))
+        setattr(sl, "n" + name, len(r))
+        total += len(r)
+    out = np.zeros((max(total, 1), 4), np.int32)
+    nk = C.c_int64(0)
+    ctx._check(ctx.lib.a12_atlas_select_all(
+        ctx.h, C.byref(a), _ptr(keep["wavetab"]), C.byref(sl), out.ctypes.data_as(C.c_void_p),
+        C.c_int64(len(out)), C.byref(nk)), "a12_atlas_select_all")
     return out[:nk.value]
 
 

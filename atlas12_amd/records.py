@@ -138,41 +138,43 @@ def predict_to_lte(rec: np.ndarray, grid: Grid) -> np.ndarray:
 # ---------------------------------------------------------------------------
 
 # molec_dispatch (:2455-2586): isotope code [and molecule code] -> nelion,
-# x1 + x2 (the isotopic log-abundance corrections added to log gf).  A table
+# x1 + x2 (the isotopic log-abundance corrections added to log gf), and the
+# ATLAS12 MOLCODE (0: a species outside ATLAS12's diatomic set).  A table
 # the list format defines.  Entries keyed (iso, icode) override (iso, None).
+# fudge is 0 in every case.
 _MOL_DISPATCH = {
-    (1, None): (240, 0.0, -5.0), (2, None): (240, 0.0, -4.469),
-    (12, 606): (264, -.005, -.005), (12, 608): (276, -.005, -.001),
-    (12, 106): (246, 0.0, -.005), (12, None): (270, -.005, -.002),
-    (13, 606): (264, -.005, -1.955), (13, 608): (276, -1.955, -.001),
-    (13, 106): (246, 0.0, -1.955), (13, None): (270, -1.955, -.002),
-    (14, None): (252, 0.0, -.002),
-    (15, 607): (270, -.005, -2.444), (15, None): (252, 0.0, -2.444),
-    (16, 813): (324, 0.0, -.001), (16, None): (258, 0.0, -.001),
-    (17, 813): (324, 0.0, -3.398), (17, None): (276, -.005, -3.398),
-    (18, 814): (330, -.035, -2.690), (18, 608): (276, -.005, -2.690),
-    (18, 813): (324, 0.0, -2.690), (18, None): (258, 0.0, -2.690),
-    (23, None): (492, 0.0, 0.0),
-    (24, 812): (318, -.001, -.102), (24, None): (300, 0.0, -.105),
-    (25, 812): (318, -.001, -1.000), (25, None): (300, 0.0, -.996),
-    (26, 812): (318, -.001, -.958), (26, None): (300, 0.0, -.947),
-    (28, 814): (330, -.035, -.001), (28, None): (312, 0.0, -.035),
-    (29, 814): (330, -1.328, -.001), (29, None): (312, 0.0, -1.331),
-    (30, 814): (330, -1.510, -.001), (30, None): (312, 0.0, -1.516),
-    (33, None): (264, -1.955, -1.955), (39, None): (498, -.030, 0.0),
-    (40, 820): (354, -.013, -.001), (40, None): (342, -.013, 0.0),
-    (41, None): (498, -1.172, 0.0), (42, None): (342, -2.189, 0.0),
-    (43, None): (342, -2.870, 0.0), (44, None): (342, -1.681, 0.0),
-    (46, 120): (342, -4.398, 0.0), (46, None): (366, 0.0, -1.101),
-    (47, None): (366, 0.0, -1.138),
-    (48, 120): (342, -2.728, 0.0), (48, None): (366, 0.0, -.131),
-    (49, None): (366, 0.0, -1.259),
-    (50, 124): (432, -1.362, 0.0), (50, None): (366, 0.0, -1.272),
-    (51, None): (372, 0.0, -.001), (52, None): (432, -.077, 0.0),
-    (53, None): (432, -1.022, 0.0),
-    (54, 124): (432, -1.626, 0.0), (54, None): (444, -1.237, 0.0),
-    (56, None): (444, -.038, 0.0), (57, None): (444, -1.658, 0.0),
-    (58, None): (444, -2.553, 0.0),
+    (1, None): (240, 0.0, -5.0, 8410), (2, None): (240, 0.0, -4.469, 8411),
+    (12, 606): (264, -.005, -.005, 8680), (12, 608): (276, -.005, -.001, 8700),
+    (12, 106): (246, 0.0, -.005, 8460), (12, None): (270, -.005, -.002, 8690),
+    (13, 606): (264, -.005, -1.955, 8681), (13, 608): (276, -1.955, -.001, 8701),
+    (13, 106): (246, 0.0, -1.955, 8461), (13, None): (270, -1.955, -.002, 8691),
+    (14, None): (252, 0.0, -.002, 8470),
+    (15, 607): (270, -.005, -2.444, 8692), (15, None): (252, 0.0, -2.444, 8471),
+    (16, 813): (324, 0.0, -.001, 0), (16, None): (258, 0.0, -.001, 8480),
+    (17, 813): (324, 0.0, -3.398, 0), (17, None): (276, -.005, -3.398, 8702),
+    (18, 814): (330, -.035, -2.690, 8896), (18, 608): (276, -.005, -2.690, 8704),
+    (18, 813): (324, 0.0, -2.690, 0), (18, None): (258, 0.0, -2.690, 8482),
+    (23, None): (492, 0.0, 0.0, 0),
+    (24, 812): (318, -.001, -.102, 0), (24, None): (300, 0.0, -.105, 8510),
+    (25, 812): (318, -.001, -1.000, 0), (25, None): (300, 0.0, -.996, 8511),
+    (26, 812): (318, -.001, -.958, 0), (26, None): (300, 0.0, -.947, 8512),
+    (28, 814): (330, -.035, -.001, 8890), (28, None): (312, 0.0, -.035, 8530),
+    (29, 814): (330, -1.328, -.001, 8891), (29, None): (312, 0.0, -1.331, 8531),
+    (30, 814): (330, -1.510, -.001, 8892), (30, None): (312, 0.0, -1.516, 8532),
+    (33, None): (264, -1.955, -1.955, 8682), (39, None): (498, -.030, 0.0, 0),
+    (40, 820): (354, -.013, -.001, 0), (40, None): (342, -.013, 0.0, 8580),
+    (41, None): (498, -1.172, 0.0, 0), (42, None): (342, -2.189, 0.0, 8581),
+    (43, None): (342, -2.870, 0.0, 8582), (44, None): (342, -1.681, 0.0, 8583),
+    (46, 120): (342, -4.398, 0.0, 8584), (46, None): (366, 0.0, -1.101, 0),
+    (47, None): (366, 0.0, -1.138, 0),
+    (48, 120): (342, -2.728, 0.0, 0), (48, None): (366, 0.0, -.131, 0),
+    (49, None): (366, 0.0, -1.259, 0),
+    (50, 124): (432, -1.362, 0.0, 8620), (50, None): (366, 0.0, -1.272, 0),
+    (51, None): (372, 0.0, -.001, 8960), (52, None): (432, -.077, 0.0, 8621),
+    (53, None): (432, -1.022, 0.0, 8622),
+    (54, 124): (432, -1.626, 0.0, 8623), (54, None): (444, -1.237, 0.0, 8640),
+    (56, None): (444, -.038, 0.0, 8641), (57, None): (444, -1.658, 0.0, 8642),
+    (58, None): (444, -2.553, 0.0, 8643),
 }
 
 
@@ -189,11 +191,10 @@ MOLBROAD_TREF = 3000.0
 MOLBROAD_JFLOOR = 0.1
 
 
-def load_mol_broad(lines) -> dict:
+def load_mol_broad_table(lines) -> dict:
     """load_mol_broad (:414-485) over the text lines of data/mol_broad.dat:
-    code -> population-weighted gamma_w (MOLBROAD_GWEF), the value every MOL2
-    record of that molecule takes (molbroad_gw_elo with
-    MOLBROAD_JFROM_ELO = .FALSE., :396-411)."""
+    code -> (MOLBROAD_GW, MOLBROAD_DG, MOLBROAD_GWEF), the J = 0 width, its
+    slope in J and the population-weighted width."""
     kbol, clight, pi = 1.380649e-16, 2.99792458e10, 3.14159265358979
     pref, wh2, tref = 1.01325e6, 0.85, MOLBROAD_TREF
     out = {}
@@ -220,10 +221,18 @@ def load_mol_broad(lines) -> dict:
                 gj = max(gw - dg * x, MOLBROAD_JFLOOR * gw)
                 wsum = wsum + wj
                 gsum = gsum + wj * gj
-            out[code] = gsum / wsum if wsum > 0.0 else gw
+            gwef = gsum / wsum if wsum > 0.0 else -1.0
         else:
-            out[code] = gw
+            gwef = gw
+        out[code] = (gw, dg, gwef)
     return out
+
+
+def load_mol_broad(lines) -> dict:
+    """code -> population-weighted gamma_w (MOLBROAD_GWEF of load_mol_broad,
+    :414-485), the value every MOL2 record of that molecule takes
+    (molbroad_gw_elo with MOLBROAD_JFROM_ELO = .FALSE., :396-411)."""
+    return {c: v[2] for c, v in load_mol_broad_table(lines).items()}
 
 
 def mol2_to_lte(rec: np.ndarray, grid: Grid, molbroad: dict | None = None) -> np.ndarray:
@@ -278,3 +287,122 @@ def mol2_to_lte(rec: np.ndarray, grid: Grid, molbroad: dict | None = None) -> np
                      np.float32(gammar / frq4pi), np.float32(gammas / frq4pi),
                      np.float32(gammaw * 1.0 / frq4pi), np.float32(icode)))
     return np.array(rows, LTE_DTYPE) if rows else np.zeros(0, LTE_DTYPE)
+
+
+# ---------------------------------------------------------------------------
+# ASCII diatomic lists for ATLAS12 (read_diatomics_for_atlas)
+# ---------------------------------------------------------------------------
+
+# diatomic_record_t (src/mod_mklinelist.f90:71-78), the BIND(C) mirror of
+# a12_diatomic_rec
+DIATOMIC_DTYPE = np.dtype([("wlvac_nm", "<f8"), ("gflog_dex", "<f8"), ("elo_cm", "<f8"),
+                           ("gammar_log", "<f8"), ("gammaw_log", "<f8"), ("molcode", "<i4"),
+                           ("pad", "<i4")])
+
+
+def mol_molcode(iso: int, icode: int) -> int:
+    """The ATLAS12 MOLCODE molec_dispatch returns (:2455-2586); 0 for a
+    species outside ATLAS12's diatomic set."""
+    v = _MOL_DISPATCH.get((iso, icode)) or _MOL_DISPATCH.get((iso, None))
+    return 0 if v is None else v[3]
+
+
+def _f_field(s: str, d: int) -> float:
+    """Fw.d input editing of one field with blanks ignored (an all-blank
+    field is 0); without a decimal point the last d digits are the
+    fraction.  Raises ValueError where the Fortran read reports an error."""
+    t = s.replace(" ", "")
+    if not t:
+        return 0.0
+    t = t.replace("D", "E").replace("d", "e")
+    mant, _, ex = t.partition("E") if "E" in t else t.partition("e")
+    if not mant or mant in "+-":
+        raise ValueError(s)
+    if "." not in mant:
+        return float(f"{mant}e{int(ex or 0) - d}")
+    return float(t)
+
+
+def _i_field(s: str) -> int:
+    t = s.replace(" ", "")
+    return int(t) if t else 0
+
+
+def read_diatomic_ascii(lines, wlbeg_nm: float, wlend_nm: float, broad: dict | None = None,
+                        gammaw_scale: float = 1.0) -> np.ndarray:
+    """read_one_diatomic_ascii (:2312-2427) over the text lines of one
+    Kurucz molecular list: fixed columns (F10.4, F7.3, F5.1, F10.3, F5.1,
+    F11.3, I4, A8, A8, I2, I4) = lambda, log gf (REAL(4)), J, E, J', E',
+    code, labels, isotope, 100 log gamma_rad.  Blank lines and lines the
+    format rejects are skipped; the file ends at the first lambda beyond
+    wlend + 2.  Vacuum wavelength 1e7 / ||E'| - |E||, kept within
+    [wlbeg, wlend]; species through molec_dispatch (MOLCODE 0 skipped);
+    E_lo = min(|E|, |E'|); gamma_rad classical (2.223e13 / lambda^2) for 0;
+    gamma_w 1e-8 for an 'X' upper label else 1e-7, or the species'
+    J-dependent width from broad (load_mol_broad_table; ATLAS12 runs
+    without it), times gammaw_scale.  Returns DIATOMIC_DTYPE records in
+    file order."""
+    broad = broad or {}
+    rows = []
+    for raw in lines:
+        if isinstance(raw, bytes):
+            raw = raw.decode("latin-1")
+        buf = raw.rstrip("\n")[:160].ljust(160)
+        if not buf.strip():
+            continue
+        lt = len(buf.rstrip())
+        if buf[lt - 1] == "\r":
+            buf = buf[:lt - 1] + " " + buf[lt:]
+        try:
+            wl = _f_field(buf[0:10], 4)
+            gflog = np.float32(_f_field(buf[10:17], 3))
+            xj = np.float32(_f_field(buf[17:22], 1))
+            e = _f_field(buf[22:32], 3)
+            xjp = np.float32(_f_field(buf[32:37], 1))
+            ep = _f_field(buf[37:48], 3)
+            icode = _i_field(buf[48:52])
+            clabelp = buf[60:68]
+            iso = _i_field(buf[68:70])
+            loggr = _i_field(buf[70:74])
+        except ValueError:
+            continue
+        if abs(wl) > wlend_nm + 2.0:
+            break
+        de = abs(abs(ep) - abs(e))
+        wlvac = 1.0e7 / de if de != 0.0 else math.inf
+        if wlvac < wlbeg_nm or wlvac > wlend_nm:
+            continue
+        molcode = mol_molcode(iso, icode)
+        if molcode == 0:
+            continue
+        _, x1, x2 = mol_dispatch(iso, icode)
+        elo = min(abs(e), abs(ep))
+        gammar = 2.223e13 / (wlvac * wlvac) if loggr == 0 else math.pow(10.0, loggr * 0.01)
+        gammaw = 1.0e-8 if clabelp[0] == "X" else 1.0e-7
+        if 0 <= icode <= 11000 and broad.get(icode, (-1.0, 0.0, 0.0))[0] > 0.0:
+            gw, dg, _ = broad[icode]
+            xjlow = float(xj if abs(e) <= abs(ep) else xjp)
+            gammaw = max(gw - dg * xjlow, MOLBROAD_JFLOOR * gw)
+        gammaw = gammaw * gammaw_scale
+        rows.append((wlvac, float(gflog) + float(x1) + float(x2) + 0.0, elo,
+                     math.log10(gammar), math.log10(gammaw), molcode, 0))
+    return np.array(rows, DIATOMIC_DTYPE) if rows else np.zeros(0, DIATOMIC_DTYPE)
+
+
+def read_diatomics_for_atlas(files, wlbeg_nm: float, wlend_nm: float,
+                             broad: dict | None = None) -> np.ndarray:
+    """read_diatomics_for_atlas (:2244-2299): the ASCII molecular lists
+    named by lines.list ``mol`` entries (``.bin`` entries excluded, missing
+    files skipped), each read by :func:`read_diatomic_ascii` with the
+    SELECTLINES window [WAVETAB(1), WAVETAB(NWAVE)], concatenated in
+    manifest order, unsorted."""
+    import os
+    parts = []
+    for f in files:
+        if len(f) >= 4 and f[-4:] == ".bin":
+            continue
+        if not os.path.exists(f):
+            continue
+        with open(f, "rb") as fh:
+            parts.append(read_diatomic_ascii(fh.read().split(b"\n"), wlbeg_nm, wlend_nm, broad))
+    return np.concatenate(parts) if parts else np.zeros(0, DIATOMIC_DTYPE)

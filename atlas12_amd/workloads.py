@@ -363,6 +363,122 @@ def atlas_iteration_lists(n: int, wl0: float, wl1: float, seed: int, molecules=(
     return pred, tio
 
 
+# The other raw lists SELECTLINES reads (src/atlas12_modules.f90:19316-19781).
+# h2opokazatel.bin: 8-byte records, int32 IWL then int16 IELO, int16 IGFLOG,
+# whose signs carry the isotope (:19659-19680); h3plus.dat: 16-byte records.
+H2O_DTYPE = np.dtype([("iwl", "<i4"), ("ielo", "<i2"), ("igflog", "<i2")])
+H2O_NELION = 940
+H3P_NELION = 895
+
+
+def h2o_records(n: int, wl0: float, wl1: float, seed: int, loggf=(-4.0, 1.5),
+                elo_ev: float = 0.5, n_zero_elo: int = 0) -> np.ndarray:
+    """Seeded H2O records ascending in IWL: E_lo ~ Exp(elo_ev) in whole
+    cm^-1, log gf ~ N(loggf), isotopes 1..4 (H2 16O, 17O, 18O, HD 16O) in
+    proportion 7:1:1:1 through the signs of IELO and IGFLOG; n_zero_elo
+    records (spread over the list) with E_lo = 0 and log gf = 3."""
+    rng = np.random.default_rng(seed)
+    lam = np.sort(np.exp(rng.uniform(math.log(wl0), math.log(wl1), n)))
+    e = np.clip(np.rint(rng.exponential(elo_ev, n) * EV_TO_CM), 1, 32767).astype(np.int32)
+    code = np.maximum(_tablog_code(rng.normal(loggf[0], loggf[1], n)).astype(np.int32), 1)
+    iso = rng.choice(4, n, p=(0.7, 0.1, 0.1, 0.1)) + 1
+    if n_zero_elo:   # strong lines, so that the selection keeps them
+        z = rng.choice(n, n_zero_elo, replace=False)
+        e[z] = 0
+        code[z] = int(_tablog_code(3.0))
+    rec = np.zeros(n, H2O_DTYPE)
+    rec["iwl"] = np.rint(np.log(lam) / LOG_RATIO_2E6).astype(np.int32)
+    rec["ielo"] = np.where((iso == 1) | (iso == 2), e, -e).astype(np.int16)
+    rec["igflog"] = np.where((iso == 1) | (iso == 3), code, -code).astype(np.int16)
+    return rec
+
+
+def h3plus_records(n: int, wl0: float, wl1: float, seed: int, loggf=(0.0, 1.5)) -> np.ndarray:
+    """Seeded H3+ records (16-byte LINEDATA layout, species field 8950)."""
+    return atlas_linedata(n, wl0, wl1, seed, species=(H3P_NELION,),
+                          loggf=("normal", loggf[0], loggf[1]), elo=("exp", 1.0))
+
+
+# Kurucz molecule code, isotope code of each ASCII diatomic list the
+# generator writes (molec_dispatch, src/mod_mklinelist.f90:2455-2586)
+DIATOMIC_LISTS = {"ch.asc": ((106, 12), (106, 13)), "oh.asc": ((108, 16), (108, 18)),
+                  "c2.asc": ((606, 12), (606, 33)), "cn.asc": ((607, 12), (607, 15)),
+                  "co.asc": ((608, 12), (608, 13)), "nh.asc": ((107, 14),),
+                  "alo.asc": ((813, 16),)}
+
+
+def diatomic_ascii_lists(n: int, wl0: float, wl1: float, seed: int, loggf=(-2.0, 1.5),
+                         elo_ev: float = 0.6, dup_share: float = 0.05) -> dict:
+    """Seeded Kurucz-format ASCII molecular lists (columns F10.4, F7.3, F5.1,
+    F10.3, F5.1, F11.3, I4, A8, A8, I2, I4: lambda, log gf, J, E, J', E',
+    code, labels, isotope, 100 log gamma_rad), n lines per file of
+    DIATOMIC_LISTS, each ascending in lambda.  Terms listed in either order,
+    some with negative (predicted) energies; one in four with classical
+    damping; 'X' upper labels for a third.  dup_share of the lines repeat a
+    neighbour's terms (equal wavelengths: ties for the wavelength sort).
+    Each file also holds a blank line, a line the format rejects, and ends
+    with a line beyond the window and one after it that must not be read.
+    Returns {file name: text}."""
+    rng = np.random.default_rng(seed)
+    out = {}
+    for fname, species in DIATOMIC_LISTS.items():
+        lam = np.exp(rng.uniform(math.log(wl0), math.log(wl1), n))
+        elo = np.round(rng.exponential(elo_ev, n) * EV_TO_CM, 3)
+        eup = np.round(elo + 1.0e7 / lam, 3)
+        dup = np.nonzero(rng.random(n) < dup_share)[0]
+        dup = dup[dup > 0]
+        elo[dup], eup[dup] = elo[dup - 1], eup[dup - 1]
+        wl = 1.0e7 / np.abs(eup - elo)
+        order = np.argsort(wl, kind="stable")
+        sp = rng.integers(0, len(species), n)
+        gf = rng.normal(loggf[0], loggf[1], n)
+        swap = rng.random(n) < 0.3
+        neg = rng.random(n) < 0.1
+        xj = rng.integers(0, 80, n) + 0.5
+        lines = []
+        for k, i in enumerate(order):
+            icode, iso = species[sp[i]]
+            e, ep = (eup[i], elo[i]) if swap[i] else (elo[i], eup[i])
+            if neg[i]:
+                ep = -ep
+            xjp = xj[i] + rng.integers(-1, 2)
+            labp = ("X" if rng.random() < 0.33 else "A") + f"{int(rng.integers(0, 99)):02d}e"
+            loggr = 0 if rng.random() < 0.25 else int(rng.integers(700, 900))
+            lines.append(f"{wl[i]:10.4f}{gf[i]:7.3f}{xj[i]:5.1f}{e:10.3f}{max(xjp, 0.5):5.1f}"
+                         f"{ep:11.3f}{icode:4d}{'A01e':8s}{labp:8s}{iso:2d}{loggr:4d}")
+            if k == n // 3:
+                lines.append("")
+            if k == n // 2:
+                lines.append("  bad line" + "?" * 60)
+        lines.append(f"{wl1 * 1.5:10.4f}{-1.0:7.3f}{0.5:5.1f}{100.0:10.3f}{1.5:5.1f}"
+                     f"{100.0 + 1.0e7 / wl1:11.3f}{species[0][0]:4d}{'A':8s}{'X':8s}"
+                     f"{species[0][1]:2d}{0:4d}")
+        lines.append(lines[0])
+        out[fname] = "\n".join(lines) + "\n"
+    return out
+
+
+def atlas_all_lists(mols, seed: int, wl_range, n_pred: int, n_lowobs: int, n_hilines: int,
+                    n_h2o: int, n_h3p: int, n_dia: int):
+    """Every raw list SELECTLINES reads: the predicted and TiO lists (as
+    atlas_iteration_lists, TiO log gf ~ N(4, 1.5)), observed low lines,
+    high-excitation lines (one species in ten a slot, 1004, that stellar
+    atmospheres leave empty), H2O (six with E_lo = 0), H3+ and the ASCII
+    diatomic lists (text per file).  Returns (pred, tio, lowobs, hilines,
+    h2o, h3p, {file: text})."""
+    w0, w1 = wl_range
+    s = seed
+    pred, tio = atlas_iteration_lists(n_pred, w0, w1, seed=s, molecules=mols, tio_loggf=(4.0, 1.5))
+    lowobs = atlas_linedata(n_lowobs, w0, w1, s + 1, loggf=("normal", -1.0, 1.5))
+    hi_species = ATLAS_SPECIES + (atlas_nelion(26, 1),) * 2 + (1004,)
+    hilines = atlas_linedata(n_hilines, w0, w1, s + 2, species=hi_species,
+                             loggf=("normal", 0.0, 1.5), elo=("uniform", 4.0, 10.0))
+    h2o = h2o_records(n_h2o, w0, w1, s + 3, loggf=(0.0, 1.5), n_zero_elo=6)
+    h3p = h3plus_records(n_h3p, w0, w1, s + 4, loggf=(3.0, 1.5))
+    dia = diatomic_ascii_lists(n_dia, w0, w1, s + 5)
+    return pred, tio, lowobs, hilines, h2o, h3p, dia
+
+
 # ---------------------------------------------------------------------------
 # Batched SYNTHE: many models over one shared line list
 # ---------------------------------------------------------------------------

@@ -482,6 +482,60 @@ int a12_atlas_select_lines(a12_ctx *ctx, const a12_atlas_atm *atm, const double 
                            const int32_t *pred, int64_t npred, const a12_mol2_rec *mol2,
                            int64_t nmol2, int32_t *linedata_out, int64_t cap, int64_t *nkept);
 
+/* diatomic_record_t (src/mod_mklinelist.f90:71-78): what
+ * read_diatomics_for_atlas (:2244-2299) returns for SELECTLINES, 48 bytes. */
+typedef struct {
+    double  wlvac_nm;     /* vacuum wavelength [nm]                           */
+    double  gflog_dex;    /* log gf with the isotope corrections x1 + x2     */
+    double  elo_cm;       /* lower term energy [cm^-1]                        */
+    double  gammar_log;   /* log10 gamma_rad                                   */
+    double  gammaw_log;   /* log10 gamma_w                                     */
+    int32_t molcode;      /* ATLAS12 MOLCODE, one of the 46 SELECTLINES knows */
+    int32_t pad;
+} a12_diatomic_rec;
+
+/* h2opokazatel.bin record, 8 bytes (src/atlas12_modules.f90:19659-19680):
+ * the signs of ielo and igflog carry the isotope. */
+typedef struct {
+    int32_t iwl;      /* NINT-encoded ln(lambda) / ln(1 + 1/2e6)          */
+    int16_t ielo;     /* +-E_lo [cm^-1]                                     */
+    int16_t igflog;   /* +-(1000 log gf + 16384)                            */
+} a12_h2o_rec;
+
+/* SELECTLINES' seven lists in store order (src/atlas12_modules.f90:
+ * 19262-19767); any may be empty (NULL with count 0).  The 16-byte lists
+ * are LINEDATA records, int32 (n, 4). */
+typedef struct {
+    const int32_t *pred;              int64_t npred;     /* gfpred29dec2014.bin, :19262 */
+    const int32_t *lowobs;            int64_t nlowobs;   /* lowobsat12.bin, :19316      */
+    const int32_t *hilines;           int64_t nhilines;  /* hilines.bin, :19355         */
+    const a12_diatomic_rec *dia;      int64_t ndia;      /* ASCII diatomics, :19417     */
+    const a12_mol2_rec *mol2;         int64_t nmol2;     /* TiO MOL2, :19557            */
+    const a12_h2o_rec *h2o;           int64_t nh2o;      /* h2opokazatel.bin, :19650    */
+    const int32_t *h3p;               int64_t nh3p;      /* h3plus.dat, :19723          */
+} a12_sel_lists;
+
+/* SELECTLINES over all seven lists (src/atlas12_modules.f90:19171-19781),
+ * the store equal to the reference's record for record:
+ *   pred     kept as read; NELION outside 1..mion skipped;
+ *   lowobs   kept as read; its wavelength walk goes on from pred's;
+ *   hilines  kept as read;
+ *   dia      in read_diatomics_for_atlas's order; sorted here by
+ *            NINT(ln(lambda) / ln(1 + 1/2e6)) with SELECTLINES' heapsort
+ *            (its order among equal keys), ISOX-corrected, repacked;
+ *   mol2     TiO, repacked; skipped above TEFF 8000 K;
+ *   h2o      isotope from the signs, width of the walk's bin, repacked;
+ *            skipped above TEFF 8000 K;
+ *   h3p      log gf - 1.272, repacked.
+ * Lists must ascend in wavelength and lie within the continuum bins; a
+ * NELION outside 1..mion in lowobs or hilines (which the reference indexes
+ * unchecked) and an unknown diatomic MOLCODE (where the reference exits)
+ * return A12_ERR_ARG.  Other arguments as a12_atlas_select_lines, which is
+ * this call with pred and mol2 only.  Host pointers, synchronous. */
+int a12_atlas_select_all(a12_ctx *ctx, const a12_atlas_atm *atm, const double *wavetab,
+                         const a12_sel_lists *lists, int32_t *linedata_out, int64_t cap,
+                         int64_t *nkept);
+
 /* ---- KAPP: continuum opacity ----------------------------------------------- */
 /* KAPP (src/atlas12_modules.f90:11202-11324) with every source it calls:
  * HOP (:11359), H2PLOP (:12224), HMINOP (:12360), H2MINOP (:12611),

@@ -340,81 +340,276 @@ static int32_t pack_pair(int16_t lo, int16_t hi)
     return (int32_t)((uint32_t)(uint16_t)lo | ((uint32_t)(uint16_t)hi << 16));
 }
 
-int64_t or_atlas_select(const or_select_in *in, const int32_t *pred, int64_t npred,
-                        const or_mol2_rec *mol2, int64_t nmol2, int32_t *out)
+/* Float to default INTEGER as x86-64's truncating conversion gives it:
+ * the "integer indefinite" INT32_MIN for NaN and out-of-range values
+ * (the reference's INT(LOG10(0.0) ...) for an H2O line with E_lo = 0). */
+static int32_t f2i_trunc(double x)
 {
-    static const int TIO_ISOCORR[5] = {-1101, -1138, -131, -1259, -1272};   /* :19204 */
-    const double cen_prefac = 0.026538 / C_SQRTPI;                        /* :19179 */
-    const double ratiolg = log(1.0 + 1.0 / 2000000.0);
-    const int nw1 = in->nwave + 1;
-    const double hcktn = in->hckt[in->nrhox - 1];
-    double *xm = select_xnfdopmax(in);
-    int64_t nk = 0;
-    double freq = 0.0;
+    if (!(x > -2147483649.0 && x < 2147483648.0)) return INT32_MIN;
+    return (int32_t)x;
+}
 
-    /* (1) predicted lines, records kept as read (:19262-19313) */
-    int nu = 0;
-    for (int64_t l = 0; l < npred; ++l) {
-        const int32_t *r = pred + 4 * l;
+static int16_t abs16(int16_t v)   /* ABS of an INTEGER(2), wrapping */
+{
+    return (int16_t)(v < 0 ? -(int32_t)v : v);
+}
+
+void or_sort_indirect_by_int(int32_t *keys, int32_t *order, int64_t n)
+{
+    /* 1-based heap over keys[0..n-1] (:1985-2039) */
+    if (n <= 1) return;
+#define K(i) keys[(i) - 1]
+#define O(i) order[(i) - 1]
+#define SWAP(a, b) do { int32_t t_ = K(a); K(a) = K(b); K(b) = t_; \
+                        t_ = O(a); O(a) = O(b); O(b) = t_; } while (0)
+    for (int64_t i = n / 2; i >= 1; --i) {
+        int64_t parent = i;
+        for (;;) {
+            int64_t child = 2 * parent;
+            if (child > n) break;
+            if (child + 1 <= n && K(child + 1) > K(child)) child = child + 1;
+            if (K(parent) >= K(child)) break;
+            SWAP(parent, child);
+            parent = child;
+        }
+    }
+    for (int64_t i = n; i >= 2; --i) {
+        SWAP(1, i);
+        int64_t parent = 1;
+        for (;;) {
+            int64_t child = 2 * parent;
+            if (child > i - 1) break;
+            if (child + 1 <= i - 1 && K(child + 1) > K(child)) child = child + 1;
+            if (K(parent) >= K(child)) break;
+            SWAP(parent, child);
+            parent = child;
+        }
+    }
+#undef SWAP
+#undef O
+#undef K
+}
+
+/* SELECTLINES' MOLCODES and ISOX tables (:19181-19200) */
+static const int SEL_MOLCODES[46] = {
+    8410, 8411, 8460, 8461, 8470, 8471, 8480, 8481, 8482, 8510, 8511, 8512, 8530, 8531, 8532,
+    8580, 8581, 8582, 8583, 8584, 8620, 8621, 8622, 8623, 8640, 8641, 8642, 8643, 8680, 8681,
+    8682, 8690, 8691, 8692, 8693, 8700, 8701, 8702, 8703, 8704, 8705, 8890, 8891, 8892, 8896,
+    8960};
+static const int SEL_ISOX[46] = {
+    0, -4469, -5, -1955, -2, -2444, -1, -3398, -2690, -105, -996, -947, -35, -1331, -1516,
+    -13, -2189, -2870, -1681, -4398, -1362, -77, -1022, -1626, -1237, -38, -1658, -2553, -10,
+    -1960, -3910, -7, -1957, -2449, -4399, -6, -1956, -3403, -5353, -2695, -4645, -36, -1332,
+    -1517, -2725, -2};
+
+typedef struct {
+    const or_select_in *in;
+    const double *xm;
+    int nw1;
+    double hcktn;
+    int nu;         /* the walk's NU, 0-based */
+    double freq;    /* FREQ of the walk's last step */
+    int32_t *out;
+    int64_t nk;
+} sel_walk;
+
+static void sel_step(sel_walk *w, int32_t iwl)
+{
+    while (iwl >= w->in->iwavetab[w->nu]) {
+        w->freq = C_CLIGHT_NMS / w->in->wavetab[w->nu];
+        ++w->nu;
+    }
+}
+
+static double sel_x(const sel_walk *w, int nelion)
+{
+    return w->xm[(size_t)(nelion - 1) * w->nw1 + w->nu];
+}
+
+static void sel_put(sel_walk *w, int32_t iwl, int16_t ielion, int16_t ielo, int16_t igflog,
+                    int16_t igr, int16_t igs, int16_t igw)
+{
+    int32_t *o = w->out + 4 * w->nk++;
+    o[0] = iwl;
+    o[1] = pack_pair(ielion, ielo);
+    o[2] = pack_pair(igflog, igr);
+    o[3] = pack_pair(igs, igw);
+}
+
+/* The 16-byte paths: (1) predicted (:19262-19313), (2) observed low lines
+ * (:19316-19350; the walk goes on from path 1), (3) hilines (:19355-19395),
+ * (7) H3+ (:19723-19767).  Returns -1 for a NELION outside 1..mion. */
+static int sel_rec16(sel_walk *w, const int32_t *rec, int64_t n, int path)
+{
+    const double cen_prefac = 0.026538 / C_SQRTPI;   /* :19179 */
+    for (int64_t l = 0; l < n; ++l) {
+        const int32_t *r = rec + 4 * l;
         const int32_t iwl = r[0];
         const int16_t ielion = (int16_t)(r[1] & 0xffff), ielo = (int16_t)((uint32_t)r[1] >> 16);
-        const int16_t igflog = (int16_t)(r[2] & 0xffff);
-        while (iwl >= in->iwavetab[nu]) {
-            freq = C_CLIGHT_NMS / in->wavetab[nu];
-            ++nu;
+        int16_t igflog = (int16_t)(r[2] & 0xffff);
+        const int16_t igr = (int16_t)((uint32_t)r[2] >> 16);
+        sel_step(w, iwl);
+        int nelion = abs(ielion / 10);
+        if (path == 7) {
+            const int kgflog = igflog;
+            igflog = (int16_t)(kgflog - 1272 > 1 ? kgflog - 1272 : 1);
+            nelion = 895;
         }
-        const int nelion = abs(ielion / 10);
-        if (nelion < 1 || nelion > in->mion) continue;
-        const double x = xm[(size_t)(nelion - 1) * nw1 + nu];
-        if (x <= 1.0e-37) continue;
-        const double cen = cen_prefac * tablog(igflog) * x / freq;
+        if (nelion < 1 || nelion > w->in->mion) {
+            if (path == 1) continue;
+            return -1;
+        }
+        const double x = sel_x(w, nelion);
+        if (path <= 2 ? x <= 1.0e-37 : x == 0.0) continue;
+        const double cen = cen_prefac * tablog(igflog) * x / w->freq;
         if (cen < 1.0) continue;
         const double tablog8 = tablog(ielo);
-        if (cen * exp(-tablog8 * hcktn) < 1.0) continue;
-        memcpy(out + 4 * nk, r, 16);
-        ++nk;
+        if (cen * exp(-tablog8 * w->hcktn) < 1.0) continue;
+        if (path == 7)
+            sel_put(w, iwl, ielion, ielo, igflog, igr, 1, 9384);
+        else
+            memcpy(w->out + 4 * w->nk++, r, 16);
+    }
+    return 0;
+}
+
+int64_t or_atlas_select_all(const or_select_in *in, const or_select_lists *L, int32_t *out)
+{
+    static const int TIO_ISOCORR[5] = {-1101, -1138, -131, -1259, -1272};   /* :19204 */
+    static const int H2O_ISOCORR[4] = {-1, -3398, -2690, -5000};            /* :19206 */
+    const double cen_prefac = 0.026538 / C_SQRTPI;                        /* :19179 */
+    const double ratiolg = log(1.0 + 1.0 / 2000000.0);
+    double *xm = select_xnfdopmax(in);
+    sel_walk w = {in, xm, in->nwave + 1, in->hckt[in->nrhox - 1], 0, 0.0, out, 0};
+    int64_t rc = 0;
+
+    if (sel_rec16(&w, L->pred, L->npred, 1) < 0) { rc = -1; goto done; }
+    if (sel_rec16(&w, L->lowobs, L->nlowobs, 2) < 0) { rc = -1; goto done; }
+    w.nu = 0;
+    if (sel_rec16(&w, L->hilines, L->nhilines, 3) < 0) { rc = -1; goto done; }
+
+    /* (4) ASCII diatomics (:19417-19536): keys NINT(LOG(wl)/RATIOLG),
+     * heapsorted, then walked in key order */
+    if (L->ndia > 0) {
+        const int64_t nd = L->ndia;
+        int32_t *keys = (int32_t *)malloc(sizeof(int32_t) * nd);
+        int32_t *order = (int32_t *)malloc(sizeof(int32_t) * nd);
+        for (int64_t i = 0; i < nd; ++i) {
+            keys[i] = (int32_t)lround(log(L->dia[i].wlvac_nm) / ratiolg);
+            order[i] = (int32_t)(i + 1);
+        }
+        or_sort_indirect_by_int(keys, order, nd);
+        w.nu = 0;
+        int molcodeold = 0, imol = -1;
+        for (int64_t ii = 0; ii < nd; ++ii) {
+            const or_diatomic_rec *d = L->dia + (order[ii] - 1);
+            const int32_t iwl = keys[ii];
+            sel_step(&w, iwl);
+            const int molcode = d->molcode;
+            if (molcode != molcodeold) {
+                molcodeold = molcode;
+                imol = -1;
+                for (int k = 0; k < 46; ++k)
+                    if (molcode == SEL_MOLCODES[k]) { imol = k; break; }
+                if (imol < 0) { rc = -2; break; }
+            }
+            if (imol < 0) { rc = -2; break; }
+            const double gflog = d->gflog_dex - SEL_ISOX[imol] * 1.0e-3;
+            const int16_t ielion = (int16_t)molcode;
+            const int16_t ielo = (int16_t)(int32_t)lround(log10(fmax(d->elo_cm, 1.0e-10)) * 1000.0 + 16384.0);
+            int16_t igflog = (int16_t)(int32_t)lround(gflog * 1000.0 + 16384.0);
+            const int16_t igr = (int16_t)(int32_t)lround(d->gammar_log * 1000.0 + 16384.0);
+            const int16_t igw = (int16_t)(int32_t)lround(d->gammaw_log * 1000.0 + 16384.0);
+            const int kgflog = igflog;
+            igflog = (int16_t)(kgflog + SEL_ISOX[imol] > 1 ? kgflog + SEL_ISOX[imol] : 1);
+            const int nelion = abs(ielion / 10);
+            const double x = sel_x(&w, nelion);
+            if (x == 0.0) continue;
+            const double cen = cen_prefac * tablog(igflog) * x / w.freq;
+            if (cen < 1.0) continue;
+            const double tablog8 = tablog(ielo);
+            if (cen * exp(-tablog8 * w.hcktn) < 1.0) continue;
+            sel_put(&w, iwl, ielion, ielo, igflog, igr, 1, igw);
+        }
+        free(keys);
+        free(order);
+        if (rc < 0) goto done;
     }
 
-    /* (5) TiO MOL2 records, repacked (:19557-19647); none above 8000 K */
+    /* (5) TiO MOL2 records, repacked (:19557-19647); (6) H2O (:19650-19720);
+     * neither above 8000 K */
     if (in->teff <= 8000.0) {
-        nu = 0;
-        for (int64_t l = 0; l < nmol2; ++l) {
-            const or_mol2_rec *m = mol2 + l;
+        w.nu = 0;
+        for (int64_t l = 0; l < L->nmol2; ++l) {
+            const or_mol2_rec *m = L->mol2 + l;
             if (m->code != 822) continue;
             const int iso = m->iso - 45;
             if (iso < 1 || iso > 5) continue;
             const double wlv = (double)m->wl;
             if (wlv <= 0.0) continue;
             const int32_t iwl = (int32_t)(log(wlv) / ratiolg + 0.5);
-            while (iwl >= in->iwavetab[nu]) {
-                freq = C_CLIGHT_NMS / in->wavetab[nu];
-                ++nu;
-            }
+            sel_step(&w, iwl);
             const int kgflog = (int)lround((double)m->loggf * 1000.0) + 16384;
             const int16_t igflog = (int16_t)(kgflog + TIO_ISOCORR[iso - 1] > 1 ? kgflog + TIO_ISOCORR[iso - 1] : 1);
-            const int nelion = 895;
-            if (nelion > in->mion) { free(xm); return -1; }
-            const double x = xm[(size_t)(nelion - 1) * nw1 + nu];
+            if (895 > in->mion) { rc = -1; goto done; }
+            const double x = sel_x(&w, 895);
             if (x == 0.0) continue;
-            const double cen = cen_prefac * tablog(igflog) * x / freq;
+            const double cen = cen_prefac * tablog(igflog) * x / w.freq;
             if (cen < 1.0) continue;
             const double elotio = fmin(fabs((double)m->e1), fabs((double)m->e2));
-            if (cen * exp(-elotio * hcktn) < 1.0) continue;
-            const int16_t ielion = (int16_t)(8949 + iso);
+            if (cen * exp(-elotio * w.hcktn) < 1.0) continue;
             const int16_t ielo = (int16_t)(int)(log10(fmax(elotio, 1.0e-10)) * 1000.0 + 16384.5);
             const double gr = m->gr100 == 0 ? 2.223e13 / (wlv * wlv)
                                             : pow(10.0, (double)m->gr100 * 0.01);
             const int16_t igr = (int16_t)(int)(log10(gr) * 1000.0 + 16384.5);
-            int32_t *o = out + 4 * nk;
-            o[0] = iwl;
-            o[1] = pack_pair(ielion, ielo);
-            o[2] = pack_pair(igflog, igr);
-            o[3] = pack_pair(1, 9384);
-            ++nk;
+            sel_put(&w, iwl, (int16_t)(8949 + iso), ielo, igflog, igr, 1, 9384);
+        }
+
+        w.nu = 0;
+        int16_t igr_walk = 0;
+        for (int64_t l = 0; l < L->nh2o; ++l) {
+            const or_h2o_rec *h = L->h2o + l;
+            const int32_t iwl = h->iwl;
+            while (iwl >= in->iwavetab[w.nu]) {
+                w.freq = C_CLIGHT_NMS / in->wavetab[w.nu];
+                /* GAMMAR = 2.474D-22 * FREQ**2 * 0.001 (a REAL(4) literal) */
+                const double gammar = 2.474e-22 * (w.freq * w.freq) * (double)0.001f;
+                igr_walk = (int16_t)f2i_trunc(log10(gammar) * 1000.0 + 16384.5);
+                ++w.nu;
+            }
+            const int iso = h->ielo > 0 ? (h->igflog > 0 ? 1 : 2) : (h->igflog > 0 ? 3 : 4);
+            const double elo = (double)abs16(h->ielo);
+            const int kgflog = abs16(h->igflog);
+            const int16_t igflog = (int16_t)(kgflog + H2O_ISOCORR[iso - 1] > 1 ? kgflog + H2O_ISOCORR[iso - 1] : 1);
+            if (940 > in->mion) { rc = -1; goto done; }
+            const double x = sel_x(&w, 940);
+            if (x == 0.0) continue;
+            const double cen = cen_prefac * tablog(igflog) * x / w.freq;
+            if (cen < 1.0) continue;
+            if (cen * exp(-elo * w.hcktn) < 1.0) continue;
+            const int16_t ielo = (int16_t)f2i_trunc(log10(elo) * 1000.0 + 16384.5);
+            sel_put(&w, iwl, (int16_t)(-(9399 + iso)), ielo, igflog, igr_walk, 1, 9384);
         }
     }
+
+    w.nu = 0;
+    if (sel_rec16(&w, L->h3p, L->nh3p, 7) < 0) { rc = -1; goto done; }
+    rc = w.nk;
+done:
     free(xm);
-    return nk;
+    return rc;
+}
+
+int64_t or_atlas_select(const or_select_in *in, const int32_t *pred, int64_t npred,
+                        const or_mol2_rec *mol2, int64_t nmol2, int32_t *out)
+{
+    or_select_lists L;
+    memset(&L, 0, sizeof L);
+    L.pred = pred;
+    L.npred = npred;
+    L.mol2 = mol2;
+    L.nmol2 = nmol2;
+    return or_atlas_select_all(in, &L, out);
 }
 
 

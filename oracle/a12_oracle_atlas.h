@@ -79,6 +79,41 @@ typedef struct {
 int64_t or_atlas_select(const or_select_in *in, const int32_t *pred, int64_t npred,
                         const or_mol2_rec *mol2, int64_t nmol2, int32_t *out);
 
+/* diatomic_record_t (src/mod_mklinelist.f90:71-78), 48 bytes. */
+typedef struct {
+    double wlvac_nm, gflog_dex, elo_cm, gammar_log, gammaw_log;
+    int32_t molcode, pad;
+} or_diatomic_rec;
+
+/* h2opokazatel.bin record (src/atlas12_modules.f90:19659-19660), 8 bytes. */
+typedef struct {
+    int32_t iwl;
+    int16_t ielo, igflog;
+} or_h2o_rec;
+
+/* The seven lists SELECTLINES reads, in store order; 16-byte records are
+ * int32 (n, 4); dia in read_diatomics_for_atlas's order (unsorted). */
+typedef struct {
+    const int32_t *pred;            int64_t npred;
+    const int32_t *lowobs;          int64_t nlowobs;
+    const int32_t *hilines;         int64_t nhilines;
+    const or_diatomic_rec *dia;     int64_t ndia;
+    const or_mol2_rec *mol2;        int64_t nmol2;
+    const or_h2o_rec *h2o;          int64_t nh2o;
+    const int32_t *h3p;             int64_t nh3p;
+} or_select_lists;
+
+/* SELECTLINES (src/atlas12_modules.f90:19171-19781) over all seven lists.
+ * Returns the number of kept records written to out, -1 for a NELION
+ * outside 1..mion (the observed-low-line and hilines paths index without a
+ * bounds check), -2 for a diatomic MOLCODE outside its table. */
+int64_t or_atlas_select_all(const or_select_in *in, const or_select_lists *L, int32_t *out);
+
+/* sort_indirect_by_int (src/atlas12_modules.f90:1985-2039): the heapsort
+ * whose order among equal keys the diatomic store inherits.  keys and
+ * order (n each) are permuted together. */
+void or_sort_indirect_by_int(int32_t *keys, int32_t *order, int64_t n);
+
 /* SCALE_MODEL(TEFF_NEW, LOGG_NEW), src/atlas12_modules.f90:6967-7060.
  * in: RHOX, T, P, XNE, ABROSS, PRAD, VTURB, each (n); sc: TEFF, GRAV,
  * PRADK0, PCON, PTURB0.  out (kw each): TAUSTD, RHOX, T, P, XNE, ABROSS,

@@ -417,6 +417,67 @@ def atlas_select(z, pred, mol2):
     return out[:m]
 
 
+class OrSelectLists(C.Structure):
+    _fields_ = [("pred", C.c_void_p), ("npred", C.c_int64), ("lowobs", C.c_void_p),
+                ("nlowobs", C.c_int64), ("hilines", C.c_void_p), ("nhilines", C.c_int64),
+                ("dia", C.c_void_p), ("ndia", C.c_int64), ("mol2", C.c_void_p),
+                ("nmol2", C.c_int64), ("h2o", C.c_void_p), ("nh2o", C.c_int64),
+                ("h3p", C.c_void_p), ("nh3p", C.c_int64)]
+
+
+SELECT_LISTS = ("pred", "lowobs", "hilines", "dia", "mol2", "h2o", "h3p")
+
+
+def atlas_select_all(z, lists: dict):
+    """SELECTLINES over all seven lists: lists maps SELECT_LISTS names to
+    records (16-byte lists as (n, 4) int32, dia as DIATOMIC_DTYPE, mol2 as
+    MOL2_DTYPE, h2o as H2O_DTYPE); absent ones are empty.  z: tabcont,
+    iwavetab, wavetab, hckt, teff, mion, and xnfdop (mion, n) or its rows
+    ion_idx.  Returns the kept store (m, 4) int32."""
+    n = int(len(z["hckt"]))
+    mion = int(z["mion"])
+    xd = z["xnfdop"] if "ion_idx" not in z else full_ion(z["xnfdop"], z["ion_idx"], mion, n)
+    k = _Keep()
+    k.a = {"tabcont": _f64(z["tabcont"]), "wavetab": _f64(z["wavetab"]),
+           "xnfdop": _f64(xd), "hckt": _f64(z["hckt"])}
+    iw = np.ascontiguousarray(z["iwavetab"], np.int32)
+    s = OrSelectIn()
+    s.nrhox, s.mion, s.ldion, s.nwave = n, mion, n, len(z["wavetab"])
+    s.teff = float(z["teff"])
+    s.iwavetab = iw.ctypes.data_as(C.c_void_p)
+    for kk, v in k.a.items():
+        setattr(s, kk, _p(v))
+    sl = OrSelectLists()
+    keep, total = [], 0
+    for name in SELECT_LISTS:
+        a = lists.get(name)
+        if a is None or len(a) == 0:
+            continue
+        a = np.ascontiguousarray(a)
+        keep.append(a)
+        setattr(sl, name, a.ctypes.data_as(C.c_void_p))
+        setattr(sl, "n" + name, len(a))
+        total += len(a)
+    out = np.zeros((max(total, 1), 4), np.int32)
+    L = lib()
+    L.or_atlas_select_all.restype = C.c_int64
+    m = L.or_atlas_select_all(C.byref(s), C.byref(sl), out.ctypes.data_as(C.c_void_p))
+    if m == -1:
+        raise ValueError("or_atlas_select_all: NELION outside 1..mion")
+    if m == -2:
+        raise ValueError("or_atlas_select_all: diatomic MOLCODE outside its table")
+    return out[:m]
+
+
+def sort_indirect_by_int(keys):
+    """(sorted keys, 1-based order) of SELECTLINES' heapsort."""
+    kk = np.ascontiguousarray(keys, np.int32).copy()
+    order = np.arange(1, len(kk) + 1, dtype=np.int32)
+    lib().or_sort_indirect_by_int(kk.ctypes.data_as(C.c_void_p), order.ctypes.data_as(C.c_void_p),
+                                  C.c_int64(len(kk)))
+    return kk, order
+
+
 def scale_model(n, kw, tau1lg, steplg, sc, teff_new, logg_new, fields):
     """SCALE_MODEL on one model.  fields: (7, n) RHOX, T, P, XNE, ABROSS,
     PRAD, VTURB; sc: TEFF, GRAV, PRADK0, PCON, PTURB0.  Returns (out (10, kw):

@@ -11,11 +11,16 @@
 !
 ! lines.bin: int32 count, then count LINEDATA records (4 x int32).
 ! With <seldir>, the line store is instead SELECTLINES' product from the
-! raw lists in <seldir> (gfpred29dec2014.bin, lines.list naming a MOL2
-! TiO list); it is written to <out.bin>.sel as int32 count, the records,
-! then WAVETAB(1:NWAVE), and lines.bin is not read.
+! raw lists in <seldir> (gfpred29dec2014.bin, lowobsat12.bin, hilines.bin,
+! h2opokazatel.bin, h3plus.dat, lines.list naming a MOL2 TiO list and the
+! ASCII diatomic lists); it is written to <out.bin>.sel as int32 count, the
+! records, then WAVETAB(1:NWAVE), and lines.bin is not read.  The diatomic
+! records read_diatomics_for_atlas returns for the same window go to
+! <out.bin>.dia: int32 count, then the wavelengths, log gf, E_lo, log
+! gamma_rad, log gamma_w (REAL(8) arrays) and the MOLCODEs (int32).
 PROGRAM ref_atlas_dump
   USE mod_atlas_data
+  USE mod_mklinelist, only: read_diatomics_for_atlas, diatomic_record_t
   IMPLICIT NONE
   CHARACTER(LEN=512) :: deck, fin, fout, arg, envval, seldir
   INTEGER :: envlen, envstat, u, nustart_in, nuhi_in, nl, nu, j, nel, i
@@ -27,6 +32,9 @@ PROGRAM ref_atlas_dump
   ! CONVEC probe: the EOS evaluations it makes (EDENS, RHO at four states)
   INTEGER :: uc
   REAL(8) :: cv_e(kw, 4), cv_r(kw, 4)
+  ! diatomic reader probe
+  TYPE(diatomic_record_t), ALLOCATABLE :: drec_l(:)
+  INTEGER :: nd_l
 
   CALL GET_COMMAND_ARGUMENT(1, deck)
   CALL GET_COMMAND_ARGUMENT(2, arg); READ(arg, *) nustart_in
@@ -117,6 +125,17 @@ PROGRAM ref_atlas_dump
     WRITE(u) nl
     WRITE(u) ((LINEDATA(i, j), i = 1, 4), j = 1, nl)
     WRITE(u) WAVETAB(1:NWAVE)
+    CLOSE(u)
+    CALL read_diatomics_for_atlas(lines_list_path=TRIM(DATADIR)//'lines.list', &
+                                  datadir=TRIM(DATADIR), wlbeg_nm=WAVETAB(1), &
+                                  wlend_nm=WAVETAB(NWAVE), recs=drec_l, n_recs=nd_l)
+    OPEN(NEWUNIT=u, FILE=TRIM(fout)//'.dia', ACCESS='STREAM', FORM='UNFORMATTED', &
+         STATUS='REPLACE')
+    WRITE(u) nd_l
+    IF (nd_l .GT. 0) THEN
+      WRITE(u) drec_l(1:nd_l)%wlvac_nm, drec_l(1:nd_l)%gflog_dex, drec_l(1:nd_l)%elo_cm, &
+               drec_l(1:nd_l)%gammar_log, drec_l(1:nd_l)%gammaw_log, drec_l(1:nd_l)%molcode
+    END IF
     CLOSE(u)
   ELSE
     ! The line store (SELECTLINES' product) from the caller's file

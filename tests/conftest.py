@@ -73,6 +73,34 @@ def select_case():
 
 
 @pytest.fixture(scope="session")
+def select_all_case():
+    """tests/golden/atlas_select_all.npz with its seven raw lists
+    regenerated from the recorded configuration: (state, lists for
+    atlas_select_all with the diatomics read by the ASCII reader, the
+    diatomic list texts)."""
+    import hashlib
+    import json
+
+    from atlas12_amd import records
+    from atlas12_amd import workloads as wl
+    z = dict(np.load(os.path.join(GOLDEN, "atlas_select_all.npz")))
+    cfg = json.loads(str(z["config"]))
+    mols = tuple(int(m) for m in z["mol_nelion"])
+    pred, tio, lowobs, hilines, h2o, h3p, dia = wl.atlas_all_lists(mols, **cfg)
+    files = {"lowobsat12.bin": lowobs.tobytes(), "hilines.bin": hilines.tobytes(),
+             "h2opokazatel.bin": h2o.tobytes(), "h3plus.dat": h3p.tobytes()}
+    files.update({k: v.encode() for k, v in dia.items()})
+    digest = hashlib.sha256(b"".join([pred.tobytes(), tio.tobytes()] +
+                                     [files[k] for k in sorted(files)])).hexdigest()
+    assert digest == str(z["raw_sha256"]), "line-list generator drifted from the fixture"
+    wt = z["wavetab"]
+    drec = np.concatenate([records.read_diatomic_ascii(t.encode().split(b"\n"), wt[0], wt[-1])
+                           for t in dia.values()])
+    lists = dict(pred=pred, lowobs=lowobs, hilines=hilines, dia=drec, mol2=tio, h2o=h2o, h3p=h3p)
+    return z, lists, dia
+
+
+@pytest.fixture(scope="session")
 def gpu_ctx():
     """A device context on cuda:0 -- fails (does not skip) when the HIP
     library is missing, so a GPU run can never pass on a fallback."""

@@ -35,6 +35,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, REPO)
 from atlas12_amd import workloads as wl  # noqa: E402
+from atlas12_amd.records import DIATOMIC_DTYPE  # noqa: E402
 
 REF = "/root/reference"
 REFBIN = os.path.join(REPO, "oracle", "_ref", "bin")
@@ -456,10 +457,15 @@ def read_kapp_dump(path):
     return d
 
 
-def atlas_case(atm, nustart, nuhi, linedata, raw=None, tcorr=False, kapp_waves=None):
+def atlas_case(atm, nustart, nuhi, linedata, raw=None, tcorr=False, kapp_waves=None,
+               sel_files=None, sel_mols=()):
     """Runs ref_atlas_dump.  raw = (predicted records, TiO MOL2 records):
     the store is then SELECTLINES' selection from these lists, returned
-    as d["selected"] with d["wavetab"]."""
+    as d["selected"] with d["wavetab"].  sel_files {name: bytes} are
+    further files of the selection directory (lowobsat12.bin, hilines.bin,
+    h2opokazatel.bin, h3plus.dat, ASCII molecular lists), sel_mols the
+    ASCII lists lines.list names; read_diatomics_for_atlas's records come
+    back as d["diatomic"]."""
     tmp = tempfile.mkdtemp(prefix="a12a_")
     g = os.path.join(tmp, "gfall.dat")
     open(g, "w").close()
@@ -479,8 +485,13 @@ def atlas_case(atm, nustart, nuhi, linedata, raw=None, tcorr=False, kapp_waves=N
         with open(os.path.join(sel, "gfpred29dec2014.bin"), "wb") as fh:
             fh.write(np.ascontiguousarray(raw[0], "<i4").tobytes())
         wl.write_mol2(raw[1], os.path.join(sel, "tio.bin"))
+        for name, data in (sel_files or {}).items():
+            with open(os.path.join(sel, name), "wb") as fh:
+                fh.write(data)
         with open(os.path.join(sel, "lines.list"), "w") as fh:
             fh.write("mol tio.bin\n")
+            for name in sel_mols:
+                fh.write(f"mol {name}\n")
         extra = [sel]
     env = dict(os.environ, ATLAS12=root)
     tco = os.path.join(tmp, "tcorr.bin")
@@ -509,6 +520,15 @@ def atlas_case(atm, nustart, nuhi, linedata, raw=None, tcorr=False, kapp_waves=N
         ns = int(np.frombuffer(buf, "<i4", 1)[0])
         d["selected"] = np.frombuffer(buf, "<i4", 4 * ns, 4).reshape(ns, 4).copy()
         d["wavetab"] = np.frombuffer(buf, "<f8", 361, 4 + 16 * ns).copy()
+        with open(out + ".dia", "rb") as fh:
+            buf = fh.read()
+        nd = int(np.frombuffer(buf, "<i4", 1)[0])
+        dia = np.zeros(nd, DIATOMIC_DTYPE)
+        for k, name in enumerate(("wlvac_nm", "gflog_dex", "elo_cm", "gammar_log", "gammaw_log")):
+            dia[name] = np.frombuffer(buf, "<f8", nd, 4 + 8 * nd * k)
+        dia["molcode"] = np.frombuffer(buf, "<i4", nd, 4 + 40 * nd)
+        assert len(buf) == 4 + 44 * nd
+        d["diatomic"] = dia
     shutil.rmtree(tmp)
     return d
 
@@ -1125,6 +1145,35 @@ def golden_select():
                              "raw_sha256": digest}}
 
 
+SELECT_ALL = dict(seed=51, wl_range=(95.0, 20_000.0), n_pred=20_000, n_lowobs=8_000,
+                  n_hilines=8_000, n_h2o=8_000, n_h3p=4_000, n_dia=2_500)
+
+
+def golden_select_all():
+    """SELECTLINES over all seven of its lists on the plain-SYNTHE
+    atmosphere's first-iteration state, and read_diatomics_for_atlas's
+    records for the ASCII lists.  The raw lists are regenerated from
+    SELECT_ALL by the tests (sha256 recorded to catch generator drift)."""
+    atm = wl.scurve_atmosphere()
+    st = np.load(os.path.join(HERE, "atlas_plain_state.npz"))
+    mols = tuple(int(m) for m in st["mol_nelion"])
+    pred, tio, lowobs, hilines, h2o, h3p, dia = wl.atlas_all_lists(mols, **SELECT_ALL)
+    files = {"lowobsat12.bin": lowobs.tobytes(), "hilines.bin": hilines.tobytes(),
+             "h2opokazatel.bin": h2o.tobytes(), "h3plus.dat": h3p.tobytes()}
+    files.update({k: v.encode() for k, v in dia.items()})
+    d = atlas_case(atm, 9599, 10, None, raw=(pred, tio), sel_files=files,
+                   sel_mols=tuple(dia))
+    digest = hashlib.sha256(b"".join([pred.tobytes(), tio.tobytes()] +
+                                     [files[k] for k in sorted(files)])).hexdigest()
+    np.savez_compressed(os.path.join(HERE, "atlas_select_all.npz"), selected=d["selected"],
+                        diatomic=d["diatomic"], wavetab=d["wavetab"], tabcont=d["tabcont"],
+                        iwavetab=d["iwavetab"], hckt=d["hckt"], teff=d["teff"], mion=d["mion"],
+                        xnfdop=d["xnfdop"], mol_nelion=np.array(mols, np.int32),
+                        raw_sha256=np.array(digest), config=np.array(json.dumps(SELECT_ALL)))
+    return {"atlas_select_all": {"selected": len(d["selected"]),
+                                 "diatomic_records": len(d["diatomic"]), "raw_sha256": digest}}
+
+
 def scale_atmosphere():
     """The plain-SYNTHE structure with a rising Rosseland opacity, a
     radiative acceleration and a PRADK card, so SCALE_MODEL's INTEG, MAP1
@@ -1284,7 +1333,7 @@ def golden_records():
 
 
 STEPS = {"synthe_small": golden_synthe_small, "atlas_small": golden_atlas_small,
-         "state_atlas": golden_state_atlas, "select": golden_select, "scale": golden_scale,
+         "state_atlas": golden_state_atlas, "select": golden_select, "select_all": golden_select_all, "scale": golden_scale,
          "tcorr": golden_tcorr, "tcorr_conv": golden_tcorr_conv,
          "state_plain": golden_state_plain,
          "numerics": golden_numerics, "josh": golden_josh, "molec": golden_molec,

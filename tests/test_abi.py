@@ -78,6 +78,32 @@ def test_struct_layouts_atlas(tmp_path):
     assert vals[2:] == [getattr(native.AtlasAtm, f).offset for f in fields]
 
 
+def test_struct_layouts_select(tmp_path):
+    """a12_diatomic_rec, a12_h2o_rec and a12_sel_lists: the numpy record
+    types and the ctypes mirror match the C layout."""
+    import subprocess
+    from atlas12_amd import native
+    from atlas12_amd.records import DIATOMIC_DTYPE
+    from atlas12_amd.workloads import H2O_DTYPE
+    fields = [f[0] for f in native.SelLists._fields_]
+    dfields = [f for f in DIATOMIC_DTYPE.names]
+    src = tmp_path / "layout3.c"
+    body = "".join(f'printf("%zu\\n", offsetof(a12_sel_lists, {f}));' for f in fields)
+    body += "".join(f'printf("%zu\\n", offsetof(a12_diatomic_rec, {f}));' for f in dfields)
+    body += "".join(f'printf("%zu\\n", offsetof(a12_h2o_rec, {f}));' for f in H2O_DTYPE.names)
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "a12gpu.h"\n'
+                   'int main(void){printf("%zu\\n%zu\\n%zu\\n", sizeof(a12_diatomic_rec), '
+                   'sizeof(a12_h2o_rec), sizeof(a12_sel_lists));' + body + "return 0;}\n")
+    exe = tmp_path / "layout3"
+    subprocess.run(["gcc", "-I", os.path.join(REPO, "include"), str(src), "-o", str(exe)], check=True)
+    vals = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
+    assert vals[:3] == [DIATOMIC_DTYPE.itemsize, H2O_DTYPE.itemsize, ctypes.sizeof(native.SelLists)]
+    k = 3 + len(fields)
+    assert vals[3:k] == [getattr(native.SelLists, f).offset for f in fields]
+    assert vals[k:k + len(dfields)] == [DIATOMIC_DTYPE.fields[f][1] for f in dfields]
+    assert vals[k + len(dfields):] == [H2O_DTYPE.fields[f][1] for f in H2O_DTYPE.names]
+
+
 def test_fortran_interface_module(tmp_path):
     """atlas12_amd/fortran/a12gpu_iface.f90 compiles, its BIND(C) types have
     the C sizes, and calls through it reach the library (a null context is
@@ -108,14 +134,21 @@ PROGRAM t
   TYPE(a12_tcorr_fin_io) :: fo
   TYPE(a12_convec_in) :: vi
   TYPE(a12_convec_io) :: vo
-  INTEGER(C_INT32_T) :: rc
+  TYPE(a12_diatomic_rec) :: dr
+  TYPE(a12_h2o_rec) :: hr
+  TYPE(a12_sel_lists) :: sl
+  INTEGER(C_INT32_T) :: rc, ld(4)
+  INTEGER(C_INT64_T) :: nk
   PRINT *, C_SIZEOF(l(1)), C_SIZEOF(sa), C_SIZEOF(aa), C_SIZEOF(m), C_SIZEOF(tc), C_SIZEOF(mt)
   PRINT *, C_SIZEOF(cs), C_SIZEOF(ci), C_SIZEOF(fi), C_SIZEOF(fo), C_SIZEOF(vi), C_SIZEOF(vo)
+  PRINT *, C_SIZEOF(dr), C_SIZEOF(hr), C_SIZEOF(sl)
   rc = a12_synthe_opacity_run(C_NULL_PTR, A12_PREC_MIXED)
   PRINT *, rc
   rc = a12_atlas_linop1_run(C_NULL_PTR, A12_PREC_FP64)
   PRINT *, rc
   rc = a12_convec(C_NULL_PTR, vi, vo, 0)
+  PRINT *, rc
+  rc = a12_atlas_select_all(C_NULL_PTR, aa, [1.0D0], sl, ld, 0_C_INT64_T, nk)
   PRINT *, rc
 END PROGRAM t
 """)
@@ -125,13 +158,14 @@ END PROGRAM t
     subprocess.run([fc, *mod, iface, str(prog), "-o", str(exe), lib._name,
                     "-Wl,-rpath," + os.path.dirname(lib._name)], check=True, cwd=tmp_path)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
-    sizes = [int(x) for x in out[:12]]
+    sizes = [int(x) for x in out[:15]]
     assert sizes == [32, ctypes.sizeof(native.SyntheAtm), ctypes.sizeof(native.AtlasAtm), 32,
                      ctypes.sizeof(native.TcorrIn), ctypes.sizeof(native.MolTab),
                      ctypes.sizeof(native.CzcState), ctypes.sizeof(native.CzcIn),
                      ctypes.sizeof(native.TcorrFinIn), ctypes.sizeof(native.TcorrFinIo),
-                     ctypes.sizeof(native.ConvecIn), ctypes.sizeof(native.ConvecIo)]
-    assert [int(x) for x in out[12:15]] == [-1, -1, -1]
+                     ctypes.sizeof(native.ConvecIn), ctypes.sizeof(native.ConvecIo),
+                     48, 8, ctypes.sizeof(native.SelLists)]
+    assert [int(x) for x in out[15:19]] == [-1, -1, -1, -1]
 
 
 def test_atlas_wrappers_reject_wrong_shapes():

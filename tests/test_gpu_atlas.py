@@ -186,6 +186,62 @@ def test_selectlines_edges(gpu_ctx, select_case):
         native.atlas_select_lines(gpu_ctx, z, pred[:0], tio[::-1])
 
 
+def test_selectlines_all_lists_matches_reference(gpu_ctx, select_all_case):
+    """SELECTLINES over all seven lists on the GPU (82,000 raw records:
+    predicted, observed low lines, hilines, ASCII diatomics read by the
+    host reader, TiO, H2O, H3+) keeps the reference's store record for
+    record, including the heapsort order of 732 shared diatomic keys."""
+    from atlas12_amd import native
+    z, lists, _ = select_all_case
+    sel = native.atlas_select_all(gpu_ctx, z, lists)
+    assert sel.shape == z["selected"].shape
+    assert np.array_equal(sel, z["selected"])
+
+
+def test_selectlines_all_lists_each_alone_and_hot(gpu_ctx, select_all_case):
+    """Each list on its own, the observed low lines without the predicted
+    list before them, and a hot star (TiO and H2O skipped above 8000 K):
+    the GPU store equals the oracle's."""
+    from atlas12_amd import native
+    from oracle import oracle as orc
+    z, lists, _ = select_all_case
+    for name in lists:
+        one = {name: lists[name]}
+        assert np.array_equal(native.atlas_select_all(gpu_ctx, z, one),
+                              orc.atlas_select_all(z, one)), name
+    no_pred = dict(lists, pred=lists["pred"][:0])
+    assert np.array_equal(native.atlas_select_all(gpu_ctx, z, no_pred),
+                          orc.atlas_select_all(z, no_pred))
+    hot = dict(z)
+    hot["teff"] = np.float64(9000.0)
+    got = native.atlas_select_all(gpu_ctx, hot, lists)
+    assert np.array_equal(got, orc.atlas_select_all(hot, lists))
+    ie = got[:, 1:].copy().view(np.int16).reshape(len(got), 6)[:, 0].astype(int)
+    assert not np.any((ie <= -9400) & (ie >= -9403))
+
+
+def test_selectlines_all_lists_rejects_bad_input(gpu_ctx, select_all_case):
+    """What the reference cannot handle is an error, not a store: a NELION
+    beyond mion in the hilines list (indexed unchecked there), an unknown
+    diatomic MOLCODE (the reference exits), a descending list."""
+    from atlas12_amd import native
+    z, lists, _ = select_all_case
+    hi = lists["hilines"][:100].copy()
+    w = hi[:, 1:].copy().view(np.int16).reshape(len(hi), 6)
+    w[50, 0] = 20000
+    hi[:, 1:] = w.view(np.int32).reshape(len(hi), 3)
+    with pytest.raises(native.A12Error):
+        native.atlas_select_all(gpu_ctx, z, {"hilines": hi})
+    dia = lists["dia"][:50].copy()
+    dia["molcode"][7] = 8999
+    with pytest.raises(native.A12Error):
+        native.atlas_select_all(gpu_ctx, z, {"dia": dia})
+    with pytest.raises(native.A12Error):
+        native.atlas_select_all(gpu_ctx, z, {"h2o": lists["h2o"][::-1]})
+    # the context stays usable
+    assert np.array_equal(native.atlas_select_all(gpu_ctx, z, lists), z["selected"])
+
+
 def test_linop1_every_tile_shape_same_bits(gpu_ctx, atl):
     """The gather's tile shapes add lines in the same order at every point:
     XLINES is bitwise identical in all of them, at both precisions, and

@@ -140,6 +140,78 @@ def test_selectlines_planted_mistake_is_caught(orc, select_case):
     assert len(orc.atlas_select(zz, pred, tio)) > len(z["selected"])
 
 
+def _path_of(store):
+    """Source list of each kept record, from its species field: H2O
+    -9400..-9403, TiO 8950..8954, diatomic MOLCODEs 8410..8960, H3+ (kept
+    as read) 8950 -- the atomic paths are told apart by position only."""
+    w = store[:, 1:].copy().view(np.int16).reshape(len(store), 6)
+    return w[:, 0].astype(int)
+
+
+def test_selectlines_all_lists_bitwise(orc, select_all_case):
+    """SELECTLINES over all seven lists (predicted, observed low lines,
+    hilines, ASCII diatomics, TiO, H2O, H3+: 82,000 raw records) keeps the
+    reference's store record for record.  The fixture exercises the
+    observed-low-line walk continuing from the predicted one, the empty
+    species slot of hilines (XNFDOPMAX = 0), 732 diatomic wavelength keys
+    shared by several records (whose order only the reference's heapsort
+    gives), H2O isotopes from the signs and kept lines with E_lo = 0."""
+    z, lists, _ = select_all_case
+    sel = orc.atlas_select_all(z, lists)
+    assert sel.shape == z["selected"].shape
+    assert np.array_equal(sel, z["selected"])
+    ie = _path_of(sel)
+    assert np.sum((ie <= -9400) & (ie >= -9403)) > 1000     # H2O kept
+    assert np.sum((ie >= 8410) & (ie <= 8960) & (ie != 8950)) > 1000   # diatomics kept
+    w = sel[:, 1:].copy().view(np.int16).reshape(len(sel), 6)
+    assert np.sum((ie <= -9400) & (w[:, 1] == 0)) > 0        # E_lo = 0 H2O lines
+
+
+def test_selectlines_all_lists_planted_mistakes_are_caught(orc, select_all_case):
+    """Mistakes a restatement could make, each planted in the oracle's
+    inputs, change the store the reference fixture holds: the walk of the
+    observed low lines restarted (dropped carry-over), the diatomic keys
+    sorted stably instead of by the heapsort, and the H3+ log gf correction
+    dropped."""
+    z, lists, _ = select_all_case
+    ref = z["selected"]
+    # (a) restart the observed-low-line walk: run it as its own list after an
+    #     empty predicted list -- a different store from the reference's
+    alt = dict(lists, pred=lists["pred"][:0])
+    alone = orc.atlas_select_all(z, alt)
+    with_pred = orc.atlas_select_all(z, dict(lists))
+    n_pred_kept = len(orc.atlas_select_all(z, {"pred": lists["pred"]}))
+    assert not np.array_equal(alone, with_pred[n_pred_kept:])
+    # (b) a stable sort of the diatomic records changes the order of ties
+    keys = np.array([int(round(np.log(x) / np.log(1.0 + 1.0 / 2.0e6))) for x in lists["dia"]["wlvac_nm"]])
+    _, heap_order = orc.sort_indirect_by_int(keys)
+    stable = np.argsort(keys, kind="stable")
+    assert not np.array_equal(heap_order - 1, stable)
+    resorted = orc.atlas_select_all(z, dict(lists, dia=lists["dia"][heap_order - 1]))
+    assert not np.array_equal(resorted, ref)   # re-sorting a sorted list is not idempotent
+    # (c) H3+ without its -1.272 dex: more H3+ lines kept
+    h3 = lists["h3p"].copy()
+    w = h3[:, 1:].copy().view(np.int16).reshape(len(h3), 6)
+    w[:, 2] = np.minimum(w[:, 2].astype(int) + 1272, 32767).astype(np.int16)
+    h3[:, 1:] = w.view(np.int32).reshape(len(h3), 3)
+    assert len(orc.atlas_select_all(z, dict(lists, h3p=h3))) > len(ref)
+
+
+def test_heapsort_properties(orc):
+    """sort_indirect_by_int: keys non-decreasing, order a permutation that
+    carries the keys, and on distinct keys the same result as any sort."""
+    rng = np.random.default_rng(3)
+    for n in (0, 1, 2, 7, 100, 5000):
+        keys = rng.integers(0, max(n // 4, 1), n).astype(np.int32)
+        ks, order = orc.sort_indirect_by_int(keys)
+        assert np.all(np.diff(ks) >= 0)
+        assert np.array_equal(np.sort(order), np.arange(1, n + 1))
+        assert np.array_equal(keys[order - 1], ks)
+        distinct = rng.permutation(n).astype(np.int32)
+        _, o2 = orc.sort_indirect_by_int(distinct)
+        assert np.array_equal(o2 - 1, np.argsort(distinct))
+
+
 def test_scale_model_bitwise(orc, golden):
     """SCALE_MODEL after READIN: regrid only, an unchanged (Teff, log g) and
     three rescalings reproduce the reference's TAUSTD, remapped structure,

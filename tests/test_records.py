@@ -76,3 +76,46 @@ def test_mol_broad_population_weighting():
     slow = R.load_mol_broad([row.format(c=2, d=0.002, b=0.5)])[2]
     fast = R.load_mol_broad([row.format(c=3, d=0.002, b=5.0)])[3]
     assert 0.1 * flat < slow < fast < flat
+
+
+def test_diatomic_ascii_reader_matches_reference_bitwise(select_all_case):
+    """read_diatomics_for_atlas (src/mod_mklinelist.f90:2244-2427) on seven
+    generated Kurucz-format lists (CH, OH, C2, CN, CO, NH and AlO, the last
+    outside ATLAS12's set): every field of every record equals the
+    reference reader's, including terms listed in either order, negative
+    (predicted) energies, classical damping, 'X' labels, blank and
+    malformed lines and the early end past the window."""
+    z, lists, _ = select_all_case
+    ref, got = z["diatomic"], lists["dia"]
+    assert len(ref) == len(got) > 10_000
+    for f in ("wlvac_nm", "gflog_dex", "elo_cm", "gammar_log", "gammaw_log", "molcode"):
+        assert np.array_equal(got[f], ref[f]), f
+    assert 8470 in set(got["molcode"].tolist()) and 0 not in set(got["molcode"].tolist())
+
+
+def test_diatomic_ascii_reader_rules():
+    """The reader's per-line rules on hand-made lines: the window, the
+    MOLCODE-0 skip, the J of the lower term for a tabulated width, the
+    stop past wlend + 2 and Fortran F editing without a decimal point."""
+    import math
+
+    from atlas12_amd import records
+    fmt = "{:10.4f}{:7.3f}{:5.1f}{:10.3f}{:5.1f}{:11.3f}{:4d}{:8s}{:8s}{:2d}{:4d}"
+    lines = [fmt.format(500.0, -1.0, 3.5, 1000.0, 4.5, 21000.0, 106, "A", "X1", 12, 0),
+             fmt.format(500.0, -1.0, 3.5, 21000.0, 4.5, 1000.0, 813, "A", "A1", 16, 800),
+             fmt.format(600.0, -1.0, 3.5, 21000.0, 4.5, -1000.0, 106, "A", "A1", 12, 800),
+             fmt.format(9000.0, -1.0, 3.5, 1000.0, 4.5, 1100.0, 106, "A", "A1", 12, 0),
+             fmt.format(700.0, -1.0, 3.5, 1000.0, 4.5, 15285.714, 106, "A", "A1", 12, 0)]
+    r = records.read_diatomic_ascii(lines, 400.0, 800.0)
+    assert len(r) == 2 and list(r["molcode"]) == [8460, 8460]
+    assert r["wlvac_nm"][0] == 1.0e7 / 20000.0 and r["elo_cm"][1] == 1000.0
+    assert r["gammaw_log"][0] == math.log10(1.0e-8) and r["gammaw_log"][1] == math.log10(1.0e-7)
+    assert r["gammar_log"][0] == math.log10(2.223e13 / 500.0 ** 2)
+    assert r["gammar_log"][1] == math.log10(math.pow(10.0, 8.0))
+    # x1 + x2 of 12CH (0, -0.005) added in double to the REAL(4) log gf
+    assert r["gflog_dex"][0] == float(np.float32(-1.0)) + 0.0 + float(np.float32(-0.005))
+    # a tabulated width: J of the lower term (E' = -1000 is the lower one here)
+    broad = {106: (2.0e-8, 1.0e-10, 1.0e-8)}
+    r2 = records.read_diatomic_ascii(lines[2:3], 400.0, 800.0, broad)
+    assert r2["gammaw_log"][0] == math.log10(max(2.0e-8 - 1.0e-10 * 4.5, 0.1 * 2.0e-8))
+    assert records._f_field("  5000", 4) == 0.5 and records._f_field("   ", 3) == 0.0
