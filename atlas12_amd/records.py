@@ -1,18 +1,24 @@
-"""SYNTHE's line-record preparation for the predicted and MOL2 binary lists
-(src/mod_mklinelist.f90): the host side that turns on-disk records into
-the ``lte_line_t`` records the line-opacity build reads (LTE_DTYPE, the
-BIND(C) mirror of a12_lte_line).  The gfall reader is
-:func:`atlas12_amd.workloads.gfall_to_lte`.
+"""Line-record preparation on the host (src/mod_mklinelist.f90): the
+readers that turn on-disk line lists into the records the two hot paths
+read -- SYNTHE's ``lte_line_t`` (LTE_DTYPE, the BIND(C) mirror of
+a12_lte_line) and ATLAS12's diatomic_record_t (DIATOMIC_DTYPE, the mirror
+of a12_diatomic_rec, which a12_atlas_select_all filters).  The gfall reader
+is :func:`atlas12_amd.workloads.gfall_to_lte`.
 
-* :func:`predict_to_lte`   read_predict     (:1110-1329), constant 0.01502;
-* :func:`mol2_to_lte`      read_molec_bin   (:1371-1529) with molec_dispatch
-                           (:2455-2586), constant 0.026538/1.77245;
-* :func:`load_mol_broad`   load_mol_broad   (:414-485) and molbroad_gw_elo
-                           (:396-411), the per-species van der Waals widths
-                           MOL2 records take.
+* :func:`predict_to_lte`        read_predict        (:1110-1329), 0.01502;
+* :func:`mol2_to_lte`           read_molec_bin      (:1371-1529) with
+                                molec_dispatch      (:2455-2586);
+* :func:`molec_ascii_to_lte`    read_molec_ascii    (:1535-1689);
+* :func:`molec_poly_to_lte`     read_molec_poly     (:1690-1838);
+* :func:`molec_poly_bin_to_lte` read_molec_poly_bin (:1864-2012);
+* :func:`h2o_to_lte`            read_h2o            (:2034-2174);
+* :func:`read_diatomics_for_atlas` / :func:`read_diatomic_ascii`
+                                (:2244-2427), ATLAS12's diatomic lists;
+* :func:`load_mol_broad_table`  load_mol_broad      (:414-485) and
+                                molbroad_gw_elo     (:396-411).
 
-Expression types follow the reference (REAL(4) where the reference stores
-REAL(4)); the records equal the reference reader's bit for bit
+Expression types follow the reference (REAL(4) where the reference computes
+or stores REAL(4)); the records equal the reference readers' bit for bit
 (tests/test_records.py)."""
 from __future__ import annotations
 
@@ -328,6 +334,32 @@ def _i_field(s: str) -> int:
     return int(t) if t else 0
 
 
+def _mol_line(raw, width: int, code_w: int):
+    """One line of a Kurucz molecular list read as the readers do: into a
+    width-character buffer (padded, longer lines cut), a trailing CR
+    blanked, then (F10.4, F7.3, F5.1, F10.3, F5.1, F11.3, I<code_w>, A8, A8,
+    I2, I4).  Returns (lambda, log gf, J, E, J', E', code, upper label,
+    isotope, 100 log gamma_rad) with the REAL(4) fields as float32, or None
+    for a blank line or one the format rejects."""
+    if isinstance(raw, bytes):
+        raw = raw.decode("latin-1")
+    buf = raw.rstrip("\n")[:width].ljust(width)
+    if not buf.strip():
+        return None
+    lt = len(buf.rstrip())
+    if buf[lt - 1] == "\r":
+        buf = buf[:lt - 1] + " " + buf[lt:]
+    c = 48 + code_w
+    try:
+        return (_f_field(buf[0:10], 4), np.float32(_f_field(buf[10:17], 3)),
+                np.float32(_f_field(buf[17:22], 1)), _f_field(buf[22:32], 3),
+                np.float32(_f_field(buf[32:37], 1)), _f_field(buf[37:48], 3),
+                _i_field(buf[48:c]), buf[c + 8:c + 16], _i_field(buf[c + 16:c + 18]),
+                _i_field(buf[c + 18:c + 22]))
+    except ValueError:
+        return None
+
+
 def read_diatomic_ascii(lines, wlbeg_nm: float, wlend_nm: float, broad: dict | None = None,
                         gammaw_scale: float = 1.0) -> np.ndarray:
     """read_one_diatomic_ascii (:2312-2427) over the text lines of one
@@ -345,27 +377,10 @@ def read_diatomic_ascii(lines, wlbeg_nm: float, wlend_nm: float, broad: dict | N
     broad = broad or {}
     rows = []
     for raw in lines:
-        if isinstance(raw, bytes):
-            raw = raw.decode("latin-1")
-        buf = raw.rstrip("\n")[:160].ljust(160)
-        if not buf.strip():
+        f = _mol_line(raw, 160, 4)
+        if f is None:
             continue
-        lt = len(buf.rstrip())
-        if buf[lt - 1] == "\r":
-            buf = buf[:lt - 1] + " " + buf[lt:]
-        try:
-            wl = _f_field(buf[0:10], 4)
-            gflog = np.float32(_f_field(buf[10:17], 3))
-            xj = np.float32(_f_field(buf[17:22], 1))
-            e = _f_field(buf[22:32], 3)
-            xjp = np.float32(_f_field(buf[32:37], 1))
-            ep = _f_field(buf[37:48], 3)
-            icode = _i_field(buf[48:52])
-            clabelp = buf[60:68]
-            iso = _i_field(buf[68:70])
-            loggr = _i_field(buf[70:74])
-        except ValueError:
-            continue
+        wl, gflog, xj, e, xjp, ep, icode, clabelp, iso, loggr = f
         if abs(wl) > wlend_nm + 2.0:
             break
         de = abs(abs(ep) - abs(e))
@@ -406,3 +421,208 @@ def read_diatomics_for_atlas(files, wlbeg_nm: float, wlend_nm: float,
         with open(f, "rb") as fh:
             parts.append(read_diatomic_ascii(fh.read().split(b"\n"), wlbeg_nm, wlend_nm, broad))
     return np.concatenate(parts) if parts else np.zeros(0, DIATOMIC_DTYPE)
+
+
+# ---------------------------------------------------------------------------
+# SYNTHE's molecular ASCII, polyatomic and H2O readers
+# ---------------------------------------------------------------------------
+
+TEFF_COOL_LIMIT = 8000.0      # TiO and H2O lists read at or below (:251)
+TEFF_POLYMOL_LIMIT = 3500.0   # polyatomic lists read at or below (:247)
+
+
+def _lte_row(wlvac, grid, gf, elo, gammar, gammas, gammaw, nelion, code, congf_pref):
+    ixwl = math.floor(math.log(wlvac) / grid.ratiolg + 0.5)
+    freq = 2.99792458e17 / wlvac
+    congf = congf_pref * gf / freq
+    frq4pi = freq * 12.5664
+    return (ixwl - grid.ixwlbeg + 1, np.float32(congf), nelion, np.float32(elo),
+            np.float32(gammar / frq4pi), np.float32(gammas / frq4pi),
+            np.float32(gammaw * 1.0 / frq4pi), np.float32(code))
+
+
+def _ascii_widths(clabelp, icode, xj, xjp, e, ep, broad):
+    """gamma_s, gamma_w of the ASCII molecular readers: 3e-8, 1e-8 for an
+    'X' upper label else 3e-5, 1e-7; a tabulated width (broad: code ->
+    (GW, DG, GWEF)) at the J of the lower term, floored at JFLOOR GW."""
+    gammas, gammaw = (3.0e-8, 1.0e-8) if clabelp[0] == "X" else (3.0e-5, 1.0e-7)
+    if 0 <= icode <= 11000 and broad.get(icode, (-1.0, 0.0, 0.0))[0] > 0.0:
+        gw, dg, _ = broad[icode]
+        xjlow = float(xj if abs(e) <= abs(ep) else xjp)
+        gammaw = max(gw - dg * xjlow, MOLBROAD_JFLOOR * gw)
+    return gammas, gammaw
+
+
+def molec_ascii_to_lte(lines, grid: Grid, broad: dict | None = None) -> np.ndarray:
+    """read_molec_ascii (src/mod_mklinelist.f90:1535-1689) over the text
+    lines of one Kurucz molecular list (the ``mol`` entries without a .bin
+    sidecar): FeH code 156 read as 126; the file ends at the first lambda
+    beyond wlend + 2; vacuum wavelength 1e7 / ||E'| - |E|| kept within
+    [wlbeg - 1, wlend + 1]; isotopes through molec_dispatch (unknown ones
+    skipped); gf = exp((log gf + x1 + x2) ln 10) with the sum in REAL(4);
+    CGF 0.026538/1.77245 gf / nu; widths as :func:`_ascii_widths`.  broad:
+    :func:`load_mol_broad_table`."""
+    broad = broad or {}
+    rows = []
+    for raw in lines:
+        f = _mol_line(raw, 160, 4)
+        if f is None:
+            continue
+        wl, gflog, xj, e, xjp, ep, icode, clabelp, iso, loggr = f
+        if icode == 156:
+            icode = 126
+        if abs(wl) > grid.wlend + 2.0:
+            break
+        de = abs(abs(ep) - abs(e))
+        wlvac = 1.0e7 / de if de != 0.0 else math.inf
+        if wlvac < grid.wlbeg - 1.0 or wlvac > grid.wlend + 1.0:
+            continue
+        nel, x1, x2 = mol_dispatch(iso, icode)
+        if nel == 0:
+            continue
+        s4 = np.float32(np.float32(np.float32(gflog + x1) + x2) + np.float32(0.0))
+        gf = math.exp(float(s4) * 2.30258509299405)
+        elo = min(abs(e), abs(ep))
+        gammar = 2.223e13 / (wlvac * wlvac) if loggr == 0 else math.pow(10.0, loggr * 0.01)
+        gammas, gammaw = _ascii_widths(clabelp, icode, xj, xjp, e, ep, broad)
+        rows.append(_lte_row(wlvac, grid, gf, elo, gammar, gammas, gammaw, nel, icode,
+                             0.026538 / 1.77245))
+    return np.array(rows, LTE_DTYPE) if rows else np.zeros(0, LTE_DTYPE)
+
+
+def poly_nelion(icode: int) -> int:
+    """poly_nelion (:2018-2027): CaOH (10820) -> slot 636, else 0."""
+    return 636 if icode == 10820 else 0
+
+
+def molec_poly_to_lte(lines, grid: Grid, broad: dict | None = None) -> np.ndarray:
+    """read_molec_poly (:1690-1838): as :func:`molec_ascii_to_lte` with a
+    6-digit molecule code, a 256-character line, the species from
+    :func:`poly_nelion` and gf = exp(log gf ln 10) (no isotope terms)."""
+    broad = broad or {}
+    rows = []
+    for raw in lines:
+        f = _mol_line(raw, 256, 6)
+        if f is None:
+            continue
+        wl, gflog, xj, e, xjp, ep, icode, clabelp, iso, loggr = f
+        if abs(wl) > grid.wlend + 2.0:
+            break
+        de = abs(abs(ep) - abs(e))
+        wlvac = 1.0e7 / de if de != 0.0 else math.inf
+        if wlvac < grid.wlbeg - 1.0 or wlvac > grid.wlend + 1.0:
+            continue
+        nel = poly_nelion(icode)
+        if nel == 0:
+            continue
+        gf = math.exp(float(gflog) * 2.30258509299405)
+        elo = min(abs(e), abs(ep))
+        gammar = 2.223e13 / (wlvac * wlvac) if loggr == 0 else math.pow(10.0, loggr * 0.01)
+        gammas, gammaw = _ascii_widths(clabelp, icode, xj, xjp, e, ep, broad)
+        rows.append(_lte_row(wlvac, grid, gf, elo, gammar, gammas, gammaw, nel, icode,
+                             0.026538 / 1.77245))
+    return np.array(rows, LTE_DTYPE) if rows else np.zeros(0, LTE_DTYPE)
+
+
+def _bisect_start(iwl: np.ndarray, lo: int, hi: int, bound: float) -> int | None:
+    """The readers' bisection over 1-based records lo..hi on
+    exp(IWL ln(1 + 1/2e6)) < bound; returns the last probed record (the
+    readers start there), None when no probe is made."""
+    newlimit = None
+    while hi - lo > 1:
+        newlimit = (hi + lo) // 2
+        if math.exp(int(iwl[newlimit - 1]) * _RATIOLG_R2E6) < bound:
+            lo = newlimit
+        else:
+            hi = newlimit
+    return newlimit
+
+
+def molec_poly_bin_to_lte(words: np.ndarray, grid: Grid, gwef: dict | None = None) -> np.ndarray:
+    """read_molec_poly_bin (:1864-2012) on a polyatomic binary list as int32
+    (n, 2): header (-1, code), then (IWL, int16 E_lo | int16 IGFLOG << 16)
+    ascending.  Records from the bisection's last probe up to lambda >
+    wlend + 1, those below wlbeg - 1 skipped; CGF 0.026538/1.77245
+    tablog(|IGFLOG|) / nu (REAL(4) tablog); classical gamma_rad, gamma_s
+    3e-5, gamma_w 1e-7 or the species' population-weighted width (gwef:
+    :func:`load_mol_broad`)."""
+    w = np.ascontiguousarray(words, np.int32).reshape(-1, 2)
+    gwef = gwef or {}
+    if len(w) == 0 or w[0, 0] != -1:
+        raise ValueError("molbin header missing")
+    icode = int(w[0, 1])
+    nel = poly_nelion(icode)
+    n = len(w)
+    if nel == 0 or n < 2:
+        return np.zeros(0, LTE_DTYPE)
+    iwl = w[:, 0]
+    if math.exp(int(iwl[1]) * _RATIOLG_R2E6) > grid.wlend + 1.0:
+        return np.zeros(0, LTE_DTYPE)
+    if grid.wlbeg - 1.0 > math.exp(int(iwl[n - 1]) * _RATIOLG_R2E6):
+        return np.zeros(0, LTE_DTYPE)
+    istart = _bisect_start(iwl, 2, n, grid.wlbeg - 1.0)
+    if istart is None:
+        raise ValueError("molbin list too short for the reader's bisection")
+    halves = w[:, 1].copy().view(np.int16).reshape(-1, 2)
+    tab = _tablog_f32(np.arange(32769)).astype(np.float64)
+    gw = gwef.get(icode, -1.0) if 0 <= icode <= 11000 else -1.0
+    gamw = gw if gw > 0.0 else 1.0e-7
+    rows = []
+    for i in range(istart - 1, n):
+        wlvac = math.exp(int(iwl[i]) * _RATIOLG_R2E6)
+        if wlvac > grid.wlend + 1.0:
+            break
+        if wlvac < grid.wlbeg - 1.0:
+            continue
+        ielo, igf = int(halves[i, 0]), int(halves[i, 1])
+        elo = float(np.int16(abs(ielo)))
+        rows.append(_lte_row(wlvac, grid, tab[np.int16(abs(igf))], elo, 2.223e13 / (wlvac * wlvac),
+                             3.0e-5, gamw, nel, icode, 0.026538 / 1.77245))
+    return np.array(rows, LTE_DTYPE) if rows else np.zeros(0, LTE_DTYPE)
+
+
+H2O_XISO = np.array([0.9976, 0.0004, 0.0020, 0.00001], np.float32)
+
+
+def h2o_to_lte(rec: np.ndarray, grid: Grid, gwef: dict | None = None) -> np.ndarray:
+    """read_h2o (:2034-2174) on 8-byte H2O records (workloads.H2O_DTYPE,
+    ascending IWL): nothing when the first record lies beyond wlend + 1 or
+    the last below wlbeg - 1; else the records from the bisection's last
+    probe (with no lower bound after it) up to lambda > wlend + 1.  Isotope
+    from the signs of E_lo and IGFLOG; CGF 0.01502 tablog(|IGFLOG|) / nu
+    times the isotope fraction; gamma_rad 2.223e13 / lambda^2 / 1000,
+    gamma_s tablog(1), gamma_w tablog(9384) or H2O's population-weighted
+    width (code 10108 in gwef) rounded to REAL(4); species 534, code 10108."""
+    r = np.ascontiguousarray(rec)
+    gwef = gwef or {}
+    n = len(r)
+    if n == 0:
+        return np.zeros(0, LTE_DTYPE)
+    iwl = r["iwl"]
+    if math.exp(int(iwl[0]) * _RATIOLG_R2E6) > grid.wlend + 1.0:
+        return np.zeros(0, LTE_DTYPE)
+    if grid.wlbeg - 1.0 > math.exp(int(iwl[n - 1]) * _RATIOLG_R2E6):
+        return np.zeros(0, LTE_DTYPE)
+    istart = _bisect_start(iwl, 1, n, grid.wlbeg - 1.0)
+    if istart is None:
+        raise ValueError("H2O list too short for the reader's bisection")
+    tab = _tablog_f32(np.arange(32769)).astype(np.float64)
+    gw = gwef.get(10108, -1.0)
+    gamw0 = float(np.float32(gw)) if gw > 0.0 else tab[9384]
+    rows = []
+    for i in range(istart - 1, n):
+        wlvac = math.exp(int(iwl[i]) * _RATIOLG_R2E6)
+        if wlvac > grid.wlend + 1.0:
+            break
+        ielo, igf = int(r["ielo"][i]), int(r["igflog"][i])
+        iso = (1 if igf > 0 else 2) if ielo > 0 else (3 if igf > 0 else 4)
+        ixwl = math.floor(math.log(wlvac) / grid.ratiolg + 0.5)
+        freq = 2.99792458e17 / wlvac
+        congf = 0.01502 * tab[np.int16(abs(igf))] / freq * float(H2O_XISO[iso - 1])
+        frq4pi = freq * 12.5664
+        gammar = 2.223e13 / (wlvac * wlvac) * 0.001
+        rows.append((ixwl - grid.ixwlbeg + 1, np.float32(congf), 534,
+                     np.float32(np.int16(abs(ielo))), np.float32(gammar / frq4pi),
+                     np.float32(tab[1] / frq4pi), np.float32(gamw0 * 1.0 / frq4pi),
+                     np.float32(10108.0)))
+    return np.array(rows, LTE_DTYPE) if rows else np.zeros(0, LTE_DTYPE)

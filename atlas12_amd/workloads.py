@@ -407,55 +407,113 @@ DIATOMIC_LISTS = {"ch.asc": ((106, 12), (106, 13)), "oh.asc": ((108, 16), (108, 
                   "alo.asc": ((813, 16),)}
 
 
+def _kurucz_mol_text(rng, n: int, wl0: float, wl1: float, species, code_w: int, loggf,
+                     elo_ev: float, dup_share: float) -> str:
+    """One Kurucz-format molecular list (see diatomic_ascii_lists), the
+    molecule code in a field code_w wide."""
+    lam = np.exp(rng.uniform(math.log(wl0), math.log(wl1), n))
+    elo = np.round(rng.exponential(elo_ev, n) * EV_TO_CM, 3)
+    eup = np.round(elo + 1.0e7 / lam, 3)
+    dup = np.nonzero(rng.random(n) < dup_share)[0]
+    dup = dup[dup > 0]
+    elo[dup], eup[dup] = elo[dup - 1], eup[dup - 1]
+    wl = 1.0e7 / np.abs(eup - elo)
+    order = np.argsort(wl, kind="stable")
+    sp = rng.integers(0, len(species), n)
+    gf = rng.normal(loggf[0], loggf[1], n)
+    swap = rng.random(n) < 0.3
+    neg = rng.random(n) < 0.1
+    xj = rng.integers(0, 80, n) + 0.5
+    cf = "{:%dd}" % code_w
+    lines = []
+    for k, i in enumerate(order):
+        icode, iso = species[sp[i]]
+        e, ep = (eup[i], elo[i]) if swap[i] else (elo[i], eup[i])
+        if neg[i]:
+            ep = -ep
+        xjp = xj[i] + rng.integers(-1, 2)
+        labp = ("X" if rng.random() < 0.33 else "A") + f"{int(rng.integers(0, 99)):02d}e"
+        loggr = 0 if rng.random() < 0.25 else int(rng.integers(700, 900))
+        lines.append(f"{wl[i]:10.4f}{gf[i]:7.3f}{xj[i]:5.1f}{e:10.3f}{max(xjp, 0.5):5.1f}"
+                     f"{ep:11.3f}" + cf.format(icode) + f"{'A01e':8s}{labp:8s}{iso:2d}{loggr:4d}")
+        if k == n // 3:
+            lines.append("")
+        if k == n // 2:
+            lines.append("  bad line" + "?" * 60)
+    lines.append(f"{wl1 * 1.5:10.4f}{-1.0:7.3f}{0.5:5.1f}{100.0:10.3f}{1.5:5.1f}"
+                 f"{100.0 + 1.0e7 / wl1:11.3f}" + cf.format(species[0][0]) +
+                 f"{'A':8s}{'X':8s}{species[0][1]:2d}{0:4d}")
+    lines.append(lines[0])
+    return "\n".join(lines) + "\n"
+
+
+# the SYNTHE molecular lists of the reader fixture: FeH listed under code 156
+# (read as 126) and AlO, which SYNTHE keeps and ATLAS12 does not
+SYNTHE_MOL_LISTS = {"ch.asc": ((106, 12), (106, 13)), "co.asc": ((608, 12), (608, 13)),
+                    "feh.asc": ((156, 56), (126, 57)), "alo.asc": ((813, 16), (813, 17)),
+                    "cn.asc": ((607, 12), (607, 15), (607, 99))}
+
+
 def diatomic_ascii_lists(n: int, wl0: float, wl1: float, seed: int, loggf=(-2.0, 1.5),
-                         elo_ev: float = 0.6, dup_share: float = 0.05) -> dict:
+                         elo_ev: float = 0.6, dup_share: float = 0.05,
+                         lists: dict | None = None) -> dict:
     """Seeded Kurucz-format ASCII molecular lists (columns F10.4, F7.3, F5.1,
     F10.3, F5.1, F11.3, I4, A8, A8, I2, I4: lambda, log gf, J, E, J', E',
-    code, labels, isotope, 100 log gamma_rad), n lines per file of
-    DIATOMIC_LISTS, each ascending in lambda.  Terms listed in either order,
-    some with negative (predicted) energies; one in four with classical
-    damping; 'X' upper labels for a third.  dup_share of the lines repeat a
-    neighbour's terms (equal wavelengths: ties for the wavelength sort).
-    Each file also holds a blank line, a line the format rejects, and ends
-    with a line beyond the window and one after it that must not be read.
-    Returns {file name: text}."""
+    code, labels, isotope, 100 log gamma_rad), n lines per file of lists
+    (default DIATOMIC_LISTS: file -> (code, isotope) pairs), each ascending
+    in lambda.  Terms listed in either order, some with negative (predicted)
+    energies; one in four with classical damping; 'X' upper labels for a
+    third.  dup_share of the lines repeat a neighbour's terms (equal
+    wavelengths: ties for the wavelength sort).  Each file also holds a
+    blank line, a line the format rejects, and ends with a line beyond the
+    window and one after it that must not be read.  Returns {file: text}."""
     rng = np.random.default_rng(seed)
-    out = {}
-    for fname, species in DIATOMIC_LISTS.items():
-        lam = np.exp(rng.uniform(math.log(wl0), math.log(wl1), n))
-        elo = np.round(rng.exponential(elo_ev, n) * EV_TO_CM, 3)
-        eup = np.round(elo + 1.0e7 / lam, 3)
-        dup = np.nonzero(rng.random(n) < dup_share)[0]
-        dup = dup[dup > 0]
-        elo[dup], eup[dup] = elo[dup - 1], eup[dup - 1]
-        wl = 1.0e7 / np.abs(eup - elo)
-        order = np.argsort(wl, kind="stable")
-        sp = rng.integers(0, len(species), n)
-        gf = rng.normal(loggf[0], loggf[1], n)
-        swap = rng.random(n) < 0.3
-        neg = rng.random(n) < 0.1
-        xj = rng.integers(0, 80, n) + 0.5
-        lines = []
-        for k, i in enumerate(order):
-            icode, iso = species[sp[i]]
-            e, ep = (eup[i], elo[i]) if swap[i] else (elo[i], eup[i])
-            if neg[i]:
-                ep = -ep
-            xjp = xj[i] + rng.integers(-1, 2)
-            labp = ("X" if rng.random() < 0.33 else "A") + f"{int(rng.integers(0, 99)):02d}e"
-            loggr = 0 if rng.random() < 0.25 else int(rng.integers(700, 900))
-            lines.append(f"{wl[i]:10.4f}{gf[i]:7.3f}{xj[i]:5.1f}{e:10.3f}{max(xjp, 0.5):5.1f}"
-                         f"{ep:11.3f}{icode:4d}{'A01e':8s}{labp:8s}{iso:2d}{loggr:4d}")
-            if k == n // 3:
-                lines.append("")
-            if k == n // 2:
-                lines.append("  bad line" + "?" * 60)
-        lines.append(f"{wl1 * 1.5:10.4f}{-1.0:7.3f}{0.5:5.1f}{100.0:10.3f}{1.5:5.1f}"
-                     f"{100.0 + 1.0e7 / wl1:11.3f}{species[0][0]:4d}{'A':8s}{'X':8s}"
-                     f"{species[0][1]:2d}{0:4d}")
-        lines.append(lines[0])
-        out[fname] = "\n".join(lines) + "\n"
+    return {f: _kurucz_mol_text(rng, n, wl0, wl1, sp, 4, loggf, elo_ev, dup_share)
+            for f, sp in (lists or DIATOMIC_LISTS).items()}
+
+
+def polymol_ascii_list(n: int, wl0: float, wl1: float, seed: int, loggf=(-2.0, 1.5)) -> str:
+    """A seeded polyatomic ASCII list (the molecular columns with a 6-digit
+    code, I6): CaOH (10820) and one record in four of an unlisted code."""
+    rng = np.random.default_rng(seed)
+    return _kurucz_mol_text(rng, n, wl0, wl1, ((10820, 1), (10820, 2), (10820, 3), (10821, 1)),
+                            6, loggf, 0.4, 0.02)
+
+
+def polymol_bin_records(n: int, wl0: float, wl1: float, seed: int, icode: int = 10820,
+                        loggf=(-2.0, 1.5)) -> np.ndarray:
+    """A seeded polyatomic binary list, int32 (n + 1, 2): header (-1, code),
+    then (IWL, int16 E_lo | int16 IGFLOG << 16) ascending in IWL, IGFLOG
+    signs mixed (the reader takes |IGFLOG|)."""
+    rng = np.random.default_rng(seed)
+    lam = np.sort(np.exp(rng.uniform(math.log(wl0), math.log(wl1), n)))
+    out = np.zeros((n + 1, 2), np.int32)
+    out[0] = (-1, icode)
+    out[1:, 0] = np.rint(np.log(lam) / LOG_RATIO_2E6).astype(np.int32)
+    half = np.zeros((n, 2), np.int16)
+    half[:, 0] = np.clip(np.rint(rng.exponential(0.4, n) * EV_TO_CM), 0, 32767).astype(np.int16)
+    code = np.maximum(_tablog_code(rng.normal(loggf[0], loggf[1], n)).astype(np.int32), 1)
+    half[:, 1] = np.where(rng.random(n) < 0.2, -code, code).astype(np.int16)
+    out[1:, 1] = half.view(np.int32).reshape(n)
     return out
+
+
+SYNTHE_MOL_KEYS = ("seed", "n_gfall", "n_mol", "n_poly", "n_polybin", "n_h2o", "span")
+
+
+def synthe_mol_lists(seed: int, n_gfall: int, n_mol: int, n_poly: int, n_polybin: int,
+                     n_h2o: int, span):
+    """The raw lists of SYNTHE's molecular reader workload over span (nm):
+    (gfall records, {molecular ASCII file: text} of SYNTHE_MOL_LISTS,
+    polyatomic ASCII text, polyatomic binary words, H2O records); the
+    binary lists reach 20 nm beyond span on each side."""
+    w0, w1 = span
+    g = synthetic_gfall(n_gfall, w0 + 1.5, w1 - 1.5, seed=seed)
+    mol = diatomic_ascii_lists(n_mol, w0, w1, seed + 1, lists=SYNTHE_MOL_LISTS)
+    poly = polymol_ascii_list(n_poly, w0, w1, seed + 2)
+    pbin = polymol_bin_records(n_polybin, w0 - 20.0, w1 + 20.0, seed + 3)
+    h2o = h2o_records(n_h2o, w0 - 20.0, w1 + 20.0, seed + 4, loggf=(-2.0, 1.5))
+    return g, mol, poly, pbin, h2o
 
 
 def atlas_all_lists(mols, seed: int, wl_range, n_pred: int, n_lowobs: int, n_hilines: int,

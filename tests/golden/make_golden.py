@@ -1332,12 +1332,65 @@ def golden_records():
                         "lte_records": int(len(lines))}}
 
 
+RECORDS_MOL = dict(seed=81, n_gfall=200, n_mol=1500, n_poly=1500, n_polybin=3000, n_h2o=4000,
+                   window=(500.0, 505.0, 500000.0), span=(498.0, 507.0), teff=3400.0)
+
+
+def golden_records_mol():
+    """SYNTHE's remaining readers on a 5 nm window: Kurucz molecular ASCII
+    lists (read_molec_ascii; FeH under code 156, AlO, isotopes molec_dispatch
+    does not know), a polyatomic ASCII list (read_molec_poly, CaOH and an
+    unlisted code), a polyatomic binary list (read_molec_poly_bin) and an H2O
+    list (read_h2o), with a small gfall list first as run_mklinelist
+    requires; TEFF 3400 K so that the polyatomic lists are read.  The raw
+    lists are regenerated from RECORDS_MOL (stored as "config") by the
+    tests; the fixture keeps the reference's records and mol_broad.dat."""
+    c = RECORDS_MOL
+    tmp = tempfile.mkdtemp(prefix="a12m_")
+    g, mol, poly, pbin, h2o = wl.synthe_mol_lists(**{k: c[k] for k in wl.SYNTHE_MOL_KEYS})
+    gpath = os.path.join(tmp, "gfall.dat")
+    wl.write_gfall(g, gpath)
+    root = data_root(tmp, gpath)
+    d = os.path.join(root, "data")
+    for name, text in mol.items():
+        with open(os.path.join(d, name), "w") as fh:
+            fh.write(text)
+    with open(os.path.join(d, "caoh.asc"), "w") as fh:
+        fh.write(poly)
+    with open(os.path.join(d, "caohb.bin"), "wb") as fh:
+        fh.write(pbin.astype("<i4").tobytes())
+    with open(os.path.join(d, "h2o.bin"), "wb") as fh:
+        fh.write(h2o.tobytes())
+    with open(os.path.join(d, "lines.list"), "a") as fh:
+        for name in mol:
+            fh.write(f"mol {name}\n")
+        fh.write("polymol caoh.asc\npolymol caohb.bin\nh2o h2o.bin\n")
+    atm = wl.scurve_atmosphere(teff=c["teff"])
+    deck = os.path.join(tmp, "recm.dat")
+    with open(deck, "w") as fh:
+        fh.write(atm.deck())
+    dump = os.path.join(tmp, "dump.bin")
+    w = c["window"]
+    run([os.path.join(REFBIN, "ref_synthe_dump"), deck, str(w[0]), str(w[1]), str(w[2]), dump],
+        tmp, dict(os.environ, ATLAS12=root))
+    lines = read_dump(dump)["lines"]
+    broad = [ln.rstrip("\n") for ln in open(os.path.join(REF, "data", "mol_broad.dat"))]
+    shutil.rmtree(tmp)
+    digest = hashlib.sha256(g.tobytes() + "".join(mol.values()).encode() + poly.encode() +
+                            pbin.tobytes() + h2o.tobytes()).hexdigest()
+    np.savez_compressed(os.path.join(HERE, "records_mol_ref.npz"), lines=lines.view(np.uint8),
+                        mol_broad=np.array(broad), config=np.array(json.dumps(c)),
+                        raw_sha256=np.array(digest))
+    return {"records_mol": {"lte_records": int(len(lines)), "raw_sha256": digest}}
+
+
 STEPS = {"synthe_small": golden_synthe_small, "atlas_small": golden_atlas_small,
          "state_atlas": golden_state_atlas, "select": golden_select, "select_all": golden_select_all, "scale": golden_scale,
          "tcorr": golden_tcorr, "tcorr_conv": golden_tcorr_conv,
          "state_plain": golden_state_plain,
          "numerics": golden_numerics, "josh": golden_josh, "molec": golden_molec,
-         "kapp": golden_kapp, "cplx": golden_cplx, "records": golden_records}
+         "kapp": golden_kapp, "cplx": golden_cplx, "records": golden_records,
+         "records_mol": golden_records_mol}
 
 
 def main():
