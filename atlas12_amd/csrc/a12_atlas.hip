@@ -414,7 +414,8 @@ constexpr int kSumParts = 1024;
 // block's points.
 __global__ __launch_bounds__(256) void k_lo_linereach(int nl, int n,
                                                       const int *__restrict__ state,
-                                                      int *__restrict__ l_reach)
+                                                      int *__restrict__ l_reach,
+                                                      unsigned short *__restrict__ greach)
 {
     // one wavefront per line, four lines in flight: coalesced loads of each
     // line's n states (n <= 128), shuffle max
@@ -435,11 +436,20 @@ __global__ __launch_bounds__(256) void k_lo_linereach(int nl, int n,
         for (int u = 0; u < kLines; ++u) {
             int r = 0, b = 0;
 #pragma unroll
-            for (int h = 0; h < 2; ++h)
-                if (v[u][h] < 0) {
-                    r = max(r, v[u][h] & 0xff);
-                    b = max(b, (v[u][h] >> 8) & 0xff);
+            for (int h = 0; h < 2; ++h) {
+                // widest reach over each group of 8 depths (depth = lane + 64 h)
+                int rg = v[u][h] < 0 ? v[u][h] & 0xff : 0;
+                int bg = v[u][h] < 0 ? (v[u][h] >> 8) & 0xff : 0;
+                for (int o = 1; o < 8; o <<= 1) {
+                    rg = max(rg, __shfl_xor(rg, o, 64));
+                    bg = max(bg, __shfl_xor(bg, o, 64));
                 }
+                const int grp = (lane >> 3) + 8 * h;
+                if ((lane & 7) == 0 && l0 + u < nl && 8 * grp < n)
+                    greach[(size_t)grp * nl + l0 + u] = (unsigned short)(rg | (bg << 8));
+                r = max(r, rg);
+                b = max(b, bg);
+            }
             for (int o = 32; o > 0; o >>= 1) {
                 r = max(r, __shfl_xor(r, o, 64));
                 b = max(b, __shfl_xor(b, o, 64));
@@ -855,22 +865,156 @@ __global__ __launch_bounds__(kLoBlock) void k_lo_gather_w(
         }
 }
 
+// Point form of the gather: one lane per (depth, kPP output points).  A
+// wavefront covers 8 kPP grid points x 8 depths: lane = 8 * depth offset +
+// point offset p, its points p, p + 8, ...; wavefront w of the grid takes
+// point block w % npb and depth group w / npb.  It ballots the candidate
+// lines (NU within 100 points of its block) on their widest reach over its
+// 8 depths (greach, from k_lo_linereach) and walks the kept ones in store
+// order, one line ahead: the line's NU and wavelength are wave-uniform
+// scalar loads, each lane reads its depth's state and parameters (8
+// distinct records per line) and adds the items of its points that lie in
+// the reach into register sums.  Each element sees the same lines in the
+// same order with the same item values as in the other forms, so XLINES
+// has the same bits.  kRun > 0: workgroups are renumbered so that each XCD
+// (workgroup b runs on XCD b % 8) takes runs of kRun consecutive ones,
+// which puts neighbouring point blocks, and the line records they share,
+// in one L2.
+template <bool F64, int kPP, int kRun>
+__global__ __launch_bounds__(256) void k_lo_gather_p(
+    int n, int numnu, int npb, int ngroups, int nl, const int *__restrict__ seg,
+    const int *__restrict__ l_nu, const unsigned short *__restrict__ greach,
+    const int *__restrict__ state, const double *__restrict__ par,
+    const double *__restrict__ l_const, const double *__restrict__ waveset,
+    const double *__restrict__ h0, const double *__restrict__ h1, const double *__restrict__ h2,
+    const float4 *__restrict__ f, double *__restrict__ xlines)
+{
+#pragma clang fp contract(off)
+    int b = blockIdx.x;
+    if (kRun > 0) {
+        const int full = ((int)gridDim.x / (8 * kRun)) * (8 * kRun);
+        if (b < full) {
+            const int xcd = b & 7, r = b >> 3;
+            b = ((r / kRun) * 8 + xcd) * kRun + r % kRun;
+        }
+    }
+    const int lane = threadIdx.x & 63;
+    // wave-uniform values in scalar registers: the line loop's NU and
+    // wavelength loads and its record addresses are then scalar
+    const int wv = __builtin_amdgcn_readfirstlane(b * 4 + (int)(threadIdx.x >> 6));
+    if (wv >= npb * ngroups) return;  // whole wavefronts
+    const int pb = wv % npb, g = wv / npb;
+    const int wfirst = pb * 8 * kPP, wlast = min(wfirst + 8 * kPP, numnu) - 1;
+    const int iw = wfirst + (lane & 7), j = g * 8 + (lane >> 3);
+    const bool jlive = j < n;
+    const int jl = min(j, n - 1);
+    double w[kPP], sd[kPP];
+    float s[kPP], c[kPP];
+    bool live[kPP];
+#pragma unroll
+    for (int u = 0; u < kPP; ++u) {
+        live[u] = jlive && iw + 8 * u < numnu;
+        w[u] = waveset[min(iw + 8 * u, numnu - 1)];
+        sd[u] = 0.0;
+        s[u] = c[u] = 0.0f;
+    }
+    const int lbeg = __builtin_amdgcn_readfirstlane(seg[max(0, wfirst - kMaxWing)]);
+    const int lend = __builtin_amdgcn_readfirstlane(seg[min(numnu, wlast + kMaxWing) + 1]);
+    const unsigned short *gr = greach + (size_t)g * nl;
+    for (int c0 = lbeg; c0 < lend; c0 += 64) {
+        const int q = c0 + lane;
+        bool act = false;
+        if (q < lend) {
+            const int k = l_nu[q], rr = gr[q], rmax = rr & 0xff, bmax = rr >> 8;
+            act = (rmax > 0 && k <= wlast && k + rmax - 1 >= wfirst) ||
+                  (bmax > 0 && k - 1 >= wfirst && k - bmax <= wlast);
+        }
+        unsigned long long m = __ballot(act);
+        if (!m) continue;
+        // one line ahead: the next line's state, parameters, NU and
+        // wavelength are loaded while the current one is added
+        int row = __builtin_amdgcn_readfirstlane(c0 + __builtin_ctzll(m));  // lowest first
+        m &= m - 1;
+        const int *st_r = state + (size_t)row * n;
+        const double *pr = par + (size_t)3 * row * n;
+        int st = st_r[jl];
+        double p0 = pr[3 * jl], p1 = pr[3 * jl + 1], p2 = pr[3 * jl + 2];
+        int k = l_nu[row];
+        double wl = l_const[5 * (size_t)row];
+        for (;;) {
+            const bool more = m != 0;
+            int st_n = 0, k_n = 0;
+            double q0 = 0.0, q1 = 0.0, q2 = 0.0, wl_n = 0.0;
+            if (more) {
+                const int row_n = __builtin_amdgcn_readfirstlane(c0 + __builtin_ctzll(m));
+                m &= m - 1;
+                const int *st_rn = state + (size_t)row_n * n;
+                const double *prn = par + (size_t)3 * row_n * n;
+                st_n = st_rn[jl];
+                q0 = prn[3 * jl];
+                q1 = prn[3 * jl + 1];
+                q2 = prn[3 * jl + 2];
+                k_n = l_nu[row_n];
+                wl_n = l_const[5 * (size_t)row_n];
+            }
+            if (st < 0) {
+                // an item where -bn <= iw - NU < rn (red 0 .. rn - 1, blue
+                // -1 .. -bn)
+                const int rn = st & 0xff, bn = (st >> 8) & 0xff;
+                const LoPar qp = {p0, p1, p2};
+#pragma unroll
+                for (int u = 0; u < kPP; ++u) {
+                    if (live[u] && (unsigned)(iw + 8 * u - k + bn) < (unsigned)(rn + bn)) {
+                        // red: w - wl > 0; blue: wl - w = -(w - wl) exactly
+                        const double d = fabs(w[u] - wl);
+                        if (F64) {
+                            sd[u] = sd[u] + lo_cv<double>(qp.p0, d / qp.p2, qp.p1, h0, h1, h2);
+                        } else {
+                            const float x = lo_item_mixed(qp, d, f);
+                            const float y = x - c[u], tt = s[u] + y;
+                            c[u] = (tt - s[u]) - y;
+                            s[u] = tt;
+                        }
+                    }
+                }
+            }
+            if (!more) break;
+            st = st_n;
+            p0 = q0;
+            p1 = q1;
+            p2 = q2;
+            k = k_n;
+            wl = wl_n;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kPP; ++u)
+        if (live[u])
+            xlines[(size_t)(iw + 8 * u) * n + j] = F64 ? sd[u] : (double)s[u] - (double)c[u];
+}
+
 // Tile shapes (points per tile, lanes per block, lines per LDS batch, form:
 // 0 staged, 1 flat, 2 barrier-free per wave) the autotuner chooses from;
 // LDS = 8 B x points x lanes (+ 4 B x 128 x batch when staged).
+using LoTileFn = void (*)(int, int, int, const int *, const int *, const int *, const int *,
+                         const double *, const double *, const double *, const double *,
+                         const double *, const double *, const float4 *, double *);
+using LoPointFn = void (*)(int, int, int, int, int, const int *, const int *,
+                           const unsigned short *, const int *, const double *, const double *,
+                           const double *, const double *, const double *, const double *,
+                           const float4 *, double *);
 struct LoShape {
     int pts, block, batch;
-    int flat;
-    void (*mixed)(int, int, int, const int *, const int *, const int *, const int *,
-                  const double *, const double *, const double *, const double *, const double *,
-                  const double *, const float4 *, double *);
-    void (*fp64)(int, int, int, const int *, const int *, const int *, const int *,
-                 const double *, const double *, const double *, const double *, const double *,
-                 const double *, const float4 *, double *);
+    int flat;  // form: 0 staged, 1 flat, 2 per wave, 3 point (batch: XCD run length)
+    LoTileFn mixed, fp64;
+    LoPointFn pmixed, pfp64;
 };
 #define A12_LO_SHAPE(P, B, T, F) \
-    {P, B, T, F ? 1 : 0, k_lo_gather<false, P, B, T, F>, k_lo_gather<true, P, B, T, F>}
-#define A12_LO_WAVE(P, B) {P, B, 0, 2, k_lo_gather_w<false, P, B>, k_lo_gather_w<true, P, B>}
+    {P, B, T, F ? 1 : 0, k_lo_gather<false, P, B, T, F>, k_lo_gather<true, P, B, T, F>, nullptr, nullptr}
+#define A12_LO_WAVE(P, B) \
+    {P, B, 0, 2, k_lo_gather_w<false, P, B>, k_lo_gather_w<true, P, B>, nullptr, nullptr}
+#define A12_LO_POINT(P, R) \
+    {8 * P, 256, R, 3, nullptr, nullptr, k_lo_gather_p<false, P, R>, k_lo_gather_p<true, P, R>}
 const LoShape kLoShapes[] = {
     A12_LO_SHAPE(8, 512, 32, false),  A12_LO_SHAPE(4, 512, 64, false),
     A12_LO_SHAPE(4, 256, 32, false),  A12_LO_SHAPE(8, 256, 32, false),
@@ -879,9 +1023,13 @@ const LoShape kLoShapes[] = {
     A12_LO_SHAPE(2, 256, 16, false),  A12_LO_SHAPE(8, 256, 16, false),
     A12_LO_WAVE(4, 256),              A12_LO_WAVE(2, 256),
     A12_LO_WAVE(8, 256),              A12_LO_WAVE(4, 512),
+    A12_LO_POINT(1, 0),               A12_LO_POINT(1, 16),
+    A12_LO_POINT(2, 0),               A12_LO_POINT(2, 16),
+    A12_LO_POINT(4, 0),
 };
 #undef A12_LO_SHAPE
 #undef A12_LO_WAVE
+#undef A12_LO_POINT
 constexpr int kNumLoShapes = sizeof(kLoShapes) / sizeof(kLoShapes[0]);
 
 template <class T>
@@ -1122,6 +1270,18 @@ static int lo_gather_launch(a12_ctx *ctx, int prec, int shape)
     const double *tabs = A.tabs.as<double>();
     const float4 *tabs32 = A.tabs32.as<float4>();
     const LoShape &sh = kLoShapes[shape];
+    if (sh.flat == 3) {
+        auto g = prec == A12_PREC_FP64 ? sh.pfp64 : sh.pmixed;
+        const int npb = (numnu + sh.pts - 1) / sh.pts, ng = (n + 7) / 8;
+        const long long waves = (long long)npb * ng;
+        hipLaunchKernelGGL(g, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, ctx->stream, n,
+                           numnu, npb, ng, A.nl, A.seg.as<int>(), A.l_nu.as<int>(),
+                           A.greach.as<unsigned short>(), A.state.as<int>(), A.par.as<double>(),
+                           A.l_const.as<double>(), A.waveset.as<double>(), tabs, tabs + kTabN,
+                           tabs + 2 * kTabN, tabs32, A.xlines.as<double>());
+        A12_CHECK_LAUNCH();
+        return A12_OK;
+    }
     auto g = prec == A12_PREC_FP64 ? sh.fp64 : sh.mixed;
     const int tiles = (numnu + sh.pts - 1) / sh.pts, per = sh.block / n;
     if (per < 1) {
@@ -1154,6 +1314,7 @@ int a12i_atlas_linop1_run(a12_ctx *ctx, int prec)
     A12_HIP(A.state.reserve(sizeof(int) * nl1 * n));
     A12_HIP(A.par.reserve(sizeof(double) * 3 * nl1 * n));
     A12_HIP(A.reach.reserve(sizeof(int) * nl1));
+    A12_HIP(A.greach.reserve(sizeof(unsigned short) * nl1 * ((n + 7) / 8)));
     A12_HIP(A.xlines.reserve(sizeof(double) * (size_t)numnu * n));
     const int nb = (int)((nl1 * n + 255) / 256);  // one lane per (line, depth)
     A12_HIP(A.blk.reserve(sizeof(long long) * (nb + kSumParts)));
@@ -1183,7 +1344,8 @@ int a12i_atlas_linop1_run(a12_ctx *ctx, int prec)
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[9], s));
     if (nl > 0) {
         hipLaunchKernelGGL(k_lo_linereach, dim3(std::min((nl + 3) / 4, 65536)), dim3(256), 0, s,
-                           nl, n, A.state.as<int>(), A.reach.as<int>());
+                           nl, n, A.state.as<int>(), A.reach.as<int>(),
+                           A.greach.as<unsigned short>());
         A12_CHECK_LAUNCH();
     }
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[10], s));

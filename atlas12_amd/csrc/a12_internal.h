@@ -67,7 +67,7 @@ struct AtlasDev {
     float lg2inv = 0;  // 1 / log2 of WAVESET's mean point-to-point ratio
     DevBuf depth3, depth_lo, tabcont, xnfdop, dopple, waveset, rcoset, tabs, tabs32;
     DevBuf l_nu, l_nucont, l_nel, l_const, seg;
-    DevBuf state, par, reach, xlines, blk, evals;
+    DevBuf state, par, reach, greach, xlines, blk, evals;
     DevBuf kapp, hnu1, acc;
     // the loaded store's grid (set_state checks a state against it) and the
     // host staging of set_state

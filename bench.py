@@ -127,7 +127,7 @@ def atlas_run(args, ctx, native, z, d, pred, tio):
     native.atlas_kapp_run(ctx)
     shapes = native.atlas_shapes(ctx)
     best, tune_ms = native.atlas_autotune(ctx, native.PREC_MIXED)
-    log(f"atlas: autotuned gather shape {shapes[best]} (pts, lanes, batch, form 0 staged / 1 flat "
+    log(f"atlas: autotuned gather shape {shapes[best]} (pts, lanes, batch, form 0 staged / 1 flat / 3 point "
         "/ 2 per wave); ms per shape "
         + ", ".join(f"{s}:{m:.1f}" for s, m in zip(shapes, tune_ms)))
     # TCORR(3)'s structure columns for this deck's first iteration (the
@@ -721,7 +721,8 @@ def main():
 
     traffic = traffic_table()
     pts, lanes, batch, flat = a["shape"]
-    gather_name = (f"k_lo_gather_w<false, {pts}, {lanes}>" if flat == 2 else
+    gather_name = (f"k_lo_gather_p<false, {pts // 8}, {batch}>" if flat == 3 else
+                   f"k_lo_gather_w<false, {pts}, {lanes}>" if flat == 2 else
                    f"k_lo_gather<false, {pts}, {lanes}, {batch}, {'true' if flat else 'false'}>")
     ev = a["ev_mix"]
     value = ev / (a["lo_mix"] * 1e-3)
