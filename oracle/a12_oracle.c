@@ -738,9 +738,11 @@ static int rt_edge(const or_synthe_rt *rt, double wave)
     return ie;
 }
 
-void or_synthe_transfer_point(const or_synthe_rt *rt, int iwave,
-                              const float *opac_col, const double *opac_col64,
-                              double *h_line, double *h_cont)
+/* The transfer of one wavelength point; taunu (nullable, nrhox): TAUNU of
+ * the second JOSH call, the table synthe.f90 writes to .linform (:889-890). */
+static void synthe_point(const or_synthe_rt *rt, int iwave, const float *opac_col,
+                         const double *opac_col64, double *h_line, double *h_cont,
+                         double *taunu)
 {
     const int n = rt->nrhox;
     double acont[KW_MAX], sigmac[KW_MAX], aline[KW_MAX], sigmal[KW_MAX];
@@ -791,10 +793,33 @@ void or_synthe_transfer_point(const or_synthe_rt *rt, int iwave,
         sline[j] = bnu[j] * stim[j] / (1.0 - ehvkt[j]);
         sigmal[j] = asyn[j] * 0.0;
     }
+    io.taunu = taunu;
     double hl = or_josh(&io, 1, 1);
     double resid = hl / surf;
     *h_line = resid * surf;
     *h_cont = surf;
+}
+
+void or_synthe_transfer_point(const or_synthe_rt *rt, int iwave,
+                              const float *opac_col, const double *opac_col64,
+                              double *h_line, double *h_cont)
+{
+    synthe_point(rt, iwave, opac_col, opac_col64, h_line, h_cont, NULL);
+}
+
+void or_synthe_taunu(const or_synthe_rt *rt, const float *opac, const double *opac64,
+                     double *taunu, int i0, int i1)
+{
+    const int n = rt->nrhox, L = rt->length;
+    float col[KW_MAX];
+    double col64[KW_MAX], hl, hc;
+    for (int i = i0; i < i1; ++i) {
+        for (int j = 0; j < n; ++j) {
+            if (opac64) col64[j] = opac64[(size_t)j * L + i];
+            else col[j] = opac[(size_t)j * L + i];
+        }
+        synthe_point(rt, i, col, opac64 ? col64 : NULL, &hl, &hc, taunu + (size_t)(i - i0) * n);
+    }
 }
 
 void or_synthe_transfer(const or_synthe_rt *rt, const float *opac,

@@ -138,6 +138,10 @@ typedef struct or_synthe_rt_s {
 void or_synthe_transfer_point(const or_synthe_rt *rt, int iwave,
                               const float *opac_col, const double *opac_col64,
                               double *h_line, double *h_cont);
+/* TAUNU (i1 - i0, nrhox) of the line-included JOSH call at wavelength points
+ * [i0, i1) (src/synthe.f90:886-890, the .linform table). */
+void or_synthe_taunu(const or_synthe_rt *rt, const float *opac, const double *opac64,
+                     double *taunu, int i0, int i1);
 void or_synthe_transfer(const or_synthe_rt *rt, const float *opac,
                         const double *opac64, double *h_line, double *h_cont,
                         int i0, int i1);

@@ -280,6 +280,23 @@ def synthe_transfer(st, opac, coefj, coefh, i0=0, i1=None, threads=1):
     return hl, hc
 
 
+def synthe_taunu(st, opac, coefj, coefh, i0=0, i1=None):
+    """TAUNU (i1 - i0, nrhox) of the line-included JOSH call at wavelength
+    points [i0, i1), the table synthe.exe writes to .linform, from the
+    opacity matrix (nrhox, length) FP32 (reference) or FP64."""
+    i1 = st.length if i1 is None else i1
+    r, keep = _rt(st, coefj, coefh)
+    out = np.zeros((i1 - i0, st.nrhox))
+    opac = np.ascontiguousarray(opac)
+    if opac.dtype == np.float32:
+        args = (_p(opac, P_F), None)
+    else:
+        opac = _f64(opac)
+        args = (None, _p(opac))
+    lib().or_synthe_taunu(C.byref(r), *args, _p(out), C.c_int(i0), C.c_int(i1))
+    return out
+
+
 # ---------------------------------------------------------------- ATLAS12
 class OrAtlasLinesIn(C.Structure):
     _fields_ = [("nrhox", C.c_int), ("numnu", C.c_int), ("nlines", C.c_int), ("mion", C.c_int),

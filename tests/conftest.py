@@ -31,7 +31,8 @@ def small_state():
 def small_ref():
     from atlas12_amd.workloads import LTE_DTYPE
     z = np.load(os.path.join(GOLDEN, "synthe_small_ref.npz"))
-    return {"lines": z["lines"].view(LTE_DTYPE), "spec": z["spec"], "gfall": z["gfall"]}
+    return {"lines": z["lines"].view(LTE_DTYPE), "spec": z["spec"], "gfall": z["gfall"],
+            "taunu": z["linform_taunu"]}
 
 
 @pytest.fixture(scope="session")

@@ -137,8 +137,31 @@ def run(cmd, cwd, env=None):
     return r
 
 
+def read_linform(path: str, nrhox: int):
+    """synthe.exe's .linform (more_output=yes, src/synthe.f90:888-890): per
+    wavelength point a record (F12.4, 1P2E12.4) wave, HNU(1), surface flux,
+    then TAUNU(1..nrhox) ten per line (10E12.4).  Returns (wave, hnu1,
+    surf, taunu (points, nrhox))."""
+    per = 1 + (nrhox + 9) // 10
+    with open(path) as fh:
+        lines = fh.read().splitlines()
+    assert len(lines) % per == 0, (len(lines), per)
+    npt = len(lines) // per
+    wave, h1, surf = np.zeros(npt), np.zeros(npt), np.zeros(npt)
+    tau = np.zeros((npt, nrhox))
+    for i in range(npt):
+        head = lines[i * per]
+        wave[i], h1[i], surf[i] = float(head[0:12]), float(head[12:24]), float(head[24:36])
+        vals = []
+        for ln in lines[i * per + 1:(i + 1) * per]:
+            vals += [float(ln[k:k + 12]) for k in range(0, len(ln), 12) if ln[k:k + 12].strip()]
+        tau[i] = vals[:nrhox]
+    return wave, h1, surf, tau
+
+
 def synthe_case(name: str, atm: wl.Atmosphere, wlbeg: float, wlend: float, resolu: float,
-                nlines: int, seed: int, line_span=None, keep_spec=True, **gkw) -> dict:
+                nlines: int, seed: int, line_span=None, keep_spec=True, linform=False,
+                **gkw) -> dict:
     tmp = tempfile.mkdtemp(prefix="a12g_")
     span = line_span or (wlbeg - 0.5, wlend + 0.5)
     rec = wl.synthetic_gfall(nlines, span[0], span[1], seed, **gkw)
@@ -158,6 +181,14 @@ def synthe_case(name: str, atm: wl.Atmosphere, wlbeg: float, wlend: float, resol
         run([os.path.join(REFBIN, "synthe.exe"), deck, f"wlbeg={wlbeg}", f"wlend={wlend}",
              f"resolu={resolu}"], tmp, env)
         out["spec"] = read_spec(os.path.join(tmp, f"{name}.spec"))
+    if linform:
+        # the same synthesis with the per-depth optical-depth table
+        os.rename(os.path.join(tmp, f"{name}.spec"), os.path.join(tmp, f"{name}.spec0"))
+        run([os.path.join(REFBIN, "synthe.exe"), deck, f"wlbeg={wlbeg}", f"wlend={wlend}",
+             f"resolu={resolu}", "more_output=yes"], tmp, env)
+        spec2 = read_spec(os.path.join(tmp, f"{name}.spec"))
+        assert np.array_equal(spec2, out["spec"]), "more_output changed the spectrum"
+        out["linform"] = read_linform(os.path.join(tmp, f"{name}.linform"), d["nrhox"])
     shutil.rmtree(tmp)
     return out
 
@@ -168,13 +199,15 @@ def golden_synthe_small():
     (off-grid centres) and a few strong resonance-like lines."""
     atm = wl.scurve_atmosphere()
     c = synthe_case("synsmall", atm, 500.0, 505.0, 500000.0, 4000, seed=11,
-                    line_span=(499.5, 505.5))
+                    line_span=(499.5, 505.5), linform=True)
     d = trim_edges(c["dump"])
     save_state(os.path.join(HERE, "synthe_small_state.npz"), d)
     grid = wl.Grid.make(500.0, 505.0, 500000.0)
     np.savez_compressed(os.path.join(HERE, "synthe_small_ref.npz"),
                         lines=d["lines"].view(np.uint8), spec=c["spec"],
-                        gfall=c["gfall"], grid_length=grid.length)
+                        gfall=c["gfall"], grid_length=grid.length,
+                        linform_wave=c["linform"][0], linform_hnu1=c["linform"][1],
+                        linform_surf=c["linform"][2], linform_taunu=c["linform"][3])
     np.savez_compressed(os.path.join(HERE, "josh_coef.npz"), coefj=d["coefj"],
                         coefh=d["coefh"])
     return {"synthe_small": {"nlines": int(d["nlines"]), "length": d["length"],

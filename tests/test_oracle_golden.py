@@ -69,6 +69,33 @@ def test_synthe_spectrum_every_printed_digit(orc, small_state, small_ref, coef):
     assert np.array_equal(e156(hc), spec[:, 2])
 
 
+def _e124(a):
+    """Values as synthe.exe prints them in .linform (1PE12.4)."""
+    return np.array([float("%.4E" % v) for v in np.ravel(a)]).reshape(np.shape(a))
+
+
+def test_synthe_opacity_every_depth_pins_reference_taunu(orc, small_state, small_ref, coef):
+    """The reference's per-depth optical depths (synthe.exe more_output=yes:
+    TAUNU of every depth at every wavelength point, .linform, src/synthe.f90:
+    888-890) pin the line opacity matrix depth by depth, not only through
+    the emergent flux: the oracle's opacity (reference arithmetic) through
+    the oracle's transfer reproduces all 4,975 x 72 printed values.  A
+    1e-3 scaling of the opacity or one depth's opacity dropped changes them
+    (planted in the oracle's output)."""
+    op, _ = orc.synthe_line_opacity(small_state, small_ref["lines"], "ref")
+    ref = small_ref["taunu"]
+    assert ref.shape == (small_state.length, small_state.nrhox)
+    assert np.array_equal(_e124(orc.synthe_taunu(small_state, op, *coef)), ref)
+    scaled = (op * np.float32(1.001)).astype(np.float32)
+    assert np.mean(_e124(orc.synthe_taunu(small_state, scaled, *coef)) != ref) > 0.9
+    dropped = op.copy()
+    dropped[30] = 0.0
+    bad = _e124(orc.synthe_taunu(small_state, dropped, *coef)) != ref
+    # TAUNU integrates from the surface with parabolic coefficients that
+    # reach one depth ahead: depths above 29 are untouched
+    assert bad[:, 30:].mean() > 0.5 and not bad[:, :29].any()
+
+
 def test_record_prep_bitwise(small_ref):
     """gfall -> lte_line_t record preparation (read_gfall restated in
     atlas12_amd.workloads) equals the reference reader's records."""
