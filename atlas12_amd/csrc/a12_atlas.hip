@@ -34,8 +34,9 @@
 #include "a12_common.h"
 #include "a12_internal.h"
 
-// Wavefront sum of the device library (DPP).
+// Wavefront sum and maximum of the device library (DPP).
 extern "C" __device__ long long __ockl_wfred_add_i64(long long);
+extern "C" __device__ int __ockl_wfred_max_i32(int);
 
 namespace {
 
@@ -409,53 +410,61 @@ __global__ void k_lo_sum(const long long *__restrict__ blk, int nb, long long *_
 constexpr int kSumParts = 1024;
 
 // Widest red and blue reach of each line over its depths (0 when no depth
-// passed the centre tests): l_reach[l] = red | blue << 8.  Lets the gather
-// skip, before staging anything, the candidate lines that cannot reach a
-// block's points.
+// passed the centre tests): l_reach[l] = red | blue << 8, and over each
+// group of 8 depths: greach[g * nl + l], same packing.  Lets the gathers
+// skip the candidate lines that cannot reach a block's points (at any
+// depth, or at the group's depths).
 __global__ __launch_bounds__(256) void k_lo_linereach(int nl, int n,
                                                       const int *__restrict__ state,
                                                       int *__restrict__ l_reach,
                                                       unsigned short *__restrict__ greach)
 {
-    // one wavefront per line, four lines in flight: coalesced loads of each
-    // line's n states (n <= 128), shuffle max
-    constexpr int kLines = 4;
-    const int lane = threadIdx.x & 63;
-    const int nw = gridDim.x * (blockDim.x >> 6);
-    for (int l0 = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * kLines; l0 < nl;
-         l0 += nw * kLines) {
-        int v[kLines][2];
+    // a block takes 64 consecutive lines, each wavefront 16 of them, all in
+    // flight: coalesced loads of each line's n states (n <= 128), shuffle
+    // maxima per group of 8 depths and over the line; the group maxima are
+    // gathered in LDS and stored group by group (64 lines = 128 contiguous
+    // bytes per group)
+    constexpr int kLines = 16;
+    __shared__ unsigned short s_g[16][64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int L0 = blockIdx.x * 64, ng = (n + 7) / 8;
+    const int li0 = wid * kLines;  // first line of this wavefront in the tile
+    int v[kLines][2];
 #pragma unroll
-        for (int u = 0; u < kLines; ++u) {
-            const int l = l0 + u;
-            const int *st = state + (size_t)l * n;
-            v[u][0] = l < nl && lane < n ? st[lane] : 0;
-            v[u][1] = l < nl && lane + 64 < n ? st[lane + 64] : 0;
+    for (int u = 0; u < kLines; ++u) {
+        const int l = L0 + li0 + u;
+        const int *st = state + (size_t)l * n;
+        v[u][0] = l < nl && lane < n ? st[lane] : 0;
+        v[u][1] = l < nl && lane + 64 < n ? st[lane + 64] : 0;
+    }
+    // maximum over the 8 lanes of a group with DPP: quad exchanges (lane ^ 1,
+    // lane ^ 2), then the half-row mirror (lane ^ 7 within 8 lanes)
+    auto gmax8 = [](int x) {
+        x = max(x, __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false));
+        x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false));
+        return max(x, __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false));
+    };
+#pragma unroll
+    for (int u = 0; u < kLines; ++u) {
+        int r = 0, b = 0;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            // widest reach over each group of 8 depths (depth = lane + 64 h)
+            const int rg = gmax8(v[u][h] < 0 ? v[u][h] & 0xff : 0);
+            const int bg = gmax8(v[u][h] < 0 ? (v[u][h] >> 8) & 0xff : 0);
+            const int grp = (lane >> 3) + 8 * h;
+            if ((lane & 7) == 0) s_g[grp][li0 + u] = (unsigned short)(rg | (bg << 8));
+            r = max(r, rg);
+            b = max(b, bg);
         }
-#pragma unroll
-        for (int u = 0; u < kLines; ++u) {
-            int r = 0, b = 0;
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                // widest reach over each group of 8 depths (depth = lane + 64 h)
-                int rg = v[u][h] < 0 ? v[u][h] & 0xff : 0;
-                int bg = v[u][h] < 0 ? (v[u][h] >> 8) & 0xff : 0;
-                for (int o = 1; o < 8; o <<= 1) {
-                    rg = max(rg, __shfl_xor(rg, o, 64));
-                    bg = max(bg, __shfl_xor(bg, o, 64));
-                }
-                const int grp = (lane >> 3) + 8 * h;
-                if ((lane & 7) == 0 && l0 + u < nl && 8 * grp < n)
-                    greach[(size_t)grp * nl + l0 + u] = (unsigned short)(rg | (bg << 8));
-                r = max(r, rg);
-                b = max(b, bg);
-            }
-            for (int o = 32; o > 0; o >>= 1) {
-                r = max(r, __shfl_xor(r, o, 64));
-                b = max(b, __shfl_xor(b, o, 64));
-            }
-            if (lane == 0 && l0 + u < nl) l_reach[l0 + u] = r | (b << 8);
-        }
+        r = __ockl_wfred_max_i32(r);
+        b = __ockl_wfred_max_i32(b);
+        if (lane == 0 && L0 + li0 + u < nl) l_reach[L0 + li0 + u] = r | (b << 8);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < ng * 64; e += 256) {
+        const int grp = e >> 6, li = e & 63;
+        if (L0 + li < nl) greach[(size_t)grp * nl + L0 + li] = s_g[grp][li];
     }
 }
 
@@ -1343,9 +1352,8 @@ int a12i_atlas_linop1_run(a12_ctx *ctx, int prec)
     }
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[9], s));
     if (nl > 0) {
-        hipLaunchKernelGGL(k_lo_linereach, dim3(std::min((nl + 3) / 4, 65536)), dim3(256), 0, s,
-                           nl, n, A.state.as<int>(), A.reach.as<int>(),
-                           A.greach.as<unsigned short>());
+        hipLaunchKernelGGL(k_lo_linereach, dim3((nl + 63) / 64), dim3(256), 0, s, nl, n,
+                           A.state.as<int>(), A.reach.as<int>(), A.greach.as<unsigned short>());
         A12_CHECK_LAUNCH();
     }
     if (ctx->prof) A12_HIP(hipEventRecord(ctx->ev[10], s));
