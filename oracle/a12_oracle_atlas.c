@@ -267,6 +267,8 @@ void or_rt_pass(const or_rt_in *in, or_rt_acc *acc, double *hnu1_out)
                 snu[j] = fmax(snu[j], 1.0e-99);
                 hnu[j] = fmax(hnu[j], 1.0e-99);
             }
+        memcpy(acc->last_hnu, hnu, sizeof(double) * n);
+        memcpy(acc->last_taunu, taunu, sizeof(double) * n);
         /* JOSH's ABTOT and ALPHA (:18475-18476) */
         for (int j = 0; j < n; ++j) {
             abtot[j] = io.acont[j] + aline[j] + io.sigmac[j] + io.sigmal[j];
@@ -296,8 +298,10 @@ void or_rt_pass(const or_rt_in *in, or_rt_acc *acc, double *hnu1_out)
             }
             double diagj = term1 + term2;
             double dbdt = bnu[j] * freq * in->hkt[j] / in->t[j] / fmax(stim[j], 1.0e-30);
-            acc->rdiagj[j] = acc->rdiagj[j] + abtot[j] * (diagj - 1.0) / (1.0 - alpha[j] * diagj) *
-                                              (1.0 - alpha[j]) * dbdt * rcowt;
+            const double rdiagjnu = abtot[j] * (diagj - 1.0) / (1.0 - alpha[j] * diagj) *
+                                    (1.0 - alpha[j]) * dbdt * rcowt;
+            acc->rdiagj[j] = acc->rdiagj[j] + rdiagjnu;
+            acc->last_rdiagjnu[j] = rdiagjnu;
         }
         /* RADIAP(2) */
         for (int j = 0; j < n; ++j) {

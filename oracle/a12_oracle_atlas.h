@@ -51,6 +51,9 @@ typedef struct {
     double rjmins[128], rdabh[128], rdiagj[128], flxrad[128];
     double raden[128], h_total[128], accrad[128], abross[128];
     double pradk0;
+    /* the last frequency's RDIAGJNU (TCORR(2)'s per-frequency term, :2507-2510),
+     * HNU and TAUNU, as the reference leaves them in its module arrays */
+    double last_rdiagjnu[128], last_hnu[128], last_taunu[128];
 } or_rt_acc;
 
 void or_rt_pass(const or_rt_in *in, or_rt_acc *acc, double *hnu1_out);

@@ -316,7 +316,8 @@ class OrRtIn(C.Structure):
 class OrRtAcc(C.Structure):
     _fields_ = [(k, C.c_double * 128) for k in ("rjmins", "rdabh", "rdiagj", "flxrad", "raden",
                                                 "h_total", "accrad", "abross")] + [
-        ("pradk0", C.c_double)]
+        ("pradk0", C.c_double)] + [(k, C.c_double * 128) for k in (
+            "last_rdiagjnu", "last_hnu", "last_taunu")]
 
 
 def tabvoigt(vsteps=200.0, n=2001):
@@ -392,7 +393,8 @@ def rt_pass(z, xlines, coefj, coefh):
     h1 = np.zeros(nf)
     lib().or_rt_pass(C.byref(s), C.byref(acc), _p(h1))
     out = {k: np.array(getattr(acc, k)[:n]) for k in ("rjmins", "rdabh", "rdiagj", "flxrad",
-                                                      "raden", "h_total", "accrad", "abross")}
+                                                      "raden", "h_total", "accrad", "abross",
+                                                      "last_rdiagjnu", "last_hnu", "last_taunu")}
     out["pradk0"] = acc.pradk0
     return out, h1
 

@@ -143,6 +143,11 @@ def test_rt_pass_accumulators_bitwise(orc, golden, coef):
         assert np.array_equal(acc[k], z["acc_" + k]), k
     assert acc["pradk0"] == z["acc_pradk0"]
     assert h1[-1] == z["last_hnu"][0]
+    # the last frequency's per-depth terms the reference leaves in its module
+    # arrays: TCORR(2)'s RDIAGJNU (RDIAGJ's summand, :2507-2510; RDIAGJ and
+    # RDABH themselves are local to TCORR), HNU and TAUNU
+    for k in ("rdiagjnu", "hnu", "taunu"):
+        assert np.array_equal(acc["last_" + k], z["last_" + k]), k
     # H_TOTAL is FLXRAD's twin (the same sum in RADIAP)
     assert np.array_equal(acc["h_total"], acc["flxrad"])
 
