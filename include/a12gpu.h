@@ -235,7 +235,9 @@ int a12_atlas_linop1_run(a12_ctx *ctx, int32_t prec);
  * and, for 0 <= idx < that number, writes {points per tile, lanes per
  * block, lines per LDS batch, form} (form 0: staged; 1: a line's items
  * dealt out over the wave instead of walked per lane; 2: per wavefront
- * without barriers).  a12_atlas_set_shape picks one (default 0);
+ * without barriers; 3: point form, one lane per (depth, points per tile / 8
+ * points) with register sums, the third value then the XCD run length of
+ * its workgroup numbering).  a12_atlas_set_shape picks one (default 0);
  * a12_atlas_autotune runs LINOP1 once, times the gather in every shape on
  * the loaded store (ms[shape], -1 where not applicable) and keeps the
  * fastest.  XLINES is bitwise the same in every shape. */
