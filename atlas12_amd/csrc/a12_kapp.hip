@@ -88,15 +88,17 @@ __global__ void k_kapp_tbsig(KTab T, int nw, const double *__restrict__ waves, d
 // MBF_TOPBASE's sum, the first part of ACONT_METAL: one lane per wavelength,
 // a workgroup per (64 wavelengths, 8 depths), so a level's weights for the
 // 8 depths are the same for every lane; each depth's sum runs over the
-// levels in the reference's order (kp_cell_tb).
+// levels in the reference's order (kp_cell_tb).  The depth group is the
+// grid's fastest index: the workgroups of one wavelength tile run together
+// and share its cross sections through the caches.
 constexpr int TB_DEPTHS = 8;
 __global__ void __launch_bounds__(64) k_kapp_tbsum(KTab T, KState S, const double *__restrict__ planes,
                                                    const double *__restrict__ sig,
                                                    const double *__restrict__ waves, int nw,
                                                    double *__restrict__ tbsum)
 {
-    const int w = blockIdx.x * 64 + threadIdx.x;
-    const int n = S.n, j0 = blockIdx.y * TB_DEPTHS;
+    const int w = blockIdx.y * 64 + threadIdx.x;
+    const int n = S.n, j0 = blockIdx.x * TB_DEPTHS;
     if (w >= nw) return;
     const double freq = CLIGHT_NMS / waves[w];
     double sor[TB_DEPTHS], acc[TB_DEPTHS];
@@ -108,9 +110,8 @@ __global__ void __launch_bounds__(64) k_kapp_tbsum(KTab T, KState S, const doubl
     }
     const double *W = planes + (size_t)KS_TB * n + j0;
     const int nd = n - j0 < TB_DEPTHS ? n - j0 : TB_DEPTHS;
-    for (int l = 0; l < T.tb_nlev_tot; ++l) {
-        const double sg = sig[(size_t)l * nw + w];
-        if (sg == 0.0) continue;
+    auto add = [&](int l, double sg) {
+        if (sg == 0.0) return;
         const double *wl = W + (size_t)l * n;
         if (nd == TB_DEPTHS) {
 #pragma unroll
@@ -118,7 +119,17 @@ __global__ void __launch_bounds__(64) k_kapp_tbsum(KTab T, KState S, const doubl
         } else {
             for (int d = 0; d < nd; ++d) acc[d] = acc[d] + wl[d] * sg * sor[d];
         }
+    };
+    // four levels' cross sections loaded together, then added in level order
+    int l = 0;
+    for (; l + 4 <= T.tb_nlev_tot; l += 4) {
+        double sg[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) sg[u] = sig[(size_t)(l + u) * nw + w];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) add(l + u, sg[u]);
     }
+    for (; l < T.tb_nlev_tot; ++l) add(l, sig[(size_t)l * nw + w]);
 #pragma unroll
     for (int d = 0; d < TB_DEPTHS; ++d)
         if (d < nd) tbsum[(size_t)w * n + j0 + d] = acc[d];
@@ -228,7 +239,7 @@ int launch_cells(a12_ctx *ctx, KappDev &K, int nw, const double *d_waves, const 
     const KState S = kapp_state_view(K.st.as<double>(), K.n, K.mion);
     A12_HIP(K.tbsum.reserve(sizeof(double) * cells));
     const unsigned nb = (unsigned)((cells + 255) / 256);
-    k_kapp_tbsum<<<dim3((nw + 63) / 64, (K.n + TB_DEPTHS - 1) / TB_DEPTHS), 64, 0, ctx->stream>>>(
+    k_kapp_tbsum<<<dim3((K.n + TB_DEPTHS - 1) / TB_DEPTHS, (nw + 63) / 64), 64, 0, ctx->stream>>>(
         dev_tab(K), S, K.planes.as<double>(), d_sig, d_waves, nw, K.tbsum.as<double>());
     A12_HIP(hipGetLastError());
     A12_HIP(K.cbuf.reserve(sizeof(double) * 17 * cells));
