@@ -120,14 +120,14 @@ __global__ void __launch_bounds__(64) k_kapp_tbsum(KTab T, KState S, const doubl
             for (int d = 0; d < nd; ++d) acc[d] = acc[d] + wl[d] * sg * sor[d];
         }
     };
-    // four levels' cross sections loaded together, then added in level order
+    // eight levels' cross sections loaded together, then added in level order
     int l = 0;
-    for (; l + 4 <= T.tb_nlev_tot; l += 4) {
-        double sg[4];
+    for (; l + 8 <= T.tb_nlev_tot; l += 8) {
+        double sg[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) sg[u] = sig[(size_t)(l + u) * nw + w];
+        for (int u = 0; u < 8; ++u) sg[u] = sig[(size_t)(l + u) * nw + w];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) add(l + u, sg[u]);
+        for (int u = 0; u < 8; ++u) add(l + u, sg[u]);
     }
     for (; l < T.tb_nlev_tot; ++l) add(l, sig[(size_t)l * nw + w]);
 #pragma unroll
